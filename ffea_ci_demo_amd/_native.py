@@ -1,0 +1,79 @@
+"""ctypes binding of libffea_gpu.so (the C ABI declared in include/ffea_gpu.h).
+
+The library is loaded from this package directory; there is no fallback: a
+missing library or HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libffea_gpu.so")
+
+OK, ERR_INVERSION, ERR_CG_NO_CONVERGE, ERR_INVALID, ERR_HIP, ERR_NO_DEVICE = range(6)
+FP64, MIXED = 0, 1
+NUM_PHASES = 6
+PHASES = ("element", "assemble", "gather_cg_start", "cg_spmv", "cg_vector", "update")
+
+EXPORTED = [
+    "ffea_gpu_blob_create", "ffea_gpu_blob_destroy", "ffea_gpu_blob_update_internal_forces",
+    "ffea_gpu_blob_update_positions", "ffea_gpu_blob_step", "ffea_gpu_blob_get_positions",
+    "ffea_gpu_blob_set_positions", "ffea_gpu_blob_get_velocities", "ffea_gpu_blob_get_rhs",
+    "ffea_gpu_blob_get_element_forces", "ffea_gpu_blob_get_cg_iterations", "ffea_gpu_blob_get_rng_state",
+    "ffea_gpu_blob_set_rng_state", "ffea_gpu_blob_get_info", "ffea_gpu_blob_set_timing",
+    "ffea_gpu_blob_get_timing", "ffea_gpu_blob_synchronize", "ffea_gpu_blob_last_error", "ffea_gpu_last_error",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("kT", C.c_double), ("epsilon2", C.c_double),
+                ("max_iterations_cg", C.c_int), ("calc_noise", C.c_int), ("calc_stokes", C.c_int),
+                ("precision", C.c_int), ("device", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("num_nodes", C.c_int), ("num_elements", C.c_int), ("num_linear_nodes", C.c_int),
+                ("num_blocks", C.c_longlong), ("num_block_slots", C.c_longlong), ("precision", C.c_int),
+                ("device_bytes", C.c_longlong)]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "or python -m ffea_ci_demo_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.ffea_gpu_blob_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
+                                       C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(vp)]
+    L.ffea_gpu_blob_destroy.argtypes = [vp]
+    L.ffea_gpu_blob_destroy.restype = None
+    for f in ("update_internal_forces", "update_positions", "synchronize"):
+        getattr(L, "ffea_gpu_blob_" + f).argtypes = [vp]
+    L.ffea_gpu_blob_step.argtypes = [vp, C.c_int]
+    for f in ("get_positions", "set_positions", "get_velocities", "get_rhs", "get_element_forces"):
+        getattr(L, "ffea_gpu_blob_" + f).argtypes = [vp, _dp]
+    L.ffea_gpu_blob_get_cg_iterations.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L.ffea_gpu_blob_get_rng_state.argtypes = [vp, _up]
+    L.ffea_gpu_blob_set_rng_state.argtypes = [vp, _up]
+    L.ffea_gpu_blob_get_info.argtypes = [vp, C.POINTER(Info)]
+    L.ffea_gpu_blob_set_timing.argtypes = [vp, C.c_int]
+    L.ffea_gpu_blob_get_timing.argtypes = [vp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    L.ffea_gpu_blob_last_error.argtypes = [vp]
+    L.ffea_gpu_blob_last_error.restype = C.c_char_p
+    L.ffea_gpu_last_error.argtypes = []
+    L.ffea_gpu_last_error.restype = C.c_char_p
+    _lib = L
+    return L

@@ -1,0 +1,125 @@
+"""The blob workloads the path is measured and tested on, as meso-unit arrays.
+
+Each builder returns a `BlobSpec`: node positions, 10-node topology, per-element
+material (A, B, G, E), pins, constant forces and run parameters, ready for the
+C-ABI (`ffea_ci_demo_amd.blob.GpuBlob`) or any other consumer. Unit conversions
+are the reference loaders' (see units.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import meshgen, units
+
+# material row of a .mat file: density, shear visc, bulk visc, shear mod, bulk mod, dielectric
+DEFAULT_MATERIAL = (1.5e3, 1e-3, 1e-3, 3.6e8, 2.2e9, 1.0)
+KT_300K = 4.141945559999999e-21  # J; kB * 300 K in the reference's energy unit
+
+
+@dataclasses.dataclass
+class BlobSpec:
+    pos: np.ndarray          # (Nt,3) meso
+    top: np.ndarray          # (Ne,10) int32
+    A: np.ndarray
+    B: np.ndarray
+    G: np.ndarray
+    E: np.ndarray
+    dt: float
+    kT: float
+    epsilon2: float
+    max_iterations_cg: int
+    calc_noise: int
+    rng_state: np.ndarray    # 6 uint32 (stream 0)
+    pinned: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    ctf_node: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    ctf_force: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3)))
+    calc_stokes: int = 0
+    stokes_drag: Optional[np.ndarray] = None
+    mesh: Optional[meshgen.Mesh] = None
+    name: str = ""
+
+    @property
+    def num_elements(self):
+        return self.top.shape[0]
+
+    @property
+    def num_nodes(self):
+        return self.pos.shape[0]
+
+    def oracle_kwargs(self):
+        return dict(dt=self.dt, kT=self.kT, epsilon2=self.epsilon2, max_iterations_cg=self.max_iterations_cg,
+                    calc_noise=self.calc_noise, calc_stokes=self.calc_stokes, pinned=self.pinned,
+                    ctf_node=self.ctf_node, ctf_force=self.ctf_force, stokes_drag=self.stokes_drag,
+                    rng_state=self.rng_state)
+
+
+def default_rng_state(seed: int) -> np.ndarray:
+    """A valid MRG32k3a state derived deterministically from an integer (numpy PCG64),
+    used where no .ffea rng_seed is involved; .ffea scripts go through the glibc
+    srand/rand derivation of World.cpp:272-276 in the host library."""
+    r = np.random.default_rng(seed)
+    s = np.empty(6, np.uint32)
+    s[:3] = r.integers(1, 4294967087, size=3)
+    s[3:] = r.integers(1, 4294944443, size=3)
+    return s
+
+
+def cube_blob(n: int, *, mesh_seed: int = 1, rng_seed: int = 7, jitter: float = 0.1, scale: float = 1e-9,
+              material=DEFAULT_MATERIAL, dt: float = 1e-14, kT: float = KT_300K, epsilon: float = 1e-10,
+              max_iterations_cg: int = 100000, calc_noise: int = 1, flatten_frac: float = 0.0,
+              per_element_GE: Optional[tuple] = None, name: str = "") -> BlobSpec:
+    mesh = meshgen.kuhn_mesh(n, jitter=jitter, seed=mesh_seed, flatten_frac=flatten_frac)
+    s = units.scale_to_meso(scale)
+    pos = mesh.pos * s
+    pos = pos - pos.mean(axis=0)  # blob centroid at the origin, as World's move_into_box leaves it
+    ne = mesh.num_elements
+    _, A, B, G, E = units.material_to_meso(*material[:5])
+    A = np.full(ne, A)
+    B = np.full(ne, B)
+    if per_element_GE is not None:
+        G = np.asarray(per_element_GE[0], np.float64) / units.PRESSURE
+        E = np.asarray(per_element_GE[1], np.float64) / units.PRESSURE
+    else:
+        G = np.full(ne, G)
+        E = np.full(ne, E)
+    return BlobSpec(pos=pos, top=mesh.top, A=A, B=B, G=G, E=E, dt=units.dt_to_meso(dt), kT=units.kT_to_meso(kT),
+                    epsilon2=units.epsilon2(epsilon), max_iterations_cg=max_iterations_cg, calc_noise=calc_noise,
+                    rng_state=default_rng_state(rng_seed), mesh=mesh, name=name or f"cube{n}")
+
+
+def config0(**kw) -> BlobSpec:
+    """6x6x6 jittered cubes (1296 tets, 343 corner nodes), noise on, dt 1e-14 s."""
+    return cube_blob(6, name="cube6_1296tets", **kw)
+
+
+def config1(**kw) -> BlobSpec:
+    """100x100x100 jittered cubes (6.0M tets, 1.03M corner nodes), noise on."""
+    return cube_blob(100, name="cube100_6Mtets", **kw)
+
+
+def config3(n: int = 50, *, moduli_seed: int = 11, sigma: float = 0.5, **kw) -> BlobSpec:
+    """Differentiation blob: per-element log-normal moduli, noise off, bottom face
+    pinned (all its nodes), +1 pN along z on every top-face corner node."""
+    ne = 6 * n ** 3
+    r = np.random.default_rng(moduli_seed)
+    Gs = 3.6e8 * np.exp(sigma * r.standard_normal(ne))
+    Es = 2.2e9 * np.exp(sigma * r.standard_normal(ne))
+    spec = cube_blob(n, calc_noise=0, per_element_GE=(Gs, Es), name=f"cube{n}_diff", **kw)
+    z = spec.pos[:, 2]
+    zmin, zmax = z.min(), z.max()
+    tol = 1e-9 * (zmax - zmin)
+    spec.pinned = np.nonzero(z <= zmin + tol)[0].astype(np.int32)
+    top_lin = np.nonzero((z >= zmax - tol) & spec.mesh.is_linear)[0].astype(np.int32)
+    spec.ctf_node = top_lin
+    spec.ctf_force = np.tile(np.array([0.0, 0.0, units.force_to_meso(1e-12)]), (top_lin.size, 1))
+    return spec
+
+
+def config4(n: int = 40, **kw) -> BlobSpec:
+    """Ill-conditioned blob: 2% of tets flattened to aspect ~20, shear 1e7 Pa,
+    bulk 1e10 Pa, viscosities 1e-3 Pa s, noise on."""
+    mat = (1.5e3, 1e-3, 1e-3, 1e7, 1e10, 1.0)
+    return cube_blob(n, material=mat, flatten_frac=0.02, name=f"cube{n}_illcond", **kw)

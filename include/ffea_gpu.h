@@ -1,0 +1,145 @@
+/*
+ * ffea_gpu.h — C ABI of the MI355X blob time-step library (libffea_gpu.so).
+ *
+ * One handle is one FFEA blob on one GPU: its mesh, material, rest state, the
+ * mass-less viscosity solver and its MRG32k3a noise stream live in device
+ * memory. The entry points replace, for that blob, the reference's
+ *
+ *   Blob::init + NoMassCGSolver::init   Blob.cpp:161-247, NoMassCGSolver.cpp:56-153
+ *   Blob::update_internal_forces        Blob.cpp:400-491 (Blob.h:130)
+ *   Blob::update_positions              Blob.cpp:493-508 (Blob.h:141), with
+ *     Blob::aggregate_forces_and_solve  Blob.cpp:3147-3244
+ *     NoMassCGSolver::solve             NoMassCGSolver.cpp:156-188 (Solver.h:42-51)
+ *     Blob::euler_integrate             Blob.cpp:3249-3263
+ *   node positions / velocities         mesh_node::pos / vel (mesh_node.h)
+ *   RngStream::GetState / SetSeed       RngStream.cpp:439-443, 251-259
+ *
+ * Conventions: plain pointers and sizes; arrays are host memory owned by the
+ * caller and copied in or out; node vectors are [num_nodes][3] doubles in .node
+ * order (all nodes, midside included); element arrays follow .top order.
+ * Every call returns an ffea_gpu_status; a message for the last failure is kept
+ * per handle (ffea_gpu_blob_last_error) or per thread when no handle exists
+ * (ffea_gpu_last_error). A handle is not re-entrant; distinct handles may be
+ * used from different host threads. Each handle owns one HIP stream.
+ */
+#ifndef FFEA_GPU_H
+#define FFEA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FFEA_GPU_OK = 0,
+    FFEA_GPU_ERR_INVERSION = 1,      /* "N elements have inverted since the last step" (Blob.cpp:484-490) */
+    FFEA_GPU_ERR_CG_NO_CONVERGE = 2, /* "Could not converge after N iterations" (NoMassCGSolver.cpp:187) */
+    FFEA_GPU_ERR_INVALID = 3,        /* bad argument or mesh */
+    FFEA_GPU_ERR_HIP = 4,            /* HIP runtime failure (message holds the HIP error) */
+    FFEA_GPU_ERR_NO_DEVICE = 5
+} ffea_gpu_status;
+
+typedef enum {
+    FFEA_GPU_FP64 = 0,   /* all arithmetic in FP64, the reference's algorithm and orders */
+    FFEA_GPU_MIXED = 1   /* element algebra and the viscosity matrix in FP32; CG vectors,
+                            inner products, residual norms and positions in FP64 */
+} ffea_gpu_precision;
+
+/* Run parameters in FFEA's internal units (SimulationParams after assign/validate). */
+typedef struct {
+    double dt;               /* SimulationParams::dt */
+    double kT;               /* SimulationParams::kT */
+    double epsilon2;         /* SimulationParams::epsilon2 (epsilon squared) */
+    int max_iterations_cg;   /* SimulationParams::max_iterations_cg */
+    int calc_noise;          /* SimulationParams::calc_noise */
+    int calc_stokes;         /* SimulationParams::calc_stokes (must be 0 in this version) */
+    int precision;           /* ffea_gpu_precision */
+    int device;              /* HIP device ordinal */
+} ffea_gpu_params;
+
+typedef struct ffea_gpu_blob ffea_gpu_blob;
+
+/* Blob::init + NoMassCGSolver::init (Blob.cpp:161-247, NoMassCGSolver.cpp:56-153).
+ *   pos        [num_nodes][3] node positions (meso units, .node order, scale applied)
+ *   top        [num_elements][10] node indices (.top order; 4 corners then the
+ *              midside nodes of edges 01 02 03 12 13 23)
+ *   A, B, G, E [num_elements] shear viscosity, second viscosity coefficient, shear and
+ *              bulk modulus in meso units (Blob::load_material_params, Blob.cpp:2282-2287)
+ *   pinned     [num_pinned] node indices (Blob::load_pinned_nodes, Blob.cpp:3024-3076)
+ *   ctf_node, ctf_force  [num_ctf], [num_ctf][3] constant linear node forces in meso
+ *              units, applied in list order (Blob::apply_ctforces, Blob.cpp:1679-1683)
+ *   stokes_drag [num_nodes] or NULL (must be NULL in this version)
+ *   rng_state  6 x uint32: the state of the blob's noise stream (rng[0] of World,
+ *              World.cpp:259-295)
+ * On success *out holds the handle. */
+int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_elements,
+                         const double *pos, const int *top, const double *A, const double *B,
+                         const double *G, const double *E, int num_pinned, const int *pinned,
+                         int num_ctf, const int *ctf_node, const double *ctf_force,
+                         const double *stokes_drag, const uint32_t rng_state[6],
+                         ffea_gpu_blob **out);
+
+/* releases device memory and the stream; NULL is accepted */
+void ffea_gpu_blob_destroy(ffea_gpu_blob *blob);
+
+/* Blob::update_internal_forces (Blob.cpp:400-491): ctforces + per-element elastic and
+ * fluctuating stress -> element nodal forces; counts inverted elements. */
+int ffea_gpu_blob_update_internal_forces(ffea_gpu_blob *blob);
+
+/* Blob::update_positions (Blob.cpp:493-508): gather, linearise_force, pins, viscosity
+ * matrix build, Jacobi-PCG solve, Euler update, midside re-linearisation. */
+int ffea_gpu_blob_update_positions(ffea_gpu_blob *blob);
+
+/* nsteps x { zero forces; update_internal_forces; update_positions } — the blob's part
+ * of World::run (World.cpp:1786-2059). Stops at the first failing step. */
+int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
+
+/* node state, [num_nodes][3] (midside nodes at their edge midpoints) */
+int ffea_gpu_blob_get_positions(ffea_gpu_blob *blob, double *pos);
+int ffea_gpu_blob_set_positions(ffea_gpu_blob *blob, const double *pos);
+int ffea_gpu_blob_get_velocities(ffea_gpu_blob *blob, double *vel);
+
+/* solver right-hand side of the last update_positions ([num_nodes][3], after gather,
+ * linearise_force and pin zeroing — the `force` vector Blob.cpp:3243 hands to solve) */
+int ffea_gpu_blob_get_rhs(ffea_gpu_blob *blob, double *rhs);
+
+/* element nodal forces of the last update_internal_forces, [num_elements][4][3]
+ * (tetra_element_linear::node_force[0..3]) */
+int ffea_gpu_blob_get_element_forces(ffea_gpu_blob *blob, double *force);
+
+/* CG iterations of the last solve and the total since creation */
+int ffea_gpu_blob_get_cg_iterations(ffea_gpu_blob *blob, int *last, long long *total);
+
+/* noise stream state (RngStream::GetState / SetSeed) */
+int ffea_gpu_blob_get_rng_state(ffea_gpu_blob *blob, uint32_t state[6]);
+int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *blob, const uint32_t state[6]);
+
+/* sizes of the device-side problem */
+typedef struct {
+    int num_nodes, num_elements, num_linear_nodes;
+    long long num_blocks;        /* 3x3 blocks of the viscosity matrix (pattern, no padding) */
+    long long num_block_slots;   /* stored block slots (with slice padding) */
+    int precision;
+    long long device_bytes;      /* device memory held by the handle */
+} ffea_gpu_blob_info;
+int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
+
+/* Kernel timing with HIP events recorded on the handle's stream around each launch.
+ * Phases: 0 element forces, 1 matrix assembly, 2 gather + CG start, 3 CG SpMV step,
+ * 4 CG vector step, 5 position update. Times in ms, summed since the last reset. */
+#define FFEA_GPU_NUM_PHASES 6
+int ffea_gpu_blob_set_timing(ffea_gpu_blob *blob, int enable);
+int ffea_gpu_blob_get_timing(ffea_gpu_blob *blob, double ms[FFEA_GPU_NUM_PHASES],
+                             long long launches[FFEA_GPU_NUM_PHASES]);
+
+/* waits for all work queued on the handle's stream */
+int ffea_gpu_blob_synchronize(ffea_gpu_blob *blob);
+
+const char *ffea_gpu_blob_last_error(const ffea_gpu_blob *blob);
+const char *ffea_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
