@@ -1,0 +1,203 @@
+"""Blob time-step benchmark: element-steps/s on one MI355X.
+
+Workload (default): the 100x100x100 jittered-cube blob — 6.0M quadratic tets,
+1.03M corner nodes — with the reference material (shear/bulk modulus 3.6e8 /
+2.2e9 Pa, viscosities 1e-3 Pa s), thermal noise on at 300 K, dt = 1e-14 s, CG
+tolerance 1e-10, steric/electrostatics off. One step = one full blob update:
+element forces, gather, viscosity-matrix assembly, Jacobi-PCG to tolerance, Euler
+update. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0). `value` is the headline form's element-steps/s;
+`forms` holds every form measured. `roofline` is for the dominant kernel (the CG
+SpMV step), from HIP events recorded on the library's stream over the timed
+region. `cpu_baseline` times the oracle (the FP64 C restatement of the reference
+algorithm, single thread) on a bounded sample of the same workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--forms fp64,mixed]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "element-steps/s on 1 MI355X (FP32-mixed vs FP64) + HBM roofline fraction"
+PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_cube100_traffic.json")
+
+
+def spmv_alg_bytes(info, w):
+    """algorithmic bytes of one CG SpMV-step launch: the viscosity matrix (9 values of
+    w bytes + a 4-byte block column per 3x3 block) and four FP64 node 3-vectors
+    (p_old and z read, p_new and q written)"""
+    return info["num_blocks"] * (9 * w + 4) + 4 * 24 * info["num_linear_nodes"]
+
+
+def iter_alg_bytes(info, w):
+    """the per-CG-iteration model: Nb (9w + 4) + 216 Nl (matrix + 9 FP64 vector passes)"""
+    return info["num_blocks"] * (9 * w + 4) + 216 * info["num_linear_nodes"]
+
+
+def run_form(spec, form, steps, warmup):
+    from ffea_ci_demo_amd.blob import GpuBlob
+    import torch
+    t0 = time.perf_counter()
+    g = GpuBlob.from_spec(spec, precision=form)
+    t_create = time.perf_counter() - t0
+    info = g.info()
+    print(f"[bench] {spec.name} form={form} created in {t_create:.1f}s: Ne={info['num_elements']} "
+          f"Nl={info['num_linear_nodes']} blocks={info['num_blocks']} device_bytes={info['device_bytes']/1e9:.2f}GB",
+          file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        g.step(1)
+    _, tot0 = g.cg_iterations()
+    g.set_timing(("cg_spmv", "cg_vector"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        g.step(1)
+    g.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tim = g.timing()
+    _, tot1 = g.cg_iterations()
+    iters = tot1 - tot0
+    w = 8 if form == "fp64" else 4
+    spmv_ms, spmv_n = tim["cg_spmv"]
+    vec_ms, vec_n = tim["cg_vector"]
+    avg_spmv_s = spmv_ms / max(spmv_n, 1) / 1e3
+    ne = info["num_elements"]
+    res = {
+        "element_steps_per_s": ne * steps / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "cg_iterations_per_step": iters / steps,
+        "spmv_avg_us": avg_spmv_s * 1e6,
+        "vector_avg_us": vec_ms / max(vec_n, 1) * 1e3,
+        "spmv_alg_bytes": spmv_alg_bytes(info, w),
+        "spmv_achieved_GBs": spmv_alg_bytes(info, w) / avg_spmv_s / 1e9,
+        "iteration_model_GBs": iter_alg_bytes(info, w) * iters / wall / 1e9,
+        "info": info,
+    }
+    print(f"[bench] form={form} {res['ms_per_step']:.1f} ms/step, k={res['cg_iterations_per_step']:.1f}, "
+          f"spmv {res['spmv_avg_us']:.1f} us = {res['spmv_achieved_GBs']:.0f} GB/s", file=sys.stderr, flush=True)
+    positions = g.positions()
+    g.close()
+    return res, positions
+
+
+def cpu_baseline(spec, k_per_step, cg_iters):
+    """oracle (FP64, 1 thread) on one step of the same blob with the CG capped at
+    cg_iters iterations; a full step is element pass + gather/build + k iterations"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.perf_counter()
+    o = oracle.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    t_create = time.perf_counter() - t0
+    te, tb, ti = o.time_step(cg_iters)
+    del o
+    t_step = te + tb + k_per_step * ti
+    print(f"[bench] cpu oracle: create {t_create:.1f}s element {te:.2f}s build {tb:.2f}s cg iter {ti*1e3:.1f}ms",
+          file=sys.stderr, flush=True)
+    return {
+        "value": spec.num_elements / t_step,
+        "unit": "element-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"one step of the same blob on the oracle (FP64 C restatement, 1 thread): element pass "
+                   f"{te:.2f}s + gather/matrix build {tb:.2f}s + {cg_iters} CG iterations at {ti*1e3:.1f} ms each, "
+                   f"scaled to the {k_per_step:.0f} iterations/step the GPU FP64 run needed"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=100, help="cubes per side (100: the 6.0M-tet blob)")
+    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--headline", default=None, help="form reported as value (default: last of --forms)")
+    ap.add_argument("--cpu-cg-iters", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    from ffea_ci_demo_amd import _native, workloads
+
+    t0 = time.perf_counter()
+    spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
+    print(f"[bench] mesh n={args.n} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    forms = [f for f in args.forms.split(",") if f]
+    results, pos = {}, {}
+    for f in forms:
+        results[f], pos[f] = run_form(spec, f, args.steps, args.warmup)
+    if "fp64" in pos and "mixed" in pos:
+        d = np.abs(pos["mixed"] - pos["fp64"]).max() / np.abs(pos["fp64"]).max()
+        results["mixed"]["position_rel_err_vs_fp64"] = float(d)
+    head = args.headline or forms[-1]
+    h = results[head]
+    w = 8 if head == "fp64" else 4
+    copy_bytes = 1 << 30
+    copy_ms = _native.copy_probe(copy_bytes, iters=20)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        if t.get("workload") == spec.name and t.get("form") == head:
+            traffic = t.get("hbm_bytes_per_launch")
+    roof = {
+        "bound": "hbm",
+        "kernel": "k_cg_spmv",
+        "achieved": round(h["spmv_achieved_GBs"], 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(h["spmv_achieved_GBs"] / PEAK_HBM_GBS, 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": h["spmv_alg_bytes"],
+        "avg_launch_us": round(h["spmv_avg_us"], 2),
+        "copy_probe_GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        k_ref = results["fp64"]["cg_iterations_per_step"] if "fp64" in results else h["cg_iterations_per_step"]
+        cpu = cpu_baseline(spec, k_ref, args.cpu_cg_iters)
+    out = {
+        "metric": METRIC,
+        "value": round(h["element_steps_per_s"], 1),
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{spec.name}: {args.n}^3 jittered unit cubes (edge 1 nm) x 6 quadratic tets, "
+                        f"{spec.num_elements} tets, {results[head]['info']['num_linear_nodes']} corner nodes; "
+                        "thermal noise at 300 K, dt 1e-14 s, CG epsilon 1e-10, no pins",
+            "form": head,
+            "cg_iterations_per_step": round(h["cg_iterations_per_step"], 1),
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "forms": {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items() if k != "info"}
+                  for f, r in results.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,7 +24,7 @@ EXPORTED = [
     "ffea_gpu_blob_set_positions", "ffea_gpu_blob_get_velocities", "ffea_gpu_blob_get_rhs",
     "ffea_gpu_blob_get_element_forces", "ffea_gpu_blob_get_cg_iterations", "ffea_gpu_blob_get_rng_state",
     "ffea_gpu_blob_set_rng_state", "ffea_gpu_blob_get_info", "ffea_gpu_blob_set_timing",
-    "ffea_gpu_blob_get_timing", "ffea_gpu_blob_synchronize", "ffea_gpu_blob_last_error", "ffea_gpu_last_error",
+    "ffea_gpu_blob_get_timing", "ffea_gpu_copy_probe", "ffea_gpu_blob_synchronize", "ffea_gpu_blob_last_error", "ffea_gpu_last_error",
 ]
 
 
@@ -69,7 +69,8 @@ def lib():
     L.ffea_gpu_blob_get_rng_state.argtypes = [vp, _up]
     L.ffea_gpu_blob_set_rng_state.argtypes = [vp, _up]
     L.ffea_gpu_blob_get_info.argtypes = [vp, C.POINTER(Info)]
-    L.ffea_gpu_blob_set_timing.argtypes = [vp, C.c_int]
+    L.ffea_gpu_blob_set_timing.argtypes = [vp, C.c_uint]
+    L.ffea_gpu_copy_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.ffea_gpu_blob_get_timing.argtypes = [vp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     L.ffea_gpu_blob_last_error.argtypes = [vp]
     L.ffea_gpu_blob_last_error.restype = C.c_char_p
@@ -77,3 +78,12 @@ def lib():
     L.ffea_gpu_last_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def copy_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:
+    """ms per copy of nbytes (read nbytes + write nbytes)"""
+    ms = C.c_double()
+    rc = lib().ffea_gpu_copy_probe(device, int(nbytes), int(iters), C.byref(ms))
+    if rc != OK:
+        raise RuntimeError(lib().ffea_gpu_last_error().decode())
+    return ms.value

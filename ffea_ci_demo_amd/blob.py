@@ -145,8 +145,13 @@ class GpuBlob:
         self._check(_native.lib().ffea_gpu_blob_get_info(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in _native.Info._fields_}
 
-    def set_timing(self, enable=True):
-        self._check(_native.lib().ffea_gpu_blob_set_timing(self.h, int(enable)))
+    def set_timing(self, phases=None):
+        """time the named phases (default: all); set_timing(()) turns timing off"""
+        names = _native.PHASES if phases is None else phases
+        mask = 0
+        for nm in names:
+            mask |= 1 << _native.PHASES.index(nm)
+        self._check(_native.lib().ffea_gpu_blob_set_timing(self.h, mask))
 
     def timing(self):
         ms = np.zeros(_native.NUM_PHASES)

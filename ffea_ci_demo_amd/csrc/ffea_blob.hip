@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -64,22 +65,20 @@ std::string fmt(const char *f, ...) {
     return buf;
 }
 
-// scalars shared by the CG kernels (device memory)
+// scalars shared by the CG kernels (device memory). Every reduction is finished by
+// the kernel that consumes it: each block of that kernel adds the producer's
+// per-block partials in the same fixed order, so all blocks hold bit-identical
+// totals and no atomics or cross-block fences are needed.
 struct CgScalars {
-    double delta;     // r . z of the current residual
-    double pTq;
-    double alpha;
-    double beta;
-    double rr;
-    double f2;        // |f|^2 of the right-hand side
-    int iter;         // iterations completed this solve
-    int done;         // 1: stop (converged, failed or skipped)
-    int converged;    // 1: converged this solve
-    int first;        // next SpMV step is iteration 0 (p = z)
-    int inverted;     // inverted elements counted by the last element pass
+    double delta[2];   // delta[(i-1)&1]: r.z entering iteration i (written by the SpMV step)
+    double f2;         // |f|^2 of the right-hand side
+    double G[36];      // (Z^T Z)^-1 for the rigid-mode projection of the mixed form
+    int iter;          // iterations done when the solve stopped
+    int done;          // 1: stop (converged, failed or skipped)
+    int converged;     // 1: converged this solve
+    int inverted;      // inverted elements counted by the element pass (relaxed atomics)
+    int inverted_step; // inverted elements of the step being solved
     int pad;
-    unsigned ticket[4];
-    uint32_t rng[6];  // noise stream state (state after the last element pass)
 };
 
 // ---------------------------------------------------------------- reductions
@@ -89,56 +88,123 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Deterministic grid reduction of NV doubles: per-block sums in a fixed order, then
-// the last block to finish (atomic ticket) adds the per-block partials in index order.
-// Returns true in every thread of the last block, with totals in `out`.
+// block sum of NV values; result valid in every thread
 template <int NV>
-__device__ bool grid_reduce(double (&v)[NV], double *partials, unsigned *ticket, double (&out)[NV]) {
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __shared__ double sh[kBlock / 64][NV];
-    __shared__ bool last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) sh[wid][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += sh[w][k];
-            __hip_atomic_store(&partials[(size_t)blockIdx.x * NV + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __threadfence();
-        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&partials[(size_t)b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = wave_sum(acc[k]);
+        const double s = wave_sum(v[k]);
         if (lane == 0) sh[wid][k] = s;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double s = 0.0;
+#pragma unroll
         for (int w = 0; w < kBlock / 64; ++w) s += sh[w][k];
-        out[k] = s;
+        v[k] = s;
     }
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
+    __syncthreads();
+}
+
+// per-block partials, stored [k][nb]
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int nb, int L) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) P[(size_t)k * nb + L] = v[k];
+}
+
+// totals of partials written by the previous kernel: identical in every block
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *P, int nb, double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) acc += P[(size_t)k * nb + b];
+        out[k] = acc;
+    }
+    block_sum<NV>(out);
+}
+
+// Row range of a block in the row kernels: a fixed grid of nb blocks, each owning a
+// contiguous range of rows; blocks are mapped so that each of the 8 XCDs (which
+// receive workgroups round-robin) owns one contiguous eighth of the rows.
+struct RowRange {
+    int L, begin, end;
+};
+__device__ __forceinline__ RowRange row_range(int nrows, int rows_per_block) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int begin = L * rows_per_block;
+    const int end = min(nrows, begin + rows_per_block);
+    return {L, begin, end};
+}
+
+// rigid-mode basis of a corner node at x (relative to a fixed centre): columns are
+// the 3 translations and the 3 infinitesimal rotations e_k x x. Z^T v = (v, x cross v).
+__device__ __forceinline__ void zt_times(const double *x, const double *v, double *out6) {
+    out6[0] = v[0];
+    out6[1] = v[1];
+    out6[2] = v[2];
+    out6[3] = x[1] * v[2] - x[2] * v[1];
+    out6[4] = x[2] * v[0] - x[0] * v[2];
+    out6[5] = x[0] * v[1] - x[1] * v[0];
+}
+// Z c = c_t + c_r cross x
+__device__ __forceinline__ void z_times(const double *x, const double *c, double *out3) {
+    out3[0] = c[0] + (c[4] * x[2] - c[5] * x[1]);
+    out3[1] = c[1] + (c[5] * x[0] - c[3] * x[2]);
+    out3[2] = c[2] + (c[3] * x[1] - c[4] * x[0]);
+}
+
+// 6x6 inverse by Gauss-Jordan with partial pivoting (FP64)
+__device__ void invert6(const double *M, double *Minv) {
+    double a[6][12];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 12; ++j) a[i][j] = j < 6 ? M[6 * i + j] : (j - 6 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 6; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 6; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < 12; ++j) {
+                const double t = a[c][j];
+                a[c][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+        const double d = 1.0 / a[c][c];
+        for (int j = 0; j < 12; ++j) a[c][j] *= d;
+        for (int r = 0; r < 6; ++r)
+            if (r != c) {
+                const double f = a[r][c];
+                for (int j = 0; j < 12; ++j) a[r][j] -= f * a[c][j];
+            }
+    }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) Minv[6 * i + j] = a[i][j + 6];
+}
+
+// Z^T Z from the moments N, sum x, sum x_i x_j (m[0], m[1..3], m[4..9]: xx yy zz xy xz yz)
+__device__ void ztz_from_moments(const double *m, double *M) {
+    for (int i = 0; i < 36; ++i) M[i] = 0.0;
+    const double N = m[0], sx = m[1], sy = m[2], sz = m[3];
+    const double xx = m[4], yy = m[5], zz = m[6], xy = m[7], xz = m[8], yz = m[9];
+    M[0] = M[7] = M[14] = N;
+    // translation-rotation block: sum_n (e_k x x_n)_i
+    const double Tr[3][3] = {{0, sz, -sy}, {-sz, 0, sx}, {sy, -sx, 0}};  // [i][k]
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            M[6 * i + 3 + k] = Tr[i][k];
+            M[6 * (3 + k) + i] = Tr[i][k];
+        }
+    const double r2 = xx + yy + zz;
+    const double R[3][3] = {{r2 - xx, -xy, -xz}, {-xy, r2 - yy, -yz}, {-xz, -yz, r2 - zz}};
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) M[6 * (3 + k) + 3 + l] = R[k][l];
 }
 
 // ---------------------------------------------------------------- element pass
@@ -156,9 +222,8 @@ struct ElemArgs {
     double kT, dt;
     const mrg::Jump *jump_group;  // [ceil(ne/64)]: A^(7*64*g)
     const mrg::Jump *jump_lane;   // [64]: A^(7*l)
-    mrg::Jump step_jump;          // A^(7*ne)
+    uint32_t rng[6];              // stream state at the start of this pass
     CgScalars *S;
-    double *partials;
 };
 
 // tetra_element_linear.cpp:191-443 and :468-483 for one element (mass-less solver:
@@ -167,156 +232,137 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
 #pragma clang fp contract(off)
     const int e = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = e < a.ne;
-    int inverted = 0;
+    if (e >= a.ne) return;
+    const int ne = a.ne;
     uint32_t st[6];
     if (a.calc_noise) {
-        // this element's 7 draws start 7*e draws into the step's segment of the stream
-        const int g = e >> 6, l = e & 63;
-        for (int i = 0; i < 6; ++i) st[i] = a.S->rng[i];
-        if (active) {
-            mrg::d_apply(a.jump_group[g], st);
-            mrg::d_apply(a.jump_lane[l], st);
-        }
+        // this element's 7 draws start 7*e draws into the pass's segment of the stream
+        for (int i = 0; i < 6; ++i) st[i] = a.rng[i];
+        mrg::d_apply(a.jump_group[e >> 6], st);
+        mrg::d_apply(a.jump_lane[e & 63], st);
     }
-    if (active) {
-        const int ne = a.ne;
-        int n[4];
+    int n[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
-        double x0[3], J64[3][3];
+    for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
+    double x0[3], J64[3][3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
+    for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) J64[r][c] = a.pos[3 * n[r + 1] + c] - x0[c];
-        T J[3][3];
+        for (int c = 0; c < 3; ++c) J64[r][c] = a.pos[3 * n[r + 1] + c] - x0[c];
+    T J[3][3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) J[r][c] = (T)J64[r][c];
+        for (int c = 0; c < 3; ++c) J[r][c] = (T)J64[r][c];
 
-        // shape function derivatives and volume (tetra_element_linear.cpp:222-292)
-        T d[12];
-        d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
-        d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
-        d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
-        d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
-        d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
-        d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
-        d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
-        d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
-        d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
-        T det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
-        const double ld = a.last_det[e];
-        T vol;
-        if (ld * (double)det < 0) {
-            inverted = 1;
-            vol = a.geo[9 * ne + e];  // the reference keeps the previous volume
-        } else {
-            a.last_det[e] = (double)det;
-            vol = (T)(1.0 / 6.0) * (T)fabs(det);
-            const T rdet = (T)1.0 / det;
-            d[1] *= rdet; d[2] *= rdet; d[3] *= rdet;
-            d[5] *= rdet; d[6] *= rdet; d[7] *= rdet;
-            d[9] *= rdet; d[10] *= rdet; d[11] *= rdet;
-        }
-        d[0] = -(d[1] + d[2] + d[3]);
-        d[4] = -(d[5] + d[6] + d[7]);
-        d[8] = -(d[9] + d[10] + d[11]);
+    // shape function derivatives and volume (tetra_element_linear.cpp:222-292)
+    T d[12];
+    d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    T det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
+    const double ld = a.last_det[e];
+    T vol;
+    if (ld * (double)det < 0) {
+        __hip_atomic_fetch_add(&a.S->inverted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vol = a.geo[9 * ne + e];  // the reference keeps the previous volume
+    } else {
+        a.last_det[e] = (double)det;
+        vol = (T)(1.0 / 6.0) * (T)fabs(det);
+        const T rdet = (T)1.0 / det;
+        d[1] *= rdet; d[2] *= rdet; d[3] *= rdet;
+        d[5] *= rdet; d[6] *= rdet; d[7] *= rdet;
+        d[9] *= rdet; d[10] *= rdet; d[11] *= rdet;
+    }
+    d[0] = -(d[1] + d[2] + d[3]);
+    d[4] = -(d[5] + d[6] + d[7]);
+    d[8] = -(d[9] + d[10] + d[11]);
 
-        // shear elastic stress (tetra_element_linear.cpp:375-392)
-        T Ji[3][3];
+    // shear elastic stress (tetra_element_linear.cpp:375-392)
+    T Ji[3][3];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Ji[k / 3][k % 3] = a.jinv0[k * ne + e];
-        T F[3][3], s[3][3];
+    for (int k = 0; k < 9; ++k) Ji[k / 3][k % 3] = a.jinv0[k * ne + e];
+    T F[3][3], s[3][3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                T acc = (T)0;
+        for (int j = 0; j < 3; ++j) {
+            T acc = (T)0;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) acc += J[k][i] * Ji[j][k];
-                F[i][j] = acc;
-            }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                T acc = (T)0;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) acc += F[i][k] * F[j][k];
-                s[i][j] = acc;
-            }
-        const T G = a.G[e], E = a.E[e], v0 = a.vol0[e];
-        const T sc = G * v0 / vol;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) s[i][j] *= sc;
-        s[0][0] -= G;
-        s[1][1] -= G;
-        s[2][2] -= G;
-        // bulk elastic stress (:397-403)
-        {
-            const T c2 = E - G * (T)2.0 / (T)3.0;
-            const T c = G * ((T)1.0 - (v0 / vol)) + (T)0.5 * c2 * ((vol / v0) - (v0 / vol));
-            s[0][0] += c;
-            s[1][1] += c;
-            s[2][2] += c;
+            for (int k = 0; k < 3; ++k) acc += J[k][i] * Ji[j][k];
+            F[i][j] = acc;
         }
-        // fluctuating stress (:412-431), draws bf, s00, s11, s22, s01, s02, s12
-        if (a.calc_noise) {
-            const T A = a.A[e], B = a.B[e];
-            double u[7];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) u[k] = mrg::d_u01(st);
-            T r[7];
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int k = 0; k < 7; ++k) r[k] = (T)((-.5) + ((.5) - (-.5)) * u[k]);
-            const T c = sqrt(((T)24 * (T)a.kT) / (vol * (T)a.dt));
-            const T bf = sqrt(B) * r[0];
-            s[0][0] += c * (sqrt((T)2 * A) * r[1] + bf);
-            s[1][1] += c * (sqrt((T)2 * A) * r[2] + bf);
-            s[2][2] += c * (sqrt((T)2 * A) * r[3] + bf);
-            s[0][1] += c * sqrt(A) * r[4];
-            s[0][2] += c * sqrt(A) * r[5];
-            s[1][2] += c * sqrt(A) * r[6];
-            s[1][0] = s[0][1];
-            s[2][0] = s[0][2];
-            s[2][1] = s[1][2];
+        for (int j = 0; j < 3; ++j) {
+            T acc = (T)0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += F[i][k] * F[j][k];
+            s[i][j] = acc;
         }
-        // apply_stress_tensor (:436-443) then node_force -= du (:468-483)
-        T out[12];
+    const T G = a.G[e], E = a.E[e], v0 = a.vol0[e];
+    const T sc = G * v0 / vol;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const T du = (T)0 + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
-                out[3 * q + i] = (T)0 - du;
-            }
-        T *fo = a.force + (size_t)12 * e;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) fo[k] = out[k];
-        const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
-#pragma unroll
-        for (int k = 0; k < 9; ++k) a.geo[(size_t)k * ne + e] = d[gi[k]];
-        a.geo[(size_t)9 * ne + e] = vol;
+        for (int j = 0; j < 3; ++j) s[i][j] *= sc;
+    s[0][0] -= G;
+    s[1][1] -= G;
+    s[2][2] -= G;
+    // bulk elastic stress (:397-403)
+    {
+        const T c2 = E - G * (T)2.0 / (T)3.0;
+        const T c = G * ((T)1.0 - (v0 / vol)) + (T)0.5 * c2 * ((vol / v0) - (v0 / vol));
+        s[0][0] += c;
+        s[1][1] += c;
+        s[2][2] += c;
     }
-    // inversion count and the stream advance, done by the last block to finish
-    double v[1] = {(double)inverted}, tot[1];
-    if (grid_reduce<1>(v, a.partials, &a.S->ticket[0], tot)) {
-        if (threadIdx.x == 0) {
-            a.S->inverted = (int)tot[0];
-            if (a.calc_noise) {
-                uint32_t s6[6];
-                for (int i = 0; i < 6; ++i) s6[i] = a.S->rng[i];
-                mrg::d_apply(a.step_jump, s6);
-                for (int i = 0; i < 6; ++i) a.S->rng[i] = s6[i];
-            }
-        }
+    // fluctuating stress (:412-431), draws bf, s00, s11, s22, s01, s02, s12
+    if (a.calc_noise) {
+        const T A = a.A[e], B = a.B[e];
+        double u[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) u[k] = mrg::d_u01(st);
+        T r[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) r[k] = (T)((-.5) + ((.5) - (-.5)) * u[k]);
+        const T c = sqrt(((T)24 * (T)a.kT) / (vol * (T)a.dt));
+        const T bf = sqrt(B) * r[0];
+        s[0][0] += c * (sqrt((T)2 * A) * r[1] + bf);
+        s[1][1] += c * (sqrt((T)2 * A) * r[2] + bf);
+        s[2][2] += c * (sqrt((T)2 * A) * r[3] + bf);
+        s[0][1] += c * sqrt(A) * r[4];
+        s[0][2] += c * sqrt(A) * r[5];
+        s[1][2] += c * sqrt(A) * r[6];
+        s[1][0] = s[0][1];
+        s[2][0] = s[0][2];
+        s[2][1] = s[1][2];
     }
+    // apply_stress_tensor (:436-443) then node_force -= du (:468-483)
+    T out[12];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T du = (T)0 + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
+            out[3 * q + i] = (T)0 - du;
+        }
+    T *fo = a.force + (size_t)12 * e;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) fo[k] = out[k];
+    const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.geo[(size_t)k * ne + e] = d[gi[k]];
+    a.geo[(size_t)9 * ne + e] = vol;
 }
 
 // ---------------------------------------------------------------- matrix assembly
@@ -409,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 // ---------------------------------------------------------------- gather + CG start
 template <typename T>
 struct GatherArgs {
-    int nl;
+    int nl, rows_per_block;
     const uint32_t *inc_ptr;    // [nl+1]
     const uint32_t *inc;        // e*4 + slot, element order
     const T *force;             // [ne][12]
@@ -418,19 +464,26 @@ struct GatherArgs {
     const double *fold;         // [..][3]: linearise_force increments in element order
     const unsigned char *pinned;
     const double *dinv;
+    const double *pos;          // [nl][3] (rigid modes)
+    double3 centre;
     double *rhs, *r, *z, *x;
+    double *P;                  // partials
     CgScalars *S;
-    double *partials;
 };
 
 // Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
-// start of NoMassCGSolver::solve (NoMassCGSolver.cpp:196-220).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gather_init(GatherArgs<T> a) {
+// start of NoMassCGSolver::solve (NoMassCGSolver.cpp:196-220). With PROJ (free blob,
+// mixed form) the CG start is left to k_cg_start and the partials are |f|^2, Z^T f
+// and the moments of Z^T Z.
+template <typename T, bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
 #pragma clang fp contract(off)
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    double rz = 0.0, ff = 0.0;
-    if (n < a.nl) {
+    const RowRange rr = row_range(a.nl, a.rows_per_block);
+    constexpr int NV = PROJ ? 17 : 2;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
         double f[3] = {0.0, 0.0, 0.0};
         if (a.ctf) {
             f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
@@ -449,153 +502,263 @@ __global__ __launch_bounds__(kBlock) void k_gather_init(GatherArgs<T> a) {
                 f[2] += a.fold[3 * k + 2];
             }
         if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
-        double zz[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.rhs[3 * n + c] = f[c];
+        if constexpr (!PROJ) {
+            double zz[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                a.r[3 * n + c] = f[c];
+                a.x[3 * n + c] = 0.0;
+                zz[c] = a.dinv[3 * n + c] * f[c];
+                a.z[3 * n + c] = zz[c];
+            }
+            v[0] += f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
+            v[1] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        } else {
+            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
+                                 a.pos[3 * n + 2] - a.centre.z};
+            double zf[6];
+            zt_times(x, f, zf);
+            v[0] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[1 + k] += zf[k];
+            v[7] += 1.0;
+            v[8] += x[0]; v[9] += x[1]; v[10] += x[2];
+            v[11] += x[0] * x[0]; v[12] += x[1] * x[1]; v[13] += x[2] * x[2];
+            v[14] += x[0] * x[1]; v[15] += x[0] * x[2]; v[16] += x[1] * x[2];
+        }
+    }
+    store_partials<NV>(v, a.P, gridDim.x, rr.L);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.S->inverted_step = a.S->inverted;
+        a.S->done = 0;
+        a.S->converged = 0;
+        a.S->iter = 0;
+    }
+}
+
+// Rigid-mode projection of the right-hand side for a free blob in the mixed form:
+// r = f - Z (Z^T Z)^-1 Z^T f, then z = D^-1 r, x = 0, partials r.z.
+__global__ __launch_bounds__(kBlock) void k_cg_start(int nl, int rows_per_block, const double *Pg, const double *rhs,
+                                                    const double *pos, double3 centre, const double *dinv,
+                                                    double *r, double *z, double *x, double *P, CgScalars *S) {
+    const RowRange rr = row_range(nl, rows_per_block);
+    double t[17];
+    sum_partials<17>(Pg, gridDim.x, t);
+    double M[36], Gi[36], cf[6];
+    const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
+    ztz_from_moments(mom, M);
+    invert6(M, Gi);
+    for (int i = 0; i < 6; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; ++j) acc += Gi[6 * i + j] * t[1 + j];
+        cf[i] = acc;
+    }
+    double v[1] = {0.0};
+    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
+        const double xx[3] = {pos[3 * n] - centre.x, pos[3 * n + 1] - centre.y, pos[3 * n + 2] - centre.z};
+        double zc[3], rl[3], zl[3];
+        z_times(xx, cf, zc);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            a.rhs[3 * n + c] = f[c];
-            a.r[3 * n + c] = f[c];
-            a.x[3 * n + c] = 0.0;
-            zz[c] = a.dinv[3 * n + c] * f[c];
-            a.z[3 * n + c] = zz[c];
+            rl[c] = rhs[3 * n + c] - zc[c];
+            r[3 * n + c] = rl[c];
+            x[3 * n + c] = 0.0;
+            zl[c] = dinv[3 * n + c] * rl[c];
+            z[3 * n + c] = zl[c];
         }
-        rz = f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
-        ff = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
     }
-    double v[2] = {rz, ff}, tot[2];
-    if (grid_reduce<2>(v, a.partials, &a.S->ticket[1], tot)) {
-        if (threadIdx.x == 0) {
-            CgScalars *S = a.S;
-            S->delta = tot[0];
-            S->f2 = tot[1];
-            S->iter = 0;
-            S->converged = 0;
-            S->first = 1;
-            S->beta = 0.0;
-            S->done = (S->inverted > 0) ? 1 : 0;  // an inverted mesh is not solved
-        }
+    store_partials<1>(v, P, gridDim.x, rr.L);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->f2 = t[0];
+        for (int i = 0; i < 36; ++i) S->G[i] = Gi[i];
     }
 }
 
 // ---------------------------------------------------------------- CG iteration
 template <typename T>
 struct SpmvArgs {
-    int nl;
+    int nl, rows_per_block, iter, max_iter;
+    double epsilon2;
     const int *slice_w;
     const long long *slice_base;
     const int *col;        // [nslots]
     const T *val;          // [nslots*9]
-    const double *p_old;   // [nl][3]
     const double *z;
-    double *p_new, *q;
+    const double *pos;
+    double3 centre;
+    double *p, *q;         // updated in place
+    const double *Pin;     // partials of the previous kernel (start: r.z [and |f|^2]; later: r.r, r.z)
+    double *P1;            // partials of this kernel
     CgScalars *S;
-    double *partials;
 };
 
-// p = beta p + z (vec3_scale_and_add, mat_vec_fns.cpp:202-212) formed on the fly
-// for every column the row touches, q = V p (SparseMatrixFixedPattern.cpp:148-187),
-// p.q (NoMassCGSolver.cpp:251-265) reduced over the grid, alpha = delta / p.q.
-template <typename T>
+// Iteration `iter`: first the convergence test of the previous iteration
+// (|r|^2/|f|^2 < epsilon^2, NoMassCGSolver.cpp:223-237), then
+// p = beta p + z (mat_vec_fns.cpp:202-212) and q = V p, formed as q = V z + beta q
+// so the row gathers only z (SparseMatrixFixedPattern.cpp:148-187), and the p.q
+// partials (NoMassCGSolver.cpp:251-265).
+template <typename T, bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
-    const CgScalars *S = a.S;
+    CgScalars *S = a.S;
     if (S->done) return;
-    const bool first = S->first != 0;
-    const double beta = S->beta;
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    double pq = 0.0;
-    if (n < a.nl) {
+    const int i = a.iter;
+    const bool first = (i == 0);
+    double beta = 0.0;
+    if (first) {
+        if (S->inverted_step > 0) {  // an inverted mesh is not solved
+            if (blockIdx.x == 0 && threadIdx.x == 0) S->done = 1;
+            return;
+        }
+        if constexpr (PROJ) {
+            double t[1];
+            sum_partials<1>(a.Pin, gridDim.x, t);
+            if (blockIdx.x == 0 && threadIdx.x == 0) S->delta[1] = t[0];
+        } else {
+            double t[2];
+            sum_partials<2>(a.Pin, gridDim.x, t);
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                S->delta[1] = t[0];
+                S->f2 = t[1];
+            }
+        }
+    } else {
+        double t[2];
+        sum_partials<2>(a.Pin, gridDim.x, t);  // r.r, r.z of iteration i-1
+        const double f2 = S->f2;
+        const double res = (f2 == 0.0) ? 0.0 : t[0] / f2;
+        if (res < a.epsilon2 || i >= a.max_iter) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                S->converged = res < a.epsilon2 ? 1 : 0;
+                S->iter = i;
+                S->done = 1;
+            }
+            return;
+        }
+        beta = t[1] / S->delta[(i - 2) & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->delta[(i - 1) & 1] = t[1];
+    }
+    const RowRange rr = row_range(a.nl, a.rows_per_block);
+    constexpr int NV = PROJ ? 13 : 1;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
         const int sl = n / kSlice, lane = n % kSlice;
         const int W = a.slice_w[sl];
         const long long base = a.slice_base[sl];
         const int *colp = a.col + base + lane;
         const T *vp = a.val + (size_t)base * 9 + lane;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+#pragma unroll 2
         for (int j = 0; j < W; ++j) {
             const int c = colp[(size_t)j * kSlice];
-            const T *v = vp + (size_t)j * 9 * kSlice;
-            double pc[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                pc[k] = first ? a.z[3 * c + k] : beta * a.p_old[3 * c + k] + a.z[3 * c + k];
-            acc0 += (double)v[0 * kSlice] * pc[0]; acc0 += (double)v[1 * kSlice] * pc[1]; acc0 += (double)v[2 * kSlice] * pc[2];
-            acc1 += (double)v[3 * kSlice] * pc[0]; acc1 += (double)v[4 * kSlice] * pc[1]; acc1 += (double)v[5 * kSlice] * pc[2];
-            acc2 += (double)v[6 * kSlice] * pc[0]; acc2 += (double)v[7 * kSlice] * pc[1]; acc2 += (double)v[8 * kSlice] * pc[2];
+            const T *vv = vp + (size_t)j * 9 * kSlice;
+            const double z0 = a.z[3 * c], z1 = a.z[3 * c + 1], z2 = a.z[3 * c + 2];
+            acc0 += (double)vv[0 * kSlice] * z0; acc0 += (double)vv[1 * kSlice] * z1; acc0 += (double)vv[2 * kSlice] * z2;
+            acc1 += (double)vv[3 * kSlice] * z0; acc1 += (double)vv[4 * kSlice] * z1; acc1 += (double)vv[5 * kSlice] * z2;
+            acc2 += (double)vv[6 * kSlice] * z0; acc2 += (double)vv[7 * kSlice] * z1; acc2 += (double)vv[8 * kSlice] * z2;
         }
-        double pn[3];
+        double pn[3], qn[3] = {acc0, acc1, acc2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            pn[k] = first ? a.z[3 * n + k] : beta * a.p_old[3 * n + k] + a.z[3 * n + k];
-            a.p_new[3 * n + k] = pn[k];
+            const double zk = a.z[3 * n + k];
+            if (first) {
+                pn[k] = zk;
+            } else {
+                pn[k] = beta * a.p[3 * n + k] + zk;
+                qn[k] = beta * a.q[3 * n + k] + qn[k];
+            }
+            a.p[3 * n + k] = pn[k];
+            a.q[3 * n + k] = qn[k];
         }
-        a.q[3 * n] = acc0;
-        a.q[3 * n + 1] = acc1;
-        a.q[3 * n + 2] = acc2;
-        pq = pn[0] * acc0 + pn[1] * acc1 + pn[2] * acc2;
-    }
-    double v[1] = {pq}, tot[1];
-    if (grid_reduce<1>(v, a.partials, &a.S->ticket[2], tot)) {
-        if (threadIdx.x == 0) {
-            CgScalars *W = a.S;
-            W->pTq = tot[0];
-            W->alpha = W->delta / tot[0];
-            W->first = 0;
+        v[0] += pn[0] * qn[0] + pn[1] * qn[1] + pn[2] * qn[2];
+        if constexpr (PROJ) {
+            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
+                                 a.pos[3 * n + 2] - a.centre.z};
+            double zq[6], zp[6];
+            zt_times(x, qn, zq);
+            zt_times(x, pn, zp);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                v[1 + k] += zq[k];
+                v[7 + k] += zp[k];
+            }
         }
     }
+    store_partials<NV>(v, a.P1, gridDim.x, rr.L);
 }
 
 struct VecArgs {
-    int nl;
+    int nl, rows_per_block, iter;
     const double *p, *q, *dinv;
+    const double *pos;
+    double3 centre;
     double *x, *r, *z;
-    double epsilon2;
-    int max_iter;
+    const double *P1;  // partials of the SpMV step
+    double *P2;        // r.r, r.z partials of this step
     CgScalars *S;
-    double *partials;
 };
 
-// x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197), |r|^2/|f|^2 test
-// (NoMassCGSolver.cpp:223-237), z = D^-1 r and r.z (:268-281), beta = r.z / delta.
+// alpha = delta / p.q, x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197),
+// z = D^-1 r (NoMassCGSolver.cpp:268-281), partials of r.r and r.z.
+// With PROJ, q is first projected onto the range of the operator:
+// q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
+template <bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
-    const CgScalars *S = a.S;
+    CgScalars *S = a.S;
     if (S->done) return;
-    const double alpha = S->alpha;
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    double rr = 0.0, rz = 0.0;
-    if (n < a.nl) {
+    constexpr int NV = PROJ ? 13 : 1;
+    double t[NV];
+    sum_partials<NV>(a.P1, gridDim.x, t);
+    double pq = t[0];
+    double cq[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (PROJ) {
+        for (int k = 0; k < 6; ++k) {
+            double acc = 0.0;
+            for (int j = 0; j < 6; ++j) acc += S->G[6 * k + j] * t[1 + j];
+            cq[k] = acc;
+        }
+        for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
+    }
+    const double alpha = S->delta[(a.iter - 1) & 1] / pq;
+    const RowRange rr = row_range(a.nl, a.rows_per_block);
+    double v[2] = {0.0, 0.0};
+    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
+        double qn[3] = {a.q[3 * n], a.q[3 * n + 1], a.q[3 * n + 2]};
+        if constexpr (PROJ) {
+            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
+                                 a.pos[3 * n + 2] - a.centre.z};
+            double zc[3];
+            z_times(x, cq, zc);
+            qn[0] -= zc[0];
+            qn[1] -= zc[1];
+            qn[2] -= zc[2];
+        }
         double r3[3], z3[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             a.x[3 * n + k] += alpha * a.p[3 * n + k];
-            r3[k] = a.r[3 * n + k] + (-alpha) * a.q[3 * n + k];
+            r3[k] = a.r[3 * n + k] + (-alpha) * qn[k];
             a.r[3 * n + k] = r3[k];
             z3[k] = a.dinv[3 * n + k] * r3[k];
             a.z[3 * n + k] = z3[k];
         }
-        rr = r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
-        rz = r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
+        v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
+        v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
     }
-    double v[2] = {rr, rz}, tot[2];
-    if (grid_reduce<2>(v, a.partials, &a.S->ticket[3], tot)) {
-        if (threadIdx.x == 0) {
-            CgScalars *W = a.S;
-            W->rr = tot[0];
-            W->iter += 1;
-            const double res = (W->f2 == 0.0) ? 0.0 : tot[0] / W->f2;
-            if (res < a.epsilon2) {
-                W->converged = 1;
-                W->done = 1;
-            } else {
-                W->beta = tot[1] / W->delta;
-                W->delta = tot[1];
-                if (W->iter >= a.max_iter) W->done = 1;
-            }
-        }
-    }
+    store_partials<2>(v, a.P2, gridDim.x, rr.L);
 }
 
 // Blob::euler_integrate (Blob.cpp:3249-3263), mass-less branch: vel = x, pos += x dt
 __global__ __launch_bounds__(kBlock) void k_update(int nl, const double *x, double *vel, double *pos, double dt,
-                                                   const CgScalars *S) {
-    if (!S->converged) return;
+                                                   CgScalars *S) {
+    const int converged = S->converged;
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->inverted = 0;
+    if (!converged) return;
     const int n = blockIdx.x * kBlock + threadIdx.x;
     if (n >= nl) return;
 #pragma unroll
@@ -604,6 +767,13 @@ __global__ __launch_bounds__(kBlock) void k_update(int nl, const double *x, doub
         vel[3 * n + k] = v;
         pos[3 * n + k] += v * dt;
     }
+}
+
+// ---------------------------------------------------------------- copy probe
+// y = x over n doubles (16-byte vectors), grid-stride: reads n*8 and writes n*8 bytes
+__global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += stride) y[i] = x[i];
 }
 
 // ================================================================ host side
@@ -662,14 +832,19 @@ struct ffea_gpu_blob {
     DevBuf<long long> slice_base;
     DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, fold_ptr;
     DevBuf<unsigned char> diag_j, pinned;
-    DevBuf<double> pos, vel, last_det, dinv, rhs, r, z, x, q, pbuf0, pbuf1, partials, ctf, fold;
+    DevBuf<double> pos, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold;
+    DevBuf<double> Pg, Pi, P1, P2;  // per-block partials: gather, CG start, SpMV step, vector step
     DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
     DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
     DevBuf<mrg::Jump> jump_group, jump_lane;
     DevBuf<CgScalars> S;
     mrg::Jump step_jump{};
+    uint32_t rng[6] = {0, 0, 0, 0, 0, 0};  // noise stream state (host copy)
     bool have_ctf = false, have_fold = false;
+    bool project = false;     // rigid-mode projection in the CG (mixed form, free blob)
+    double3 centre{0, 0, 0};  // rest centroid of the corner nodes
     int grid_nodes = 0, grid_elems = 0;
+    int nb = 0, rows_per_block = 0;  // row kernels: fixed grid and rows per block
 
     // host-visible status
     CgScalars *h_status = nullptr;  // pinned [3]
@@ -679,7 +854,7 @@ struct ffea_gpu_blob {
     int expect_iters = 0;
 
     // timing
-    bool timing = false;
+    unsigned timing = 0;  // bit i: time phase i
     Phase phase[FFEA_GPU_NUM_PHASES];
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -703,17 +878,19 @@ struct ffea_gpu_blob {
     }
     void t_begin(int ph, hipEvent_t *out) {
         *out = nullptr;
-        if (!timing) return;
+        if (!(timing >> ph & 1u)) return;
         *out = take_event();
         FFEA_HIP_TRY(hipEventRecord(*out, stream));
     }
     void t_end(int ph, hipEvent_t b) {
-        if (!timing) return;
+        if (!(timing >> ph & 1u)) return;
         hipEvent_t e = take_event();
         FFEA_HIP_TRY(hipEventRecord(e, stream));
         phase[ph].ev.push_back(b);
         phase[ph].ev.push_back(e);
-        phase[ph].launches++;
+        // CG kernels queued after convergence return at once; they are counted
+        // by the iterations actually done (see solve_and_update)
+        if (ph != 3 && ph != 4) phase[ph].launches++;
     }
     void collect_timing() {
         if (!timing) return;
@@ -736,7 +913,8 @@ struct ffea_gpu_blob {
     void upload_elements(const std::vector<double> &jinv0, const std::vector<double> &vol0, const double *A,
                          const double *B, const double *G, const double *E);
     void launch_element();
-    void launch_positions();
+    template <typename T, bool PROJ>
+    void launch_solve(const T *val, const T *force);
     int check_status_after_forces();
     int solve_and_update();
 };
@@ -1052,11 +1230,21 @@ void ffea_gpu_blob::build(const double *pos_h, const int *top, const double *A, 
     cptr.alloc(cnt.size(), &dev_bytes); cptr.upload(cnt, stream);
     contrib.alloc(contrib_h.size(), &dev_bytes); contrib.upload(contrib_h, stream);
     diag_j.alloc(nl, &dev_bytes); diag_j.upload(dj, stream);
-    for (DevBuf<double> *b : {&dinv, &rhs, &r, &z, &x, &q, &pbuf0, &pbuf1}) {
+    for (DevBuf<double> *b : {&dinv, &rhs, &r, &z, &x, &q, &p}) {
         b->alloc((size_t)3 * nl, &dev_bytes);
         FFEA_HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(double) * 3 * nl, stream));
     }
-    partials.alloc((size_t)2 * std::max(grid_nodes, grid_elems), &dev_bytes);
+    nb = std::min((nl + kBlock - 1) / kBlock, 1024);
+    if (nb >= 8) nb -= nb % 8;
+    rows_per_block = ((nl + nb - 1) / nb + kSlice - 1) / kSlice * kSlice;
+    nb = (nl + rows_per_block - 1) / rows_per_block;
+    if (nb >= 8 && nb % 8) {  // keep the XCD mapping: grow to a multiple of 8 blocks
+        nb += 8 - nb % 8;
+    }
+    Pg.alloc((size_t)17 * nb, &dev_bytes);
+    Pi.alloc((size_t)2 * nb, &dev_bytes);
+    P1.alloc((size_t)13 * nb, &dev_bytes);
+    P2.alloc((size_t)2 * nb, &dev_bytes);
     if (have_ctf) { ctf.alloc(ctf_h.size(), &dev_bytes); ctf.upload(ctf_h, stream); }
     if (have_fold) {
         fold_ptr.alloc(fp.size(), &dev_bytes); fold_ptr.upload(fp, stream);
@@ -1066,8 +1254,33 @@ void ffea_gpu_blob::build(const double *pos_h, const int *top, const double *A, 
     jump_lane.alloc(64, &dev_bytes); jump_lane.upload(jl, stream);
     S.alloc(1, &dev_bytes);
     CgScalars s0{};
-    for (int i = 0; i < 6; ++i) s0.rng[i] = rng_state[i];
     S.upload(&s0, 1, stream);
+    for (int i = 0; i < 6; ++i) rng[i] = rng_state[i];
+    // free blob (no pins, no stokes drag): the viscosity matrix is singular with the 6
+    // rigid modes as null space. The mixed form projects them out of every SpMV result.
+    {
+        double c[3] = {0, 0, 0};
+        for (int n = 0; n < nl; ++n)
+            for (int k = 0; k < 3; ++k) c[k] += p0[3 * n + k];
+        centre = make_double3(c[0] / nl, c[1] / nl, c[2] / nl);
+        bool connected = true;
+        {  // union-find over element corners
+            std::vector<int> par(nl);
+            std::iota(par.begin(), par.end(), 0);
+            std::function<int(int)> find = [&](int v) {
+                while (par[v] != v) v = par[v] = par[par[v]];
+                return v;
+            };
+            for (int e = 0; e < ne; ++e)
+                for (int k = 1; k < 4; ++k) {
+                    int ra = find(conn_h[e]), rb = find(conn_h[(size_t)k * ne + e]);
+                    if (ra != rb) par[ra] = rb;
+                }
+            const int r0 = find(0);
+            for (int n = 1; n < nl && connected; ++n) connected = find(n) == r0;
+        }
+        project = mixed && npin == 0 && connected;
+    }
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     if (mixed) {
         upload_elements<float>(jinv0, vol0, A, B, G, E);
@@ -1089,16 +1302,18 @@ void ffea_gpu_blob::launch_element() {
     if (mixed) {
         ElemArgs<float> a{ne, conn.p, pos.p, f_jinv0.p, f_vol0.p, f_G.p, f_E.p, f_A.p, f_B.p, last_det.p,
                           f_force.p, f_geo.p, prm.calc_noise, prm.kT, prm.dt, jump_group.p, jump_lane.p,
-                          step_jump, S.p, partials.p};
+                          {rng[0], rng[1], rng[2], rng[3], rng[4], rng[5]}, S.p};
         hipLaunchKernelGGL(k_element<float>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     } else {
         ElemArgs<double> a{ne, conn.p, pos.p, d_jinv0.p, d_vol0.p, d_G.p, d_E.p, d_A.p, d_B.p, last_det.p,
                            d_force.p, d_geo.p, prm.calc_noise, prm.kT, prm.dt, jump_group.p, jump_lane.p,
-                           step_jump, S.p, partials.p};
+                           {rng[0], rng[1], rng[2], rng[3], rng[4], rng[5]}, S.p};
         hipLaunchKernelGGL(k_element<double>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(0, t0);
+    // the pass consumed 7 draws per element (tetra_element_linear.cpp:416-426)
+    if (prm.calc_noise) mrg::h_apply(step_jump, rng, rng);
 }
 
 int ffea_gpu_blob::check_status_after_forces() {
@@ -1111,6 +1326,58 @@ int ffea_gpu_blob::check_status_after_forces() {
         return FFEA_GPU_ERR_INVERSION;
     }
     return FFEA_GPU_OK;
+}
+
+template <typename T, bool PROJ>
+void ffea_gpu_blob::launch_solve(const T *val, const T *force) {
+    hipEvent_t t0;
+    t_begin(2, &t0);
+    GatherArgs<T> g{nl, rows_per_block, inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
+                    have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr, pinned.p, dinv.p, pos.p,
+                    centre, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p, S.p};
+    hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, g);
+    if (PROJ)
+        hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kBlock), 0, stream, nl, rows_per_block, (const double *)Pg.p,
+                           (const double *)rhs.p, (const double *)pos.p, centre, (const double *)dinv.p, r.p, z.p,
+                           x.p, Pi.p, S.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    t_end(2, t0);
+
+    // CG iterations, queued in chunks. Iteration i is an SpMV step (which first tests
+    // the convergence of iteration i-1) and a vector step. Kernels queued after the
+    // solve stopped return at once; the host stops queuing once a completed chunk
+    // reports done.
+    const int imax = prm.max_iterations_cg;
+    int launched = 0, chunk_idx = 0;
+    int chunk = std::max(4, expect_iters - 2);
+    bool stop = false;
+    while (launched <= imax && !stop) {
+        const int nit = std::min(chunk, imax + 1 - launched);
+        for (int k = 0; k < nit; ++k) {
+            const int it = launched + k;
+            t_begin(3, &t0);
+            SpmvArgs<T> sa{nl, rows_per_block, it, imax, prm.epsilon2, slice_w.p, slice_base.p, col.p, val, z.p,
+                           pos.p, centre, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p, S.p};
+            hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
+            t_end(3, t0);
+            if (it == imax) break;  // that SpMV step only tests the last iteration
+            t_begin(4, &t0);
+            VecArgs va{nl, rows_per_block, it, p.p, q.p, dinv.p, pos.p, centre, x.p, r.p, z.p, P1.p, P2.p, S.p};
+            hipLaunchKernelGGL((k_cg_vec<PROJ>), dim3(nb), dim3(kBlock), 0, stream, va);
+            t_end(4, t0);
+        }
+        FFEA_HIP_TRY(hipGetLastError());
+        launched += nit;
+        const int slot = chunk_idx & 1;
+        FFEA_HIP_TRY(hipMemcpyAsync(&h_status[slot], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipEventRecord(chunk_ev[slot], stream));
+        if (chunk_idx > 0) {
+            FFEA_HIP_TRY(hipEventSynchronize(chunk_ev[slot ^ 1]));
+            if (h_status[slot ^ 1].done) stop = true;
+        }
+        ++chunk_idx;
+        chunk = 4;
+    }
 }
 
 int ffea_gpu_blob::solve_and_update() {
@@ -1128,74 +1395,28 @@ int ffea_gpu_blob::solve_and_update() {
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(1, t0);
-    // gather + CG start
-    t_begin(2, &t0);
     if (mixed) {
-        GatherArgs<float> g{nl, inc_ptr.p, inc.p, f_force.p, have_ctf ? ctf.p : nullptr,
-                            have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr, pinned.p, dinv.p,
-                            rhs.p, r.p, z.p, x.p, S.p, partials.p};
-        hipLaunchKernelGGL(k_gather_init<float>, dim3(grid_nodes), dim3(kBlock), 0, stream, g);
+        if (project) launch_solve<float, true>(f_val.p, f_force.p);
+        else launch_solve<float, false>(f_val.p, f_force.p);
     } else {
-        GatherArgs<double> g{nl, inc_ptr.p, inc.p, d_force.p, have_ctf ? ctf.p : nullptr,
-                             have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr, pinned.p, dinv.p,
-                             rhs.p, r.p, z.p, x.p, S.p, partials.p};
-        hipLaunchKernelGGL(k_gather_init<double>, dim3(grid_nodes), dim3(kBlock), 0, stream, g);
-    }
-    FFEA_HIP_TRY(hipGetLastError());
-    t_end(2, t0);
-
-    // CG iterations in chunks; the kernels stop themselves once done, the host
-    // stops queuing once a completed chunk reports done
-    const int imax = prm.max_iterations_cg;
-    int launched = 0, chunk_idx = 0;
-    int chunk = std::max(8, expect_iters - 4);
-    bool stop = false;
-    double *pb[2] = {pbuf0.p, pbuf1.p};
-    while (launched < imax && !stop) {
-        const int nit = std::min(chunk, imax - launched);
-        for (int i = 0; i < nit; ++i) {
-            const int it = launched + i;
-            double *p_old = pb[it & 1], *p_new = pb[(it + 1) & 1];
-            t_begin(3, &t0);
-            if (mixed) {
-                SpmvArgs<float> s{nl, slice_w.p, slice_base.p, col.p, f_val.p, p_old, z.p, p_new, q.p, S.p, partials.p};
-                hipLaunchKernelGGL(k_cg_spmv<float>, dim3(grid_nodes), dim3(kBlock), 0, stream, s);
-            } else {
-                SpmvArgs<double> s{nl, slice_w.p, slice_base.p, col.p, d_val.p, p_old, z.p, p_new, q.p, S.p, partials.p};
-                hipLaunchKernelGGL(k_cg_spmv<double>, dim3(grid_nodes), dim3(kBlock), 0, stream, s);
-            }
-            t_end(3, t0);
-            t_begin(4, &t0);
-            VecArgs v{nl, p_new, q.p, dinv.p, x.p, r.p, z.p, prm.epsilon2, imax, S.p, partials.p};
-            hipLaunchKernelGGL(k_cg_vec, dim3(grid_nodes), dim3(kBlock), 0, stream, v);
-            t_end(4, t0);
-        }
-        FFEA_HIP_TRY(hipGetLastError());
-        launched += nit;
-        const int slot = chunk_idx & 1;
-        FFEA_HIP_TRY(hipMemcpyAsync(&h_status[slot], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
-        FFEA_HIP_TRY(hipEventRecord(chunk_ev[slot], stream));
-        if (chunk_idx > 0) {
-            FFEA_HIP_TRY(hipEventSynchronize(chunk_ev[slot ^ 1]));
-            if (h_status[slot ^ 1].done) stop = true;
-        }
-        ++chunk_idx;
-        chunk = 8;
+        launch_solve<double, false>(d_val.p, d_force.p);
     }
     // Euler update (skipped unless converged)
     t_begin(5, &t0);
-    hipLaunchKernelGGL(k_update, dim3(grid_nodes), dim3(kBlock), 0, stream, nl, x.p, vel.p, pos.p, prm.dt,
-                       (const CgScalars *)S.p);
+    hipLaunchKernelGGL(k_update, dim3(grid_nodes), dim3(kBlock), 0, stream, nl, (const double *)x.p, vel.p, pos.p,
+                       prm.dt, S.p);
     FFEA_HIP_TRY(hipGetLastError());
     t_end(5, t0);
     FFEA_HIP_TRY(hipMemcpyAsync(&h_status[2], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     const CgScalars &st = h_status[2];
+    if (timing >> 3 & 1u) phase[3].launches += st.iter;
+    if (timing >> 4 & 1u) phase[4].launches += st.iter;
     last_iters = st.iter;
     total_iters += st.iter;
-    expect_iters = st.iter;
-    if (st.inverted > 0) {
-        const int inv = st.inverted;
+    if (st.iter > 0) expect_iters = st.iter;
+    if (st.inverted_step > 0) {
+        const int inv = st.inverted_step;
         err = inv == 1 ? "1 element has inverted since the last step. Aborting simulation.\n"
                        : fmt("%d elements have inverted since the last step. Aborting simulation.\n", inv);
         return FFEA_GPU_ERR_INVERSION;
@@ -1203,7 +1424,7 @@ int ffea_gpu_blob::solve_and_update() {
     if (!st.converged) {
         err = fmt("Conjugate gradient solver: Could not converge after %d iterations.\n\tEither epsilon or "
                   "max_iterations_cg are set too low, or something went wrong with the simulation.\n",
-                  imax);
+                  prm.max_iterations_cg);
         return FFEA_GPU_ERR_CG_NO_CONVERGE;
     }
     return FFEA_GPU_OK;
@@ -1420,13 +1641,8 @@ int ffea_gpu_blob_get_cg_iterations(ffea_gpu_blob *b, int *last, long long *tota
 
 int ffea_gpu_blob_get_rng_state(ffea_gpu_blob *b, uint32_t state[6]) {
     if (!b || !state) return FFEA_GPU_ERR_INVALID;
-    return guarded(b, [&]() {
-        FFEA_HIP_TRY(hipSetDevice(b->device));
-        FFEA_HIP_TRY(hipMemcpyAsync(b->h_status, b->S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, b->stream));
-        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-        for (int i = 0; i < 6; ++i) state[i] = b->h_status->rng[i];
-        return (int)FFEA_GPU_OK;
-    });
+    for (int i = 0; i < 6; ++i) state[i] = b->rng[i];
+    return FFEA_GPU_OK;
 }
 
 int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *b, const uint32_t state[6]) {
@@ -1435,15 +1651,8 @@ int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *b, const uint32_t state[6]) {
         b->err = "invalid MRG32k3a state";
         return FFEA_GPU_ERR_INVALID;
     }
-    return guarded(b, [&]() {
-        FFEA_HIP_TRY(hipSetDevice(b->device));
-        FFEA_HIP_TRY(hipMemcpyAsync(b->h_status, b->S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, b->stream));
-        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-        for (int i = 0; i < 6; ++i) b->h_status->rng[i] = state[i];
-        FFEA_HIP_TRY(hipMemcpyAsync(b->S.p, b->h_status, sizeof(CgScalars), hipMemcpyHostToDevice, b->stream));
-        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-        return (int)FFEA_GPU_OK;
-    });
+    for (int i = 0; i < 6; ++i) b->rng[i] = state[i];
+    return FFEA_GPU_OK;
 }
 
 int ffea_gpu_blob_get_info(ffea_gpu_blob *b, ffea_gpu_blob_info *info) {
@@ -1458,11 +1667,11 @@ int ffea_gpu_blob_get_info(ffea_gpu_blob *b, ffea_gpu_blob_info *info) {
     return FFEA_GPU_OK;
 }
 
-int ffea_gpu_blob_set_timing(ffea_gpu_blob *b, int enable) {
+int ffea_gpu_blob_set_timing(ffea_gpu_blob *b, unsigned phase_mask) {
     if (!b) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         b->collect_timing();
-        b->timing = enable != 0;
+        b->timing = phase_mask & ((1u << FFEA_GPU_NUM_PHASES) - 1u);
         for (auto &p : b->phase) {
             p.ms = 0;
             p.launches = 0;
@@ -1489,6 +1698,35 @@ int ffea_gpu_blob_synchronize(ffea_gpu_blob *b) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_copy) {
+    if (bytes <= 0 || iters <= 0 || !ms_per_copy) return FFEA_GPU_ERR_INVALID;
+    return guarded(nullptr, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(device));
+        const size_t n2 = (size_t)bytes / 16;
+        double2 *x = nullptr, *y = nullptr;
+        FFEA_HIP_TRY(hipMalloc(&x, n2 * 16));
+        FFEA_HIP_TRY(hipMalloc(&y, n2 * 16));
+        FFEA_HIP_TRY(hipMemset(x, 0, n2 * 16));
+        hipEvent_t e0, e1;
+        FFEA_HIP_TRY(hipEventCreate(&e0));
+        FFEA_HIP_TRY(hipEventCreate(&e1));
+        const int grid = 256 * 32;
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, x, y, n2);  // warm-up
+        FFEA_HIP_TRY(hipEventRecord(e0, 0));
+        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, x, y, n2);
+        FFEA_HIP_TRY(hipEventRecord(e1, 0));
+        FFEA_HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        FFEA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(x);
+        (void)hipFree(y);
         return (int)FFEA_GPU_OK;
     });
 }

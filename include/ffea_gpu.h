@@ -125,13 +125,21 @@ typedef struct {
 } ffea_gpu_blob_info;
 int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
 
-/* Kernel timing with HIP events recorded on the handle's stream around each launch.
+/* Kernel timing with HIP events recorded on the handle's stream around each launch of
+ * the phases selected by phase_mask (bit i = phase i; 0 turns timing off).
  * Phases: 0 element forces, 1 matrix assembly, 2 gather + CG start, 3 CG SpMV step,
- * 4 CG vector step, 5 position update. Times in ms, summed since the last reset. */
+ * 4 CG vector step, 5 position update. Times in ms, summed since the last set_timing;
+ * launches of phases 3 and 4 count CG iterations done (launches queued after
+ * convergence return at once and are not counted). */
 #define FFEA_GPU_NUM_PHASES 6
-int ffea_gpu_blob_set_timing(ffea_gpu_blob *blob, int enable);
+int ffea_gpu_blob_set_timing(ffea_gpu_blob *blob, unsigned phase_mask);
 int ffea_gpu_blob_get_timing(ffea_gpu_blob *blob, double ms[FFEA_GPU_NUM_PHASES],
                              long long launches[FFEA_GPU_NUM_PHASES]);
+
+/* Achievable-bandwidth probe: mean time of a device-to-device copy kernel moving
+ * `bytes` in and `bytes` out, over `iters` launches. Also the calibration kernel for
+ * hardware byte counters. No counterpart in the reference. */
+int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_copy);
 
 /* waits for all work queued on the handle's stream */
 int ffea_gpu_blob_synchronize(ffea_gpu_blob *blob);

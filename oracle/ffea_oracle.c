@@ -39,6 +39,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ======================================================================
  * MRG32k3a (RngStream.cpp; L'Ecuyer, Simard, Chen, Kelton, Oper. Res. 50(6), 2002)
@@ -970,4 +971,48 @@ void orc_element_noise_stress(double vol, double A, double B, double kT, double 
     double s[3][3] = {{0}};
     elem_noise_stress(vol, A, B, kT, dt, u7, s);
     memcpy(stress9, s, sizeof(s));
+}
+
+/* ---- timing sample for the CPU baseline: one step with the CG capped at
+ * `cg_iters` iterations; wall times of the element pass (update_internal_forces),
+ * of gather + matrix build, and per CG iteration. Positions are not updated. */
+static double orc_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int orc_blob_time_step(orc_blob *b, int cg_iters, double *t_elem, double *t_build, double *t_iter) {
+    memset(b->node_force, 0, sizeof(double) * 30 * (size_t)b->ne);
+    memset(b->force, 0, sizeof(double) * 3 * (size_t)b->nt);
+    double t0 = orc_now();
+    int inv = orc_blob_update_internal_forces(b);
+    double t1 = orc_now();
+    const int nt = b->nt;
+    double *F = b->force;
+    for (int n = 0; n < nt; ++n)
+        for (int m = b->contrib_ptr[n]; m < b->contrib_ptr[n + 1]; ++m) {
+            const double *src = b->node_force + 3 * b->contrib[m];
+            F[3 * n] += src[0];
+            F[3 * n + 1] += src[1];
+            F[3 * n + 2] += src[2];
+        }
+    linearise_force(b);
+    for (int k = 0; k < b->npin; ++k)
+        for (int c = 0; c < 3; ++c) F[3 * b->pinned[k] + c] = 0;
+    matrix_build(b);
+    double t2 = orc_now();
+    const int keep = b->p.max_iterations_cg;
+    b->p.max_iterations_cg = cg_iters;
+    double *x = malloc(sizeof(double) * 3 * (size_t)nt);
+    memcpy(x, F, sizeof(double) * 3 * (size_t)nt);
+    double t3 = orc_now();
+    int it = cg_solve(b, x);
+    double t4 = orc_now();
+    b->p.max_iterations_cg = keep;
+    free(x);
+    *t_elem = t1 - t0;
+    *t_build = t2 - t1;
+    *t_iter = (t4 - t3) / (it > 0 ? it : cg_iters);
+    return inv;
 }

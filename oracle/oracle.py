@@ -97,6 +97,8 @@ def lib():
         L.orc_blob_apply_matrix.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_blob_solve.argtypes = [C.c_void_p, _dp]
         L.orc_blob_solve.restype = C.c_int
+        L.orc_blob_time_step.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.orc_blob_time_step.restype = C.c_int
         L.orc_element_elastic_force.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_element_shape.argtypes = [_dp, _dp, P(C.c_double)]
         L.orc_element_shape.restype = C.c_double
@@ -303,6 +305,12 @@ class Blob:
         out = np.zeros(3 * self.nt)
         lib().orc_blob_apply_matrix(self.h, np.ascontiguousarray(x, np.float64).reshape(-1), out)
         return out.reshape(-1, 3)
+
+    def time_step(self, cg_iters):
+        """(t_element, t_gather_and_build, t_per_cg_iteration) in seconds for one step"""
+        te, tb, ti = C.c_double(), C.c_double(), C.c_double()
+        lib().orc_blob_time_step(self.h, int(cg_iters), C.byref(te), C.byref(tb), C.byref(ti))
+        return te.value, tb.value, ti.value
 
     def solve(self, b):
         x = np.ascontiguousarray(b, np.float64).reshape(-1).copy()

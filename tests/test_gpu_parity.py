@@ -127,3 +127,27 @@ def test_mixed_precision_close(oracle_mod):
         assert o.step() == 0
     g.step(20)
     assert rel_err(g.positions(), o.positions()) <= 1e-5
+
+
+def test_positions_medium_mesh(oracle_mod):
+    """16^3 cubes (24576 tets): many row blocks, XCD-mapped reductions"""
+    spec = workloads.cube_blob(16, rng_seed=12)
+    o, g = make_pair(oracle_mod, spec)
+    for _ in range(5):
+        assert o.step() == 0
+    g.step(5)
+    assert rel_err(g.rhs(), o.rhs()) <= 1e-9
+    assert rel_err(g.positions(), o.positions()) <= POS_TOL
+    assert abs(g.cg_iterations()[0] - o.cg_iterations()) <= 2
+
+
+def test_mixed_precision_medium_mesh(oracle_mod):
+    spec = workloads.cube_blob(16, rng_seed=13)
+    o, g = make_pair(oracle_mod, spec, precision="mixed")
+    for _ in range(5):
+        assert o.step() == 0
+    g.step(5)
+    e = rel_err(g.positions(), o.positions())
+    print("mixed position rel err after 5 steps:", e)
+    assert e <= 1e-5
+    assert abs(g.cg_iterations()[0] - o.cg_iterations()) <= 5
