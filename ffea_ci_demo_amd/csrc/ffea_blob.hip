@@ -1431,6 +1431,8 @@ int ffea_gpu_blob::solve_and_update() {
 }
 
 // ================================================================ C ABI
+void ffea_internal_set_error(const std::string &msg) { g_last_error = msg; }
+
 namespace {
 template <typename F>
 int guarded(ffea_gpu_blob *b, F &&f) {
@@ -1652,6 +1654,25 @@ int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *b, const uint32_t state[6]) {
         return FFEA_GPU_ERR_INVALID;
     }
     for (int i = 0; i < 6; ++i) b->rng[i] = state[i];
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]) {
+    if (!state || !out || ndraws < 0 || !mrg::h_valid_state(state)) return FFEA_GPU_ERR_INVALID;
+    uint32_t t[6];
+    mrg::h_apply(mrg::h_pow((unsigned long long)ndraws), state, t);
+    for (int i = 0; i < 6; ++i) out[i] = t[i];
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *b, const uint32_t state[6], long long draws_per_step) {
+    if (!b || !state || draws_per_step < 0) return FFEA_GPU_ERR_INVALID;
+    if (!mrg::h_valid_state(state)) {
+        b->err = "invalid MRG32k3a state";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    for (int i = 0; i < 6; ++i) b->rng[i] = state[i];
+    b->step_jump = mrg::h_pow((unsigned long long)draws_per_step);
     return FFEA_GPU_OK;
 }
 

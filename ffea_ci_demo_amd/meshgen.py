@@ -258,8 +258,6 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
          "\t<calc_ssint = 0>", "\t<calc_steric = 0>", f"\t<calc_noise = {calc_noise}>",
          "\t<calc_kinetics = 0>", "\t<calc_es = 0>", "\t<es_N_x = 0>", "\t<es_N_y = 0>", "\t<es_N_z = 0>",
          f"\t<calc_ctforces = {calc_ctforces}>"]
-    if ctforces_fname:
-        p.append(f"\t<ctforces_fname = {ctforces_fname}>")
     p += ["\t<num_blobs = 1>", "\t<num_conformations = (1)>", "\t<num_states = (1)>", "</param>", "",
           "<system>", "\t<blob>", "\t\t<conformation>", "\t\t\t<motion_state = DYNAMIC>",
           f"\t\t\t<nodes = {base}.node>", f"\t\t\t<topology = {base}.top>", f"\t\t\t<surface = {base}.surf>",
@@ -268,5 +266,19 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
           f"\t\t<scale = {scale!r}>"]
     if centroid is not None:
         p.append("\t\t<centroid = (%r,%r,%r)>" % tuple(centroid))
-    p += ["\t</blob>", "</system>"]
+    p += ["\t</blob>"]
+    if ctforces_fname:
+        p += ["\t<interactions>", "\t\t<ctforces>", f"\t\t\t<ctforces_fname = {ctforces_fname}>",
+              "\t\t</ctforces>", "\t</interactions>"]
+    p += ["</system>"]
     _w(path, p)
+
+
+def write_ctforces(path, node_forces, blob_index=0):
+    """A .ctforces file of linear node forces: rows (F [N], direction x y z, node)
+    (format of docs/2-UserManual/7-FFEAFiles.md:210-272)."""
+    lines = ["ffea ctforces file", f"num_ctforces {len(node_forces)}", f"num_linear_forces {len(node_forces)}",
+             "linear forces:"]
+    for F, d, node in node_forces:
+        lines.append(f"{F!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 {node}")
+    _w(path, lines)

@@ -115,6 +115,16 @@ int ffea_gpu_blob_get_cg_iterations(ffea_gpu_blob *blob, int *last, long long *t
 int ffea_gpu_blob_get_rng_state(ffea_gpu_blob *blob, uint32_t state[6]);
 int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *blob, const uint32_t state[6]);
 
+/* Noise stream shared with other blobs: `state` is this blob's stream state for its next
+ * element pass and every pass then moves the stream `draws_per_step` draws on (7 per
+ * element of every blob that draws from the same stream in a step; the default is 7
+ * per element of this blob). World::run draws blob after blob from rng[0] in one
+ * thread (World.cpp:2031-2038), so blob b starts 7 * (elements of blobs 0..b-1) on. */
+int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *blob, const uint32_t state[6], long long draws_per_step);
+
+/* state after `ndraws` draws from `state` (exact jump-ahead; host only) */
+int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]);
+
 /* sizes of the device-side problem */
 typedef struct {
     int num_nodes, num_elements, num_linear_nodes;

@@ -1,0 +1,68 @@
+/*
+ * ffea_script.h — host-side reader of FFEA input (no GPU needed), and creation of
+ * GPU blobs from it. Part of libffea_gpu.so.
+ *
+ * Reads what the reference's World::init reads for the blob path:
+ *   .ffea script       FFEA_input_reader.cpp:38-196 (comments, <param>, <system>,
+ *                      <blob>, <conformation>, <interactions><ctforces>);
+ *                      keys and defaults of SimulationParams.cpp:26-117, 239-776
+ *                      (dt, kT, epsilon -> epsilon^2, stokes_visc, scale in meso units)
+ *   .node .top .mat    Blob.cpp:1911-2085, 2241-2293
+ *   .stokes .pin       Blob.cpp:2297-2358, 3024-3076
+ *   .surf              Blob.cpp:2090-2170 (faces must be midside sub-triangles,
+ *                      tetra_element_linear.cpp:629-671)
+ *   .ctforces          Blob.cpp:2478-2640, linear node forces only
+ *   rng_seed           World.cpp:259-295 (glibc srand/rand -> 6 seeds; stream 0)
+ * Positions are placed as World::init leaves them: blob centroid moved to the
+ * `centroid` tag if given (World.cpp:2748-2768), then the system centroid moved to
+ * the centre of the (empty) box when move_into_box is 1 (World.cpp:442-474).
+ */
+#ifndef FFEA_SCRIPT_H
+#define FFEA_SCRIPT_H
+
+#include <stdint.h>
+
+#include "ffea_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ffea_script ffea_script;
+
+typedef struct {
+    int num_blobs;
+    long long num_steps;      /* the run loop does num_steps + 1 updates (World.cpp:1747) */
+    int check;
+    int rng_seed;
+    int restart;
+    int calc_noise, calc_stokes, calc_ctforces, calc_ssint, calc_steric, calc_es, calc_kinetics;
+    int calc_springs, calc_preComp, move_into_box;
+    int max_iterations_cg;
+    double dt, kT, epsilon2, stokes_visc;  /* meso units */
+} ffea_script_params;
+
+/* parses the script and every blob's files; 0 on success, else an ffea_gpu_status with
+ * the reference's message in ffea_gpu_last_error() */
+int ffea_script_load(const char *path, ffea_script **out);
+void ffea_script_free(ffea_script *s);
+int ffea_script_get_params(const ffea_script *s, ffea_script_params *out);
+
+/* blob b (first conformation): sizes, then arrays (any pointer may be NULL) */
+int ffea_script_blob_sizes(const ffea_script *s, int blob, int *num_nodes, int *num_elements, int *num_pinned,
+                           int *num_ctf, int *num_surface_faces);
+int ffea_script_blob_arrays(const ffea_script *s, int blob, double *pos, int *top, double *A, double *B,
+                            double *G, double *E, double *rho, int *pinned, int *ctf_node, double *ctf_force,
+                            double *stokes_radius);
+
+/* the noise stream state of the single-thread run (rng[0] after SetPackageSeed) */
+int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]);
+
+/* blob b of the script on the GPU (ffea_gpu_blob_create with the script's params) */
+int ffea_gpu_blob_create_from_script(const ffea_script *s, int blob, int precision, int device,
+                                     ffea_gpu_blob **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1016,3 +1016,11 @@ int orc_blob_time_step(orc_blob *b, int cg_iters, double *t_elem, double *t_buil
     *t_iter = (t4 - t3) / (it > 0 ? it : cg_iters);
     return inv;
 }
+
+/* element viscosity matrices of the last build, [ne][12][12] */
+void orc_blob_get_element_matrices(const orc_blob *b, double *out) {
+    memcpy(out, b->V, sizeof(double) * 144 * (size_t)b->ne);
+}
+
+/* pinned flags per node */
+void orc_blob_get_pinned_flags(const orc_blob *b, int *out) { memcpy(out, b->is_pinned, sizeof(int) * b->nt); }

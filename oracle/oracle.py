@@ -97,6 +97,8 @@ def lib():
         L.orc_blob_apply_matrix.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_blob_solve.argtypes = [C.c_void_p, _dp]
         L.orc_blob_solve.restype = C.c_int
+        L.orc_blob_get_element_matrices.argtypes = [C.c_void_p, _dp]
+        L.orc_blob_get_pinned_flags.argtypes = [C.c_void_p, _ip]
         L.orc_blob_time_step.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.orc_blob_time_step.restype = C.c_int
         L.orc_element_elastic_force.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
@@ -232,6 +234,7 @@ class Blob:
         self.h = h
         self.nt = pos.shape[0]
         self.ne = ne
+        self._top = top
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -306,6 +309,14 @@ class Blob:
         lib().orc_blob_apply_matrix(self.h, np.ascontiguousarray(x, np.float64).reshape(-1), out)
         return out.reshape(-1, 3)
 
+    def element_matrices(self):
+        return self._get("orc_blob_get_element_matrices", (self.ne, 12, 12))
+
+    def pinned_flags(self):
+        out = np.zeros(self.nt, np.int32)
+        lib().orc_blob_get_pinned_flags(self.h, out)
+        return out
+
     def time_step(self, cg_iters):
         """(t_element, t_gather_and_build, t_per_cg_iteration) in seconds for one step"""
         te, tb, ti = C.c_double(), C.c_double(), C.c_double()
@@ -316,3 +327,30 @@ class Blob:
         x = np.ascontiguousarray(b, np.float64).reshape(-1).copy()
         it = lib().orc_blob_solve(self.h, x)
         return x.reshape(-1, 3), it
+
+
+def ref_matrix(blob: "Blob", x=None):
+    """The reference's own fixed-pattern matrix (oracle/_ref) for the oracle blob's
+    current element matrices: (rowptr, col, val, inv_diag, y=V x or None)."""
+    R = ref_lib()
+    if R is None:
+        return None
+    V = np.ascontiguousarray(blob.element_matrices())
+    top = np.ascontiguousarray(blob._top, np.int32)
+    pin = np.ascontiguousarray(blob.pinned_flags(), np.int32)
+    drag = blob._drag
+    dp = None if drag is None else drag.ctypes.data
+    nnz = R.ref_nomass_matrix(blob.nt, blob.ne, top, pin, V, dp, None, None, None, None, None, None)
+    rp = np.zeros(3 * blob.nt + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float64)
+    inv = np.zeros(3 * blob.nt, np.float64)
+    y = None
+    xp = yp = None
+    if x is not None:
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.zeros(3 * blob.nt, np.float64)
+        xp, yp = x.ctypes.data, y.ctypes.data
+    R.ref_nomass_matrix(blob.nt, blob.ne, top, pin, V, dp, rp.ctypes.data, col.ctypes.data, val.ctypes.data,
+                        inv.ctypes.data, xp, yp)
+    return rp, col, val, inv, (None if y is None else y.reshape(-1, 3))

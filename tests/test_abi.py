@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every function include/*.h declares; calls
+that need no GPU behave (argument checks, no-device status, the host-only reader
+and RNG jump)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ffea_ci_demo_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z_0-9 ]*?\**\s*\b(ffea_[a-z_0-9]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 25
+    L = _native.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_invalid_arguments_rejected_without_device():
+    L = _native.lib()
+    assert L.ffea_gpu_blob_step(None, 1) == _native.ERR_INVALID
+    assert L.ffea_gpu_blob_update_positions(None) == _native.ERR_INVALID
+    L.ffea_gpu_blob_destroy(None)
+
+
+def test_rng_advance_matches_oracle(oracle_mod):
+    L = _native.lib()
+    L.ffea_gpu_advance_rng_state.argtypes = [np.ctypeslib.ndpointer(np.uint32), C.c_longlong,
+                                             np.ctypeslib.ndpointer(np.uint32)]
+    s0 = np.array([11, 22, 33, 44, 55, 66], np.uint32)
+    out = np.zeros(6, np.uint32)
+    for n in (0, 1, 7, 7 * 1296, 123456789):
+        assert L.ffea_gpu_advance_rng_state(s0, n, out) == 0
+        g = oracle_mod.Rng(state=s0)
+        if n:
+            g.advance(0, n) if n < 2 ** 31 else None
+        assert np.array_equal(out, g.state())
