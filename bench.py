@@ -35,9 +35,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_cube100_traffic.json")
 
 def spmv_alg_bytes(info, w):
     """algorithmic bytes of one CG SpMV-step launch: the viscosity matrix (9 values of
-    w bytes + a 4-byte block column per 3x3 block) and four FP64 node 3-vectors
-    (p_old and z read, p_new and q written)"""
-    return info["num_blocks"] * (9 * w + 4) + 4 * 24 * info["num_linear_nodes"]
+    w bytes + a 4-byte block column per 3x3 block) and five FP64 node 3-vectors (z
+    gathered once, p and q read and written)"""
+    return info["num_blocks"] * (9 * w + 4) + 5 * 24 * info["num_linear_nodes"]
 
 
 def iter_alg_bytes(info, w):

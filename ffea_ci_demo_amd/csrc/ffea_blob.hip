@@ -38,6 +38,14 @@ namespace {
 constexpr int kBlock = 256;   // threads per workgroup: 4 wavefronts of 64
 constexpr int kSlice = 64;    // rows per ELL slice (one wavefront)
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;
+constexpr int kMaxUnrolledRow = 16;  // rows up to this many blocks take the unrolled SpMV path
+
+// CG node vectors are tiled by 64 nodes: x[64] y[64] z[64] per tile, so row-wise
+// loads and stores of one component are 512 contiguous bytes and stencil gathers of
+// neighbouring rows stay coalesced
+__host__ __device__ __forceinline__ size_t vi(int n, int k) {
+    return ((size_t)(n >> 6) * 3 + (size_t)k) * 64 + (size_t)(n & 63);
+}
 
 thread_local std::string g_last_error;
 
@@ -447,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
         for (int k = 0; k < 9; ++k) v[(size_t)k * kSlice] = (T)acc[k];
         if (j == a.diag_j[n]) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) a.dinv[3 * n + i] = 1.0 / (double)(T)acc[4 * i];
+            for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
         }
     }
 }
@@ -503,15 +511,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
             }
         if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a.rhs[3 * n + c] = f[c];
+        for (int c = 0; c < 3; ++c) a.rhs[vi(n, c)] = f[c];
         if constexpr (!PROJ) {
             double zz[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                a.r[3 * n + c] = f[c];
-                a.x[3 * n + c] = 0.0;
-                zz[c] = a.dinv[3 * n + c] * f[c];
-                a.z[3 * n + c] = zz[c];
+                a.r[vi(n, c)] = f[c];
+                a.x[vi(n, c)] = 0.0;
+                zz[c] = a.dinv[vi(n, c)] * f[c];
+                a.z[vi(n, c)] = zz[c];
             }
             v[0] += f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
             v[1] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
@@ -562,11 +570,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(int nl, int rows_per_block,
         z_times(xx, cf, zc);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            rl[c] = rhs[3 * n + c] - zc[c];
-            r[3 * n + c] = rl[c];
-            x[3 * n + c] = 0.0;
-            zl[c] = dinv[3 * n + c] * rl[c];
-            z[3 * n + c] = zl[c];
+            rl[c] = rhs[vi(n, c)] - zc[c];
+            r[vi(n, c)] = rl[c];
+            x[vi(n, c)] = 0.0;
+            zl[c] = dinv[vi(n, c)] * rl[c];
+            z[vi(n, c)] = zl[c];
         }
         v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
     }
@@ -652,27 +660,39 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
         const int *colp = a.col + base + lane;
         const T *vp = a.val + (size_t)base * 9 + lane;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-#pragma unroll 2
-        for (int j = 0; j < W; ++j) {
-            const int c = colp[(size_t)j * kSlice];
+        auto block = [&](int j, int c) {
             const T *vv = vp + (size_t)j * 9 * kSlice;
-            const double z0 = a.z[3 * c], z1 = a.z[3 * c + 1], z2 = a.z[3 * c + 2];
+            const double z0 = a.z[vi(c, 0)], z1 = a.z[vi(c, 1)], z2 = a.z[vi(c, 2)];
             acc0 += (double)vv[0 * kSlice] * z0; acc0 += (double)vv[1 * kSlice] * z1; acc0 += (double)vv[2 * kSlice] * z2;
             acc1 += (double)vv[3 * kSlice] * z0; acc1 += (double)vv[4 * kSlice] * z1; acc1 += (double)vv[5 * kSlice] * z2;
             acc2 += (double)vv[6 * kSlice] * z0; acc2 += (double)vv[7 * kSlice] * z1; acc2 += (double)vv[8 * kSlice] * z2;
+        };
+        if (W <= kMaxUnrolledRow) {
+            // all block columns of the row first, so the gathers and value loads of every
+            // block can be in flight together
+            int cc[kMaxUnrolledRow];
+#pragma unroll
+            for (int j = 0; j < kMaxUnrolledRow; ++j)
+                if (j < W) cc[j] = colp[(size_t)j * kSlice];
+#pragma unroll
+            for (int j = 0; j < kMaxUnrolledRow; ++j)
+                if (j < W) block(j, cc[j]);
+        } else {
+#pragma unroll 2
+            for (int j = 0; j < W; ++j) block(j, colp[(size_t)j * kSlice]);
         }
         double pn[3], qn[3] = {acc0, acc1, acc2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double zk = a.z[3 * n + k];
+            const double zk = a.z[vi(n, k)];
             if (first) {
                 pn[k] = zk;
             } else {
-                pn[k] = beta * a.p[3 * n + k] + zk;
-                qn[k] = beta * a.q[3 * n + k] + qn[k];
+                pn[k] = beta * a.p[vi(n, k)] + zk;
+                qn[k] = beta * a.q[vi(n, k)] + qn[k];
             }
-            a.p[3 * n + k] = pn[k];
-            a.q[3 * n + k] = qn[k];
+            a.p[vi(n, k)] = pn[k];
+            a.q[vi(n, k)] = qn[k];
         }
         v[0] += pn[0] * qn[0] + pn[1] * qn[1] + pn[2] * qn[2];
         if constexpr (PROJ) {
@@ -727,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
     const RowRange rr = row_range(a.nl, a.rows_per_block);
     double v[2] = {0.0, 0.0};
     for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
-        double qn[3] = {a.q[3 * n], a.q[3 * n + 1], a.q[3 * n + 2]};
+        double qn[3] = {a.q[vi(n, 0)], a.q[vi(n, 1)], a.q[vi(n, 2)]};
         if constexpr (PROJ) {
             const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
                                  a.pos[3 * n + 2] - a.centre.z};
@@ -740,11 +760,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
         double r3[3], z3[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            a.x[3 * n + k] += alpha * a.p[3 * n + k];
-            r3[k] = a.r[3 * n + k] + (-alpha) * qn[k];
-            a.r[3 * n + k] = r3[k];
-            z3[k] = a.dinv[3 * n + k] * r3[k];
-            a.z[3 * n + k] = z3[k];
+            a.x[vi(n, k)] += alpha * a.p[vi(n, k)];
+            r3[k] = a.r[vi(n, k)] + (-alpha) * qn[k];
+            a.r[vi(n, k)] = r3[k];
+            z3[k] = a.dinv[vi(n, k)] * r3[k];
+            a.z[vi(n, k)] = z3[k];
         }
         v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
         v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
@@ -763,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int nl, const double *x, doub
     if (n >= nl) return;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double v = x[3 * n + k];
+        const double v = x[vi(n, k)];
         vel[3 * n + k] = v;
         pos[3 * n + k] += v * dt;
     }
@@ -800,6 +820,7 @@ struct DevBuf {
 
 struct Phase {
     std::vector<hipEvent_t> ev;  // pairs
+    std::vector<int> iter;       // CG phases: iteration of each timed pair
     double ms = 0;
     long long launches = 0;
 };
@@ -855,6 +876,7 @@ struct ffea_gpu_blob {
 
     // timing
     unsigned timing = 0;  // bit i: time phase i
+    int cg_sample = 8;    // CG phases: time one iteration in cg_sample
     Phase phase[FFEA_GPU_NUM_PHASES];
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -878,30 +900,36 @@ struct ffea_gpu_blob {
     }
     void t_begin(int ph, hipEvent_t *out) {
         *out = nullptr;
-        if (!(timing >> ph & 1u)) return;
+        if (!(timing >> ph & 1u) && !(ph >= 3 && ph <= 4)) return;
         *out = take_event();
         FFEA_HIP_TRY(hipEventRecord(*out, stream));
     }
-    void t_end(int ph, hipEvent_t b) {
-        if (!(timing >> ph & 1u)) return;
+    void t_end(int ph, hipEvent_t b, int it = -1) {
+        if (!b) return;
         hipEvent_t e = take_event();
         FFEA_HIP_TRY(hipEventRecord(e, stream));
         phase[ph].ev.push_back(b);
         phase[ph].ev.push_back(e);
-        // CG kernels queued after convergence return at once; they are counted
-        // by the iterations actually done (see solve_and_update)
-        if (ph != 3 && ph != 4) phase[ph].launches++;
+        phase[ph].iter.push_back(it);
     }
-    void collect_timing() {
+    // CG phases are timed on every cg_sample-th iteration only (events between
+    // kernels delay the next launch); pairs of iterations past the converged one
+    // (launches that return at once) are dropped when collected
+    bool t_cg(int ph, int it) const { return (timing >> ph & 1u) && it % cg_sample == 0; }
+    void collect_timing(int last_iter = -1) {
         if (!timing) return;
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
         for (auto &p : phase) {
             for (size_t i = 0; i + 1 < p.ev.size(); i += 2) {
+                const int it = p.iter[i / 2];
+                if (it >= 0 && last_iter >= 0 && it >= last_iter) continue;
                 float ms = 0;
                 FFEA_HIP_TRY(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
                 p.ms += ms;
+                p.launches++;
             }
             p.ev.clear();
+            p.iter.clear();
         }
         ev_used = 0;
     }
@@ -1230,11 +1258,17 @@ void ffea_gpu_blob::build(const double *pos_h, const int *top, const double *A, 
     cptr.alloc(cnt.size(), &dev_bytes); cptr.upload(cnt, stream);
     contrib.alloc(contrib_h.size(), &dev_bytes); contrib.upload(contrib_h, stream);
     diag_j.alloc(nl, &dev_bytes); diag_j.upload(dj, stream);
+    const size_t vlen = (size_t)nslices * 3 * kSlice;  // tiled node vectors (vi)
     for (DevBuf<double> *b : {&dinv, &rhs, &r, &z, &x, &q, &p}) {
-        b->alloc((size_t)3 * nl, &dev_bytes);
-        FFEA_HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(double) * 3 * nl, stream));
+        b->alloc(vlen, &dev_bytes);
+        FFEA_HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, stream));
     }
-    nb = std::min((nl + kBlock - 1) / kBlock, 1024);
+    {
+        // row kernels: a fixed grid of at most max_row_blocks blocks (tuning knob)
+        int max_row_blocks = 1024;
+        if (const char *e = getenv("FFEA_GPU_MAX_ROW_BLOCKS")) max_row_blocks = std::max(8, atoi(e));
+        nb = std::min((nl + kBlock - 1) / kBlock, max_row_blocks);
+    }
     if (nb >= 8) nb -= nb % 8;
     rows_per_block = ((nl + nb - 1) / nb + kSlice - 1) / kSlice * kSlice;
     nb = (nl + rows_per_block - 1) / rows_per_block;
@@ -1355,16 +1389,18 @@ void ffea_gpu_blob::launch_solve(const T *val, const T *force) {
         const int nit = std::min(chunk, imax + 1 - launched);
         for (int k = 0; k < nit; ++k) {
             const int it = launched + k;
-            t_begin(3, &t0);
+            hipEvent_t ts = nullptr;
+            if (t_cg(3, it)) t_begin(3, &ts);
             SpmvArgs<T> sa{nl, rows_per_block, it, imax, prm.epsilon2, slice_w.p, slice_base.p, col.p, val, z.p,
                            pos.p, centre, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p, S.p};
             hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
-            t_end(3, t0);
+            t_end(3, ts, it);
             if (it == imax) break;  // that SpMV step only tests the last iteration
-            t_begin(4, &t0);
+            hipEvent_t tv = nullptr;
+            if (t_cg(4, it)) t_begin(4, &tv);
             VecArgs va{nl, rows_per_block, it, p.p, q.p, dinv.p, pos.p, centre, x.p, r.p, z.p, P1.p, P2.p, S.p};
             hipLaunchKernelGGL((k_cg_vec<PROJ>), dim3(nb), dim3(kBlock), 0, stream, va);
-            t_end(4, t0);
+            t_end(4, tv, it);
         }
         FFEA_HIP_TRY(hipGetLastError());
         launched += nit;
@@ -1410,8 +1446,7 @@ int ffea_gpu_blob::solve_and_update() {
     FFEA_HIP_TRY(hipMemcpyAsync(&h_status[2], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     const CgScalars &st = h_status[2];
-    if (timing >> 3 & 1u) phase[3].launches += st.iter;
-    if (timing >> 4 & 1u) phase[4].launches += st.iter;
+    collect_timing(st.iter);
     last_iters = st.iter;
     total_iters += st.iter;
     if (st.iter > 0) expect_iters = st.iter;
@@ -1593,27 +1628,27 @@ int ffea_gpu_blob_set_positions(ffea_gpu_blob *b, const double *in) {
     });
 }
 
-static int get_node_vec(ffea_gpu_blob *b, const double *dev, double *out) {
+static int get_node_vec(ffea_gpu_blob *b, const double *dev, double *out, bool tiled) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        std::vector<double> p((size_t)3 * b->nl);
+        std::vector<double> p(tiled ? (size_t)b->nslices * 3 * kSlice : (size_t)3 * b->nl);
         FFEA_HIP_TRY(hipMemcpyAsync(p.data(), dev, sizeof(double) * p.size(), hipMemcpyDeviceToHost, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         std::fill(out, out + (size_t)3 * b->nt, 0.0);
         for (int n = 0; n < b->nl; ++n)
-            for (int c = 0; c < 3; ++c) out[3 * b->orig_of_lin[n] + c] = p[3 * n + c];
+            for (int c = 0; c < 3; ++c) out[3 * b->orig_of_lin[n] + c] = tiled ? p[vi(n, c)] : p[3 * n + c];
         return (int)FFEA_GPU_OK;
     });
 }
 
 int ffea_gpu_blob_get_velocities(ffea_gpu_blob *b, double *out) {
     if (!b || !out) return FFEA_GPU_ERR_INVALID;
-    return get_node_vec(b, b->vel.p, out);
+    return get_node_vec(b, b->vel.p, out, false);
 }
 
 int ffea_gpu_blob_get_rhs(ffea_gpu_blob *b, double *out) {
     if (!b || !out) return FFEA_GPU_ERR_INVALID;
-    return get_node_vec(b, b->rhs.p, out);
+    return get_node_vec(b, b->rhs.p, out, true);
 }
 
 int ffea_gpu_blob_get_element_forces(ffea_gpu_blob *b, double *out) {
