@@ -1,0 +1,104 @@
+// ffea_gpu.hpp — header-only C++ wrapper of the C ABI (ffea_gpu.h, ffea_script.h) for
+// host code written like the reference's: a blob object with the per-step pair
+// update_internal_forces() / update_positions() (Blob.h:130, 141) and failures thrown
+// as exceptions carrying the reference's messages (FFEA_return_codes.h:52-69).
+#ifndef FFEA_GPU_HPP
+#define FFEA_GPU_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ffea_gpu.h"
+#include "ffea_script.h"
+
+namespace ffea_gpu {
+
+class Exception : public std::runtime_error {
+public:
+    Exception(int status, const std::string &msg) : std::runtime_error(msg), status_(status) {}
+    int status() const { return status_; }
+
+private:
+    int status_;
+};
+
+class Script {
+public:
+    explicit Script(const std::string &path) {
+        if (int rc = ffea_script_load(path.c_str(), &s_)) throw Exception(rc, ffea_gpu_last_error());
+    }
+    ~Script() { ffea_script_free(s_); }
+    Script(const Script &) = delete;
+    Script &operator=(const Script &) = delete;
+    ffea_script_params params() const {
+        ffea_script_params p{};
+        ffea_script_get_params(s_, &p);
+        return p;
+    }
+    const ffea_script *get() const { return s_; }
+
+private:
+    ffea_script *s_ = nullptr;
+};
+
+// One blob on the GPU: the reference's Blob for the mass-less dynamic path.
+class Blob {
+public:
+    Blob(const Script &script, int blob_index, int precision = FFEA_GPU_FP64, int device = 0) {
+        if (int rc = ffea_gpu_blob_create_from_script(script.get(), blob_index, precision, device, &b_))
+            throw Exception(rc, ffea_gpu_last_error());
+        int nn = 0;
+        ffea_script_blob_sizes(script.get(), blob_index, &nn, nullptr, nullptr, nullptr, nullptr);
+        num_nodes_ = nn;
+    }
+    Blob(const ffea_gpu_params &p, int num_nodes, int num_elements, const double *pos, const int *top,
+         const double *A, const double *B, const double *G, const double *E, int num_pinned, const int *pinned,
+         int num_ctf, const int *ctf_node, const double *ctf_force, const uint32_t rng_state[6])
+        : num_nodes_(num_nodes) {
+        if (int rc = ffea_gpu_blob_create(&p, num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned,
+                                          num_ctf, ctf_node, ctf_force, nullptr, rng_state, &b_))
+            throw Exception(rc, ffea_gpu_last_error());
+    }
+    ~Blob() { ffea_gpu_blob_destroy(b_); }
+    Blob(const Blob &) = delete;
+    Blob &operator=(const Blob &) = delete;
+
+    void update_internal_forces() { check(ffea_gpu_blob_update_internal_forces(b_)); }
+    void update_positions() { check(ffea_gpu_blob_update_positions(b_)); }
+    void step(int n = 1) { check(ffea_gpu_blob_step(b_, n)); }
+
+    std::vector<double> positions() { return get(ffea_gpu_blob_get_positions); }
+    std::vector<double> velocities() { return get(ffea_gpu_blob_get_velocities); }
+    std::vector<double> rhs() { return get(ffea_gpu_blob_get_rhs); }
+    void set_positions(const std::vector<double> &p) { check(ffea_gpu_blob_set_positions(b_, p.data())); }
+    int cg_iterations() {
+        int last = 0;
+        ffea_gpu_blob_get_cg_iterations(b_, &last, nullptr);
+        return last;
+    }
+    std::vector<uint32_t> rng_state() {
+        std::vector<uint32_t> s(6);
+        check(ffea_gpu_blob_get_rng_state(b_, s.data()));
+        return s;
+    }
+    int num_nodes() const { return num_nodes_; }
+    ffea_gpu_blob *handle() { return b_; }
+
+private:
+    void check(int rc) {
+        if (rc) throw Exception(rc, ffea_gpu_blob_last_error(b_));
+    }
+    std::vector<double> get(int (*f)(ffea_gpu_blob *, double *)) {
+        std::vector<double> v((size_t)3 * num_nodes_);
+        check(f(b_, v.data()));
+        return v;
+    }
+    ffea_gpu_blob *b_ = nullptr;
+    int num_nodes_ = 0;
+};
+
+}  // namespace ffea_gpu
+
+#endif

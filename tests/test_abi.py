@@ -51,3 +51,20 @@ def test_rng_advance_matches_oracle(oracle_mod):
         if n:
             g.advance(0, n) if n < 2 ** 31 else None
         assert np.array_equal(out, g.state())
+
+
+def test_cpp_wrapper_and_example_build(tmp_path):
+    """include/ffea_gpu.hpp and the script-runner example compile and link against the
+    library (run on the GPU in tests/test_gpu_cpp.py)"""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    out = str(tmp_path / "run_script")
+    cmd = [gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "ffea_ci_demo_amd", "examples", "run_script.cpp"),
+           "-L", os.path.join(ROOT, "ffea_ci_demo_amd"), "-lffea_gpu",
+           "-Wl,-rpath," + os.path.join(ROOT, "ffea_ci_demo_amd"), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
