@@ -24,7 +24,12 @@ EXPORTED = [
     "ffea_gpu_blob_set_positions", "ffea_gpu_blob_get_velocities", "ffea_gpu_blob_get_rhs",
     "ffea_gpu_blob_get_element_forces", "ffea_gpu_blob_get_cg_iterations", "ffea_gpu_blob_get_rng_state",
     "ffea_gpu_blob_set_rng_state", "ffea_gpu_blob_get_info", "ffea_gpu_blob_set_timing",
-    "ffea_gpu_blob_get_timing", "ffea_gpu_copy_probe", "ffea_gpu_blob_synchronize", "ffea_gpu_blob_last_error", "ffea_gpu_last_error",
+    "ffea_gpu_blob_get_timing", "ffea_gpu_copy_probe", "ffea_gpu_blob_synchronize", "ffea_gpu_blob_last_error",
+    "ffea_gpu_last_error", "ffea_gpu_batch_create", "ffea_gpu_batch_get_positions", "ffea_gpu_batch_set_positions",
+    "ffea_gpu_batch_get_velocities", "ffea_gpu_batch_get_rhs", "ffea_gpu_batch_get_element_forces",
+    "ffea_gpu_batch_get_cg_iterations", "ffea_gpu_batch_get_rng_state", "ffea_gpu_batch_set_rng_stream",
+    "ffea_gpu_rng_stream_state", "ffea_gpu_advance_rng_state", "ffea_gpu_blob_set_row_blocks",
+    "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout",
 ]
 
 
@@ -37,7 +42,8 @@ class Params(C.Structure):
 class Info(C.Structure):
     _fields_ = [("num_nodes", C.c_int), ("num_elements", C.c_int), ("num_linear_nodes", C.c_int),
                 ("num_blocks", C.c_longlong), ("num_block_slots", C.c_longlong), ("precision", C.c_int),
-                ("device_bytes", C.c_longlong)]
+                ("device_bytes", C.c_longlong), ("num_blobs", C.c_int), ("num_chunks", C.c_int),
+                ("row_blocks", C.c_int), ("graph_iterations", C.c_int)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -72,6 +78,23 @@ def lib():
     L.ffea_gpu_blob_set_timing.argtypes = [vp, C.c_uint]
     L.ffea_gpu_copy_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.ffea_gpu_blob_get_timing.argtypes = [vp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
+    pp = C.POINTER(C.c_void_p)
+    L.ffea_gpu_batch_create.argtypes = [C.POINTER(Params), C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp,
+                                        vp, _up, C.POINTER(vp)]
+    L.ffea_gpu_check_layout.argtypes = [C.c_int, C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp, vp, _up,
+                                        C.POINTER(Info)]
+    for f in ("get_positions", "set_positions", "get_velocities", "get_rhs", "get_element_forces"):
+        getattr(L, "ffea_gpu_batch_" + f).argtypes = [vp, C.c_int, _dp]
+    L.ffea_gpu_batch_get_cg_iterations.argtypes = [vp, _ip, np.ctypeslib.ndpointer(dtype=np.int64,
+                                                                                     flags="C_CONTIGUOUS")]
+    L.ffea_gpu_batch_get_rng_state.argtypes = [vp, C.c_int, _up]
+    L.ffea_gpu_batch_set_rng_stream.argtypes = [vp, C.c_int, _up, C.c_longlong]
+    L.ffea_gpu_blob_set_rng_stream.argtypes = [vp, _up, C.c_longlong]
+    L.ffea_gpu_rng_stream_state.argtypes = [_up, C.c_longlong, _up]
+    L.ffea_gpu_advance_rng_state.argtypes = [_up, C.c_longlong, _up]
+    L.ffea_gpu_blob_set_row_blocks.argtypes = [vp, C.c_int]
+    L.ffea_gpu_blob_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.ffea_gpu_blob_use_graph.argtypes = [vp, C.c_int, C.c_int]
     L.ffea_gpu_blob_last_error.argtypes = [vp]
     L.ffea_gpu_blob_last_error.restype = C.c_char_p
     L.ffea_gpu_last_error.argtypes = []
@@ -87,3 +110,21 @@ def copy_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:
     if rc != OK:
         raise RuntimeError(lib().ffea_gpu_last_error().decode())
     return ms.value
+
+
+def rng_stream_state(package_seed, stream: int) -> np.ndarray:
+    """initial state of the stream-th RngStream after SetPackageSeed(package_seed)"""
+    out = np.zeros(6, np.uint32)
+    rc = lib().ffea_gpu_rng_stream_state(np.ascontiguousarray(package_seed, np.uint32), int(stream), out)
+    if rc != OK:
+        raise ValueError("invalid MRG32k3a package seed or stream index")
+    return out
+
+
+def advance_rng_state(state, ndraws: int) -> np.ndarray:
+    """state after ndraws draws (exact jump-ahead)"""
+    out = np.zeros(6, np.uint32)
+    rc = lib().ffea_gpu_advance_rng_state(np.ascontiguousarray(state, np.uint32), int(ndraws), out)
+    if rc != OK:
+        raise ValueError("invalid MRG32k3a state or draw count")
+    return out

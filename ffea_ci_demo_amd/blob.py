@@ -32,7 +32,81 @@ def _opt(a, dtype):
     return arr.ctypes.data, arr
 
 
-class GpuBlob:
+class _Handle:
+    """what every handle (one blob or an ensemble) offers"""
+
+    h = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            _native.lib().ffea_gpu_blob_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != _native.OK:
+            msg = _native.lib().ffea_gpu_blob_last_error(self.h).decode()
+            raise FFEAException(rc, msg)
+
+    def step(self, nsteps=1):
+        """nsteps x (update_internal_forces; update_positions) for every blob of the handle"""
+        self._check(_native.lib().ffea_gpu_blob_step(self.h, int(nsteps)))
+
+    def info(self):
+        i = _native.Info()
+        self._check(_native.lib().ffea_gpu_blob_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _native.Info._fields_}
+
+    def set_row_blocks(self, blocks):
+        self._check(_native.lib().ffea_gpu_blob_set_row_blocks(self.h, int(blocks)))
+
+    def autotune(self, iterations=20):
+        """time CG iterations at each candidate launch shape, keep the fastest (result-neutral)"""
+        chosen = C.c_int()
+        self._check(_native.lib().ffea_gpu_blob_autotune(self.h, int(iterations), C.byref(chosen)))
+        return chosen.value
+
+    def use_graph(self, enable=True, iterations=0):
+        """run each step() as one captured HIP graph launch holding `iterations` CG
+        iterations (0: a little over the last step's count)"""
+        self._check(_native.lib().ffea_gpu_blob_use_graph(self.h, 1 if enable else 0, int(iterations)))
+
+    def set_timing(self, phases=None):
+        """time the named phases (default: all); set_timing(()) turns timing off"""
+        names = _native.PHASES if phases is None else phases
+        mask = 0
+        for nm in names:
+            mask |= 1 << _native.PHASES.index(nm)
+        self._check(_native.lib().ffea_gpu_blob_set_timing(self.h, mask))
+
+    def timing(self):
+        ms = np.zeros(_native.NUM_PHASES)
+        n = np.zeros(_native.NUM_PHASES, np.int64)
+        self._check(_native.lib().ffea_gpu_blob_get_timing(self.h, ms, n))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_native.PHASES)}
+
+    def synchronize(self):
+        self._check(_native.lib().ffea_gpu_blob_synchronize(self.h))
+
+
+def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precision, device):
+    prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
+    return _native.Params(float(dt), float(kT), float(epsilon2), int(max_iterations_cg), int(calc_noise),
+                          int(calc_stokes), prec, int(device))
+
+
+class GpuBlob(_Handle):
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=1000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
                  precision="fp64", device=0):
@@ -45,9 +119,7 @@ class GpuBlob:
             return np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (ne,)))
 
         mats = [per_elem(A), per_elem(B), per_elem(G), per_elem(E)]
-        prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
-        p = _native.Params(float(dt), float(kT), float(epsilon2), int(max_iterations_cg), int(calc_noise),
-                           int(calc_stokes), prec, int(device))
+        p = _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precision, device)
         pin_p, pins = _opt(pinned, np.int32)
         cn_p, cn = _opt(ctf_node, np.int32)
         cf_p, cf = _opt(ctf_force, np.float64)
@@ -73,37 +145,12 @@ class GpuBlob:
                    ctf_force=spec.ctf_force, stokes_drag=spec.stokes_drag, rng_state=spec.rng_state,
                    precision=precision, **kw)
 
-    def close(self):
-        if getattr(self, "h", None):
-            _native.lib().ffea_gpu_blob_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *a):
-        self.close()
-
-    def _check(self, rc):
-        if rc != _native.OK:
-            msg = _native.lib().ffea_gpu_blob_last_error(self.h).decode()
-            raise FFEAException(rc, msg)
-
     # -- the reference's per-step interface
     def update_internal_forces(self):
         self._check(_native.lib().ffea_gpu_blob_update_internal_forces(self.h))
 
     def update_positions(self):
         self._check(_native.lib().ffea_gpu_blob_update_positions(self.h))
-
-    def step(self, nsteps=1):
-        self._check(_native.lib().ffea_gpu_blob_step(self.h, int(nsteps)))
 
     # -- state
     def _get(self, fn, shape):
@@ -140,24 +187,122 @@ class GpuBlob:
     def set_rng_state(self, s):
         self._check(_native.lib().ffea_gpu_blob_set_rng_state(self.h, np.ascontiguousarray(s, np.uint32)))
 
-    def info(self):
-        i = _native.Info()
-        self._check(_native.lib().ffea_gpu_blob_get_info(self.h, C.byref(i)))
-        return {f: getattr(i, f) for f, _ in _native.Info._fields_}
+    def set_rng_stream(self, s, draws_per_step):
+        self._check(_native.lib().ffea_gpu_blob_set_rng_stream(self.h, np.ascontiguousarray(s, np.uint32),
+                                                               int(draws_per_step)))
 
-    def set_timing(self, phases=None):
-        """time the named phases (default: all); set_timing(()) turns timing off"""
-        names = _native.PHASES if phases is None else phases
-        mask = 0
-        for nm in names:
-            mask |= 1 << _native.PHASES.index(nm)
-        self._check(_native.lib().ffea_gpu_blob_set_timing(self.h, mask))
 
-    def timing(self):
-        ms = np.zeros(_native.NUM_PHASES)
-        n = np.zeros(_native.NUM_PHASES, np.int64)
-        self._check(_native.lib().ffea_gpu_blob_get_timing(self.h, ms, n))
-        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_native.PHASES)}
+class GpuBatch(_Handle):
+    """An ensemble of independent blobs stepped together in one handle.
 
-    def synchronize(self):
-        self._check(_native.lib().ffea_gpu_blob_synchronize(self.h))
+    `specs` are BlobSpec-like objects (pos, top, A, B, G, E, pinned, ctf_node, ctf_force,
+    rng_state); run parameters (dt, kT, epsilon2, max_iterations_cg, calc_noise) are
+    shared and taken from the first spec. The reference steps the blobs of a World one
+    after another (World.cpp:2031-2059); every kernel here covers all of them.
+    """
+
+    def __init__(self, specs, precision="fp64", device=0):
+        L = _native.lib()
+        specs = list(specs)
+        s0 = specs[0]
+        for s in specs[1:]:
+            if (s.dt, s.kT, s.epsilon2, s.max_iterations_cg, s.calc_noise) != \
+                    (s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise):
+                raise ValueError("blobs of one batch share dt, kT, epsilon2, max_iterations_cg and calc_noise")
+        p = _params(s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise, 0, precision, device)
+        args, keep = _batch_arrays(specs)
+        h = C.c_void_p()
+        rc = L.ffea_gpu_batch_create(C.byref(p), *args, C.byref(h))
+        del keep
+        if rc != _native.OK:
+            raise FFEAException(rc, L.ffea_gpu_last_error().decode())
+        self.h = h
+        self.num_blobs = len(specs)
+        self.nt = [s.pos.reshape(-1, 3).shape[0] for s in specs]
+        self.ne = [s.top.reshape(-1, 10).shape[0] for s in specs]
+        self.precision = precision
+
+    def _get(self, fn, b, shape):
+        out = np.zeros(shape, np.float64)
+        self._check(getattr(_native.lib(), fn)(self.h, int(b), out.reshape(-1)))
+        return out
+
+    def positions(self, b):
+        return self._get("ffea_gpu_batch_get_positions", b, (self.nt[b], 3))
+
+    def set_positions(self, b, p):
+        self._check(_native.lib().ffea_gpu_batch_set_positions(self.h, int(b),
+                                                               np.ascontiguousarray(p, np.float64).reshape(-1)))
+
+    def velocities(self, b):
+        return self._get("ffea_gpu_batch_get_velocities", b, (self.nt[b], 3))
+
+    def rhs(self, b):
+        return self._get("ffea_gpu_batch_get_rhs", b, (self.nt[b], 3))
+
+    def element_forces(self, b):
+        return self._get("ffea_gpu_batch_get_element_forces", b, (self.ne[b], 4, 3))
+
+    def cg_iterations(self):
+        last = np.zeros(self.num_blobs, np.int32)
+        total = np.zeros(self.num_blobs, np.int64)
+        self._check(_native.lib().ffea_gpu_batch_get_cg_iterations(self.h, last, total))
+        return last, total
+
+    def rng_state(self, b):
+        out = np.zeros(6, np.uint32)
+        self._check(_native.lib().ffea_gpu_batch_get_rng_state(self.h, int(b), out))
+        return out
+
+    def set_rng_stream(self, b, s, draws_per_step=-1):
+        self._check(_native.lib().ffea_gpu_batch_set_rng_stream(self.h, int(b), np.ascontiguousarray(s, np.uint32),
+                                                                int(draws_per_step)))
+
+
+def _batch_arrays(specs):
+    """per-blob arrays of ffea_gpu_batch_create (after the params), and the numpy/ctypes
+    objects that must stay alive during the call"""
+    nb = len(specs)
+    keep = []
+
+    def arr(a, dtype, shape=None):
+        x = np.ascontiguousarray(a, dtype)
+        if shape is not None:
+            x = x.reshape(shape)
+        keep.append(x)
+        return x
+
+    pos = [arr(s.pos, np.float64, (-1, 3)) for s in specs]
+    top = [arr(s.top, np.int32, (-1, 10)) for s in specs]
+    ne = [t.shape[0] for t in top]
+    mats = {k: [arr(np.broadcast_to(np.asarray(getattr(s, k), np.float64), (n,)), np.float64)
+                for s, n in zip(specs, ne)] for k in "ABGE"}
+    pins = [arr(s.pinned if s.pinned is not None else [], np.int32) for s in specs]
+    cn = [arr(s.ctf_node if s.ctf_node is not None else [], np.int32) for s in specs]
+    cf = [arr(s.ctf_force if s.ctf_force is not None else np.zeros((0, 3)), np.float64, (-1, 3)) for s in specs]
+    rng = arr(np.stack([np.asarray(s.rng_state, np.uint32) for s in specs]).reshape(-1), np.uint32)
+
+    def ptrs(lst):
+        a = (C.c_void_p * nb)(*[x.ctypes.data if x.size else None for x in lst])
+        keep.append(a)
+        return C.cast(a, C.POINTER(C.c_void_p))
+
+    npin = arr([x.size for x in pins], np.int32)
+    nctf = arr([x.size for x in cn], np.int32)
+    args = (nb, arr([x.shape[0] for x in pos], np.int32), arr(ne, np.int32), ptrs(pos), ptrs(top), ptrs(mats["A"]),
+            ptrs(mats["B"]), ptrs(mats["G"]), ptrs(mats["E"]), npin.ctypes.data, C.cast(ptrs(pins), C.c_void_p),
+            nctf.ctypes.data, C.cast(ptrs(cn), C.c_void_p), C.cast(ptrs(cf), C.c_void_p), rng)
+    return args, keep
+
+
+def check_layout(specs, precision="fp64"):
+    """Build the device tables of a batch on the host (no device) and check every kernel
+    index against its buffer; returns the sizes (ffea_gpu_check_layout)."""
+    L = _native.lib()
+    args, keep = _batch_arrays(list(specs))
+    info = _native.Info()
+    rc = L.ffea_gpu_check_layout({"fp64": _native.FP64, "mixed": _native.MIXED}[precision], *args, C.byref(info))
+    del keep
+    if rc != _native.OK:
+        raise FFEAException(rc, L.ffea_gpu_last_error().decode())
+    return {f: getattr(info, f) for f, _ in _native.Info._fields_}

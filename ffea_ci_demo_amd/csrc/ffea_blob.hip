@@ -1,57 +1,52 @@
-// MI355X (gfx950) blob time step: element forces, node gather, viscosity matrix
-// assembly, Jacobi-PCG, Euler update — behind the C ABI of include/ffea_gpu.h.
+// MI355X (gfx950) blob time step behind the C ABI of include/ffea_gpu.h: host side.
+// The kernels are in kernels.h.
 //
-// Data layout in HBM (one blob):
-//   corner ("linear") nodes are renumbered 0..Nl-1 in ascending .node order;
-//   positions, velocities and all CG vectors are FP64 [Nl][3];
-//   element data are structure-of-arrays over elements (conn [4][Ne], J^-1_0 [9][Ne],
-//   vol_0, G, E, A, B, last_det) so one-element-per-lane loads coalesce;
-//   element nodal forces are [Ne][12] (4 nodes x 3), the shape derivatives kept for
-//   the matrix are [10][Ne] (the 9 independent dpsi entries and the volume);
-//   the viscosity matrix is 3x3 blocks over corner nodes in a sliced-ELL layout:
-//   64 rows per slice, per slice width W = its longest row, values
-//   [slot_base*9 + (j*9 + k)*64 + lane] and block columns [slot_base + j*64 + lane],
-//   so lane-per-row loads of one (j, k) are 512 contiguous bytes.
+// A handle holds one blob (ffea_gpu_blob_create) or many independent blobs
+// (ffea_gpu_batch_create) stepped together: each kernel launch covers every blob, and
+// each blob keeps its own CG scalars, noise stream and convergence state.
 //
-// The node-force scatter is a gather over a node->element CSR in element order
-// (no atomics): results are bit-reproducible run to run.
+// Data layout in HBM:
+//   the corner ("linear") nodes of a blob are renumbered 0..Nl-1 in ascending .node
+//   order; blobs are concatenated, each starting on a 64-row boundary;
+//   positions and velocities are FP64 [rows][3]; CG vectors are FP64, tiled by 64 rows;
+//   element data are structure-of-arrays over the elements of all blobs;
+//   element nodal forces are [Ne][12], shape data [10][Ne];
+//   the viscosity matrix is 3x3 blocks in a sliced-ELL layout: 64 rows per slice, slice
+//   width W = its longest row, values [slot_base*9 + (j*9 + k)*64 + lane] and block
+//   columns [slot_base + j*64 + lane], so lane-per-row loads of one (j, k) are 512
+//   contiguous bytes.
+// The node-force scatter is a gather over a node->element CSR in element order and
+// every reduction is summed in a fixed order: results are bit-reproducible and do not
+// depend on the launch grid.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <array>
-#include <type_traits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ffea_gpu.h"
+#include "kernels.h"
 #include "mrg32k3a.h"
+
+using namespace ffk;
 
 namespace {
 
-constexpr int kBlock = 256;   // threads per workgroup: 4 wavefronts of 64
-constexpr int kSlice = 64;    // rows per ELL slice (one wavefront)
-constexpr uint32_t kIdentity = 0xFFFFFFFFu;
-constexpr int kMaxUnrolledRow = 16;  // rows up to this many blocks take the unrolled SpMV path
-
-// CG node vectors are tiled by 64 nodes: x[64] y[64] z[64] per tile, so row-wise
-// loads and stores of one component are 512 contiguous bytes and stencil gathers of
-// neighbouring rows stay coalesced
-__host__ __device__ __forceinline__ size_t vi(int n, int k) {
-    return ((size_t)(n >> 6) * 3 + (size_t)k) * 64 + (size_t)(n & 63);
-}
-
 thread_local std::string g_last_error;
 
-#define FFEA_HIP_TRY(expr)                                                                 \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
+#define FFEA_HIP_TRY(expr)                                                                        \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
         if (_e != hipSuccess) throw HipError(std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
@@ -73,730 +68,6 @@ std::string fmt(const char *f, ...) {
     return buf;
 }
 
-// scalars shared by the CG kernels (device memory). Every reduction is finished by
-// the kernel that consumes it: each block of that kernel adds the producer's
-// per-block partials in the same fixed order, so all blocks hold bit-identical
-// totals and no atomics or cross-block fences are needed.
-struct CgScalars {
-    double delta[2];   // delta[(i-1)&1]: r.z entering iteration i (written by the SpMV step)
-    double f2;         // |f|^2 of the right-hand side
-    double G[36];      // (Z^T Z)^-1 for the rigid-mode projection of the mixed form
-    int iter;          // iterations done when the solve stopped
-    int done;          // 1: stop (converged, failed or skipped)
-    int converged;     // 1: converged this solve
-    int inverted;      // inverted elements counted by the element pass (relaxed atomics)
-    int inverted_step; // inverted elements of the step being solved
-    int pad;
-};
-
-// ---------------------------------------------------------------- reductions
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// block sum of NV values; result valid in every thread
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[kBlock / 64][NV];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double s = wave_sum(v[k]);
-        if (lane == 0) sh[wid][k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) s += sh[w][k];
-        v[k] = s;
-    }
-    __syncthreads();
-}
-
-// per-block partials, stored [k][nb]
-template <int NV>
-__device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int nb, int L) {
-    block_sum<NV>(v);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) P[(size_t)k * nb + L] = v[k];
-}
-
-// totals of partials written by the previous kernel: identical in every block
-template <int NV>
-__device__ __forceinline__ void sum_partials(const double *P, int nb, double (&out)[NV]) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) acc += P[(size_t)k * nb + b];
-        out[k] = acc;
-    }
-    block_sum<NV>(out);
-}
-
-// Row range of a block in the row kernels: a fixed grid of nb blocks, each owning a
-// contiguous range of rows; blocks are mapped so that each of the 8 XCDs (which
-// receive workgroups round-robin) owns one contiguous eighth of the rows.
-struct RowRange {
-    int L, begin, end;
-};
-__device__ __forceinline__ RowRange row_range(int nrows, int rows_per_block) {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-    const int begin = L * rows_per_block;
-    const int end = min(nrows, begin + rows_per_block);
-    return {L, begin, end};
-}
-
-// rigid-mode basis of a corner node at x (relative to a fixed centre): columns are
-// the 3 translations and the 3 infinitesimal rotations e_k x x. Z^T v = (v, x cross v).
-__device__ __forceinline__ void zt_times(const double *x, const double *v, double *out6) {
-    out6[0] = v[0];
-    out6[1] = v[1];
-    out6[2] = v[2];
-    out6[3] = x[1] * v[2] - x[2] * v[1];
-    out6[4] = x[2] * v[0] - x[0] * v[2];
-    out6[5] = x[0] * v[1] - x[1] * v[0];
-}
-// Z c = c_t + c_r cross x
-__device__ __forceinline__ void z_times(const double *x, const double *c, double *out3) {
-    out3[0] = c[0] + (c[4] * x[2] - c[5] * x[1]);
-    out3[1] = c[1] + (c[5] * x[0] - c[3] * x[2]);
-    out3[2] = c[2] + (c[3] * x[1] - c[4] * x[0]);
-}
-
-// 6x6 inverse by Gauss-Jordan with partial pivoting (FP64)
-__device__ void invert6(const double *M, double *Minv) {
-    double a[6][12];
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 12; ++j) a[i][j] = j < 6 ? M[6 * i + j] : (j - 6 == i ? 1.0 : 0.0);
-    for (int c = 0; c < 6; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < 6; ++r)
-            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
-        if (piv != c)
-            for (int j = 0; j < 12; ++j) {
-                const double t = a[c][j];
-                a[c][j] = a[piv][j];
-                a[piv][j] = t;
-            }
-        const double d = 1.0 / a[c][c];
-        for (int j = 0; j < 12; ++j) a[c][j] *= d;
-        for (int r = 0; r < 6; ++r)
-            if (r != c) {
-                const double f = a[r][c];
-                for (int j = 0; j < 12; ++j) a[r][j] -= f * a[c][j];
-            }
-    }
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) Minv[6 * i + j] = a[i][j + 6];
-}
-
-// Z^T Z from the moments N, sum x, sum x_i x_j (m[0], m[1..3], m[4..9]: xx yy zz xy xz yz)
-__device__ void ztz_from_moments(const double *m, double *M) {
-    for (int i = 0; i < 36; ++i) M[i] = 0.0;
-    const double N = m[0], sx = m[1], sy = m[2], sz = m[3];
-    const double xx = m[4], yy = m[5], zz = m[6], xy = m[7], xz = m[8], yz = m[9];
-    M[0] = M[7] = M[14] = N;
-    // translation-rotation block: sum_n (e_k x x_n)_i
-    const double Tr[3][3] = {{0, sz, -sy}, {-sz, 0, sx}, {sy, -sx, 0}};  // [i][k]
-    for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < 3; ++k) {
-            M[6 * i + 3 + k] = Tr[i][k];
-            M[6 * (3 + k) + i] = Tr[i][k];
-        }
-    const double r2 = xx + yy + zz;
-    const double R[3][3] = {{r2 - xx, -xy, -xz}, {-xy, r2 - yy, -yz}, {-xz, -yz, r2 - zz}};
-    for (int k = 0; k < 3; ++k)
-        for (int l = 0; l < 3; ++l) M[6 * (3 + k) + 3 + l] = R[k][l];
-}
-
-// ---------------------------------------------------------------- element pass
-template <typename T>
-struct ElemArgs {
-    int ne;
-    const int *conn;          // [4][ne] corner-node indices
-    const double *pos;        // [nl][3]
-    const T *jinv0;           // [9][ne]
-    const T *vol0, *G, *E, *A, *B;
-    double *last_det;         // [ne]
-    T *force;                 // [ne][12]
-    T *geo;                   // [10][ne]: dpsi 1,2,3,5,6,7,9,10,11 then vol
-    int calc_noise;
-    double kT, dt;
-    const mrg::Jump *jump_group;  // [ceil(ne/64)]: A^(7*64*g)
-    const mrg::Jump *jump_lane;   // [64]: A^(7*l)
-    uint32_t rng[6];              // stream state at the start of this pass
-    CgScalars *S;
-};
-
-// tetra_element_linear.cpp:191-443 and :468-483 for one element (mass-less solver:
-// du starts at zero). Arithmetic in the reference's order, no contraction.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
-#pragma clang fp contract(off)
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= a.ne) return;
-    const int ne = a.ne;
-    uint32_t st[6];
-    if (a.calc_noise) {
-        // this element's 7 draws start 7*e draws into the pass's segment of the stream
-        for (int i = 0; i < 6; ++i) st[i] = a.rng[i];
-        mrg::d_apply(a.jump_group[e >> 6], st);
-        mrg::d_apply(a.jump_lane[e & 63], st);
-    }
-    int n[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
-    double x0[3], J64[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) J64[r][c] = a.pos[3 * n[r + 1] + c] - x0[c];
-    T J[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) J[r][c] = (T)J64[r][c];
-
-    // shape function derivatives and volume (tetra_element_linear.cpp:222-292)
-    T d[12];
-    d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
-    d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
-    d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
-    d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
-    d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
-    d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
-    d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
-    d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
-    d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
-    T det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
-    const double ld = a.last_det[e];
-    T vol;
-    if (ld * (double)det < 0) {
-        __hip_atomic_fetch_add(&a.S->inverted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vol = a.geo[9 * ne + e];  // the reference keeps the previous volume
-    } else {
-        a.last_det[e] = (double)det;
-        vol = (T)(1.0 / 6.0) * (T)fabs(det);
-        const T rdet = (T)1.0 / det;
-        d[1] *= rdet; d[2] *= rdet; d[3] *= rdet;
-        d[5] *= rdet; d[6] *= rdet; d[7] *= rdet;
-        d[9] *= rdet; d[10] *= rdet; d[11] *= rdet;
-    }
-    d[0] = -(d[1] + d[2] + d[3]);
-    d[4] = -(d[5] + d[6] + d[7]);
-    d[8] = -(d[9] + d[10] + d[11]);
-
-    // shear elastic stress (tetra_element_linear.cpp:375-392)
-    T Ji[3][3];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Ji[k / 3][k % 3] = a.jinv0[k * ne + e];
-    T F[3][3], s[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            T acc = (T)0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) acc += J[k][i] * Ji[j][k];
-            F[i][j] = acc;
-        }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            T acc = (T)0;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) acc += F[i][k] * F[j][k];
-            s[i][j] = acc;
-        }
-    const T G = a.G[e], E = a.E[e], v0 = a.vol0[e];
-    const T sc = G * v0 / vol;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) s[i][j] *= sc;
-    s[0][0] -= G;
-    s[1][1] -= G;
-    s[2][2] -= G;
-    // bulk elastic stress (:397-403)
-    {
-        const T c2 = E - G * (T)2.0 / (T)3.0;
-        const T c = G * ((T)1.0 - (v0 / vol)) + (T)0.5 * c2 * ((vol / v0) - (v0 / vol));
-        s[0][0] += c;
-        s[1][1] += c;
-        s[2][2] += c;
-    }
-    // fluctuating stress (:412-431), draws bf, s00, s11, s22, s01, s02, s12
-    if (a.calc_noise) {
-        const T A = a.A[e], B = a.B[e];
-        double u[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) u[k] = mrg::d_u01(st);
-        T r[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) r[k] = (T)((-.5) + ((.5) - (-.5)) * u[k]);
-        const T c = sqrt(((T)24 * (T)a.kT) / (vol * (T)a.dt));
-        const T bf = sqrt(B) * r[0];
-        s[0][0] += c * (sqrt((T)2 * A) * r[1] + bf);
-        s[1][1] += c * (sqrt((T)2 * A) * r[2] + bf);
-        s[2][2] += c * (sqrt((T)2 * A) * r[3] + bf);
-        s[0][1] += c * sqrt(A) * r[4];
-        s[0][2] += c * sqrt(A) * r[5];
-        s[1][2] += c * sqrt(A) * r[6];
-        s[1][0] = s[0][1];
-        s[2][0] = s[0][2];
-        s[2][1] = s[1][2];
-    }
-    // apply_stress_tensor (:436-443) then node_force -= du (:468-483)
-    T out[12];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const T du = (T)0 + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
-            out[3 * q + i] = (T)0 - du;
-        }
-    T *fo = a.force + (size_t)12 * e;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) fo[k] = out[k];
-    const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
-#pragma unroll
-    for (int k = 0; k < 9; ++k) a.geo[(size_t)k * ne + e] = d[gi[k]];
-    a.geo[(size_t)9 * ne + e] = vol;
-}
-
-// ---------------------------------------------------------------- matrix assembly
-template <typename T>
-struct AsmArgs {
-    int nl, ne;
-    const int *slice_w;          // [nslices]
-    const long long *slice_base; // [nslices] first slot of the slice
-    const uint32_t *cptr;        // [nslots+1]
-    const uint32_t *contrib;     // (e<<4)|(a<<2)|b, element order; kIdentity = identity block
-    const unsigned char *diag_j; // [nl] slot index j of the diagonal block
-    const T *geo;                // [10][ne]
-    const T *A, *B;
-    T *val;                      // [nslots*9]
-    double *dinv;                // [nl][3]
-};
-
-// Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
-// reference's create_viscosity_matrix does (tetra_element_linear.cpp:298-327, macros
-// tetra_element_linear.h:56-117, del2 :585-622): upper triangle computed, the lower
-// one mirrored, everything times vol at the end.
-template <typename T>
-__device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
-#pragma clang fp contract(off)
-    int I = 4 * i, Jc = 4 * j, p = a, q = b;
-    if (I > Jc || (I == Jc && p > q)) {  // mirror to the upper triangle
-        int t = I; I = Jc; Jc = t;
-        t = p; p = q; q = t;
-    }
-    T v;
-    if (I == Jc) {
-        const T del2 = (d[p] * d[q] + d[4 + p] * d[4 + q] + d[8 + p] * d[8 + q]);
-        v = d[I + q] * d[I + p] * (A + B);
-        v += del2 * A;
-    } else {
-        if (p == q) v = (B + A) * (d[I + p] * d[Jc + p]);
-        else v = B * (d[I + p] * d[Jc + q]) + A * (d[I + q] * d[Jc + p]);
-    }
-    return v * vol;
-}
-
-// SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
-// 305-312): every block entry is the sum of its element sources in element order.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
-#pragma clang fp contract(off)
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    if (n >= a.nl) return;
-    const int sl = n / kSlice, lane = n % kSlice;
-    const int W = a.slice_w[sl];
-    const long long base = a.slice_base[sl];
-    const int ne = a.ne;
-    for (int j = 0; j < W; ++j) {
-        const long long slot = base + (long long)j * kSlice + lane;
-        const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
-        double acc[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) {
-            const uint32_t code = a.contrib[c];
-            if (code == kIdentity) {
-                acc[0] += 1.0; acc[4] += 1.0; acc[8] += 1.0;
-                continue;
-            }
-            const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
-            T d[12];
-            d[1] = a.geo[(size_t)0 * ne + e]; d[2] = a.geo[(size_t)1 * ne + e]; d[3] = a.geo[(size_t)2 * ne + e];
-            d[5] = a.geo[(size_t)3 * ne + e]; d[6] = a.geo[(size_t)4 * ne + e]; d[7] = a.geo[(size_t)5 * ne + e];
-            d[9] = a.geo[(size_t)6 * ne + e]; d[10] = a.geo[(size_t)7 * ne + e]; d[11] = a.geo[(size_t)8 * ne + e];
-            const T vol = a.geo[(size_t)9 * ne + e];
-            d[0] = -(d[1] + d[2] + d[3]);
-            d[4] = -(d[5] + d[6] + d[7]);
-            d[8] = -(d[9] + d[10] + d[11]);
-            const T A = a.A[e], B = a.B[e];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj) acc[3 * i + jj] += (double)visc_entry<T>(d, A, B, vol, i, pa, jj, pb);
-        }
-        T *v = a.val + (size_t)base * 9 + (size_t)j * 9 * kSlice + lane;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) v[(size_t)k * kSlice] = (T)acc[k];
-        if (j == a.diag_j[n]) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
-        }
-    }
-}
-
-// ---------------------------------------------------------------- gather + CG start
-template <typename T>
-struct GatherArgs {
-    int nl, rows_per_block;
-    const uint32_t *inc_ptr;    // [nl+1]
-    const uint32_t *inc;        // e*4 + slot, element order
-    const T *force;             // [ne][12]
-    const double *ctf;          // [nl][3] or null: constant forces summed in list order
-    const uint32_t *fold_ptr;   // [nl+1] or null
-    const double *fold;         // [..][3]: linearise_force increments in element order
-    const unsigned char *pinned;
-    const double *dinv;
-    const double *pos;          // [nl][3] (rigid modes)
-    double3 centre;
-    double *rhs, *r, *z, *x;
-    double *P;                  // partials
-    CgScalars *S;
-};
-
-// Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
-// start of NoMassCGSolver::solve (NoMassCGSolver.cpp:196-220). With PROJ (free blob,
-// mixed form) the CG start is left to k_cg_start and the partials are |f|^2, Z^T f
-// and the moments of Z^T Z.
-template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
-#pragma clang fp contract(off)
-    const RowRange rr = row_range(a.nl, a.rows_per_block);
-    constexpr int NV = PROJ ? 17 : 2;
-    double v[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
-        double f[3] = {0.0, 0.0, 0.0};
-        if (a.ctf) {
-            f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
-        }
-        for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
-            const uint32_t es = a.inc[k];
-            const T *src = a.force + (size_t)12 * (es >> 2) + 3 * (es & 3);
-            f[0] += (double)src[0];
-            f[1] += (double)src[1];
-            f[2] += (double)src[2];
-        }
-        if (a.fold_ptr)
-            for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
-                f[0] += a.fold[3 * k];
-                f[1] += a.fold[3 * k + 1];
-                f[2] += a.fold[3 * k + 2];
-            }
-        if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) a.rhs[vi(n, c)] = f[c];
-        if constexpr (!PROJ) {
-            double zz[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                a.r[vi(n, c)] = f[c];
-                a.x[vi(n, c)] = 0.0;
-                zz[c] = a.dinv[vi(n, c)] * f[c];
-                a.z[vi(n, c)] = zz[c];
-            }
-            v[0] += f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
-            v[1] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
-        } else {
-            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
-                                 a.pos[3 * n + 2] - a.centre.z};
-            double zf[6];
-            zt_times(x, f, zf);
-            v[0] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) v[1 + k] += zf[k];
-            v[7] += 1.0;
-            v[8] += x[0]; v[9] += x[1]; v[10] += x[2];
-            v[11] += x[0] * x[0]; v[12] += x[1] * x[1]; v[13] += x[2] * x[2];
-            v[14] += x[0] * x[1]; v[15] += x[0] * x[2]; v[16] += x[1] * x[2];
-        }
-    }
-    store_partials<NV>(v, a.P, gridDim.x, rr.L);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.S->inverted_step = a.S->inverted;
-        a.S->done = 0;
-        a.S->converged = 0;
-        a.S->iter = 0;
-    }
-}
-
-// Rigid-mode projection of the right-hand side for a free blob in the mixed form:
-// r = f - Z (Z^T Z)^-1 Z^T f, then z = D^-1 r, x = 0, partials r.z.
-__global__ __launch_bounds__(kBlock) void k_cg_start(int nl, int rows_per_block, const double *Pg, const double *rhs,
-                                                    const double *pos, double3 centre, const double *dinv,
-                                                    double *r, double *z, double *x, double *P, CgScalars *S) {
-    const RowRange rr = row_range(nl, rows_per_block);
-    double t[17];
-    sum_partials<17>(Pg, gridDim.x, t);
-    double M[36], Gi[36], cf[6];
-    const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
-    ztz_from_moments(mom, M);
-    invert6(M, Gi);
-    for (int i = 0; i < 6; ++i) {
-        double acc = 0.0;
-        for (int j = 0; j < 6; ++j) acc += Gi[6 * i + j] * t[1 + j];
-        cf[i] = acc;
-    }
-    double v[1] = {0.0};
-    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
-        const double xx[3] = {pos[3 * n] - centre.x, pos[3 * n + 1] - centre.y, pos[3 * n + 2] - centre.z};
-        double zc[3], rl[3], zl[3];
-        z_times(xx, cf, zc);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            rl[c] = rhs[vi(n, c)] - zc[c];
-            r[vi(n, c)] = rl[c];
-            x[vi(n, c)] = 0.0;
-            zl[c] = dinv[vi(n, c)] * rl[c];
-            z[vi(n, c)] = zl[c];
-        }
-        v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
-    }
-    store_partials<1>(v, P, gridDim.x, rr.L);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        S->f2 = t[0];
-        for (int i = 0; i < 36; ++i) S->G[i] = Gi[i];
-    }
-}
-
-// ---------------------------------------------------------------- CG iteration
-template <typename T>
-struct SpmvArgs {
-    int nl, rows_per_block, iter, max_iter;
-    double epsilon2;
-    const int *slice_w;
-    const long long *slice_base;
-    const int *col;        // [nslots]
-    const T *val;          // [nslots*9]
-    const double *z;
-    const double *pos;
-    double3 centre;
-    double *p, *q;         // updated in place
-    const double *Pin;     // partials of the previous kernel (start: r.z [and |f|^2]; later: r.r, r.z)
-    double *P1;            // partials of this kernel
-    CgScalars *S;
-};
-
-// Iteration `iter`: first the convergence test of the previous iteration
-// (|r|^2/|f|^2 < epsilon^2, NoMassCGSolver.cpp:223-237), then
-// p = beta p + z (mat_vec_fns.cpp:202-212) and q = V p, formed as q = V z + beta q
-// so the row gathers only z (SparseMatrixFixedPattern.cpp:148-187), and the p.q
-// partials (NoMassCGSolver.cpp:251-265).
-template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
-    CgScalars *S = a.S;
-    if (S->done) return;
-    const int i = a.iter;
-    const bool first = (i == 0);
-    double beta = 0.0;
-    if (first) {
-        if (S->inverted_step > 0) {  // an inverted mesh is not solved
-            if (blockIdx.x == 0 && threadIdx.x == 0) S->done = 1;
-            return;
-        }
-        if constexpr (PROJ) {
-            double t[1];
-            sum_partials<1>(a.Pin, gridDim.x, t);
-            if (blockIdx.x == 0 && threadIdx.x == 0) S->delta[1] = t[0];
-        } else {
-            double t[2];
-            sum_partials<2>(a.Pin, gridDim.x, t);
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                S->delta[1] = t[0];
-                S->f2 = t[1];
-            }
-        }
-    } else {
-        double t[2];
-        sum_partials<2>(a.Pin, gridDim.x, t);  // r.r, r.z of iteration i-1
-        const double f2 = S->f2;
-        const double res = (f2 == 0.0) ? 0.0 : t[0] / f2;
-        if (res < a.epsilon2 || i >= a.max_iter) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                S->converged = res < a.epsilon2 ? 1 : 0;
-                S->iter = i;
-                S->done = 1;
-            }
-            return;
-        }
-        beta = t[1] / S->delta[(i - 2) & 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0) S->delta[(i - 1) & 1] = t[1];
-    }
-    const RowRange rr = row_range(a.nl, a.rows_per_block);
-    constexpr int NV = PROJ ? 13 : 1;
-    double v[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
-        const int sl = n / kSlice, lane = n % kSlice;
-        const int W = a.slice_w[sl];
-        const long long base = a.slice_base[sl];
-        const int *colp = a.col + base + lane;
-        const T *vp = a.val + (size_t)base * 9 + lane;
-        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        auto block = [&](int j, int c) {
-            const T *vv = vp + (size_t)j * 9 * kSlice;
-            const double z0 = a.z[vi(c, 0)], z1 = a.z[vi(c, 1)], z2 = a.z[vi(c, 2)];
-            acc0 += (double)vv[0 * kSlice] * z0; acc0 += (double)vv[1 * kSlice] * z1; acc0 += (double)vv[2 * kSlice] * z2;
-            acc1 += (double)vv[3 * kSlice] * z0; acc1 += (double)vv[4 * kSlice] * z1; acc1 += (double)vv[5 * kSlice] * z2;
-            acc2 += (double)vv[6 * kSlice] * z0; acc2 += (double)vv[7 * kSlice] * z1; acc2 += (double)vv[8 * kSlice] * z2;
-        };
-        if (W <= kMaxUnrolledRow) {
-            // all block columns of the row first, so the gathers and value loads of every
-            // block can be in flight together
-            int cc[kMaxUnrolledRow];
-#pragma unroll
-            for (int j = 0; j < kMaxUnrolledRow; ++j)
-                if (j < W) cc[j] = colp[(size_t)j * kSlice];
-#pragma unroll
-            for (int j = 0; j < kMaxUnrolledRow; ++j)
-                if (j < W) block(j, cc[j]);
-        } else {
-#pragma unroll 2
-            for (int j = 0; j < W; ++j) block(j, colp[(size_t)j * kSlice]);
-        }
-        double pn[3], qn[3] = {acc0, acc1, acc2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double zk = a.z[vi(n, k)];
-            if (first) {
-                pn[k] = zk;
-            } else {
-                pn[k] = beta * a.p[vi(n, k)] + zk;
-                qn[k] = beta * a.q[vi(n, k)] + qn[k];
-            }
-            a.p[vi(n, k)] = pn[k];
-            a.q[vi(n, k)] = qn[k];
-        }
-        v[0] += pn[0] * qn[0] + pn[1] * qn[1] + pn[2] * qn[2];
-        if constexpr (PROJ) {
-            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
-                                 a.pos[3 * n + 2] - a.centre.z};
-            double zq[6], zp[6];
-            zt_times(x, qn, zq);
-            zt_times(x, pn, zp);
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                v[1 + k] += zq[k];
-                v[7 + k] += zp[k];
-            }
-        }
-    }
-    store_partials<NV>(v, a.P1, gridDim.x, rr.L);
-}
-
-struct VecArgs {
-    int nl, rows_per_block, iter;
-    const double *p, *q, *dinv;
-    const double *pos;
-    double3 centre;
-    double *x, *r, *z;
-    const double *P1;  // partials of the SpMV step
-    double *P2;        // r.r, r.z partials of this step
-    CgScalars *S;
-};
-
-// alpha = delta / p.q, x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197),
-// z = D^-1 r (NoMassCGSolver.cpp:268-281), partials of r.r and r.z.
-// With PROJ, q is first projected onto the range of the operator:
-// q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
-template <bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
-    CgScalars *S = a.S;
-    if (S->done) return;
-    constexpr int NV = PROJ ? 13 : 1;
-    double t[NV];
-    sum_partials<NV>(a.P1, gridDim.x, t);
-    double pq = t[0];
-    double cq[6] = {0, 0, 0, 0, 0, 0};
-    if constexpr (PROJ) {
-        for (int k = 0; k < 6; ++k) {
-            double acc = 0.0;
-            for (int j = 0; j < 6; ++j) acc += S->G[6 * k + j] * t[1 + j];
-            cq[k] = acc;
-        }
-        for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
-    }
-    const double alpha = S->delta[(a.iter - 1) & 1] / pq;
-    const RowRange rr = row_range(a.nl, a.rows_per_block);
-    double v[2] = {0.0, 0.0};
-    for (int n = rr.begin + threadIdx.x; n < rr.end; n += kBlock) {
-        double qn[3] = {a.q[vi(n, 0)], a.q[vi(n, 1)], a.q[vi(n, 2)]};
-        if constexpr (PROJ) {
-            const double x[3] = {a.pos[3 * n] - a.centre.x, a.pos[3 * n + 1] - a.centre.y,
-                                 a.pos[3 * n + 2] - a.centre.z};
-            double zc[3];
-            z_times(x, cq, zc);
-            qn[0] -= zc[0];
-            qn[1] -= zc[1];
-            qn[2] -= zc[2];
-        }
-        double r3[3], z3[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            a.x[vi(n, k)] += alpha * a.p[vi(n, k)];
-            r3[k] = a.r[vi(n, k)] + (-alpha) * qn[k];
-            a.r[vi(n, k)] = r3[k];
-            z3[k] = a.dinv[vi(n, k)] * r3[k];
-            a.z[vi(n, k)] = z3[k];
-        }
-        v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
-        v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
-    }
-    store_partials<2>(v, a.P2, gridDim.x, rr.L);
-}
-
-// Blob::euler_integrate (Blob.cpp:3249-3263), mass-less branch: vel = x, pos += x dt
-__global__ __launch_bounds__(kBlock) void k_update(int nl, const double *x, double *vel, double *pos, double dt,
-                                                   CgScalars *S) {
-    const int converged = S->converged;
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) S->inverted = 0;
-    if (!converged) return;
-    const int n = blockIdx.x * kBlock + threadIdx.x;
-    if (n >= nl) return;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double v = x[vi(n, k)];
-        vel[3 * n + k] = v;
-        pos[3 * n + k] += v * dt;
-    }
-}
-
-// ---------------------------------------------------------------- copy probe
-// y = x over n doubles (16-byte vectors), grid-stride: reads n*8 and writes n*8 bytes
-__global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += stride) y[i] = x[i];
-}
-
-// ================================================================ host side
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -808,14 +79,16 @@ struct DevBuf {
         if (p) (void)hipFree(p);
     }
     void alloc(size_t count, size_t *total) {
-        n = count;
-        FFEA_HIP_TRY(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
-        *total += std::max<size_t>(1, count) * sizeof(T);
+        n = std::max<size_t>(1, count);
+        FFEA_HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+        *total += n * sizeof(T);
     }
     void upload(const T *h, size_t count, hipStream_t s) {
-        FFEA_HIP_TRY(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) FFEA_HIP_TRY(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    // not for code that may be captured into a graph
+    void zero(hipStream_t s) { FFEA_HIP_TRY(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
 };
 
 struct Phase {
@@ -825,129 +98,19 @@ struct Phase {
     long long launches = 0;
 };
 
-struct BlobBase {
-    virtual ~BlobBase() = default;
+// one blob as the caller hands it over (host pointers)
+struct BlobInput {
+    int nt = 0, ne = 0;
+    const double *pos = nullptr;
+    const int *top = nullptr;
+    const double *A = nullptr, *B = nullptr, *G = nullptr, *E = nullptr;
+    int npin = 0;
+    const int *pin = nullptr;
+    int nctf = 0;
+    const int *ctf_node = nullptr;
+    const double *ctf_force = nullptr;
+    const uint32_t *rng = nullptr;
 };
-
-}  // namespace
-
-// ---------------------------------------------------------------- the handle
-struct ffea_gpu_blob {
-    ffea_gpu_params prm{};
-    int nt = 0, ne = 0, nl = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    bool mixed = false;
-    size_t dev_bytes = 0;
-
-    // host plan
-    std::vector<int> lin_of, orig_of_lin;
-    std::vector<int> mid_a, mid_b;  // for every node: edge ends (orig) for midside nodes, -1 for corners
-    int nslices = 0;
-    long long nslots = 0, nblocks = 0;
-    bool all_pinned = false;
-
-    // device arrays (element data in T via void pointers sized by precision)
-    DevBuf<int> conn, slice_w, col;
-    DevBuf<long long> slice_base;
-    DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, fold_ptr;
-    DevBuf<unsigned char> diag_j, pinned;
-    DevBuf<double> pos, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold;
-    DevBuf<double> Pg, Pi, P1, P2;  // per-block partials: gather, CG start, SpMV step, vector step
-    DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
-    DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
-    DevBuf<mrg::Jump> jump_group, jump_lane;
-    DevBuf<CgScalars> S;
-    mrg::Jump step_jump{};
-    uint32_t rng[6] = {0, 0, 0, 0, 0, 0};  // noise stream state (host copy)
-    bool have_ctf = false, have_fold = false;
-    bool project = false;     // rigid-mode projection in the CG (mixed form, free blob)
-    double3 centre{0, 0, 0};  // rest centroid of the corner nodes
-    int grid_nodes = 0, grid_elems = 0;
-    int nb = 0, rows_per_block = 0;  // row kernels: fixed grid and rows per block
-
-    // host-visible status
-    CgScalars *h_status = nullptr;  // pinned [3]
-    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
-    int last_iters = 0;
-    long long total_iters = 0;
-    int expect_iters = 0;
-
-    // timing
-    unsigned timing = 0;  // bit i: time phase i
-    int cg_sample = 8;    // CG phases: time one iteration in cg_sample
-    Phase phase[FFEA_GPU_NUM_PHASES];
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used = 0;
-
-    ~ffea_gpu_blob() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto &e : chunk_ev)
-            if (e) (void)hipEventDestroy(e);
-        for (auto e : ev_pool) (void)hipEventDestroy(e);
-        if (h_status) (void)hipHostFree(h_status);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    hipEvent_t take_event() {
-        if (ev_used == ev_pool.size()) {
-            hipEvent_t e;
-            FFEA_HIP_TRY(hipEventCreate(&e));
-            ev_pool.push_back(e);
-        }
-        return ev_pool[ev_used++];
-    }
-    void t_begin(int ph, hipEvent_t *out) {
-        *out = nullptr;
-        if (!(timing >> ph & 1u) && !(ph >= 3 && ph <= 4)) return;
-        *out = take_event();
-        FFEA_HIP_TRY(hipEventRecord(*out, stream));
-    }
-    void t_end(int ph, hipEvent_t b, int it = -1) {
-        if (!b) return;
-        hipEvent_t e = take_event();
-        FFEA_HIP_TRY(hipEventRecord(e, stream));
-        phase[ph].ev.push_back(b);
-        phase[ph].ev.push_back(e);
-        phase[ph].iter.push_back(it);
-    }
-    // CG phases are timed on every cg_sample-th iteration only (events between
-    // kernels delay the next launch); pairs of iterations past the converged one
-    // (launches that return at once) are dropped when collected
-    bool t_cg(int ph, int it) const { return (timing >> ph & 1u) && it % cg_sample == 0; }
-    void collect_timing(int last_iter = -1) {
-        if (!timing) return;
-        FFEA_HIP_TRY(hipStreamSynchronize(stream));
-        for (auto &p : phase) {
-            for (size_t i = 0; i + 1 < p.ev.size(); i += 2) {
-                const int it = p.iter[i / 2];
-                if (it >= 0 && last_iter >= 0 && it >= last_iter) continue;
-                float ms = 0;
-                FFEA_HIP_TRY(hipEventElapsedTime(&ms, p.ev[i], p.ev[i + 1]));
-                p.ms += ms;
-                p.launches++;
-            }
-            p.ev.clear();
-            p.iter.clear();
-        }
-        ev_used = 0;
-    }
-
-    void build(const double *pos_h, const int *top, const double *A, const double *B, const double *G,
-               const double *E, int npin, const int *pin, int nctf, const int *ctf_node, const double *ctf_force,
-               const uint32_t *rng_state);
-    template <typename T>
-    void upload_elements(const std::vector<double> &jinv0, const std::vector<double> &vol0, const double *A,
-                         const double *B, const double *G, const double *E);
-    void launch_element();
-    template <typename T, bool PROJ>
-    void launch_solve(const T *val, const T *force);
-    int check_status_after_forces();
-    int solve_and_update();
-};
-
-namespace {
 
 // closed-form rest state (Blob::calc_rest_state_info, Blob.cpp:3080-3143;
 // mat3_invert, mat_vec_fns.cpp:125-154): J^-1_0 and vol_0 = |det J|/6 per element
@@ -977,234 +140,227 @@ void rest_state(const double *p0, const double *p1, const double *p2, const doub
     *vol0 = (1.0 / 6.0) * fabs(det);
 }
 
-}  // namespace
+// host plan of one blob, in local indices
+struct BlobPlan {
+    int nt = 0, ne = 0, nl = 0, nslices = 0, npin = 0;
+    bool all_pinned = false, connected = true, have_ctf = false, have_fold = false;
+    std::vector<int> lin_of, orig_of_lin, mid_a, mid_b;
+    std::vector<unsigned char> pin_lin;
+    std::vector<double> p0;         // [nl][3] linearised rest positions
+    std::vector<int> conn;          // [4][ne]
+    std::vector<double> jinv0, vol0, det0;
+    std::vector<uint32_t> ip, ic;   // node -> element incidence (e*4 + corner)
+    std::vector<int> sw;            // slice widths
+    std::vector<long long> sb;      // slice bases (nslices+1)
+    std::vector<int> col;           // [nslots] block columns
+    std::vector<uint32_t> cnt;      // [nslots+1] source prefix
+    std::vector<uint32_t> contrib;  // (e<<4)|(a<<2)|b in element order, or kIdentity
+    std::vector<unsigned char> dj;  // [nl] slot index of the diagonal block
+    std::vector<double> ctf;        // [nl][3]
+    std::vector<uint32_t> fp;       // [nl+1]
+    std::vector<double> fv;         // linearise_force increments [..][3]
+    double centre[3] = {0, 0, 0};
+    long long nblocks = 0;
+    long long nslots() const { return sb.empty() ? 0 : sb.back(); }
+};
 
-template <typename T>
-void ffea_gpu_blob::upload_elements(const std::vector<double> &jinv0, const std::vector<double> &vol0,
-                                    const double *A, const double *B, const double *G, const double *E) {
-    DevBuf<T> *bj, *bv, *bG, *bE, *bA, *bB, *bf, *bg;
-    if constexpr (std::is_same<T, double>::value) {
-        bj = &d_jinv0; bv = &d_vol0; bG = &d_G; bE = &d_E; bA = &d_A; bB = &d_B; bf = &d_force; bg = &d_geo;
-    } else {
-        bj = &f_jinv0; bv = &f_vol0; bG = &f_G; bE = &f_E; bA = &f_A; bB = &f_B; bf = &f_force; bg = &f_geo;
-    }
-    std::vector<T> tmp((size_t)9 * ne);
-    for (int k = 0; k < 9; ++k)
-        for (int e = 0; e < ne; ++e) tmp[(size_t)k * ne + e] = (T)jinv0[(size_t)9 * e + k];
-    bj->alloc(tmp.size(), &dev_bytes);
-    bj->upload(tmp, stream);
-    FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    auto up = [&](DevBuf<T> *b, const double *src) {
-        std::vector<T> t(ne);
-        for (int e = 0; e < ne; ++e) t[e] = (T)src[e];
-        b->alloc(ne, &dev_bytes);
-        b->upload(t, stream);
-        FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    };
-    up(bv, vol0.data());
-    up(bA, A);
-    up(bB, B);
-    up(bG, G);
-    up(bE, E);
-    bf->alloc((size_t)12 * ne, &dev_bytes);
-    bg->alloc((size_t)10 * ne, &dev_bytes);
-    // the geometry array is read for the previous volume of an inverted element
-    std::vector<T> g0((size_t)10 * ne, (T)0);
-    for (int e = 0; e < ne; ++e) g0[(size_t)9 * ne + e] = (T)vol0[e];
-    bg->upload(g0, stream);
-    FFEA_HIP_TRY(hipStreamSynchronize(stream));
-}
-
-void ffea_gpu_blob::build(const double *pos_h, const int *top, const double *A, const double *B, const double *G,
-                          const double *E, int npin, const int *pin, int nctf, const int *ctf_node,
-                          const double *ctf_force, const uint32_t *rng_state) {
+BlobPlan make_plan(const BlobInput &in, int blob_index) {
+    BlobPlan P;
+    const int nt = in.nt, ne = in.ne;
+    const int *top = in.top;
+    P.nt = nt;
+    P.ne = ne;
+    if ((long long)ne >= (1ll << 28) - 1) throw Invalid("mesh too large for 32-bit source lists");
+    const std::string who = blob_index >= 0 ? fmt("blob %d: ", blob_index) : std::string();
     // ---- corner / midside classification (Blob::load_topology, Blob.cpp:2071-2077)
     std::vector<signed char> kind(nt, 0);  // 1 corner, 2 midside
     for (int e = 0; e < ne; ++e)
         for (int s = 0; s < 10; ++s) {
             const int v = top[10 * e + s];
-            if (v < 0 || v >= nt) throw Invalid(fmt("Error: Element %d references an out of bounds node index.", e));
+            if (v < 0 || v >= nt)
+                throw Invalid(who + fmt("Error: Element %d references an out of bounds node index.", e));
             const signed char k = s < 4 ? 1 : 2;
             if (kind[v] && kind[v] != k)
-                throw Invalid(fmt("node %d is a corner of one element and a midside node of another", v));
+                throw Invalid(who + fmt("node %d is a corner of one element and a midside node of another", v));
             kind[v] = k;
         }
     for (int e = 0; e < ne; ++e)
         for (int s = 0; s < 4; ++s)
             for (int t = s + 1; t < 4; ++t)
-                if (top[10 * e + s] == top[10 * e + t]) throw Invalid(fmt("element %d repeats a corner node", e));
-    lin_of.assign(nt, -1);
-    orig_of_lin.clear();
+                if (top[10 * e + s] == top[10 * e + t]) throw Invalid(who + fmt("element %d repeats a corner node", e));
+    P.lin_of.assign(nt, -1);
     for (int v = 0; v < nt; ++v)
         if (kind[v] == 1) {
-            lin_of[v] = (int)orig_of_lin.size();
-            orig_of_lin.push_back(v);
+            P.lin_of[v] = (int)P.orig_of_lin.size();
+            P.orig_of_lin.push_back(v);
         }
-    nl = (int)orig_of_lin.size();
-    mid_a.assign(nt, -1);
-    mid_b.assign(nt, -1);
+    const int nl = P.nl = (int)P.orig_of_lin.size();
+    P.mid_a.assign(nt, -1);
+    P.mid_b.assign(nt, -1);
     static const int ends[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
     for (int e = 0; e < ne; ++e)
         for (int k = 0; k < 6; ++k) {
             const int m = top[10 * e + 4 + k];
-            mid_a[m] = top[10 * e + ends[k][0]];
-            mid_b[m] = top[10 * e + ends[k][1]];
+            P.mid_a[m] = top[10 * e + ends[k][0]];
+            P.mid_b[m] = top[10 * e + ends[k][1]];
         }
-
     // ---- pins
-    std::vector<unsigned char> pin_node(nt, 0), pin_lin(nl, 0);
-    for (int i = 0; i < npin; ++i) {
-        if (pin[i] < 0 || pin[i] >= nt) throw Invalid(fmt("Error: Pinned node %d references an out of bounds node index.", i));
-        pin_node[pin[i]] = 1;
+    std::vector<unsigned char> pin_node(nt, 0);
+    P.pin_lin.assign(nl, 0);
+    for (int i = 0; i < in.npin; ++i) {
+        if (in.pin[i] < 0 || in.pin[i] >= nt)
+            throw Invalid(who + fmt("Error: Pinned node %d references an out of bounds node index.", i));
+        pin_node[in.pin[i]] = 1;
     }
-    for (int n = 0; n < nl; ++n) pin_lin[n] = pin_node[orig_of_lin[n]];
-    all_pinned = (npin == nt);  // Blob.cpp:405 compares the list length with the node count
-
+    for (int n = 0; n < nl; ++n) P.pin_lin[n] = pin_node[P.orig_of_lin[n]];
+    P.npin = in.npin;
+    P.all_pinned = (in.npin == nt);  // Blob.cpp:405 compares the list length with the node count
     // ---- linearised rest positions and rest state (Blob.cpp:203-208)
-    std::vector<double> p0((size_t)3 * nl);
+    P.p0.resize((size_t)3 * nl);
     for (int n = 0; n < nl; ++n)
-        for (int c = 0; c < 3; ++c) p0[3 * n + c] = pos_h[3 * orig_of_lin[n] + c];
-    std::vector<int> conn_h((size_t)4 * ne);
+        for (int c = 0; c < 3; ++c) P.p0[3 * n + c] = in.pos[3 * P.orig_of_lin[n] + c];
+    P.conn.resize((size_t)4 * ne);
     for (int e = 0; e < ne; ++e)
-        for (int s = 0; s < 4; ++s) conn_h[(size_t)s * ne + e] = lin_of[top[10 * e + s]];
-    std::vector<double> jinv0((size_t)9 * ne), vol0(ne), det0(ne);
+        for (int s = 0; s < 4; ++s) P.conn[(size_t)s * ne + e] = P.lin_of[top[10 * e + s]];
+    P.jinv0.resize((size_t)9 * ne);
+    P.vol0.resize(ne);
+    P.det0.resize(ne);
     for (int e = 0; e < ne; ++e)
-        rest_state(&p0[3 * conn_h[e]], &p0[3 * conn_h[(size_t)ne + e]], &p0[3 * conn_h[(size_t)2 * ne + e]],
-                   &p0[3 * conn_h[(size_t)3 * ne + e]], &jinv0[(size_t)9 * e], &vol0[e], &det0[e]);
-
+        rest_state(&P.p0[3 * P.conn[e]], &P.p0[3 * P.conn[(size_t)ne + e]], &P.p0[3 * P.conn[(size_t)2 * ne + e]],
+                   &P.p0[3 * P.conn[(size_t)3 * ne + e]], &P.jinv0[(size_t)9 * e], &P.vol0[e], &P.det0[e]);
+    for (int n = 0; n < nl; ++n)
+        for (int k = 0; k < 3; ++k) P.centre[k] += P.p0[3 * n + k];
+    for (int k = 0; k < 3; ++k) P.centre[k] /= nl;
     // ---- node -> element incidence in element order (Blob.cpp:3284-3314)
-    std::vector<uint32_t> ip(nl + 1, 0), ic((size_t)4 * ne);
+    P.ip.assign(nl + 1, 0);
+    P.ic.resize((size_t)4 * ne);
     for (int e = 0; e < ne; ++e)
-        for (int s = 0; s < 4; ++s) ip[conn_h[(size_t)s * ne + e] + 1]++;
-    for (int n = 0; n < nl; ++n) ip[n + 1] += ip[n];
+        for (int s = 0; s < 4; ++s) P.ip[P.conn[(size_t)s * ne + e] + 1]++;
+    for (int n = 0; n < nl; ++n) P.ip[n + 1] += P.ip[n];
     {
-        std::vector<uint32_t> fill(ip.begin(), ip.end() - 1);
+        std::vector<uint32_t> fill(P.ip.begin(), P.ip.end() - 1);
         for (int e = 0; e < ne; ++e)
-            for (int s = 0; s < 4; ++s) ic[fill[conn_h[(size_t)s * ne + e]]++] = (uint32_t)e * 4u + (uint32_t)s;
+            for (int s = 0; s < 4; ++s) P.ic[fill[P.conn[(size_t)s * ne + e]]++] = (uint32_t)e * 4u + (uint32_t)s;
     }
-
+    {  // are the corner nodes one connected mesh? (union-find over elements)
+        std::vector<int> par(nl);
+        std::iota(par.begin(), par.end(), 0);
+        std::function<int(int)> find = [&](int v) {
+            while (par[v] != v) v = par[v] = par[par[v]];
+            return v;
+        };
+        for (int e = 0; e < ne; ++e)
+            for (int k = 1; k < 4; ++k) {
+                const int ra = find(P.conn[e]), rb = find(P.conn[(size_t)k * ne + e]);
+                if (ra != rb) par[ra] = rb;
+            }
+        const int r0 = find(0);
+        for (int n = 1; n < nl && P.connected; ++n) P.connected = find(n) == r0;
+    }
     // ---- block pattern (NoMassCGSolver.cpp:81-127 restricted to corner nodes):
     // row n holds a block for each unpinned corner neighbour m (ascending), whose
     // sources are the elements containing both, in element order; pinned rows and
     // rows without element sources hold an identity diagonal block only.
-    struct Nb { int m; uint32_t code; };
-    std::vector<int> rowlen(nl);
-    nslices = (nl + kSlice - 1) / kSlice;
-    std::vector<int> sw(nslices, 0);
-    // first pass: row lengths
+    struct Nb {
+        int m;
+        uint32_t code;
+    };
+    P.nslices = (nl + kSlice - 1) / kSlice;
+    P.sw.assign(P.nslices, 0);
     std::vector<Nb> tmp;
     auto row_entries = [&](int n, std::vector<Nb> &out) {
         out.clear();
-        if (pin_lin[n]) return;
-        for (uint32_t k = ip[n]; k < ip[n + 1]; ++k) {
-            const int e = (int)(ic[k] >> 2), a = (int)(ic[k] & 3);
+        if (P.pin_lin[n]) return;
+        for (uint32_t k = P.ip[n]; k < P.ip[n + 1]; ++k) {
+            const int e = (int)(P.ic[k] >> 2), a = (int)(P.ic[k] & 3);
             for (int b = 0; b < 4; ++b) {
-                const int m = conn_h[(size_t)b * ne + e];
-                if (pin_lin[m]) continue;
+                const int m = P.conn[(size_t)b * ne + e];
+                if (P.pin_lin[m]) continue;
                 out.push_back({m, ((uint32_t)e << 4) | ((uint32_t)a << 2) | (uint32_t)b});
             }
         }
         std::stable_sort(out.begin(), out.end(), [](const Nb &x, const Nb &y) { return x.m < y.m; });
     };
-    long long total_contrib = 0;
     for (int n = 0; n < nl; ++n) {
         row_entries(n, tmp);
         int distinct = 0;
         for (size_t t = 0; t < tmp.size(); ++t)
             if (t == 0 || tmp[t].m != tmp[t - 1].m) ++distinct;
         if (distinct == 0) distinct = 1;  // identity diagonal
-        rowlen[n] = distinct;
-        total_contrib += tmp.empty() ? 1 : (long long)tmp.size();
-        sw[n / kSlice] = std::max(sw[n / kSlice], distinct);
+        if (distinct > 254) throw Invalid(who + fmt("node %d has more than 253 neighbours", P.orig_of_lin[n]));
+        P.sw[n / kSlice] = std::max(P.sw[n / kSlice], distinct);
     }
-    if ((long long)ne * 16 >= (1ll << 32) - 1 || (long long)ne >= (1ll << 28))
-        throw Invalid("mesh too large for 32-bit contributor lists");
-    std::vector<long long> sb(nslices + 1, 0);
-    for (int s = 0; s < nslices; ++s) sb[s + 1] = sb[s] + (long long)sw[s] * kSlice;
-    nslots = sb[nslices];
-    if (nslots >= (1ll << 31)) throw Invalid("mesh too large for 32-bit slot indices");
-    std::vector<int> col_h(nslots);
-    std::vector<uint32_t> cnt(nslots + 1, 0);
-    std::vector<unsigned char> dj(nl, 0);
-    // second pass: counts per slot, columns
-    std::vector<uint32_t> contrib_h;
-    contrib_h.reserve((size_t)total_contrib);
-    nblocks = 0;
-    // count pass
+    P.sb.assign(P.nslices + 1, 0);
+    for (int s = 0; s < P.nslices; ++s) P.sb[s + 1] = P.sb[s] + (long long)P.sw[s] * kSlice;
+    const long long nslots = P.nslots();
+    P.col.assign(nslots, 0);
+    P.cnt.assign(nslots + 1, 0);
+    P.dj.assign(nl, 0);
     for (int n = 0; n < nl; ++n) {
         const int s = n / kSlice, lane = n % kSlice;
         row_entries(n, tmp);
-        const long long base = sb[s];
+        const long long base = P.sb[s];
         int j = 0;
         if (tmp.empty()) {
-            const long long slot = base + lane;
-            col_h[slot] = n;
-            cnt[slot + 1] = 1;
-            dj[n] = 0;
+            P.col[base + lane] = n;
+            P.cnt[base + lane + 1] = 1;
+            P.dj[n] = 0;
             j = 1;
-            nblocks += 1;
         } else {
             size_t t = 0;
             while (t < tmp.size()) {
                 size_t t1 = t;
                 while (t1 < tmp.size() && tmp[t1].m == tmp[t].m) ++t1;
                 const long long slot = base + (long long)j * kSlice + lane;
-                col_h[slot] = tmp[t].m;
-                cnt[slot + 1] = (uint32_t)(t1 - t);
-                if (tmp[t].m == n) dj[n] = (unsigned char)j;
+                P.col[slot] = tmp[t].m;
+                P.cnt[slot + 1] = (uint32_t)(t1 - t);
+                if (tmp[t].m == n) P.dj[n] = (unsigned char)j;
                 ++j;
                 t = t1;
             }
-            nblocks += j;
         }
-        for (; j < sw[s]; ++j) col_h[base + (long long)j * kSlice + lane] = n;  // padding: zero block
+        P.nblocks += j;
+        for (; j < P.sw[s]; ++j) P.col[base + (long long)j * kSlice + lane] = n;  // padding: zero block
     }
-    // rows past nl in the last slice: pad columns with row 0
-    for (int n = nl; n < nslices * kSlice; ++n) {
-        const int s = n / kSlice, lane = n % kSlice;
-        for (int j = 0; j < sw[s]; ++j) col_h[sb[s] + (long long)j * kSlice + lane] = 0;
-    }
-    for (long long k = 0; k < nslots; ++k) cnt[k + 1] += cnt[k];
-    contrib_h.assign(cnt[nslots], 0);
+    for (long long k = 0; k < nslots; ++k) P.cnt[k + 1] += P.cnt[k];
+    P.contrib.assign(P.cnt[nslots], 0);
     for (int n = 0; n < nl; ++n) {
         const int s = n / kSlice, lane = n % kSlice;
         row_entries(n, tmp);
-        const long long base = sb[s];
+        const long long base = P.sb[s];
         if (tmp.empty()) {
-            contrib_h[cnt[base + lane]] = kIdentity;
+            P.contrib[P.cnt[base + lane]] = kIdentity;
             continue;
         }
         size_t t = 0;
         int j = 0;
         while (t < tmp.size()) {
             const long long slot = base + (long long)j * kSlice + lane;
-            uint32_t w = cnt[slot];
-            while (t < tmp.size() && tmp[t].m == col_h[slot]) contrib_h[w++] = tmp[t++].code;
+            uint32_t w = P.cnt[slot];
+            while (t < tmp.size() && tmp[t].m == P.col[slot]) P.contrib[w++] = tmp[t++].code;
             ++j;
         }
     }
-
     // ---- constant forces and the linearise_force increments of midside ones
-    std::vector<double> ctf_node_total((size_t)3 * nt, 0.0);
+    std::vector<double> ctf_total((size_t)3 * nt, 0.0);
     std::vector<unsigned char> has_ctf(nt, 0);
-    for (int i = 0; i < nctf; ++i) {
-        const int v = ctf_node[i];
-        if (v < 0 || v >= nt) throw Invalid(fmt("constant force %d references an out of bounds node index", i));
-        for (int c = 0; c < 3; ++c) ctf_node_total[3 * v + c] += ctf_force[3 * i + c];
+    for (int i = 0; i < in.nctf; ++i) {
+        const int v = in.ctf_node[i];
+        if (v < 0 || v >= nt) throw Invalid(who + fmt("constant force %d references an out of bounds node index", i));
+        for (int c = 0; c < 3; ++c) ctf_total[3 * v + c] += in.ctf_force[3 * i + c];
         has_ctf[v] = 1;
     }
-    std::vector<double> ctf_h((size_t)3 * nl, 0.0);
+    P.have_ctf = in.nctf > 0;
+    P.ctf.assign((size_t)3 * nl, 0.0);
     for (int n = 0; n < nl; ++n)
-        for (int c = 0; c < 3; ++c) ctf_h[3 * n + c] = ctf_node_total[3 * orig_of_lin[n] + c];
-    have_ctf = nctf > 0;
+        for (int c = 0; c < 3; ++c) P.ctf[3 * n + c] = ctf_total[3 * P.orig_of_lin[n] + c];
     // Blob::linearise_force (Blob.cpp:781-799): a midside force is folded half onto each
     // edge end by the first element (in element order) that holds it, then zeroed.
     std::vector<std::vector<std::array<double, 3>>> folds(nl);
-    bool any_mid_ctf = false;
     for (int v = 0; v < nt; ++v)
-        if (has_ctf[v] && kind[v] == 2) any_mid_ctf = true;
-    if (any_mid_ctf) {
-        std::vector<double> fm(ctf_node_total);  // only midside entries are read
+        if (has_ctf[v] && kind[v] == 2) P.have_fold = true;
+    if (P.have_fold) {
+        std::vector<double> fm(ctf_total);  // only midside entries are read
         static const int adj[4][3] = {{4, 5, 6}, {4, 7, 8}, {5, 7, 9}, {6, 8, 9}};
         for (int e = 0; e < ne; ++e) {
             const int *nn = top + 10 * e;
@@ -1215,254 +371,775 @@ void ffea_gpu_blob::build(const double *pos_h, const int *top, const double *A, 
                     inc3[c] = 0.5 * (fm[3 * nn[adj[a][0]] + c] + fm[3 * nn[adj[a][1]] + c] + fm[3 * nn[adj[a][2]] + c]);
                     if (inc3[c] != 0.0) nz = true;
                 }
-                if (nz) folds[lin_of[nn[a]]].push_back(inc3);
+                if (nz) folds[P.lin_of[nn[a]]].push_back(inc3);
             }
             for (int k = 4; k < 10; ++k)
                 for (int c = 0; c < 3; ++c) fm[3 * nn[k] + c] = 0.0;
         }
     }
-    std::vector<uint32_t> fp(nl + 1, 0);
-    std::vector<double> fv;
+    P.fp.assign(nl + 1, 0);
     for (int n = 0; n < nl; ++n) {
-        for (auto &f : folds[n]) fv.insert(fv.end(), f.begin(), f.end());
-        fp[n + 1] = fp[n] + (uint32_t)folds[n].size();
+        for (auto &f : folds[n]) P.fv.insert(P.fv.end(), f.begin(), f.end());
+        P.fp[n + 1] = P.fp[n] + (uint32_t)folds[n].size();
     }
-    have_fold = any_mid_ctf;
+    return P;
+}
 
-    // ---- noise stream jump tables
-    const int ngroups = (ne + 63) / 64;
-    std::vector<mrg::Jump> jg(ngroups), jl(64);
-    const mrg::Jump a7 = mrg::h_pow(7), a448 = mrg::h_pow(7ull * 64ull);
-    jl[0] = mrg::h_identity();
-    for (int l = 1; l < 64; ++l) jl[l] = mrg::h_mul(a7, jl[l - 1]);
-    jg[0] = mrg::h_identity();
-    for (int g = 1; g < ngroups; ++g) jg[g] = mrg::h_mul(a448, jg[g - 1]);
-    step_jump = mrg::h_pow(7ull * (unsigned long long)ne);
+// per-blob bookkeeping kept on the host
+struct BlobMeta {
+    int nt = 0, ne = 0, nl = 0;
+    int row_off = 0, elem_off = 0;
+    long long nblocks = 0;
+    bool all_pinned = false, project = false;
+    std::vector<int> lin_of, orig_of_lin, mid_a, mid_b;
+    int last_iters = 0;
+    long long total_iters = 0;
+};
 
-    // ---- device upload
-    grid_nodes = (nl + kBlock - 1) / kBlock;
-    grid_elems = (ne + kBlock - 1) / kBlock;
-    conn.alloc(conn_h.size(), &dev_bytes); conn.upload(conn_h, stream);
-    pos.alloc((size_t)3 * nl, &dev_bytes); pos.upload(p0, stream);
-    vel.alloc((size_t)3 * nl, &dev_bytes);
-    FFEA_HIP_TRY(hipMemsetAsync(vel.p, 0, sizeof(double) * 3 * nl, stream));
-    last_det.alloc(ne, &dev_bytes); last_det.upload(det0, stream);
-    inc_ptr.alloc(ip.size(), &dev_bytes); inc_ptr.upload(ip, stream);
-    inc.alloc(ic.size(), &dev_bytes); inc.upload(ic, stream);
-    std::vector<unsigned char> pin_u(pin_lin);
-    pinned.alloc(nl, &dev_bytes); pinned.upload(pin_u, stream);
-    slice_w.alloc(nslices, &dev_bytes); slice_w.upload(sw, stream);
-    std::vector<long long> sbase(sb.begin(), sb.end() - 1);
-    slice_base.alloc(nslices, &dev_bytes); slice_base.upload(sbase, stream);
-    col.alloc(nslots, &dev_bytes); col.upload(col_h, stream);
-    cptr.alloc(cnt.size(), &dev_bytes); cptr.upload(cnt, stream);
-    contrib.alloc(contrib_h.size(), &dev_bytes); contrib.upload(contrib_h, stream);
-    diag_j.alloc(nl, &dev_bytes); diag_j.upload(dj, stream);
-    const size_t vlen = (size_t)nslices * 3 * kSlice;  // tiled node vectors (vi)
-    for (DevBuf<double> *b : {&dinv, &rhs, &r, &z, &x, &q, &p}) {
-        b->alloc(vlen, &dev_bytes);
-        FFEA_HIP_TRY(hipMemsetAsync(b->p, 0, sizeof(double) * vlen, stream));
+// Every table the kernels read, for all blobs of a handle, built on the host.
+struct Tables {
+    int nblob = 0, nrows = 0, ne = 0, nslices = 0, nchunks = 0, max_blob_ne = 0, active_blobs = 0;
+    long long nslots = 0;
+    bool have_ctf = false, have_fold = false, any_project = false;
+    std::vector<BlobMeta> meta;
+    std::vector<int> conn, sw, col, eblob, eoff, proj;
+    std::vector<long long> sb;
+    std::vector<uint32_t> cnt, contrib, ip, ic, fp, rng;
+    std::vector<unsigned char> dj, pin;
+    std::vector<double> pos, det, ctf, fv;
+    std::vector<double> jinv0, vol0, A, B, G, E;  // [9][ne] and [ne], element order
+    std::vector<Chunk> chunks;
+    std::vector<int2> seg;
+    std::vector<double3> centre;
+    std::vector<mrg::Jump> step_jump, jump_group, jump_lane;
+};
+
+Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
+    Tables t;
+    const int nblob = t.nblob = (int)in.size();
+    std::vector<BlobPlan> plans;
+    plans.reserve(nblob);
+    for (int b = 0; b < nblob; ++b) plans.push_back(make_plan(in[b], nblob > 1 ? b : -1));
+    // ---- concatenation offsets: every blob starts on a 64-row slice boundary
+    t.meta.resize(nblob);
+    long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0;
+    for (int b = 0; b < nblob; ++b) {
+        BlobMeta &m = t.meta[b];
+        const BlobPlan &P = plans[b];
+        m.nt = P.nt;
+        m.ne = P.ne;
+        m.nl = P.nl;
+        m.row_off = (int)std::min<long long>(rows, 1ll << 30);
+        m.elem_off = (int)std::min<long long>(elems, 1ll << 30);
+        m.nblocks = P.nblocks;
+        m.all_pinned = P.all_pinned;
+        // free blob (no pins): the viscosity matrix is singular with the 6 rigid modes as
+        // null space; the mixed form projects them out of every SpMV result
+        m.project = mixed && P.npin == 0 && P.connected && !P.all_pinned;
+        m.lin_of = P.lin_of;
+        m.orig_of_lin = P.orig_of_lin;
+        m.mid_a = P.mid_a;
+        m.mid_b = P.mid_b;
+        rows += (long long)P.nslices * kSlice;
+        elems += P.ne;
+        slots += P.nslots();
+        slices += P.nslices;
+        contribs += (long long)P.contrib.size();
+        t.max_blob_ne = std::max(t.max_blob_ne, P.ne);
+        if (!m.all_pinned) ++t.active_blobs;
+        t.have_ctf |= P.have_ctf;
+        t.have_fold |= P.have_fold;
+        t.any_project |= m.project;
     }
-    {
-        // row kernels: a fixed grid of at most max_row_blocks blocks (tuning knob)
-        int max_row_blocks = 1024;
-        if (const char *e = getenv("FFEA_GPU_MAX_ROW_BLOCKS")) max_row_blocks = std::max(8, atoi(e));
-        nb = std::min((nl + kBlock - 1) / kBlock, max_row_blocks);
-    }
-    if (nb >= 8) nb -= nb % 8;
-    rows_per_block = ((nl + nb - 1) / nb + kSlice - 1) / kSlice * kSlice;
-    nb = (nl + rows_per_block - 1) / rows_per_block;
-    if (nb >= 8 && nb % 8) {  // keep the XCD mapping: grow to a multiple of 8 blocks
-        nb += 8 - nb % 8;
-    }
-    Pg.alloc((size_t)17 * nb, &dev_bytes);
-    Pi.alloc((size_t)2 * nb, &dev_bytes);
-    P1.alloc((size_t)13 * nb, &dev_bytes);
-    P2.alloc((size_t)2 * nb, &dev_bytes);
-    if (have_ctf) { ctf.alloc(ctf_h.size(), &dev_bytes); ctf.upload(ctf_h, stream); }
-    if (have_fold) {
-        fold_ptr.alloc(fp.size(), &dev_bytes); fold_ptr.upload(fp, stream);
-        fold.alloc(fv.size(), &dev_bytes); fold.upload(fv, stream);
-    }
-    jump_group.alloc(ngroups, &dev_bytes); jump_group.upload(jg, stream);
-    jump_lane.alloc(64, &dev_bytes); jump_lane.upload(jl, stream);
-    S.alloc(1, &dev_bytes);
-    CgScalars s0{};
-    S.upload(&s0, 1, stream);
-    for (int i = 0; i < 6; ++i) rng[i] = rng_state[i];
-    // free blob (no pins, no stokes drag): the viscosity matrix is singular with the 6
-    // rigid modes as null space. The mixed form projects them out of every SpMV result.
-    {
-        double c[3] = {0, 0, 0};
-        for (int n = 0; n < nl; ++n)
-            for (int k = 0; k < 3; ++k) c[k] += p0[3 * n + k];
-        centre = make_double3(c[0] / nl, c[1] / nl, c[2] / nl);
-        bool connected = true;
-        {  // union-find over element corners
-            std::vector<int> par(nl);
-            std::iota(par.begin(), par.end(), 0);
-            std::function<int(int)> find = [&](int v) {
-                while (par[v] != v) v = par[v] = par[par[v]];
-                return v;
-            };
-            for (int e = 0; e < ne; ++e)
-                for (int k = 1; k < 4; ++k) {
-                    int ra = find(conn_h[e]), rb = find(conn_h[(size_t)k * ne + e]);
-                    if (ra != rb) par[ra] = rb;
-                }
-            const int r0 = find(0);
-            for (int n = 1; n < nl && connected; ++n) connected = find(n) == r0;
+    if (rows >= (1ll << 30) || elems >= (1ll << 28) - 1 || slots >= (1ll << 31) || contribs >= (1ll << 32) - 1)
+        throw Invalid("problem too large for 32-bit indices");
+    const int nrows = t.nrows = (int)rows;
+    const int ne = t.ne = (int)elems;
+    const long long nslots = t.nslots = slots;
+    t.nslices = (int)slices;
+    // ---- concatenated arrays
+    t.conn.assign((size_t)4 * ne, 0);
+    t.sw.assign(t.nslices, 0);
+    t.sb.assign(t.nslices, 0);
+    t.col.assign(nslots, 0);
+    t.cnt.assign(nslots + 1, 0);
+    t.contrib.reserve((size_t)contribs);
+    t.ip.assign(nrows + 1, 0);
+    t.ic.assign((size_t)4 * ne, 0);
+    t.fp.assign(nrows + 1, 0);
+    t.dj.assign(nrows, 0xFF);
+    t.pin.assign(nrows, 0);
+    t.pos.assign((size_t)3 * nrows, 0.0);
+    t.det.assign(ne, 0.0);
+    if (t.have_ctf) t.ctf.assign((size_t)3 * nrows, 0.0);
+    if (nblob > 1) t.eblob.assign(ne, 0);
+    t.eoff.assign(nblob, 0);
+    t.proj.assign(nblob, 0);
+    t.seg.resize(nblob);
+    t.centre.resize(nblob);
+    t.step_jump.resize(nblob);
+    t.rng.assign((size_t)6 * nblob, 0);
+    t.jinv0.assign((size_t)9 * ne, 0.0);
+    t.vol0.assign(ne, 0.0);
+    t.A.assign(ne, 0.0);
+    t.B.assign(ne, 0.0);
+    t.G.assign(ne, 0.0);
+    t.E.assign(ne, 0.0);
+    long long slot_off = 0;
+    int slice_off = 0;
+    for (int b = 0; b < nblob; ++b) {
+        BlobPlan &P = plans[b];
+        const BlobMeta &m = t.meta[b];
+        const int ro = m.row_off, eo = m.elem_off;
+        t.eoff[b] = eo;
+        t.proj[b] = m.project ? 1 : 0;
+        for (int s = 0; s < 4; ++s)
+            for (int e = 0; e < P.ne; ++e) t.conn[(size_t)s * ne + eo + e] = P.conn[(size_t)s * P.ne + e] + ro;
+        for (int e = 0; e < P.ne; ++e) {
+            if (nblob > 1) t.eblob[eo + e] = b;
+            t.det[eo + e] = P.det0[e];
+            for (int k = 0; k < 9; ++k) t.jinv0[(size_t)k * ne + eo + e] = P.jinv0[(size_t)9 * e + k];
+            t.vol0[eo + e] = P.vol0[e];
+            t.A[eo + e] = in[b].A[e];
+            t.B[eo + e] = in[b].B[e];
+            t.G[eo + e] = in[b].G[e];
+            t.E[eo + e] = in[b].E[e];
         }
-        project = mixed && npin == 0 && connected;
+        for (int s = 0; s < P.nslices; ++s) {
+            t.sw[slice_off + s] = P.sw[s];
+            t.sb[slice_off + s] = slot_off + P.sb[s];
+        }
+        const uint32_t cbase = (uint32_t)t.contrib.size();
+        for (long long k = 0; k < P.nslots(); ++k) {
+            t.col[slot_off + k] = P.col[k] + ro;
+            t.cnt[slot_off + k + 1] = cbase + P.cnt[k + 1];
+        }
+        for (uint32_t code : P.contrib)
+            t.contrib.push_back(code == kIdentity ? kIdentity : code + ((uint32_t)eo << 4));
+        for (int n = 0; n < P.nl; ++n) {
+            t.dj[ro + n] = P.dj[n];
+            t.pin[ro + n] = P.pin_lin[n];
+            for (int k = 0; k < 3; ++k) t.pos[3 * (size_t)(ro + n) + k] = P.p0[3 * n + k];
+            if (t.have_ctf)
+                for (int k = 0; k < 3; ++k) t.ctf[3 * (size_t)(ro + n) + k] = P.ctf[3 * n + k];
+            // per-row counts, prefix-summed over all rows below
+            t.ip[ro + n + 1] = P.ip[n + 1] - P.ip[n];
+            t.fp[ro + n + 1] = P.fp[n + 1] - P.fp[n];
+        }
+        for (size_t k = 0; k < P.ic.size(); ++k) t.ic[(size_t)4 * eo + k] = P.ic[k] + 4u * (uint32_t)eo;
+        t.fv.insert(t.fv.end(), P.fv.begin(), P.fv.end());
+        // reduction chunks: up to kChunkRows rows of one blob each
+        t.seg[b].x = (int)t.chunks.size();
+        if (!m.all_pinned) {
+            const int first = (int)t.chunks.size();
+            for (int r0 = 0; r0 < P.nl; r0 += kChunkRows)
+                t.chunks.push_back(Chunk{b, ro + r0, ro + std::min(P.nl, r0 + kChunkRows), first});
+        }
+        t.seg[b].y = (int)t.chunks.size();
+        t.centre[b] = make_double3(P.centre[0], P.centre[1], P.centre[2]);
+        // noise stream: 7 draws per element per pass; none for an all-pinned blob, whose
+        // update_internal_forces returns at once (Blob.cpp:405-407)
+        t.step_jump[b] = m.all_pinned ? mrg::h_identity() : mrg::h_pow(7ull * (unsigned long long)P.ne);
+        for (int i = 0; i < 6; ++i) t.rng[6 * b + i] = in[b].rng[i];
+        slot_off += P.nslots();
+        slice_off += P.nslices;
+        P = BlobPlan();  // release the blob's host plan
     }
-    FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    if (mixed) {
-        upload_elements<float>(jinv0, vol0, A, B, G, E);
-        f_val.alloc((size_t)nslots * 9, &dev_bytes);
-        FFEA_HIP_TRY(hipMemsetAsync(f_val.p, 0, sizeof(float) * 9 * nslots, stream));
+    for (int n = 0; n < nrows; ++n) {
+        t.ip[n + 1] += t.ip[n];
+        t.fp[n + 1] += t.fp[n];
+    }
+    t.nchunks = (int)t.chunks.size();
+    // ---- noise jump tables: A^(7*64*g) per group of 64 elements, A^(7*l) per lane
+    const int ngroups = std::max(1, (t.max_blob_ne + 63) / 64);
+    t.jump_group.resize(ngroups);
+    t.jump_lane.resize(64);
+    const mrg::Jump a7 = mrg::h_pow(7), a448 = mrg::h_pow(7ull * 64ull);
+    t.jump_lane[0] = mrg::h_identity();
+    for (int l = 1; l < 64; ++l) t.jump_lane[l] = mrg::h_mul(a7, t.jump_lane[l - 1]);
+    t.jump_group[0] = mrg::h_identity();
+    for (int g = 1; g < ngroups; ++g) t.jump_group[g] = mrg::h_mul(a448, t.jump_group[g - 1]);
+    return t;
+}
+
+// Every index a kernel forms from the tables lies inside the buffer it addresses
+// (the kernels themselves do not check). Throws Invalid naming the first violation.
+void validate(const Tables &t) {
+    auto fail = [](const char *what, long long i) { throw Invalid(fmt("internal layout error: %s at %lld", what, i)); };
+    const long long nrows = t.nrows, ne = t.ne, nslots = t.nslots, nc = t.nchunks;
+    if (nrows != (long long)t.nslices * kSlice) fail("row count is not whole slices", nrows);
+    if ((long long)t.conn.size() != 4 * ne) fail("conn size", (long long)t.conn.size());
+    for (size_t i = 0; i < t.conn.size(); ++i)
+        if (t.conn[i] < 0 || t.conn[i] >= nrows) fail("conn", (long long)i);
+    long long expect_base = 0;
+    for (int s = 0; s < t.nslices; ++s) {
+        if (t.sb[s] != expect_base || t.sw[s] < 1 || t.sw[s] > 254) fail("slice base/width", s);
+        expect_base += (long long)t.sw[s] * kSlice;
+    }
+    if (expect_base != nslots || (long long)t.col.size() != nslots) fail("slot count", expect_base);
+    for (long long k = 0; k < nslots; ++k)
+        if (t.col[k] < 0 || t.col[k] >= nrows) fail("block column", k);
+    if ((long long)t.cnt.size() != nslots + 1 || t.cnt[0] != 0 || t.cnt[nslots] != t.contrib.size())
+        fail("source prefix ends", nslots);
+    for (long long k = 0; k < nslots; ++k)
+        if (t.cnt[k + 1] < t.cnt[k]) fail("source prefix order", k);
+    for (size_t i = 0; i < t.contrib.size(); ++i)
+        if (t.contrib[i] != kIdentity && (long long)(t.contrib[i] >> 4) >= ne) fail("source element", (long long)i);
+    if ((long long)t.dj.size() != nrows || (long long)t.pin.size() != nrows) fail("row table size", nrows);
+    for (long long n = 0; n < nrows; ++n)
+        if (t.dj[n] != 0xFF && t.dj[n] >= t.sw[n / kSlice]) fail("diagonal slot", n);
+    if ((long long)t.ip.size() != nrows + 1 || t.ip[0] != 0 || (long long)t.ip[nrows] != 4 * ne ||
+        (long long)t.ic.size() != 4 * ne)
+        fail("incidence ends", nrows);
+    for (long long n = 0; n < nrows; ++n)
+        if (t.ip[n + 1] < t.ip[n]) fail("incidence order", n);
+    for (size_t i = 0; i < t.ic.size(); ++i)
+        if ((long long)(t.ic[i] >> 2) >= ne) fail("incidence element", (long long)i);
+    if (t.have_fold) {
+        if ((long long)t.fp.size() != nrows + 1 || t.fp[0] != 0 || 3 * (size_t)t.fp[nrows] != t.fv.size())
+            fail("fold ends", nrows);
+        for (long long n = 0; n < nrows; ++n)
+            if (t.fp[n + 1] < t.fp[n]) fail("fold order", n);
+    }
+    if (t.have_ctf && (long long)t.ctf.size() != 3 * nrows) fail("ctf size", (long long)t.ctf.size());
+    if ((long long)t.pos.size() != 3 * nrows || (long long)t.det.size() != ne) fail("node/element sizes", nrows);
+    for (const std::vector<double> *v : {&t.vol0, &t.A, &t.B, &t.G, &t.E})
+        if ((long long)v->size() != ne) fail("element array size", (long long)v->size());
+    if ((long long)t.jinv0.size() != 9 * ne) fail("rest-state size", (long long)t.jinv0.size());
+    if ((int)t.meta.size() != t.nblob || (int)t.seg.size() != t.nblob || (int)t.eoff.size() != t.nblob ||
+        (int)t.proj.size() != t.nblob || (int)t.centre.size() != t.nblob || (int)t.step_jump.size() != t.nblob ||
+        (long long)t.rng.size() != 6ll * t.nblob)
+        fail("per-blob table size", t.nblob);
+    if (t.nblob > 1 && (long long)t.eblob.size() != ne) fail("element blob size", (long long)t.eblob.size());
+    const long long ngroups = (long long)t.jump_group.size();
+    for (int b = 0; b < t.nblob; ++b) {
+        const BlobMeta &m = t.meta[b];
+        if (m.elem_off != t.eoff[b] || m.elem_off + (long long)m.ne > ne || m.row_off % kSlice ||
+            m.row_off + (long long)m.nl > nrows)
+            fail("blob offsets", b);
+        if (((long long)m.ne + 63) / 64 > ngroups) fail("jump group table", b);
+        for (int e = 0; e < m.ne && t.nblob > 1; ++e)
+            if (t.eblob[m.elem_off + e] != b) fail("element blob", (long long)m.elem_off + e);
+        const int2 sg = t.seg[b];
+        if (sg.x < 0 || sg.x > sg.y || sg.y > nc) fail("chunk segment", b);
+        for (int c = sg.x; c < sg.y; ++c) {
+            const Chunk &ch = t.chunks[c];
+            if (ch.blob != b || ch.first != sg.x || ch.begin < m.row_off || ch.end > m.row_off + m.nl ||
+                ch.begin >= ch.end)
+                fail("chunk", c);
+        }
+    }
+    if ((long long)t.chunks.size() != nc || t.jump_lane.size() != 64) fail("chunk/jump table size", nc);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- the handle
+struct ffea_gpu_blob {
+    ffea_gpu_params prm{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool mixed = false, any_project = false;
+    size_t dev_bytes = 0;
+
+    int nblob = 0, nrows = 0, ne = 0, nslices = 0, nchunks = 0, max_blob_ne = 0;
+    long long nslots = 0;
+    std::vector<BlobMeta> meta;
+    int active_blobs = 0;  // blobs that are solved (not all pinned)
+
+    // device arrays
+    DevBuf<int> conn, slice_w, col, elem_blob, elem_off, project;
+    DevBuf<long long> slice_base;
+    DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, fold_ptr, rng;
+    DevBuf<unsigned char> diag_j, pinned;
+    DevBuf<double> pos, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
+    DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
+    DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
+    DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
+    DevBuf<mrg::Jump> jump_group, jump_lane, step_jump;
+    DevBuf<CgScalars> S;
+    DevBuf<Chunk> chunks;
+    DevBuf<int2> seg;
+    DevBuf<double3> centre;
+    DevBuf<int> num_done;
+    bool have_ctf = false, have_fold = false;
+    int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
+    int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
+
+    // host-visible status (pinned)
+    CgScalars *h_S = nullptr;  // [nblob]
+    int *h_done = nullptr;     // [2]
+    hipEvent_t chunk_ev[2] = {nullptr, nullptr};
+    int expect_iters = 0;
+
+    // one whole step captured as a graph
+    bool use_graph = false, capturing = false;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_iters = 0;
+    int graph_fixed_iters = 0;  // > 0: CG iterations captured; 0: from the last step's count
+
+    // timing
+    unsigned timing = 0;  // bit i: time phase i
+    int cg_sample = 8;    // CG phases: time one iteration in cg_sample
+    Phase phase[FFEA_GPU_NUM_PHASES];
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+
+    ~ffea_gpu_blob() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        for (auto &e : chunk_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (h_S) (void)hipHostFree(h_S);
+        if (h_done) (void)hipHostFree(h_done);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    hipEvent_t take_event() {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e;
+            FFEA_HIP_TRY(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_used++];
+    }
+    bool timing_on(int ph) const { return (timing >> ph & 1u) && !capturing; }
+    void t_begin(int ph, hipEvent_t *out) {
+        *out = nullptr;
+        if (!timing_on(ph)) return;
+        *out = take_event();
+        FFEA_HIP_TRY(hipEventRecord(*out, stream));
+    }
+    void t_end(int ph, hipEvent_t b, int it = -1) {
+        if (!b) return;
+        hipEvent_t e = take_event();
+        FFEA_HIP_TRY(hipEventRecord(e, stream));
+        phase[ph].ev.push_back(b);
+        phase[ph].ev.push_back(e);
+        phase[ph].iter.push_back(it);
+    }
+    // CG phases are timed on every cg_sample-th iteration only (events between kernels
+    // delay the next launch); pairs of iterations at or past the last one of the step
+    // (launches that return at once) are dropped when collected
+    bool t_cg(int ph, int it) const { return timing_on(ph) && it % cg_sample == 0; }
+    void collect_timing(int last_iter = -1) {
+        if (!timing) return;
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+        for (auto &ph : phase) {
+            for (size_t i = 0; i + 1 < ph.ev.size(); i += 2) {
+                const int it = ph.iter[i / 2];
+                if (it >= 0 && last_iter >= 0 && it >= last_iter) continue;
+                float ms = 0;
+                FFEA_HIP_TRY(hipEventElapsedTime(&ms, ph.ev[i], ph.ev[i + 1]));
+                ph.ms += ms;
+                ph.launches++;
+            }
+            ph.ev.clear();
+            ph.iter.clear();
+        }
+        ev_used = 0;
+    }
+
+    void build(const std::vector<BlobInput> &in);
+    template <typename T>
+    void upload_elements(const Tables &t);
+    void set_row_blocks(int blocks);
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+    }
+    RowArgs row_args() { return RowArgs{nchunks, cpb, chunks.p, seg.p, S.p, num_done.p}; }
+    void launch_element();
+    void launch_assemble_gather();
+    template <typename T, bool PROJ>
+    void launch_gather(const T *force);
+    void launch_cg_iteration(int it, bool spmv_only);
+    void launch_cg_vec(int it);
+    template <typename T, bool PROJ>
+    void launch_cg_iteration_t(int it, bool spmv_only, const T *val);
+    void launch_update();
+    int run_cg_eager(int start_iter);
+    int finish_step();
+    int step_once();
+    void capture_graph(int iters);
+    int check_status_after_forces();
+};
+
+template <typename T>
+static void upload_converted(DevBuf<T> &buf, const std::vector<double> &h, hipStream_t s, size_t *total) {
+    std::vector<T> t(h.size());
+    for (size_t i = 0; i < h.size(); ++i) t[i] = (T)h[i];
+    buf.alloc(t.size(), total);
+    buf.upload(t, s);
+    FFEA_HIP_TRY(hipStreamSynchronize(s));  // t goes out of scope
+}
+
+template <typename T>
+void ffea_gpu_blob::upload_elements(const Tables &t) {
+    DevBuf<T> *bj, *bv, *bG, *bE, *bA, *bB, *bf, *bg, *bval;
+    if constexpr (std::is_same<T, double>::value) {
+        bj = &d_jinv0; bv = &d_vol0; bG = &d_G; bE = &d_E; bA = &d_A; bB = &d_B; bf = &d_force; bg = &d_geo;
+        bval = &d_val;
     } else {
-        upload_elements<double>(jinv0, vol0, A, B, G, E);
-        d_val.alloc((size_t)nslots * 9, &dev_bytes);
-        FFEA_HIP_TRY(hipMemsetAsync(d_val.p, 0, sizeof(double) * 9 * nslots, stream));
+        bj = &f_jinv0; bv = &f_vol0; bG = &f_G; bE = &f_E; bA = &f_A; bB = &f_B; bf = &f_force; bg = &f_geo;
+        bval = &f_val;
     }
-    FFEA_HIP_TRY(hipHostMalloc(&h_status, sizeof(CgScalars) * 3, hipHostMallocDefault));
-    for (auto &e : chunk_ev) FFEA_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    upload_converted(*bj, t.jinv0, stream, &dev_bytes);
+    upload_converted(*bv, t.vol0, stream, &dev_bytes);
+    upload_converted(*bA, t.A, stream, &dev_bytes);
+    upload_converted(*bB, t.B, stream, &dev_bytes);
+    upload_converted(*bG, t.G, stream, &dev_bytes);
+    upload_converted(*bE, t.E, stream, &dev_bytes);
+    bf->alloc((size_t)12 * ne, &dev_bytes);
+    bf->zero(stream);
+    // shape data: the volume row is read for the previous volume of an inverted element
+    std::vector<double> g0((size_t)10 * ne, 0.0);
+    std::copy(t.vol0.begin(), t.vol0.end(), g0.begin() + (size_t)9 * ne);
+    upload_converted(*bg, g0, stream, &dev_bytes);
+    bval->alloc((size_t)nslots * 9, &dev_bytes);
+    bval->zero(stream);
+}
+
+void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
+    Tables t = make_tables(in, mixed);
+    validate(t);
+    nblob = t.nblob;
+    nrows = t.nrows;
+    ne = t.ne;
+    nslices = t.nslices;
+    nslots = t.nslots;
+    nchunks = t.nchunks;
+    max_blob_ne = t.max_blob_ne;
+    active_blobs = t.active_blobs;
+    have_ctf = t.have_ctf;
+    have_fold = t.have_fold;
+    any_project = t.any_project;
+    meta = std::move(t.meta);
+    grid_rows = (nrows + kBlock - 1) / kBlock;
+    grid_elems = (ne + kBlock - 1) / kBlock;
+    grid_blobs = (nblob + kBlock - 1) / kBlock;
+    conn.alloc(t.conn.size(), &dev_bytes); conn.upload(t.conn, stream);
+    pos.alloc(t.pos.size(), &dev_bytes); pos.upload(t.pos, stream);
+    vel.alloc(t.pos.size(), &dev_bytes); vel.zero(stream);
+    last_det.alloc(t.det.size(), &dev_bytes); last_det.upload(t.det, stream);
+    inc_ptr.alloc(t.ip.size(), &dev_bytes); inc_ptr.upload(t.ip, stream);
+    inc.alloc(t.ic.size(), &dev_bytes); inc.upload(t.ic, stream);
+    pinned.alloc(t.pin.size(), &dev_bytes); pinned.upload(t.pin, stream);
+    slice_w.alloc(t.sw.size(), &dev_bytes); slice_w.upload(t.sw, stream);
+    slice_base.alloc(t.sb.size(), &dev_bytes); slice_base.upload(t.sb, stream);
+    col.alloc(t.col.size(), &dev_bytes); col.upload(t.col, stream);
+    cptr.alloc(t.cnt.size(), &dev_bytes); cptr.upload(t.cnt, stream);
+    contrib.alloc(t.contrib.size(), &dev_bytes); contrib.upload(t.contrib, stream);
+    diag_j.alloc(t.dj.size(), &dev_bytes); diag_j.upload(t.dj, stream);
+    if (nblob > 1) {
+        elem_blob.alloc(t.eblob.size(), &dev_bytes);
+        elem_blob.upload(t.eblob, stream);
+    }
+    elem_off.alloc(t.eoff.size(), &dev_bytes); elem_off.upload(t.eoff, stream);
+    project.alloc(t.proj.size(), &dev_bytes); project.upload(t.proj, stream);
+    const size_t vlen = (size_t)nslices * 3 * kSlice;  // tiled node vectors (vi)
+    for (DevBuf<double> *buf : {&dinv, &rhs, &r, &z, &x, &q, &p}) {
+        buf->alloc(vlen, &dev_bytes);
+        buf->zero(stream);
+    }
+    const size_t nc = (size_t)std::max(1, nchunks);
+    Pg.alloc(17 * nc, &dev_bytes); Pg.zero(stream);
+    Pi.alloc(2 * nc, &dev_bytes); Pi.zero(stream);
+    P1.alloc(13 * nc, &dev_bytes); P1.zero(stream);
+    P2.alloc(2 * nc, &dev_bytes); P2.zero(stream);
+    if (have_ctf) {
+        ctf.alloc(t.ctf.size(), &dev_bytes);
+        ctf.upload(t.ctf, stream);
+    }
+    if (have_fold) {
+        fold_ptr.alloc(t.fp.size(), &dev_bytes);
+        fold_ptr.upload(t.fp, stream);
+        fold.alloc(t.fv.size(), &dev_bytes);
+        fold.upload(t.fv, stream);
+    }
+    jump_group.alloc(t.jump_group.size(), &dev_bytes); jump_group.upload(t.jump_group, stream);
+    jump_lane.alloc(t.jump_lane.size(), &dev_bytes); jump_lane.upload(t.jump_lane, stream);
+    step_jump.alloc(t.step_jump.size(), &dev_bytes); step_jump.upload(t.step_jump, stream);
+    rng.alloc(t.rng.size(), &dev_bytes); rng.upload(t.rng, stream);
+    std::vector<CgScalars> s0(nblob);
+    for (int b = 0; b < nblob; ++b) {
+        s0[b] = CgScalars{};
+        s0[b].done = meta[b].all_pinned ? 1 : 0;
+    }
+    S.alloc(nblob, &dev_bytes); S.upload(s0, stream);
+    chunks.alloc(t.chunks.size(), &dev_bytes); chunks.upload(t.chunks, stream);
+    seg.alloc(t.seg.size(), &dev_bytes); seg.upload(t.seg, stream);
+    centre.alloc(t.centre.size(), &dev_bytes); centre.upload(t.centre, stream);
+    Gmat.alloc((size_t)36 * nblob, &dev_bytes); Gmat.zero(stream);
+    num_done.alloc(1, &dev_bytes); num_done.zero(stream);
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    if (mixed) upload_elements<float>(t);
+    else upload_elements<double>(t);
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    FFEA_HIP_TRY(hipHostMalloc(&h_S, sizeof(CgScalars) * nblob, hipHostMallocDefault));
+    FFEA_HIP_TRY(hipHostMalloc(&h_done, sizeof(int) * 2, hipHostMallocDefault));
+    for (auto &e : chunk_ev) FFEA_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // every buffer a kernel reads must exist
+    const void *need[] = {conn.p, pos.p, vel.p, last_det.p, inc_ptr.p, inc.p, pinned.p, slice_w.p, slice_base.p,
+                          col.p, cptr.p, contrib.p, diag_j.p, elem_off.p, project.p, dinv.p, rhs.p, r.p, z.p, x.p,
+                          q.p, p.p, Pg.p, Pi.p, P1.p, P2.p, jump_group.p, jump_lane.p, step_jump.p, rng.p, S.p,
+                          chunks.p, seg.p, centre.p, Gmat.p, num_done.p,
+                          mixed ? (const void *)f_jinv0.p : (const void *)d_jinv0.p,
+                          mixed ? (const void *)f_vol0.p : (const void *)d_vol0.p,
+                          mixed ? (const void *)f_A.p : (const void *)d_A.p,
+                          mixed ? (const void *)f_B.p : (const void *)d_B.p,
+                          mixed ? (const void *)f_G.p : (const void *)d_G.p,
+                          mixed ? (const void *)f_E.p : (const void *)d_E.p,
+                          mixed ? (const void *)f_force.p : (const void *)d_force.p,
+                          mixed ? (const void *)f_geo.p : (const void *)d_geo.p,
+                          mixed ? (const void *)f_val.p : (const void *)d_val.p};
+    for (const void *ptr : need)
+        if (!ptr) throw Invalid("internal error: a device buffer was not allocated");
+    if ((nblob > 1 && !elem_blob.p) || (have_ctf && !ctf.p) || (have_fold && (!fold_ptr.p || !fold.p)))
+        throw Invalid("internal error: a device buffer was not allocated");
+    // row-kernel grid: one block per chunk up to 1024 blocks (tunable, result-neutral)
+    set_row_blocks(1024);
+}
+
+void ffea_gpu_blob::set_row_blocks(int blocks) {
+    const int nc = std::max(1, nchunks);
+    blocks = std::max(1, std::min(blocks, nc));
+    cpb = (nc + blocks - 1) / blocks;
+    nb = (nc + cpb - 1) / cpb;
+    if (nb >= 8 && nb % 8) nb += 8 - nb % 8;  // the XCD mapping wants a multiple of 8 blocks
+    drop_graph();                             // the graph holds the old grid
 }
 
 void ffea_gpu_blob::launch_element() {
     hipEvent_t t0;
     t_begin(0, &t0);
+    const int *eb = nblob > 1 ? elem_blob.p : nullptr;
     if (mixed) {
-        ElemArgs<float> a{ne, conn.p, pos.p, f_jinv0.p, f_vol0.p, f_G.p, f_E.p, f_A.p, f_B.p, last_det.p,
-                          f_force.p, f_geo.p, prm.calc_noise, prm.kT, prm.dt, jump_group.p, jump_lane.p,
-                          {rng[0], rng[1], rng[2], rng[3], rng[4], rng[5]}, S.p};
+        ElemArgs<float> a{ne, conn.p, pos.p, f_jinv0.p, f_vol0.p, f_G.p, f_E.p, f_A.p, f_B.p, last_det.p, f_force.p,
+                          f_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
+                          rng.p, S.p};
         hipLaunchKernelGGL(k_element<float>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     } else {
-        ElemArgs<double> a{ne, conn.p, pos.p, d_jinv0.p, d_vol0.p, d_G.p, d_E.p, d_A.p, d_B.p, last_det.p,
-                           d_force.p, d_geo.p, prm.calc_noise, prm.kT, prm.dt, jump_group.p, jump_lane.p,
-                           {rng[0], rng[1], rng[2], rng[3], rng[4], rng[5]}, S.p};
+        ElemArgs<double> a{ne, conn.p, pos.p, d_jinv0.p, d_vol0.p, d_G.p, d_E.p, d_A.p, d_B.p, last_det.p, d_force.p,
+                           d_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
+                           rng.p, S.p};
         hipLaunchKernelGGL(k_element<double>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(0, t0);
     // the pass consumed 7 draws per element (tetra_element_linear.cpp:416-426)
-    if (prm.calc_noise) mrg::h_apply(step_jump, rng, rng);
+    if (prm.calc_noise) {
+        hipLaunchKernelGGL(k_rng_advance, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, rng.p,
+                           (const mrg::Jump *)step_jump.p);
+        FFEA_HIP_TRY(hipGetLastError());
+    }
+}
+
+static std::string inversion_message(int inv) {
+    return inv == 1 ? std::string("1 element has inverted since the last step. Aborting simulation.\n")
+                    : fmt("%d elements have inverted since the last step. Aborting simulation.\n", inv);
 }
 
 int ffea_gpu_blob::check_status_after_forces() {
-    FFEA_HIP_TRY(hipMemcpyAsync(h_status, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars) * nblob, hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    const int inv = h_status->inverted;
-    if (inv > 0) {
-        err = inv == 1 ? "1 element has inverted since the last step. Aborting simulation.\n"
-                       : fmt("%d elements have inverted since the last step. Aborting simulation.\n", inv);
+    for (int b = 0; b < nblob; ++b) {
+        if (meta[b].all_pinned || h_S[b].inverted <= 0) continue;
+        err = (nblob > 1 ? fmt("blob %d: ", b) : std::string()) + inversion_message(h_S[b].inverted);
         return FFEA_GPU_ERR_INVERSION;
     }
     return FFEA_GPU_OK;
 }
 
 template <typename T, bool PROJ>
-void ffea_gpu_blob::launch_solve(const T *val, const T *force) {
-    hipEvent_t t0;
-    t_begin(2, &t0);
-    GatherArgs<T> g{nl, rows_per_block, inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
+void ffea_gpu_blob::launch_gather(const T *force) {
+    GatherArgs<T> g{row_args(), inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
                     have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr, pinned.p, dinv.p, pos.p,
-                    centre, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p, S.p};
+                    centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, g);
-    if (PROJ)
-        hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kBlock), 0, stream, nl, rows_per_block, (const double *)Pg.p,
-                           (const double *)rhs.p, (const double *)pos.p, centre, (const double *)dinv.p, r.p, z.p,
-                           x.p, Pi.p, S.p);
+    if (PROJ) {
+        StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
+        hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kBlock), 0, stream, sa);
+    }
+}
+
+void ffea_gpu_blob::launch_assemble_gather() {
+    hipEvent_t t0;
+    t_begin(1, &t0);
+    if (mixed) {
+        AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
+                         f_val.p, dinv.p};
+        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
+    } else {
+        AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
+                          d_val.p, dinv.p};
+        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
+    }
+    FFEA_HIP_TRY(hipGetLastError());
+    t_end(1, t0);
+    t_begin(2, &t0);
+    if (mixed) {
+        if (any_project) launch_gather<float, true>(f_force.p);
+        else launch_gather<float, false>(f_force.p);
+    } else {
+        launch_gather<double, false>(d_force.p);
+    }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(2, t0);
+}
 
-    // CG iterations, queued in chunks. Iteration i is an SpMV step (which first tests
-    // the convergence of iteration i-1) and a vector step. Kernels queued after the
-    // solve stopped return at once; the host stops queuing once a completed chunk
-    // reports done.
+template <typename T, bool PROJ>
+void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) {
+    hipEvent_t ts = nullptr;
+    if (t_cg(3, it)) t_begin(3, &ts);
+    SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, prm.epsilon2, slice_w.p, slice_base.p, col.p, val, z.p,
+                   pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
+    hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
+    t_end(3, ts, it);
+    if (!spmv_only) launch_cg_vec(it);
+}
+
+void ffea_gpu_blob::launch_cg_vec(int it) {
+    hipEvent_t tv = nullptr;
+    if (t_cg(4, it)) t_begin(4, &tv);
+    VecArgs va{row_args(), it, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
+    if (mixed && any_project) hipLaunchKernelGGL(k_cg_vec<true>, dim3(nb), dim3(kBlock), 0, stream, va);
+    else hipLaunchKernelGGL(k_cg_vec<false>, dim3(nb), dim3(kBlock), 0, stream, va);
+    t_end(4, tv, it);
+}
+
+void ffea_gpu_blob::launch_cg_iteration(int it, bool spmv_only) {
+    if (mixed) {
+        if (any_project) launch_cg_iteration_t<float, true>(it, spmv_only, f_val.p);
+        else launch_cg_iteration_t<float, false>(it, spmv_only, f_val.p);
+    } else {
+        launch_cg_iteration_t<double, false>(it, spmv_only, d_val.p);
+    }
+}
+
+void ffea_gpu_blob::launch_update() {
+    hipEvent_t t0;
+    t_begin(5, &t0);
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(kBlock), 0, stream, row_args(), (const double *)x.p, vel.p, pos.p,
+                       prm.dt);
+    hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    t_end(5, t0);
+}
+
+// CG iterations from start_iter on, queued in chunks. Iteration i is an SpMV step (which
+// first tests each blob's iteration i-1) and a vector step. Kernels of blobs whose solve
+// stopped return at once; the host stops queuing once a completed chunk reports every
+// solved blob done. Returns the first iteration not queued.
+int ffea_gpu_blob::run_cg_eager(int start_iter) {
     const int imax = prm.max_iterations_cg;
-    int launched = 0, chunk_idx = 0;
-    int chunk = std::max(4, expect_iters - 2);
+    int launched = start_iter, chunk_idx = 0;
+    int chunk = std::max(4, expect_iters - 2 - start_iter);
     bool stop = false;
     while (launched <= imax && !stop) {
         const int nit = std::min(chunk, imax + 1 - launched);
-        for (int k = 0; k < nit; ++k) {
-            const int it = launched + k;
-            hipEvent_t ts = nullptr;
-            if (t_cg(3, it)) t_begin(3, &ts);
-            SpmvArgs<T> sa{nl, rows_per_block, it, imax, prm.epsilon2, slice_w.p, slice_base.p, col.p, val, z.p,
-                           pos.p, centre, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p, S.p};
-            hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
-            t_end(3, ts, it);
-            if (it == imax) break;  // that SpMV step only tests the last iteration
-            hipEvent_t tv = nullptr;
-            if (t_cg(4, it)) t_begin(4, &tv);
-            VecArgs va{nl, rows_per_block, it, p.p, q.p, dinv.p, pos.p, centre, x.p, r.p, z.p, P1.p, P2.p, S.p};
-            hipLaunchKernelGGL((k_cg_vec<PROJ>), dim3(nb), dim3(kBlock), 0, stream, va);
-            t_end(4, tv, it);
-        }
+        // the SpMV step of iteration imax only tests iteration imax-1
+        for (int k = 0; k < nit; ++k) launch_cg_iteration(launched + k, launched + k == imax);
         FFEA_HIP_TRY(hipGetLastError());
         launched += nit;
         const int slot = chunk_idx & 1;
-        FFEA_HIP_TRY(hipMemcpyAsync(&h_status[slot], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipMemcpyAsync(&h_done[slot], num_done.p, sizeof(int), hipMemcpyDeviceToHost, stream));
         FFEA_HIP_TRY(hipEventRecord(chunk_ev[slot], stream));
         if (chunk_idx > 0) {
             FFEA_HIP_TRY(hipEventSynchronize(chunk_ev[slot ^ 1]));
-            if (h_status[slot ^ 1].done) stop = true;
+            if (h_done[slot ^ 1] >= active_blobs) stop = true;
         }
         ++chunk_idx;
         chunk = 4;
     }
+    return launched;
 }
 
-int ffea_gpu_blob::solve_and_update() {
-    // matrix assembly
-    hipEvent_t t0;
-    t_begin(1, &t0);
-    if (mixed) {
-        AsmArgs<float> a{nl, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
-                         f_val.p, dinv.p};
-        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_nodes), dim3(kBlock), 0, stream, a);
-    } else {
-        AsmArgs<double> a{nl, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
-                          d_val.p, dinv.p};
-        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_nodes), dim3(kBlock), 0, stream, a);
-    }
-    FFEA_HIP_TRY(hipGetLastError());
-    t_end(1, t0);
-    if (mixed) {
-        if (project) launch_solve<float, true>(f_val.p, f_force.p);
-        else launch_solve<float, false>(f_val.p, f_force.p);
-    } else {
-        launch_solve<double, false>(d_val.p, d_force.p);
-    }
-    // Euler update (skipped unless converged)
-    t_begin(5, &t0);
-    hipLaunchKernelGGL(k_update, dim3(grid_nodes), dim3(kBlock), 0, stream, nl, (const double *)x.p, vel.p, pos.p,
-                       prm.dt, S.p);
-    FFEA_HIP_TRY(hipGetLastError());
-    t_end(5, t0);
-    FFEA_HIP_TRY(hipMemcpyAsync(&h_status[2], S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+// status of the step just queued (one synchronisation)
+int ffea_gpu_blob::finish_step() {
+    FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars) * nblob, hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    const CgScalars &st = h_status[2];
-    collect_timing(st.iter);
-    last_iters = st.iter;
-    total_iters += st.iter;
-    if (st.iter > 0) expect_iters = st.iter;
-    if (st.inverted_step > 0) {
-        const int inv = st.inverted_step;
-        err = inv == 1 ? "1 element has inverted since the last step. Aborting simulation.\n"
-                       : fmt("%d elements have inverted since the last step. Aborting simulation.\n", inv);
-        return FFEA_GPU_ERR_INVERSION;
+    int max_it = 0;
+    for (int b = 0; b < nblob; ++b) {
+        if (meta[b].all_pinned) continue;
+        meta[b].last_iters = h_S[b].iter;
+        meta[b].total_iters += h_S[b].iter;
+        max_it = std::max(max_it, h_S[b].iter);
     }
-    if (!st.converged) {
-        err = fmt("Conjugate gradient solver: Could not converge after %d iterations.\n\tEither epsilon or "
-                  "max_iterations_cg are set too low, or something went wrong with the simulation.\n",
-                  prm.max_iterations_cg);
-        return FFEA_GPU_ERR_CG_NO_CONVERGE;
+    collect_timing(max_it);
+    if (max_it > 0) expect_iters = max_it;
+    for (int b = 0; b < nblob; ++b) {
+        if (meta[b].all_pinned) continue;
+        const std::string who = nblob > 1 ? fmt("blob %d: ", b) : std::string();
+        if (h_S[b].inverted_step > 0) {
+            err = who + inversion_message(h_S[b].inverted_step);
+            return FFEA_GPU_ERR_INVERSION;
+        }
+        if (!h_S[b].converged) {
+            err = who + fmt("Conjugate gradient solver: Could not converge after %d iterations.\n\tEither epsilon or "
+                            "max_iterations_cg are set too low, or something went wrong with the simulation.\n",
+                            prm.max_iterations_cg);
+            return FFEA_GPU_ERR_CG_NO_CONVERGE;
+        }
     }
     return FFEA_GPU_OK;
+}
+
+// One step captured as a graph: element pass, stream advance, assembly, gather, `iters`
+// CG iterations, the SpMV step that tests the last of them, update. Kernels of blobs
+// whose solve stopped return at once; blobs still running after the graph continue
+// eagerly. No memset or copy is captured: every node is a kernel.
+void ffea_gpu_blob::capture_graph(int iters) {
+    drop_graph();
+    iters = std::min(iters, prm.max_iterations_cg);
+    hipGraph_t g = nullptr;
+    FFEA_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+    try {
+        launch_element();
+        launch_assemble_gather();
+        for (int it = 0; it <= iters; ++it) launch_cg_iteration(it, it == iters);
+        launch_update();
+    } catch (...) {
+        capturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    capturing = false;
+    FFEA_HIP_TRY(hipStreamEndCapture(stream, &g));
+    const hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FFEA_HIP_TRY(e);
+    graph_iters = iters;
+}
+
+int ffea_gpu_blob::step_once() {
+    if (active_blobs == 0) return FFEA_GPU_OK;
+    if (use_graph && expect_iters > 0) {
+        // enough iterations to cover the last step's count with a margin, in steps of 16
+        int want = std::min(prm.max_iterations_cg, ((expect_iters + expect_iters / 8 + 8 + 15) / 16) * 16);
+        if (graph_fixed_iters > 0) want = std::min(prm.max_iterations_cg, graph_fixed_iters);
+        if (!graph_exec || want > graph_iters || want < graph_iters / 2) capture_graph(want);
+        FFEA_HIP_TRY(hipGraphLaunch(graph_exec, stream));
+        FFEA_HIP_TRY(hipMemcpyAsync(&h_done[0], num_done.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+        if (h_done[0] < active_blobs && graph_iters < prm.max_iterations_cg) {
+            // iteration graph_iters has had its SpMV step in the graph: finish it, go on
+            launch_cg_vec(graph_iters);
+            FFEA_HIP_TRY(hipGetLastError());
+            run_cg_eager(graph_iters + 1);
+            launch_update();
+        }
+        return finish_step();
+    }
+    launch_element();
+    launch_assemble_gather();
+    run_cg_eager(0);
+    launch_update();
+    return finish_step();
 }
 
 // ================================================================ C ABI
@@ -1487,6 +1164,88 @@ int guarded(ffea_gpu_blob *b, F &&f) {
         return FFEA_GPU_ERR_INVALID;
     }
 }
+
+int check_params(const char *who, const ffea_gpu_params *params) {
+    const std::string w(who);
+    if (params->calc_stokes) {
+        g_last_error = w + ": calc_stokes is not supported by this version";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    if (params->precision != FFEA_GPU_FP64 && params->precision != FFEA_GPU_MIXED) {
+        g_last_error = w + ": unknown precision";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    if (!(params->dt > 0) || params->max_iterations_cg <= 0 || !(params->epsilon2 >= 0)) {
+        g_last_error = w + ": dt, epsilon2 and max_iterations_cg must be positive";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_last_error = w + ": no HIP device";
+        return FFEA_GPU_ERR_NO_DEVICE;
+    }
+    if (params->device < 0 || params->device >= ndev) {
+        g_last_error = w + ": device ordinal out of range";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return FFEA_GPU_OK;
+}
+
+bool valid_blob_input(const BlobInput &b) {
+    return b.pos && b.top && b.A && b.B && b.G && b.E && b.rng && b.nt > 0 && b.ne > 0 && b.npin >= 0 && b.nctf >= 0 &&
+           (b.npin == 0 || b.pin) && (b.nctf == 0 || (b.ctf_node && b.ctf_force));
+}
+
+int create_handle(const char *who, const ffea_gpu_params *params, const std::vector<BlobInput> &in,
+                  ffea_gpu_blob **out) {
+    if (int rc = check_params(who, params)) return rc;
+    std::unique_ptr<ffea_gpu_blob> b(new ffea_gpu_blob());
+    b->prm = *params;
+    b->device = params->device;
+    b->mixed = params->precision == FFEA_GPU_MIXED;
+    const int rc = guarded(b.get(), [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        FFEA_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->build(in);
+        return (int)FFEA_GPU_OK;
+    });
+    if (rc != FFEA_GPU_OK) {
+        g_last_error = b->err;
+        return rc;
+    }
+    *out = b.release();
+    return FFEA_GPU_OK;
+}
+
+bool bad_blob(const ffea_gpu_blob *b, int blob) { return !b || blob < 0 || blob >= b->nblob; }
+
+// per-blob inputs from the arrays of ffea_gpu_batch_create / ffea_gpu_check_layout
+int batch_inputs(const char *who, int num_blobs, const int *num_nodes, const int *num_elements,
+                 const double *const *pos, const int *const *top, const double *const *A, const double *const *B,
+                 const double *const *G, const double *const *E, const int *num_pinned, const int *const *pinned,
+                 const int *num_ctf, const int *const *ctf_node, const double *const *ctf_force,
+                 const uint32_t *rng_states, std::vector<BlobInput> *out) {
+    if (num_blobs <= 0 || !num_nodes || !num_elements || !pos || !top || !A || !B || !G || !E || !rng_states) {
+        g_last_error = std::string(who) + ": invalid argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    out->resize(num_blobs);
+    for (int b = 0; b < num_blobs; ++b) {
+        (*out)[b] = BlobInput{num_nodes[b], num_elements[b], pos[b], top[b], A[b], B[b], G[b], E[b],
+                              num_pinned ? num_pinned[b] : 0, pinned ? pinned[b] : nullptr,
+                              num_ctf ? num_ctf[b] : 0, ctf_node ? ctf_node[b] : nullptr,
+                              ctf_force ? ctf_force[b] : nullptr, rng_states + 6 * b};
+        if (!valid_blob_input((*out)[b])) {
+            g_last_error = fmt("%s: invalid argument for blob %d", who, b);
+            return FFEA_GPU_ERR_INVALID;
+        }
+        if (!mrg::h_valid_state((*out)[b].rng)) {
+            g_last_error = fmt("%s: invalid MRG32k3a state for blob %d", who, b);
+            return FFEA_GPU_ERR_INVALID;
+        }
+    }
+    return FFEA_GPU_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1497,9 +1256,9 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out) {
     if (out) *out = nullptr;
-    if (!params || !out || !pos || !top || !A || !B || !G || !E || !rng_state || num_nodes <= 0 ||
-        num_elements <= 0 || num_pinned < 0 || num_ctf < 0 || (num_pinned && !pinned) ||
-        (num_ctf && (!ctf_node || !ctf_force))) {
+    const BlobInput bi{num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
+                       ctf_force, rng_state};
+    if (!params || !out || !valid_blob_input(bi)) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
     }
@@ -1507,45 +1266,59 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
         g_last_error = "ffea_gpu_blob_create: calc_stokes is not supported by this version";
         return FFEA_GPU_ERR_INVALID;
     }
-    if (params->precision != FFEA_GPU_FP64 && params->precision != FFEA_GPU_MIXED) {
-        g_last_error = "ffea_gpu_blob_create: unknown precision";
-        return FFEA_GPU_ERR_INVALID;
-    }
-    if (!(params->dt > 0) || params->max_iterations_cg <= 0 || !(params->epsilon2 >= 0)) {
-        g_last_error = "ffea_gpu_blob_create: dt, epsilon2 and max_iterations_cg must be positive";
-        return FFEA_GPU_ERR_INVALID;
-    }
     if (!mrg::h_valid_state(rng_state)) {
         g_last_error = "ffea_gpu_blob_create: invalid MRG32k3a state";
         return FFEA_GPU_ERR_INVALID;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        g_last_error = "ffea_gpu_blob_create: no HIP device";
-        return FFEA_GPU_ERR_NO_DEVICE;
-    }
-    if (params->device < 0 || params->device >= ndev) {
-        g_last_error = "ffea_gpu_blob_create: device ordinal out of range";
+    return create_handle("ffea_gpu_blob_create", params, {bi}, out);
+}
+
+int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
+                          const int *num_elements, const double *const *pos, const int *const *top,
+                          const double *const *A, const double *const *B, const double *const *G,
+                          const double *const *E, const int *num_pinned, const int *const *pinned,
+                          const int *num_ctf, const int *const *ctf_node, const double *const *ctf_force,
+                          const uint32_t *rng_states, ffea_gpu_blob **out) {
+    if (out) *out = nullptr;
+    if (!params || !out) {
+        g_last_error = "ffea_gpu_batch_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
     }
-    std::unique_ptr<ffea_gpu_blob> b(new ffea_gpu_blob());
-    b->prm = *params;
-    b->nt = num_nodes;
-    b->ne = num_elements;
-    b->device = params->device;
-    b->mixed = params->precision == FFEA_GPU_MIXED;
-    int rc = guarded(b.get(), [&]() {
-        FFEA_HIP_TRY(hipSetDevice(b->device));
-        FFEA_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        b->build(pos, top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node, ctf_force, rng_state);
+    std::vector<BlobInput> in;
+    if (int rc = batch_inputs("ffea_gpu_batch_create", num_blobs, num_nodes, num_elements, pos, top, A, B, G, E,
+                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, rng_states, &in))
+        return rc;
+    return create_handle("ffea_gpu_batch_create", params, in, out);
+}
+
+int ffea_gpu_check_layout(int precision, int num_blobs, const int *num_nodes, const int *num_elements,
+                          const double *const *pos, const int *const *top, const double *const *A,
+                          const double *const *B, const double *const *G, const double *const *E,
+                          const int *num_pinned, const int *const *pinned, const int *num_ctf,
+                          const int *const *ctf_node, const double *const *ctf_force, const uint32_t *rng_states,
+                          ffea_gpu_blob_info *info) {
+    std::vector<BlobInput> in;
+    if (int rc = batch_inputs("ffea_gpu_check_layout", num_blobs, num_nodes, num_elements, pos, top, A, B, G, E,
+                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, rng_states, &in))
+        return rc;
+    return guarded(nullptr, [&]() {
+        const Tables t = make_tables(in, precision == FFEA_GPU_MIXED);
+        validate(t);
+        if (info) {
+            *info = ffea_gpu_blob_info{};
+            for (const auto &m : t.meta) {
+                info->num_nodes += m.nt;
+                info->num_linear_nodes += m.nl;
+                info->num_blocks += m.nblocks;
+            }
+            info->num_elements = t.ne;
+            info->num_block_slots = t.nslots;
+            info->precision = precision;
+            info->num_blobs = t.nblob;
+            info->num_chunks = t.nchunks;
+        }
         return (int)FFEA_GPU_OK;
     });
-    if (rc != FFEA_GPU_OK) {
-        g_last_error = b->err;
-        return rc;
-    }
-    *out = b.release();
-    return FFEA_GPU_OK;
 }
 
 void ffea_gpu_blob_destroy(ffea_gpu_blob *blob) {
@@ -1558,7 +1331,7 @@ int ffea_gpu_blob_update_internal_forces(ffea_gpu_blob *b) {
     if (!b) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        if (b->all_pinned) return (int)FFEA_GPU_OK;
+        if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
         b->launch_element();
         return b->check_status_after_forces();
     });
@@ -1568,9 +1341,11 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
     if (!b) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        int rc = b->solve_and_update();
-        b->collect_timing();
-        return rc;
+        if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
+        b->launch_assemble_gather();
+        b->run_cg_eager(0);
+        b->launch_update();
+        return b->finish_step();
     });
 }
 
@@ -1579,117 +1354,163 @@ int ffea_gpu_blob_step(ffea_gpu_blob *b, int nsteps) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         for (int s = 0; s < nsteps; ++s) {
-            if (b->all_pinned) continue;
-            b->launch_element();
-            int rc = b->solve_and_update();
-            if (rc != FFEA_GPU_OK) {
-                b->collect_timing();
-                return rc;
-            }
+            const int rc = b->step_once();
+            if (rc != FFEA_GPU_OK) return rc;
         }
-        b->collect_timing();
         return (int)FFEA_GPU_OK;
     });
 }
 
-int ffea_gpu_blob_get_positions(ffea_gpu_blob *b, double *out) {
-    if (!b || !out) return FFEA_GPU_ERR_INVALID;
+int ffea_gpu_batch_get_positions(ffea_gpu_blob *b, int blob, double *out) {
+    if (bad_blob(b, blob) || !out) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        std::vector<double> p((size_t)3 * b->nl);
-        FFEA_HIP_TRY(hipMemcpyAsync(p.data(), b->pos.p, sizeof(double) * p.size(), hipMemcpyDeviceToHost, b->stream));
+        const BlobMeta &m = b->meta[blob];
+        std::vector<double> p((size_t)3 * m.nl);
+        FFEA_HIP_TRY(hipMemcpyAsync(p.data(), b->pos.p + 3 * (size_t)m.row_off, sizeof(double) * p.size(),
+                                    hipMemcpyDeviceToHost, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-        for (int n = 0; n < b->nl; ++n)
-            for (int c = 0; c < 3; ++c) out[3 * b->orig_of_lin[n] + c] = p[3 * n + c];
+        for (int n = 0; n < m.nl; ++n)
+            for (int c = 0; c < 3; ++c) out[3 * m.orig_of_lin[n] + c] = p[3 * n + c];
         // tetra_element_linear::linearise_element (tetra_element_linear.cpp:553-577)
-        for (int v = 0; v < b->nt; ++v)
-            if (b->lin_of[v] < 0) {
-                if (b->mid_a[v] < 0) {
+        for (int v = 0; v < m.nt; ++v)
+            if (m.lin_of[v] < 0) {
+                if (m.mid_a[v] < 0) {
                     out[3 * v] = out[3 * v + 1] = out[3 * v + 2] = 0.0;
                     continue;
                 }
-                const double *pa = &p[3 * b->lin_of[b->mid_a[v]]], *pb = &p[3 * b->lin_of[b->mid_b[v]]];
+                const double *pa = &p[3 * m.lin_of[m.mid_a[v]]], *pb = &p[3 * m.lin_of[m.mid_b[v]]];
                 for (int c = 0; c < 3; ++c) out[3 * v + c] = .5 * (pa[c] + pb[c]);
             }
         return (int)FFEA_GPU_OK;
     });
 }
 
-int ffea_gpu_blob_set_positions(ffea_gpu_blob *b, const double *in) {
-    if (!b || !in) return FFEA_GPU_ERR_INVALID;
+int ffea_gpu_batch_set_positions(ffea_gpu_blob *b, int blob, const double *in) {
+    if (bad_blob(b, blob) || !in) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        std::vector<double> p((size_t)3 * b->nl);
-        for (int n = 0; n < b->nl; ++n)
-            for (int c = 0; c < 3; ++c) p[3 * n + c] = in[3 * b->orig_of_lin[n] + c];
-        FFEA_HIP_TRY(hipMemcpyAsync(b->pos.p, p.data(), sizeof(double) * p.size(), hipMemcpyHostToDevice, b->stream));
+        const BlobMeta &m = b->meta[blob];
+        std::vector<double> p((size_t)3 * m.nl);
+        for (int n = 0; n < m.nl; ++n)
+            for (int c = 0; c < 3; ++c) p[3 * n + c] = in[3 * m.orig_of_lin[n] + c];
+        FFEA_HIP_TRY(hipMemcpyAsync(b->pos.p + 3 * (size_t)m.row_off, p.data(), sizeof(double) * p.size(),
+                                    hipMemcpyHostToDevice, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         return (int)FFEA_GPU_OK;
     });
 }
 
-static int get_node_vec(ffea_gpu_blob *b, const double *dev, double *out, bool tiled) {
+static int get_node_vec(ffea_gpu_blob *b, int blob, const double *dev, double *out, bool tiled) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        std::vector<double> p(tiled ? (size_t)b->nslices * 3 * kSlice : (size_t)3 * b->nl);
-        FFEA_HIP_TRY(hipMemcpyAsync(p.data(), dev, sizeof(double) * p.size(), hipMemcpyDeviceToHost, b->stream));
+        const BlobMeta &m = b->meta[blob];
+        const size_t ns = (size_t)(m.nl + kSlice - 1) / kSlice;
+        std::vector<double> p(tiled ? ns * 3 * kSlice : (size_t)3 * m.nl);
+        const double *src = tiled ? dev + vi(m.row_off, 0) : dev + 3 * (size_t)m.row_off;
+        FFEA_HIP_TRY(hipMemcpyAsync(p.data(), src, sizeof(double) * p.size(), hipMemcpyDeviceToHost, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-        std::fill(out, out + (size_t)3 * b->nt, 0.0);
-        for (int n = 0; n < b->nl; ++n)
-            for (int c = 0; c < 3; ++c) out[3 * b->orig_of_lin[n] + c] = tiled ? p[vi(n, c)] : p[3 * n + c];
+        std::fill(out, out + (size_t)3 * m.nt, 0.0);
+        for (int n = 0; n < m.nl; ++n)
+            for (int c = 0; c < 3; ++c) out[3 * m.orig_of_lin[n] + c] = tiled ? p[vi(n, c)] : p[3 * n + c];
         return (int)FFEA_GPU_OK;
     });
 }
 
-int ffea_gpu_blob_get_velocities(ffea_gpu_blob *b, double *out) {
-    if (!b || !out) return FFEA_GPU_ERR_INVALID;
-    return get_node_vec(b, b->vel.p, out, false);
+int ffea_gpu_batch_get_velocities(ffea_gpu_blob *b, int blob, double *out) {
+    if (bad_blob(b, blob) || !out) return FFEA_GPU_ERR_INVALID;
+    return get_node_vec(b, blob, b->vel.p, out, false);
 }
 
-int ffea_gpu_blob_get_rhs(ffea_gpu_blob *b, double *out) {
-    if (!b || !out) return FFEA_GPU_ERR_INVALID;
-    return get_node_vec(b, b->rhs.p, out, true);
+int ffea_gpu_batch_get_rhs(ffea_gpu_blob *b, int blob, double *out) {
+    if (bad_blob(b, blob) || !out) return FFEA_GPU_ERR_INVALID;
+    return get_node_vec(b, blob, b->rhs.p, out, true);
 }
 
-int ffea_gpu_blob_get_element_forces(ffea_gpu_blob *b, double *out) {
-    if (!b || !out) return FFEA_GPU_ERR_INVALID;
+int ffea_gpu_batch_get_element_forces(ffea_gpu_blob *b, int blob, double *out) {
+    if (bad_blob(b, blob) || !out) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
-        const size_t n = (size_t)12 * b->ne;
+        const BlobMeta &m = b->meta[blob];
+        const size_t n = (size_t)12 * m.ne, off = (size_t)12 * m.elem_off;
         if (b->mixed) {
             std::vector<float> f(n);
-            FFEA_HIP_TRY(hipMemcpyAsync(f.data(), b->f_force.p, sizeof(float) * n, hipMemcpyDeviceToHost, b->stream));
+            FFEA_HIP_TRY(hipMemcpyAsync(f.data(), b->f_force.p + off, sizeof(float) * n, hipMemcpyDeviceToHost,
+                                        b->stream));
             FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
             for (size_t i = 0; i < n; ++i) out[i] = f[i];
         } else {
-            FFEA_HIP_TRY(hipMemcpyAsync(out, b->d_force.p, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+            FFEA_HIP_TRY(hipMemcpyAsync(out, b->d_force.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
             FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         }
         return (int)FFEA_GPU_OK;
     });
 }
 
-int ffea_gpu_blob_get_cg_iterations(ffea_gpu_blob *b, int *last, long long *total) {
+int ffea_gpu_batch_get_cg_iterations(ffea_gpu_blob *b, int *last, long long *total) {
     if (!b) return FFEA_GPU_ERR_INVALID;
-    if (last) *last = b->last_iters;
-    if (total) *total = b->total_iters;
+    for (int i = 0; i < b->nblob; ++i) {
+        if (last) last[i] = b->meta[i].last_iters;
+        if (total) total[i] = b->meta[i].total_iters;
+    }
     return FFEA_GPU_OK;
 }
 
-int ffea_gpu_blob_get_rng_state(ffea_gpu_blob *b, uint32_t state[6]) {
-    if (!b || !state) return FFEA_GPU_ERR_INVALID;
-    for (int i = 0; i < 6; ++i) state[i] = b->rng[i];
-    return FFEA_GPU_OK;
+int ffea_gpu_batch_get_rng_state(ffea_gpu_blob *b, int blob, uint32_t state[6]) {
+    if (bad_blob(b, blob) || !state) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        FFEA_HIP_TRY(hipMemcpyAsync(state, b->rng.p + 6 * (size_t)blob, sizeof(uint32_t) * 6, hipMemcpyDeviceToHost,
+                                    b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
 }
 
-int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *b, const uint32_t state[6]) {
-    if (!b || !state) return FFEA_GPU_ERR_INVALID;
+int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *b, int blob, const uint32_t state[6], long long draws_per_step) {
+    if (bad_blob(b, blob) || !state) return FFEA_GPU_ERR_INVALID;
     if (!mrg::h_valid_state(state)) {
         b->err = "invalid MRG32k3a state";
         return FFEA_GPU_ERR_INVALID;
     }
-    for (int i = 0; i < 6; ++i) b->rng[i] = state[i];
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        FFEA_HIP_TRY(hipMemcpyAsync(b->rng.p + 6 * (size_t)blob, state, sizeof(uint32_t) * 6, hipMemcpyHostToDevice,
+                                    b->stream));
+        mrg::Jump j{};
+        if (draws_per_step >= 0) {  // a graph reads the jump from device memory: it stays valid
+            j = b->meta[blob].all_pinned ? mrg::h_identity() : mrg::h_pow((unsigned long long)draws_per_step);
+            FFEA_HIP_TRY(hipMemcpyAsync(b->step_jump.p + blob, &j, sizeof(j), hipMemcpyHostToDevice, b->stream));
+        }
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_get_positions(ffea_gpu_blob *b, double *out) { return ffea_gpu_batch_get_positions(b, 0, out); }
+int ffea_gpu_blob_set_positions(ffea_gpu_blob *b, const double *in) { return ffea_gpu_batch_set_positions(b, 0, in); }
+int ffea_gpu_blob_get_velocities(ffea_gpu_blob *b, double *out) { return ffea_gpu_batch_get_velocities(b, 0, out); }
+int ffea_gpu_blob_get_rhs(ffea_gpu_blob *b, double *out) { return ffea_gpu_batch_get_rhs(b, 0, out); }
+int ffea_gpu_blob_get_element_forces(ffea_gpu_blob *b, double *out) {
+    return ffea_gpu_batch_get_element_forces(b, 0, out);
+}
+
+int ffea_gpu_blob_get_cg_iterations(ffea_gpu_blob *b, int *last, long long *total) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    if (last) *last = b->meta[0].last_iters;
+    if (total) *total = b->meta[0].total_iters;
     return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_get_rng_state(ffea_gpu_blob *b, uint32_t state[6]) { return ffea_gpu_batch_get_rng_state(b, 0, state); }
+
+int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *b, const uint32_t state[6]) {
+    return ffea_gpu_batch_set_rng_stream(b, 0, state, -1);
+}
+
+int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *b, const uint32_t state[6], long long draws_per_step) {
+    if (draws_per_step < 0) return FFEA_GPU_ERR_INVALID;
+    return ffea_gpu_batch_set_rng_stream(b, 0, state, draws_per_step);
 }
 
 int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]) {
@@ -1700,26 +1521,115 @@ int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32
     return FFEA_GPU_OK;
 }
 
-int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *b, const uint32_t state[6], long long draws_per_step) {
-    if (!b || !state || draws_per_step < 0) return FFEA_GPU_ERR_INVALID;
-    if (!mrg::h_valid_state(state)) {
-        b->err = "invalid MRG32k3a state";
-        return FFEA_GPU_ERR_INVALID;
-    }
-    for (int i = 0; i < 6; ++i) b->rng[i] = state[i];
-    b->step_jump = mrg::h_pow((unsigned long long)draws_per_step);
+int ffea_gpu_rng_stream_state(const uint32_t package_seed[6], long long stream, uint32_t out[6]) {
+    if (!package_seed || !out || stream < 0 || !mrg::h_valid_state(package_seed)) return FFEA_GPU_ERR_INVALID;
+    uint32_t t[6];
+    mrg::h_apply(mrg::h_stream_jump((unsigned long long)stream), package_seed, t);
+    for (int i = 0; i < 6; ++i) out[i] = t[i];
     return FFEA_GPU_OK;
 }
 
 int ffea_gpu_blob_get_info(ffea_gpu_blob *b, ffea_gpu_blob_info *info) {
     if (!b || !info) return FFEA_GPU_ERR_INVALID;
-    info->num_nodes = b->nt;
+    *info = ffea_gpu_blob_info{};
+    for (const auto &m : b->meta) {
+        info->num_nodes += m.nt;
+        info->num_linear_nodes += m.nl;
+        info->num_blocks += m.nblocks;
+    }
     info->num_elements = b->ne;
-    info->num_linear_nodes = b->nl;
-    info->num_blocks = b->nblocks;
     info->num_block_slots = b->nslots;
     info->precision = b->prm.precision;
     info->device_bytes = (long long)b->dev_bytes;
+    info->num_blobs = b->nblob;
+    info->num_chunks = b->nchunks;
+    info->row_blocks = b->nb;
+    info->graph_iterations = b->graph_exec ? b->graph_iters : 0;
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_set_row_blocks(ffea_gpu_blob *b, int blocks) {
+    if (!b || blocks <= 0) return FFEA_GPU_ERR_INVALID;
+    b->set_row_blocks(blocks);
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_autotune(ffea_gpu_blob *b, int iterations, int *chosen_blocks) {
+    if (!b || iterations <= 0) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        if (b->nchunks == 0) {
+            if (chosen_blocks) *chosen_blocks = b->nb;
+            return (int)FFEA_GPU_OK;
+        }
+        std::vector<int> cand;
+        for (int c : {b->nchunks, 1024, 768, 512, 384, 256, 128, 64, 32})
+            if (c <= b->nchunks && std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+        // time CG iterations with every blob running (the convergence test never passes)
+        std::vector<CgScalars> run(b->nblob);
+        for (int i = 0; i < b->nblob; ++i) {
+            run[i] = CgScalars{};
+            run[i].delta[0] = run[i].delta[1] = 1.0;
+            run[i].f2 = 1.0;
+            run[i].done = b->meta[i].all_pinned ? 1 : 0;
+        }
+        const ffea_gpu_params keep = b->prm;
+        const unsigned keep_timing = b->timing;
+        b->prm.epsilon2 = -1.0;
+        b->prm.max_iterations_cg = 1 << 30;
+        b->timing = 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        double best = 1e300;
+        int best_blocks = b->nb;
+        try {
+            FFEA_HIP_TRY(hipEventCreate(&e0));
+            FFEA_HIP_TRY(hipEventCreate(&e1));
+            for (int c : cand) {
+                b->set_row_blocks(c);
+                b->S.upload(run, b->stream);
+                for (int it = 1; it <= 2; ++it) b->launch_cg_iteration(it, false);  // warm-up
+                FFEA_HIP_TRY(hipEventRecord(e0, b->stream));
+                for (int it = 3; it < 3 + iterations; ++it) b->launch_cg_iteration(it, false);
+                FFEA_HIP_TRY(hipEventRecord(e1, b->stream));
+                FFEA_HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                FFEA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_blocks = c;
+                }
+            }
+        } catch (...) {
+            b->prm = keep;
+            b->timing = keep_timing;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->prm = keep;
+        b->timing = keep_timing;
+        b->set_row_blocks(best_blocks);
+        // the CG vectors are rebuilt by the next gather; restore the solver scalars
+        std::vector<CgScalars> s0(b->nblob);
+        for (int i = 0; i < b->nblob; ++i) {
+            s0[i] = CgScalars{};
+            s0[i].done = b->meta[i].all_pinned ? 1 : 0;
+        }
+        b->S.upload(s0, b->stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        if (chosen_blocks) *chosen_blocks = b->nb;
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_use_graph(ffea_gpu_blob *b, int enable, int iterations) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    b->use_graph = enable != 0;
+    if (iterations != b->graph_fixed_iters) b->drop_graph();
+    b->graph_fixed_iters = std::max(0, iterations);
+    if (!b->use_graph) b->drop_graph();
     return FFEA_GPU_OK;
 }
 
@@ -1728,10 +1638,11 @@ int ffea_gpu_blob_set_timing(ffea_gpu_blob *b, unsigned phase_mask) {
     return guarded(b, [&]() {
         b->collect_timing();
         b->timing = phase_mask & ((1u << FFEA_GPU_NUM_PHASES) - 1u);
-        for (auto &p : b->phase) {
-            p.ms = 0;
-            p.launches = 0;
-            p.ev.clear();
+        for (auto &ph : b->phase) {
+            ph.ms = 0;
+            ph.launches = 0;
+            ph.ev.clear();
+            ph.iter.clear();
         }
         return (int)FFEA_GPU_OK;
     });
@@ -1763,17 +1674,17 @@ int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_c
     return guarded(nullptr, [&]() {
         FFEA_HIP_TRY(hipSetDevice(device));
         const size_t n2 = (size_t)bytes / 16;
-        double2 *x = nullptr, *y = nullptr;
-        FFEA_HIP_TRY(hipMalloc(&x, n2 * 16));
-        FFEA_HIP_TRY(hipMalloc(&y, n2 * 16));
-        FFEA_HIP_TRY(hipMemset(x, 0, n2 * 16));
+        double2 *xx = nullptr, *yy = nullptr;
+        FFEA_HIP_TRY(hipMalloc(&xx, n2 * 16));
+        FFEA_HIP_TRY(hipMalloc(&yy, n2 * 16));
+        FFEA_HIP_TRY(hipMemset(xx, 0, n2 * 16));
         hipEvent_t e0, e1;
         FFEA_HIP_TRY(hipEventCreate(&e0));
         FFEA_HIP_TRY(hipEventCreate(&e1));
         const int grid = 256 * 32;
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, x, y, n2);  // warm-up
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, xx, yy, n2);  // warm-up
         FFEA_HIP_TRY(hipEventRecord(e0, 0));
-        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, x, y, n2);
+        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, xx, yy, n2);
         FFEA_HIP_TRY(hipEventRecord(e1, 0));
         FFEA_HIP_TRY(hipEventSynchronize(e1));
         float ms = 0;
@@ -1781,8 +1692,8 @@ int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_c
         *ms_per_copy = ms / iters;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        (void)hipFree(x);
-        (void)hipFree(y);
+        (void)hipFree(xx);
+        (void)hipFree(yy);
         return (int)FFEA_GPU_OK;
     });
 }

@@ -1,0 +1,840 @@
+// Device code of the blob time step (MI355X, gfx950). Included by ffea_blob.hip.
+//
+// A handle holds one or more independent blobs. Their corner nodes, elements and
+// matrix slices are concatenated; every blob's rows start on a 64-row slice boundary.
+// Rows are cut into chunks of up to kChunkRows rows inside each blob; every chunk
+// contributes one partial sum per reduced quantity, and the kernel that needs a
+// blob's total adds that blob's chunk partials in chunk order. Totals are therefore
+// independent of the grid size and of which block processed which chunk, and the
+// grid of the row kernels is a free tuning parameter.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "mrg32k3a.h"
+
+namespace ffk {
+
+constexpr int kBlock = 256;          // threads per workgroup: 4 wavefronts of 64
+constexpr int kSlice = 64;           // rows per ELL slice (one wavefront)
+constexpr int kChunkRows = 1024;     // rows per reduction chunk (4 per thread)
+constexpr uint32_t kIdentity = 0xFFFFFFFFu;
+constexpr int kMaxUnrolledRow = 16;  // rows with up to this many blocks take the unrolled SpMV path
+
+// CG node vectors are tiled by 64 nodes: x[64] y[64] z[64] per tile, so row-wise
+// loads and stores of one component are 512 contiguous bytes and stencil gathers of
+// neighbouring rows stay coalesced
+__host__ __device__ __forceinline__ size_t vi(int n, int k) {
+    return ((size_t)(n >> 6) * 3 + (size_t)k) * 64 + (size_t)(n & 63);
+}
+
+// per-blob CG state (device memory)
+struct CgScalars {
+    double delta[2];   // delta[(i-1)&1]: r.z entering iteration i (written by the SpMV step)
+    double f2;         // |f|^2 of the right-hand side
+    int iter;          // iterations done when the solve stopped
+    int done;          // 1: stop (converged, failed or skipped)
+    int converged;     // 1: converged this solve
+    int inverted;      // inverted elements counted by the element pass (relaxed atomics)
+    int inverted_step; // inverted elements of the step being solved
+    int updated;       // positions of this solve already applied
+};
+
+struct Chunk {
+    int blob, begin, end, first;  // rows [begin, end) of blob; first = first chunk of the blob
+};
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block sum of NV values; result valid in every thread
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double sh[kBlock / 64][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh[wid][k] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) s += sh[w][k];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// chunk partials, stored [k][nchunks]
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int nchunks, int c) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) P[(size_t)k * nchunks + c] = v[k];
+}
+
+// totals of a blob's chunk partials [c0, c1): identical in every block
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *P, int nchunks, int c0, int c1, double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        for (int c = c0 + threadIdx.x; c < c1; c += kBlock) acc += P[(size_t)k * nchunks + c];
+        out[k] = acc;
+    }
+    block_sum<NV>(out);
+}
+
+// The chunks of this block: a contiguous range; blocks are mapped so that each of the
+// 8 XCDs (which receive workgroups round-robin) owns one contiguous eighth of them.
+struct ChunkRange {
+    int c0, c1;
+};
+__device__ __forceinline__ ChunkRange my_chunks(int nchunks, int cpb) {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int L = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int c0 = min(nchunks, L * cpb);
+    return {c0, min(nchunks, c0 + cpb)};
+}
+
+// rigid-mode basis of a corner node at x (relative to a fixed centre): columns are
+// the 3 translations and the 3 infinitesimal rotations e_k cross x. Z^T v = (v, x cross v).
+__device__ __forceinline__ void zt_times(const double *x, const double *v, double *out6) {
+    out6[0] = v[0];
+    out6[1] = v[1];
+    out6[2] = v[2];
+    out6[3] = x[1] * v[2] - x[2] * v[1];
+    out6[4] = x[2] * v[0] - x[0] * v[2];
+    out6[5] = x[0] * v[1] - x[1] * v[0];
+}
+// Z c = c_t + c_r cross x
+__device__ __forceinline__ void z_times(const double *x, const double *c, double *out3) {
+    out3[0] = c[0] + (c[4] * x[2] - c[5] * x[1]);
+    out3[1] = c[1] + (c[5] * x[0] - c[3] * x[2]);
+    out3[2] = c[2] + (c[3] * x[1] - c[4] * x[0]);
+}
+
+// 6x6 inverse by Gauss-Jordan with partial pivoting (FP64)
+__device__ inline void invert6(const double *M, double *Minv) {
+    double a[6][12];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 12; ++j) a[i][j] = j < 6 ? M[6 * i + j] : (j - 6 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 6; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 6; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < 12; ++j) {
+                const double t = a[c][j];
+                a[c][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+        const double d = 1.0 / a[c][c];
+        for (int j = 0; j < 12; ++j) a[c][j] *= d;
+        for (int r = 0; r < 6; ++r)
+            if (r != c) {
+                const double f = a[r][c];
+                for (int j = 0; j < 12; ++j) a[r][j] -= f * a[c][j];
+            }
+    }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) Minv[6 * i + j] = a[i][j + 6];
+}
+
+// Z^T Z from the moments N, sum x, sum x_i x_j (m[0], m[1..3], m[4..9]: xx yy zz xy xz yz)
+__device__ inline void ztz_from_moments(const double *m, double *M) {
+    for (int i = 0; i < 36; ++i) M[i] = 0.0;
+    const double N = m[0], sx = m[1], sy = m[2], sz = m[3];
+    const double xx = m[4], yy = m[5], zz = m[6], xy = m[7], xz = m[8], yz = m[9];
+    M[0] = M[7] = M[14] = N;
+    const double Tr[3][3] = {{0, sz, -sy}, {-sz, 0, sx}, {sy, -sx, 0}};  // sum_n (e_k x x_n)_i as [i][k]
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            M[6 * i + 3 + k] = Tr[i][k];
+            M[6 * (3 + k) + i] = Tr[i][k];
+        }
+    const double r2 = xx + yy + zz;
+    const double R[3][3] = {{r2 - xx, -xy, -xz}, {-xy, r2 - yy, -yz}, {-xz, -yz, r2 - zz}};
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) M[6 * (3 + k) + 3 + l] = R[k][l];
+}
+
+// ---------------------------------------------------------------- element pass
+template <typename T>
+struct ElemArgs {
+    int ne;                       // elements of all blobs
+    const int *conn;              // [4][ne] corner-node indices
+    const double *pos;            // [nl][3]
+    const T *jinv0;               // [9][ne]
+    const T *vol0, *G, *E, *A, *B;
+    double *last_det;             // [ne]
+    T *force;                     // [ne][12]
+    T *geo;                       // [10][ne]: dpsi 1,2,3,5,6,7,9,10,11 then vol
+    int calc_noise;
+    double kT, dt;
+    const int *elem_blob;         // [ne] blob of each element (null: one blob)
+    const int *elem_off;          // [nblob] first element of each blob
+    const mrg::Jump *jump_group;  // [ceil(max blob ne / 64)]: A^(7*64*g)
+    const mrg::Jump *jump_lane;   // [64]: A^(7*l)
+    const uint32_t *rng;          // [nblob][6]: stream state at the start of this pass
+    CgScalars *S;                 // [nblob]
+};
+
+// tetra_element_linear.cpp:191-443 and :468-483 for one element (mass-less solver:
+// du starts at zero). Arithmetic in the reference's order, no contraction.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
+    const int ne = a.ne;
+    const int blob = a.elem_blob ? a.elem_blob[e] : 0;
+    uint32_t st[6];
+    if (a.calc_noise) {
+        // the element's 7 draws start 7*e_local draws into the pass's segment of its blob's stream
+        const int el = e - a.elem_off[blob];
+        for (int i = 0; i < 6; ++i) st[i] = a.rng[6 * blob + i];
+        mrg::d_apply(a.jump_group[el >> 6], st);
+        mrg::d_apply(a.jump_lane[el & 63], st);
+    }
+    int n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
+    double x0[3], J64[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J64[r][c] = a.pos[3 * n[r + 1] + c] - x0[c];
+    T J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[r][c] = (T)J64[r][c];
+
+    // shape function derivatives and volume (tetra_element_linear.cpp:222-292)
+    T d[12];
+    d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    T det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
+    const double ld = a.last_det[e];
+    T vol;
+    if (ld * (double)det < 0) {
+        __hip_atomic_fetch_add(&a.S[blob].inverted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vol = a.geo[(size_t)9 * ne + e];  // the reference keeps the previous volume
+    } else {
+        a.last_det[e] = (double)det;
+        vol = (T)(1.0 / 6.0) * (T)fabs(det);
+        const T rdet = (T)1.0 / det;
+        d[1] *= rdet; d[2] *= rdet; d[3] *= rdet;
+        d[5] *= rdet; d[6] *= rdet; d[7] *= rdet;
+        d[9] *= rdet; d[10] *= rdet; d[11] *= rdet;
+    }
+    d[0] = -(d[1] + d[2] + d[3]);
+    d[4] = -(d[5] + d[6] + d[7]);
+    d[8] = -(d[9] + d[10] + d[11]);
+
+    // shear elastic stress (tetra_element_linear.cpp:375-392)
+    T Ji[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ji[k / 3][k % 3] = a.jinv0[(size_t)k * ne + e];
+    T F[3][3], s[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T acc = (T)0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += J[k][i] * Ji[j][k];
+            F[i][j] = acc;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T acc = (T)0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc += F[i][k] * F[j][k];
+            s[i][j] = acc;
+        }
+    const T G = a.G[e], E = a.E[e], v0 = a.vol0[e];
+    const T sc = G * v0 / vol;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s[i][j] *= sc;
+    s[0][0] -= G;
+    s[1][1] -= G;
+    s[2][2] -= G;
+    // bulk elastic stress (:397-403)
+    {
+        const T c2 = E - G * (T)2.0 / (T)3.0;
+        const T c = G * ((T)1.0 - (v0 / vol)) + (T)0.5 * c2 * ((vol / v0) - (v0 / vol));
+        s[0][0] += c;
+        s[1][1] += c;
+        s[2][2] += c;
+    }
+    // fluctuating stress (:412-431), draws bf, s00, s11, s22, s01, s02, s12
+    if (a.calc_noise) {
+        const T A = a.A[e], B = a.B[e];
+        double u[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) u[k] = mrg::d_u01(st);
+        T r[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) r[k] = (T)((-.5) + ((.5) - (-.5)) * u[k]);
+        const T c = sqrt(((T)24 * (T)a.kT) / (vol * (T)a.dt));
+        const T bf = sqrt(B) * r[0];
+        s[0][0] += c * (sqrt((T)2 * A) * r[1] + bf);
+        s[1][1] += c * (sqrt((T)2 * A) * r[2] + bf);
+        s[2][2] += c * (sqrt((T)2 * A) * r[3] + bf);
+        s[0][1] += c * sqrt(A) * r[4];
+        s[0][2] += c * sqrt(A) * r[5];
+        s[1][2] += c * sqrt(A) * r[6];
+        s[1][0] = s[0][1];
+        s[2][0] = s[0][2];
+        s[2][1] = s[1][2];
+    }
+    // apply_stress_tensor (:436-443) then node_force -= du (:468-483)
+    T out[12];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T du = (T)0 + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
+            out[3 * q + i] = (T)0 - du;
+        }
+    T *fo = a.force + (size_t)12 * e;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) fo[k] = out[k];
+    const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.geo[(size_t)k * ne + e] = d[gi[k]];
+    a.geo[(size_t)9 * ne + e] = vol;
+}
+
+// the pass consumed 7 draws per element of each blob (tetra_element_linear.cpp:416-426)
+__global__ void k_rng_advance(int nblob, uint32_t *rng, const mrg::Jump *step_jump) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblob) return;
+    uint32_t s[6];
+    for (int i = 0; i < 6; ++i) s[i] = rng[6 * b + i];
+    mrg::d_apply(step_jump[b], s);
+    for (int i = 0; i < 6; ++i) rng[6 * b + i] = s[i];
+}
+
+// ---------------------------------------------------------------- matrix assembly
+template <typename T>
+struct AsmArgs {
+    int nrows, ne;
+    const int *slice_w;          // [nslices]
+    const long long *slice_base; // [nslices] first slot of the slice
+    const uint32_t *cptr;        // [nslots+1]
+    const uint32_t *contrib;     // (e<<4)|(a<<2)|b, element order; kIdentity = identity block
+    const unsigned char *diag_j; // [nrows] slot index j of the diagonal block
+    const T *geo;                // [10][ne]
+    const T *A, *B;
+    T *val;                      // [nslots*9]
+    double *dinv;                // tiled [nrows][3]
+};
+
+// Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
+// reference's create_viscosity_matrix does (tetra_element_linear.cpp:298-327, macros
+// tetra_element_linear.h:56-117, del2 :585-622): upper triangle computed, the lower
+// one mirrored, everything times vol at the end.
+template <typename T>
+__device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
+#pragma clang fp contract(off)
+    int I = 4 * i, Jc = 4 * j, p = a, q = b;
+    if (I > Jc || (I == Jc && p > q)) {  // mirror to the upper triangle
+        int t = I; I = Jc; Jc = t;
+        t = p; p = q; q = t;
+    }
+    T v;
+    if (I == Jc) {
+        const T del2 = (d[p] * d[q] + d[4 + p] * d[4 + q] + d[8 + p] * d[8 + q]);
+        v = d[I + q] * d[I + p] * (A + B);
+        v += del2 * A;
+    } else {
+        if (p == q) v = (B + A) * (d[I + p] * d[Jc + p]);
+        else v = B * (d[I + p] * d[Jc + q]) + A * (d[I + q] * d[Jc + p]);
+    }
+    return v * vol;
+}
+
+// SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
+// 305-312): every block entry is the sum of its element sources in element order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
+#pragma clang fp contract(off)
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= a.nrows) return;
+    const int sl = n / kSlice, lane = n % kSlice;
+    const int W = a.slice_w[sl];
+    const long long base = a.slice_base[sl];
+    const int ne = a.ne;
+    for (int j = 0; j < W; ++j) {
+        const long long slot = base + (long long)j * kSlice + lane;
+        const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
+        double acc[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t code = a.contrib[c];
+            if (code == kIdentity) {
+                acc[0] += 1.0; acc[4] += 1.0; acc[8] += 1.0;
+                continue;
+            }
+            const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
+            T d[12];
+            d[1] = a.geo[(size_t)0 * ne + e]; d[2] = a.geo[(size_t)1 * ne + e]; d[3] = a.geo[(size_t)2 * ne + e];
+            d[5] = a.geo[(size_t)3 * ne + e]; d[6] = a.geo[(size_t)4 * ne + e]; d[7] = a.geo[(size_t)5 * ne + e];
+            d[9] = a.geo[(size_t)6 * ne + e]; d[10] = a.geo[(size_t)7 * ne + e]; d[11] = a.geo[(size_t)8 * ne + e];
+            const T vol = a.geo[(size_t)9 * ne + e];
+            d[0] = -(d[1] + d[2] + d[3]);
+            d[4] = -(d[5] + d[6] + d[7]);
+            d[8] = -(d[9] + d[10] + d[11]);
+            const T A = a.A[e], B = a.B[e];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int jj = 0; jj < 3; ++jj) acc[3 * i + jj] += (double)visc_entry<T>(d, A, B, vol, i, pa, jj, pb);
+        }
+        T *v = a.val + (size_t)base * 9 + (size_t)j * 9 * kSlice + lane;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[(size_t)k * kSlice] = (T)acc[k];
+        if (j == a.diag_j[n]) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- shared row-kernel arguments
+struct RowArgs {
+    int nchunks, cpb;
+    const Chunk *chunks;
+    const int2 *seg;       // [nblob]: chunk range of each blob
+    CgScalars *S;          // [nblob]
+    int *num_done;         // blobs whose solve stopped
+};
+
+// ---------------------------------------------------------------- gather + CG start
+template <typename T>
+struct GatherArgs {
+    RowArgs ra;
+    const uint32_t *inc_ptr;    // [nrows+1]
+    const uint32_t *inc;        // e*4 + slot, element order
+    const T *force;             // [ne][12]
+    const double *ctf;          // [nrows][3] or null: constant forces summed in list order
+    const uint32_t *fold_ptr;   // [nrows+1] or null
+    const double *fold;         // [..][3]: linearise_force increments in element order
+    const unsigned char *pinned;
+    const double *dinv;
+    const double *pos;          // [nrows][3] (rigid modes)
+    const double3 *centre;      // [nblob]
+    double *rhs, *r, *z, *x;
+    double *P;                  // chunk partials
+};
+
+// Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
+// start of NoMassCGSolver::solve (NoMassCGSolver.cpp:196-220). With PROJ (free blob,
+// mixed form) the CG start is left to k_cg_start and the partials are |f|^2, Z^T f
+// and the moments of Z^T Z.
+template <typename T, bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
+#pragma clang fp contract(off)
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.ra.num_done = 0;
+    constexpr int NV = PROJ ? 17 : 2;
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        double v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = 0.0;
+        const double3 cen = a.centre[ch.blob];
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            double f[3] = {0.0, 0.0, 0.0};
+            if (a.ctf) {
+                f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
+            }
+            for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+                const uint32_t es = a.inc[k];
+                const T *src = a.force + (size_t)12 * (es >> 2) + 3 * (es & 3);
+                f[0] += (double)src[0];
+                f[1] += (double)src[1];
+                f[2] += (double)src[2];
+            }
+            if (a.fold_ptr)
+                for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
+                    f[0] += a.fold[3 * k];
+                    f[1] += a.fold[3 * k + 1];
+                    f[2] += a.fold[3 * k + 2];
+                }
+            if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a.rhs[vi(n, k)] = f[k];
+            if constexpr (!PROJ) {
+                double zz[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    a.r[vi(n, k)] = f[k];
+                    a.x[vi(n, k)] = 0.0;
+                    zz[k] = a.dinv[vi(n, k)] * f[k];
+                    a.z[vi(n, k)] = zz[k];
+                }
+                v[0] += f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
+                v[1] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+            } else {
+                const double x[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+                double zf[6];
+                zt_times(x, f, zf);
+                v[0] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) v[1 + k] += zf[k];
+                v[7] += 1.0;
+                v[8] += x[0]; v[9] += x[1]; v[10] += x[2];
+                v[11] += x[0] * x[0]; v[12] += x[1] * x[1]; v[13] += x[2] * x[2];
+                v[14] += x[0] * x[1]; v[15] += x[0] * x[2]; v[16] += x[1] * x[2];
+            }
+        }
+        store_partials<NV>(v, a.P, a.ra.nchunks, c);
+        if (c == ch.first && threadIdx.x == 0) {
+            CgScalars *S = a.ra.S + ch.blob;
+            S->inverted_step = S->inverted;
+            S->done = 0;
+            S->converged = 0;
+            S->iter = 0;
+            S->updated = 0;
+        }
+    }
+}
+
+// Rigid-mode projection of the right-hand side for a free blob in the mixed form:
+// r = f - Z (Z^T Z)^-1 Z^T f, then z = D^-1 r, x = 0, partials r.z.
+struct StartArgs {
+    RowArgs ra;
+    const double *Pg, *rhs, *pos, *dinv;
+    const double3 *centre;
+    const int *project;         // [nblob]: 1 = free blob, project; 0 = leave r = f
+    double *r, *z, *x, *P, *G;  // G: [nblob][36] ((Z^T Z)^-1, zero for unprojected blobs)
+};
+__global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    int cur = -1;
+    double Gi[36], cf[6], f2 = 0.0;
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        if (ch.blob != cur) {
+            cur = ch.blob;
+            const int2 sg = a.ra.seg[cur];
+            double t[17];
+            sum_partials<17>(a.Pg, a.ra.nchunks, sg.x, sg.y, t);
+            double M[36];
+            const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
+            if (a.project[cur]) {
+                ztz_from_moments(mom, M);
+                invert6(M, Gi);
+            } else {
+                for (int i = 0; i < 36; ++i) Gi[i] = 0.0;
+            }
+            for (int i = 0; i < 6; ++i) {
+                double acc = 0.0;
+                for (int j = 0; j < 6; ++j) acc += Gi[6 * i + j] * t[1 + j];
+                cf[i] = acc;
+            }
+            f2 = t[0];
+        }
+        const double3 cen = a.centre[cur];
+        double v[1] = {0.0};
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            const double xx[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+            double zc[3], rl[3], zl[3];
+            z_times(xx, cf, zc);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                rl[k] = a.rhs[vi(n, k)] - zc[k];
+                a.r[vi(n, k)] = rl[k];
+                a.x[vi(n, k)] = 0.0;
+                zl[k] = a.dinv[vi(n, k)] * rl[k];
+                a.z[vi(n, k)] = zl[k];
+            }
+            v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
+        }
+        store_partials<1>(v, a.P, a.ra.nchunks, c);
+        if (c == ch.first && threadIdx.x == 0) {
+            a.ra.S[cur].f2 = f2;
+            for (int i = 0; i < 36; ++i) a.G[36 * cur + i] = Gi[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CG iteration
+template <typename T>
+struct SpmvArgs {
+    RowArgs ra;
+    int iter, max_iter;
+    double epsilon2;
+    const int *slice_w;
+    const long long *slice_base;
+    const int *col;        // [nslots]
+    const T *val;          // [nslots*9]
+    const double *z;
+    const double *pos;
+    const double3 *centre;
+    double *p, *q;         // updated in place
+    const double *Pin;     // chunk partials of the previous kernel (start: r.z [and |f|^2]; later: r.r, r.z)
+    double *P1;            // chunk partials of this kernel
+};
+
+// Iteration `iter` of every blob still running: first the convergence test of the
+// blob's previous iteration (|r|^2/|f|^2 < epsilon^2, NoMassCGSolver.cpp:223-237), then
+// p = beta p + z (mat_vec_fns.cpp:202-212) and q = V p, formed as q = V z + beta q so
+// the row gathers only z (SparseMatrixFixedPattern.cpp:148-187), and the p.q partials
+// (NoMassCGSolver.cpp:251-265).
+template <typename T, bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    const int i = a.iter;
+    const bool first = (i == 0);
+    int cur = -1;
+    bool skip = false;
+    double beta = 0.0;
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        if (ch.blob != cur) {
+            cur = ch.blob;
+            CgScalars *S = a.ra.S + cur;
+            const int2 sg = a.ra.seg[cur];
+            const bool lead = (c == ch.first) && threadIdx.x == 0;
+            skip = S->done != 0;
+            if (!skip && first) {
+                if (S->inverted_step > 0) {  // an inverted mesh is not solved
+                    if (lead) {
+                        S->done = 1;
+                        __hip_atomic_fetch_add(a.ra.num_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    skip = true;
+                } else if constexpr (PROJ) {
+                    double t[1];
+                    sum_partials<1>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
+                    if (lead) S->delta[1] = t[0];
+                } else {
+                    double t[2];
+                    sum_partials<2>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
+                    if (lead) {
+                        S->delta[1] = t[0];
+                        S->f2 = t[1];
+                    }
+                }
+            } else if (!skip) {
+                double t[2];
+                sum_partials<2>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);  // r.r, r.z of iteration i-1
+                const double f2 = S->f2;
+                const double res = (f2 == 0.0) ? 0.0 : t[0] / f2;
+                if (res < a.epsilon2 || i >= a.max_iter) {
+                    if (lead) {
+                        S->converged = res < a.epsilon2 ? 1 : 0;
+                        S->iter = i;
+                        S->done = 1;
+                        __hip_atomic_fetch_add(a.ra.num_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    skip = true;
+                } else {
+                    beta = t[1] / S->delta[(i - 2) & 1];
+                    if (lead) S->delta[(i - 1) & 1] = t[1];
+                }
+            }
+        }
+        if (skip) continue;
+        const double3 cen = a.centre[cur];
+        constexpr int NV = PROJ ? 13 : 1;
+        double v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = 0.0;
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            const int sl = n / kSlice, lane = n % kSlice;
+            const int W = a.slice_w[sl];
+            const long long base = a.slice_base[sl];
+            const int *colp = a.col + base + lane;
+            const T *vp = a.val + (size_t)base * 9 + lane;
+            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+            auto block = [&](int j, int cc) {
+                const T *vv = vp + (size_t)j * 9 * kSlice;
+                const double z0 = a.z[vi(cc, 0)], z1 = a.z[vi(cc, 1)], z2 = a.z[vi(cc, 2)];
+                acc0 += (double)vv[0 * kSlice] * z0; acc0 += (double)vv[1 * kSlice] * z1; acc0 += (double)vv[2 * kSlice] * z2;
+                acc1 += (double)vv[3 * kSlice] * z0; acc1 += (double)vv[4 * kSlice] * z1; acc1 += (double)vv[5 * kSlice] * z2;
+                acc2 += (double)vv[6 * kSlice] * z0; acc2 += (double)vv[7 * kSlice] * z1; acc2 += (double)vv[8 * kSlice] * z2;
+            };
+            if (W <= kMaxUnrolledRow) {
+                // all block columns of the row first, so the gathers and value loads of every
+                // block can be in flight together
+                int cc[kMaxUnrolledRow];
+#pragma unroll
+                for (int j = 0; j < kMaxUnrolledRow; ++j)
+                    if (j < W) cc[j] = colp[(size_t)j * kSlice];
+#pragma unroll
+                for (int j = 0; j < kMaxUnrolledRow; ++j)
+                    if (j < W) block(j, cc[j]);
+            } else {
+#pragma unroll 2
+                for (int j = 0; j < W; ++j) block(j, colp[(size_t)j * kSlice]);
+            }
+            double pn[3], qn[3] = {acc0, acc1, acc2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double zk = a.z[vi(n, k)];
+                if (first) {
+                    pn[k] = zk;
+                } else {
+                    pn[k] = beta * a.p[vi(n, k)] + zk;
+                    qn[k] = beta * a.q[vi(n, k)] + qn[k];
+                }
+                a.p[vi(n, k)] = pn[k];
+                a.q[vi(n, k)] = qn[k];
+            }
+            v[0] += pn[0] * qn[0] + pn[1] * qn[1] + pn[2] * qn[2];
+            if constexpr (PROJ) {
+                const double x[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+                double zq[6], zp[6];
+                zt_times(x, qn, zq);
+                zt_times(x, pn, zp);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    v[1 + k] += zq[k];
+                    v[7 + k] += zp[k];
+                }
+            }
+        }
+        store_partials<NV>(v, a.P1, a.ra.nchunks, c);
+    }
+}
+
+struct VecArgs {
+    RowArgs ra;
+    int iter;
+    const double *p, *q, *dinv;
+    const double *pos;
+    const double3 *centre;
+    const double *G;   // [nblob][36]
+    double *x, *r, *z;
+    const double *P1;  // chunk partials of the SpMV step
+    double *P2;        // r.r, r.z partials of this step
+};
+
+// alpha = delta / p.q, x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197),
+// z = D^-1 r (NoMassCGSolver.cpp:268-281), partials of r.r and r.z.
+// With PROJ, q is first projected onto the range of the operator:
+// q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
+template <bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    constexpr int NV = PROJ ? 13 : 1;
+    int cur = -1;
+    bool skip = false;
+    double alpha = 0.0;
+    double cq[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        if (ch.blob != cur) {
+            cur = ch.blob;
+            const CgScalars *S = a.ra.S + cur;
+            skip = S->done != 0;
+            if (!skip) {
+                const int2 sg = a.ra.seg[cur];
+                double t[NV];
+                sum_partials<NV>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
+                double pq = t[0];
+                if constexpr (PROJ) {
+                    const double *G = a.G + 36 * cur;
+                    for (int k = 0; k < 6; ++k) {
+                        double acc = 0.0;
+                        for (int j = 0; j < 6; ++j) acc += G[6 * k + j] * t[1 + j];
+                        cq[k] = acc;
+                    }
+                    for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
+                }
+                alpha = S->delta[(a.iter - 1) & 1] / pq;
+            }
+        }
+        if (skip) continue;
+        const double3 cen = a.centre[cur];
+        double v[2] = {0.0, 0.0};
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            double qn[3] = {a.q[vi(n, 0)], a.q[vi(n, 1)], a.q[vi(n, 2)]};
+            if constexpr (PROJ) {
+                const double x[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+                double zc[3];
+                z_times(x, cq, zc);
+                qn[0] -= zc[0];
+                qn[1] -= zc[1];
+                qn[2] -= zc[2];
+            }
+            double r3[3], z3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a.x[vi(n, k)] += alpha * a.p[vi(n, k)];
+                r3[k] = a.r[vi(n, k)] + (-alpha) * qn[k];
+                a.r[vi(n, k)] = r3[k];
+                z3[k] = a.dinv[vi(n, k)] * r3[k];
+                a.z[vi(n, k)] = z3[k];
+            }
+            v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
+            v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
+        }
+        store_partials<2>(v, a.P2, a.ra.nchunks, c);
+    }
+}
+
+// Blob::euler_integrate (Blob.cpp:3249-3263), mass-less branch: vel = x, pos += x dt,
+// for every blob whose solve converged and is not yet applied
+__global__ __launch_bounds__(kBlock) void k_update(RowArgs ra, const double *x, double *vel, double *pos,
+                                                   double dt) {
+    const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = ra.chunks[c];
+        const CgScalars &S = ra.S[ch.blob];
+        if (!S.converged || S.updated) continue;
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double v = x[vi(n, k)];
+                vel[3 * n + k] = v;
+                pos[3 * n + k] += v * dt;
+            }
+    }
+}
+
+// after k_update: mark applied solves and clear the inversion counters of the pass
+__global__ void k_mark_updated(int nblob, CgScalars *S) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblob) return;
+    if (S[b].converged) S[b].updated = 1;
+    S[b].inverted = 0;
+}
+
+// ---------------------------------------------------------------- copy probe
+// y = x over n2 16-byte vectors, grid-stride: reads n2*16 and writes n2*16 bytes
+__global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += stride) y[i] = x[i];
+}
+
+}  // namespace ffk

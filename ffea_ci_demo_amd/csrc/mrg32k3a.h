@@ -78,6 +78,25 @@ inline Jump h_pow(unsigned long long n) {
     return R;
 }
 
+// W^n for a jump W
+inline Jump h_pow_of(Jump W, unsigned long long n) {
+    Jump R = h_identity();
+    while (n) {
+        if (n & 1ull) R = h_mul(W, R);
+        W = h_mul(W, W);
+        n >>= 1;
+    }
+    return R;
+}
+
+// A^(n * 2^127): the jump from stream k to stream k+n of one package seed (each new
+// RngStream starts 2^127 draws after the previous one, RngStream.cpp:327-345)
+inline Jump h_stream_jump(unsigned long long n) {
+    Jump W = h_pow(1);
+    for (int i = 0; i < 127; ++i) W = h_mul(W, W);
+    return h_pow_of(W, n);
+}
+
 inline void h_apply(const Jump &J, const uint32_t *s, uint32_t *out) {
     uint32_t t[6];
     for (int c = 0; c < 2; ++c) {

@@ -123,3 +123,31 @@ def config4(n: int = 40, **kw) -> BlobSpec:
     bulk 1e10 Pa, viscosities 1e-3 Pa s, noise on."""
     mat = (1.5e3, 1e-3, 1e-3, 1e7, 1e10, 1.0)
     return cube_blob(n, material=mat, flatten_frac=0.02, name=f"cube{n}_illcond", **kw)
+
+
+def ensemble(num_blobs: int = 512, *, nmin: int = 8, nmax: int = 40, moduli=(1e7, 1e9), seed: int = 2024,
+             package_seed=(12345, 12345, 12345, 12345, 12345, 12345), **kw) -> list:
+    """Ensemble of independent jittered-cube blobs: blob b has n_b^3 cubes with n_b drawn
+    log-uniform in [nmin, nmax], its shear and bulk moduli drawn log-uniform in `moduli`
+    (Pa), its own mesh jitter seed, and noise stream b of `package_seed` (the b-th
+    RngStream after SetPackageSeed, streams 2^127 draws apart; RngStream.cpp:327-345)."""
+    from . import _native  # host-only jump-ahead of the library
+
+    r = np.random.default_rng(seed)
+    lo, hi = np.log(nmin), np.log(nmax + 1)
+    ns = np.minimum(np.floor(np.exp(r.uniform(lo, hi, num_blobs))).astype(int), nmax)
+    Gs = np.exp(r.uniform(np.log(moduli[0]), np.log(moduli[1]), num_blobs))
+    Es = np.exp(r.uniform(np.log(moduli[0]), np.log(moduli[1]), num_blobs))
+    specs = []
+    for b in range(num_blobs):
+        mat = (DEFAULT_MATERIAL[0], DEFAULT_MATERIAL[1], DEFAULT_MATERIAL[2], Gs[b], Es[b], 1.0)
+        spec = cube_blob(int(ns[b]), mesh_seed=seed * 1000 + b, material=mat, name=f"ens{b}_cube{ns[b]}", **kw)
+        spec.rng_state = _native.rng_stream_state(package_seed, b)
+        specs.append(spec)
+    return specs
+
+
+def config2(num_blobs: int = 512, **kw) -> list:
+    """Batched ensemble: 512 blobs of n^3 jittered cubes, n log-uniform in [8, 40],
+    moduli log-uniform in [1e7, 1e9] Pa, one noise stream per blob, noise on."""
+    return ensemble(num_blobs, **kw)

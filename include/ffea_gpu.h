@@ -1,9 +1,10 @@
 /*
  * ffea_gpu.h — C ABI of the MI355X blob time-step library (libffea_gpu.so).
  *
- * One handle is one FFEA blob on one GPU: its mesh, material, rest state, the
- * mass-less viscosity solver and its MRG32k3a noise stream live in device
- * memory. The entry points replace, for that blob, the reference's
+ * A handle is one FFEA blob on one GPU (ffea_gpu_blob_create), or an ensemble of
+ * independent blobs stepped together (ffea_gpu_batch_create): meshes, materials,
+ * rest states, the mass-less viscosity solver and the MRG32k3a noise streams live
+ * in device memory. The entry points replace, for each blob, the reference's
  *
  *   Blob::init + NoMassCGSolver::init   Blob.cpp:161-247, NoMassCGSolver.cpp:56-153
  *   Blob::update_internal_forces        Blob.cpp:400-491 (Blob.h:130)
@@ -60,6 +61,19 @@ typedef struct {
 
 typedef struct ffea_gpu_blob ffea_gpu_blob;
 
+/* sizes of the device-side problem */
+typedef struct {
+    int num_nodes, num_elements, num_linear_nodes;
+    long long num_blocks;        /* 3x3 blocks of the viscosity matrix (pattern, no padding) */
+    long long num_block_slots;   /* stored block slots (with slice padding) */
+    int precision;
+    long long device_bytes;      /* device memory held by the handle */
+    int num_blobs;
+    int num_chunks;              /* reduction chunks (up to 1024 node rows of one blob each) */
+    int row_blocks;              /* workgroups of the row kernels (gather, CG, update) */
+    int graph_iterations;        /* CG iterations in the captured step graph (0: none) */
+} ffea_gpu_blob_info;
+
 /* Blob::init + NoMassCGSolver::init (Blob.cpp:161-247, NoMassCGSolver.cpp:56-153).
  *   pos        [num_nodes][3] node positions (meso units, .node order, scale applied)
  *   top        [num_elements][10] node indices (.top order; 4 corners then the
@@ -79,6 +93,35 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          int num_ctf, const int *ctf_node, const double *ctf_force,
                          const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out);
+
+/* An ensemble of independent blobs in one handle (no counterpart in the reference, which
+ * steps the blobs of a World one after another, World.cpp:2031-2059). Blob b is given as
+ * for ffea_gpu_blob_create by element b of each array: num_nodes[b], pos[b], top[b], ...;
+ * num_pinned / pinned and num_ctf / ctf_node / ctf_force may be NULL (none); rng_states
+ * is [num_blobs][6], one noise stream per blob, each moving 7 draws per element of its
+ * blob per step (ffea_gpu_rng_stream_state gives independent RngStream streams). Every
+ * kernel launch covers all blobs; each blob has its own CG scalars and stops its solve
+ * on its own convergence test. Single-blob calls act on blob 0; the ffea_gpu_batch_*
+ * calls take the blob index. A step fails if any blob fails (the message names it). */
+int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
+                          const int *num_elements, const double *const *pos, const int *const *top,
+                          const double *const *A, const double *const *B, const double *const *G,
+                          const double *const *E, const int *num_pinned, const int *const *pinned,
+                          const int *num_ctf, const int *const *ctf_node,
+                          const double *const *ctf_force, const uint32_t *rng_states,
+                          ffea_gpu_blob **out);
+
+/* Builds the device tables of ffea_gpu_batch_create's arguments on the host, without a
+ * device, and checks that every index the kernels form from them stays inside the
+ * buffer it addresses (the kernels do not check). Fills *info (may be NULL; fields that
+ * need a device are 0). ffea_gpu_batch_create runs the same check before any upload.
+ * No counterpart in the reference. */
+int ffea_gpu_check_layout(int precision, int num_blobs, const int *num_nodes, const int *num_elements,
+                          const double *const *pos, const int *const *top, const double *const *A,
+                          const double *const *B, const double *const *G, const double *const *E,
+                          const int *num_pinned, const int *const *pinned, const int *num_ctf,
+                          const int *const *ctf_node, const double *const *ctf_force,
+                          const uint32_t *rng_states, ffea_gpu_blob_info *info);
 
 /* releases device memory and the stream; NULL is accepted */
 void ffea_gpu_blob_destroy(ffea_gpu_blob *blob);
@@ -125,15 +168,43 @@ int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *blob, const uint32_t state[6], l
 /* state after `ndraws` draws from `state` (exact jump-ahead; host only) */
 int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]);
 
-/* sizes of the device-side problem */
-typedef struct {
-    int num_nodes, num_elements, num_linear_nodes;
-    long long num_blocks;        /* 3x3 blocks of the viscosity matrix (pattern, no padding) */
-    long long num_block_slots;   /* stored block slots (with slice padding) */
-    int precision;
-    long long device_bytes;      /* device memory held by the handle */
-} ffea_gpu_blob_info;
+/* initial state of the stream-th RngStream created after RngStream::SetPackageSeed(
+ * package_seed): streams start 2^127 draws apart (RngStream.cpp:327-345, 380-390);
+ * World gives thread t stream t (World.cpp:278-295). Host only. */
+int ffea_gpu_rng_stream_state(const uint32_t package_seed[6], long long stream, uint32_t out[6]);
+
+/* per-blob forms of the node-state, force and stream calls above (blob = 0..num_blobs-1;
+ * arrays sized for that blob). ffea_gpu_batch_set_rng_stream with draws_per_step < 0
+ * keeps the blob's stride. ffea_gpu_batch_get_cg_iterations fills [num_blobs] arrays. */
+int ffea_gpu_batch_get_positions(ffea_gpu_blob *blob, int index, double *pos);
+int ffea_gpu_batch_set_positions(ffea_gpu_blob *blob, int index, const double *pos);
+int ffea_gpu_batch_get_velocities(ffea_gpu_blob *blob, int index, double *vel);
+int ffea_gpu_batch_get_rhs(ffea_gpu_blob *blob, int index, double *rhs);
+int ffea_gpu_batch_get_element_forces(ffea_gpu_blob *blob, int index, double *force);
+int ffea_gpu_batch_get_cg_iterations(ffea_gpu_blob *blob, int *last, long long *total);
+int ffea_gpu_batch_get_rng_state(ffea_gpu_blob *blob, int index, uint32_t state[6]);
+int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *blob, int index, const uint32_t state[6],
+                                  long long draws_per_step);
+
 int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
+
+/* Launch shape of the row kernels. Totals are summed per reduction chunk in a fixed
+ * order, so the shape never changes results. ffea_gpu_blob_set_row_blocks sets the
+ * number of workgroups (rounded to whole chunks per workgroup and a multiple of 8, one
+ * share per XCD). ffea_gpu_blob_autotune times `iterations` CG iterations of the current
+ * matrix at each candidate shape, keeps the fastest and returns it in *chosen (may be
+ * NULL); it leaves positions, velocities and noise streams untouched. No counterpart in
+ * the reference. */
+int ffea_gpu_blob_set_row_blocks(ffea_gpu_blob *blob, int blocks);
+int ffea_gpu_blob_autotune(ffea_gpu_blob *blob, int iterations, int *chosen);
+
+/* enable != 0: ffea_gpu_blob_step runs each step as one HIP graph launch (element pass,
+ * stream advance, assembly, gather, CG iterations, update), continuing eagerly only if a
+ * blob needs more CG iterations than the graph holds. The graph holds `iterations` CG
+ * iterations, or (iterations <= 0) a little over the last step's count, recaptured when
+ * that count moves. Results are identical to eager steps. Per-phase timing does not see
+ * graph steps. No counterpart in the reference. */
+int ffea_gpu_blob_use_graph(ffea_gpu_blob *blob, int enable, int iterations);
 
 /* Kernel timing with HIP events recorded on the handle's stream around each launch of
  * the phases selected by phase_mask (bit i = phase i; 0 turns timing off).

@@ -68,3 +68,26 @@ def test_cpp_wrapper_and_example_build(tmp_path):
            "-Wl,-rpath," + os.path.join(ROOT, "ffea_ci_demo_amd"), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_rng_stream_state_matches_oracle_streams(oracle_mod):
+    """ffea_gpu_rng_stream_state(seed, k) is the k-th RngStream after SetPackageSeed(seed)
+    (streams 2^127 draws apart); the oracle's package streams are pinned to RngStream"""
+    for seed in ([12345] * 6, [1, 2, 3, 4, 5, 6], [4294967086, 7, 9, 4294944442, 11, 1]):
+        pk = oracle_mod.Package()
+        assert pk.set_seed(seed)
+        streams = [oracle_mod.Rng(package=pk) for _ in range(6)]
+        for k, g in enumerate(streams):
+            assert np.array_equal(_native.rng_stream_state(seed, k), g.state()), (seed, k)
+    # L'Ecuyer's published second stream of the default package seed 12345 x 6
+    assert list(_native.rng_stream_state([12345] * 6, 1)) == [3692455944, 1366884236, 2968912127,
+                                                              335948734, 4161675175, 475798818]
+
+
+def test_ensemble_workload_streams_are_distinct():
+    from ffea_ci_demo_amd import workloads
+    specs = workloads.ensemble(6, nmin=2, nmax=4, seed=1)
+    states = {tuple(s.rng_state) for s in specs}
+    assert len(states) == 6
+    ns = [round(s.num_elements / 6 ** 0) for s in specs]
+    assert all(n in {6 * k ** 3 for k in (2, 3, 4)} for n in ns)

@@ -1,0 +1,150 @@
+"""The ensemble form (many independent blobs in one handle), captured step graphs and
+the launch-shape autotuner, through the C ABI.
+
+Each blob of a batch must give exactly what it gives alone (bit-identical: its chunk
+sums are formed in the same order), and so what the oracle gives (positions within
+1e-10 relative in FP64, per-step forces within 1e-12); a step replayed from a graph
+must equal the eager step bit for bit at every replay; the autotuner and the row grid
+must not change any result.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from ffea_ci_demo_amd import _native, workloads
+from ffea_ci_demo_amd.blob import FFEAException, GpuBatch, GpuBlob
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-10
+FORCE_TOL = 1e-12
+
+
+def small_ensemble(noise=1):
+    """5 blobs of uneven size and material, one pinned and loaded, own noise streams"""
+    specs = workloads.ensemble(4, nmin=2, nmax=6, seed=5, calc_noise=noise)
+    pinned = workloads.config3(n=3)
+    pinned.calc_noise = noise
+    pinned.rng_state = _native.rng_stream_state([12345] * 6, 4)
+    specs.append(pinned)
+    print("[case] ensemble", [s.name for s in specs], flush=True)
+    return specs
+
+
+def test_batch_equals_single_blobs_and_oracle(oracle_mod):
+    specs = small_ensemble()
+    nsteps = 8
+    with GpuBatch(specs) as batch:
+        assert batch.info()["num_blobs"] == len(specs)
+        batch.step(nsteps)
+        last, total = batch.cg_iterations()
+        for b, s in enumerate(specs):
+            with GpuBlob.from_spec(s) as g:
+                g.step(nsteps)
+                assert np.array_equal(batch.positions(b), g.positions()), b
+                assert np.array_equal(batch.rhs(b), g.rhs()), b
+                assert np.array_equal(batch.element_forces(b), g.element_forces()), b
+                assert np.array_equal(batch.rng_state(b), g.rng_state()), b
+                assert (last[b], total[b]) == g.cg_iterations(), b
+            o = oracle_mod.Blob(s.pos, s.top, s.A, s.B, s.G, s.E, **s.oracle_kwargs())
+            for _ in range(nsteps):
+                assert o.step() == 0
+            assert rel_err(batch.positions(b), o.positions()) <= POS_TOL, b
+            assert rel_err(batch.rhs(b), o.rhs()) <= FORCE_TOL, b
+            assert np.array_equal(batch.rng_state(b), o.rng_state()), b
+
+
+def test_batch_streams_advance_per_blob():
+    specs = small_ensemble()
+    with GpuBatch(specs) as batch:
+        batch.step(3)
+        for b, s in enumerate(specs):
+            want = _native.advance_rng_state(s.rng_state, 3 * 7 * s.num_elements)
+            assert np.array_equal(batch.rng_state(b), want), b
+
+
+def test_graph_replays_equal_eager_steps():
+    """every replay of the captured step against an eager twin, with host reads between"""
+    specs = small_ensemble()
+    with GpuBatch(specs) as eager, GpuBatch(specs) as graph:
+        graph.use_graph(True)
+        for k in range(8):
+            eager.step(1)
+            graph.step(1)
+            for b in range(len(specs)):
+                assert np.array_equal(graph.positions(b), eager.positions(b)), (k, b)
+                assert np.array_equal(graph.velocities(b), eager.velocities(b)), (k, b)
+            assert np.array_equal(graph.cg_iterations()[0], eager.cg_iterations()[0]), k
+        assert graph.info()["graph_iterations"] > 0
+
+
+def test_graph_single_blob_matches_oracle(oracle_mod):
+    spec = workloads.config0(rng_seed=9)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    with GpuBlob.from_spec(spec) as g:
+        g.use_graph(True)
+        for k in range(12):
+            g.step(1)
+            assert o.step() == 0
+            assert rel_err(g.positions(), o.positions()) <= POS_TOL, k
+        assert np.array_equal(g.rng_state(), o.rng_state())
+
+
+def test_graph_with_short_capture_continues_eagerly():
+    """a graph holding fewer CG iterations than a step needs: the step finishes eagerly
+    and equals the eager step"""
+    specs = small_ensemble()
+    with GpuBatch(specs) as eager, GpuBatch(specs) as graph:
+        graph.use_graph(True, iterations=5)
+        for k in range(4):
+            eager.step(1)
+            graph.step(1)
+            for b in range(len(specs)):
+                assert np.array_equal(graph.positions(b), eager.positions(b)), (k, b)
+        assert graph.info()["graph_iterations"] == 5
+        assert eager.cg_iterations()[0].max() > 5
+
+
+def test_row_grid_and_autotune_are_result_neutral():
+    spec = workloads.cube_blob(24, mesh_seed=3, rng_seed=8)  # 15625 rows: 16 reduction chunks
+    with GpuBlob.from_spec(spec) as ref:
+        ref.step(3)
+        want = ref.positions()
+    for blocks in (1, 3, 8, 16, 1000):
+        with GpuBlob.from_spec(spec) as g:
+            g.set_row_blocks(blocks)
+            g.step(3)
+            assert np.array_equal(g.positions(), want), blocks
+    with GpuBlob.from_spec(spec) as g:
+        g.step(1)
+        chosen = g.autotune(iterations=10)
+        assert chosen == g.info()["row_blocks"] and chosen >= 1
+        g.step(2)
+        assert np.array_equal(g.positions(), want)
+
+
+def test_mixed_batch_close_to_fp64():
+    """mixed form: free blobs are projected, the pinned one is not; each blob within
+    1e-5 relative of its FP64 trajectory after 20 steps, noise on"""
+    specs = small_ensemble()
+    with GpuBatch(specs) as d, GpuBatch(specs, precision="mixed") as m:
+        d.step(20)
+        m.step(20)
+        for b in range(len(specs)):
+            e = rel_err(m.positions(b), d.positions(b))
+            assert e <= 1e-5, (b, e)
+
+
+def test_batch_failure_names_the_blob():
+    specs = small_ensemble(noise=0)
+    with GpuBatch(specs) as batch:
+        p = specs[2].pos.copy()
+        top = specs[2].top
+        # invert the first element of blob 2: swap two of its corners' positions
+        a, c = top[0, 0], top[0, 1]
+        p[[a, c]] = p[[c, a]]
+        batch.set_positions(2, p)
+        with pytest.raises(FFEAException) as ei:
+            batch.step(1)
+        assert ei.value.status == _native.ERR_INVERSION
+        assert str(ei.value).startswith("blob 2:")

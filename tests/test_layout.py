@@ -1,0 +1,68 @@
+"""Host-side layout of one or many blobs (ffea_gpu_check_layout, no device): the tables
+the kernels read are built exactly as ffea_gpu_batch_create builds them and every index
+the kernels form is checked against its buffer. Sizes are checked against counts that
+follow from the meshes."""
+import numpy as np
+import pytest
+
+from ffea_ci_demo_amd import _native, workloads
+from ffea_ci_demo_amd.blob import FFEAException, check_layout
+
+
+def expected_chunks(spec):
+    nl = int(spec.mesh.is_linear.sum()) if spec.mesh is not None else None
+    if spec.pinned is not None and len(spec.pinned) == spec.num_nodes:
+        return 0
+    return -(-nl // 1024)
+
+
+def mixed_specs():
+    specs = workloads.ensemble(5, nmin=2, nmax=7, seed=3)
+    specs.append(workloads.config3(n=4))                     # pinned face, linear constant forces
+    folded = workloads.cube_blob(3, rng_seed=2)                # constant forces on midside nodes
+    mids = np.nonzero(~folded.mesh.is_linear)[0][:7].astype(np.int32)
+    folded.ctf_node = mids
+    folded.ctf_force = np.tile([0.1, -0.2, 0.3], (mids.size, 1))
+    specs.append(folded)
+    still = workloads.cube_blob(2, rng_seed=4)                 # every node pinned: no solve
+    still.pinned = np.arange(still.num_nodes, dtype=np.int32)
+    specs.append(still)
+    return specs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_layout_valid_and_sized(precision):
+    specs = mixed_specs()
+    info = check_layout(specs, precision)
+    assert info["num_blobs"] == len(specs)
+    assert info["num_elements"] == sum(s.num_elements for s in specs)
+    assert info["num_nodes"] == sum(s.num_nodes for s in specs)
+    assert info["num_linear_nodes"] == sum(int(s.mesh.is_linear.sum()) for s in specs)
+    assert info["num_chunks"] == sum(expected_chunks(s) for s in specs)
+    assert info["num_block_slots"] >= info["num_blocks"]
+
+
+def test_single_large_blob_layout():
+    spec = workloads.cube_blob(40)
+    info = check_layout([spec])
+    nl = 41 ** 3
+    assert info["num_linear_nodes"] == nl and info["num_chunks"] == -(-nl // 1024)
+    # interior rows of a Kuhn-split cube mesh couple to 14 neighbours and themselves
+    assert info["num_blocks"] <= 15 * nl
+
+
+def test_reference_sphere_layout():
+    from test_oracle_solver import sphere_spec
+    info = check_layout([sphere_spec(rng_seed=1)])
+    assert info["num_chunks"] == 1
+
+
+def test_layout_rejects_bad_topology():
+    spec = workloads.cube_blob(2)
+    bad = workloads.cube_blob(2)
+    bad.top = bad.top.copy()
+    bad.top[3, 0] = bad.num_nodes + 5
+    with pytest.raises(FFEAException) as ei:
+        check_layout([spec, bad])
+    assert "blob 1" in str(ei.value) and "out of bounds" in str(ei.value)
+    assert ei.value.status == _native.ERR_INVALID
