@@ -5,15 +5,21 @@ Workload (default): the 100x100x100 jittered-cube blob — 6.0M quadratic tets,
 2.2e9 Pa, viscosities 1e-3 Pa s), thermal noise on at 300 K, dt = 1e-14 s, CG
 tolerance 1e-10, steric/electrostatics off. One step = one full blob update:
 element forces, gather, viscosity-matrix assembly, Jacobi-PCG to tolerance, Euler
-update. Inputs are resident in HBM before the timed region.
+update. Inputs are resident in HBM before the timed region, which is bracketed by a
+synchronisation of the library's HIP stream on both sides (all work runs on it; one
+GPU, so there is no barrier across ranks). torch is not used: it bundles a HIP runtime
+of its own that must not share the process with the library's.
 
-Prints one JSON line (rank 0). `value` is the headline form's element-steps/s;
-`forms` holds every form measured. `roofline` is for the dominant kernel (the CG
-SpMV step), from HIP events recorded on the library's stream over the timed
-region. `cpu_baseline` times the oracle (the FP64 C restatement of the reference
-algorithm, single thread) on a bounded sample of the same workload.
+Prints one JSON line (rank 0). `value` is the headline form's element-steps/s
+(default: FP32-mixed); `forms` holds every form measured (default FP64 and mixed,
+with the mixed form's position error against FP64). `roofline` is for the dominant
+kernel (the CG SpMV step), from HIP events recorded on the library's stream over the
+timed region. `cpu_baseline` times the oracle (the FP64 C restatement of the
+reference algorithm, single thread) on a bounded sample of the same workload.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--forms fp64,mixed]
+  python bench.py --workload ensemble [--blobs 512]   # many blobs in one handle,
+                                                      # graph-captured steps
 """
 from __future__ import annotations
 
@@ -47,7 +53,6 @@ def iter_alg_bytes(info, w):
 
 def run_form(spec, form, steps, warmup):
     from ffea_ci_demo_amd.blob import GpuBlob
-    import torch
     t0 = time.perf_counter()
     g = GpuBlob.from_spec(spec, precision=form)
     t_create = time.perf_counter() - t0
@@ -59,12 +64,11 @@ def run_form(spec, form, steps, warmup):
         g.step(1)
     _, tot0 = g.cg_iterations()
     g.set_timing(("cg_spmv", "cg_vector"))
-    torch.cuda.synchronize()
+    g.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
         g.step(1)
     g.synchronize()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     tim = g.timing()
     _, tot1 = g.cg_iterations()
@@ -90,6 +94,61 @@ def run_form(spec, form, steps, warmup):
     positions = g.positions()
     g.close()
     return res, positions
+
+
+def run_ensemble(specs, form, steps, warmup, tune_iters=20):
+    """all blobs in one handle: autotuned row grid, one eager timed step for the SpMV
+    average, then graph-captured steps timed end to end"""
+    from ffea_ci_demo_amd.blob import GpuBatch
+    t0 = time.perf_counter()
+    g = GpuBatch(specs, precision=form)
+    t_create = time.perf_counter() - t0
+    info = g.info()
+    print(f"[bench] ensemble of {len(specs)} blobs form={form} created in {t_create:.1f}s: Ne={info['num_elements']} "
+          f"Nl={info['num_linear_nodes']} chunks={info['num_chunks']} device_bytes={info['device_bytes']/1e9:.2f}GB",
+          file=sys.stderr, flush=True)
+    g.step(1)
+    chosen = g.autotune(tune_iters)
+    print(f"[bench] autotuned row grid: {chosen} workgroups", file=sys.stderr, flush=True)
+    g.set_timing(("cg_spmv", "cg_vector"))
+    g.step(1)
+    tim = g.timing()
+    g.set_timing(())
+    g.use_graph(True)
+    for _ in range(warmup):
+        g.step(1)
+    last0, tot0 = g.cg_iterations()
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    last1, tot1 = g.cg_iterations()
+    k = (tot1 - tot0) / steps
+    info = g.info()
+    w = 8 if form == "fp64" else 4
+    spmv_ms, spmv_n = tim["cg_spmv"]
+    avg_spmv_s = spmv_ms / max(spmv_n, 1) / 1e3
+    ne = info["num_elements"]
+    res = {
+        "element_steps_per_s": ne * steps / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "cg_iterations_per_step_max": float(k.max()),
+        "cg_iterations_per_step_median": float(np.median(k)),
+        "cg_iterations_per_step_min": float(k.min()),
+        "spmv_avg_us_all_blobs_running": avg_spmv_s * 1e6,
+        "spmv_alg_bytes": spmv_alg_bytes(info, w),
+        "spmv_achieved_GBs": spmv_alg_bytes(info, w) / avg_spmv_s / 1e9,
+        "row_workgroups": chosen,
+        "graph_cg_iterations": info["graph_iterations"],
+        "info": info,
+    }
+    print(f"[bench] ensemble form={form} {res['ms_per_step']:.1f} ms/step, k max/median "
+          f"{res['cg_iterations_per_step_max']:.0f}/{res['cg_iterations_per_step_median']:.0f}, "
+          f"spmv {res['spmv_avg_us_all_blobs_running']:.1f} us = {res['spmv_achieved_GBs']:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    g.close()
+    return res
 
 
 def cpu_baseline(spec, k_per_step, cg_iters):
@@ -122,8 +181,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=100, help="cubes per side (100: the 6.0M-tet blob)")
-    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--headline", default=None, help="form reported as value (default: last of --forms)")
+    ap.add_argument("--workload", default="cube", choices=("cube", "ensemble"))
+    ap.add_argument("--blobs", type=int, default=512, help="ensemble: number of blobs")
     ap.add_argument("--cpu-cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -133,6 +194,8 @@ def main():
         sys.exit(2)
     from ffea_ci_demo_amd import _native, workloads
 
+    if args.workload == "ensemble":
+        return main_ensemble(args)
     t0 = time.perf_counter()
     spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
     print(f"[bench] mesh n={args.n} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -195,6 +258,63 @@ def main():
         "cpu_baseline": cpu,
         "forms": {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items() if k != "info"}
                   for f, r in results.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main_ensemble(args):
+    from ffea_ci_demo_amd import _native, workloads
+    t0 = time.perf_counter()
+    specs = workloads.config2(args.blobs)
+    ne_tot = sum(s.num_elements for s in specs)
+    ns = [round((s.num_elements / 6) ** (1 / 3)) for s in specs]
+    print(f"[bench] {len(specs)} meshes ({ne_tot} tets, n {min(ns)}..{max(ns)}) built in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    form = (args.headline or "mixed") if args.forms == "fp64,mixed" else args.forms.split(",")[-1]
+    h = run_ensemble(specs, form, args.steps, args.warmup)
+    copy_bytes = 1 << 30
+    copy_ms = _native.copy_probe(copy_bytes, iters=20)
+    roof = {
+        "bound": "hbm",
+        "kernel": "k_cg_spmv",
+        "achieved": round(h["spmv_achieved_GBs"], 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(h["spmv_achieved_GBs"] / PEAK_HBM_GBS, 4),
+        "traffic": None,
+        "alg_bytes_per_launch": h["spmv_alg_bytes"],
+        "avg_launch_us": round(h["spmv_avg_us_all_blobs_running"], 2),
+        "copy_probe_GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        mid = sorted(specs, key=lambda s: s.num_elements)[len(specs) // 2]
+        cpu = cpu_baseline(mid, h["cg_iterations_per_step_median"], args.cpu_cg_iters)
+        cpu["sample"] = f"median blob ({mid.name}, {mid.num_elements} tets): " + cpu["sample"]
+    out = {
+        "metric": METRIC,
+        "value": round(h["element_steps_per_s"], 1),
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if form == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ensemble of {len(specs)} independent jittered-cube blobs, n^3 cubes with n = "
+                        f"round(exp(U(ln 8, ln 40))) ({min(ns)}..{max(ns)}), {ne_tot} tets in all; shear and bulk "
+                        "moduli each log-uniform in [1e7, 1e9] Pa; one MRG32k3a stream per blob; noise at 300 K, "
+                        "dt 1e-14 s, CG epsilon 1e-10; one handle, graph-captured steps",
+            "form": form,
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "forms": {form: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in h.items() if k != "info"}},
     }
     print(json.dumps(out), flush=True)
 

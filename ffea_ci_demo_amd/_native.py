@@ -29,7 +29,8 @@ EXPORTED = [
     "ffea_gpu_batch_get_velocities", "ffea_gpu_batch_get_rhs", "ffea_gpu_batch_get_element_forces",
     "ffea_gpu_batch_get_cg_iterations", "ffea_gpu_batch_get_rng_state", "ffea_gpu_batch_set_rng_stream",
     "ffea_gpu_rng_stream_state", "ffea_gpu_advance_rng_state", "ffea_gpu_blob_set_row_blocks",
-    "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout",
+    "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout", "ffea_gpu_blob_set_material",
+    "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint",
 ]
 
 
@@ -95,6 +96,9 @@ def lib():
     L.ffea_gpu_blob_set_row_blocks.argtypes = [vp, C.c_int]
     L.ffea_gpu_blob_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.ffea_gpu_blob_use_graph.argtypes = [vp, C.c_int, C.c_int]
+    L.ffea_gpu_blob_set_material.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.ffea_gpu_blob_run_checkpointed.argtypes = [vp, C.c_int, C.c_int]
+    L.ffea_gpu_blob_adjoint.argtypes = [vp, _dp, vp, vp, vp, vp, vp]
     L.ffea_gpu_blob_last_error.argtypes = [vp]
     L.ffea_gpu_blob_last_error.restype = C.c_char_p
     L.ffea_gpu_last_error.argtypes = []

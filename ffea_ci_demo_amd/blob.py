@@ -191,6 +191,25 @@ class GpuBlob(_Handle):
         self._check(_native.lib().ffea_gpu_blob_set_rng_stream(self.h, np.ascontiguousarray(s, np.uint32),
                                                                int(draws_per_step)))
 
+    # -- differentiation (noise off, some nodes pinned)
+    def set_material(self, A, B, G, E):
+        """per-element A, B, G, E in meso units (scalars broadcast)"""
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.ne,))) for v in (A, B, G, E)]
+        self._check(_native.lib().ffea_gpu_blob_set_material(self.h, *arrs))
+
+    def run_checkpointed(self, nsteps, interval=20):
+        """nsteps steps from the current state, positions kept every `interval` steps"""
+        self._check(_native.lib().ffea_gpu_blob_run_checkpointed(self.h, int(nsteps), int(interval)))
+
+    def adjoint(self, dL_dpos):
+        """reverse of the last checkpointed run: dL/dpos at its end -> (dL/dA, dL/dB, dL/dG,
+        dL/dE per element, dL/dpos at its start)"""
+        g = np.ascontiguousarray(dL_dpos, np.float64).reshape(-1)
+        outs = [np.zeros(self.ne) for _ in range(4)]
+        d0 = np.zeros((self.nt, 3))
+        self._check(_native.lib().ffea_gpu_blob_adjoint(self.h, g, *[o.ctypes.data for o in outs], d0.ctypes.data))
+        return outs[0], outs[1], outs[2], outs[3], d0
+
 
 class GpuBatch(_Handle):
     """An ensemble of independent blobs stepped together in one handle.

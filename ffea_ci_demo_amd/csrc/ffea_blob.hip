@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -31,6 +32,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -417,9 +419,30 @@ struct Tables {
 Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
     Tables t;
     const int nblob = t.nblob = (int)in.size();
-    std::vector<BlobPlan> plans;
-    plans.reserve(nblob);
-    for (int b = 0; b < nblob; ++b) plans.push_back(make_plan(in[b], nblob > 1 ? b : -1));
+    // per-blob plans are independent: build them on up to 16 host threads
+    std::vector<BlobPlan> plans(nblob);
+    {
+        const int nthreads = std::max(1, std::min({nblob, 16, (int)std::thread::hardware_concurrency()}));
+        std::atomic<int> next{0};
+        std::vector<std::string> errors(nblob);  // the first failing blob is reported
+        auto work = [&]() {
+            for (int b = next++; b < nblob; b = next++) {
+                try {
+                    plans[b] = make_plan(in[b], nblob > 1 ? b : -1);
+                } catch (const Invalid &e) {
+                    errors[b] = e.msg;
+                } catch (const std::bad_alloc &) {
+                    errors[b] = "host allocation failed";
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        for (const auto &e : errors)
+            if (!e.empty()) throw Invalid(e);
+    }
     // ---- concatenation offsets: every blob starts on a 64-row slice boundary
     t.meta.resize(nblob);
     long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0;
@@ -679,6 +702,13 @@ struct ffea_gpu_blob {
     int graph_iters = 0;
     int graph_fixed_iters = 0;  // > 0: CG iterations captured; 0: from the last step's count
 
+    // checkpointed run and its reverse (single blob, noise off)
+    int zero_rhs_solved = 0;  // CG: a zero right-hand side is solved by x = 0 (adjoint solves)
+    int run_steps = 0, run_interval = 0;
+    DevBuf<double> ckpt, seg_x, seg_v, x_end, v_end, lam, grad;
+    DevBuf<double> d_w;
+    DevBuf<float> f_w;
+
     // timing
     unsigned timing = 0;  // bit i: time phase i
     int cg_sample = 8;    // CG phases: time one iteration in cg_sample
@@ -752,7 +782,14 @@ struct ffea_gpu_blob {
     }
     RowArgs row_args() { return RowArgs{nchunks, cpb, chunks.p, seg.p, S.p, num_done.p}; }
     void launch_element();
+    void launch_assemble();
     void launch_assemble_gather();
+    template <typename T>
+    void set_material_t(const double *A, const double *B, const double *G, const double *E);
+    int forward_checkpointed(int nsteps, int interval);
+    int adjoint(const double *dLdx, double *dA, double *dB, double *dG, double *dE, double *dLdx0);
+    template <typename T>
+    void launch_adjoint_step(const double *pos_k, const double *vel_k, T *w);
     template <typename T, bool PROJ>
     void launch_gather(const T *force);
     void launch_cg_iteration(int it, bool spmv_only);
@@ -966,6 +1003,20 @@ void ffea_gpu_blob::launch_gather(const T *force) {
 }
 
 void ffea_gpu_blob::launch_assemble_gather() {
+    launch_assemble();
+    hipEvent_t t0;
+    t_begin(2, &t0);
+    if (mixed) {
+        if (any_project) launch_gather<float, true>(f_force.p);
+        else launch_gather<float, false>(f_force.p);
+    } else {
+        launch_gather<double, false>(d_force.p);
+    }
+    FFEA_HIP_TRY(hipGetLastError());
+    t_end(2, t0);
+}
+
+void ffea_gpu_blob::launch_assemble() {
     hipEvent_t t0;
     t_begin(1, &t0);
     if (mixed) {
@@ -979,22 +1030,13 @@ void ffea_gpu_blob::launch_assemble_gather() {
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(1, t0);
-    t_begin(2, &t0);
-    if (mixed) {
-        if (any_project) launch_gather<float, true>(f_force.p);
-        else launch_gather<float, false>(f_force.p);
-    } else {
-        launch_gather<double, false>(d_force.p);
-    }
-    FFEA_HIP_TRY(hipGetLastError());
-    t_end(2, t0);
 }
 
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) {
     hipEvent_t ts = nullptr;
     if (t_cg(3, it)) t_begin(3, &ts);
-    SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, prm.epsilon2, slice_w.p, slice_base.p, col.p, val, z.p,
+    SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
     hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
     t_end(3, ts, it);
@@ -1140,6 +1182,171 @@ int ffea_gpu_blob::step_once() {
     run_cg_eager(0);
     launch_update();
     return finish_step();
+}
+
+// ---------------------------------------------------------------- checkpointed run and reverse
+template <typename T>
+void ffea_gpu_blob::set_material_t(const double *A, const double *B, const double *G, const double *E) {
+    DevBuf<T> *bA, *bB, *bG, *bE;
+    if constexpr (std::is_same<T, double>::value) {
+        bA = &d_A; bB = &d_B; bG = &d_G; bE = &d_E;
+    } else {
+        bA = &f_A; bB = &f_B; bG = &f_G; bE = &f_E;
+    }
+    const int ne0 = meta[0].ne;
+    std::vector<T> t(ne0);
+    const std::pair<DevBuf<T> *, const double *> items[4] = {{bA, A}, {bB, B}, {bG, G}, {bE, E}};
+    for (const auto &it : items) {
+        for (int e = 0; e < ne0; ++e) t[e] = (T)it.second[e];
+        it.first->upload(t, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    }
+}
+
+static void ensure(DevBuf<double> &b, size_t n, size_t *total) {
+    if (b.p && b.n >= n) return;
+    if (b.p) {
+        (void)hipFree(b.p);
+        *total -= b.n * sizeof(double);
+        b.p = nullptr;
+    }
+    b.alloc(n, total);
+}
+
+// nsteps forward steps from the current state; the positions at every interval-th step
+// are kept on the device for ffea_gpu_blob_adjoint
+int ffea_gpu_blob::forward_checkpointed(int nsteps, int interval) {
+    if (nblob != 1 || prm.calc_noise || meta[0].all_pinned)
+        throw Invalid("a checkpointed run needs a single blob, calc_noise = 0 and unpinned nodes");
+    const size_t vec = (size_t)3 * nrows;
+    const int nck = (nsteps + interval - 1) / interval;
+    ensure(ckpt, vec * std::max(1, nck), &dev_bytes);
+    ensure(x_end, vec, &dev_bytes);
+    ensure(v_end, vec, &dev_bytes);
+    run_steps = 0;
+    for (int k = 0; k < nsteps; ++k) {
+        if (k % interval == 0)
+            FFEA_HIP_TRY(hipMemcpyAsync(ckpt.p + vec * (k / interval), pos.p, vec * sizeof(double),
+                                        hipMemcpyDeviceToDevice, stream));
+        const int rc = step_once();
+        if (rc != FFEA_GPU_OK) return rc;
+    }
+    FFEA_HIP_TRY(hipMemcpyAsync(x_end.p, pos.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(v_end.p, vel.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    run_steps = nsteps;
+    run_interval = interval;
+    return FFEA_GPU_OK;
+}
+
+// one reverse step at positions pos_k with the step's velocity vel_k: K(x_k) mu = P lambda,
+// then the element reverse and lambda += dt * gather(w)
+template <typename T>
+void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k, T *w) {
+    const size_t vec = (size_t)3 * nrows;
+    FFEA_HIP_TRY(hipMemcpyAsync(pos.p, pos_k, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    launch_element();  // shape data of x_k for the operator
+    launch_assemble();
+    AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p};
+    hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kBlock), 0, stream, st);
+    FFEA_HIP_TRY(hipGetLastError());
+    zero_rhs_solved = 1;
+    try {
+        run_cg_eager(0);
+    } catch (...) {
+        zero_rhs_solved = 0;
+        throw;
+    }
+    zero_rhs_solved = 0;
+    FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    if (!h_S[0].converged)
+        throw Invalid(fmt("adjoint solve: conjugate gradient could not converge after %d iterations",
+                          prm.max_iterations_cg));
+    AdjArgs<T> a{ne, conn.p, pos.p, x.p, vel_k, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, prm.dt, w,
+                 grad.p};
+    if constexpr (std::is_same<T, double>::value) {
+        a.jinv0 = d_jinv0.p; a.vol0 = d_vol0.p; a.G = d_G.p; a.E = d_E.p; a.A = d_A.p; a.B = d_B.p;
+    } else {
+        a.jinv0 = f_jinv0.p; a.vol0 = f_vol0.p; a.G = f_G.p; a.E = f_E.p; a.A = f_A.p; a.B = f_B.p;
+    }
+    hipLaunchKernelGGL(k_element_adjoint<T>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
+    hipLaunchKernelGGL(k_adjoint_gather<T>, dim3(nb), dim3(kBlock), 0, stream, row_args(), (const uint32_t *)inc_ptr.p,
+                       (const uint32_t *)inc.p, (const T *)w, prm.dt, lam.p);
+    FFEA_HIP_TRY(hipGetLastError());
+}
+
+// reverse of the last checkpointed run: dL/dx at its end in, dL/d(A, B, G, E) per element
+// and dL/dx at its start out; the blob is left in the state at the end of the run
+int ffea_gpu_blob::adjoint(const double *dLdx, double *dA, double *dB, double *dG, double *dE, double *dLdx0) {
+    if (run_steps <= 0) throw Invalid("no checkpointed run to reverse");
+    const BlobMeta &m = meta[0];
+    const size_t vec = (size_t)3 * nrows, vlen = (size_t)nslices * 3 * kSlice;
+    ensure(lam, vlen, &dev_bytes);
+    ensure(grad, (size_t)4 * ne, &dev_bytes);
+    ensure(seg_x, vec * run_interval, &dev_bytes);
+    ensure(seg_v, vec * run_interval, &dev_bytes);
+    if (mixed && !f_w.p) f_w.alloc((size_t)12 * ne, &dev_bytes);
+    if (!mixed && !d_w.p) d_w.alloc((size_t)12 * ne, &dev_bytes);
+    // dL/dx on the corner rows: a midside node sits at its edge midpoint, so half of its
+    // derivative goes to each edge end
+    std::vector<double> l(vlen, 0.0);
+    for (int v = 0; v < m.nt; ++v) {
+        if (m.lin_of[v] >= 0) {
+            for (int c = 0; c < 3; ++c) l[vi(m.lin_of[v], c)] += dLdx[3 * v + c];
+        } else if (m.mid_a[v] >= 0) {
+            for (int c = 0; c < 3; ++c) {
+                l[vi(m.lin_of[m.mid_a[v]], c)] += 0.5 * dLdx[3 * v + c];
+                l[vi(m.lin_of[m.mid_b[v]], c)] += 0.5 * dLdx[3 * v + c];
+            }
+        }
+    }
+    lam.upload(l, stream);
+    grad.zero(stream);
+    const bool keep_graph = use_graph;
+    use_graph = false;  // recomputed steps run eagerly (identical results)
+    try {
+        const int nseg = (run_steps + run_interval - 1) / run_interval;
+        for (int sg = nseg - 1; sg >= 0; --sg) {
+            const int k0 = sg * run_interval, k1 = std::min(run_steps, k0 + run_interval);
+            FFEA_HIP_TRY(hipMemcpyAsync(pos.p, ckpt.p + vec * sg, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            for (int k = k0; k < k1; ++k) {
+                FFEA_HIP_TRY(hipMemcpyAsync(seg_x.p + vec * (k - k0), pos.p, vec * sizeof(double),
+                                            hipMemcpyDeviceToDevice, stream));
+                const int rc = step_once();
+                if (rc != FFEA_GPU_OK) throw Invalid("recomputed step failed: " + err);
+                FFEA_HIP_TRY(hipMemcpyAsync(seg_v.p + vec * (k - k0), vel.p, vec * sizeof(double),
+                                            hipMemcpyDeviceToDevice, stream));
+            }
+            for (int k = k1 - 1; k >= k0; --k) {
+                if (mixed) launch_adjoint_step<float>(seg_x.p + vec * (k - k0), seg_v.p + vec * (k - k0), f_w.p);
+                else launch_adjoint_step<double>(seg_x.p + vec * (k - k0), seg_v.p + vec * (k - k0), d_w.p);
+            }
+        }
+    } catch (...) {
+        use_graph = keep_graph;
+        (void)hipMemcpyAsync(pos.p, x_end.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream);
+        (void)hipMemcpyAsync(vel.p, v_end.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream);
+        (void)hipStreamSynchronize(stream);
+        throw;
+    }
+    use_graph = keep_graph;
+    FFEA_HIP_TRY(hipMemcpyAsync(pos.p, x_end.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(vel.p, v_end.p, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    std::vector<double> g((size_t)4 * ne);
+    FFEA_HIP_TRY(hipMemcpyAsync(g.data(), grad.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(l.data(), lam.p, l.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    double *outs[4] = {dA, dB, dG, dE};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k]) std::copy(g.begin() + (size_t)k * ne, g.begin() + (size_t)(k + 1) * ne, outs[k]);
+    if (dLdx0) {
+        // midside positions are re-derived from the corners at set-up: they carry no derivative
+        std::fill(dLdx0, dLdx0 + (size_t)3 * m.nt, 0.0);
+        for (int n = 0; n < m.nl; ++n)
+            for (int c = 0; c < 3; ++c) dLdx0[3 * m.orig_of_lin[n] + c] = l[vi(n, c)];
+    }
+    return FFEA_GPU_OK;
 }
 
 // ================================================================ C ABI
@@ -1621,6 +1828,38 @@ int ffea_gpu_blob_autotune(ffea_gpu_blob *b, int iterations, int *chosen_blocks)
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         if (chosen_blocks) *chosen_blocks = b->nb;
         return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_set_material(ffea_gpu_blob *b, const double *A, const double *B, const double *G,
+                                const double *E) {
+    if (!b || !A || !B || !G || !E) return FFEA_GPU_ERR_INVALID;
+    if (b->nblob != 1) {
+        b->err = "ffea_gpu_blob_set_material: single-blob handles only";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        if (b->mixed) b->set_material_t<float>(A, B, G, E);
+        else b->set_material_t<double>(A, B, G, E);
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_run_checkpointed(ffea_gpu_blob *b, int nsteps, int interval) {
+    if (!b || nsteps <= 0 || interval <= 0) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        return b->forward_checkpointed(nsteps, interval);
+    });
+}
+
+int ffea_gpu_blob_adjoint(ffea_gpu_blob *b, const double *dL_dpos, double *dL_dA, double *dL_dB, double *dL_dG,
+                          double *dL_dE, double *dL_dpos0) {
+    if (!b || !dL_dpos) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        return b->adjoint(dL_dpos, dL_dA, dL_dB, dL_dG, dL_dE, dL_dpos0);
     });
 }
 

@@ -592,6 +592,8 @@ struct SpmvArgs {
     RowArgs ra;
     int iter, max_iter;
     double epsilon2;
+    int zero_rhs_solved;   // 1: a zero right-hand side has the solution x = 0 (adjoint solves);
+                           // 0: the reference's arithmetic (0/0 in the first step)
     const int *slice_w;
     const long long *slice_base;
     const int *col;        // [nslots]
@@ -610,7 +612,7 @@ struct SpmvArgs {
 // the row gathers only z (SparseMatrixFixedPattern.cpp:148-187), and the p.q partials
 // (NoMassCGSolver.cpp:251-265).
 template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
+__device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
     const int i = a.iter;
     const bool first = (i == 0);
@@ -639,7 +641,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
                 } else {
                     double t[2];
                     sum_partials<2>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
-                    if (lead) {
+                    if (a.zero_rhs_solved && t[1] == 0.0) {
+                        if (lead) {
+                            S->converged = 1;
+                            S->iter = 0;
+                            S->done = 1;
+                            __hip_atomic_fetch_add(a.ra.num_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        skip = true;
+                    } else if (lead) {
                         S->delta[1] = t[0];
                         S->f2 = t[1];
                     }
@@ -712,6 +722,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
             }
             v[0] += pn[0] * qn[0] + pn[1] * qn[1] + pn[2] * qn[2];
             if constexpr (PROJ) {
+                // the rigid-mode sums need FP64: their FP32 rounding is as large as the
+                // null-space error the projection removes
                 const double x[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
                 double zq[6], zp[6];
                 zt_times(x, qn, zq);
@@ -725,6 +737,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
         }
         store_partials<NV>(v, a.P1, a.ra.nchunks, c);
     }
+}
+
+template <typename T, bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
+    cg_spmv<T, PROJ>(a);
+}
+// the projected FP32 step is held to 128 VGPRs (4 waves per SIMD): one 1024-row chunk
+// per workgroup then fits the 6M-tet blob's 1007 chunks on the 256 CUs in one round
+template <>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
+    SpmvArgs<float> a) {
+    cg_spmv<float, true>(a);
 }
 
 struct VecArgs {
@@ -828,6 +852,257 @@ __global__ void k_mark_updated(int nblob, CgScalars *S) {
     if (b >= nblob) return;
     if (S[b].converged) S[b].updated = 1;
     S[b].inverted = 0;
+}
+
+// ---------------------------------------------------------------- adjoint (reverse of a step)
+// One mass-less step is x' = x + dt v with K(x) v = P f(x) (P zeroes pinned rows). For
+// lambda' = dL/dx' the reverse is: solve K mu = P lambda'; then
+//   lambda = lambda' + dt [ d(mu . f)/dx - d(mu^T K(x) v)/dx ]
+//   dL/dtheta += dt [ d(mu . f)/dtheta - d(mu^T K v)/dtheta ]   (theta = G, E, A, B per element)
+// K comes from tetra_element_linear.cpp:298-327, which for an element is
+//   (K_e v)_a = vol sigma(v) grad psi_a,  sigma(v) = A (L + L^T) + B tr(L) I,  L = sum_b v_b (x) grad psi_b,
+// so mu^T K_e v = vol sigma(v) : L_mu. The elastic force is f_a = -vol s grad psi_a with s the
+// shear + bulk stress of tetra_element_linear.cpp:375-403. Both scalars are functions of the
+// element Jacobian J only; their derivatives are taken in forward mode over the 9 entries of J.
+
+template <typename T>
+struct Dual {
+    T v, d;  // value and derivative along one direction
+};
+template <typename T>
+__device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator-(Dual<T> a) { return {-a.v, -a.d}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.v * b.d + a.d * b.v}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.d * b}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator*(T b, Dual<T> a) { return {a.v * b, a.d * b}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
+    const T r = (T)1 / b.v;
+    return {a.v * r, (a.d - a.v * r * b.d) * r};
+}
+template <typename T>
+__device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
+    const T r = (T)1 / b.v;
+    return {a * r, -a * r * r * b.d};
+}
+template <typename T>
+__device__ __forceinline__ Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.d}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator-(T b, Dual<T> a) { return {b - a.v, -a.d}; }
+template <typename T>
+__device__ __forceinline__ Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.d}; }
+template <typename T>
+__device__ __forceinline__ T value_of(Dual<T> a) { return a.v; }
+template <typename T>
+__device__ __forceinline__ T value_of(T a) { return a; }
+
+// mu . f_el(J) - mu^T K(J) v for one element (noise off, element not inverted) and, for
+// the value pass, its partial derivatives in G, E, A, B
+template <typename S, typename T>
+__device__ __forceinline__ S adjoint_scalar(const S (&J)[3][3], const T (&Ji0)[3][3], T v0, T G, T E, T A, T B,
+                                            const T (&mu)[4][3], const T (&vel)[4][3], T *part) {
+    S d[12];
+    d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    const S det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
+    const S vol = (value_of(det) < (T)0 ? -det : det) * (T)(1.0 / 6.0);
+    const S rdet = (T)1 / det;
+#pragma unroll
+    for (int k = 1; k < 12; ++k)
+        if (k & 3) d[k] = d[k] * rdet;
+    d[0] = -(d[1] + d[2] + d[3]);
+    d[4] = -(d[5] + d[6] + d[7]);
+    d[8] = -(d[9] + d[10] + d[11]);
+    // F = J^T Ji0^T and F F^T
+    S F[3][3], FF[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = J[0][i] * Ji0[j][0] + J[1][i] * Ji0[j][1] + J[2][i] * Ji0[j][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) FF[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+    // velocity gradients L[i][j] = sum_q w_q[i] grad psi_q[j], grad psi_q[j] = d[4j+q]
+    S Lm[3][3], Lv[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Lm[i][j] = mu[0][i] * d[4 * j] + mu[1][i] * d[4 * j + 1] + mu[2][i] * d[4 * j + 2] + mu[3][i] * d[4 * j + 3];
+            Lv[i][j] = vel[0][i] * d[4 * j] + vel[1][i] * d[4 * j + 1] + vel[2][i] * d[4 * j + 2] +
+                       vel[3][i] * d[4 * j + 3];
+        }
+    const S r0 = v0 / vol, r1 = vol * ((T)1 / v0);
+    const S trm = Lm[0][0] + Lm[1][1] + Lm[2][2];
+    const S trv = Lv[0][0] + Lv[1][1] + Lv[2][2];
+    // stress = G sG + E sE: sG = r0 FF^T + (-r0 - (r1 - r0)/3) I, sE = (r1 - r0)/2 I
+    S ffm = FF[0][0] * Lm[0][0];
+    ffm = ffm + FF[0][1] * Lm[0][1] + FF[0][2] * Lm[0][2] + FF[1][0] * Lm[1][0] + FF[1][1] * Lm[1][1] +
+          FF[1][2] * Lm[1][2] + FF[2][0] * Lm[2][0] + FF[2][1] * Lm[2][1] + FF[2][2] * Lm[2][2];
+    const S sG_m = r0 * ffm - (r0 + (r1 - r0) * (T)(1.0 / 3.0)) * trm;  // sG : L_mu
+    const S sE_m = (r1 - r0) * (T)0.5 * trm;                             // sE : L_mu
+    S sym = Lv[0][0] * Lm[0][0] * (T)2;
+    sym = sym + (Lv[0][1] + Lv[1][0]) * (Lm[0][1] + Lm[1][0]) + (Lv[0][2] + Lv[2][0]) * (Lm[0][2] + Lm[2][0]) +
+          (Lv[1][2] + Lv[2][1]) * (Lm[1][2] + Lm[2][1]) + Lv[1][1] * Lm[1][1] * (T)2 + Lv[2][2] * Lm[2][2] * (T)2;
+    const S tt = trv * trm;
+    const S phi = -(vol * (G * sG_m + E * sE_m));   // mu . f_el
+    const S psi = vol * (A * sym + B * tt);         // mu^T K v
+    if (part) {
+        part[0] = -value_of(vol * sym);   // d/dA of (phi - psi)
+        part[1] = -value_of(vol * tt);    // d/dB
+        part[2] = -value_of(vol * sG_m);  // d/dG
+        part[3] = -value_of(vol * sE_m);  // d/dE
+    }
+    return phi - psi;
+}
+
+template <typename T>
+struct AdjArgs {
+    int ne;
+    const int *conn;      // [4][ne]
+    const double *pos;    // [rows][3] positions at the step
+    const double *mu;     // tiled: the adjoint solve
+    const double *vel;    // [rows][3]: the step's velocity solution
+    const T *jinv0, *vol0, *G, *E, *A, *B;
+    double dt;
+    T *w;                 // [ne][12]: d(phi - psi)/dx per corner
+    double *grad;         // [4][ne]: dL/dA, dL/dB, dL/dG, dL/dE accumulated
+};
+
+// reverse of the element pass and of the viscosity operator for one step, one element
+// per lane; every element writes only its own outputs (no atomics)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_element_adjoint(AdjArgs<T> a) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
+    const int ne = a.ne;
+    int n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
+    double x0[3];
+    T Jv[3][3], mu[4][3], vel[4][3], Ji0[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Jv[r][c] = (T)(a.pos[3 * n[r + 1] + c] - x0[c]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            mu[q][c] = (T)a.mu[vi(n[q], c)];
+            vel[q][c] = (T)a.vel[3 * n[q] + c];
+        }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ji0[k / 3][k % 3] = a.jinv0[(size_t)k * ne + e];
+    const T v0 = a.vol0[e], G = a.G[e], E = a.E[e], A = a.A[e], B = a.B[e];
+    T part[4];
+    {
+        T J[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) J[r][c] = Jv[r][c];
+        (void)adjoint_scalar<T, T>(J, Ji0, v0, G, E, A, B, mu, vel, part);
+    }
+    T dJ[3][3];
+    for (int k = 0; k < 9; ++k) {
+        Dual<T> J[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) J[r][c] = Dual<T>{Jv[r][c], (T)((3 * r + c) == k ? 1 : 0)};
+        dJ[k / 3][k % 3] = adjoint_scalar<Dual<T>, T>(J, Ji0, v0, G, E, A, B, mu, vel, nullptr).d;
+    }
+    // J rows are x_{r+1} - x_0
+    T *wo = a.w + (size_t)12 * e;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        wo[c] = -(dJ[0][c] + dJ[1][c] + dJ[2][c]);
+        wo[3 + c] = dJ[0][c];
+        wo[6 + c] = dJ[1][c];
+        wo[9 + c] = dJ[2][c];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.grad[(size_t)k * ne + e] += a.dt * (double)part[k];
+}
+
+// adjoint right-hand side and CG start: rhs = P lambda (tiled), r = rhs, x = 0, z = D^-1 r,
+// chunk partials r.z and |rhs|^2 as k_gather leaves them
+struct AdjStartArgs {
+    RowArgs ra;
+    const double *lam;            // tiled
+    const unsigned char *pinned;
+    const double *dinv;
+    double *rhs, *r, *z, *x, *P;
+};
+__global__ __launch_bounds__(kBlock) void k_adjoint_start(AdjStartArgs a) {
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.ra.num_done = 0;
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        double v[2] = {0.0, 0.0};
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            const bool pin = a.pinned[n] != 0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double f = pin ? 0.0 : a.lam[vi(n, k)];
+                const double zz = a.dinv[vi(n, k)] * f;
+                a.rhs[vi(n, k)] = f;
+                a.r[vi(n, k)] = f;
+                a.x[vi(n, k)] = 0.0;
+                a.z[vi(n, k)] = zz;
+                v[0] += f * zz;
+                v[1] += f * f;
+            }
+        }
+        store_partials<2>(v, a.P, a.ra.nchunks, c);
+        if (c == ch.first && threadIdx.x == 0) {
+            CgScalars *S = a.ra.S + ch.blob;
+            S->inverted_step = 0;
+            S->done = 0;
+            S->converged = 0;
+            S->iter = 0;
+            S->updated = 1;  // an adjoint solve never moves the blob
+        }
+    }
+}
+
+// lambda += dt * sum over the node's element corners (element order) of w
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_adjoint_gather(RowArgs ra, const uint32_t *inc_ptr, const uint32_t *inc,
+                                                           const T *w, double dt, double *lam) {
+    const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = ra.chunks[c];
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+            double s[3] = {0.0, 0.0, 0.0};
+            for (uint32_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+                const uint32_t es = inc[k];
+                const T *src = w + (size_t)12 * (es >> 2) + 3 * (es & 3);
+                s[0] += (double)src[0];
+                s[1] += (double)src[1];
+                s[2] += (double)src[2];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) lam[vi(n, k)] += dt * s[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- copy probe

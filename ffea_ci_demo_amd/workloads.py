@@ -127,15 +127,15 @@ def config4(n: int = 40, **kw) -> BlobSpec:
 
 def ensemble(num_blobs: int = 512, *, nmin: int = 8, nmax: int = 40, moduli=(1e7, 1e9), seed: int = 2024,
              package_seed=(12345, 12345, 12345, 12345, 12345, 12345), **kw) -> list:
-    """Ensemble of independent jittered-cube blobs: blob b has n_b^3 cubes with n_b drawn
-    log-uniform in [nmin, nmax], its shear and bulk moduli drawn log-uniform in `moduli`
+    """Ensemble of independent jittered-cube blobs: blob b has n_b^3 cubes with
+    n_b = round(exp(U(ln nmin, ln nmax))), its shear and bulk moduli drawn independently
+    log-uniform in `moduli`
     (Pa), its own mesh jitter seed, and noise stream b of `package_seed` (the b-th
     RngStream after SetPackageSeed, streams 2^127 draws apart; RngStream.cpp:327-345)."""
     from . import _native  # host-only jump-ahead of the library
 
     r = np.random.default_rng(seed)
-    lo, hi = np.log(nmin), np.log(nmax + 1)
-    ns = np.minimum(np.floor(np.exp(r.uniform(lo, hi, num_blobs))).astype(int), nmax)
+    ns = np.rint(np.exp(r.uniform(np.log(nmin), np.log(nmax), num_blobs))).astype(int)
     Gs = np.exp(r.uniform(np.log(moduli[0]), np.log(moduli[1]), num_blobs))
     Es = np.exp(r.uniform(np.log(moduli[0]), np.log(moduli[1]), num_blobs))
     specs = []

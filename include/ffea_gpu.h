@@ -188,6 +188,26 @@ int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *blob, int index, const uint32_t
 
 int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
 
+/* Differentiation of a blob trajectory (single-blob handles, calc_noise = 0, some nodes
+ * pinned). No counterpart in the reference.
+ *   ffea_gpu_blob_set_material   replaces the per-element A, B, G, E (meso units, .top order)
+ *   ffea_gpu_blob_run_checkpointed  nsteps steps of ffea_gpu_blob_step from the current
+ *                                state, keeping the positions of every interval-th step on
+ *                                the device
+ *   ffea_gpu_blob_adjoint        reverse of that run (discrete adjoint: each step's
+ *                                x' = x + dt K(x)^-1 P f(x) reversed exactly, one CG solve
+ *                                with K per step, segments recomputed from the checkpoints):
+ *                                from dL/dpos at the end of the run ([num_nodes][3], .node
+ *                                order) it returns dL/dA, dL/dB, dL/dG, dL/dE ([num_elements]
+ *                                each, any may be NULL) and dL/dpos at the start (may be
+ *                                NULL; zero on midside nodes, which are derived from the
+ *                                corners). The blob is left in the state at the end of the run. */
+int ffea_gpu_blob_set_material(ffea_gpu_blob *blob, const double *A, const double *B, const double *G,
+                               const double *E);
+int ffea_gpu_blob_run_checkpointed(ffea_gpu_blob *blob, int nsteps, int interval);
+int ffea_gpu_blob_adjoint(ffea_gpu_blob *blob, const double *dL_dpos, double *dL_dA, double *dL_dB,
+                          double *dL_dG, double *dL_dE, double *dL_dpos0);
+
 /* Launch shape of the row kernels. Totals are summed per reduction chunk in a fixed
  * order, so the shape never changes results. ffea_gpu_blob_set_row_blocks sets the
  * number of workgroups (rounded to whole chunks per workgroup and a multiple of 8, one
