@@ -183,7 +183,9 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="cubes per side (100: the 6.0M-tet blob)")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--headline", default=None, help="form reported as value (default: last of --forms)")
-    ap.add_argument("--workload", default="cube", choices=("cube", "ensemble"))
+    ap.add_argument("--workload", default="cube", choices=("cube", "ensemble", "adjoint", "illcond", "drift"))
+    ap.add_argument("--run-steps", type=int, default=None,
+                    help="adjoint: trajectory length K (200); illcond/drift: steps compared (2000/1000)")
     ap.add_argument("--blobs", type=int, default=512, help="ensemble: number of blobs")
     ap.add_argument("--cpu-cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -196,6 +198,10 @@ def main():
 
     if args.workload == "ensemble":
         return main_ensemble(args)
+    if args.workload == "adjoint":
+        return main_adjoint(args)
+    if args.workload in ("illcond", "drift"):
+        return main_drift(args)
     t0 = time.perf_counter()
     spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
     print(f"[bench] mesh n={args.n} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -316,6 +322,143 @@ def main_ensemble(args):
         "cpu_baseline": cpu,
         "forms": {form: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in h.items() if k != "info"}},
     }
+    print(json.dumps(out), flush=True)
+
+
+def _line(value, unit, ms, steps, warmup, dtype, config, forms, higher=True, roof=None, cpu=None):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": cpu, "forms": forms}
+
+
+def main_adjoint(args):
+    """Differentiation blob: 50^3 jittered cubes (750k tets), per-element log-normal moduli,
+    bottom face pinned, +1 pN z on the top-face corner nodes, noise off. Target field: the
+    positions after K steps with a second moduli seed. Loss: sum over corner nodes of
+    |pos_K - target|^2. Gradient in G, E, A, B per element by the adjoint with checkpoints
+    every 20 steps, FP64 and mixed; a directional central difference of the FP64 forward
+    checks the FP64 gradient."""
+    from ffea_ci_demo_amd import workloads
+    from ffea_ci_demo_amd.blob import GpuBlob
+    n = args.n if args.n != 100 else 50
+    K = args.run_steps or 200
+    interval = 20
+    spec = workloads.config3(n=n)
+    tgt_spec = workloads.config3(n=n, moduli_seed=12)
+    corner = spec.mesh.is_linear
+    with GpuBlob.from_spec(tgt_spec) as g:
+        g.step(K)
+        target = g.positions()
+
+    def loss(pos):
+        d = (pos - target) * corner[:, None]
+        return float((d ** 2).sum()), 2.0 * d
+
+    forms, grads = {}, {}
+    for form in ("fp64", "mixed"):
+        with GpuBlob.from_spec(spec, precision=form) as g:
+            g.step(2)  # warm-up (kernels loaded, iteration count learned)
+            g.set_positions(spec.pos)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.run_checkpointed(K, interval)
+            t_fwd = time.perf_counter() - t0
+            _, it_total0 = g.cg_iterations()
+            L, seed = loss(g.positions())
+            t0 = time.perf_counter()
+            dA, dB, dG, dE, _ = g.adjoint(seed)
+            t_bwd = time.perf_counter() - t0
+            grads[form] = {"A": dA, "B": dB, "G": dG, "E": dE}
+            forms[form] = {"loss": L, "forward_ms_per_step": t_fwd / K * 1e3, "reverse_ms_per_step": t_bwd / K * 1e3,
+                           "gradient_element_steps_per_s": spec.num_elements * K / (t_fwd + t_bwd)}
+            print(f"[bench] adjoint {form}: forward {t_fwd:.2f}s reverse {t_bwd:.2f}s loss {L:.6e}",
+                  file=sys.stderr, flush=True)
+            if form == "fp64":
+                r = np.random.default_rng(7)
+                d = r.uniform(-1.0, 1.0, spec.num_elements)
+                h = 1e-4
+                vals = []
+                for sgn in (1.0, -1.0):
+                    g.set_material(spec.A, spec.B, spec.G * (1 + sgn * h * d), spec.E)
+                    g.set_positions(spec.pos)
+                    g.step(K)
+                    vals.append(loss(g.positions())[0])
+                fd = (vals[0] - vals[1]) / (2 * h)
+                ad = float((dG * spec.G * d).sum())
+                forms[form]["fd_check_G_direction_rel_err"] = abs(fd - ad) / abs(fd)
+    for name in "ABGE":
+        a, b = grads["mixed"][name], grads["fp64"][name]
+        forms["mixed"][f"grad_{name}_rel_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    h = forms["fp64"]
+    out = _line(round(h["gradient_element_steps_per_s"], 1), "element-steps/s", round(h["forward_ms_per_step"] +
+                h["reverse_ms_per_step"], 3), K, 2, "f64",
+                {"workload": f"differentiation blob: {n}^3 jittered cubes, {spec.num_elements} tets, log-normal G/E "
+                             "(median 3.6e8/2.2e9 Pa, sigma 0.5), bottom face pinned, +1 pN z on top-face corner "
+                             f"nodes, noise off; loss vs the positions of a second moduli seed after K={K} steps; "
+                             f"adjoint with checkpoints every {interval} steps; value = (forward + reverse) "
+                             "element-steps/s of the FP64 gradient", "form": "fp64", "parallelism": "single GPU"},
+                {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
+                 for f, r in forms.items()})
+    print(json.dumps(out), flush=True)
+
+
+def main_drift(args):
+    """FP64 and mixed handles stepped side by side on the same blob; the mixed form's
+    position error against FP64 and both CG iteration counts along the run"""
+    from ffea_ci_demo_amd import workloads
+    from ffea_ci_demo_amd.blob import GpuBlob
+    if args.workload == "illcond":
+        spec = workloads.config4(n=args.n if args.n != 100 else 40)
+        K = args.run_steps or 2000
+    else:
+        spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
+        K = args.run_steps or 1000
+    marks = sorted({m for m in [1, 10, K] + list(range(50, K + 1, 50)) if m <= K})
+    d = GpuBlob.from_spec(spec, precision="fp64")
+    m = GpuBlob.from_spec(spec, precision="mixed")
+    d.use_graph(True)
+    m.use_graph(True)
+    scale = np.abs(spec.pos).max()
+    from ffea_ci_demo_amd.blob import FFEAException
+    rows, t_d, t_m, done, aborted = [], 0.0, 0.0, 0, None
+    for mark in marks:
+        n = mark - done
+        t0 = time.perf_counter()
+        try:
+            d.step(n)
+            d.synchronize()
+            t1 = time.perf_counter()
+            m.step(n)
+            m.synchronize()
+        except FFEAException as ex:  # the reference aborts the same way (Blob.cpp:484-490, NoMassCGSolver.cpp:187)
+            aborted = {"after_step": done, "within_next": n, "message": str(ex).strip()}
+            print(f"[bench] aborted after step {done} (+<= {n}): {str(ex).strip()}", file=sys.stderr, flush=True)
+            break
+        t2 = time.perf_counter()
+        t_d += t1 - t0
+        t_m += t2 - t1
+        done = mark
+        pd, pm = d.positions(), m.positions()
+        err = float(np.abs(pm - pd).max() / np.abs(pd).max())
+        rows.append({"step": mark, "mixed_pos_rel_err": err, "k_fp64": d.cg_iterations()[0],
+                     "k_mixed": m.cg_iterations()[0]})
+        print(f"[bench] {spec.name} step {mark}: mixed rel err {err:.3e} k {rows[-1]['k_fp64']}/{rows[-1]['k_mixed']}",
+              file=sys.stderr, flush=True)
+    d.close()
+    m.close()
+    K = done
+    forms = {"fp64": {"element_steps_per_s": spec.num_elements * K / t_d, "ms_per_step": t_d / K * 1e3},
+             "mixed": {"element_steps_per_s": spec.num_elements * K / t_m, "ms_per_step": t_m / K * 1e3,
+                       "drift": rows}}
+    if aborted:
+        forms["aborted"] = aborted
+    out = _line(round(forms["mixed"]["element_steps_per_s"], 1), "element-steps/s",
+                round(forms["mixed"]["ms_per_step"], 3), K, 0, "f32+f64",
+                {"workload": f"{spec.name}: {spec.num_elements} tets, {K} steps of FP64 and FP32-mixed side by side "
+                             "(graph-captured steps); positions relative error max|mixed-fp64|/max|fp64|",
+                 "form": "mixed", "parallelism": "single GPU"},
+                {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
+                 for f, r in forms.items()})
     print(json.dumps(out), flush=True)
 
 

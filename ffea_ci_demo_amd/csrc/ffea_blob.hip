@@ -676,7 +676,7 @@ struct ffea_gpu_blob {
     DevBuf<long long> slice_base;
     DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, fold_ptr, rng;
     DevBuf<unsigned char> diag_j, pinned;
-    DevBuf<double> pos, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
+    DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
     DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
     DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
@@ -859,6 +859,7 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     grid_blobs = (nblob + kBlock - 1) / kBlock;
     conn.alloc(t.conn.size(), &dev_bytes); conn.upload(t.conn, stream);
     pos.alloc(t.pos.size(), &dev_bytes); pos.upload(t.pos, stream);
+    rest.alloc(t.pos.size(), &dev_bytes); rest.upload(t.pos, stream);
     vel.alloc(t.pos.size(), &dev_bytes); vel.zero(stream);
     last_det.alloc(t.det.size(), &dev_bytes); last_det.upload(t.det, stream);
     inc_ptr.alloc(t.ip.size(), &dev_bytes); inc_ptr.upload(t.ip, stream);
@@ -919,7 +920,7 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     FFEA_HIP_TRY(hipHostMalloc(&h_done, sizeof(int) * 2, hipHostMallocDefault));
     for (auto &e : chunk_ev) FFEA_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // every buffer a kernel reads must exist
-    const void *need[] = {conn.p, pos.p, vel.p, last_det.p, inc_ptr.p, inc.p, pinned.p, slice_w.p, slice_base.p,
+    const void *need[] = {conn.p, pos.p, rest.p, vel.p, last_det.p, inc_ptr.p, inc.p, pinned.p, slice_w.p, slice_base.p,
                           col.p, cptr.p, contrib.p, diag_j.p, elem_off.p, project.p, dinv.p, rhs.p, r.p, z.p, x.p,
                           q.p, p.p, Pg.p, Pi.p, P1.p, P2.p, jump_group.p, jump_lane.p, step_jump.p, rng.p, S.p,
                           chunks.p, seg.p, centre.p, Gmat.p, num_done.p,
@@ -956,12 +957,12 @@ void ffea_gpu_blob::launch_element() {
     if (mixed) {
         ElemArgs<float> a{ne, conn.p, pos.p, f_jinv0.p, f_vol0.p, f_G.p, f_E.p, f_A.p, f_B.p, last_det.p, f_force.p,
                           f_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
-                          rng.p, S.p};
+                          rng.p, S.p, rest.p};
         hipLaunchKernelGGL(k_element<float>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     } else {
         ElemArgs<double> a{ne, conn.p, pos.p, d_jinv0.p, d_vol0.p, d_G.p, d_E.p, d_A.p, d_B.p, last_det.p, d_force.p,
                            d_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
-                           rng.p, S.p};
+                           rng.p, S.p, nullptr};  // FP64: the reference's arithmetic
         hipLaunchKernelGGL(k_element<double>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
@@ -1263,7 +1264,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     if (!h_S[0].converged)
         throw Invalid(fmt("adjoint solve: conjugate gradient could not converge after %d iterations",
                           prm.max_iterations_cg));
-    AdjArgs<T> a{ne, conn.p, pos.p, x.p, vel_k, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, prm.dt, w,
+    AdjArgs<T> a{ne, conn.p, pos.p, rest.p, x.p, vel_k, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, prm.dt, w,
                  grad.p};
     if constexpr (std::is_same<T, double>::value) {
         a.jinv0 = d_jinv0.p; a.vol0 = d_vol0.p; a.G = d_G.p; a.E = d_E.p; a.A = d_A.p; a.B = d_B.p;

@@ -12,6 +12,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mrg32k3a.h"
 
 namespace ffk {
@@ -187,7 +189,29 @@ struct ElemArgs {
     const mrg::Jump *jump_lane;   // [64]: A^(7*l)
     const uint32_t *rng;          // [nblob][6]: stream state at the start of this pass
     CgScalars *S;                 // [nblob]
+    const double *rest;           // [nl][3] rest positions (mixed form: cancellation-free stress)
 };
+
+// Elastic stress without cancellation, for FP32: with H = F - I formed from displacements
+// (F - I = (J - J_0)^T J_0^-T, the edge displacements taken in FP64 before rounding),
+// F F^T - I = H + H^T + H H^T and det F - 1 = tr H + ((tr H)^2 - tr H^2)/2 + det H, the
+// reference's stress (tetra_element_linear.cpp:375-403)
+//   s = (G v0/vol) F F^T - G I + [G (1 - v0/vol) + (E - 2G/3)(vol/v0 - v0/vol)/2] I
+// is, exactly, s = (G/j)(F F^T - I) + (E - 2G/3) e (2 + e) / (2 j) I with j = 1 + e = vol/v0.
+template <typename S, typename T>
+__device__ __forceinline__ void strain_terms(const S (&H)[3][3], S (&FFm)[3][3], S *e) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            FFm[i][j] = H[i][j] + H[j][i] + (H[i][0] * H[j][0] + H[i][1] * H[j][1] + H[i][2] * H[j][2]);
+    const S tr = H[0][0] + H[1][1] + H[2][2];
+    S tr2 = H[0][0] * H[0][0];
+    tr2 = tr2 + H[1][1] * H[1][1] + H[2][2] * H[2][2] + (H[0][1] * H[1][0] + H[0][2] * H[2][0] + H[1][2] * H[2][1]) * (T)2;
+    const S det = H[0][0] * (H[1][1] * H[2][2] - H[1][2] * H[2][1]) - H[0][1] * (H[1][0] * H[2][2] - H[1][2] * H[2][0]) +
+                  H[0][2] * (H[1][0] * H[2][1] - H[1][1] * H[2][0]);
+    *e = tr + (tr * tr - tr2) * (T)0.5 + det;
+}
 
 // tetra_element_linear.cpp:191-443 and :468-483 for one element (mass-less solver:
 // du starts at zero). Arithmetic in the reference's order, no contraction.
@@ -275,16 +299,38 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
             s[i][j] = acc;
         }
     const T G = a.G[e], E = a.E[e], v0 = a.vol0[e];
-    const T sc = G * v0 / vol;
+    if (std::is_same<T, float>::value && a.rest && ld * (double)det >= 0) {
+        // FP32: the same stress from the displacement gradient (see strain_terms)
+        double u0[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 3; ++c) u0[c] = x0[c] - a.rest[3 * n[0] + c];
+        T DJ[3][3], H[3][3], FFm[3][3], ee;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s[i][j] *= sc;
-    s[0][0] -= G;
-    s[1][1] -= G;
-    s[2][2] -= G;
-    // bulk elastic stress (:397-403)
-    {
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                DJ[r][c] = (T)((a.pos[3 * n[r + 1] + c] - a.rest[3 * n[r + 1] + c]) - u0[c]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) H[i][j] = DJ[0][i] * Ji[j][0] + DJ[1][i] * Ji[j][1] + DJ[2][i] * Ji[j][2];
+        strain_terms<T, T>(H, FFm, &ee);
+        const T jr = (T)1 / ((T)1 + ee);
+        const T bulk = (E - G * (T)2.0 / (T)3.0) * ee * ((T)2 + ee) * (T)0.5 * jr;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s[i][j] = G * jr * FFm[i][j] + (i == j ? bulk : (T)0);
+    } else {
+        const T sc = G * v0 / vol;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s[i][j] *= sc;
+        s[0][0] -= G;
+        s[1][1] -= G;
+        s[2][2] -= G;
+        // bulk elastic stress (:397-403)
         const T c2 = E - G * (T)2.0 / (T)3.0;
         const T c = G * ((T)1.0 - (v0 / vol)) + (T)0.5 * c2 * ((vol / v0) - (v0 / vol));
         s[0][0] += c;
@@ -905,8 +951,8 @@ __device__ __forceinline__ T value_of(T a) { return a; }
 // mu . f_el(J) - mu^T K(J) v for one element (noise off, element not inverted) and, for
 // the value pass, its partial derivatives in G, E, A, B
 template <typename S, typename T>
-__device__ __forceinline__ S adjoint_scalar(const S (&J)[3][3], const T (&Ji0)[3][3], T v0, T G, T E, T A, T B,
-                                            const T (&mu)[4][3], const T (&vel)[4][3], T *part) {
+__device__ __forceinline__ S adjoint_scalar(const S (&J)[3][3], const S (&DJ)[3][3], const T (&Ji0)[3][3], T v0, T G,
+                                            T E, T A, T B, const T (&mu)[4][3], const T (&vel)[4][3], T *part) {
     S d[12];
     d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
     d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
@@ -926,16 +972,13 @@ __device__ __forceinline__ S adjoint_scalar(const S (&J)[3][3], const T (&Ji0)[3
     d[0] = -(d[1] + d[2] + d[3]);
     d[4] = -(d[5] + d[6] + d[7]);
     d[8] = -(d[9] + d[10] + d[11]);
-    // F = J^T Ji0^T and F F^T
-    S F[3][3], FF[3][3];
+    // H = F - I from the edge displacements DJ = J - J_0; F F^T - I and e = det F - 1 (strain_terms)
+    S H[3][3], FF[3][3], e;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) F[i][j] = J[0][i] * Ji0[j][0] + J[1][i] * Ji0[j][1] + J[2][i] * Ji0[j][2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) FF[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+        for (int j = 0; j < 3; ++j) H[i][j] = DJ[0][i] * Ji0[j][0] + DJ[1][i] * Ji0[j][1] + DJ[2][i] * Ji0[j][2];
+    strain_terms<S, T>(H, FF, &e);
     // velocity gradients L[i][j] = sum_q w_q[i] grad psi_q[j], grad psi_q[j] = d[4j+q]
     S Lm[3][3], Lv[3][3];
 #pragma unroll
@@ -946,15 +989,17 @@ __device__ __forceinline__ S adjoint_scalar(const S (&J)[3][3], const T (&Ji0)[3
             Lv[i][j] = vel[0][i] * d[4 * j] + vel[1][i] * d[4 * j + 1] + vel[2][i] * d[4 * j + 2] +
                        vel[3][i] * d[4 * j + 3];
         }
-    const S r0 = v0 / vol, r1 = vol * ((T)1 / v0);
     const S trm = Lm[0][0] + Lm[1][1] + Lm[2][2];
     const S trv = Lv[0][0] + Lv[1][1] + Lv[2][2];
-    // stress = G sG + E sE: sG = r0 FF^T + (-r0 - (r1 - r0)/3) I, sE = (r1 - r0)/2 I
+    // stress = G sG + E sE with j = 1 + e = vol/v0:
+    //   sG = (F F^T - I)/j - e (2 + e)/(3 j) I,   sE = e (2 + e)/(2 j) I
     S ffm = FF[0][0] * Lm[0][0];
     ffm = ffm + FF[0][1] * Lm[0][1] + FF[0][2] * Lm[0][2] + FF[1][0] * Lm[1][0] + FF[1][1] * Lm[1][1] +
           FF[1][2] * Lm[1][2] + FF[2][0] * Lm[2][0] + FF[2][1] * Lm[2][1] + FF[2][2] * Lm[2][2];
-    const S sG_m = r0 * ffm - (r0 + (r1 - r0) * (T)(1.0 / 3.0)) * trm;  // sG : L_mu
-    const S sE_m = (r1 - r0) * (T)0.5 * trm;                             // sE : L_mu
+    const S jr = (T)1 / (e + (T)1);
+    const S vb = e * (e + (T)2) * jr;                            // vol/v0 - v0/vol
+    const S sG_m = jr * ffm - vb * trm * (T)(1.0 / 3.0);         // sG : L_mu
+    const S sE_m = vb * trm * (T)0.5;                            // sE : L_mu
     S sym = Lv[0][0] * Lm[0][0] * (T)2;
     sym = sym + (Lv[0][1] + Lv[1][0]) * (Lm[0][1] + Lm[1][0]) + (Lv[0][2] + Lv[2][0]) * (Lm[0][2] + Lm[2][0]) +
           (Lv[1][2] + Lv[2][1]) * (Lm[1][2] + Lm[2][1]) + Lv[1][1] * Lm[1][1] * (T)2 + Lv[2][2] * Lm[2][2] * (T)2;
@@ -975,6 +1020,7 @@ struct AdjArgs {
     int ne;
     const int *conn;      // [4][ne]
     const double *pos;    // [rows][3] positions at the step
+    const double *rest;   // [rows][3] rest positions
     const double *mu;     // tiled: the adjoint solve
     const double *vel;    // [rows][3]: the step's velocity solution
     const T *jinv0, *vol0, *G, *E, *A, *B;
@@ -993,14 +1039,20 @@ __global__ __launch_bounds__(kBlock) void k_element_adjoint(AdjArgs<T> a) {
     int n[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
-    double x0[3];
-    T Jv[3][3], mu[4][3], vel[4][3], Ji0[3][3];
+    double x0[3], u0[3];
+    T Jv[3][3], DJv[3][3], mu[4][3], vel[4][3], Ji0[3][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) x0[c] = a.pos[3 * n[0] + c];
+    for (int c = 0; c < 3; ++c) {
+        x0[c] = a.pos[3 * n[0] + c];
+        u0[c] = x0[c] - a.rest[3 * n[0] + c];
+    }
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Jv[r][c] = (T)(a.pos[3 * n[r + 1] + c] - x0[c]);
+        for (int c = 0; c < 3; ++c) {
+            Jv[r][c] = (T)(a.pos[3 * n[r + 1] + c] - x0[c]);
+            DJv[r][c] = (T)((a.pos[3 * n[r + 1] + c] - a.rest[3 * n[r + 1] + c]) - u0[c]);
+        }
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -1012,22 +1064,19 @@ __global__ __launch_bounds__(kBlock) void k_element_adjoint(AdjArgs<T> a) {
     for (int k = 0; k < 9; ++k) Ji0[k / 3][k % 3] = a.jinv0[(size_t)k * ne + e];
     const T v0 = a.vol0[e], G = a.G[e], E = a.E[e], A = a.A[e], B = a.B[e];
     T part[4];
-    {
-        T J[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) J[r][c] = Jv[r][c];
-        (void)adjoint_scalar<T, T>(J, Ji0, v0, G, E, A, B, mu, vel, part);
-    }
+    (void)adjoint_scalar<T, T>(Jv, DJv, Ji0, v0, G, E, A, B, mu, vel, part);
     T dJ[3][3];
     for (int k = 0; k < 9; ++k) {
-        Dual<T> J[3][3];
+        Dual<T> J[3][3], DJ[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) J[r][c] = Dual<T>{Jv[r][c], (T)((3 * r + c) == k ? 1 : 0)};
-        dJ[k / 3][k % 3] = adjoint_scalar<Dual<T>, T>(J, Ji0, v0, G, E, A, B, mu, vel, nullptr).d;
+            for (int c = 0; c < 3; ++c) {
+                const T t = (T)((3 * r + c) == k ? 1 : 0);
+                J[r][c] = Dual<T>{Jv[r][c], t};
+                DJ[r][c] = Dual<T>{DJv[r][c], t};
+            }
+        dJ[k / 3][k % 3] = adjoint_scalar<Dual<T>, T>(J, DJ, Ji0, v0, G, E, A, B, mu, vel, nullptr).d;
     }
     // J rows are x_{r+1} - x_0
     T *wo = a.w + (size_t)12 * e;

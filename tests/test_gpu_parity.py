@@ -183,3 +183,34 @@ def test_blob_from_ffea_script(oracle_mod, tmp_path):
     g.step(10)
     assert rel_err(g.positions(), o.positions()) <= POS_TOL
     assert np.array_equal(g.rng_state(), o.rng_state())
+
+
+def test_mixed_element_forces_at_small_strain():
+    """the FP32 element stress is formed from the displacement gradient: at 1e-4 strain
+    the mixed element forces stay within FP32 rounding of the FP64 ones (the direct
+    formula would lose ~4 digits to cancellation)"""
+    spec = workloads.config0(calc_noise=0)
+    r = np.random.default_rng(4)
+    p = spec.pos * np.array([1.0, 1.0, 1.0001]) + r.uniform(-1e-4, 1e-4, spec.pos.shape)
+    out = {}
+    for form in ("fp64", "mixed"):
+        with GpuBlob.from_spec(spec, precision=form) as g:
+            g.set_positions(p)
+            g.update_internal_forces()
+            out[form] = g.element_forces()
+    e = rel_err(out["mixed"], out["fp64"])
+    print("mixed element force rel err at 1e-4 strain:", e)
+    assert e <= 2e-6
+
+
+def test_config0_1000_steps_noise_on(oracle_mod):
+    """the 1296-tet blob, 1000 steps with thermal noise: positions within 1e-10"""
+    spec = workloads.config0(rng_seed=1)
+    o, g = make_pair(oracle_mod, spec)
+    for _ in range(1000):
+        assert o.step() == 0
+    g.step(1000)
+    e = rel_err(g.positions(), o.positions())
+    print("config0 1000 steps position rel err:", e)
+    assert e <= POS_TOL
+    assert np.array_equal(g.rng_state(), o.rng_state())
