@@ -735,9 +735,14 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
             auto block = [&](int j, int cc) {
                 const T *vv = vp + (size_t)j * 9 * kSlice;
                 const double z0 = a.z[vi(cc, 0)], z1 = a.z[vi(cc, 1)], z2 = a.z[vi(cc, 2)];
-                acc0 += (double)vv[0 * kSlice] * z0; acc0 += (double)vv[1 * kSlice] * z1; acc0 += (double)vv[2 * kSlice] * z2;
-                acc1 += (double)vv[3 * kSlice] * z0; acc1 += (double)vv[4 * kSlice] * z1; acc1 += (double)vv[5 * kSlice] * z2;
-                acc2 += (double)vv[6 * kSlice] * z0; acc2 += (double)vv[7 * kSlice] * z1; acc2 += (double)vv[8 * kSlice] * z2;
+                // the matrix streams through once per iteration: non-temporal loads keep it
+                // from evicting the gathered z and the p, q the vector step reads next
+                T w[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) w[k] = __builtin_nontemporal_load(vv + k * kSlice);
+                acc0 += (double)w[0] * z0; acc0 += (double)w[1] * z1; acc0 += (double)w[2] * z2;
+                acc1 += (double)w[3] * z0; acc1 += (double)w[4] * z1; acc1 += (double)w[5] * z2;
+                acc2 += (double)w[6] * z0; acc2 += (double)w[7] * z1; acc2 += (double)w[8] * z2;
             };
             if (W <= kMaxUnrolledRow) {
                 // all block columns of the row first, so the gathers and value loads of every
@@ -745,13 +750,13 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
                 int cc[kMaxUnrolledRow];
 #pragma unroll
                 for (int j = 0; j < kMaxUnrolledRow; ++j)
-                    if (j < W) cc[j] = colp[(size_t)j * kSlice];
+                    if (j < W) cc[j] = __builtin_nontemporal_load(colp + (size_t)j * kSlice);
 #pragma unroll
                 for (int j = 0; j < kMaxUnrolledRow; ++j)
                     if (j < W) block(j, cc[j]);
             } else {
 #pragma unroll 2
-                for (int j = 0; j < W; ++j) block(j, colp[(size_t)j * kSlice]);
+                for (int j = 0; j < W; ++j) block(j, __builtin_nontemporal_load(colp + (size_t)j * kSlice));
             }
             double pn[3], qn[3] = {acc0, acc1, acc2};
 #pragma unroll
