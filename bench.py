@@ -221,8 +221,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
-        if t.get("workload") == spec.name and t.get("form") == head:
-            traffic = t.get("hbm_bytes_per_launch")
+        if t.get("workload") == spec.name and head in t.get("forms", {}):
+            traffic = t["forms"][head]["hbm_bytes_per_launch"]
     roof = {
         "bound": "hbm",
         "kernel": "k_cg_spmv",

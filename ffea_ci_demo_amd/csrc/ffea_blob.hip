@@ -678,6 +678,7 @@ struct ffea_gpu_blob {
     DevBuf<unsigned char> diag_j, pinned;
     DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
+    DevBuf<double> coef;            // [nblob][8]: projected-form step coefficients
     DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
     DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
     DevBuf<mrg::Jump> jump_group, jump_lane, step_jump;
@@ -833,7 +834,7 @@ void ffea_gpu_blob::upload_elements(const Tables &t) {
     bf->zero(stream);
     // shape data: the volume row is read for the previous volume of an inverted element
     std::vector<double> g0((size_t)10 * ne, 0.0);
-    std::copy(t.vol0.begin(), t.vol0.end(), g0.begin() + (size_t)9 * ne);
+    for (int e = 0; e < ne; ++e) g0[(size_t)10 * e + 9] = t.vol0[e];
     upload_converted(*bg, g0, stream, &dev_bytes);
     bval->alloc((size_t)nslots * 9, &dev_bytes);
     bval->zero(stream);
@@ -911,6 +912,7 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     seg.alloc(t.seg.size(), &dev_bytes); seg.upload(t.seg, stream);
     centre.alloc(t.centre.size(), &dev_bytes); centre.upload(t.centre, stream);
     Gmat.alloc((size_t)36 * nblob, &dev_bytes); Gmat.zero(stream);
+    coef.alloc((size_t)8 * nblob, &dev_bytes); coef.zero(stream);
     num_done.alloc(1, &dev_bytes); num_done.zero(stream);
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     if (mixed) upload_elements<float>(t);
@@ -1047,9 +1049,14 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
 void ffea_gpu_blob::launch_cg_vec(int it) {
     hipEvent_t tv = nullptr;
     if (t_cg(4, it)) t_begin(4, &tv);
-    VecArgs va{row_args(), it, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
-    if (mixed && any_project) hipLaunchKernelGGL(k_cg_vec<true>, dim3(nb), dim3(kBlock), 0, stream, va);
-    else hipLaunchKernelGGL(k_cg_vec<false>, dim3(nb), dim3(kBlock), 0, stream, va);
+    VecArgs va{row_args(), it, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
+    if (mixed && any_project) {
+        hipLaunchKernelGGL(k_cg_coef, dim3(nblob), dim3(kBlock), 0, stream, row_args(), it, (const double *)P1.p,
+                           (const double *)Gmat.p, coef.p);
+        hipLaunchKernelGGL(k_cg_vec<true>, dim3(nb), dim3(kBlock), 0, stream, va);
+    } else {
+        hipLaunchKernelGGL(k_cg_vec<false>, dim3(nb), dim3(kBlock), 0, stream, va);
+    }
     t_end(4, tv, it);
 }
 

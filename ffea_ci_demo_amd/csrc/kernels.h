@@ -31,6 +31,16 @@ __host__ __device__ __forceinline__ size_t vi(int n, int k) {
     return ((size_t)(n >> 6) * 3 + (size_t)k) * 64 + (size_t)(n & 63);
 }
 
+// Block values of one slot (64 rows x 9 entries) are ordered so that a lane reads its 9
+// entries as 16-byte vectors: FP64 as pairs (0,1) (2,3) (4,5) (6,7) then 8, FP32 as
+// quads (0..3) (4..7) then 8. Each wave load is one contiguous 1 KiB span.
+template <typename T>
+__host__ __device__ __forceinline__ size_t val_index(long long slot_base, int j, int lane, int k) {
+    const size_t b = (size_t)slot_base * 9 + (size_t)j * 9 * kSlice;
+    constexpr int per = 16 / sizeof(T);  // entries per 16-byte vector
+    return k < 8 ? b + (size_t)(k / per) * per * kSlice + (size_t)lane * per + (k % per) : b + 8 * kSlice + lane;
+}
+
 // per-blob CG state (device memory)
 struct CgScalars {
     double delta[2];   // delta[(i-1)&1]: r.z entering iteration i (written by the SpMV step)
@@ -180,7 +190,7 @@ struct ElemArgs {
     const T *vol0, *G, *E, *A, *B;
     double *last_det;             // [ne]
     T *force;                     // [ne][12]
-    T *geo;                       // [10][ne]: dpsi 1,2,3,5,6,7,9,10,11 then vol
+    T *geo;                       // [ne][10]: dpsi 1,2,3,5,6,7,9,10,11 then vol
     int calc_noise;
     double kT, dt;
     const int *elem_blob;         // [ne] blob of each element (null: one blob)
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
     T vol;
     if (ld * (double)det < 0) {
         __hip_atomic_fetch_add(&a.S[blob].inverted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vol = a.geo[(size_t)9 * ne + e];  // the reference keeps the previous volume
+        vol = a.geo[(size_t)10 * e + 9];  // the reference keeps the previous volume
     } else {
         a.last_det[e] = (double)det;
         vol = (T)(1.0 / 6.0) * (T)fabs(det);
@@ -371,9 +381,10 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) fo[k] = out[k];
     const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
+    T *go = a.geo + (size_t)10 * e;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a.geo[(size_t)k * ne + e] = d[gi[k]];
-    a.geo[(size_t)9 * ne + e] = vol;
+    for (int k = 0; k < 9; ++k) go[k] = d[gi[k]];
+    go[9] = vol;
 }
 
 // the pass consumed 7 draws per element of each blob (tetra_element_linear.cpp:416-426)
@@ -395,7 +406,7 @@ struct AsmArgs {
     const uint32_t *cptr;        // [nslots+1]
     const uint32_t *contrib;     // (e<<4)|(a<<2)|b, element order; kIdentity = identity block
     const unsigned char *diag_j; // [nrows] slot index j of the diagonal block
-    const T *geo;                // [10][ne]
+    const T *geo;                // [ne][10]
     const T *A, *B;
     T *val;                      // [nslots*9]
     double *dinv;                // tiled [nrows][3]
@@ -450,10 +461,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
             }
             const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
             T d[12];
-            d[1] = a.geo[(size_t)0 * ne + e]; d[2] = a.geo[(size_t)1 * ne + e]; d[3] = a.geo[(size_t)2 * ne + e];
-            d[5] = a.geo[(size_t)3 * ne + e]; d[6] = a.geo[(size_t)4 * ne + e]; d[7] = a.geo[(size_t)5 * ne + e];
-            d[9] = a.geo[(size_t)6 * ne + e]; d[10] = a.geo[(size_t)7 * ne + e]; d[11] = a.geo[(size_t)8 * ne + e];
-            const T vol = a.geo[(size_t)9 * ne + e];
+            const T *gr = a.geo + (size_t)10 * e;  // the element's shape record
+            d[1] = gr[0]; d[2] = gr[1]; d[3] = gr[2];
+            d[5] = gr[3]; d[6] = gr[4]; d[7] = gr[5];
+            d[9] = gr[6]; d[10] = gr[7]; d[11] = gr[8];
+            const T vol = gr[9];
             d[0] = -(d[1] + d[2] + d[3]);
             d[4] = -(d[5] + d[6] + d[7]);
             d[8] = -(d[9] + d[10] + d[11]);
@@ -463,9 +475,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) acc[3 * i + jj] += (double)visc_entry<T>(d, A, B, vol, i, pa, jj, pb);
         }
-        T *v = a.val + (size_t)base * 9 + (size_t)j * 9 * kSlice + lane;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[(size_t)k * kSlice] = (T)acc[k];
+        for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)acc[k];
         if (j == a.diag_j[n]) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
@@ -584,8 +595,10 @@ struct StartArgs {
 };
 __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    // (Z^T Z)^-1 and its product with Z^T f: formed once per blob by thread 0, shared
+    __shared__ double sGi[36], scf[6];
     int cur = -1;
-    double Gi[36], cf[6], f2 = 0.0;
+    double cf[6], f2 = 0.0;
     for (int c = cr.c0; c < cr.c1; ++c) {
         const Chunk ch = a.ra.chunks[c];
         if (ch.blob != cur) {
@@ -593,19 +606,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
             const int2 sg = a.ra.seg[cur];
             double t[17];
             sum_partials<17>(a.Pg, a.ra.nchunks, sg.x, sg.y, t);
-            double M[36];
-            const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
-            if (a.project[cur]) {
-                ztz_from_moments(mom, M);
-                invert6(M, Gi);
-            } else {
-                for (int i = 0; i < 36; ++i) Gi[i] = 0.0;
+            if (threadIdx.x == 0) {
+                double M[36], Gi[36];
+                const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
+                if (a.project[cur]) {
+                    ztz_from_moments(mom, M);
+                    invert6(M, Gi);
+                } else {
+                    for (int i = 0; i < 36; ++i) Gi[i] = 0.0;
+                }
+                for (int i = 0; i < 36; ++i) sGi[i] = Gi[i];
+                for (int i = 0; i < 6; ++i) {
+                    double acc = 0.0;
+                    for (int j = 0; j < 6; ++j) acc += Gi[6 * i + j] * t[1 + j];
+                    scf[i] = acc;
+                }
             }
-            for (int i = 0; i < 6; ++i) {
-                double acc = 0.0;
-                for (int j = 0; j < 6; ++j) acc += Gi[6 * i + j] * t[1 + j];
-                cf[i] = acc;
-            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 6; ++i) cf[i] = scf[i];
             f2 = t[0];
         }
         const double3 cen = a.centre[cur];
@@ -624,10 +643,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
             }
             v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
         }
-        store_partials<1>(v, a.P, a.ra.nchunks, c);
+        store_partials<1>(v, a.P, a.ra.nchunks, c);  // its block sum also orders the sGi reads before reuse
         if (c == ch.first && threadIdx.x == 0) {
             a.ra.S[cur].f2 = f2;
-            for (int i = 0; i < 36; ++i) a.G[36 * cur + i] = Gi[i];
+            for (int i = 0; i < 36; ++i) a.G[36 * cur + i] = sGi[i];
         }
     }
 }
@@ -657,6 +676,35 @@ struct SpmvArgs {
 // p = beta p + z (mat_vec_fns.cpp:202-212) and q = V p, formed as q = V z + beta q so
 // the row gathers only z (SparseMatrixFixedPattern.cpp:148-187), and the p.q partials
 // (NoMassCGSolver.cpp:251-265).
+typedef double vd2 __attribute__((ext_vector_type(2)));
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
+// one lane's 9 block entries of one slot (val_index order), non-temporal 16-byte loads
+template <typename T>
+__device__ __forceinline__ void load_block(const T *slot, int lane, T (&w)[9]);
+template <>
+__device__ __forceinline__ void load_block<double>(const double *slot, int lane, double (&w)[9]) {
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const vd2 v = __builtin_nontemporal_load((const vd2 *)(slot + h * 2 * kSlice) + lane);
+        w[2 * h] = v.x;
+        w[2 * h + 1] = v.y;
+    }
+    w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
+}
+template <>
+__device__ __forceinline__ void load_block<float>(const float *slot, int lane, float (&w)[9]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const vf4 v = __builtin_nontemporal_load((const vf4 *)(slot + h * 4 * kSlice) + lane);
+        w[4 * h] = v.x;
+        w[4 * h + 1] = v.y;
+        w[4 * h + 2] = v.z;
+        w[4 * h + 3] = v.w;
+    }
+    w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
+}
+
 template <typename T, bool PROJ>
 __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
@@ -730,7 +778,7 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
             const int W = a.slice_w[sl];
             const long long base = a.slice_base[sl];
             const int *colp = a.col + base + lane;
-            const T *vp = a.val + (size_t)base * 9 + lane;
+            const T *vp = a.val + (size_t)base * 9;
             double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
             auto block = [&](int j, int cc) {
                 const T *vv = vp + (size_t)j * 9 * kSlice;
@@ -738,8 +786,7 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
                 // the matrix streams through once per iteration: non-temporal loads keep it
                 // from evicting the gathered z and the p, q the vector step reads next
                 T w[9];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) w[k] = __builtin_nontemporal_load(vv + k * kSlice);
+                load_block<T>(vv, lane, w);
                 acc0 += (double)w[0] * z0; acc0 += (double)w[1] * z1; acc0 += (double)w[2] * z2;
                 acc1 += (double)w[3] * z0; acc1 += (double)w[4] * z1; acc1 += (double)w[5] * z2;
                 acc2 += (double)w[6] * z0; acc2 += (double)w[7] * z1; acc2 += (double)w[8] * z2;
@@ -805,6 +852,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 struct VecArgs {
     RowArgs ra;
     int iter;
+    const double *coef;  // [nblob][8]: alpha, (Z^T Z)^-1 Z^T q of the projected form
     const double *p, *q, *dinv;
     const double *pos;
     const double3 *centre;
@@ -818,10 +866,32 @@ struct VecArgs {
 // z = D^-1 r (NoMassCGSolver.cpp:268-281), partials of r.r and r.z.
 // With PROJ, q is first projected onto the range of the operator:
 // q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
+// projected form: per blob (one workgroup each) the 13 sums of the SpMV step, the
+// projection coefficients c = (Z^T Z)^-1 Z^T q and alpha = delta / (p.q - (Z^T p).c)
+__global__ __launch_bounds__(kBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
+                                                    double *coef) {
+    const int b = blockIdx.x;
+    const CgScalars *S = ra.S + b;
+    if (S->done) return;
+    const int2 sg = ra.seg[b];
+    double t[13];
+    sum_partials<13>(P1, ra.nchunks, sg.x, sg.y, t);
+    if (threadIdx.x == 0) {
+        double cq[6], pq = t[0];
+        for (int k = 0; k < 6; ++k) {
+            double acc = 0.0;
+            for (int j = 0; j < 6; ++j) acc += G[36 * b + 6 * k + j] * t[1 + j];
+            cq[k] = acc;
+        }
+        for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
+        coef[8 * b] = S->delta[(iter - 1) & 1] / pq;
+        for (int k = 0; k < 6; ++k) coef[8 * b + 1 + k] = cq[k];
+    }
+}
+
 template <bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
-    constexpr int NV = PROJ ? 13 : 1;
     int cur = -1;
     bool skip = false;
     double alpha = 0.0;
@@ -833,20 +903,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
             const CgScalars *S = a.ra.S + cur;
             skip = S->done != 0;
             if (!skip) {
-                const int2 sg = a.ra.seg[cur];
-                double t[NV];
-                sum_partials<NV>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
-                double pq = t[0];
                 if constexpr (PROJ) {
-                    const double *G = a.G + 36 * cur;
-                    for (int k = 0; k < 6; ++k) {
-                        double acc = 0.0;
-                        for (int j = 0; j < 6; ++j) acc += G[6 * k + j] * t[1 + j];
-                        cq[k] = acc;
-                    }
-                    for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
+                    alpha = a.coef[8 * cur];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) cq[k] = a.coef[8 * cur + 1 + k];
+                } else {
+                    const int2 sg = a.ra.seg[cur];
+                    double t[1];
+                    sum_partials<1>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
+                    alpha = S->delta[(a.iter - 1) & 1] / t[0];
                 }
-                alpha = S->delta[(a.iter - 1) & 1] / pq;
             }
         }
         if (skip) continue;

@@ -86,6 +86,29 @@ public:
     int num_nodes() const { return num_nodes_; }
     ffea_gpu_blob *handle() { return b_; }
 
+    // gradients (noise off, some nodes pinned): per-element A, B, G, E; a checkpointed run;
+    // its reverse from dL/dpos at the end (.node order) to dL/d(A, B, G, E) and dL/dpos0
+    void set_material(const std::vector<double> &A, const std::vector<double> &B, const std::vector<double> &G,
+                      const std::vector<double> &E) {
+        check(ffea_gpu_blob_set_material(b_, A.data(), B.data(), G.data(), E.data()));
+    }
+    void run_checkpointed(int nsteps, int interval = 20) { check(ffea_gpu_blob_run_checkpointed(b_, nsteps, interval)); }
+    void adjoint(const std::vector<double> &dL_dpos, std::vector<double> &dA, std::vector<double> &dB,
+                 std::vector<double> &dG, std::vector<double> &dE, std::vector<double> *dL_dpos0 = nullptr) {
+        ffea_gpu_blob_info info{};
+        check(ffea_gpu_blob_get_info(b_, &info));
+        for (auto *v : {&dA, &dB, &dG, &dE}) v->assign(info.num_elements, 0.0);
+        if (dL_dpos0) dL_dpos0->assign((size_t)3 * num_nodes_, 0.0);
+        check(ffea_gpu_blob_adjoint(b_, dL_dpos.data(), dA.data(), dB.data(), dG.data(), dE.data(),
+                                    dL_dpos0 ? dL_dpos0->data() : nullptr));
+    }
+    void use_graph(bool on, int iterations = 0) { check(ffea_gpu_blob_use_graph(b_, on ? 1 : 0, iterations)); }
+    int autotune(int iterations = 20) {
+        int chosen = 0;
+        check(ffea_gpu_blob_autotune(b_, iterations, &chosen));
+        return chosen;
+    }
+
 private:
     void check(int rc) {
         if (rc) throw Exception(rc, ffea_gpu_blob_last_error(b_));
