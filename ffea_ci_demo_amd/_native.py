@@ -81,9 +81,9 @@ def lib():
     L.ffea_gpu_blob_get_timing.argtypes = [vp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     pp = C.POINTER(C.c_void_p)
     L.ffea_gpu_batch_create.argtypes = [C.POINTER(Params), C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp,
-                                        vp, _up, C.POINTER(vp)]
-    L.ffea_gpu_check_layout.argtypes = [C.c_int, C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp, vp, _up,
-                                        C.POINTER(Info)]
+                                        vp, vp, _up, C.POINTER(vp)]
+    L.ffea_gpu_check_layout.argtypes = [C.POINTER(Params), C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp,
+                                        vp, vp, _up, C.POINTER(Info)]
     for f in ("get_positions", "set_positions", "get_velocities", "get_rhs", "get_element_forces"):
         getattr(L, "ffea_gpu_batch_" + f).argtypes = [vp, C.c_int, _dp]
     L.ffea_gpu_batch_get_cg_iterations.argtypes = [vp, _ip, np.ctypeslib.ndpointer(dtype=np.int64,

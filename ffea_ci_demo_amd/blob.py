@@ -225,10 +225,11 @@ class GpuBatch(_Handle):
         specs = list(specs)
         s0 = specs[0]
         for s in specs[1:]:
-            if (s.dt, s.kT, s.epsilon2, s.max_iterations_cg, s.calc_noise) != \
-                    (s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise):
-                raise ValueError("blobs of one batch share dt, kT, epsilon2, max_iterations_cg and calc_noise")
-        p = _params(s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise, 0, precision, device)
+            if (s.dt, s.kT, s.epsilon2, s.max_iterations_cg, s.calc_noise, s.calc_stokes) != \
+                    (s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise, s0.calc_stokes):
+                raise ValueError("blobs of one batch share dt, kT, epsilon2, max_iterations_cg, calc_noise and "
+                                 "calc_stokes")
+        p = _params(s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise, s0.calc_stokes, precision, device)
         args, keep = _batch_arrays(specs)
         h = C.c_void_p()
         rc = L.ffea_gpu_batch_create(C.byref(p), *args, C.byref(h))
@@ -300,6 +301,7 @@ def _batch_arrays(specs):
     cn = [arr(s.ctf_node if s.ctf_node is not None else [], np.int32) for s in specs]
     cf = [arr(s.ctf_force if s.ctf_force is not None else np.zeros((0, 3)), np.float64, (-1, 3)) for s in specs]
     rng = arr(np.stack([np.asarray(s.rng_state, np.uint32) for s in specs]).reshape(-1), np.uint32)
+    stokes = [arr(s.stokes_drag if s.stokes_drag is not None else [], np.float64) for s in specs]
 
     def ptrs(lst):
         a = (C.c_void_p * nb)(*[x.ctypes.data if x.size else None for x in lst])
@@ -310,7 +312,8 @@ def _batch_arrays(specs):
     nctf = arr([x.size for x in cn], np.int32)
     args = (nb, arr([x.shape[0] for x in pos], np.int32), arr(ne, np.int32), ptrs(pos), ptrs(top), ptrs(mats["A"]),
             ptrs(mats["B"]), ptrs(mats["G"]), ptrs(mats["E"]), npin.ctypes.data, C.cast(ptrs(pins), C.c_void_p),
-            nctf.ctypes.data, C.cast(ptrs(cn), C.c_void_p), C.cast(ptrs(cf), C.c_void_p), rng)
+            nctf.ctypes.data, C.cast(ptrs(cn), C.c_void_p), C.cast(ptrs(cf), C.c_void_p),
+            C.cast(ptrs(stokes), C.c_void_p) if any(x.size for x in stokes) else None, rng)
     return args, keep
 
 
@@ -318,9 +321,12 @@ def check_layout(specs, precision="fp64"):
     """Build the device tables of a batch on the host (no device) and check every kernel
     index against its buffer; returns the sizes (ffea_gpu_check_layout)."""
     L = _native.lib()
-    args, keep = _batch_arrays(list(specs))
+    specs = list(specs)
+    s0 = specs[0]
+    p = _params(s0.dt, s0.kT, s0.epsilon2, s0.max_iterations_cg, s0.calc_noise, s0.calc_stokes, precision, 0)
+    args, keep = _batch_arrays(specs)
     info = _native.Info()
-    rc = L.ffea_gpu_check_layout({"fp64": _native.FP64, "mixed": _native.MIXED}[precision], *args, C.byref(info))
+    rc = L.ffea_gpu_check_layout(C.byref(p), *args, C.byref(info))
     del keep
     if rc != _native.OK:
         raise FFEAException(rc, L.ffea_gpu_last_error().decode())

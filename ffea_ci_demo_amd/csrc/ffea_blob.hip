@@ -112,6 +112,7 @@ struct BlobInput {
     const int *ctf_node = nullptr;
     const double *ctf_force = nullptr;
     const uint32_t *rng = nullptr;
+    const double *drag = nullptr;  // [nt] Stokes drag per node (calc_stokes), else null
 };
 
 // closed-form rest state (Blob::calc_rest_state_info, Blob.cpp:3080-3143;
@@ -160,13 +161,18 @@ struct BlobPlan {
     std::vector<unsigned char> dj;  // [nl] slot index of the diagonal block
     std::vector<double> ctf;        // [nl][3]
     std::vector<uint32_t> fp;       // [nl+1]
-    std::vector<double> fv;         // linearise_force increments [..][3]
+    std::vector<double> fv;         // linearise_force increments [..][3] (constant forces only)
+    // Stokes noise: per (element, corner) in element order, the midside nodes first folded
+    // there (local node indices, -1 if already folded), and per node its constant force
+    std::vector<std::array<int, 3>> fm;
+    std::vector<double> node_ctf;   // [nt][3]
+    std::vector<double> drag_lin;   // [nl] drag of unpinned corner rows
     double centre[3] = {0, 0, 0};
     long long nblocks = 0;
     long long nslots() const { return sb.empty() ? 0 : sb.back(); }
 };
 
-BlobPlan make_plan(const BlobInput &in, int blob_index) {
+BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
     BlobPlan P;
     const int nt = in.nt, ne = in.ne;
     const int *top = in.top;
@@ -380,9 +386,41 @@ BlobPlan make_plan(const BlobInput &in, int blob_index) {
         }
     }
     P.fp.assign(nl + 1, 0);
-    for (int n = 0; n < nl; ++n) {
-        for (auto &f : folds[n]) P.fv.insert(P.fv.end(), f.begin(), f.end());
-        P.fp[n + 1] = P.fp[n] + (uint32_t)folds[n].size();
+    if (stokes_noise) {
+        // every node carries noise: the folds are formed each step from the records
+        std::vector<std::vector<std::array<int, 3>>> rec(nl);
+        std::vector<unsigned char> folded(nt, 0);
+        static const int adj[4][3] = {{4, 5, 6}, {4, 7, 8}, {5, 7, 9}, {6, 8, 9}};
+        for (int e = 0; e < ne; ++e) {
+            const int *nn = top + 10 * e;
+            for (int a = 0; a < 4; ++a) {
+                std::array<int, 3> r3{-1, -1, -1};
+                bool any = false;
+                for (int t = 0; t < 3; ++t)
+                    if (!folded[nn[adj[a][t]]]) {
+                        r3[t] = nn[adj[a][t]];
+                        any = true;
+                    }
+                if (any) rec[P.lin_of[nn[a]]].push_back(r3);
+            }
+            for (int k = 4; k < 10; ++k) folded[nn[k]] = 1;
+        }
+        for (int n = 0; n < nl; ++n) {
+            P.fm.insert(P.fm.end(), rec[n].begin(), rec[n].end());
+            P.fp[n + 1] = P.fp[n] + (uint32_t)rec[n].size();
+        }
+        P.have_fold = true;
+        if (P.have_ctf) P.node_ctf = ctf_total;
+    } else {
+        for (int n = 0; n < nl; ++n) {
+            for (auto &f : folds[n]) P.fv.insert(P.fv.end(), f.begin(), f.end());
+            P.fp[n + 1] = P.fp[n] + (uint32_t)folds[n].size();
+        }
+    }
+    if (in.drag) {
+        P.drag_lin.assign(nl, 0.0);
+        for (int n = 0; n < nl; ++n)
+            if (!P.pin_lin[n]) P.drag_lin[n] = in.drag[P.orig_of_lin[n]];
     }
     return P;
 }
@@ -414,11 +452,21 @@ struct Tables {
     std::vector<int2> seg;
     std::vector<double3> centre;
     std::vector<mrg::Jump> step_jump, jump_group, jump_lane;
+    // Stokes branch
+    bool stokes = false, stokes_noise = false;
+    int nnodes = 0;                      // nodes of all blobs (.node files), noise index
+    std::vector<double> drag;            // [nrows]
+    std::vector<int> node_blob, node_off, row_node;
+    std::vector<double> sc, node_ctf;    // [nnodes], [nnodes][3]
+    std::vector<int3> fold_mid;
+    std::vector<mrg::Jump> node_jump, njump_group, njump_lane;
 };
 
-Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
+Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_params &prm) {
     Tables t;
     const int nblob = t.nblob = (int)in.size();
+    t.stokes = prm.calc_stokes != 0;
+    t.stokes_noise = t.stokes && prm.calc_noise != 0;
     // per-blob plans are independent: build them on up to 16 host threads
     std::vector<BlobPlan> plans(nblob);
     {
@@ -428,7 +476,7 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
         auto work = [&]() {
             for (int b = next++; b < nblob; b = next++) {
                 try {
-                    plans[b] = make_plan(in[b], nblob > 1 ? b : -1);
+                    plans[b] = make_plan(in[b], nblob > 1 ? b : -1, t.stokes_noise);
                 } catch (const Invalid &e) {
                     errors[b] = e.msg;
                 } catch (const std::bad_alloc &) {
@@ -445,7 +493,7 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
     }
     // ---- concatenation offsets: every blob starts on a 64-row slice boundary
     t.meta.resize(nblob);
-    long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0;
+    long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0, nodes = 0;
     for (int b = 0; b < nblob; ++b) {
         BlobMeta &m = t.meta[b];
         const BlobPlan &P = plans[b];
@@ -458,7 +506,7 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
         m.all_pinned = P.all_pinned;
         // free blob (no pins): the viscosity matrix is singular with the 6 rigid modes as
         // null space; the mixed form projects them out of every SpMV result
-        m.project = mixed && P.npin == 0 && P.connected && !P.all_pinned;
+        m.project = mixed && P.npin == 0 && P.connected && !P.all_pinned && !t.stokes;
         m.lin_of = P.lin_of;
         m.orig_of_lin = P.orig_of_lin;
         m.mid_a = P.mid_a;
@@ -468,13 +516,15 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
         slots += P.nslots();
         slices += P.nslices;
         contribs += (long long)P.contrib.size();
+        nodes += P.nt;
         t.max_blob_ne = std::max(t.max_blob_ne, P.ne);
         if (!m.all_pinned) ++t.active_blobs;
         t.have_ctf |= P.have_ctf;
         t.have_fold |= P.have_fold;
         t.any_project |= m.project;
     }
-    if (rows >= (1ll << 30) || elems >= (1ll << 28) - 1 || slots >= (1ll << 31) || contribs >= (1ll << 32) - 1)
+    if (rows >= (1ll << 30) || elems >= (1ll << 28) - 1 || slots >= (1ll << 31) || contribs >= (1ll << 32) - 1 ||
+        nodes >= (1ll << 31) / 3)
         throw Invalid("problem too large for 32-bit indices");
     const int nrows = t.nrows = (int)rows;
     const int ne = t.ne = (int)elems;
@@ -502,6 +552,16 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
     t.centre.resize(nblob);
     t.step_jump.resize(nblob);
     t.rng.assign((size_t)6 * nblob, 0);
+    t.nnodes = (int)nodes;
+    if (t.stokes) t.drag.assign(nrows, 0.0);
+    if (t.stokes_noise) {
+        t.node_off.assign(nblob, 0);
+        if (nblob > 1) t.node_blob.assign(t.nnodes, 0);
+        t.row_node.assign(nrows, -1);
+        t.sc.assign(t.nnodes, 0.0);
+        if (t.have_ctf) t.node_ctf.assign((size_t)3 * t.nnodes, 0.0);
+        t.node_jump.resize(nblob);
+    }
     t.jinv0.assign((size_t)9 * ne, 0.0);
     t.vol0.assign(ne, 0.0);
     t.A.assign(ne, 0.0);
@@ -509,9 +569,27 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
     t.G.assign(ne, 0.0);
     t.E.assign(ne, 0.0);
     long long slot_off = 0;
-    int slice_off = 0;
+    int slice_off = 0, node_off = 0, max_nt = 0;
     for (int b = 0; b < nblob; ++b) {
         BlobPlan &P = plans[b];
+        if (t.stokes)
+            for (int n = 0; n < P.nl; ++n) t.drag[t.meta[b].row_off + n] = P.drag_lin[n];
+        if (t.stokes_noise) {
+            t.node_off[b] = node_off;
+            max_nt = std::max(max_nt, P.nt);
+            for (int i = 0; i < P.nt; ++i) {
+                if (nblob > 1) t.node_blob[node_off + i] = b;
+                // Blob.cpp:3214-3216, evaluated on the host in IEEE double
+                t.sc[node_off + i] = std::sqrt((24 * prm.kT * in[b].drag[i]) / (prm.dt));
+                if (t.have_ctf && !P.node_ctf.empty())
+                    for (int c = 0; c < 3; ++c) t.node_ctf[3 * (size_t)(node_off + i) + c] = P.node_ctf[3 * (size_t)i + c];
+            }
+            for (int n = 0; n < P.nl; ++n) t.row_node[t.meta[b].row_off + n] = node_off + P.orig_of_lin[n];
+            for (const auto &r3 : P.fm)
+                t.fold_mid.push_back(make_int3(r3[0] < 0 ? -1 : node_off + r3[0], r3[1] < 0 ? -1 : node_off + r3[1],
+                                               r3[2] < 0 ? -1 : node_off + r3[2]));
+            t.node_jump[b] = mrg::h_pow(7ull * (unsigned long long)P.ne);
+        }
         const BlobMeta &m = t.meta[b];
         const int ro = m.row_off, eo = m.elem_off;
         t.eoff[b] = eo;
@@ -562,10 +640,14 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
         t.centre[b] = make_double3(P.centre[0], P.centre[1], P.centre[2]);
         // noise stream: 7 draws per element per pass; none for an all-pinned blob, whose
         // update_internal_forces returns at once (Blob.cpp:405-407)
-        t.step_jump[b] = m.all_pinned ? mrg::h_identity() : mrg::h_pow(7ull * (unsigned long long)P.ne);
+        // with Stokes noise the step also draws 3 per node (Blob.cpp:3214-3216)
+        t.step_jump[b] = m.all_pinned ? mrg::h_identity()
+                                      : mrg::h_pow(7ull * (unsigned long long)P.ne +
+                                                   (t.stokes_noise ? 3ull * (unsigned long long)P.nt : 0ull));
         for (int i = 0; i < 6; ++i) t.rng[6 * b + i] = in[b].rng[i];
         slot_off += P.nslots();
         slice_off += P.nslices;
+        node_off += P.nt;
         P = BlobPlan();  // release the blob's host plan
     }
     for (int n = 0; n < nrows; ++n) {
@@ -582,6 +664,16 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed) {
     for (int l = 1; l < 64; ++l) t.jump_lane[l] = mrg::h_mul(a7, t.jump_lane[l - 1]);
     t.jump_group[0] = mrg::h_identity();
     for (int g = 1; g < ngroups; ++g) t.jump_group[g] = mrg::h_mul(a448, t.jump_group[g - 1]);
+    if (t.stokes_noise) {  // node draws: A^(3*64*g) per group of 64 nodes, A^(3*l) per lane
+        const int nng = std::max(1, (max_nt + 63) / 64);
+        t.njump_group.resize(nng);
+        t.njump_lane.resize(64);
+        const mrg::Jump a3 = mrg::h_pow(3), a192 = mrg::h_pow(3ull * 64ull);
+        t.njump_lane[0] = mrg::h_identity();
+        for (int l = 1; l < 64; ++l) t.njump_lane[l] = mrg::h_mul(a3, t.njump_lane[l - 1]);
+        t.njump_group[0] = mrg::h_identity();
+        for (int g = 1; g < nng; ++g) t.njump_group[g] = mrg::h_mul(a192, t.njump_group[g - 1]);
+    }
     return t;
 }
 
@@ -618,7 +710,7 @@ void validate(const Tables &t) {
         if (t.ip[n + 1] < t.ip[n]) fail("incidence order", n);
     for (size_t i = 0; i < t.ic.size(); ++i)
         if ((long long)(t.ic[i] >> 2) >= ne) fail("incidence element", (long long)i);
-    if (t.have_fold) {
+    if (t.have_fold && !t.stokes_noise) {
         if ((long long)t.fp.size() != nrows + 1 || t.fp[0] != 0 || 3 * (size_t)t.fp[nrows] != t.fv.size())
             fail("fold ends", nrows);
         for (long long n = 0; n < nrows; ++n)
@@ -653,6 +745,33 @@ void validate(const Tables &t) {
         }
     }
     if ((long long)t.chunks.size() != nc || t.jump_lane.size() != 64) fail("chunk/jump table size", nc);
+    if (t.stokes && (long long)t.drag.size() != nrows) fail("drag size", (long long)t.drag.size());
+    if (t.stokes_noise) {
+        const long long nn = t.nnodes;
+        if ((long long)t.sc.size() != nn || (long long)t.row_node.size() != nrows || (int)t.node_off.size() != t.nblob ||
+            (int)t.node_jump.size() != t.nblob || t.njump_lane.size() != 64 ||
+            (t.nblob > 1 && (long long)t.node_blob.size() != nn) || (t.have_ctf && (long long)t.node_ctf.size() != 3 * nn))
+            fail("noise table size", nn);
+        if ((long long)t.fp.size() != nrows + 1 || (size_t)t.fp[nrows] != t.fold_mid.size()) fail("fold record ends", nrows);
+        for (long long n = 0; n < nrows; ++n)
+            if (t.fp[n + 1] < t.fp[n]) fail("fold record order", n);
+        for (int b = 0; b < t.nblob; ++b) {
+            const BlobMeta &m = t.meta[b];
+            if (t.node_off[b] + (long long)m.nt > nn || ((long long)m.nt + 63) / 64 > (long long)t.njump_group.size())
+                fail("noise node range", b);
+            for (int n = 0; n < m.nl; ++n) {
+                const int g = t.row_node[m.row_off + n];
+                if (g < t.node_off[b] || g >= t.node_off[b] + m.nt) fail("row node", (long long)m.row_off + n);
+            }
+            for (int i = 0; i < m.nt && t.nblob > 1; ++i)
+                if (t.node_blob[t.node_off[b] + i] != b) fail("node blob", (long long)t.node_off[b] + i);
+        }
+        for (size_t k = 0; k < t.fold_mid.size(); ++k) {
+            const int m3[3] = {t.fold_mid[k].x, t.fold_mid[k].y, t.fold_mid[k].z};
+            for (int q : m3)
+                if (q < -1 || q >= nn) fail("fold record node", (long long)k);
+        }
+    }
 }
 
 }  // namespace
@@ -688,6 +807,13 @@ struct ffea_gpu_blob {
     DevBuf<double3> centre;
     DevBuf<int> num_done;
     bool have_ctf = false, have_fold = false;
+    // Stokes branch
+    bool stokes = false, stokes_noise = false;
+    int nnodes = 0, grid_nodes = 0;
+    DevBuf<double> drag, node_sc, node_ctf, noise;
+    DevBuf<int> node_blob, node_off, row_node;
+    DevBuf<int3> fold_mid;
+    DevBuf<mrg::Jump> node_jump, njump_group, njump_lane;
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
 
@@ -841,7 +967,7 @@ void ffea_gpu_blob::upload_elements(const Tables &t) {
 }
 
 void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
-    Tables t = make_tables(in, mixed);
+    Tables t = make_tables(in, mixed, prm);
     validate(t);
     nblob = t.nblob;
     nrows = t.nrows;
@@ -898,6 +1024,26 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
         fold.alloc(t.fv.size(), &dev_bytes);
         fold.upload(t.fv, stream);
     }
+    stokes = t.stokes;
+    stokes_noise = t.stokes_noise;
+    nnodes = t.nnodes;
+    if (stokes) {
+        drag.alloc(t.drag.size(), &dev_bytes);
+        drag.upload(t.drag, stream);
+    }
+    if (stokes_noise) {
+        grid_nodes = (nnodes + kBlock - 1) / kBlock;
+        node_sc.alloc(t.sc.size(), &dev_bytes); node_sc.upload(t.sc, stream);
+        node_off.alloc(t.node_off.size(), &dev_bytes); node_off.upload(t.node_off, stream);
+        row_node.alloc(t.row_node.size(), &dev_bytes); row_node.upload(t.row_node, stream);
+        fold_mid.alloc(t.fold_mid.size(), &dev_bytes); fold_mid.upload(t.fold_mid, stream);
+        node_jump.alloc(t.node_jump.size(), &dev_bytes); node_jump.upload(t.node_jump, stream);
+        njump_group.alloc(t.njump_group.size(), &dev_bytes); njump_group.upload(t.njump_group, stream);
+        njump_lane.alloc(t.njump_lane.size(), &dev_bytes); njump_lane.upload(t.njump_lane, stream);
+        noise.alloc((size_t)3 * nnodes, &dev_bytes); noise.zero(stream);
+        if (nblob > 1) { node_blob.alloc(t.node_blob.size(), &dev_bytes); node_blob.upload(t.node_blob, stream); }
+        if (have_ctf) { node_ctf.alloc(t.node_ctf.size(), &dev_bytes); node_ctf.upload(t.node_ctf, stream); }
+    }
     jump_group.alloc(t.jump_group.size(), &dev_bytes); jump_group.upload(t.jump_group, stream);
     jump_lane.alloc(t.jump_lane.size(), &dev_bytes); jump_lane.upload(t.jump_lane, stream);
     step_jump.alloc(t.step_jump.size(), &dev_bytes); step_jump.upload(t.step_jump, stream);
@@ -937,7 +1083,10 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
                           mixed ? (const void *)f_val.p : (const void *)d_val.p};
     for (const void *ptr : need)
         if (!ptr) throw Invalid("internal error: a device buffer was not allocated");
-    if ((nblob > 1 && !elem_blob.p) || (have_ctf && !ctf.p) || (have_fold && (!fold_ptr.p || !fold.p)))
+    if ((nblob > 1 && !elem_blob.p) || (have_ctf && !ctf.p) || (have_fold && (!fold_ptr.p || !fold.p)) ||
+        (stokes && !drag.p) ||
+        (stokes_noise && (!node_sc.p || !node_off.p || !row_node.p || !fold_mid.p || !node_jump.p || !njump_group.p ||
+                          !njump_lane.p || !noise.p || (nblob > 1 && !node_blob.p) || (have_ctf && !node_ctf.p))))
         throw Invalid("internal error: a device buffer was not allocated");
     // row-kernel grid: one block per chunk up to 1024 blocks (tunable, result-neutral)
     set_row_blocks(1024);
@@ -969,6 +1118,11 @@ void ffea_gpu_blob::launch_element() {
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(0, t0);
+    if (stokes_noise) {  // the node draws follow the element draws of the step
+        NoiseArgs na{nnodes, nblob > 1 ? node_blob.p : nullptr, node_off.p, node_sc.p, rng.p, node_jump.p,
+                     njump_group.p, njump_lane.p, noise.p};
+        hipLaunchKernelGGL(k_node_noise, dim3(grid_nodes), dim3(kBlock), 0, stream, na);
+    }
     // the pass consumed 7 draws per element (tetra_element_linear.cpp:416-426)
     if (prm.calc_noise) {
         hipLaunchKernelGGL(k_rng_advance, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, rng.p,
@@ -996,8 +1150,10 @@ int ffea_gpu_blob::check_status_after_forces() {
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_gather(const T *force) {
     GatherArgs<T> g{row_args(), inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
-                    have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr, pinned.p, dinv.p, pos.p,
-                    centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
+                    have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr,
+                    stokes_noise ? noise.p : nullptr, stokes_noise ? row_node.p : nullptr,
+                    stokes_noise ? fold_mid.p : nullptr, stokes_noise && have_ctf ? node_ctf.p : nullptr, pinned.p,
+                    dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
@@ -1024,11 +1180,11 @@ void ffea_gpu_blob::launch_assemble() {
     t_begin(1, &t0);
     if (mixed) {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
-                         f_val.p, dinv.p};
+                         f_val.p, dinv.p, stokes ? drag.p : nullptr};
         hipLaunchKernelGGL(k_assemble<float>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
-                          d_val.p, dinv.p};
+                          d_val.p, dinv.p, stokes ? drag.p : nullptr};
         hipLaunchKernelGGL(k_assemble<double>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
@@ -1382,10 +1538,6 @@ int guarded(ffea_gpu_blob *b, F &&f) {
 
 int check_params(const char *who, const ffea_gpu_params *params) {
     const std::string w(who);
-    if (params->calc_stokes) {
-        g_last_error = w + ": calc_stokes is not supported by this version";
-        return FFEA_GPU_ERR_INVALID;
-    }
     if (params->precision != FFEA_GPU_FP64 && params->precision != FFEA_GPU_MIXED) {
         g_last_error = w + ": unknown precision";
         return FFEA_GPU_ERR_INVALID;
@@ -1439,8 +1591,10 @@ int batch_inputs(const char *who, int num_blobs, const int *num_nodes, const int
                  const double *const *pos, const int *const *top, const double *const *A, const double *const *B,
                  const double *const *G, const double *const *E, const int *num_pinned, const int *const *pinned,
                  const int *num_ctf, const int *const *ctf_node, const double *const *ctf_force,
-                 const uint32_t *rng_states, std::vector<BlobInput> *out) {
-    if (num_blobs <= 0 || !num_nodes || !num_elements || !pos || !top || !A || !B || !G || !E || !rng_states) {
+                 const double *const *stokes_drag, bool calc_stokes, const uint32_t *rng_states,
+                 std::vector<BlobInput> *out) {
+    if (num_blobs <= 0 || !num_nodes || !num_elements || !pos || !top || !A || !B || !G || !E || !rng_states ||
+        (calc_stokes && !stokes_drag)) {
         g_last_error = std::string(who) + ": invalid argument";
         return FFEA_GPU_ERR_INVALID;
     }
@@ -1449,8 +1603,9 @@ int batch_inputs(const char *who, int num_blobs, const int *num_nodes, const int
         (*out)[b] = BlobInput{num_nodes[b], num_elements[b], pos[b], top[b], A[b], B[b], G[b], E[b],
                               num_pinned ? num_pinned[b] : 0, pinned ? pinned[b] : nullptr,
                               num_ctf ? num_ctf[b] : 0, ctf_node ? ctf_node[b] : nullptr,
-                              ctf_force ? ctf_force[b] : nullptr, rng_states + 6 * b};
-        if (!valid_blob_input((*out)[b])) {
+                              ctf_force ? ctf_force[b] : nullptr, rng_states + 6 * b,
+                              calc_stokes ? stokes_drag[b] : nullptr};
+        if (!valid_blob_input((*out)[b]) || (calc_stokes && !(*out)[b].drag)) {
             g_last_error = fmt("%s: invalid argument for blob %d", who, b);
             return FFEA_GPU_ERR_INVALID;
         }
@@ -1471,14 +1626,19 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out) {
     if (out) *out = nullptr;
-    const BlobInput bi{num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
-                       ctf_force, rng_state};
-    if (!params || !out || !valid_blob_input(bi)) {
+    if (!params || !out) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
     }
-    if (params->calc_stokes || stokes_drag) {
-        g_last_error = "ffea_gpu_blob_create: calc_stokes is not supported by this version";
+    // the drag is used only with calc_stokes = 1 (NoMassCGSolver.cpp:119-127, Blob.cpp:3172)
+    const BlobInput bi{num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
+                       ctf_force, rng_state, params->calc_stokes ? stokes_drag : nullptr};
+    if (!valid_blob_input(bi)) {
+        g_last_error = "ffea_gpu_blob_create: invalid argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    if (params->calc_stokes && !stokes_drag) {
+        g_last_error = "ffea_gpu_blob_create: calc_stokes = 1 needs the per-node Stokes drag";
         return FFEA_GPU_ERR_INVALID;
     }
     if (!mrg::h_valid_state(rng_state)) {
@@ -1493,7 +1653,7 @@ int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const in
                           const double *const *A, const double *const *B, const double *const *G,
                           const double *const *E, const int *num_pinned, const int *const *pinned,
                           const int *num_ctf, const int *const *ctf_node, const double *const *ctf_force,
-                          const uint32_t *rng_states, ffea_gpu_blob **out) {
+                          const double *const *stokes_drag, const uint32_t *rng_states, ffea_gpu_blob **out) {
     if (out) *out = nullptr;
     if (!params || !out) {
         g_last_error = "ffea_gpu_batch_create: invalid argument";
@@ -1501,23 +1661,30 @@ int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const in
     }
     std::vector<BlobInput> in;
     if (int rc = batch_inputs("ffea_gpu_batch_create", num_blobs, num_nodes, num_elements, pos, top, A, B, G, E,
-                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, rng_states, &in))
+                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, stokes_drag, params->calc_stokes != 0,
+                              rng_states, &in))
         return rc;
     return create_handle("ffea_gpu_batch_create", params, in, out);
 }
 
-int ffea_gpu_check_layout(int precision, int num_blobs, const int *num_nodes, const int *num_elements,
-                          const double *const *pos, const int *const *top, const double *const *A,
-                          const double *const *B, const double *const *G, const double *const *E,
-                          const int *num_pinned, const int *const *pinned, const int *num_ctf,
-                          const int *const *ctf_node, const double *const *ctf_force, const uint32_t *rng_states,
-                          ffea_gpu_blob_info *info) {
+int ffea_gpu_check_layout(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
+                          const int *num_elements, const double *const *pos, const int *const *top,
+                          const double *const *A, const double *const *B, const double *const *G,
+                          const double *const *E, const int *num_pinned, const int *const *pinned,
+                          const int *num_ctf, const int *const *ctf_node, const double *const *ctf_force,
+                          const double *const *stokes_drag, const uint32_t *rng_states, ffea_gpu_blob_info *info) {
+    if (!params) {
+        g_last_error = "ffea_gpu_check_layout: invalid argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    const int precision = params->precision;
     std::vector<BlobInput> in;
     if (int rc = batch_inputs("ffea_gpu_check_layout", num_blobs, num_nodes, num_elements, pos, top, A, B, G, E,
-                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, rng_states, &in))
+                              num_pinned, pinned, num_ctf, ctf_node, ctf_force, stokes_drag,
+                              params->calc_stokes != 0, rng_states, &in))
         return rc;
     return guarded(nullptr, [&]() {
-        const Tables t = make_tables(in, precision == FFEA_GPU_MIXED);
+        const Tables t = make_tables(in, precision == FFEA_GPU_MIXED, *params);
         validate(t);
         if (info) {
             *info = ffea_gpu_blob_info{};
@@ -1697,6 +1864,21 @@ int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *b, int blob, const uint32_t sta
             j = b->meta[blob].all_pinned ? mrg::h_identity() : mrg::h_pow((unsigned long long)draws_per_step);
             FFEA_HIP_TRY(hipMemcpyAsync(b->step_jump.p + blob, &j, sizeof(j), hipMemcpyHostToDevice, b->stream));
         }
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *b, int blob, long long offset) {
+    if (bad_blob(b, blob) || offset < 0) return FFEA_GPU_ERR_INVALID;
+    if (!b->stokes_noise) {
+        b->err = "node draws exist only with calc_stokes = 1 and calc_noise = 1";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        const mrg::Jump j = mrg::h_pow((unsigned long long)offset);
+        FFEA_HIP_TRY(hipMemcpyAsync(b->node_jump.p + blob, &j, sizeof(j), hipMemcpyHostToDevice, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         return (int)FFEA_GPU_OK;
     });

@@ -737,10 +737,6 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
     }
     const BlobData &B = s->blobs[b];
     const ffea_script_params &p = s->P.p;
-    if (p.calc_stokes) {
-        ffea_internal_set_error("calc_stokes = 1 is not supported by this build; set <calc_stokes = 0>");
-        return FFEA_GPU_ERR_INVALID;
-    }
     if (p.calc_ssint || p.calc_steric || p.calc_es || p.calc_springs || p.calc_preComp || p.calc_kinetics) {
         ffea_internal_set_error("surface interactions, electrostatics, springs, pre-computed potentials and "
                                 "kinetics are not part of this build; set calc_ssint, calc_steric, calc_es, "
@@ -753,25 +749,44 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
     gp.epsilon2 = p.epsilon2;
     gp.max_iterations_cg = p.max_iterations_cg;
     gp.calc_noise = p.calc_noise;
-    gp.calc_stokes = 0;
+    gp.calc_stokes = p.calc_stokes;
     gp.precision = precision;
     gp.device = device;
+    // Blob::init (Blob.cpp:264-268): drag = 6 pi stokes_visc stokes_radius per node
+    const double pi = 3.141592653589793238462643383279502884;
+    std::vector<double> drag;
+    if (p.calc_stokes) {
+        drag.resize(B.nt);
+        for (int i = 0; i < B.nt; ++i) drag[i] = 6.0 * pi * p.stokes_visc * B.stokes_radius[i];
+    }
     int rc = ffea_gpu_blob_create(&gp, B.nt, B.ne, B.pos.data(), B.top.data(), B.A.data(), B.B.data(), B.G.data(),
                                 B.E.data(), (int)B.pinned.size(), B.pinned.empty() ? nullptr : B.pinned.data(),
                                 (int)B.ctf_node.size(), B.ctf_node.empty() ? nullptr : B.ctf_node.data(),
-                                B.ctf_force.empty() ? nullptr : B.ctf_force.data(), nullptr, s->rng_state, out);
+                                B.ctf_force.empty() ? nullptr : B.ctf_force.data(),
+                                p.calc_stokes ? drag.data() : nullptr, s->rng_state, out);
     if (rc != FFEA_GPU_OK || s->blobs.size() == 1) return rc;
-    // several blobs draw from the one stream in turn (World.cpp:2031-2038)
-    long long before = 0, total = 0;
+    // several blobs draw from the one stream in turn (World.cpp:2031-2058): all element
+    // draws first, blob after blob, then (Stokes noise) all node draws, blob after blob
+    const bool node_draws = p.calc_stokes && p.calc_noise;
+    long long before = 0, total = 0, elem_all = 0, nodes_before = 0;
     for (size_t i = 0; i < s->blobs.size(); ++i) {
-        if ((int)i < b) before += 7ll * s->blobs[i].ne;
-        total += 7ll * s->blobs[i].ne;
+        if ((int)i < b) {
+            before += 7ll * s->blobs[i].ne;
+            nodes_before += 3ll * s->blobs[i].nt;
+        }
+        elem_all += 7ll * s->blobs[i].ne;
+        total += 7ll * s->blobs[i].ne + (node_draws ? 3ll * s->blobs[i].nt : 0ll);
     }
     if (!p.calc_noise) return rc;
     uint32_t st[6];
     ffea_script_rng_state(s, st);
     rc = ffea_gpu_advance_rng_state(st, before, st);
     if (rc == FFEA_GPU_OK) rc = ffea_gpu_blob_set_rng_stream(*out, st, total);
+    if (rc == FFEA_GPU_OK && node_draws) rc = ffea_gpu_batch_set_node_draw_offset(*out, 0, elem_all + nodes_before - before);
+    if (rc != FFEA_GPU_OK) {
+        ffea_gpu_blob_destroy(*out);
+        *out = nullptr;
+    }
     return rc;
 }
 

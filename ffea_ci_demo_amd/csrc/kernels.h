@@ -397,6 +397,39 @@ __global__ void k_rng_advance(int nblob, uint32_t *rng, const mrg::Jump *step_ju
     for (int i = 0; i < 6; ++i) rng[6 * b + i] = s[i];
 }
 
+// Stokes noise (Blob.cpp:3200-3222): after the element pass's draws, 3 draws per node for
+// every node of the blob's .node file (midside and pinned ones included), in node order;
+// node i's force loses RAND(-.5,.5) * sqrt(24 kT drag_i / dt) per component. The
+// subtrahend is stored for the gather, which applies and folds it.
+struct NoiseArgs {
+    int nnodes;                    // nodes of all blobs
+    const int *node_blob;          // [nnodes] or null (one blob)
+    const int *node_off;           // [nblob] first node of each blob
+    const double *sc;              // [nnodes] sqrt(24 kT drag / dt)
+    const uint32_t *rng;           // [nblob][6] stream state at the start of the step's draws
+    const mrg::Jump *node_jump;    // [nblob] A^(7 * elements of the blob)
+    const mrg::Jump *jump_group;   // [ceil(max nodes / 64)] A^(3*64*g)
+    const mrg::Jump *jump_lane;    // [64] A^(3*l)
+    double *noise;                 // [nnodes][3]
+};
+__global__ __launch_bounds__(kBlock) void k_node_noise(NoiseArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.nnodes) return;
+    const int blob = a.node_blob ? a.node_blob[i] : 0;
+    const int il = i - a.node_off[blob];
+    uint32_t st[6];
+    for (int k = 0; k < 6; ++k) st[k] = a.rng[6 * blob + k];
+    mrg::d_apply(a.node_jump[blob], st);
+    mrg::d_apply(a.jump_group[il >> 6], st);
+    mrg::d_apply(a.jump_lane[il & 63], st);
+    const double sc = a.sc[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double u = mrg::d_u01(st);
+        a.noise[3 * i + c] = ((-.5) + ((.5) - (-.5)) * u) * sc;
+    }
+}
+
 // ---------------------------------------------------------------- matrix assembly
 template <typename T>
 struct AsmArgs {
@@ -410,6 +443,7 @@ struct AsmArgs {
     const T *A, *B;
     T *val;                      // [nslots*9]
     double *dinv;                // tiled [nrows][3]
+    const double *drag;          // [nrows] Stokes drag of unpinned rows, or null
 };
 
 // Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
@@ -475,6 +509,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma unroll
                 for (int jj = 0; jj < 3; ++jj) acc[3 * i + jj] += (double)visc_entry<T>(d, A, B, vol, i, pa, jj, pb);
         }
+        if (a.drag && j == a.diag_j[n]) {  // NoMassCGSolver.cpp:119-127: after the element sources
+            const double dr = a.drag[n];
+            acc[0] += dr;
+            acc[4] += dr;
+            acc[8] += dr;
+        }
 #pragma unroll
         for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)acc[k];
         if (j == a.diag_j[n]) {
@@ -502,7 +542,11 @@ struct GatherArgs {
     const T *force;             // [ne][12]
     const double *ctf;          // [nrows][3] or null: constant forces summed in list order
     const uint32_t *fold_ptr;   // [nrows+1] or null
-    const double *fold;         // [..][3]: linearise_force increments in element order
+    const double *fold;         // [..][3]: linearise_force increments in element order (static)
+    const double *noise;        // [nodes][3] Stokes noise subtrahends, or null
+    const int *row_node;        // [nrows] node (noise index) of each corner row (with noise)
+    const int3 *fold_mid;       // [..]: midside nodes folded onto the row per record (with noise)
+    const double *node_ctf;     // [nodes][3] constant forces per node (with noise), or null
     const unsigned char *pinned;
     const double *dinv;
     const double *pos;          // [nrows][3] (rigid modes)
@@ -539,12 +583,40 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
                 f[1] += (double)src[1];
                 f[2] += (double)src[2];
             }
-            if (a.fold_ptr)
+            if (a.noise) {
+                // the node's own noise, then Blob::linearise_force (Blob.cpp:781-799): each
+                // record is one (element, corner) in element order; its midside nodes not
+                // folded by an earlier element contribute half of (constant force - noise)
+                const int g = a.row_node[n];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[k] -= a.noise[3 * g + k];
+                for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
+                    const int3 mm = a.fold_mid[k];
+                    const int m3[3] = {mm.x, mm.y, mm.z};
+                    double s3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        double val[3] = {0.0, 0.0, 0.0};
+                        if (m3[t] >= 0) {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const double cf = a.node_ctf ? a.node_ctf[3 * m3[t] + c] : 0.0;
+                                val[c] = cf - a.noise[3 * m3[t] + c];
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) s3[c] = t == 0 ? val[c] : s3[c] + val[c];
+                    }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) f[c] += 0.5 * s3[c];
+                }
+            } else if (a.fold_ptr) {
                 for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
                     f[0] += a.fold[3 * k];
                     f[1] += a.fold[3 * k + 1];
                     f[2] += a.fold[3 * k + 2];
                 }
+            }
             if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) a.rhs[vi(n, k)] = f[k];

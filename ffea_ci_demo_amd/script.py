@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _native
+from . import _native, units
 from .workloads import BlobSpec
 
 
@@ -91,10 +91,13 @@ class Script:
     def blob_spec(self, b: int = 0) -> BlobSpec:
         a = self.blob_arrays(b)
         p = self.params()
+        drag = None
+        if p["calc_stokes"]:  # Blob::init (Blob.cpp:264-268)
+            drag = units.stokes_drag(a["stokes_radius"], p["stokes_visc"])
         return BlobSpec(pos=a["pos"], top=a["top"], A=a["A"], B=a["B"], G=a["G"], E=a["E"], dt=p["dt"], kT=p["kT"],
                         epsilon2=p["epsilon2"], max_iterations_cg=p["max_iterations_cg"], calc_noise=p["calc_noise"],
                         rng_state=self.rng_state(), pinned=a["pinned"], ctf_node=a["ctf_node"],
-                        ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], name=self.path)
+                        ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path)
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

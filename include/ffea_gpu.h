@@ -54,7 +54,7 @@ typedef struct {
     double epsilon2;         /* SimulationParams::epsilon2 (epsilon squared) */
     int max_iterations_cg;   /* SimulationParams::max_iterations_cg */
     int calc_noise;          /* SimulationParams::calc_noise */
-    int calc_stokes;         /* SimulationParams::calc_stokes (must be 0 in this version) */
+    int calc_stokes;         /* SimulationParams::calc_stokes (1: per-node Stokes drag and noise) */
     int precision;           /* ffea_gpu_precision */
     int device;              /* HIP device ordinal */
 } ffea_gpu_params;
@@ -83,7 +83,10 @@ typedef struct {
  *   pinned     [num_pinned] node indices (Blob::load_pinned_nodes, Blob.cpp:3024-3076)
  *   ctf_node, ctf_force  [num_ctf], [num_ctf][3] constant linear node forces in meso
  *              units, applied in list order (Blob::apply_ctforces, Blob.cpp:1679-1683)
- *   stokes_drag [num_nodes] or NULL (must be NULL in this version)
+ *   stokes_drag [num_nodes] 6 pi stokes_visc stokes_radius per node (meso units; Blob.cpp:264-268),
+ *              used when params->calc_stokes = 1: drag on the viscosity matrix diagonal of
+ *              unpinned nodes (NoMassCGSolver.cpp:119-127) and, with calc_noise, 3 draws per
+ *              node of noise force after the element draws (Blob.cpp:3200-3222)
  *   rng_state  6 x uint32: the state of the blob's noise stream (rng[0] of World,
  *              World.cpp:259-295)
  * On success *out holds the handle. */
@@ -101,26 +104,28 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
  * is [num_blobs][6], one noise stream per blob, each moving 7 draws per element of its
  * blob per step (ffea_gpu_rng_stream_state gives independent RngStream streams). Every
  * kernel launch covers all blobs; each blob has its own CG scalars and stops its solve
- * on its own convergence test. Single-blob calls act on blob 0; the ffea_gpu_batch_*
+ * on its own convergence test; stokes_drag (may be NULL without calc_stokes) is per blob as for
+ * ffea_gpu_blob_create. Single-blob calls act on blob 0; the ffea_gpu_batch_*
  * calls take the blob index. A step fails if any blob fails (the message names it). */
 int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
                           const int *num_elements, const double *const *pos, const int *const *top,
                           const double *const *A, const double *const *B, const double *const *G,
                           const double *const *E, const int *num_pinned, const int *const *pinned,
                           const int *num_ctf, const int *const *ctf_node,
-                          const double *const *ctf_force, const uint32_t *rng_states,
-                          ffea_gpu_blob **out);
+                          const double *const *ctf_force, const double *const *stokes_drag,
+                          const uint32_t *rng_states, ffea_gpu_blob **out);
 
 /* Builds the device tables of ffea_gpu_batch_create's arguments on the host, without a
  * device, and checks that every index the kernels form from them stays inside the
  * buffer it addresses (the kernels do not check). Fills *info (may be NULL; fields that
  * need a device are 0). ffea_gpu_batch_create runs the same check before any upload.
  * No counterpart in the reference. */
-int ffea_gpu_check_layout(int precision, int num_blobs, const int *num_nodes, const int *num_elements,
-                          const double *const *pos, const int *const *top, const double *const *A,
-                          const double *const *B, const double *const *G, const double *const *E,
-                          const int *num_pinned, const int *const *pinned, const int *num_ctf,
-                          const int *const *ctf_node, const double *const *ctf_force,
+int ffea_gpu_check_layout(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
+                          const int *num_elements, const double *const *pos, const int *const *top,
+                          const double *const *A, const double *const *B, const double *const *G,
+                          const double *const *E, const int *num_pinned, const int *const *pinned,
+                          const int *num_ctf, const int *const *ctf_node,
+                          const double *const *ctf_force, const double *const *stokes_drag,
                           const uint32_t *rng_states, ffea_gpu_blob_info *info);
 
 /* releases device memory and the stream; NULL is accepted */
@@ -164,6 +169,12 @@ int ffea_gpu_blob_set_rng_state(ffea_gpu_blob *blob, const uint32_t state[6]);
  * per element of this blob). World::run draws blob after blob from rng[0] in one
  * thread (World.cpp:2031-2038), so blob b starts 7 * (elements of blobs 0..b-1) on. */
 int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *blob, const uint32_t state[6], long long draws_per_step);
+
+/* With calc_stokes and calc_noise a step draws 7 per element, then 3 per node. World::run
+ * takes the element draws of all blobs first and the node draws after them
+ * (World.cpp:2034-2058), so in a multi-blob world blob b's node draws start `offset` draws
+ * after its own element draws (default: 7 x its elements). */
+int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *blob, int index, long long offset);
 
 /* state after `ndraws` draws from `state` (exact jump-ahead; host only) */
 int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]);

@@ -66,3 +66,24 @@ def test_layout_rejects_bad_topology():
         check_layout([spec, bad])
     assert "blob 1" in str(ei.value) and "out of bounds" in str(ei.value)
     assert ei.value.status == _native.ERR_INVALID
+
+
+def stokes_spec(n=3, seed=1, radius=0.3e-9, noise=1):
+    """a cube blob with Stokes drag on every node (radius in m)"""
+    from ffea_ci_demo_amd import units
+    spec = workloads.cube_blob(n, rng_seed=seed, calc_noise=noise)
+    visc = 1e-3 / (units.PRESSURE * units.TIME)
+    spec.calc_stokes = 1
+    spec.stokes_drag = units.stokes_drag(np.full(spec.num_nodes, radius * units.scale_to_meso(1.0)), visc)
+    return spec
+
+
+def test_stokes_layout_has_fold_records():
+    specs = [stokes_spec(3), stokes_spec(2, seed=4)]
+    pinned = workloads.config3(n=3)
+    pinned.calc_stokes = 1
+    pinned.stokes_drag = stokes_spec(3).stokes_drag
+    pinned.calc_noise = 1
+    specs.append(pinned)
+    info = check_layout(specs)
+    assert info["num_blobs"] == 3
