@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libffea_gpu.so")
 
-OK, ERR_INVERSION, ERR_CG_NO_CONVERGE, ERR_INVALID, ERR_HIP, ERR_NO_DEVICE = range(6)
+OK, ERR_INVERSION, ERR_CG_NO_CONVERGE, ERR_INVALID, ERR_HIP, ERR_NO_DEVICE, ERR_IO = range(7)
 FP64, MIXED = 0, 1
 NUM_PHASES = 6
 PHASES = ("element", "assemble", "gather_cg_start", "cg_spmv", "cg_vector", "update")
@@ -30,7 +30,9 @@ EXPORTED = [
     "ffea_gpu_batch_get_cg_iterations", "ffea_gpu_batch_get_rng_state", "ffea_gpu_batch_set_rng_stream",
     "ffea_gpu_rng_stream_state", "ffea_gpu_advance_rng_state", "ffea_gpu_blob_set_row_blocks",
     "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout", "ffea_gpu_blob_set_material",
-    "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint",
+    "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint", "ffea_gpu_batch_strain_energy",
+    "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
+    "ffea_world_last_error",
 ]
 
 
@@ -99,6 +101,16 @@ def lib():
     L.ffea_gpu_blob_set_material.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.ffea_gpu_blob_run_checkpointed.argtypes = [vp, C.c_int, C.c_int]
     L.ffea_gpu_blob_adjoint.argtypes = [vp, _dp, vp, vp, vp, vp, vp]
+    L.ffea_gpu_batch_strain_energy.argtypes = [vp, _dp]
+    L.ffea_world_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
+    L.ffea_world_run.argtypes = [vp, C.c_longlong]
+    L.ffea_world_current_step.argtypes = [vp]
+    L.ffea_world_current_step.restype = C.c_longlong
+    L.ffea_world_blob.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.ffea_world_destroy.argtypes = [vp]
+    L.ffea_world_destroy.restype = None
+    L.ffea_world_last_error.argtypes = [vp]
+    L.ffea_world_last_error.restype = C.c_char_p
     L.ffea_gpu_blob_last_error.argtypes = [vp]
     L.ffea_gpu_blob_last_error.restype = C.c_char_p
     L.ffea_gpu_last_error.argtypes = []

@@ -68,6 +68,13 @@ class _Handle:
         self._check(_native.lib().ffea_gpu_blob_get_info(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in _native.Info._fields_}
 
+    def strain_energies(self):
+        """Blob::calculate_strain_energy (Blob.cpp:1052-1064) of every blob at its current
+        positions, meso energy units"""
+        out = np.zeros(self.info()["num_blobs"])
+        self._check(_native.lib().ffea_gpu_batch_strain_energy(self.h, out))
+        return out
+
     def set_row_blocks(self, blocks):
         self._check(_native.lib().ffea_gpu_blob_set_row_blocks(self.h, int(blocks)))
 
@@ -151,6 +158,9 @@ class GpuBlob(_Handle):
 
     def update_positions(self):
         self._check(_native.lib().ffea_gpu_blob_update_positions(self.h))
+
+    def strain_energy(self):
+        return float(self.strain_energies()[0])
 
     # -- state
     def _get(self, fn, shape):

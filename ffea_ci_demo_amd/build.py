@@ -7,9 +7,11 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libffea_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", "ffea_blob.hip"), os.path.join(HERE, "csrc", "ffea_script.cpp")]
-HEADERS = [os.path.join(HERE, "csrc", "mrg32k3a.h"), os.path.join(ROOT, "include", "ffea_gpu.h"),
-           os.path.join(ROOT, "include", "ffea_script.h")]
+SOURCES = [os.path.join(HERE, "csrc", "ffea_blob.hip"), os.path.join(HERE, "csrc", "ffea_script.cpp"),
+           os.path.join(HERE, "csrc", "ffea_world.cpp")]
+HEADERS = [os.path.join(HERE, "csrc", "mrg32k3a.h"), os.path.join(HERE, "csrc", "kernels.h"),
+           os.path.join(HERE, "csrc", "script_internal.h"), os.path.join(ROOT, "include", "ffea_gpu.h"),
+           os.path.join(ROOT, "include", "ffea_script.h"), os.path.join(ROOT, "include", "ffea_world.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -833,6 +833,8 @@ struct ffea_gpu_blob {
     int zero_rhs_solved = 0;  // CG: a zero right-hand side is solved by x = 0 (adjoint solves)
     int run_steps = 0, run_interval = 0;
     DevBuf<double> ckpt, seg_x, seg_v, x_end, v_end, lam, grad;
+    DevBuf<double> energy_w, energy_out;  // strain energy per element / per blob
+    DevBuf<int> elem_cnt;
     DevBuf<double> d_w;
     DevBuf<float> f_w;
 
@@ -1879,6 +1881,38 @@ int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *b, int blob, long long of
         FFEA_HIP_TRY(hipSetDevice(b->device));
         const mrg::Jump j = mrg::h_pow((unsigned long long)offset);
         FFEA_HIP_TRY(hipMemcpyAsync(b->node_jump.p + blob, &j, sizeof(j), hipMemcpyHostToDevice, b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_batch_strain_energy(ffea_gpu_blob *b, double *energy) {
+    if (!b || !energy) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        if (!b->energy_w.p) {
+            b->energy_w.alloc(b->ne, &b->dev_bytes);
+            b->energy_out.alloc(b->nblob, &b->dev_bytes);
+            std::vector<int> cnt(b->nblob);
+            for (int i = 0; i < b->nblob; ++i) cnt[i] = b->meta[i].ne;
+            b->elem_cnt.alloc(b->nblob, &b->dev_bytes);
+            b->elem_cnt.upload(cnt, b->stream);
+        }
+        if (b->mixed)
+            hipLaunchKernelGGL(k_strain_energy<float>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->pos.p, (const double *)b->rest.p,
+                               (const float *)b->f_jinv0.p, (const float *)b->f_vol0.p, (const float *)b->f_G.p,
+                               (const float *)b->f_E.p, b->energy_w.p);
+        else
+            hipLaunchKernelGGL(k_strain_energy<double>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->pos.p, (const double *)b->rest.p,
+                               (const double *)b->d_jinv0.p, (const double *)b->d_vol0.p, (const double *)b->d_G.p,
+                               (const double *)b->d_E.p, b->energy_w.p);
+        hipLaunchKernelGGL(k_blob_energy, dim3(b->nblob), dim3(kBlock), 0, b->stream, (const int *)b->elem_off.p,
+                           (const int *)b->elem_cnt.p, (const double *)b->energy_w.p, b->energy_out.p);
+        FFEA_HIP_TRY(hipGetLastError());
+        FFEA_HIP_TRY(hipMemcpyAsync(energy, b->energy_out.p, sizeof(double) * b->nblob, hipMemcpyDeviceToHost,
+                                    b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         return (int)FFEA_GPU_OK;
     });

@@ -17,6 +17,7 @@
 
 #include "../../include/ffea_gpu.h"
 #include "../../include/ffea_script.h"
+#include "script_internal.h"
 
 void ffea_internal_set_error(const std::string &msg);  // ffea_blob.hip
 
@@ -163,6 +164,9 @@ struct Params {
     ffea_script_params p{};
     int es_N[3] = {-1, -1, -1};
     double ssint_cutoff = 3e-9 / meso::length;
+    // output and checkpoint files (SimulationParams.cpp:682-749, 972-990), as written: the
+    // script's directory is prepended when the runner resolves them
+    std::string traj_out, meas_out, ckpt_in, ckpt_out;
 };
 
 // defaults of SimulationParams::SimulationParams (SimulationParams.cpp:26-117)
@@ -222,6 +226,10 @@ void assign(Params &P, const std::string &lv, const std::string &rv) {
     else if (lv == "calc_stokes" || lv == "do_stokes") p.calc_stokes = atoi(r);
     else if (lv == "stokes_visc") p.stokes_visc = atof(r) / (meso::pressure * meso::time);
     else if (lv == "ssint_cutoff" || lv == "vdw_cutoff") P.ssint_cutoff = atof(r) / meso::length;
+    else if (lv == "trajectory_out_fname") P.traj_out = rv;
+    else if (lv == "measurement_out_fname") P.meas_out = rv;
+    else if (lv == "checkpoint_in") P.ckpt_in = rv;
+    else if (lv == "checkpoint_out") P.ckpt_out = rv;
     else {
         static const char *accepted[] = {
             "num_rods", "num_couplings", "num_conformations", "num_states", "es_update", "kinetics_update",
@@ -659,6 +667,22 @@ void load_script(ffea_script &S) {
 }
 
 }  // namespace
+
+bool ffea_internal_script_files(const ffea_script *s, ScriptFiles *out) {
+    if (!s || !out) return false;
+    const std::string dir = dirname_of(s->path);
+    std::string base = s->path.substr(s->path.find_last_of('/') == std::string::npos ? 0 : s->path.find_last_of('/') + 1);
+    const size_t dot = base.find_last_of('.');
+    const std::string stem = dot == std::string::npos ? base : base.substr(0, dot);
+    const Params &P = s->P;
+    out->script = s->path;
+    out->trajectory = P.traj_out.empty() ? dir + "/" + stem + "/.ftj" : join(dir, P.traj_out);
+    out->measurement = P.meas_out.empty() ? dir + "/" + stem + "/.fm" : join(dir, P.meas_out);
+    out->checkpoint_in = P.ckpt_in.empty() ? std::string() : join(dir, P.ckpt_in);
+    if (!P.ckpt_out.empty()) out->checkpoint_out = join(dir, P.ckpt_out);
+    else out->checkpoint_out = (dot == std::string::npos ? s->path : s->path.substr(0, s->path.size() - (base.size() - dot))) + ".fcp";
+    return true;
+}
 
 extern "C" {
 

@@ -1,0 +1,460 @@
+// The World loop of an FFEA script over GPU blob handles: trajectory (.ftj), global
+// measurement (.fm) and checkpoint (.fcp) files in the reference's formats, and restart
+// from them. One handle per blob; each blob keeps its own positions on the GPU and the
+// host only sees them at output steps.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "../../include/ffea_gpu.h"
+#include "../../include/ffea_script.h"
+#include "../../include/ffea_world.h"
+#include "script_internal.h"
+
+void ffea_internal_set_error(const std::string &msg);  // ffea_blob.hip
+
+namespace {
+
+// mesoDimensions (dimensions.h): length, Energy and time in SI
+constexpr double kLength = 1.7e-10;
+constexpr double kEnergy = 4.141945559999999e-21;
+constexpr double kTime = 3.7302915560886126e-13;
+constexpr double kPressure = 8.4305832688784826e8;
+
+struct WorldError {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw WorldError{code, msg}; }
+
+bool file_exists(const std::string &p) {
+    struct stat st;
+    return stat(p.c_str(), &st) == 0;
+}
+
+// SimulationParams::checkFileName (SimulationParams.cpp:779-802): an existing output file
+// is renamed to __<name>__bckp.N in its directory, N the first number not yet taken
+void move_aside(const std::string &path) {
+    if (!file_exists(path)) return;
+    const size_t slash = path.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? std::string() : path.substr(0, slash + 1);
+    const std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
+    const std::string base = dir + "__" + name + "__bckp.";
+    int n = 1;
+    while (file_exists(base + std::to_string(n))) ++n;
+    if (rename(path.c_str(), (base + std::to_string(n)).c_str()) != 0)
+        fail(FFEA_GPU_ERR_IO, "could not move " + path + " aside: " + strerror(errno));
+    fprintf(stderr, "FFEA: moving %s to %s\n", path.c_str(), (base + std::to_string(n)).c_str());
+}
+
+std::string read_whole(const std::string &path) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) fail(FFEA_GPU_ERR_IO, "failed to open " + path);
+    std::string s;
+    char buf[1 << 16];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, k);
+    fclose(f);
+    return s;
+}
+
+void truncate_to(const std::string &path, long long size) {
+    if (truncate(path.c_str(), (off_t)size) != 0)
+        fail(FFEA_GPU_ERR_IO, "error when trying to truncate " + path + ": " + strerror(errno));
+}
+
+std::vector<std::string> split_spaces(const std::string &line) {  // boost::split on single spaces
+    std::vector<std::string> out(1);
+    for (char c : line) {
+        if (c == ' ') out.emplace_back();
+        else out.back() += c;
+    }
+    return out;
+}
+
+}  // namespace
+
+struct ffea_world {
+    ScriptFiles files;
+    ffea_script_params p{};
+    int precision = FFEA_GPU_FP64;
+    std::vector<ffea_gpu_blob *> blobs;
+    std::vector<int> nt;
+    std::vector<std::vector<double>> pos0;  // positions at set-up, the RMSD origin (World.cpp:505-513)
+    std::vector<long long> draws_before;   // noise draws of the blobs before this one in a step
+    std::vector<std::string> old_seeds;     // extra seed lines of an earlier multi-threaded run
+    FILE *traj = nullptr, *meas = nullptr, *ckpt = nullptr;
+    long long step = 0, step_initial = 0;
+    std::vector<double> host;
+    std::string err;
+
+    ~ffea_world() {
+        for (FILE *f : {traj, meas, ckpt})
+            if (f) fclose(f);
+        for (ffea_gpu_blob *b : blobs) ffea_gpu_blob_destroy(b);
+    }
+
+    void check_rc(int rc, int blob, const char *what) {
+        if (rc == FFEA_GPU_OK) return;
+        const char *m = blob >= 0 ? ffea_gpu_blob_last_error(blobs[blob]) : ffea_gpu_last_error();
+        fail(rc, std::string(what) + (blob >= 0 ? " (blob " + std::to_string(blob) + ")" : std::string()) + ": " +
+                     (m ? m : ""));
+    }
+
+    // World::init headers (World.cpp:527-603)
+    void write_headers(const char *script_path) {
+        fprintf(traj, "FFEA_trajectory_file\n\nInitialisation:\nNumber of Blobs %d\nNumber of Conformations",
+                p.num_blobs);
+        for (int i = 0; i < p.num_blobs; ++i) fprintf(traj, " %d", 1);
+        fprintf(traj, "\n");
+        for (int i = 0; i < p.num_blobs; ++i) fprintf(traj, "Blob %d:\tConformation %d Nodes %d\t\n", i, 0, nt[i]);
+        fprintf(traj, "\n*\n");
+        fflush(traj);
+
+        fprintf(meas, "FFEA Global Measurement File\n\nSimulation Details:\n");
+        fprintf(meas, "\tGPU blob path: libffea_gpu (HIP, gfx950), %s\n",
+                precision == FFEA_GPU_MIXED ? "mixed FP32/FP64" : "FP64");
+        const time_t now = time(nullptr);
+        const tm *t = localtime(&now);
+        fprintf(meas, "\tSimulation Began on %d/%d/%d at %d:%d:%d\n", t->tm_mday, 1 + t->tm_mon, 1900 + t->tm_year,
+                t->tm_hour, t->tm_min, t->tm_sec);
+        fprintf(meas, "\tScript Filename = %s\n", script_path);
+        fprintf(meas, "\tSimulation Type = %s\n\n", "Full");
+        // SimulationParams::write_to_file (SimulationParams.cpp:1075-1233), the keys of this path
+        fprintf(meas, "Parameters:\n\tSystem parameters:\n");
+        fprintf(meas, "\trestart = %d\n", p.restart);
+        fprintf(meas, "\tdt = %e\n", p.dt * kTime);
+        fprintf(meas, "\tnum_steps = %lld\n", p.num_steps);
+        fprintf(meas, "\tcheck = %d\n", p.check);
+        fprintf(meas, "\trng_seed = %d\n", p.rng_seed);
+        fprintf(meas, "\tkT = %e\n", p.kT * kEnergy);
+        fprintf(meas, "\tmax_iterations_cg = %d\n", p.max_iterations_cg);
+        fprintf(meas, "\tepsilon = %e\n", std::sqrt(p.epsilon2));
+        if (p.calc_stokes == 1) fprintf(meas, "\tstokes_visc = %e\n", p.stokes_visc * kPressure * kTime);
+        fprintf(meas, "\tnum_blobs = %d\n", p.num_blobs);
+        fprintf(meas, "\tnum_rods = %d\n", 0);
+        fprintf(meas, "\n\tCalculations enabled:\n");
+        fprintf(meas, "\tcalc_noise = %d\n\tcalc_stokes = %d\n", p.calc_noise, p.calc_stokes);
+        fprintf(meas, "\tcalc_ssint = %d\n\tcalc_steric = %d\n\tcalc_preComp = %d\n\tcalc_springs = %d\n", 0, 0, 0, 0);
+        fprintf(meas, "\tcalc_ctforces = %d\n\tcalc_kinetics = %d\n\tcalc_es = %d\n", p.calc_ctforces, 0, 0);
+        fprintf(meas, "\n\tSimulation box configuration:\n\tmove_into_box = %d\n", p.move_into_box);
+        fprintf(meas, "\n\n");
+        fprintf(meas, "Measurements:\n");
+        fprintf(meas, "%-14s", "Time");
+        fprintf(meas, "%-14s", "StrainEnergy");
+        fprintf(meas, "%-14s%-14s%-14s%-14s", "Centroid[0]", "Centroid[1]", "Centroid[2]", "RMSD");
+        fprintf(meas, "\n");
+        fflush(meas);
+    }
+
+    // World::print_checkpoints (World.cpp:4546-4596): the noise stream's state before this
+    // step's forces; the blobs share it in turn, blob 0 first, so blob 0's state is the stream's
+    void print_checkpoint() {
+        uint32_t s[6];
+        check_rc(ffea_gpu_blob_get_rng_state(blobs[0], s), 0, "reading the noise stream state");
+        rewind(ckpt);
+        fprintf(ckpt, "RNGStreams dedicated to the thermal stress: %d\n", 1 + (int)old_seeds.size());
+        fprintf(ckpt, "%u %u %u %u %u %u\n", s[0], s[1], s[2], s[3], s[4], s[5]);
+        for (const std::string &l : old_seeds) fprintf(ckpt, "%s\n", l.c_str());
+        fflush(ckpt);
+    }
+
+    // trajectory frame (World.cpp:4461-4488, Blob::write_nodes_to_file Blob.cpp:855-889)
+    // and global measurements (World::make_measurements, write_measurements_to_file,
+    // World.cpp:4598-4692; Blob::make_measurements Blob.cpp:1066-1202)
+    void print_outputs(long long s) {
+        double strain = 0.0, cog[3] = {0, 0, 0}, rmsd2 = 0.0;
+        long long nodes = 0;
+        for (int b = 0; b < p.num_blobs; ++b) {
+            host.resize(3 * (size_t)nt[b]);
+            check_rc(ffea_gpu_blob_get_positions(blobs[b], host.data()), b, "reading positions");
+            fprintf(traj, "Blob %d, Conformation %d, step %d\n", b, 0, (int)s);
+            fprintf(traj, "DYNAMIC\n");
+            double c[3] = {0, 0, 0}, r = 0.0;
+            for (int i = 0; i < nt[b]; ++i) {
+                const double *x = &host[3 * (size_t)i];
+                fprintf(traj, "%e %e %e %e %e %e %e %e %e %e\n", x[0] * kLength, x[1] * kLength, x[2] * kLength, 0., 0.,
+                        0., 0., 0., 0., 0.);
+                for (int k = 0; k < 3; ++k) {
+                    c[k] += x[k];
+                    const double d = x[k] - pos0[b][3 * (size_t)i + k];
+                    r += d * d;
+                }
+            }
+            double e = 0.0;
+            check_rc(ffea_gpu_batch_strain_energy(blobs[b], &e), b, "strain energy");
+            strain += e;
+            const double rb = std::sqrt(r / nt[b]);
+            rmsd2 += rb * rb * nt[b];
+            for (int k = 0; k < 3; ++k) cog[k] += c[k] / nt[b] * nt[b];
+            nodes += nt[b];
+        }
+        fprintf(traj, "*\n");
+        fprintf(traj, "Conformation Changes:\n");
+        for (int b = 0; b < p.num_blobs; ++b) fprintf(traj, "Blob %d: Conformation %d -> Conformation %d\n", b, 0, 0);
+        fprintf(traj, "*\n");
+        fflush(traj);
+
+        const double rmsd = std::sqrt(rmsd2 / nodes);
+        for (double &c : cog) c *= 1.0 / nodes;
+        fprintf(meas, "%-14.6e", (int)s * p.dt * kTime);
+        fprintf(meas, "%-14.6e", strain * kEnergy);
+        fprintf(meas, "%-14.6e%-14.6e%-14.6e", cog[0] * kLength, cog[1] * kLength, cog[2] * kLength);
+        fprintf(meas, "%-14.6e", rmsd * kLength);
+        fprintf(meas, "\n");
+        fflush(meas);
+    }
+
+    // restart (World.cpp:302-426): the checkpoint's first thermal stream becomes the stream;
+    // each blob starts at its own offset in it
+    void read_checkpoint() {
+        const std::string text = read_whole(files.checkpoint_in);
+        std::vector<std::string> lines;
+        size_t a = 0;
+        while (a < text.size()) {
+            size_t e = text.find('\n', a);
+            if (e == std::string::npos) e = text.size();
+            lines.push_back(text.substr(a, e - a));
+            a = e + 1;
+        }
+        if (lines.empty()) fail(FFEA_GPU_ERR_IO, "Error reading the number of stress seeds: empty checkpoint");
+        int nseeds = 0;
+        {
+            const std::string last = split_spaces(lines[0]).back();
+            char *end = nullptr;
+            const long v = strtol(last.c_str(), &end, 10);
+            if (end == last.c_str()) fail(FFEA_GPU_ERR_IO, "Error reading the number of stress seeds: stoi");
+            nseeds = (int)v;
+        }
+        if (nseeds < 1 || (int)lines.size() < nseeds + 1) fail(FFEA_GPU_ERR_IO, "ERROR reading seeds");
+        uint32_t seed[6];
+        for (int i = 1; i <= nseeds; ++i) {
+            const std::vector<std::string> v = split_spaces(lines[i]);
+            if (v.size() != 6) fail(FFEA_GPU_ERR_IO, "ERROR reading seeds");
+            for (int j = 0; j < 6; ++j) {
+                char *end = nullptr;
+                const unsigned long u = strtoul(v[j].c_str(), &end, 10);
+                if (end == v[j].c_str()) fail(FFEA_GPU_ERR_IO, "Error reading seeds as integers: stoul");
+                if (i == 1) seed[j] = (uint32_t)u;
+            }
+            if (i > 1) old_seeds.push_back(lines[i]);
+        }
+        for (int b = 0; b < p.num_blobs; ++b) {
+            uint32_t st[6];
+            check_rc(ffea_gpu_advance_rng_state(seed, draws_before[b], st), -1, "checkpoint stream state");
+            check_rc(ffea_gpu_blob_set_rng_state(blobs[b], st), b, "setting the restart stream state");
+        }
+    }
+
+    // restart from the trajectory (World.cpp:680-836): searching back from the end for the
+    // fourth asterisk finds the start of the last completely written frame; that frame gives
+    // the positions and the step to repeat, and the file is cut back to its start
+    void restart_trajectory() {
+        const std::string t = read_whole(files.trajectory);
+        long long pos = (long long)t.size();
+        int found = 0;
+        bool single = false;
+        while (found != 4) {
+            if (pos - 2 < 0) {  // the file's start: only one frame; take the header's asterisk
+                single = true;
+                size_t q = t.find('*', (size_t)pos);
+                if (q == std::string::npos) fail(FFEA_GPU_ERR_IO, "no complete frame in " + files.trajectory);
+                pos = (long long)q;
+            } else {
+                pos -= 2;
+            }
+            const char c = t[(size_t)pos];
+            pos += 1;
+            if (c == '*') ++found;
+            if (single && found != 4) fail(FFEA_GPU_ERR_IO, "no complete frame in " + files.trajectory);
+        }
+        long long cut = pos;
+        if (pos < (long long)t.size() && t[(size_t)pos] == '\n') cut = ++pos;
+        const char *base = t.c_str();
+        const char *cur = base + pos;
+        auto expect = [&](const char *lit) {
+            const size_t n = strlen(lit);
+            if (strncmp(cur, lit, n) != 0) fail(FFEA_GPU_ERR_IO, std::string("malformed trajectory near '") + lit + "'");
+            cur += n;
+        };
+        auto skip_line = [&]() {
+            const char *e = strchr(cur, '\n');
+            cur = e ? e + 1 : base + t.size();
+        };
+        if (!single) {
+            expect("Conformation Changes:\n");
+            for (int b = 0; b < p.num_blobs; ++b) skip_line();
+            expect("*\n");
+            cut = cur - base;
+        }
+        long long rstep = -1;
+        for (int b = 0; b < p.num_blobs; ++b) {
+            int id = -1, conf = -1, used = 0;
+            if (sscanf(cur, "Blob %d, Conformation %d, step %lld\n%n", &id, &conf, &rstep, &used) != 3 || used == 0)
+                fail(FFEA_GPU_ERR_IO, "Error reading header info for Blob " + std::to_string(b));
+            if (id != b)
+                fail(FFEA_GPU_ERR_IO, "Mismatch in trajectory file - found blob id = " + std::to_string(id) +
+                                          ", was expecting blob id = " + std::to_string(b));
+            cur += used;
+            skip_line();  // DYNAMIC
+            host.resize(3 * (size_t)nt[b]);
+            for (int i = 0; i < nt[b]; ++i) {
+                double v[10];
+                char *end = const_cast<char *>(cur);
+                for (int k = 0; k < 10; ++k) {
+                    const char *start = end;
+                    v[k] = strtod(start, &end);
+                    if (end == start)
+                        fail(FFEA_GPU_ERR_IO,
+                             "(When restarting) Error reading from trajectory file, for node " + std::to_string(i));
+                }
+                cur = end;
+                while (*cur == ' ' || *cur == '\n' || *cur == '\t' || *cur == '\r') ++cur;
+                for (int k = 0; k < 3; ++k) host[3 * (size_t)i + k] = v[k] / kLength;
+            }
+            check_rc(ffea_gpu_blob_set_positions(blobs[b], host.data()), b, "setting restart positions");
+        }
+        step_initial = rstep;
+        truncate_to(files.trajectory, cut);
+
+        // the measurement file loses its last line, the step that is repeated (World.cpp:838-866)
+        const std::string m = read_whole(files.measurement);
+        long long mp = (long long)m.size();
+        for (;;) {
+            if (mp - 2 < 0) fail(FFEA_GPU_ERR_IO, "Error when trying to find last frame from file " + files.measurement);
+            mp -= 2;
+            const char c = m[(size_t)mp];
+            mp += 1;
+            if (c == '\n') break;
+        }
+        truncate_to(files.measurement, mp);
+    }
+};
+
+extern "C" {
+
+int ffea_world_create(const char *path, int precision, int device, ffea_world **out) {
+    if (!path || !out) {
+        ffea_internal_set_error("ffea_world_create: invalid argument");
+        return FFEA_GPU_ERR_INVALID;
+    }
+    *out = nullptr;
+    ffea_script *s = nullptr;
+    int rc = ffea_script_load(path, &s);
+    if (rc != FFEA_GPU_OK) return rc;
+    ffea_world *w = new ffea_world();
+    w->precision = precision;
+    try {
+        ffea_script_get_params(s, &w->p);
+        ffea_internal_script_files(s, &w->files);
+        const ffea_script_params &p = w->p;
+        // SimulationParams::validate (SimulationParams.cpp:805-1009), the file checks
+        if (p.restart != 0 && p.restart != 1)
+            fail(FFEA_GPU_ERR_INVALID, "Required: Restart flag, 'restart', must be 0 (false) or 1 (true).");
+        if (p.restart == 1 && w->files.checkpoint_in.empty())
+            fail(FFEA_GPU_ERR_INVALID, "Checkpoint input file required if restart is required.");
+        if (w->files.checkpoint_out == w->files.checkpoint_in)
+            fail(FFEA_GPU_ERR_INVALID,
+                 "it is not allowed to set up checkpoint_in and checkpoint_out with the same file names");
+        if (p.check < 1) fail(FFEA_GPU_ERR_INVALID, "'check' must be at least 1");
+
+        long long before = 0;
+        for (int b = 0; b < p.num_blobs; ++b) {
+            int nn = 0, ne = 0, npin = 0, nctf = 0, nf = 0;
+            ffea_script_blob_sizes(s, b, &nn, &ne, &npin, &nctf, &nf);
+            ffea_gpu_blob *h = nullptr;
+            rc = ffea_gpu_blob_create_from_script(s, b, precision, device, &h);
+            if (rc != FFEA_GPU_OK) fail(rc, ffea_gpu_last_error());
+            w->blobs.push_back(h);
+            w->nt.push_back(nn);
+            w->pos0.emplace_back(3 * (size_t)nn);
+            w->check_rc(ffea_gpu_blob_get_positions(h, w->pos0.back().data()), b, "reading initial positions");
+            w->draws_before.push_back(before);
+            before += 7ll * ne;
+        }
+        ffea_script_free(s);
+        s = nullptr;
+
+        move_aside(w->files.checkpoint_out);
+        if (p.restart == 0) {
+            move_aside(w->files.measurement);
+            move_aside(w->files.trajectory);
+        }
+        if (p.restart == 1) w->read_checkpoint();
+        if (!(w->ckpt = fopen(w->files.checkpoint_out.c_str(), "wb")))
+            fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.checkpoint_out);
+        if (p.restart == 0) {
+            if (!(w->traj = fopen(w->files.trajectory.c_str(), "wb")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.trajectory);
+            if (!(w->meas = fopen(w->files.measurement.c_str(), "wb")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.measurement);
+            w->write_headers(path);
+        } else {
+            w->restart_trajectory();
+            if (!(w->traj = fopen(w->files.trajectory.c_str(), "ab")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.trajectory);
+            if (!(w->meas = fopen(w->files.measurement.c_str(), "ab")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.measurement);
+        }
+        w->step = w->step_initial;
+    } catch (const WorldError &e) {
+        if (s) ffea_script_free(s);
+        ffea_internal_set_error(e.msg);
+        delete w;
+        return e.code;
+    }
+    *out = w;
+    return FFEA_GPU_OK;
+}
+
+int ffea_world_run(ffea_world *w, long long max_steps) {
+    if (!w) return FFEA_GPU_ERR_INVALID;
+    const ffea_script_params &p = w->p;
+    long long done = 0;
+    try {
+        // World::run (World.cpp:1747-2060)
+        for (; w->step < p.num_steps + 1 && (max_steps < 0 || done < max_steps); ++w->step, ++done) {
+            const bool out = w->step % p.check == 0;
+            if (out) w->print_checkpoint();
+            for (int b = 0; b < p.num_blobs; ++b) {
+                const int rc = ffea_gpu_blob_update_internal_forces(w->blobs[b]);
+                if (rc != FFEA_GPU_OK) {
+                    const std::string m = ffea_gpu_blob_last_error(w->blobs[b]);
+                    // World::die_with_dignity (World.cpp:4841-4855): the failing step is dumped
+                    if (w->step != w->step_initial) w->print_outputs(w->step);
+                    fail(rc, "step " + std::to_string(w->step) + ", blob " + std::to_string(b) + ": " + m);
+                }
+            }
+            if (out) w->print_outputs(w->step);
+            for (int b = 0; b < p.num_blobs; ++b)
+                w->check_rc(ffea_gpu_blob_update_positions(w->blobs[b]), b, "update_positions");
+        }
+    } catch (const WorldError &e) {
+        w->err = e.msg;
+        ffea_internal_set_error(e.msg);
+        return e.code;
+    }
+    return FFEA_GPU_OK;
+}
+
+long long ffea_world_current_step(const ffea_world *w) { return w ? w->step : -1; }
+
+int ffea_world_blob(ffea_world *w, int index, ffea_gpu_blob **blob) {
+    if (!w || !blob || index < 0 || index >= (int)w->blobs.size()) return FFEA_GPU_ERR_INVALID;
+    *blob = w->blobs[index];
+    return FFEA_GPU_OK;
+}
+
+void ffea_world_destroy(ffea_world *w) { delete w; }
+
+const char *ffea_world_last_error(const ffea_world *w) { return w ? w->err.c_str() : ffea_gpu_last_error(); }
+
+}  // extern "C"

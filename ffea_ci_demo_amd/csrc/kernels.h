@@ -1297,6 +1297,56 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_gather(RowArgs ra, const uin
     }
 }
 
+// ---------------------------------------------------------------- measurements
+// Blob::calculate_strain_energy (Blob.cpp:1052-1064) per element, in FP64 from the current
+// positions: W = vol0 [G (tr F^T F - 3) + C (det F^2 - 1)/2 - (2G + C) ln det F], C = E - 2G/3,
+// with F = I + H formed from displacements (tr F^T F - 3 = 2 tr H + |H|^2,
+// det F = 1 + e as in strain_terms) so that small strains keep their digits
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_strain_energy(int ne, const int *conn, const double *pos, const double *rest,
+                                                          const T *jinv0, const T *vol0, const T *G, const T *E,
+                                                          double *w) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    int n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[k] = conn[k * ne + e];
+    double u0[3], DJ[3][3], Ji[3][3], H[3][3], FFm[3][3], ee;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u0[c] = pos[3 * n[0] + c] - rest[3 * n[0] + c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) DJ[r][c] = (pos[3 * n[r + 1] + c] - rest[3 * n[r + 1] + c]) - u0[c];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ji[k / 3][k % 3] = (double)jinv0[(size_t)k * ne + e];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) H[i][j] = DJ[0][i] * Ji[j][0] + DJ[1][i] * Ji[j][1] + DJ[2][i] * Ji[j][2];
+    strain_terms<double, double>(H, FFm, &ee);
+    double hh = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) hh += H[i][j] * H[i][j];
+    const double g = (double)G[e], C = (double)E[e] - g * 2.0 / 3.0;
+    const double trc = 2.0 * (H[0][0] + H[1][1] + H[2][2]) + hh;
+    // vol / vol_0 = |det F| (the element volume is unsigned), so an inverted element takes log|det F|
+    const double lj = ee > -1.0 ? log1p(ee) : log(-1.0 - ee);
+    w[e] = (double)vol0[e] * (g * trc + 0.5 * C * ee * (2.0 + ee) - (2.0 * g + C) * lj);
+}
+
+// per blob (one workgroup each): half the sum of its elements' W
+__global__ __launch_bounds__(kBlock) void k_blob_energy(const int *elem_off, const int *elem_cnt, const double *w,
+                                                        double *out) {
+    const int b = blockIdx.x;
+    double v[1] = {0.0};
+    for (int e = elem_off[b] + threadIdx.x; e < elem_off[b] + elem_cnt[b]; e += kBlock) v[0] += w[e];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) out[b] = 0.5 * v[0];
+}
+
 // ---------------------------------------------------------------- copy probe
 // y = x over n2 16-byte vectors, grid-stride: reads n2*16 and writes n2*16 bytes
 __global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {

@@ -1,12 +1,11 @@
-// Runs the blobs of an .ffea script on the GPU for num_steps + 1 updates (the
-// reference's World::run loop length, World.cpp:1747) and prints progress every
-// `check` steps: the mass-less blob path of `ffea <script>` (ffea.cpp:53-308).
+// `ffea <script>` for the mass-less blob path (ffea.cpp:53-308): every blob of an .ffea
+// script on the GPU through the World loop, writing the script's trajectory, measurement
+// and checkpoint files (restart = 1 resumes from them).
 //   run_script <script.ffea> [max_updates] [fp64|mixed]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <memory>
 #include <vector>
 
 #include "ffea_gpu.hpp"
@@ -17,24 +16,22 @@ int main(int argc, char **argv) {
         return 2;
     }
     try {
-        ffea_gpu::Script script(argv[1]);
-        const ffea_script_params p = script.params();
-        long long updates = p.num_steps + 1;
-        if (argc > 2) updates = std::min(updates, std::atoll(argv[2]));
+        const long long max_updates = argc > 2 ? std::atoll(argv[2]) : -1;
         const int precision = (argc > 3 && std::strcmp(argv[3], "mixed") == 0) ? FFEA_GPU_MIXED : FFEA_GPU_FP64;
-        std::vector<std::unique_ptr<ffea_gpu::Blob>> blobs;
-        for (int b = 0; b < p.num_blobs; ++b) blobs.emplace_back(new ffea_gpu::Blob(script, b, precision));
+        ffea_gpu::World world(argv[1], precision);
+        const long long first = world.step();
         const auto t0 = std::chrono::steady_clock::now();
-        for (long long step = 0; step < updates; ++step) {
-            for (auto &b : blobs) b->update_internal_forces();
-            for (auto &b : blobs) b->update_positions();
-            if (p.check > 0 && step % p.check == 0)
-                std::printf("step %lld: cg iterations %d\n", step, blobs[0]->cg_iterations());
-        }
+        world.run(max_updates);
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        std::vector<double> pos = blobs[0]->positions();
-        std::printf("done: %lld updates in %.3f s; blob 0 node 0 at (%.10e, %.10e, %.10e)\n", updates, s, pos[0],
-                    pos[1], pos[2]);
+        ffea_gpu_blob_info info{};
+        ffea_gpu_blob *b0 = world.blob(0);
+        ffea_gpu_blob_get_info(b0, &info);
+        std::vector<double> pos((size_t)3 * info.num_nodes);
+        ffea_gpu_blob_get_positions(b0, pos.data());
+        int iters = 0;
+        ffea_gpu_blob_get_cg_iterations(b0, &iters, nullptr);
+        std::printf("done: %lld updates in %.3f s (last cg iterations %d); blob 0 node 0 at (%.10e, %.10e, %.10e)\n",
+                    world.step() - first, s, iters, pos[0], pos[1], pos[2]);
     } catch (const ffea_gpu::Exception &e) {
         std::fprintf(stderr, "FFEA error (status %d): %s\n", e.status(), e.what());
         return 1;

@@ -38,7 +38,8 @@ typedef enum {
     FFEA_GPU_ERR_CG_NO_CONVERGE = 2, /* "Could not converge after N iterations" (NoMassCGSolver.cpp:187) */
     FFEA_GPU_ERR_INVALID = 3,        /* bad argument or mesh */
     FFEA_GPU_ERR_HIP = 4,            /* HIP runtime failure (message holds the HIP error) */
-    FFEA_GPU_ERR_NO_DEVICE = 5
+    FFEA_GPU_ERR_NO_DEVICE = 5,
+    FFEA_GPU_ERR_IO = 6              /* an input or output file of a world run (FFEAFileException) */
 } ffea_gpu_status;
 
 typedef enum {
@@ -175,6 +176,10 @@ int ffea_gpu_blob_set_rng_stream(ffea_gpu_blob *blob, const uint32_t state[6], l
  * (World.cpp:2034-2058), so in a multi-blob world blob b's node draws start `offset` draws
  * after its own element draws (default: 7 x its elements). */
 int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *blob, int index, long long offset);
+
+/* strain energy of every blob of the handle at the current positions, energy[num_blobs]
+ * (Blob::calculate_strain_energy, Blob.cpp:1052-1064; the measurement file's StrainEnergy) */
+int ffea_gpu_batch_strain_energy(ffea_gpu_blob *blob, double *energy);
 
 /* state after `ndraws` draws from `state` (exact jump-ahead; host only) */
 int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]);

@@ -12,6 +12,7 @@
 
 #include "ffea_gpu.h"
 #include "ffea_script.h"
+#include "ffea_world.h"
 
 namespace ffea_gpu {
 
@@ -120,6 +121,32 @@ private:
     }
     ffea_gpu_blob *b_ = nullptr;
     int num_nodes_ = 0;
+};
+
+// Every blob of a script with the reference's output files: World::init / World::run
+// (World.cpp:155-930, 1747-2060), restart included.
+class World {
+public:
+    explicit World(const std::string &script_path, int precision = FFEA_GPU_FP64, int device = 0) {
+        if (int rc = ffea_world_create(script_path.c_str(), precision, device, &w_))
+            throw Exception(rc, ffea_gpu_last_error());
+    }
+    ~World() { ffea_world_destroy(w_); }
+    World(const World &) = delete;
+    World &operator=(const World &) = delete;
+    // to num_steps (inclusive), or at most max_steps loop iterations
+    void run(long long max_steps = -1) {
+        if (int rc = ffea_world_run(w_, max_steps)) throw Exception(rc, ffea_world_last_error(w_));
+    }
+    long long step() const { return ffea_world_current_step(w_); }
+    ffea_gpu_blob *blob(int index) {
+        ffea_gpu_blob *b = nullptr;
+        if (int rc = ffea_world_blob(w_, index, &b)) throw Exception(rc, "no such blob");
+        return b;
+    }
+
+private:
+    ffea_world *w_ = nullptr;
 };
 
 }  // namespace ffea_gpu

@@ -1,6 +1,6 @@
 """The C++ drop-in (include/ffea_gpu.hpp) driving a script on the GPU: the example
-program runs the reference's World loop for a blob and reports the same positions as
-the Python binding."""
+program runs the reference's World loop for a script, writes its output files and
+reports the same positions as the Python binding."""
 import os
 import shutil
 import subprocess
@@ -34,8 +34,10 @@ def test_cpp_script_runner(tmp_path):
     assert r.returncode == 0, r.stderr
     last = r.stdout.strip().splitlines()[-1]
     assert last.startswith("done: 10 updates")
+    for name in ("b_trajectory.out", "b_measurement.out", "b.fcp"):
+        assert os.path.getsize(tmp_path / name) > 0, name
     g = Script(path).gpu_blob(0)
     g.step(10)
     x0 = g.positions()[0]
-    got = [float(v) for v in last.split("(")[1].rstrip(")").split(",")]
+    got = [float(v) for v in last.rsplit("(", 1)[1].rstrip(")").split(",")]
     assert np.allclose(got, x0, rtol=1e-9, atol=0)
