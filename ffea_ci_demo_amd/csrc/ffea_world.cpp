@@ -382,6 +382,10 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
         }
         ffea_script_free(s);
         s = nullptr;
+        // one blob: whole steps between outputs are replayed as graphs (results identical to
+        // the eager path). With several blobs every stage runs for all blobs in turn, as the
+        // reference does, so a failure leaves every blob at the failing step's positions.
+        if (p.num_blobs == 1) w->check_rc(ffea_gpu_blob_use_graph(w->blobs[0], 1, 0), 0, "graph mode");
 
         move_aside(w->files.checkpoint_out);
         if (p.restart == 0) {
@@ -424,6 +428,17 @@ int ffea_world_run(ffea_world *w, long long max_steps) {
         for (; w->step < p.num_steps + 1 && (max_steps < 0 || done < max_steps); ++w->step, ++done) {
             const bool out = w->step % p.check == 0;
             if (out) w->print_checkpoint();
+            if (!out && p.num_blobs == 1) {
+                // a step with no output: forces and positions in one call (one graph launch
+                // once the iteration count is known); an inversion stops it before the update
+                const int rc = ffea_gpu_blob_step(w->blobs[0], 1);
+                if (rc != FFEA_GPU_OK) {
+                    const std::string m = ffea_gpu_blob_last_error(w->blobs[0]);
+                    if (rc == FFEA_GPU_ERR_INVERSION && w->step != w->step_initial) w->print_outputs(w->step);
+                    fail(rc, "step " + std::to_string(w->step) + ": " + m);
+                }
+                continue;
+            }
             for (int b = 0; b < p.num_blobs; ++b) {
                 const int rc = ffea_gpu_blob_update_internal_forces(w->blobs[b]);
                 if (rc != FFEA_GPU_OK) {

@@ -95,13 +95,18 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int n
 }
 
 // totals of a blob's chunk partials [c0, c1): identical in every block
+// The NV loads of one chunk are issued together (one memory latency per chunk a thread
+// visits, not one per sum); each sum still adds its chunks in ascending order.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double *P, int nchunks, int c0, int c1, double (&out)[NV]) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double acc = 0.0;
-        for (int c = c0 + threadIdx.x; c < c1; c += kBlock) acc += P[(size_t)k * nchunks + c];
-        out[k] = acc;
+    for (int k = 0; k < NV; ++k) out[k] = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += kBlock) {
+        double v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] = P[(size_t)k * nchunks + c];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] += v[k];
     }
     block_sum<NV>(out);
 }
