@@ -46,7 +46,8 @@ class Info(C.Structure):
     _fields_ = [("num_nodes", C.c_int), ("num_elements", C.c_int), ("num_linear_nodes", C.c_int),
                 ("num_blocks", C.c_longlong), ("num_block_slots", C.c_longlong), ("precision", C.c_int),
                 ("device_bytes", C.c_longlong), ("num_blobs", C.c_int), ("num_chunks", C.c_int),
-                ("row_blocks", C.c_int), ("graph_iterations", C.c_int)]
+                ("row_blocks", C.c_int), ("graph_iterations", C.c_int),
+                ("single_launch_cg", C.c_int)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

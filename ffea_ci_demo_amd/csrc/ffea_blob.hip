@@ -815,6 +815,9 @@ struct ffea_gpu_blob {
     DevBuf<int3> fold_mid;
     DevBuf<mrg::Jump> node_jump, njump_group, njump_lane;
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
+    int max_slice_w = 1;  // widest slice: the assembly grid's second dimension
+    // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
+    bool fused_cg = false;
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
 
     // host-visible status (pinned)
@@ -922,6 +925,9 @@ struct ffea_gpu_blob {
     template <typename T, bool PROJ>
     void launch_gather(const T *force);
     void launch_cg_iteration(int it, bool spmv_only);
+    template <typename T, bool PROJ>
+    void launch_cg_fused_t(const T *val);
+    void launch_cg_fused();
     void launch_cg_vec(int it);
     template <typename T, bool PROJ>
     void launch_cg_iteration_t(int it, bool spmv_only, const T *val);
@@ -984,6 +990,9 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     any_project = t.any_project;
     meta = std::move(t.meta);
     grid_rows = (nrows + kBlock - 1) / kBlock;
+    for (int w : t.sw) max_slice_w = std::max(max_slice_w, w);
+    fused_cg = nblob > 0;
+    for (const BlobMeta &m : meta) fused_cg = fused_cg && m.nl <= kFusedMaxRows;
     grid_elems = (ne + kBlock - 1) / kBlock;
     grid_blobs = (nblob + kBlock - 1) / kBlock;
     conn.alloc(t.conn.size(), &dev_bytes); conn.upload(t.conn, stream);
@@ -1183,11 +1192,11 @@ void ffea_gpu_blob::launch_assemble() {
     if (mixed) {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
                          f_val.p, dinv.p, stokes ? drag.p : nullptr};
-        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
+        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_rows, max_slice_w), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
                           d_val.p, dinv.p, stokes ? drag.p : nullptr};
-        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_rows), dim3(kBlock), 0, stream, a);
+        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_rows, max_slice_w), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(1, t0);
@@ -1227,6 +1236,27 @@ void ffea_gpu_blob::launch_cg_iteration(int it, bool spmv_only) {
     }
 }
 
+template <typename T, bool PROJ>
+void ffea_gpu_blob::launch_cg_fused_t(const T *val) {
+    SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
+                   pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
+    VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
+    hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(kBlock), 0, stream, sa, va, (const double *)Pi.p);
+    FFEA_HIP_TRY(hipGetLastError());
+}
+
+void ffea_gpu_blob::launch_cg_fused() {
+    hipEvent_t ts = nullptr;
+    if (t_cg(3, 0)) t_begin(3, &ts);
+    if (mixed) {
+        if (any_project) launch_cg_fused_t<float, true>(f_val.p);
+        else launch_cg_fused_t<float, false>(f_val.p);
+    } else {
+        launch_cg_fused_t<double, false>(d_val.p);
+    }
+    t_end(3, ts, 0);
+}
+
 void ffea_gpu_blob::launch_update() {
     hipEvent_t t0;
     t_begin(5, &t0);
@@ -1243,6 +1273,10 @@ void ffea_gpu_blob::launch_update() {
 // solved blob done. Returns the first iteration not queued.
 int ffea_gpu_blob::run_cg_eager(int start_iter) {
     const int imax = prm.max_iterations_cg;
+    if (fused_cg && start_iter == 0) {  // the whole solve in one launch
+        launch_cg_fused();
+        return imax + 1;
+    }
     int launched = start_iter, chunk_idx = 0;
     int chunk = std::max(4, expect_iters - 2 - start_iter);
     bool stop = false;
@@ -1308,7 +1342,12 @@ void ffea_gpu_blob::capture_graph(int iters) {
     try {
         launch_element();
         launch_assemble_gather();
-        for (int it = 0; it <= iters; ++it) launch_cg_iteration(it, it == iters);
+        if (fused_cg) {
+            launch_cg_fused();
+            iters = prm.max_iterations_cg;
+        } else {
+            for (int it = 0; it <= iters; ++it) launch_cg_iteration(it, it == iters);
+        }
         launch_update();
     } catch (...) {
         capturing = false;
@@ -1330,7 +1369,7 @@ int ffea_gpu_blob::step_once() {
         // enough iterations to cover the last step's count with a margin, in steps of 16
         int want = std::min(prm.max_iterations_cg, ((expect_iters + expect_iters / 8 + 8 + 15) / 16) * 16);
         if (graph_fixed_iters > 0) want = std::min(prm.max_iterations_cg, graph_fixed_iters);
-        if (!graph_exec || want > graph_iters || want < graph_iters / 2) capture_graph(want);
+        if (!graph_exec || (!fused_cg && (want > graph_iters || want < graph_iters / 2))) capture_graph(want);
         FFEA_HIP_TRY(hipGraphLaunch(graph_exec, stream));
         FFEA_HIP_TRY(hipMemcpyAsync(&h_done[0], num_done.p, sizeof(int), hipMemcpyDeviceToHost, stream));
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
@@ -1976,6 +2015,7 @@ int ffea_gpu_blob_get_info(ffea_gpu_blob *b, ffea_gpu_blob_info *info) {
     info->num_chunks = b->nchunks;
     info->row_blocks = b->nb;
     info->graph_iterations = b->graph_exec ? b->graph_iters : 0;
+    info->single_launch_cg = b->fused_cg ? 1 : 0;
     return FFEA_GPU_OK;
 }
 

@@ -22,6 +22,7 @@ constexpr int kBlock = 256;          // threads per workgroup: 4 wavefronts of 6
 constexpr int kSlice = 64;           // rows per ELL slice (one wavefront)
 constexpr int kChunkRows = 1024;     // rows per reduction chunk (4 per thread)
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;
+constexpr int kFusedMaxRows = 1024;  // blobs up to this many rows solve in one launch (k_cg_fused)
 constexpr int kMaxUnrolledRow = 16;  // rows with up to this many blocks take the unrolled SpMV path
 
 // CG node vectors are tiled by 64 nodes: x[64] y[64] z[64] per tile, so row-wise
@@ -477,6 +478,8 @@ __device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int 
 
 // SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
 // 305-312): every block entry is the sum of its element sources in element order.
+// Grid (row blocks, widest slice): blockIdx.y is the slot j, so each lane follows the
+// source chain of one block only (the rows' slots are independent).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma clang fp contract(off)
@@ -485,8 +488,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
     const int sl = n / kSlice, lane = n % kSlice;
     const int W = a.slice_w[sl];
     const long long base = a.slice_base[sl];
-    const int ne = a.ne;
-    for (int j = 0; j < W; ++j) {
+    const int j = blockIdx.y;
+    if (j >= W) return;
+    {
         const long long slot = base + (long long)j * kSlice + lane;
         const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
         double acc[9];
@@ -783,8 +787,7 @@ __device__ __forceinline__ void load_block<float>(const float *slot, int lane, f
 }
 
 template <typename T, bool PROJ>
-__device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
-    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+__device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange cr) {
     const int i = a.iter;
     const bool first = (i == 0);
     int cur = -1;
@@ -916,14 +919,14 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a) {
 
 template <typename T, bool PROJ>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
-    cg_spmv<T, PROJ>(a);
+    cg_spmv<T, PROJ>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
 // the projected FP32 step is held to 128 VGPRs (4 waves per SIMD): one 1024-row chunk
 // per workgroup then fits the 6M-tet blob's 1007 chunks on the 256 CUs in one round
 template <>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
     SpmvArgs<float> a) {
-    cg_spmv<float, true>(a);
+    cg_spmv<float, true>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
 
 struct VecArgs {
@@ -945,9 +948,8 @@ struct VecArgs {
 // q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
 // projected form: per blob (one workgroup each) the 13 sums of the SpMV step, the
 // projection coefficients c = (Z^T Z)^-1 Z^T q and alpha = delta / (p.q - (Z^T p).c)
-__global__ __launch_bounds__(kBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
-                                                    double *coef) {
-    const int b = blockIdx.x;
+__device__ __forceinline__ void cg_coef(const RowArgs &ra, int iter, const double *P1, const double *G,
+                                        double *coef, int b) {
     const CgScalars *S = ra.S + b;
     if (S->done) return;
     const int2 sg = ra.seg[b];
@@ -965,10 +967,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_coef(RowArgs ra, int iter, const 
         for (int k = 0; k < 6; ++k) coef[8 * b + 1 + k] = cq[k];
     }
 }
+__global__ __launch_bounds__(kBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
+                                                    double *coef) {
+    cg_coef(ra, iter, P1, G, coef, blockIdx.x);
+}
 
 template <bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
-    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+__device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
     int cur = -1;
     bool skip = false;
     double alpha = 0.0;
@@ -1018,6 +1023,38 @@ __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
             v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
         }
         store_partials<2>(v, a.P2, a.ra.nchunks, c);
+    }
+}
+template <bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
+    cg_vec<PROJ>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
+}
+
+// Small blobs: the whole solve in one launch, one workgroup per blob running the SpMV
+// step, (projected form) the coefficient step and the vector step of each iteration
+// back to back over the blob's chunks. Partials, scalars and every sum are those of
+// the separate kernels, so the results are identical; a barrier with a workgroup fence
+// between the steps stands in for the kernel boundary.
+template <typename T, bool PROJ>
+__global__ __launch_bounds__(kBlock) void k_cg_fused(SpmvArgs<T> sa, VecArgs va, const double *Pstart) {
+    const int b = blockIdx.x;
+    const int2 sg = sa.ra.seg[b];
+    const ChunkRange cr{sg.x, sg.y};
+    const CgScalars *S = sa.ra.S + b;
+    double *const P2 = va.P2;
+    for (int it = 0; it <= sa.max_iter; ++it) {
+        sa.iter = it;
+        sa.Pin = it == 0 ? Pstart : P2;
+        cg_spmv<T, PROJ>(sa, cr);
+        __syncthreads();
+        if (S->done || it == sa.max_iter) break;  // the step of max_iter only tests max_iter - 1
+        va.iter = it;
+        if constexpr (PROJ) {
+            cg_coef(va.ra, it, va.P1, va.G, const_cast<double *>(va.coef), b);
+            __syncthreads();
+        }
+        cg_vec<PROJ>(va, cr);
+        __syncthreads();
     }
 }
 

@@ -73,6 +73,8 @@ typedef struct {
     int num_chunks;              /* reduction chunks (up to 1024 node rows of one blob each) */
     int row_blocks;              /* workgroups of the row kernels (gather, CG, update) */
     int graph_iterations;        /* CG iterations in the captured step graph (0: none) */
+    int single_launch_cg;        /* 1: every blob is small enough (up to 1024 corner nodes) that
+                                    its whole CG solve runs in one launch, one workgroup per blob */
 } ffea_gpu_blob_info;
 
 /* Blob::init + NoMassCGSolver::init (Blob.cpp:161-247, NoMassCGSolver.cpp:56-153).

@@ -20,22 +20,32 @@ POS_TOL = 1e-10
 FORCE_TOL = 1e-12
 
 
-def small_ensemble(noise=1):
-    """5 blobs of uneven size and material, one pinned and loaded, own noise streams"""
+def small_ensemble(noise=1, big=False):
+    """5 blobs of uneven size and material, one pinned and loaded, own noise streams;
+    all small enough for the single-launch CG, unless `big` adds a 1728-row blob, which
+    puts the whole handle on the chunked CG"""
     specs = workloads.ensemble(4, nmin=2, nmax=6, seed=5, calc_noise=noise)
     pinned = workloads.config3(n=3)
     pinned.calc_noise = noise
     pinned.rng_state = _native.rng_stream_state([12345] * 6, 4)
     specs.append(pinned)
+    if big:
+        large = workloads.cube_blob(11, mesh_seed=2, rng_seed=4, calc_noise=noise)
+        large.rng_state = _native.rng_stream_state([12345] * 6, 5)
+        specs.append(large)
     print("[case] ensemble", [s.name for s in specs], flush=True)
     return specs
 
 
-def test_batch_equals_single_blobs_and_oracle(oracle_mod):
-    specs = small_ensemble()
+@pytest.mark.parametrize("big", [False, True])
+def test_batch_equals_single_blobs_and_oracle(oracle_mod, big):
+    """with `big` the batch runs the chunked CG while each small blob alone runs the
+    single-launch CG: the two must agree bit for bit"""
+    specs = small_ensemble(big=big)
     nsteps = 8
     with GpuBatch(specs) as batch:
         assert batch.info()["num_blobs"] == len(specs)
+        assert batch.info()["single_launch_cg"] == (0 if big else 1)
         batch.step(nsteps)
         last, total = batch.cg_iterations()
         for b, s in enumerate(specs):
@@ -63,9 +73,10 @@ def test_batch_streams_advance_per_blob():
             assert np.array_equal(batch.rng_state(b), want), b
 
 
-def test_graph_replays_equal_eager_steps():
+@pytest.mark.parametrize("big", [False, True])
+def test_graph_replays_equal_eager_steps(big):
     """every replay of the captured step against an eager twin, with host reads between"""
-    specs = small_ensemble()
+    specs = small_ensemble(big=big)
     with GpuBatch(specs) as eager, GpuBatch(specs) as graph:
         graph.use_graph(True)
         for k in range(8):
@@ -93,7 +104,7 @@ def test_graph_single_blob_matches_oracle(oracle_mod):
 def test_graph_with_short_capture_continues_eagerly():
     """a graph holding fewer CG iterations than a step needs: the step finishes eagerly
     and equals the eager step"""
-    specs = small_ensemble()
+    specs = small_ensemble(big=True)  # chunked CG: the single-launch CG needs no continuation
     with GpuBatch(specs) as eager, GpuBatch(specs) as graph:
         graph.use_graph(True, iterations=5)
         for k in range(4):
