@@ -1165,10 +1165,10 @@ void ffea_gpu_blob::launch_gather(const T *force) {
                     stokes_noise ? noise.p : nullptr, stokes_noise ? row_node.p : nullptr,
                     stokes_noise ? fold_mid.p : nullptr, stokes_noise && have_ctf ? node_ctf.p : nullptr, pinned.p,
                     dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
-    hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, g);
+    hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
-        hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kBlock), 0, stream, sa);
+        hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kMixedBlock), 0, stream, sa);
     }
 }
 
@@ -1208,7 +1208,7 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
     if (t_cg(3, it)) t_begin(3, &ts);
     SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
-    hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(kBlock), 0, stream, sa);
+    hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, sa);
     t_end(3, ts, it);
     if (!spmv_only) launch_cg_vec(it);
 }
@@ -1218,11 +1218,14 @@ void ffea_gpu_blob::launch_cg_vec(int it) {
     if (t_cg(4, it)) t_begin(4, &tv);
     VecArgs va{row_args(), it, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
     if (mixed && any_project) {
-        hipLaunchKernelGGL(k_cg_coef, dim3(nblob), dim3(kBlock), 0, stream, row_args(), it, (const double *)P1.p,
+        hipLaunchKernelGGL(k_cg_coef, dim3(nblob), dim3(kMixedBlock), 0, stream, row_args(), it, (const double *)P1.p,
                            (const double *)Gmat.p, coef.p);
-        hipLaunchKernelGGL(k_cg_vec<true>, dim3(nb), dim3(kBlock), 0, stream, va);
+        hipLaunchKernelGGL((k_cg_vec<true, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
     } else {
-        hipLaunchKernelGGL(k_cg_vec<false>, dim3(nb), dim3(kBlock), 0, stream, va);
+        if (mixed)
+            hipLaunchKernelGGL((k_cg_vec<false, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
+        else
+            hipLaunchKernelGGL((k_cg_vec<false, kRowBlock>), dim3(nb), dim3(kRowBlock), 0, stream, va);
     }
     t_end(4, tv, it);
 }
@@ -1241,7 +1244,7 @@ void ffea_gpu_blob::launch_cg_fused_t(const T *val) {
     SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
     VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
-    hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(kBlock), 0, stream, sa, va, (const double *)Pi.p);
+    hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(row_block<T>), 0, stream, sa, va, (const double *)Pi.p);
     FFEA_HIP_TRY(hipGetLastError());
 }
 
@@ -1260,7 +1263,7 @@ void ffea_gpu_blob::launch_cg_fused() {
 void ffea_gpu_blob::launch_update() {
     hipEvent_t t0;
     t_begin(5, &t0);
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(kBlock), 0, stream, row_args(), (const double *)x.p, vel.p, pos.p,
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)x.p, vel.p, pos.p,
                        prm.dt);
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
     FFEA_HIP_TRY(hipGetLastError());
@@ -1453,7 +1456,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     launch_element();  // shape data of x_k for the operator
     launch_assemble();
     AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p};
-    hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kBlock), 0, stream, st);
+    hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kRowBlock), 0, stream, st);
     FFEA_HIP_TRY(hipGetLastError());
     zero_rhs_solved = 1;
     try {
@@ -1476,7 +1479,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
         a.jinv0 = f_jinv0.p; a.vol0 = f_vol0.p; a.G = f_G.p; a.E = f_E.p; a.A = f_A.p; a.B = f_B.p;
     }
     hipLaunchKernelGGL(k_element_adjoint<T>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
-    hipLaunchKernelGGL(k_adjoint_gather<T>, dim3(nb), dim3(kBlock), 0, stream, row_args(), (const uint32_t *)inc_ptr.p,
+    hipLaunchKernelGGL(k_adjoint_gather<T>, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const uint32_t *)inc_ptr.p,
                        (const uint32_t *)inc.p, (const T *)w, prm.dt, lam.p);
     FFEA_HIP_TRY(hipGetLastError());
 }
@@ -1947,7 +1950,7 @@ int ffea_gpu_batch_strain_energy(ffea_gpu_blob *b, double *energy) {
                                (const int *)b->conn.p, (const double *)b->pos.p, (const double *)b->rest.p,
                                (const double *)b->d_jinv0.p, (const double *)b->d_vol0.p, (const double *)b->d_G.p,
                                (const double *)b->d_E.p, b->energy_w.p);
-        hipLaunchKernelGGL(k_blob_energy, dim3(b->nblob), dim3(kBlock), 0, b->stream, (const int *)b->elem_off.p,
+        hipLaunchKernelGGL(k_blob_energy, dim3(b->nblob), dim3(kRowBlock), 0, b->stream, (const int *)b->elem_off.p,
                            (const int *)b->elem_cnt.p, (const double *)b->energy_w.p, b->energy_out.p);
         FFEA_HIP_TRY(hipGetLastError());
         FFEA_HIP_TRY(hipMemcpyAsync(energy, b->energy_out.p, sizeof(double) * b->nblob, hipMemcpyDeviceToHost,

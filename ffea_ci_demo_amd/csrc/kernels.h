@@ -6,7 +6,8 @@
 // contributes one partial sum per reduced quantity, and the kernel that needs a
 // blob's total adds that blob's chunk partials in chunk order. Totals are therefore
 // independent of the grid size and of which block processed which chunk, and the
-// grid of the row kernels is a free tuning parameter.
+// grid of the row kernels is a free tuning parameter. A row-kernel workgroup has one
+// thread per row of a chunk, so a chunk's rows are all in flight at once.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,9 +19,16 @@
 
 namespace ffk {
 
-constexpr int kBlock = 256;          // threads per workgroup: 4 wavefronts of 64
+constexpr int kBlock = 256;          // threads per workgroup of the element kernels: 4 wavefronts of 64
+constexpr int kRowBlock = 1024;      // threads per workgroup of the row kernels: one row of a chunk each
+constexpr int kMixedBlock = 256;     // ... of the FP32-mixed form's: its projected kernels keep 13-17 FP64
+                                     // sums per row, more than the 128 VGPRs of a 1024-thread workgroup;
+                                     // its unprojected ones match them, so a blob sums alike alone or
+                                     // batched with projected blobs
+template <typename T>
+constexpr int row_block = std::is_same<T, double>::value ? kRowBlock : kMixedBlock;
 constexpr int kSlice = 64;           // rows per ELL slice (one wavefront)
-constexpr int kChunkRows = 1024;     // rows per reduction chunk (4 per thread)
+constexpr int kChunkRows = 1024;     // rows per reduction chunk (one per thread of a row workgroup)
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;
 constexpr int kFusedMaxRows = 1024;  // blobs up to this many rows solve in one launch (k_cg_fused)
 constexpr int kMaxUnrolledRow = 16;  // rows with up to this many blocks take the unrolled SpMV path
@@ -66,9 +74,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // block sum of NV values; result valid in every thread
-template <int NV>
+template <int NV, int BS>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[kBlock / 64][NV];
+    __shared__ double sh[BS / 64][NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -80,16 +88,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     for (int k = 0; k < NV; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) s += sh[w][k];
+        for (int w = 0; w < BS / 64; ++w) s += sh[w][k];
         v[k] = s;
     }
     __syncthreads();
 }
 
 // chunk partials, stored [k][nchunks]
-template <int NV>
+template <int NV, int BS>
 __device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int nchunks, int c) {
-    block_sum<NV>(v);
+    block_sum<NV, BS>(v);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) P[(size_t)k * nchunks + c] = v[k];
@@ -98,18 +106,18 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double *P, int n
 // totals of a blob's chunk partials [c0, c1): identical in every block
 // The NV loads of one chunk are issued together (one memory latency per chunk a thread
 // visits, not one per sum); each sum still adds its chunks in ascending order.
-template <int NV>
+template <int NV, int BS>
 __device__ __forceinline__ void sum_partials(const double *P, int nchunks, int c0, int c1, double (&out)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = 0.0;
-    for (int c = c0 + threadIdx.x; c < c1; c += kBlock) {
+    for (int c = c0 + threadIdx.x; c < c1; c += BS) {
         double v[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] = P[(size_t)k * nchunks + c];
 #pragma unroll
         for (int k = 0; k < NV; ++k) out[k] += v[k];
     }
-    block_sum<NV>(out);
+    block_sum<NV, BS>(out);
 }
 
 // The chunks of this block: a contiguous range; blocks are mapped so that each of the
@@ -569,8 +577,9 @@ struct GatherArgs {
 // mixed form) the CG start is left to k_cg_start and the partials are |f|^2, Z^T f
 // and the moments of Z^T Z.
 template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
+__global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
 #pragma clang fp contract(off)
+    constexpr int BS = row_block<T>;
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.ra.num_done = 0;
     constexpr int NV = PROJ ? 17 : 2;
@@ -580,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] = 0.0;
         const double3 cen = a.centre[ch.blob];
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
             double f[3] = {0.0, 0.0, 0.0};
             if (a.ctf) {
                 f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
@@ -653,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs<T> a) {
                 v[14] += x[0] * x[1]; v[15] += x[0] * x[2]; v[16] += x[1] * x[2];
             }
         }
-        store_partials<NV>(v, a.P, a.ra.nchunks, c);
+        store_partials<NV, BS>(v, a.P, a.ra.nchunks, c);
         if (c == ch.first && threadIdx.x == 0) {
             CgScalars *S = a.ra.S + ch.blob;
             S->inverted_step = S->inverted;
@@ -674,7 +683,7 @@ struct StartArgs {
     const int *project;         // [nblob]: 1 = free blob, project; 0 = leave r = f
     double *r, *z, *x, *P, *G;  // G: [nblob][36] ((Z^T Z)^-1, zero for unprojected blobs)
 };
-__global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
+__global__ __launch_bounds__(kMixedBlock) void k_cg_start(StartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
     // (Z^T Z)^-1 and its product with Z^T f: formed once per blob by thread 0, shared
     __shared__ double sGi[36], scf[6];
@@ -686,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
             cur = ch.blob;
             const int2 sg = a.ra.seg[cur];
             double t[17];
-            sum_partials<17>(a.Pg, a.ra.nchunks, sg.x, sg.y, t);
+            sum_partials<17, kMixedBlock>(a.Pg, a.ra.nchunks, sg.x, sg.y, t);
             if (threadIdx.x == 0) {
                 double M[36], Gi[36];
                 const double mom[10] = {t[7], t[8], t[9], t[10], t[11], t[12], t[13], t[14], t[15], t[16]};
@@ -710,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
         }
         const double3 cen = a.centre[cur];
         double v[1] = {0.0};
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kMixedBlock) {
             const double xx[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
             double zc[3], rl[3], zl[3];
             z_times(xx, cf, zc);
@@ -724,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(StartArgs a) {
             }
             v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
         }
-        store_partials<1>(v, a.P, a.ra.nchunks, c);  // its block sum also orders the sGi reads before reuse
+        store_partials<1, kMixedBlock>(v, a.P, a.ra.nchunks, c);  // its block sum also orders the sGi reads before reuse
         if (c == ch.first && threadIdx.x == 0) {
             a.ra.S[cur].f2 = f2;
             for (int i = 0; i < 36; ++i) a.G[36 * cur + i] = sGi[i];
@@ -788,6 +797,7 @@ __device__ __forceinline__ void load_block<float>(const float *slot, int lane, f
 
 template <typename T, bool PROJ>
 __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange cr) {
+    constexpr int BS = row_block<T>;
     const int i = a.iter;
     const bool first = (i == 0);
     int cur = -1;
@@ -810,11 +820,11 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
                     skip = true;
                 } else if constexpr (PROJ) {
                     double t[1];
-                    sum_partials<1>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
+                    sum_partials<1, BS>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
                     if (lead) S->delta[1] = t[0];
                 } else {
                     double t[2];
-                    sum_partials<2>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
+                    sum_partials<2, BS>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);
                     if (a.zero_rhs_solved && t[1] == 0.0) {
                         if (lead) {
                             S->converged = 1;
@@ -830,7 +840,7 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
                 }
             } else if (!skip) {
                 double t[2];
-                sum_partials<2>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);  // r.r, r.z of iteration i-1
+                sum_partials<2, BS>(a.Pin, a.ra.nchunks, sg.x, sg.y, t);  // r.r, r.z of iteration i-1
                 const double f2 = S->f2;
                 const double res = (f2 == 0.0) ? 0.0 : t[0] / f2;
                 if (res < a.epsilon2 || i >= a.max_iter) {
@@ -853,7 +863,7 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
         double v[NV];
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] = 0.0;
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
             const int sl = n / kSlice, lane = n % kSlice;
             const int W = a.slice_w[sl];
             const long long base = a.slice_base[sl];
@@ -913,18 +923,18 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
                 }
             }
         }
-        store_partials<NV>(v, a.P1, a.ra.nchunks, c);
+        store_partials<NV, BS>(v, a.P1, a.ra.nchunks, c);
     }
 }
 
 template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(SpmvArgs<T> a) {
+__global__ __launch_bounds__(row_block<T>) void k_cg_spmv(SpmvArgs<T> a) {
     cg_spmv<T, PROJ>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
 // the projected FP32 step is held to 128 VGPRs (4 waves per SIMD): one 1024-row chunk
 // per workgroup then fits the 6M-tet blob's 1007 chunks on the 256 CUs in one round
 template <>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
     SpmvArgs<float> a) {
     cg_spmv<float, true>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
@@ -954,7 +964,7 @@ __device__ __forceinline__ void cg_coef(const RowArgs &ra, int iter, const doubl
     if (S->done) return;
     const int2 sg = ra.seg[b];
     double t[13];
-    sum_partials<13>(P1, ra.nchunks, sg.x, sg.y, t);
+    sum_partials<13, kMixedBlock>(P1, ra.nchunks, sg.x, sg.y, t);
     if (threadIdx.x == 0) {
         double cq[6], pq = t[0];
         for (int k = 0; k < 6; ++k) {
@@ -967,12 +977,12 @@ __device__ __forceinline__ void cg_coef(const RowArgs &ra, int iter, const doubl
         for (int k = 0; k < 6; ++k) coef[8 * b + 1 + k] = cq[k];
     }
 }
-__global__ __launch_bounds__(kBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
+__global__ __launch_bounds__(kMixedBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
                                                     double *coef) {
     cg_coef(ra, iter, P1, G, coef, blockIdx.x);
 }
 
-template <bool PROJ>
+template <bool PROJ, int BS>
 __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
     int cur = -1;
     bool skip = false;
@@ -992,7 +1002,7 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
                 } else {
                     const int2 sg = a.ra.seg[cur];
                     double t[1];
-                    sum_partials<1>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
+                    sum_partials<1, BS>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
                     alpha = S->delta[(a.iter - 1) & 1] / t[0];
                 }
             }
@@ -1000,7 +1010,7 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
         if (skip) continue;
         const double3 cen = a.centre[cur];
         double v[2] = {0.0, 0.0};
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
             double qn[3] = {a.q[vi(n, 0)], a.q[vi(n, 1)], a.q[vi(n, 2)]};
             if constexpr (PROJ) {
                 const double x[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
@@ -1022,12 +1032,12 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
             v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
             v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
         }
-        store_partials<2>(v, a.P2, a.ra.nchunks, c);
+        store_partials<2, BS>(v, a.P2, a.ra.nchunks, c);
     }
 }
-template <bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
-    cg_vec<PROJ>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
+template <bool PROJ, int BS>
+__global__ __launch_bounds__(BS) void k_cg_vec(VecArgs a) {
+    cg_vec<PROJ, BS>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
 
 // Small blobs: the whole solve in one launch, one workgroup per blob running the SpMV
@@ -1036,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_vec(VecArgs a) {
 // the separate kernels, so the results are identical; a barrier with a workgroup fence
 // between the steps stands in for the kernel boundary.
 template <typename T, bool PROJ>
-__global__ __launch_bounds__(kBlock) void k_cg_fused(SpmvArgs<T> sa, VecArgs va, const double *Pstart) {
+__global__ __launch_bounds__(row_block<T>) void k_cg_fused(SpmvArgs<T> sa, VecArgs va, const double *Pstart) {
     const int b = blockIdx.x;
     const int2 sg = sa.ra.seg[b];
     const ChunkRange cr{sg.x, sg.y};
@@ -1053,21 +1063,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_fused(SpmvArgs<T> sa, VecArgs va,
             cg_coef(va.ra, it, va.P1, va.G, const_cast<double *>(va.coef), b);
             __syncthreads();
         }
-        cg_vec<PROJ>(va, cr);
+        cg_vec<PROJ, row_block<T>>(va, cr);
         __syncthreads();
     }
 }
 
 // Blob::euler_integrate (Blob.cpp:3249-3263), mass-less branch: vel = x, pos += x dt,
 // for every blob whose solve converged and is not yet applied
-__global__ __launch_bounds__(kBlock) void k_update(RowArgs ra, const double *x, double *vel, double *pos,
+__global__ __launch_bounds__(kRowBlock) void k_update(RowArgs ra, const double *x, double *vel, double *pos,
                                                    double dt) {
     const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
     for (int c = cr.c0; c < cr.c1; ++c) {
         const Chunk ch = ra.chunks[c];
         const CgScalars &S = ra.S[ch.blob];
         if (!S.converged || S.updated) continue;
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock)
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const double v = x[vi(n, k)];
@@ -1285,13 +1295,13 @@ struct AdjStartArgs {
     const double *dinv;
     double *rhs, *r, *z, *x, *P;
 };
-__global__ __launch_bounds__(kBlock) void k_adjoint_start(AdjStartArgs a) {
+__global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.ra.num_done = 0;
     for (int c = cr.c0; c < cr.c1; ++c) {
         const Chunk ch = a.ra.chunks[c];
         double v[2] = {0.0, 0.0};
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
             const bool pin = a.pinned[n] != 0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -1305,7 +1315,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_start(AdjStartArgs a) {
                 v[1] += f * f;
             }
         }
-        store_partials<2>(v, a.P, a.ra.nchunks, c);
+        store_partials<2, kRowBlock>(v, a.P, a.ra.nchunks, c);
         if (c == ch.first && threadIdx.x == 0) {
             CgScalars *S = a.ra.S + ch.blob;
             S->inverted_step = 0;
@@ -1319,12 +1329,12 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_start(AdjStartArgs a) {
 
 // lambda += dt * sum over the node's element corners (element order) of w
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_adjoint_gather(RowArgs ra, const uint32_t *inc_ptr, const uint32_t *inc,
+__global__ __launch_bounds__(kRowBlock) void k_adjoint_gather(RowArgs ra, const uint32_t *inc_ptr, const uint32_t *inc,
                                                            const T *w, double dt, double *lam) {
     const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
     for (int c = cr.c0; c < cr.c1; ++c) {
         const Chunk ch = ra.chunks[c];
-        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kBlock) {
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
             double s[3] = {0.0, 0.0, 0.0};
             for (uint32_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
                 const uint32_t es = inc[k];
@@ -1380,12 +1390,12 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(int ne, const int *con
 }
 
 // per blob (one workgroup each): half the sum of its elements' W
-__global__ __launch_bounds__(kBlock) void k_blob_energy(const int *elem_off, const int *elem_cnt, const double *w,
+__global__ __launch_bounds__(kRowBlock) void k_blob_energy(const int *elem_off, const int *elem_cnt, const double *w,
                                                         double *out) {
     const int b = blockIdx.x;
     double v[1] = {0.0};
-    for (int e = elem_off[b] + threadIdx.x; e < elem_off[b] + elem_cnt[b]; e += kBlock) v[0] += w[e];
-    block_sum<1>(v);
+    for (int e = elem_off[b] + threadIdx.x; e < elem_off[b] + elem_cnt[b]; e += kRowBlock) v[0] += w[e];
+    block_sum<1, kRowBlock>(v);
     if (threadIdx.x == 0) out[b] = 0.5 * v[0];
 }
 
