@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "element-steps/s on 1 MI355X (FP32-mixed vs FP64) + HBM roofline fraction"
 PEAK_HBM_GBS = 8000.0
+DEFAULT_FORMS = "fp64,mixed,fp64+mg,mixed+mg"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_cube100_traffic.json")
 
 
@@ -51,11 +52,24 @@ def iter_alg_bytes(info, w):
     return info["num_blocks"] * (9 * w + 4) + 216 * info["num_linear_nodes"]
 
 
+def split_form(form):
+    """"fp64" / "mixed" (the reference's Jacobi-PCG), "fp64+mg" / "mixed+mg" (multigrid-
+    preconditioned CG): (precision, preconditioner)"""
+    prec, _, pc = form.partition("+")
+    return prec, ("multigrid" if pc == "mg" else "jacobi")
+
+
 def run_form(spec, form, steps, warmup):
     from ffea_ci_demo_amd.blob import GpuBlob
+    prec, pc = split_form(form)
     t0 = time.perf_counter()
-    g = GpuBlob.from_spec(spec, precision=form)
+    g = GpuBlob.from_spec(spec, precision=prec)
     t_create = time.perf_counter() - t0
+    mg = None
+    if pc == "multigrid":
+        mg = g.set_preconditioner("multigrid")
+        print(f"[bench] multigrid: levels {mg['rows']} (blocks {mg['block']}), coarsest {mg['coarsest_unknowns']} "
+              f"unknowns, set-up {mg['setup_seconds']:.1f}s", file=sys.stderr, flush=True)
     info = g.info()
     print(f"[bench] {spec.name} form={form} created in {t_create:.1f}s: Ne={info['num_elements']} "
           f"Nl={info['num_linear_nodes']} blocks={info['num_blocks']} device_bytes={info['device_bytes']/1e9:.2f}GB",
@@ -73,7 +87,7 @@ def run_form(spec, form, steps, warmup):
     tim = g.timing()
     _, tot1 = g.cg_iterations()
     iters = tot1 - tot0
-    w = 8 if form == "fp64" else 4
+    w = 8 if prec == "fp64" else 4
     spmv_ms, spmv_n = tim["cg_spmv"]
     vec_ms, vec_n = tim["cg_vector"]
     avg_spmv_s = spmv_ms / max(spmv_n, 1) / 1e3
@@ -89,6 +103,11 @@ def run_form(spec, form, steps, warmup):
         "iteration_model_GBs": iter_alg_bytes(info, w) * iters / wall / 1e9,
         "info": info,
     }
+    if mg is not None:
+        res["vector_avg_us"] = None
+        res["vector_and_vcycle_avg_us"] = vec_ms / max(vec_n, 1) * 1e3
+        res["multigrid"] = {k: mg[k] for k in ("rows", "block", "blocks", "transfer_blocks", "coarsest_unknowns",
+                                                "setup_seconds")}
     print(f"[bench] form={form} {res['ms_per_step']:.1f} ms/step, k={res['cg_iterations_per_step']:.1f}, "
           f"spmv {res['spmv_avg_us']:.1f} us = {res['spmv_achieved_GBs']:.0f} GB/s", file=sys.stderr, flush=True)
     positions = g.positions()
@@ -181,7 +200,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=100, help="cubes per side (100: the 6.0M-tet blob)")
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default=DEFAULT_FORMS,
+                    help="fp64 / mixed: the reference's Jacobi-PCG; +mg: multigrid-preconditioned CG")
     ap.add_argument("--headline", default=None, help="form reported as value (default: last of --forms)")
     ap.add_argument("--workload", default="cube", choices=("cube", "ensemble", "adjoint", "illcond", "drift"))
     ap.add_argument("--run-steps", type=int, default=None,
@@ -209,12 +229,14 @@ def main():
     results, pos = {}, {}
     for f in forms:
         results[f], pos[f] = run_form(spec, f, args.steps, args.warmup)
-    if "fp64" in pos and "mixed" in pos:
-        d = np.abs(pos["mixed"] - pos["fp64"]).max() / np.abs(pos["fp64"]).max()
-        results["mixed"]["position_rel_err_vs_fp64"] = float(d)
+    if "fp64" in pos:
+        for f in forms:
+            if f != "fp64":
+                d = np.abs(pos[f] - pos["fp64"]).max() / np.abs(pos["fp64"]).max()
+                results[f]["position_rel_err_vs_fp64"] = float(d)
     head = args.headline or forms[-1]
     h = results[head]
-    w = 8 if head == "fp64" else 4
+    w = 8 if split_form(head)[0] == "fp64" else 4
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
     traffic = None
@@ -250,7 +272,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "dtype": "f64" if split_form(head)[0] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"{spec.name}: {args.n}^3 jittered unit cubes (edge 1 nm) x 6 quadratic tets, "
@@ -276,7 +298,7 @@ def main_ensemble(args):
     ns = [round((s.num_elements / 6) ** (1 / 3)) for s in specs]
     print(f"[bench] {len(specs)} meshes ({ne_tot} tets, n {min(ns)}..{max(ns)}) built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    form = (args.headline or "mixed") if args.forms == "fp64,mixed" else args.forms.split(",")[-1]
+    form = (args.headline or "mixed") if args.forms == DEFAULT_FORMS else args.forms.split(",")[-1]
     h = run_ensemble(specs, form, args.steps, args.warmup)
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
@@ -403,60 +425,67 @@ def main_adjoint(args):
 
 
 def main_drift(args):
-    """FP64 and mixed handles stepped side by side on the same blob; the mixed form's
-    position error against FP64 and both CG iteration counts along the run"""
+    """the FP64 form (the reference's algorithm) and the other forms of --forms stepped
+    side by side on the same blob; each form's position error against FP64 and the CG
+    iteration counts along the run"""
     from ffea_ci_demo_amd import workloads
-    from ffea_ci_demo_amd.blob import GpuBlob
+    from ffea_ci_demo_amd.blob import FFEAException, GpuBlob
     if args.workload == "illcond":
         spec = workloads.config4(n=args.n if args.n != 100 else 40)
         K = args.run_steps or 2000
     else:
         spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
         K = args.run_steps or 1000
+    others = [f for f in args.forms.split(",") if f and f != "fp64"] or ["mixed"]
     marks = sorted({m for m in [1, 10, K] + list(range(50, K + 1, 50)) if m <= K})
-    d = GpuBlob.from_spec(spec, precision="fp64")
-    m = GpuBlob.from_spec(spec, precision="mixed")
-    d.use_graph(True)
-    m.use_graph(True)
-    scale = np.abs(spec.pos).max()
-    from ffea_ci_demo_amd.blob import FFEAException
-    rows, t_d, t_m, done, aborted = [], 0.0, 0.0, 0, None
+    handles = {"fp64": GpuBlob.from_spec(spec, precision="fp64")}
+    for f in others:
+        prec, pc = split_form(f)
+        handles[f] = GpuBlob.from_spec(spec, precision=prec)
+        if pc == "multigrid":
+            handles[f].set_preconditioner("multigrid")
+    for h in handles.values():
+        h.use_graph(True)
+    rows = {f: [] for f in others}
+    wall = {f: 0.0 for f in handles}
+    done, aborted = 0, None
     for mark in marks:
         n = mark - done
-        t0 = time.perf_counter()
         try:
-            d.step(n)
-            d.synchronize()
-            t1 = time.perf_counter()
-            m.step(n)
-            m.synchronize()
+            for f, h in handles.items():
+                t0 = time.perf_counter()
+                h.step(n)
+                h.synchronize()
+                wall[f] += time.perf_counter() - t0
         except FFEAException as ex:  # the reference aborts the same way (Blob.cpp:484-490, NoMassCGSolver.cpp:187)
-            aborted = {"after_step": done, "within_next": n, "message": str(ex).strip()}
-            print(f"[bench] aborted after step {done} (+<= {n}): {str(ex).strip()}", file=sys.stderr, flush=True)
+            aborted = {"after_step": done, "within_next": n, "form": f, "message": str(ex).strip()}
+            print(f"[bench] {f} aborted after step {done} (+<= {n}): {str(ex).strip()}", file=sys.stderr, flush=True)
             break
-        t2 = time.perf_counter()
-        t_d += t1 - t0
-        t_m += t2 - t1
         done = mark
-        pd, pm = d.positions(), m.positions()
-        err = float(np.abs(pm - pd).max() / np.abs(pd).max())
-        rows.append({"step": mark, "mixed_pos_rel_err": err, "k_fp64": d.cg_iterations()[0],
-                     "k_mixed": m.cg_iterations()[0]})
-        print(f"[bench] {spec.name} step {mark}: mixed rel err {err:.3e} k {rows[-1]['k_fp64']}/{rows[-1]['k_mixed']}",
-              file=sys.stderr, flush=True)
-    d.close()
-    m.close()
+        pd = handles["fp64"].positions()
+        kd = handles["fp64"].cg_iterations()[0]
+        msg = []
+        for f in others:
+            err = float(np.abs(handles[f].positions() - pd).max() / np.abs(pd).max())
+            k = handles[f].cg_iterations()[0]
+            rows[f].append({"step": mark, "pos_rel_err_vs_fp64": err, "k_fp64": kd, "k": k})
+            msg.append(f"{f} rel err {err:.3e} k {k}")
+        print(f"[bench] {spec.name} step {mark}: k_fp64 {kd}; " + "; ".join(msg), file=sys.stderr, flush=True)
+    for h in handles.values():
+        h.close()
     K = done
-    forms = {"fp64": {"element_steps_per_s": spec.num_elements * K / t_d, "ms_per_step": t_d / K * 1e3},
-             "mixed": {"element_steps_per_s": spec.num_elements * K / t_m, "ms_per_step": t_m / K * 1e3,
-                       "drift": rows}}
+    forms = {f: {"element_steps_per_s": spec.num_elements * K / wall[f], "ms_per_step": wall[f] / K * 1e3}
+             for f in handles}
+    for f in others:
+        forms[f]["drift"] = rows[f]
     if aborted:
         forms["aborted"] = aborted
-    out = _line(round(forms["mixed"]["element_steps_per_s"], 1), "element-steps/s",
-                round(forms["mixed"]["ms_per_step"], 3), K, 0, "f32+f64",
-                {"workload": f"{spec.name}: {spec.num_elements} tets, {K} steps of FP64 and FP32-mixed side by side "
-                             "(graph-captured steps); positions relative error max|mixed-fp64|/max|fp64|",
-                 "form": "mixed", "parallelism": "single GPU"},
+    head = args.headline or others[-1]
+    out = _line(round(forms[head]["element_steps_per_s"], 1), "element-steps/s",
+                round(forms[head]["ms_per_step"], 3), K, 0, "f32+f64" if split_form(head)[0] == "mixed" else "f64",
+                {"workload": f"{spec.name}: {spec.num_elements} tets, {K} steps of FP64 and {', '.join(others)} side by "
+                             "side (graph-captured steps); positions relative error max|form-fp64|/max|fp64|",
+                 "form": head, "parallelism": "single GPU"},
                 {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
                  for f, r in forms.items()})
     print(json.dumps(out), flush=True)

@@ -32,8 +32,12 @@ EXPORTED = [
     "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout", "ffea_gpu_blob_set_material",
     "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint", "ffea_gpu_batch_strain_energy",
     "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
-    "ffea_world_last_error",
+    "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
+    "ffea_gpu_multigrid_inspect",
 ]
+
+PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+MG_MAX_LEVELS = 8
 
 
 class Params(C.Structure):
@@ -48,6 +52,20 @@ class Info(C.Structure):
                 ("device_bytes", C.c_longlong), ("num_blobs", C.c_int), ("num_chunks", C.c_int),
                 ("row_blocks", C.c_int), ("graph_iterations", C.c_int),
                 ("single_launch_cg", C.c_int)]
+
+
+class MultigridInfo(C.Structure):
+    _fields_ = [("num_levels", C.c_int), ("rows", C.c_int * MG_MAX_LEVELS), ("block", C.c_int * MG_MAX_LEVELS),
+                ("blocks", C.c_longlong * MG_MAX_LEVELS), ("transfer_blocks", C.c_longlong * MG_MAX_LEVELS),
+                ("omega", C.c_double * MG_MAX_LEVELS), ("lmax", C.c_double * MG_MAX_LEVELS),
+                ("coarsest_unknowns", C.c_int), ("setup_seconds", C.c_double)]
+
+    def as_dict(self):
+        L = self.num_levels
+        return {"num_levels": L, "rows": list(self.rows[:L]), "block": list(self.block[:L]),
+                "blocks": list(self.blocks[:L]), "transfer_blocks": list(self.transfer_blocks[:max(0, L - 1)]),
+                "omega": list(self.omega[:L]), "lmax": list(self.lmax[:L]),
+                "coarsest_unknowns": self.coarsest_unknowns, "setup_seconds": self.setup_seconds}
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -103,6 +121,10 @@ def lib():
     L.ffea_gpu_blob_run_checkpointed.argtypes = [vp, C.c_int, C.c_int]
     L.ffea_gpu_blob_adjoint.argtypes = [vp, _dp, vp, vp, vp, vp, vp]
     L.ffea_gpu_batch_strain_energy.argtypes = [vp, _dp]
+    L.ffea_gpu_blob_set_preconditioner.argtypes = [vp, C.c_int]
+    L.ffea_gpu_blob_get_multigrid_info.argtypes = [vp, C.POINTER(MultigridInfo)]
+    L.ffea_gpu_multigrid_inspect.argtypes = [C.c_int, _ip, _ip, _dp, _dp, vp, C.c_int, C.POINTER(MultigridInfo),
+                                             vp, vp, vp, vp, vp, vp]
     L.ffea_world_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
     L.ffea_world_run.argtypes = [vp, C.c_longlong]
     L.ffea_world_current_step.argtypes = [vp]
@@ -118,6 +140,37 @@ def lib():
     L.ffea_gpu_last_error.restype = C.c_char_p
     _lib = L
     return L
+
+
+def multigrid_inspect(rowptr, col, val, pos, pinned=None, free_blob=True, with_matrices=False):
+    """Host-only multigrid set-up of a 3x3-block CSR corner-node matrix (no device):
+    returns the level info dict and, with with_matrices, the first prolongator P0 and
+    coarse operator A1 as (rowptr, col, blocks) triples."""
+    L = lib()
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64).reshape(-1)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+    n = rowptr.size - 1
+    pin = None if pinned is None else np.ascontiguousarray(pinned, np.uint8)
+    pin_p = None if pin is None else pin.ctypes.data
+    info = MultigridInfo()
+    rc = L.ffea_gpu_multigrid_inspect(n, rowptr, col, val, pos, pin_p, int(bool(free_blob)), C.byref(info),
+                                      None, None, None, None, None, None)
+    if rc != OK:
+        raise RuntimeError(L.ffea_gpu_last_error().decode())
+    d = info.as_dict()
+    if not with_matrices:
+        return d
+    npb, n1, nab = d["transfer_blocks"][0], d["rows"][1], d["blocks"][1]
+    prp, pcol, pval = np.zeros(n + 1, np.int32), np.zeros(npb, np.int32), np.zeros((npb, 3, 6))
+    arp, acol, aval = np.zeros(n1 + 1, np.int32), np.zeros(nab, np.int32), np.zeros((nab, 6, 6))
+    rc = L.ffea_gpu_multigrid_inspect(n, rowptr, col, val, pos, pin_p, int(bool(free_blob)), C.byref(info),
+                                      prp.ctypes.data, pcol.ctypes.data, pval.ctypes.data, arp.ctypes.data,
+                                      acol.ctypes.data, aval.ctypes.data)
+    if rc != OK:
+        raise RuntimeError(L.ffea_gpu_last_error().decode())
+    return d, (prp, pcol, pval), (arp, acol, aval)
 
 
 def copy_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:

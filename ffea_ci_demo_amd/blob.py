@@ -89,6 +89,19 @@ class _Handle:
         iterations (0: a little over the last step's count)"""
         self._check(_native.lib().ffea_gpu_blob_use_graph(self.h, 1 if enable else 0, int(iterations)))
 
+    def set_preconditioner(self, kind="jacobi"):
+        """CG preconditioner: "jacobi" (the reference's) or "multigrid" (a smoothed-
+        aggregation V-cycle built from the matrix at the current positions); returns the
+        multigrid level info (empty for Jacobi)"""
+        k = {"jacobi": _native.PRECOND_JACOBI, "multigrid": _native.PRECOND_MULTIGRID}[kind]
+        self._check(_native.lib().ffea_gpu_blob_set_preconditioner(self.h, k))
+        return self.multigrid_info()
+
+    def multigrid_info(self):
+        i = _native.MultigridInfo()
+        self._check(_native.lib().ffea_gpu_blob_get_multigrid_info(self.h, C.byref(i)))
+        return i.as_dict()
+
     def set_timing(self, phases=None):
         """time the named phases (default: all); set_timing(()) turns timing off"""
         names = _native.PHASES if phases is None else phases

@@ -8,8 +8,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libffea_gpu.so")
 SOURCES = [os.path.join(HERE, "csrc", "ffea_blob.hip"), os.path.join(HERE, "csrc", "ffea_script.cpp"),
-           os.path.join(HERE, "csrc", "ffea_world.cpp")]
+           os.path.join(HERE, "csrc", "ffea_world.cpp"), os.path.join(HERE, "csrc", "amg_setup.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "mrg32k3a.h"), os.path.join(HERE, "csrc", "kernels.h"),
+           os.path.join(HERE, "csrc", "amg_kernels.h"), os.path.join(HERE, "csrc", "amg_setup.h"),
            os.path.join(HERE, "csrc", "script_internal.h"), os.path.join(ROOT, "include", "ffea_gpu.h"),
            os.path.join(ROOT, "include", "ffea_script.h"), os.path.join(ROOT, "include", "ffea_world.h")]
 

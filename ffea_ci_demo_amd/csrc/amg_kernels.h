@@ -1,0 +1,339 @@
+// Device code of the multigrid preconditioner (MI355X, gfx950). Included by ffea_blob.hip
+// after kernels.h. Single-blob handles only: every kernel returns at once when blob 0's
+// solve has stopped.
+//
+// One V-cycle z = M r, with level 0 the corner-node viscosity matrix K (3x3 blocks, the
+// CG's own sliced-ELL storage) and levels 1.. the Galerkin operators over aggregates
+// (6x6 blocks). The smoother is one damped-Jacobi step before and after the coarse
+// correction, x += w D^-1 (b - A x), which keeps M symmetric. Level-0 vectors are the
+// CG's 64-row tiled FP64 vectors; coarse vectors are FP64 [rows][6]. Transfer and coarse
+// operators are FP32 in their own sliced ELL: 64 block rows per slice, slot width W =
+// the slice's longest row, value k of the block in slot j of lane l at
+// (base + j*64) * BR*BC + k*64 + l, block column at base + j*64 + l (-1: padding).
+#pragma once
+#include "kernels.h"
+
+namespace ffk {
+
+constexpr int kMgBlock = 256;  // threads per workgroup of the coarse-level kernels
+
+// vector layouts: level 0 tiled by 64 rows (vi), coarse levels plain [rows][6]
+template <int B>
+struct VecLayout;
+template <>
+struct VecLayout<3> {
+    static __device__ __forceinline__ size_t at(int n, int k) { return vi(n, k); }
+};
+template <>
+struct VecLayout<6> {
+    static __device__ __forceinline__ size_t at(int n, int k) { return (size_t)n * 6 + k; }
+};
+
+enum MgMode { MG_RESID = 0, MG_RESTRICT = 1, MG_PROLONG = 2, MG_POST = 3 };
+
+struct MgArgs {
+    const CgScalars *S;     // blob 0
+    int n;                  // block rows
+    const int *sw;          // [slices] slot widths
+    const long long *sb;    // [slices] slot bases
+    const int *col;         // [slots]
+    const float *val;       // [slots * BR*BC]
+    const double *x;        // input gathered by block column
+    const double *b;        // RESID, POST: right-hand side (row layout)
+    const double *base;     // PROLONG: y = bscale base + A x (row layout)
+    const double *dinv;     // RESTRICT: the next level's D^-1; POST: this level's (row layout)
+    double omega, bscale;
+    double *y, *y2;         // RESTRICT: y = A x, y2 = omega D^-1 y (if not null)
+};
+
+// y = A x and the mode's row update, one thread per block row:
+//   RESID     y = b - A x
+//   RESTRICT  y = A x (A = P^T), y2 = omega D^-1 y (the pre-smoothing step on the next level)
+//   PROLONG   y = bscale base + A x (A = P)
+//   POST      y = x + omega D^-1 (b - A x) (the post-smoothing step; A square)
+template <int BR, int BC, int MODE>
+__global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
+    if (a.S->done) return;
+    const int row = blockIdx.x * kMgBlock + threadIdx.x;
+    if (row >= a.n) return;
+    const int sl = row >> 6, lane = row & 63;
+    const int W = a.sw[sl];
+    const long long base = a.sb[sl];
+    double acc[BR];
+#pragma unroll
+    for (int p = 0; p < BR; ++p) acc[p] = 0.0;
+    for (int j = 0; j < W; ++j) {
+        const long long slot = base + (long long)j * 64;
+        const int c = a.col[slot + lane];
+        if (c < 0) continue;
+        double xv[BC];
+#pragma unroll
+        for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+        const float *v = a.val + (size_t)slot * (BR * BC) + lane;
+#pragma unroll
+        for (int p = 0; p < BR; ++p)
+#pragma unroll
+            for (int q = 0; q < BC; ++q) acc[p] += (double)__builtin_nontemporal_load(v + (p * BC + q) * 64) * xv[q];
+    }
+#pragma unroll
+    for (int p = 0; p < BR; ++p) {
+        const size_t o = VecLayout<BR>::at(row, p);
+        if constexpr (MODE == MG_RESID) {
+            a.y[o] = a.b[o] - acc[p];
+        } else if constexpr (MODE == MG_RESTRICT) {
+            a.y[o] = acc[p];
+            if (a.y2) a.y2[o] = a.omega * a.dinv[o] * acc[p];
+        } else if constexpr (MODE == MG_PROLONG) {
+            a.y[o] = a.bscale * a.base[o] + acc[p];
+        } else {
+            a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - acc[p]);
+        }
+    }
+}
+
+// The same operations for matrices with few, long rows (the coarse operators, 6x6 blocks
+// and ~45 per row, and the restrictions, ~100 blocks per row): block CSR and one
+// wavefront per block row. Lane l takes the row's blocks l, l+64, ...; the lanes'
+// partial sums are combined by a fixed butterfly, so results do not depend on timing.
+// A row's W blocks are stored vector-major: with V-float vectors (V = 4 for 6x6 blocks,
+// 2 for 6x3), vector t of block j is at (rp[row] * BR*BC) + (t W + j) V, so one vector
+// load of a wavefront reads consecutive blocks' vectors (contiguous bytes).
+struct MgCsrArgs {
+    const CgScalars *S;
+    int n;
+    const int *rp, *ci;
+    const float *val;       // [nnzb][BR*BC]
+    const double *x, *b, *dinv;
+    double omega;
+    double *y, *y2;
+};
+
+template <int BR, int BC, int MODE>
+__global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
+    if (a.S->done) return;
+    const int row = blockIdx.x * (kMgBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= a.n) return;  // whole wavefronts
+    double acc[BR];
+#pragma unroll
+    for (int p = 0; p < BR; ++p) acc[p] = 0.0;
+    const int r0 = a.rp[row], W = a.rp[row + 1] - r0;
+    const float *rowv = a.val + (size_t)r0 * (BR * BC);
+    for (int j = lane; j < W; j += 64) {
+        const int c = a.ci[r0 + j];
+        double xv[BC];
+#pragma unroll
+        for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+        float w[BR * BC];
+        if constexpr ((BR * BC) % 4 == 0) {
+#pragma unroll
+            for (int t = 0; t < BR * BC / 4; ++t) {
+                const vf4 u = __builtin_nontemporal_load((const vf4 *)rowv + (size_t)t * W + j);
+                w[4 * t] = u.x; w[4 * t + 1] = u.y; w[4 * t + 2] = u.z; w[4 * t + 3] = u.w;
+            }
+        } else {
+            typedef float vf2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+            for (int t = 0; t < BR * BC / 2; ++t) {
+                const vf2 u = __builtin_nontemporal_load((const vf2 *)rowv + (size_t)t * W + j);
+                w[2 * t] = u.x; w[2 * t + 1] = u.y;
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < BR; ++p)
+#pragma unroll
+            for (int q = 0; q < BC; ++q) acc[p] += (double)w[p * BC + q] * xv[q];
+    }
+#pragma unroll
+    for (int p = 0; p < BR; ++p) acc[p] = wave_sum(acc[p]);
+    double mine = acc[0];
+#pragma unroll
+    for (int p = 1; p < BR; ++p)
+        if (lane == p) mine = acc[p];
+    if (lane >= BR) return;
+    const size_t o = VecLayout<BR>::at(row, lane);
+    if constexpr (MODE == MG_RESID) {
+        a.y[o] = a.b[o] - mine;
+    } else if constexpr (MODE == MG_RESTRICT) {
+        a.y[o] = mine;
+        if (a.y2) a.y2[o] = a.omega * a.dinv[o] * mine;
+    } else {
+        static_assert(MODE == MG_POST, "k_mg_csr: residual, restriction or post-smoothing");
+        a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - mine);
+    }
+}
+
+// coarsest level: e = C b with the dense inverse C (n x n, FP32, symmetric: row `row`
+// is read as contiguous bytes), one wavefront per row
+__global__ __launch_bounds__(kMgBlock) void k_mg_dense(const CgScalars *S, int n, const float *C, const double *b,
+                                                    double *e) {
+    if (S->done) return;
+    const int row = blockIdx.x * (kMgBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (int k = lane; k < n; k += 64) acc += (double)C[(size_t)row * n + k] * b[k];
+    acc = wave_sum(acc);
+    if (lane == 0) e[row] = acc;
+}
+
+// Level-0 steps on the CG's own matrix storage, one chunk of rows per workgroup
+// (kRowBlock threads, one row each):
+//   FINE_RESID  y = r - omega K z          (z = D^-1 r from the vector step: the
+//                                           pre-smoothed iterate is omega z)
+//   FINE_POST   z = xb + omega D^-1 (r - K xb), and the chunk partials of r.z
+constexpr int MG_FINE_RESID = 0, MG_FINE_POST = 1;
+
+template <typename T>
+struct MgFineArgs {
+    const CgScalars *S;
+    const Chunk *chunks;
+    int nchunks;
+    const int *slice_w;
+    const long long *slice_base;
+    const int *col;
+    const T *val;
+    const double *x;       // gathered: z (RESID) or xb (POST)
+    const double *r, *dinv;
+    double omega;
+    double *y;             // RESID: residual; POST: z
+    double *P;             // POST: partials row (P[c])
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kRowBlock) void k_mg_fine(MgFineArgs<T> a) {
+    if (a.S->done) return;
+    const int c = blockIdx.x;
+    const Chunk ch = a.chunks[c];
+    const int n = ch.begin + threadIdx.x;
+    double v[1] = {0.0};
+    if (n < ch.end) {
+        const int sl = n / kSlice, lane = n % kSlice;
+        const int W = a.slice_w[sl];
+        const long long base = a.slice_base[sl];
+        const int *colp = a.col + base + lane;
+        const T *vp = a.val + (size_t)base * 9;
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        for (int j = 0; j < W; ++j) {
+            const int cc = __builtin_nontemporal_load(colp + (size_t)j * kSlice);
+            const double x0 = a.x[vi(cc, 0)], x1 = a.x[vi(cc, 1)], x2 = a.x[vi(cc, 2)];
+            T w[9];
+            load_block<T>(vp + (size_t)j * 9 * kSlice, lane, w);
+            acc0 += (double)w[0] * x0; acc0 += (double)w[1] * x1; acc0 += (double)w[2] * x2;
+            acc1 += (double)w[3] * x0; acc1 += (double)w[4] * x1; acc1 += (double)w[5] * x2;
+            acc2 += (double)w[6] * x0; acc2 += (double)w[7] * x1; acc2 += (double)w[8] * x2;
+        }
+        const double acc[3] = {acc0, acc1, acc2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const size_t o = vi(n, k);
+            const double rk = a.r[o];
+            if constexpr (MODE == MG_FINE_RESID) {
+                a.y[o] = rk - a.omega * acc[k];
+            } else {
+                const double zk = a.x[o] + a.omega * a.dinv[o] * (rk - acc[k]);
+                a.y[o] = zk;
+                v[0] += rk * zk;
+            }
+        }
+    }
+    if constexpr (MODE == MG_FINE_POST) store_partials<1, kRowBlock>(v, a.P, a.nchunks, c);
+}
+
+// D-weighted rigid-mode gauge of a free blob's solution. Jacobi-PCG from x = 0 returns
+// the solution with Z^T D x = 0 (its iterates lie in D^-1 range(K)); the multigrid
+// preconditioner does not keep that, so after the solve x <- x - Z (Z^T D Z)^-1 Z^T D x
+// with D the matrix diagonal. Partials per chunk: Z^T D x (6) and the upper triangle
+// of Z^T D Z (21).
+struct MgRigidArgs {
+    const CgScalars *S;
+    const Chunk *chunks;
+    int nchunks;
+    const double *x, *dinv, *pos;
+    const double3 *centre;  // blob 0's rigid-mode centre
+    double *P;      // [27][nchunks]
+    double *coef;   // [6]
+    double *xout;   // x (corrected in place)
+};
+
+__device__ __forceinline__ void rigid_basis(const double *d, double (&Z)[3][6]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Z[i][k] = 0.0;
+    Z[0][0] = Z[1][1] = Z[2][2] = 1.0;
+    // column 3+k: e_k cross d
+    Z[1][3] = -d[2]; Z[2][3] = d[1];
+    Z[0][4] = d[2];  Z[2][4] = -d[0];
+    Z[0][5] = -d[1]; Z[1][5] = d[0];
+}
+
+// 256 threads per chunk (4 rows each): the 27 sums stay in registers
+__global__ __launch_bounds__(kMgBlock) void k_mg_rigid_sums(MgRigidArgs a) {
+    const CgScalars &S = *a.S;
+    if (!S.converged || S.updated) return;
+    const int c = blockIdx.x;
+    const Chunk ch = a.chunks[c];
+    const double3 cen = a.centre[0];
+    double v[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) v[k] = 0.0;
+    for (int n = ch.begin + threadIdx.x; n < ch.end; n += kMgBlock) {
+        const double d[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+        double Z[3][6];
+        rigid_basis(d, Z);
+        double w[3], xd[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double di = a.dinv[vi(n, i)];
+            w[i] = di > 0.0 ? 1.0 / di : 0.0;
+            xd[i] = w[i] * a.x[vi(n, i)];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] += Z[0][k] * xd[0] + Z[1][k] * xd[1] + Z[2][k] * xd[2];
+        int t = 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int l = k; l < 6; ++l) v[t++] += w[0] * Z[0][k] * Z[0][l] + w[1] * Z[1][k] * Z[1][l] + w[2] * Z[2][k] * Z[2][l];
+    }
+    store_partials<27, kMgBlock>(v, a.P, a.nchunks, c);
+}
+
+__global__ __launch_bounds__(kMgBlock) void k_mg_rigid_coef(MgRigidArgs a) {
+    const CgScalars &S = *a.S;
+    if (!S.converged || S.updated) return;
+    double t[27];
+    sum_partials<27, kMgBlock>(a.P, a.nchunks, 0, a.nchunks, t);
+    if (threadIdx.x == 0) {
+        double M[36], Mi[36];
+        int u = 6;
+        for (int k = 0; k < 6; ++k)
+            for (int l = k; l < 6; ++l) {
+                M[6 * k + l] = M[6 * l + k] = t[u];
+                ++u;
+            }
+        invert6(M, Mi);
+        for (int k = 0; k < 6; ++k) {
+            double acc = 0.0;
+            for (int l = 0; l < 6; ++l) acc += Mi[6 * k + l] * t[l];
+            a.coef[k] = acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kRowBlock) void k_mg_rigid_fix(MgRigidArgs a) {
+    const CgScalars &S = *a.S;
+    if (!S.converged || S.updated) return;
+    const Chunk ch = a.chunks[blockIdx.x];
+    const int n = ch.begin + threadIdx.x;
+    if (n >= ch.end) return;
+    const double3 cen = a.centre[0];
+        const double d[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+    double cf[6], zc[3];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cf[k] = a.coef[k];
+    z_times(d, cf, zc);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a.xout[vi(n, i)] -= zc[i];
+}
+
+}  // namespace ffk

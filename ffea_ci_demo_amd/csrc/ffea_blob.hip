@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -37,6 +38,8 @@
 #include <vector>
 
 #include "../../include/ffea_gpu.h"
+#include "amg_kernels.h"
+#include "amg_setup.h"
 #include "kernels.h"
 #include "mrg32k3a.h"
 
@@ -91,6 +94,11 @@ struct DevBuf {
     void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
     // not for code that may be captured into a graph
     void zero(hipStream_t s) { FFEA_HIP_TRY(hipMemsetAsync(p, 0, n * sizeof(T), s)); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
 };
 
 struct Phase {
@@ -431,6 +439,7 @@ struct BlobMeta {
     int row_off = 0, elem_off = 0;
     long long nblocks = 0;
     bool all_pinned = false, project = false;
+    bool free_blob = false;  // no pins, one connected mesh, no drag: singular viscosity matrix
     std::vector<int> lin_of, orig_of_lin, mid_a, mid_b;
     int last_iters = 0;
     long long total_iters = 0;
@@ -507,6 +516,7 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         // free blob (no pins): the viscosity matrix is singular with the 6 rigid modes as
         // null space; the mixed form projects them out of every SpMV result
         m.project = mixed && P.npin == 0 && P.connected && !P.all_pinned && !t.stokes;
+        m.free_blob = P.npin == 0 && P.connected && !P.all_pinned && !t.stokes;
         m.lin_of = P.lin_of;
         m.orig_of_lin = P.orig_of_lin;
         m.mid_a = P.mid_a;
@@ -841,6 +851,34 @@ struct ffea_gpu_blob {
     DevBuf<double> d_w;
     DevBuf<float> f_w;
 
+    // multigrid preconditioner (single blob): levels 0..L-1, level L-1 inverted densely
+    struct MgEll {
+        int n = 0;
+        DevBuf<int> sw, col;
+        DevBuf<long long> sb;
+        DevBuf<float> val;
+    };
+    struct MgCsr {
+        int n = 0;
+        DevBuf<int> rp, ci;
+        DevBuf<float> val;
+    };
+    struct MgLevel {
+        int n = 0;             // block rows
+        double omega = 0.0;    // smoother weight
+        MgCsr A, R;            // A: levels >= 1 (level 0 is the CG's matrix); R: to level + 1
+        MgEll P;               // prolongation from level + 1 (few blocks per row: a thread per row)
+        DevBuf<double> dinv, b, xa, res, xb, e;  // levels >= 1, [n][6]
+    };
+    int precond = FFEA_GPU_PRECOND_JACOBI;
+    bool mg_suspend = false;   // the adjoint's reverse solves keep the reference's Jacobi
+    std::vector<std::unique_ptr<MgLevel>> mg;
+    DevBuf<float> mg_cinv;
+    int mg_nc = 0;
+    DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
+    ffea_gpu_multigrid_info mg_info{};
+    bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID && !mg_suspend; }
+
     // timing
     unsigned timing = 0;  // bit i: time phase i
     int cg_sample = 8;    // CG phases: time one iteration in cg_sample
@@ -913,9 +951,14 @@ struct ffea_gpu_blob {
         graph_exec = nullptr;
     }
     RowArgs row_args() { return RowArgs{nchunks, cpb, chunks.p, seg.p, S.p, num_done.p}; }
-    void launch_element();
+    void launch_element(bool advance = true);
     void launch_assemble();
     void launch_assemble_gather();
+    void mg_setup();
+    template <typename T>
+    void launch_mg_vcycle_t(const T *val, double *Pout);
+    void launch_mg_vcycle(double *Pout);
+    void launch_mg_gauge();
     template <typename T>
     void set_material_t(const double *A, const double *B, const double *G, const double *E);
     int forward_checkpointed(int nsteps, int interval);
@@ -1112,7 +1155,7 @@ void ffea_gpu_blob::set_row_blocks(int blocks) {
     drop_graph();                             // the graph holds the old grid
 }
 
-void ffea_gpu_blob::launch_element() {
+void ffea_gpu_blob::launch_element(bool advance) {
     hipEvent_t t0;
     t_begin(0, &t0);
     const int *eb = nblob > 1 ? elem_blob.p : nullptr;
@@ -1129,6 +1172,7 @@ void ffea_gpu_blob::launch_element() {
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(0, t0);
+    if (!advance) return;  // forces and shape data only (multigrid set-up)
     if (stokes_noise) {  // the node draws follow the element draws of the step
         NoiseArgs na{nnodes, nblob > 1 ? node_blob.p : nullptr, node_off.p, node_sc.p, rng.p, node_jump.p,
                      njump_group.p, njump_lane.p, noise.p};
@@ -1183,6 +1227,7 @@ void ffea_gpu_blob::launch_assemble_gather() {
         launch_gather<double, false>(d_force.p);
     }
     FFEA_HIP_TRY(hipGetLastError());
+    if (mg_on()) launch_mg_vcycle(Pi.p);  // z = M r of the start; its r.z replaces row 0
     t_end(2, t0);
 }
 
@@ -1227,6 +1272,7 @@ void ffea_gpu_blob::launch_cg_vec(int it) {
         else
             hipLaunchKernelGGL((k_cg_vec<false, kRowBlock>), dim3(nb), dim3(kRowBlock), 0, stream, va);
     }
+    if (mg_on()) launch_mg_vcycle(P2.p + nchunks);  // z = M r; its r.z replaces row 1
     t_end(4, tv, it);
 }
 
@@ -1263,11 +1309,270 @@ void ffea_gpu_blob::launch_cg_fused() {
 void ffea_gpu_blob::launch_update() {
     hipEvent_t t0;
     t_begin(5, &t0);
+    if (mg_on() && meta[0].free_blob) launch_mg_gauge();
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)x.p, vel.p, pos.p,
                        prm.dt);
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
     FFEA_HIP_TRY(hipGetLastError());
     t_end(5, t0);
+}
+
+// ---------------------------------------------------------------- multigrid preconditioner
+namespace {
+// a block CSR level matrix in the sliced ELL of amg_kernels.h (FP32 values, -1 padding)
+struct EllHost {
+    std::vector<int> sw, col;
+    std::vector<long long> sb;
+    std::vector<float> val;
+};
+EllHost to_ell(const amg::BCsr &A) {
+    EllHost E;
+    const int ns = (A.n + 63) / 64, bb = A.br * A.bc;
+    E.sw.assign(std::max(1, ns), 1);
+    E.sb.assign(std::max(1, ns), 0);
+    long long slots = 0;
+    for (int s = 0; s < ns; ++s) {
+        int w = 1;
+        for (int i = s * 64; i < std::min(A.n, s * 64 + 64); ++i) w = std::max(w, A.rp[i + 1] - A.rp[i]);
+        E.sw[s] = w;
+        E.sb[s] = slots;
+        slots += (long long)w * 64;
+    }
+    E.col.assign(std::max<long long>(1, slots), -1);
+    E.val.assign((size_t)std::max<long long>(1, slots) * bb, 0.0f);
+    for (int i = 0; i < A.n; ++i) {
+        const int s = i / 64, lane = i % 64;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const long long slot = E.sb[s] + (long long)(k - A.rp[i]) * 64;
+            if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
+            E.col[slot + lane] = A.ci[k];
+            for (int e = 0; e < bb; ++e) E.val[(size_t)slot * bb + (size_t)e * 64 + lane] = (float)A.v[(size_t)k * bb + e];
+        }
+    }
+    return E;
+}
+}  // namespace
+
+// Builds the multigrid hierarchy from the viscosity matrix at the current positions: an
+// element pass (no stream advance) and an assembly on the device, the matrix read back,
+// the set-up on the host (amg::build), the levels uploaded. The hierarchy is kept for
+// the rest of the run; each step's CG still applies that step's own matrix and diagonal
+// on level 0.
+void ffea_gpu_blob::mg_setup() {
+    if (nblob != 1) throw Invalid("the multigrid preconditioner takes single-blob handles");
+    const BlobMeta &m = meta[0];
+    if (m.all_pinned) throw Invalid("multigrid preconditioner: every node is pinned, nothing is solved");
+    if (fused_cg)
+        throw Invalid("multigrid preconditioner: a blob of up to 1024 corner nodes solves in one launch; "
+                      "the preconditioner is for larger blobs");
+    const auto t0 = std::chrono::steady_clock::now();
+    launch_element(false);
+    launch_assemble();
+    hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);  // pass's inversion count
+    FFEA_HIP_TRY(hipGetLastError());
+    const int nl = m.nl;
+    std::vector<int> h_sw(nslices), h_col(nslots);
+    std::vector<long long> h_sb(nslices);
+    std::vector<unsigned char> h_pin(nrows);
+    std::vector<double> h_pos((size_t)3 * nrows);
+    FFEA_HIP_TRY(hipMemcpyAsync(h_sw.data(), slice_w.p, sizeof(int) * nslices, hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(h_sb.data(), slice_base.p, sizeof(long long) * nslices, hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(h_col.data(), col.p, sizeof(int) * nslots, hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(h_pin.data(), pinned.p, nrows, hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(h_pos.data(), pos.p, sizeof(double) * 3 * nrows, hipMemcpyDeviceToHost, stream));
+    std::vector<double> h_val((size_t)nslots * 9);
+    if (mixed) {
+        std::vector<float> f((size_t)nslots * 9);
+        FFEA_HIP_TRY(hipMemcpyAsync(f.data(), f_val.p, sizeof(float) * f.size(), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < f.size(); ++i) h_val[i] = f[i];
+    } else {
+        FFEA_HIP_TRY(hipMemcpyAsync(h_val.data(), d_val.p, sizeof(double) * h_val.size(), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    amg::BCsr fine;
+    fine.n = fine.ncols = nl;
+    fine.br = fine.bc = 3;
+    fine.rp.assign(nl + 1, 0);
+    for (int n = 0; n < nl; ++n) {
+        const int s = n / kSlice, lane = n % kSlice;
+        std::vector<std::pair<int, int>> ent;  // (column, slot j)
+        for (int j = 0; j < h_sw[s]; ++j) {
+            const int c = h_col[h_sb[s] + (long long)j * kSlice + lane];
+            bool nz = false;
+            for (int k = 0; k < 9; ++k)
+                nz |= h_val[mixed ? val_index<float>(h_sb[s], j, lane, k) : val_index<double>(h_sb[s], j, lane, k)] != 0.0;
+            if (!nz) continue;  // slice padding
+            if (c < 0 || c >= nl) throw Invalid("internal error: matrix column outside the blob");
+            ent.push_back({c, j});
+        }
+        std::sort(ent.begin(), ent.end());
+        for (size_t t = 1; t < ent.size(); ++t)
+            if (ent[t].first == ent[t - 1].first) throw Invalid("internal error: repeated matrix column");
+        fine.rp[n + 1] = fine.rp[n] + (int)ent.size();
+        for (auto &e : ent) {
+            fine.ci.push_back(e.first);
+            for (int k = 0; k < 9; ++k)
+                fine.v.push_back(h_val[mixed ? val_index<float>(h_sb[s], e.second, lane, k)
+                                             : val_index<double>(h_sb[s], e.second, lane, k)]);
+        }
+    }
+    h_val.clear();
+    h_val.shrink_to_fit();
+    amg::Options opt;
+    opt.threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    amg::Hierarchy H;
+    try {
+        H = amg::build(fine, h_pos.data(), h_pin.data(), m.free_blob, opt);
+    } catch (const std::runtime_error &e) {
+        throw Invalid(e.what());
+    }
+    const int L = (int)H.lv.size();
+    if (L < 2 || L > 8) throw Invalid(fmt("multigrid preconditioner: %d levels (needs 2 to 8)", L));
+    // upload
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    mg.clear();
+    mg_cinv.reset();
+    for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef}) bf->reset();
+    auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
+        const EllHost E = to_ell(A);
+        D.n = A.n;
+        D.sw.alloc(E.sw.size(), &dev_bytes); D.sw.upload(E.sw, stream);
+        D.sb.alloc(E.sb.size(), &dev_bytes); D.sb.upload(E.sb, stream);
+        D.col.alloc(E.col.size(), &dev_bytes); D.col.upload(E.col, stream);
+        D.val.alloc(E.val.size(), &dev_bytes); D.val.upload(E.val, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));  // E goes out of scope
+    };
+    auto up_csr = [&](MgCsr &D, const amg::BCsr &A) {
+        for (long long k = 0; k < A.nnzb(); ++k)
+            if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
+        // vector-major rows (k_mg_csr): vector t of block j of row i at (rp[i] BB) + (t W + j) V
+        const int BB = A.br * A.bc, V = BB % 4 == 0 ? 4 : 2;
+        if (BB % V) throw Invalid("internal error: multigrid block size");
+        std::vector<float> v(A.v.size());
+        for (int i = 0; i < A.n; ++i) {
+            const int r0 = A.rp[i], W = A.rp[i + 1] - r0;
+            for (int j = 0; j < W; ++j)
+                for (int t = 0; t < BB / V; ++t)
+                    for (int u = 0; u < V; ++u)
+                        v[(size_t)r0 * BB + ((size_t)t * W + j) * V + u] = (float)A.v[((size_t)r0 + j) * BB + t * V + u];
+        }
+        D.n = A.n;
+        D.rp.alloc(A.rp.size(), &dev_bytes); D.rp.upload(A.rp, stream);
+        D.ci.alloc(A.ci.size(), &dev_bytes); D.ci.upload(A.ci, stream);
+        D.val.alloc(v.size(), &dev_bytes); D.val.upload(v, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));  // v goes out of scope
+    };
+    mg_info = ffea_gpu_multigrid_info{};
+    mg_info.num_levels = L;
+    for (int l = 0; l < L; ++l) {
+        const amg::Level &hl = H.lv[l];
+        auto lev = std::make_unique<MgLevel>();
+        lev->n = hl.A.n;
+        lev->omega = hl.omega;
+        if (l >= 1) {
+            if (hl.A.br != 6 || hl.A.bc != 6) throw Invalid("internal error: coarse blocks are not 6x6");
+            if (l < L - 1) up_csr(lev->A, hl.A);
+            const size_t nv = (size_t)hl.A.n * 6;
+            std::vector<double> dv(hl.dinv);
+            lev->dinv.alloc(nv, &dev_bytes);
+            lev->dinv.upload(dv, stream);
+            for (DevBuf<double> *bf : {&lev->b, &lev->xa, &lev->res, &lev->xb, &lev->e}) {
+                bf->alloc(nv, &dev_bytes);
+                bf->zero(stream);
+            }
+        }
+        if (l < L - 1) {
+            const amg::BCsr &P = hl.P, &R = hl.R;
+            if (P.n != hl.A.n || P.ncols != H.lv[l + 1].A.n || R.n != P.ncols || R.ncols != P.n)
+                throw Invalid("internal error: multigrid transfer shapes");
+            up_ell(lev->P, P);
+            up_csr(lev->R, R);
+            mg_info.transfer_blocks[l] = P.nnzb();
+        }
+        mg_info.rows[l] = hl.A.n;
+        mg_info.block[l] = hl.A.br;
+        mg_info.blocks[l] = hl.A.nnzb();
+        mg_info.omega[l] = hl.omega;
+        mg_info.lmax[l] = hl.lmax;
+        mg.push_back(std::move(lev));
+    }
+    mg_nc = H.nc;
+    if (mg_nc != H.lv[L - 1].A.n * 6 || (long long)H.cinv.size() != (long long)mg_nc * mg_nc)
+        throw Invalid("internal error: coarsest inverse size");
+    mg_info.coarsest_unknowns = mg_nc;
+    {
+        std::vector<float> ci(H.cinv.begin(), H.cinv.end());
+        mg_cinv.alloc(ci.size(), &dev_bytes);
+        mg_cinv.upload(ci, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    const size_t vlen = (size_t)nslices * 3 * kSlice;
+    mg_res0.alloc(vlen, &dev_bytes); mg_res0.zero(stream);
+    mg_xb0.alloc(vlen, &dev_bytes); mg_xb0.zero(stream);
+    mg_rp.alloc((size_t)27 * std::max(1, nchunks), &dev_bytes); mg_rp.zero(stream);
+    mg_coef.alloc(8, &dev_bytes); mg_coef.zero(stream);
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    if (mg[0]->n != nl) throw Invalid("internal error: multigrid level 0 rows");
+    mg_info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// z = M r (one V-cycle), r and z = D^-1 r given by the CG start or vector step; the
+// chunk partials of r.z go to Pout[c]
+template <typename T>
+void ffea_gpu_blob::launch_mg_vcycle_t(const T *val, double *Pout) {
+    const int L = (int)mg.size();
+    auto grid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock - 1) / kMgBlock)); };
+    MgLevel &f = *mg[0];
+    MgFineArgs<T> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, val, z.p, r.p, dinv.p, f.omega,
+                     mg_res0.p, nullptr};
+    hipLaunchKernelGGL((k_mg_fine<T, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+    auto wgrid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock / 64 - 1) / (kMgBlock / 64))); };
+    for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
+        MgLevel &cur = *mg[l], &nx = *mg[l + 1];
+        const bool coarsest = l + 1 == L - 1;
+        MgCsrArgs ra{S.p, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, l == 0 ? mg_res0.p : cur.res.p, nullptr,
+                     coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p};
+        if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
+        else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
+        if (!coarsest) {
+            MgCsrArgs rs{S.p, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.xa.p, nx.b.p, nullptr, 0.0, nx.res.p, nullptr};
+            hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESID>), wgrid(nx.n), dim3(kMgBlock), 0, stream, rs);
+        }
+    }
+    MgLevel &c = *mg[L - 1];
+    hipLaunchKernelGGL(k_mg_dense, wgrid(mg_nc), dim3(kMgBlock), 0, stream, (const CgScalars *)S.p, mg_nc,
+                       (const float *)mg_cinv.p, (const double *)c.b.p, c.e.p);
+    for (int l = L - 2; l >= 1; --l) {  // up: prolong, post-smooth
+        MgLevel &cur = *mg[l], &nx = *mg[l + 1];
+        MgArgs pa{S.p, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, nx.e.p, nullptr, cur.xa.p, nullptr,
+                  0.0, 1.0, cur.xb.p, nullptr};
+        hipLaunchKernelGGL((k_mg_apply<6, 6, MG_PROLONG>), grid(cur.n), dim3(kMgBlock), 0, stream, pa);
+        MgCsrArgs po{S.p, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.xb.p, cur.b.p, cur.dinv.p, cur.omega,
+                     cur.e.p, nullptr};
+        hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(cur.n), dim3(kMgBlock), 0, stream, po);
+    }
+    MgArgs p0{S.p, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, mg[1]->e.p, nullptr, z.p, nullptr, 0.0, f.omega,
+              mg_xb0.p, nullptr};
+    hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
+    MgFineArgs<T> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, val, mg_xb0.p, r.p, dinv.p, f.omega,
+                     z.p, Pout};
+    hipLaunchKernelGGL((k_mg_fine<T, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+    FFEA_HIP_TRY(hipGetLastError());
+}
+
+void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
+    if (mixed) launch_mg_vcycle_t<float>(f_val.p, Pout);
+    else launch_mg_vcycle_t<double>(d_val.p, Pout);
+}
+
+// the free blob's solution in the Jacobi-PCG gauge, Z^T D x = 0, before the update
+void ffea_gpu_blob::launch_mg_gauge() {
+    MgRigidArgs ra{S.p, chunks.p, nchunks, x.p, dinv.p, pos.p, centre.p, mg_rp.p, mg_coef.p, x.p};
+    hipLaunchKernelGGL(k_mg_rigid_sums, dim3(nchunks), dim3(kMgBlock), 0, stream, ra);
+    hipLaunchKernelGGL(k_mg_rigid_coef, dim3(1), dim3(kMgBlock), 0, stream, ra);
+    hipLaunchKernelGGL(k_mg_rigid_fix, dim3(nchunks), dim3(kRowBlock), 0, stream, ra);
+    FFEA_HIP_TRY(hipGetLastError());
 }
 
 // CG iterations from start_iter on, queued in chunks. Iteration i is an SpMV step (which
@@ -1459,13 +1764,16 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kRowBlock), 0, stream, st);
     FFEA_HIP_TRY(hipGetLastError());
     zero_rhs_solved = 1;
+    mg_suspend = true;  // the start above set z = D^-1 r: the whole solve stays Jacobi
     try {
         run_cg_eager(0);
     } catch (...) {
         zero_rhs_solved = 0;
+        mg_suspend = false;
         throw;
     }
     zero_rhs_solved = 0;
+    mg_suspend = false;
     FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     if (!h_S[0].converged)
@@ -2137,6 +2445,93 @@ int ffea_gpu_blob_use_graph(ffea_gpu_blob *b, int enable, int iterations) {
     b->graph_fixed_iters = std::max(0, iterations);
     if (!b->use_graph) b->drop_graph();
     return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    if (kind != FFEA_GPU_PRECOND_JACOBI && kind != FFEA_GPU_PRECOND_MULTIGRID) {
+        b->err = "unknown preconditioner";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        b->drop_graph();  // a captured step holds the other preconditioner's launches
+        if (kind == FFEA_GPU_PRECOND_JACOBI) {
+            FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+            b->precond = FFEA_GPU_PRECOND_JACOBI;
+            b->mg.clear();
+            b->mg_cinv.reset();
+            for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef}) bf->reset();
+            b->mg_info = ffea_gpu_multigrid_info{};
+            return (int)FFEA_GPU_OK;
+        }
+        b->precond = FFEA_GPU_PRECOND_JACOBI;  // until the hierarchy is complete
+        b->mg_setup();
+        b->precond = FFEA_GPU_PRECOND_MULTIGRID;
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_get_multigrid_info(ffea_gpu_blob *b, ffea_gpu_multigrid_info *info) {
+    if (!b || !info) return FFEA_GPU_ERR_INVALID;
+    *info = b->precond == FFEA_GPU_PRECOND_MULTIGRID ? b->mg_info : ffea_gpu_multigrid_info{};
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_multigrid_inspect(int num_rows, const int *rowptr, const int *col, const double *val, const double *pos,
+                               const unsigned char *pinned, int free_blob, ffea_gpu_multigrid_info *info,
+                               int *p_rowptr, int *p_col, double *p_val, int *a_rowptr, int *a_col, double *a_val) {
+    if (num_rows <= 0 || !rowptr || !col || !val || !pos || !info) {
+        g_last_error = "ffea_gpu_multigrid_inspect: missing argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(nullptr, [&]() {
+        if (rowptr[0] != 0) throw Invalid("ffea_gpu_multigrid_inspect: rowptr[0] must be 0");
+        for (int i = 0; i < num_rows; ++i)
+            if (rowptr[i + 1] < rowptr[i]) throw Invalid("ffea_gpu_multigrid_inspect: rowptr must not decrease");
+        amg::BCsr A;
+        A.n = A.ncols = num_rows;
+        A.br = A.bc = 3;
+        A.rp.assign(rowptr, rowptr + num_rows + 1);
+        A.ci.assign(col, col + rowptr[num_rows]);
+        A.v.assign(val, val + (size_t)rowptr[num_rows] * 9);
+        amg::Options opt;
+        opt.threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+        const auto t0 = std::chrono::steady_clock::now();
+        amg::Hierarchy H;
+        try {
+            H = amg::build(A, pos, pinned, free_blob != 0, opt);
+        } catch (const std::runtime_error &e) {
+            throw Invalid(e.what());
+        }
+        const int L = (int)H.lv.size();
+        if (L > FFEA_GPU_MG_MAX_LEVELS) throw Invalid("ffea_gpu_multigrid_inspect: too many levels");
+        *info = ffea_gpu_multigrid_info{};
+        info->num_levels = L;
+        for (int l = 0; l < L; ++l) {
+            info->rows[l] = H.lv[l].A.n;
+            info->block[l] = H.lv[l].A.br;
+            info->blocks[l] = H.lv[l].A.nnzb();
+            info->omega[l] = H.lv[l].omega;
+            info->lmax[l] = H.lv[l].lmax;
+            if (l + 1 < L) info->transfer_blocks[l] = H.lv[l].P.nnzb();
+        }
+        info->coarsest_unknowns = H.nc;
+        info->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (L >= 2 && p_rowptr && p_col && p_val) {
+            const amg::BCsr &P = H.lv[0].P;
+            std::copy(P.rp.begin(), P.rp.end(), p_rowptr);
+            std::copy(P.ci.begin(), P.ci.end(), p_col);
+            std::copy(P.v.begin(), P.v.end(), p_val);
+        }
+        if (L >= 2 && a_rowptr && a_col && a_val) {
+            const amg::BCsr &C = H.lv[1].A;
+            std::copy(C.rp.begin(), C.rp.end(), a_rowptr);
+            std::copy(C.ci.begin(), C.ci.end(), a_col);
+            std::copy(C.v.begin(), C.v.end(), a_val);
+        }
+        return (int)FFEA_GPU_OK;
+    });
 }
 
 int ffea_gpu_blob_set_timing(ffea_gpu_blob *b, unsigned phase_mask) {

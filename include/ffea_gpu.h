@@ -244,6 +244,54 @@ int ffea_gpu_blob_autotune(ffea_gpu_blob *blob, int iterations, int *chosen);
  * graph steps. No counterpart in the reference. */
 int ffea_gpu_blob_use_graph(ffea_gpu_blob *blob, int enable, int iterations);
 
+/* Preconditioner of the viscosity-matrix CG. JACOBI is the reference's
+ * (NoMassCGSolver.cpp:268-281, parallel_apply_preconditioner) and the default.
+ * MULTIGRID has no counterpart in the reference: a smoothed-aggregation multigrid V-cycle
+ * whose coarse unknowns are the 3 translations and 3 rotations of node aggregates, with
+ * one damped-Jacobi smoothing step on each side of every coarse correction. Its
+ * hierarchy is built when the preconditioner is set, from the viscosity matrix at the
+ * current positions (an element pass and an assembly that leave positions, velocities
+ * and noise streams untouched); every later step applies its own matrix on the finest
+ * level. The CG keeps the reference's start (x = 0) and stopping test, so each solve
+ * meets the same tolerance in far fewer iterations; the solution differs from the
+ * Jacobi-preconditioned one within that tolerance. A free blob's solution is returned in
+ * the rigid-mode gauge of Jacobi-PCG (Z^T D x = 0, Z the rigid modes, D the diagonal).
+ * Single-blob handles with more than 1024 corner nodes; adjoint solves keep Jacobi. */
+typedef enum {
+    FFEA_GPU_PRECOND_JACOBI = 0,
+    FFEA_GPU_PRECOND_MULTIGRID = 1
+} ffea_gpu_preconditioner;
+
+#define FFEA_GPU_MG_MAX_LEVELS 8
+typedef struct {
+    int num_levels;                                 /* 0: Jacobi; else levels including the coarsest */
+    int rows[FFEA_GPU_MG_MAX_LEVELS];               /* block rows (level 0: corner nodes; then aggregates) */
+    int block[FFEA_GPU_MG_MAX_LEVELS];              /* unknowns per block row: 3 on level 0, 6 above */
+    long long blocks[FFEA_GPU_MG_MAX_LEVELS];       /* blocks of each level's operator */
+    long long transfer_blocks[FFEA_GPU_MG_MAX_LEVELS]; /* blocks of the prolongator from level l to l+1 */
+    double omega[FFEA_GPU_MG_MAX_LEVELS];           /* smoothing weights */
+    double lmax[FFEA_GPU_MG_MAX_LEVELS];            /* estimates of the largest eigenvalue of D^-1 A */
+    int coarsest_unknowns;                          /* the densely inverted level */
+    double setup_seconds;
+} ffea_gpu_multigrid_info;
+
+int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *blob, int kind);
+int ffea_gpu_blob_get_multigrid_info(ffea_gpu_blob *blob, ffea_gpu_multigrid_info *info);
+
+/* Host-only (no device): the multigrid hierarchy of a corner-node matrix given in 3x3
+ * block CSR (num_rows block rows, rowptr [num_rows+1], col [nnzb], val [nnzb][9]
+ * row-major blocks; pinned rows hold an identity block only), positions [num_rows][3],
+ * pinned [num_rows] flags (may be NULL), free_blob = 1 for a singular (unpinned, no
+ * drag) matrix. Fills *info; where the pointers are not NULL also copies the first
+ * prolongator P0 (rows num_rows, 3x6 blocks: p_rowptr [num_rows+1], p_col
+ * [transfer_blocks[0]], p_val [transfer_blocks[0]][18]) and the first coarse operator
+ * A1 = P0^T A P0 (6x6 blocks: a_rowptr [rows[1]+1], a_col [blocks[1]], a_val
+ * [blocks[1]][36]). */
+int ffea_gpu_multigrid_inspect(int num_rows, const int *rowptr, const int *col, const double *val,
+                               const double *pos, const unsigned char *pinned, int free_blob,
+                               ffea_gpu_multigrid_info *info, int *p_rowptr, int *p_col, double *p_val,
+                               int *a_rowptr, int *a_col, double *a_val);
+
 /* Kernel timing with HIP events recorded on the handle's stream around each launch of
  * the phases selected by phase_mask (bit i = phase i; 0 turns timing off).
  * Phases: 0 element forces, 1 matrix assembly, 2 gather + CG start, 3 CG SpMV step,
