@@ -871,13 +871,12 @@ struct ffea_gpu_blob {
         DevBuf<double> dinv, b, xa, res, xb, e;  // levels >= 1, [n][6]
     };
     int precond = FFEA_GPU_PRECOND_JACOBI;
-    bool mg_suspend = false;   // the adjoint's reverse solves keep the reference's Jacobi
     std::vector<std::unique_ptr<MgLevel>> mg;
     DevBuf<float> mg_cinv;
     int mg_nc = 0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
     ffea_gpu_multigrid_info mg_info{};
-    bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID && !mg_suspend; }
+    bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID; }
 
     // timing
     unsigned timing = 0;  // bit i: time phase i
@@ -1763,17 +1762,15 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p};
     hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kRowBlock), 0, stream, st);
     FFEA_HIP_TRY(hipGetLastError());
+    if (mg_on()) launch_mg_vcycle(Pi.p);  // z = M r of the start; its r.z replaces row 0
     zero_rhs_solved = 1;
-    mg_suspend = true;  // the start above set z = D^-1 r: the whole solve stays Jacobi
     try {
         run_cg_eager(0);
     } catch (...) {
         zero_rhs_solved = 0;
-        mg_suspend = false;
         throw;
     }
     zero_rhs_solved = 0;
-    mg_suspend = false;
     FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     if (!h_S[0].converged)

@@ -256,7 +256,8 @@ int ffea_gpu_blob_use_graph(ffea_gpu_blob *blob, int enable, int iterations);
  * meets the same tolerance in far fewer iterations; the solution differs from the
  * Jacobi-preconditioned one within that tolerance. A free blob's solution is returned in
  * the rigid-mode gauge of Jacobi-PCG (Z^T D x = 0, Z the rigid modes, D the diagonal).
- * Single-blob handles with more than 1024 corner nodes; adjoint solves keep Jacobi. */
+ * Single-blob handles with more than 1024 corner nodes; the adjoint's solves
+ * (ffea_gpu_blob_adjoint) use it too. */
 typedef enum {
     FFEA_GPU_PRECOND_JACOBI = 0,
     FFEA_GPU_PRECOND_MULTIGRID = 1

@@ -142,3 +142,25 @@ def test_refused_where_it_does_not_apply():
     with pytest.raises(FFEAException):
         batch.set_preconditioner("multigrid")
     batch.close()
+
+
+def test_adjoint_with_multigrid_matches_jacobi_adjoint():
+    """the reverse solves (K mu = P lambda per step) with the multigrid preconditioner give
+    the Jacobi-preconditioned adjoint's gradients to the CG tolerance's reach"""
+    spec = workloads.config3(n=11, epsilon=1e-13)
+    r = np.random.default_rng(1)
+    pos0 = spec.pos * np.array([1.0, 1.0, 1.02]) + r.uniform(-0.02, 0.02, spec.pos.shape)
+    pos0[spec.pinned] = spec.pos[spec.pinned]
+    target = spec.pos + r.uniform(-0.05, 0.05, spec.pos.shape)
+    corner = spec.mesh.is_linear
+    out = {}
+    for pc in ("jacobi", "multigrid"):
+        print(f"[case] adjoint {spec.name} ne={spec.num_elements} {pc}", flush=True)
+        with GpuBlob.from_spec(spec) as g:
+            g.set_positions(pos0)
+            g.set_preconditioner(pc)
+            g.run_checkpointed(4, 2)
+            seed = 2.0 * (g.positions() - target) * corner[:, None]
+            out[pc] = g.adjoint(seed)
+    for a, b in zip(out["multigrid"], out["jacobi"]):
+        assert rel_err(a, b) <= 1e-6
