@@ -214,3 +214,22 @@ def test_config0_1000_steps_noise_on(oracle_mod):
     print("config0 1000 steps position rel err:", e)
     assert e <= POS_TOL
     assert np.array_equal(g.rng_state(), o.rng_state())
+
+
+def test_large_mesh_assembly_against_oracle(oracle_mod):
+    """a blob of 64^3 cubes (274,625 corner nodes, 1.57M tets): its matrix is assembled with
+    each lane walking its rows' slots (the large-mesh launch shape); one noise-off step
+    from deformed positions with a loose CG tolerance keeps the oracle's solve short"""
+    spec = workloads.cube_blob(64, calc_noise=0, epsilon=1e-2, name="cube64")
+    r = np.random.default_rng(2)
+    p = spec.pos * np.array([1.0, 1.0, 1.01]) + r.uniform(-0.02, 0.02, spec.pos.shape)
+    o, g = make_pair(oracle_mod, spec)
+    assert g.info()["num_linear_nodes"] >= 1024 * 256
+    o.set_positions(p)
+    g.set_positions(p)
+    assert o.step() == 0
+    g.step(1)
+    assert rel_err(g.rhs(), o.rhs()) <= FORCE_TOL
+    assert g.cg_iterations()[0] == o.cg_iterations()
+    assert rel_err(g.velocities(), o.velocities()) <= 1e-8
+    g.close()
