@@ -6,9 +6,13 @@
 // CG's own sliced-ELL storage) and levels 1.. the Galerkin operators over aggregates
 // (6x6 blocks). The smoother is one damped-Jacobi step before and after the coarse
 // correction, x += w D^-1 (b - A x), which keeps M symmetric. Level-0 vectors are the
-// CG's 64-row tiled FP64 vectors; coarse vectors are FP64 [rows][6]. Transfer and coarse
-// operators are FP32 in their own sliced ELL: 64 block rows per slice, slot width W =
-// the slice's longest row, value k of the block in slot j of lane l at
+// CG's 64-row tiled FP64 vectors; coarse vectors are FP64 [rows][6]. The preconditioner's
+// matrices - a copy of K written by the assembly, the transfers and the coarse operators -
+// are FP16 scaled by a power of two per matrix (largest entry near 2^10; a sum is
+// multiplied back by the exact inverse), with FP64 accumulation: M only has to be a
+// fixed symmetric positive operator, and half the bytes make the V-cycle cheaper.
+// Prolongations (few blocks per row) are in a sliced ELL: 64 block rows per slice, slot
+// width W = the slice's longest row, value k of the block in slot j of lane l at
 // (base + j*64) * BR*BC + k*64 + l, block column at base + j*64 + l (-1: padding).
 #pragma once
 #include "kernels.h"
@@ -37,7 +41,8 @@ struct MgArgs {
     const int *sw;          // [slices] slot widths
     const long long *sb;    // [slices] slot bases
     const int *col;         // [slots]
-    const float *val;       // [slots * BR*BC]
+    const _Float16 *val;    // [slots * BR*BC], scaled
+    double mscale;          // inverse of the values' scale
     const double *x;        // input gathered by block column
     const double *b;        // RESID, POST: right-hand side (row layout)
     const double *base;     // PROLONG: y = bscale base + A x (row layout)
@@ -69,7 +74,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
         double xv[BC];
 #pragma unroll
         for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
-        const float *v = a.val + (size_t)slot * (BR * BC) + lane;
+        const _Float16 *v = a.val + (size_t)slot * (BR * BC) + lane;
 #pragma unroll
         for (int p = 0; p < BR; ++p)
 #pragma unroll
@@ -77,6 +82,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
     }
 #pragma unroll
     for (int p = 0; p < BR; ++p) {
+        acc[p] *= a.mscale;
         const size_t o = VecLayout<BR>::at(row, p);
         if constexpr (MODE == MG_RESID) {
             a.y[o] = a.b[o] - acc[p];
@@ -95,14 +101,15 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
 // and ~45 per row, and the restrictions, ~100 blocks per row): block CSR and one
 // wavefront per block row. Lane l takes the row's blocks l, l+64, ...; the lanes'
 // partial sums are combined by a fixed butterfly, so results do not depend on timing.
-// A row's W blocks are stored vector-major: with V-float vectors (V = 4 for 6x6 blocks,
+// A row's W blocks are stored vector-major: with V-value vectors (V = 4 for 6x6 blocks,
 // 2 for 6x3), vector t of block j is at (rp[row] * BR*BC) + (t W + j) V, so one vector
 // load of a wavefront reads consecutive blocks' vectors (contiguous bytes).
 struct MgCsrArgs {
     const CgScalars *S;
     int n;
     const int *rp, *ci;
-    const float *val;       // [nnzb][BR*BC]
+    const _Float16 *val;    // [nnzb][BR*BC], scaled
+    double mscale;          // inverse of the values' scale
     const double *x, *b, *dinv;
     double omega;
     double *y, *y2;
@@ -117,7 +124,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
 #pragma unroll
     for (int p = 0; p < BR; ++p) acc[p] = 0.0;
     const int r0 = a.rp[row], W = a.rp[row + 1] - r0;
-    const float *rowv = a.val + (size_t)r0 * (BR * BC);
+    const _Float16 *rowv = a.val + (size_t)r0 * (BR * BC);
     for (int j = lane; j < W; j += 64) {
         const int c = a.ci[r0 + j];
         double xv[BC];
@@ -125,16 +132,17 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
         for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
         float w[BR * BC];
         if constexpr ((BR * BC) % 4 == 0) {
+            typedef _Float16 vh4 __attribute__((ext_vector_type(4)));
 #pragma unroll
             for (int t = 0; t < BR * BC / 4; ++t) {
-                const vf4 u = __builtin_nontemporal_load((const vf4 *)rowv + (size_t)t * W + j);
+                const vh4 u = __builtin_nontemporal_load((const vh4 *)rowv + (size_t)t * W + j);
                 w[4 * t] = u.x; w[4 * t + 1] = u.y; w[4 * t + 2] = u.z; w[4 * t + 3] = u.w;
             }
         } else {
-            typedef float vf2 __attribute__((ext_vector_type(2)));
+            typedef _Float16 vh2 __attribute__((ext_vector_type(2)));
 #pragma unroll
             for (int t = 0; t < BR * BC / 2; ++t) {
-                const vf2 u = __builtin_nontemporal_load((const vf2 *)rowv + (size_t)t * W + j);
+                const vh2 u = __builtin_nontemporal_load((const vh2 *)rowv + (size_t)t * W + j);
                 w[2 * t] = u.x; w[2 * t + 1] = u.y;
             }
         }
@@ -144,7 +152,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
             for (int q = 0; q < BC; ++q) acc[p] += (double)w[p * BC + q] * xv[q];
     }
 #pragma unroll
-    for (int p = 0; p < BR; ++p) acc[p] = wave_sum(acc[p]);
+    for (int p = 0; p < BR; ++p) acc[p] = wave_sum(acc[p]) * a.mscale;
     double mine = acc[0];
 #pragma unroll
     for (int p = 1; p < BR; ++p)
@@ -190,7 +198,8 @@ struct MgFineArgs {
     const int *slice_w;
     const long long *slice_base;
     const int *col;
-    const T *val;
+    const T *val;          // the assembly's scaled FP16 copy of K (the CG's slots)
+    double mscale;         // inverse of its scale
     const double *x;       // gathered: z (RESID) or xb (POST)
     const double *r, *dinv;
     double omega;
@@ -221,7 +230,7 @@ __global__ __launch_bounds__(kRowBlock) void k_mg_fine(MgFineArgs<T> a) {
             acc1 += (double)w[3] * x0; acc1 += (double)w[4] * x1; acc1 += (double)w[5] * x2;
             acc2 += (double)w[6] * x0; acc2 += (double)w[7] * x1; acc2 += (double)w[8] * x2;
         }
-        const double acc[3] = {acc0, acc1, acc2};
+        const double acc[3] = {acc0 * a.mscale, acc1 * a.mscale, acc2 * a.mscale};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const size_t o = vi(n, k);

@@ -854,14 +854,16 @@ struct ffea_gpu_blob {
     // multigrid preconditioner (single blob): levels 0..L-1, level L-1 inverted densely
     struct MgEll {
         int n = 0;
+        double mscale = 1.0;   // inverse of the values' power-of-two scale
         DevBuf<int> sw, col;
         DevBuf<long long> sb;
-        DevBuf<float> val;
+        DevBuf<_Float16> val;
     };
     struct MgCsr {
         int n = 0;
+        double mscale = 1.0;
         DevBuf<int> rp, ci;
-        DevBuf<float> val;
+        DevBuf<_Float16> val;
     };
     struct MgLevel {
         int n = 0;             // block rows
@@ -874,6 +876,8 @@ struct ffea_gpu_blob {
     std::vector<std::unique_ptr<MgLevel>> mg;
     DevBuf<float> mg_cinv;
     int mg_nc = 0;
+    DevBuf<_Float16> mg_hval;  // scaled FP16 copy of the viscosity matrix, written by the assembly
+    double mg_hscale = 1.0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
     ffea_gpu_multigrid_info mg_info{};
     bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID; }
@@ -950,12 +954,13 @@ struct ffea_gpu_blob {
         graph_exec = nullptr;
     }
     RowArgs row_args() { return RowArgs{nchunks, cpb, chunks.p, seg.p, S.p, num_done.p}; }
+    // assembly grid: a workgroup per slot and row block for small meshes, per row block
+    // (its lanes walking their rows' slots) from 1024 row blocks on
+    dim3 asm_grid() const { return grid_rows >= 1024 ? dim3(grid_rows, 1) : dim3(grid_rows, max_slice_w); }
     void launch_element(bool advance = true);
     void launch_assemble();
     void launch_assemble_gather();
     void mg_setup();
-    template <typename T>
-    void launch_mg_vcycle_t(const T *val, double *Pout);
     void launch_mg_vcycle(double *Pout);
     void launch_mg_gauge();
     template <typename T>
@@ -1235,12 +1240,12 @@ void ffea_gpu_blob::launch_assemble() {
     t_begin(1, &t0);
     if (mixed) {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
-                         f_val.p, dinv.p, stokes ? drag.p : nullptr};
-        hipLaunchKernelGGL(k_assemble<float>, dim3(grid_rows, max_slice_w), dim3(kBlock), 0, stream, a);
+                         f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
+        hipLaunchKernelGGL(k_assemble<float>, asm_grid(), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
-                          d_val.p, dinv.p, stokes ? drag.p : nullptr};
-        hipLaunchKernelGGL(k_assemble<double>, dim3(grid_rows, max_slice_w), dim3(kBlock), 0, stream, a);
+                          d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
+        hipLaunchKernelGGL(k_assemble<double>, asm_grid(), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(1, t0);
@@ -1322,9 +1327,21 @@ namespace {
 struct EllHost {
     std::vector<int> sw, col;
     std::vector<long long> sb;
-    std::vector<float> val;
+    std::vector<_Float16> val;
 };
-EllHost to_ell(const amg::BCsr &A) {
+// power of two s with max|v| s in [512, 1024): FP16 keeps 64x headroom above it
+double fp16_scale(double maxabs) {
+    if (!(maxabs > 0.0) || !std::isfinite(maxabs)) return 1.0;
+    int e = 0;
+    (void)std::frexp(maxabs, &e);
+    return std::ldexp(1.0, 10 - e);
+}
+double max_abs(const std::vector<double> &v) {
+    double m = 0.0;
+    for (double x : v) m = std::max(m, std::fabs(x));
+    return m;
+}
+EllHost to_ell(const amg::BCsr &A, double scale) {
     EllHost E;
     const int ns = (A.n + 63) / 64, bb = A.br * A.bc;
     E.sw.assign(std::max(1, ns), 1);
@@ -1338,14 +1355,15 @@ EllHost to_ell(const amg::BCsr &A) {
         slots += (long long)w * 64;
     }
     E.col.assign(std::max<long long>(1, slots), -1);
-    E.val.assign((size_t)std::max<long long>(1, slots) * bb, 0.0f);
+    E.val.assign((size_t)std::max<long long>(1, slots) * bb, (_Float16)0.0f);
     for (int i = 0; i < A.n; ++i) {
         const int s = i / 64, lane = i % 64;
         for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
             const long long slot = E.sb[s] + (long long)(k - A.rp[i]) * 64;
             if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
             E.col[slot + lane] = A.ci[k];
-            for (int e = 0; e < bb; ++e) E.val[(size_t)slot * bb + (size_t)e * 64 + lane] = (float)A.v[(size_t)k * bb + e];
+            for (int e = 0; e < bb; ++e)
+                E.val[(size_t)slot * bb + (size_t)e * 64 + lane] = (_Float16)(A.v[(size_t)k * bb + e] * scale);
         }
     }
     return E;
@@ -1432,10 +1450,13 @@ void ffea_gpu_blob::mg_setup() {
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     mg.clear();
     mg_cinv.reset();
+    mg_hval.reset();
     for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef}) bf->reset();
     auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
-        const EllHost E = to_ell(A);
+        const double sc = fp16_scale(max_abs(A.v));
+        const EllHost E = to_ell(A, sc);
         D.n = A.n;
+        D.mscale = 1.0 / sc;
         D.sw.alloc(E.sw.size(), &dev_bytes); D.sw.upload(E.sw, stream);
         D.sb.alloc(E.sb.size(), &dev_bytes); D.sb.upload(E.sb, stream);
         D.col.alloc(E.col.size(), &dev_bytes); D.col.upload(E.col, stream);
@@ -1448,15 +1469,18 @@ void ffea_gpu_blob::mg_setup() {
         // vector-major rows (k_mg_csr): vector t of block j of row i at (rp[i] BB) + (t W + j) V
         const int BB = A.br * A.bc, V = BB % 4 == 0 ? 4 : 2;
         if (BB % V) throw Invalid("internal error: multigrid block size");
-        std::vector<float> v(A.v.size());
+        const double sc = fp16_scale(max_abs(A.v));
+        std::vector<_Float16> v(A.v.size());
         for (int i = 0; i < A.n; ++i) {
             const int r0 = A.rp[i], W = A.rp[i + 1] - r0;
             for (int j = 0; j < W; ++j)
                 for (int t = 0; t < BB / V; ++t)
                     for (int u = 0; u < V; ++u)
-                        v[(size_t)r0 * BB + ((size_t)t * W + j) * V + u] = (float)A.v[((size_t)r0 + j) * BB + t * V + u];
+                        v[(size_t)r0 * BB + ((size_t)t * W + j) * V + u] =
+                            (_Float16)(A.v[((size_t)r0 + j) * BB + t * V + u] * sc);
         }
         D.n = A.n;
+        D.mscale = 1.0 / sc;
         D.rp.alloc(A.rp.size(), &dev_bytes); D.rp.upload(A.rp, stream);
         D.ci.alloc(A.ci.size(), &dev_bytes); D.ci.upload(A.ci, stream);
         D.val.alloc(v.size(), &dev_bytes); D.val.upload(v, stream);
@@ -1506,6 +1530,9 @@ void ffea_gpu_blob::mg_setup() {
         mg_cinv.upload(ci, stream);
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
+    mg_hscale = fp16_scale(max_abs(fine.v));
+    mg_hval.alloc((size_t)nslots * 9, &dev_bytes);
+    launch_assemble();  // fills the FP16 copy at the current positions
     const size_t vlen = (size_t)nslices * 3 * kSlice;
     mg_res0.alloc(vlen, &dev_bytes); mg_res0.zero(stream);
     mg_xb0.alloc(vlen, &dev_bytes); mg_xb0.zero(stream);
@@ -1518,24 +1545,25 @@ void ffea_gpu_blob::mg_setup() {
 
 // z = M r (one V-cycle), r and z = D^-1 r given by the CG start or vector step; the
 // chunk partials of r.z go to Pout[c]
-template <typename T>
-void ffea_gpu_blob::launch_mg_vcycle_t(const T *val, double *Pout) {
+void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     const int L = (int)mg.size();
     auto grid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock - 1) / kMgBlock)); };
     MgLevel &f = *mg[0];
-    MgFineArgs<T> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, val, z.p, r.p, dinv.p, f.omega,
-                     mg_res0.p, nullptr};
-    hipLaunchKernelGGL((k_mg_fine<T, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+    const double hms = 1.0 / mg_hscale;
+    MgFineArgs<_Float16> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, z.p, r.p,
+                            dinv.p, f.omega, mg_res0.p, nullptr};
+    hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
     auto wgrid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock / 64 - 1) / (kMgBlock / 64))); };
     for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
         const bool coarsest = l + 1 == L - 1;
-        MgCsrArgs ra{S.p, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, l == 0 ? mg_res0.p : cur.res.p, nullptr,
-                     coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p};
+        MgCsrArgs ra{S.p, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, cur.R.mscale, l == 0 ? mg_res0.p : cur.res.p,
+                     nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p};
         if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         if (!coarsest) {
-            MgCsrArgs rs{S.p, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.xa.p, nx.b.p, nullptr, 0.0, nx.res.p, nullptr};
+            MgCsrArgs rs{S.p, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.A.mscale, nx.xa.p, nx.b.p, nullptr, 0.0,
+                         nx.res.p, nullptr};
             hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESID>), wgrid(nx.n), dim3(kMgBlock), 0, stream, rs);
         }
     }
@@ -1544,25 +1572,20 @@ void ffea_gpu_blob::launch_mg_vcycle_t(const T *val, double *Pout) {
                        (const float *)mg_cinv.p, (const double *)c.b.p, c.e.p);
     for (int l = L - 2; l >= 1; --l) {  // up: prolong, post-smooth
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
-        MgArgs pa{S.p, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, nx.e.p, nullptr, cur.xa.p, nullptr,
-                  0.0, 1.0, cur.xb.p, nullptr};
+        MgArgs pa{S.p, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, cur.P.mscale, nx.e.p, nullptr,
+                  cur.xa.p, nullptr, 0.0, 1.0, cur.xb.p, nullptr};
         hipLaunchKernelGGL((k_mg_apply<6, 6, MG_PROLONG>), grid(cur.n), dim3(kMgBlock), 0, stream, pa);
-        MgCsrArgs po{S.p, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.xb.p, cur.b.p, cur.dinv.p, cur.omega,
-                     cur.e.p, nullptr};
+        MgCsrArgs po{S.p, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.A.mscale, cur.xb.p, cur.b.p, cur.dinv.p,
+                     cur.omega, cur.e.p, nullptr};
         hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(cur.n), dim3(kMgBlock), 0, stream, po);
     }
-    MgArgs p0{S.p, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, mg[1]->e.p, nullptr, z.p, nullptr, 0.0, f.omega,
-              mg_xb0.p, nullptr};
+    MgArgs p0{S.p, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, mg[1]->e.p, nullptr, z.p, nullptr,
+              0.0, f.omega, mg_xb0.p, nullptr};
     hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
-    MgFineArgs<T> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, val, mg_xb0.p, r.p, dinv.p, f.omega,
-                     z.p, Pout};
-    hipLaunchKernelGGL((k_mg_fine<T, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+    MgFineArgs<_Float16> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, mg_xb0.p, r.p,
+                            dinv.p, f.omega, z.p, Pout};
+    hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
     FFEA_HIP_TRY(hipGetLastError());
-}
-
-void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
-    if (mixed) launch_mg_vcycle_t<float>(f_val.p, Pout);
-    else launch_mg_vcycle_t<double>(d_val.p, Pout);
 }
 
 // the free blob's solution in the Jacobi-PCG gauge, Z^T D x = 0, before the update
@@ -2458,6 +2481,7 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->precond = FFEA_GPU_PRECOND_JACOBI;
             b->mg.clear();
             b->mg_cinv.reset();
+            b->mg_hval.reset();
             for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef}) bf->reset();
             b->mg_info = ffea_gpu_multigrid_info{};
             return (int)FFEA_GPU_OK;

@@ -458,6 +458,8 @@ struct AsmArgs {
     T *val;                      // [nslots*9]
     double *dinv;                // tiled [nrows][3]
     const double *drag;          // [nrows] Stokes drag of unpinned rows, or null
+    _Float16 *hval;              // [nslots*9] the multigrid preconditioner's scaled copy, or null
+    double hscale;               // its scale
 };
 
 // Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
@@ -487,7 +489,11 @@ __device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int 
 // SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
 // 305-312): every block entry is the sum of its element sources in element order.
 // Grid (row blocks, widest slice): blockIdx.y is the slot j, so each lane follows the
-// source chain of one block only (the rows' slots are independent).
+// source chain of one block only (the rows' slots are independent) - the shape for small
+// meshes. With a grid of height 1 each lane takes all slots of its row in turn: a
+// workgroup then reads the shape records of its rows' elements while they are cached,
+// the shape for large meshes (the elements' 16 sources otherwise fall in 16 passes over
+// the whole mesh). Either way each block is the same sum.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma clang fp contract(off)
@@ -496,9 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
     const int sl = n / kSlice, lane = n % kSlice;
     const int W = a.slice_w[sl];
     const long long base = a.slice_base[sl];
-    const int j = blockIdx.y;
-    if (j >= W) return;
-    {
+    const int j0 = gridDim.y > 1 ? (int)blockIdx.y : 0;
+    const int j1 = gridDim.y > 1 ? min(j0 + 1, W) : W;
+    for (int j = j0; j < j1; ++j) {
         const long long slot = base + (long long)j * kSlice + lane;
         const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
         double acc[9];
@@ -534,6 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)acc[k];
+        if (a.hval)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a.hval[val_index<_Float16>(base, j, lane, k)] = (_Float16)(acc[k] * a.hscale);
         if (j == a.diag_j[n]) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
@@ -780,6 +789,14 @@ __device__ __forceinline__ void load_block<double>(const double *slot, int lane,
         w[2 * h] = v.x;
         w[2 * h + 1] = v.y;
     }
+    w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
+}
+template <>
+__device__ __forceinline__ void load_block<_Float16>(const _Float16 *slot, int lane, _Float16 (&w)[9]) {
+    typedef _Float16 vh8 __attribute__((ext_vector_type(8)));
+    const vh8 v = __builtin_nontemporal_load((const vh8 *)slot + lane);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = v[k];
     w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
 }
 template <>
