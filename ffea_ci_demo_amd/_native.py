@@ -33,7 +33,7 @@ EXPORTED = [
     "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint", "ffea_gpu_batch_strain_energy",
     "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
-    "ffea_gpu_multigrid_inspect",
+    "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -52,6 +52,11 @@ class Info(C.Structure):
                 ("device_bytes", C.c_longlong), ("num_blobs", C.c_int), ("num_chunks", C.c_int),
                 ("row_blocks", C.c_int), ("graph_iterations", C.c_int),
                 ("single_launch_cg", C.c_int)]
+
+
+class SurfaceForces(C.Structure):
+    _fields_ = [("num_faces", C.c_int), ("face_nodes", C.c_void_p), ("num_sets", C.c_int),
+                ("set_force", C.c_void_p), ("set_ptr", C.c_void_p), ("set_faces", C.c_void_p)]
 
 
 class MultigridInfo(C.Structure):
@@ -86,6 +91,9 @@ def lib():
     vp = C.c_void_p
     L.ffea_gpu_blob_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
                                        C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(vp)]
+    L.ffea_gpu_blob_create_ex.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
+                                          C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(SurfaceForces),
+                                          C.POINTER(vp)]
     L.ffea_gpu_blob_destroy.argtypes = [vp]
     L.ffea_gpu_blob_destroy.restype = None
     for f in ("update_internal_forces", "update_positions", "synchronize"):

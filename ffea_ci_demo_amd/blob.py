@@ -129,7 +129,9 @@ def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precis
 class GpuBlob(_Handle):
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=1000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
-                 precision="fp64", device=0):
+                 surface_forces=None, precision="fp64", device=0):
+        """surface_forces: None or (face_nodes [nf][3], set_force [ns][3] meso, set_ptr [ns+1],
+        set_faces): linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806)"""
         L = _native.lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
@@ -148,8 +150,15 @@ class GpuBlob(_Handle):
         nctf = 0 if cn is None else cn.size
         st = np.ascontiguousarray(rng_state if rng_state is not None else [12345] * 6, np.uint32)
         h = C.c_void_p()
-        rc = L.ffea_gpu_blob_create(C.byref(p), pos.shape[0], ne, pos, top, *mats, npin, pin_p, nctf, cn_p, cf_p,
-                                    drag_p, st, C.byref(h))
+        sf = None
+        if surface_forces is not None:
+            fn, sfo, sp, sfc = surface_forces
+            keep = [np.ascontiguousarray(fn, np.int32).reshape(-1), np.ascontiguousarray(sfo, np.float64).reshape(-1),
+                    np.ascontiguousarray(sp, np.int32), np.ascontiguousarray(sfc, np.int32)]
+            sf = _native.SurfaceForces(keep[0].size // 3, keep[0].ctypes.data, keep[2].size - 1, keep[1].ctypes.data,
+                                       keep[2].ctypes.data, keep[3].ctypes.data)
+        rc = L.ffea_gpu_blob_create_ex(C.byref(p), pos.shape[0], ne, pos, top, *mats, npin, pin_p, nctf, cn_p, cf_p,
+                                       drag_p, st, None if sf is None else C.byref(sf), C.byref(h))
         if rc != _native.OK:
             raise FFEAException(rc, L.ffea_gpu_last_error().decode())
         self.h = h
@@ -163,7 +172,7 @@ class GpuBlob(_Handle):
                    epsilon2=spec.epsilon2, max_iterations_cg=spec.max_iterations_cg, calc_noise=spec.calc_noise,
                    calc_stokes=spec.calc_stokes, pinned=spec.pinned, ctf_node=spec.ctf_node,
                    ctf_force=spec.ctf_force, stokes_drag=spec.stokes_drag, rng_state=spec.rng_state,
-                   precision=precision, **kw)
+                   surface_forces=getattr(spec, "surface_forces", None), precision=precision, **kw)
 
     # -- the reference's per-step interface
     def update_internal_forces(self):

@@ -121,6 +121,7 @@ struct BlobInput {
     const double *ctf_force = nullptr;
     const uint32_t *rng = nullptr;
     const double *drag = nullptr;  // [nt] Stokes drag per node (calc_stokes), else null
+    const ffea_gpu_surface_forces *sf = nullptr;  // surface constant forces, or null
 };
 
 // closed-form rest state (Blob::calc_rest_state_info, Blob.cpp:3080-3143;
@@ -175,6 +176,15 @@ struct BlobPlan {
     std::vector<std::array<int, 3>> fm;
     std::vector<double> node_ctf;   // [nt][3]
     std::vector<double> drag_lin;   // [nl] drag of unpinned corner rows
+    // surface constant forces: the faces some set uses, in face order ("used faces")
+    bool has_sf = false;
+    std::vector<int> sf_rows;       // [nfu][3][2] corner rows whose midpoint is each face node's position
+    std::vector<double> sf_force;   // [ns][3]
+    std::vector<int> sf_set_ptr;    // [ns+1] entries of each set
+    std::vector<int> sf_entry_face; // [entries] used face of each entry (set order)
+    std::vector<int> sf_entry_set;  // [entries]
+    std::vector<int> sf_fe_ptr, sf_fe;  // per used face: its entries in set order
+    std::vector<int> sf_nf_ptr, sf_nf;  // per node of the .node file: used faces holding it, face order
     double centre[3] = {0, 0, 0};
     long long nblocks = 0;
     long long nslots() const { return sb.empty() ? 0 : sb.back(); }
@@ -393,8 +403,79 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
                 for (int c = 0; c < 3; ++c) fm[3 * nn[k] + c] = 0.0;
         }
     }
+    // ---- surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806)
+    if (in.sf && in.sf->num_sets > 0) {
+        const ffea_gpu_surface_forces &F = *in.sf;
+        if (F.num_faces < 0 || (F.num_faces && !F.face_nodes) || !F.set_force || !F.set_ptr || !F.set_faces ||
+            F.set_ptr[0] != 0)
+            throw Invalid(who + "surface constant forces: invalid argument");
+        for (int k = 0; k < 3 * F.num_faces; ++k)
+            if (F.face_nodes[k] < 0 || F.face_nodes[k] >= nt)
+                throw Invalid(who + fmt("surface face %d references an out of bounds node index", k / 3));
+        for (int st = 0; st < F.num_sets; ++st)
+            if (F.set_ptr[st + 1] < F.set_ptr[st]) throw Invalid(who + "surface constant forces: set_ptr decreases");
+        const int nent = F.set_ptr[F.num_sets];
+        std::vector<int> compact(F.num_faces, -1);
+        for (int k = 0; k < nent; ++k) {
+            const int f = F.set_faces[k];
+            if (f < 0 || f >= F.num_faces)
+                throw Invalid(who + fmt("surface constant force on an out of bounds face index %d", f));
+            compact[f] = 0;
+        }
+        int nfu = 0;
+        std::vector<int> used;
+        for (int f = 0; f < F.num_faces; ++f)
+            if (compact[f] == 0) {
+                compact[f] = nfu++;
+                used.push_back(f);
+            }
+        P.has_sf = nent > 0;
+        P.sf_rows.resize((size_t)6 * nfu);
+        for (int u = 0; u < nfu; ++u)
+            for (int i = 0; i < 3; ++i) {
+                const int v = F.face_nodes[3 * used[u] + i];
+                int ra, rb;
+                if (P.lin_of[v] >= 0) {
+                    ra = rb = P.lin_of[v];
+                } else if (P.mid_a[v] >= 0) {
+                    ra = P.lin_of[P.mid_a[v]];
+                    rb = P.lin_of[P.mid_b[v]];
+                } else {
+                    throw Invalid(who + fmt("surface face node %d belongs to no element", v));
+                }
+                P.sf_rows[6 * u + 2 * i] = ra;
+                P.sf_rows[6 * u + 2 * i + 1] = rb;
+            }
+        P.sf_force.assign(F.set_force, F.set_force + 3 * (size_t)F.num_sets);
+        P.sf_set_ptr.assign(F.set_ptr, F.set_ptr + F.num_sets + 1);
+        P.sf_entry_face.resize(nent);
+        P.sf_entry_set.resize(nent);
+        for (int st = 0; st < F.num_sets; ++st)
+            for (int k = F.set_ptr[st]; k < F.set_ptr[st + 1]; ++k) {
+                P.sf_entry_face[k] = compact[F.set_faces[k]];
+                P.sf_entry_set[k] = st;
+            }
+        P.sf_fe_ptr.assign(nfu + 1, 0);
+        for (int k = 0; k < nent; ++k) P.sf_fe_ptr[P.sf_entry_face[k] + 1]++;
+        for (int u = 0; u < nfu; ++u) P.sf_fe_ptr[u + 1] += P.sf_fe_ptr[u];
+        P.sf_fe.resize(nent);
+        {
+            std::vector<int> fill(P.sf_fe_ptr.begin(), P.sf_fe_ptr.end() - 1);
+            for (int k = 0; k < nent; ++k) P.sf_fe[fill[P.sf_entry_face[k]]++] = k;  // entries ascend: set order
+        }
+        P.sf_nf_ptr.assign(nt + 1, 0);
+        for (int u = 0; u < nfu; ++u)
+            for (int i = 0; i < 3; ++i) P.sf_nf_ptr[F.face_nodes[3 * used[u] + i] + 1]++;
+        for (int v = 0; v < nt; ++v) P.sf_nf_ptr[v + 1] += P.sf_nf_ptr[v];
+        P.sf_nf.resize(P.sf_nf_ptr[nt]);
+        {
+            std::vector<int> fill(P.sf_nf_ptr.begin(), P.sf_nf_ptr.end() - 1);
+            for (int u = 0; u < nfu; ++u)  // face order
+                for (int i = 0; i < 3; ++i) P.sf_nf[fill[F.face_nodes[3 * used[u] + i]]++] = u;
+        }
+    }
     P.fp.assign(nl + 1, 0);
-    if (stokes_noise) {
+    if (stokes_noise || P.has_sf) {
         // every node carries noise: the folds are formed each step from the records
         std::vector<std::vector<std::array<int, 3>>> rec(nl);
         std::vector<unsigned char> folded(nt, 0);
@@ -469,6 +550,13 @@ struct Tables {
     std::vector<double> sc, node_ctf;    // [nnodes], [nnodes][3]
     std::vector<int3> fold_mid;
     std::vector<mrg::Jump> node_jump, njump_group, njump_lane;
+    // per-step midside folds (Stokes noise or surface constant forces): node-indexed tables
+    bool dyn_fold = false;
+    // surface constant forces (single-blob handles)
+    bool have_sf = false;
+    int sf_nfaces = 0, sf_nsets = 0;
+    std::vector<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
+    std::vector<double> sf_force;
 };
 
 Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_params &prm) {
@@ -500,6 +588,9 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         for (const auto &e : errors)
             if (!e.empty()) throw Invalid(e);
     }
+    for (int b = 0; b < nblob; ++b) t.have_sf |= plans[b].has_sf;
+    if (t.have_sf && nblob > 1) throw Invalid("surface constant forces need a single-blob handle");
+    t.dyn_fold = t.stokes_noise || t.have_sf;
     // ---- concatenation offsets: every blob starts on a 64-row slice boundary
     t.meta.resize(nblob);
     long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0, nodes = 0;
@@ -564,12 +655,14 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
     t.rng.assign((size_t)6 * nblob, 0);
     t.nnodes = (int)nodes;
     if (t.stokes) t.drag.assign(nrows, 0.0);
-    if (t.stokes_noise) {
+    if (t.dyn_fold) {
         t.node_off.assign(nblob, 0);
-        if (nblob > 1) t.node_blob.assign(t.nnodes, 0);
         t.row_node.assign(nrows, -1);
-        t.sc.assign(t.nnodes, 0.0);
         if (t.have_ctf) t.node_ctf.assign((size_t)3 * t.nnodes, 0.0);
+    }
+    if (t.stokes_noise) {
+        if (nblob > 1) t.node_blob.assign(t.nnodes, 0);
+        t.sc.assign(t.nnodes, 0.0);
         t.node_jump.resize(nblob);
     }
     t.jinv0.assign((size_t)9 * ne, 0.0);
@@ -584,13 +677,15 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         BlobPlan &P = plans[b];
         if (t.stokes)
             for (int n = 0; n < P.nl; ++n) t.drag[t.meta[b].row_off + n] = P.drag_lin[n];
-        if (t.stokes_noise) {
+        if (t.dyn_fold) {
             t.node_off[b] = node_off;
             max_nt = std::max(max_nt, P.nt);
             for (int i = 0; i < P.nt; ++i) {
-                if (nblob > 1) t.node_blob[node_off + i] = b;
-                // Blob.cpp:3214-3216, evaluated on the host in IEEE double
-                t.sc[node_off + i] = std::sqrt((24 * prm.kT * in[b].drag[i]) / (prm.dt));
+                if (t.stokes_noise) {
+                    if (nblob > 1) t.node_blob[node_off + i] = b;
+                    // Blob.cpp:3214-3216, evaluated on the host in IEEE double
+                    t.sc[node_off + i] = std::sqrt((24 * prm.kT * in[b].drag[i]) / (prm.dt));
+                }
                 if (t.have_ctf && !P.node_ctf.empty())
                     for (int c = 0; c < 3; ++c) t.node_ctf[3 * (size_t)(node_off + i) + c] = P.node_ctf[3 * (size_t)i + c];
             }
@@ -598,7 +693,20 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
             for (const auto &r3 : P.fm)
                 t.fold_mid.push_back(make_int3(r3[0] < 0 ? -1 : node_off + r3[0], r3[1] < 0 ? -1 : node_off + r3[1],
                                                r3[2] < 0 ? -1 : node_off + r3[2]));
-            t.node_jump[b] = mrg::h_pow(7ull * (unsigned long long)P.ne);
+            if (t.stokes_noise) t.node_jump[b] = mrg::h_pow(7ull * (unsigned long long)P.ne);
+        }
+        if (P.has_sf) {  // single blob: rows and nodes need no offsets
+            t.sf_nfaces = (int)P.sf_fe_ptr.size() - 1;
+            t.sf_nsets = (int)P.sf_set_ptr.size() - 1;
+            t.sf_rows = std::move(P.sf_rows);
+            t.sf_force = std::move(P.sf_force);
+            t.sf_set_ptr = std::move(P.sf_set_ptr);
+            t.sf_entry_face = std::move(P.sf_entry_face);
+            t.sf_entry_set = std::move(P.sf_entry_set);
+            t.sf_fe_ptr = std::move(P.sf_fe_ptr);
+            t.sf_fe = std::move(P.sf_fe);
+            t.sf_nf_ptr = std::move(P.sf_nf_ptr);
+            t.sf_nf = std::move(P.sf_nf);
         }
         const BlobMeta &m = t.meta[b];
         const int ro = m.row_off, eo = m.elem_off;
@@ -720,7 +828,7 @@ void validate(const Tables &t) {
         if (t.ip[n + 1] < t.ip[n]) fail("incidence order", n);
     for (size_t i = 0; i < t.ic.size(); ++i)
         if ((long long)(t.ic[i] >> 2) >= ne) fail("incidence element", (long long)i);
-    if (t.have_fold && !t.stokes_noise) {
+    if (t.have_fold && !t.dyn_fold) {
         if ((long long)t.fp.size() != nrows + 1 || t.fp[0] != 0 || 3 * (size_t)t.fp[nrows] != t.fv.size())
             fail("fold ends", nrows);
         for (long long n = 0; n < nrows; ++n)
@@ -756,24 +864,27 @@ void validate(const Tables &t) {
     }
     if ((long long)t.chunks.size() != nc || t.jump_lane.size() != 64) fail("chunk/jump table size", nc);
     if (t.stokes && (long long)t.drag.size() != nrows) fail("drag size", (long long)t.drag.size());
-    if (t.stokes_noise) {
+    if (t.dyn_fold) {
         const long long nn = t.nnodes;
-        if ((long long)t.sc.size() != nn || (long long)t.row_node.size() != nrows || (int)t.node_off.size() != t.nblob ||
-            (int)t.node_jump.size() != t.nblob || t.njump_lane.size() != 64 ||
-            (t.nblob > 1 && (long long)t.node_blob.size() != nn) || (t.have_ctf && (long long)t.node_ctf.size() != 3 * nn))
+        if ((long long)t.row_node.size() != nrows || (int)t.node_off.size() != t.nblob ||
+            (t.have_ctf && (long long)t.node_ctf.size() != 3 * nn))
+            fail("node table size", nn);
+        if (t.stokes_noise && ((long long)t.sc.size() != nn || (int)t.node_jump.size() != t.nblob ||
+                               t.njump_lane.size() != 64 || (t.nblob > 1 && (long long)t.node_blob.size() != nn)))
             fail("noise table size", nn);
         if ((long long)t.fp.size() != nrows + 1 || (size_t)t.fp[nrows] != t.fold_mid.size()) fail("fold record ends", nrows);
         for (long long n = 0; n < nrows; ++n)
             if (t.fp[n + 1] < t.fp[n]) fail("fold record order", n);
         for (int b = 0; b < t.nblob; ++b) {
             const BlobMeta &m = t.meta[b];
-            if (t.node_off[b] + (long long)m.nt > nn || ((long long)m.nt + 63) / 64 > (long long)t.njump_group.size())
+            if (t.node_off[b] + (long long)m.nt > nn) fail("node range", b);
+            if (t.stokes_noise && ((long long)m.nt + 63) / 64 > (long long)t.njump_group.size())
                 fail("noise node range", b);
             for (int n = 0; n < m.nl; ++n) {
                 const int g = t.row_node[m.row_off + n];
                 if (g < t.node_off[b] || g >= t.node_off[b] + m.nt) fail("row node", (long long)m.row_off + n);
             }
-            for (int i = 0; i < m.nt && t.nblob > 1; ++i)
+            for (int i = 0; t.stokes_noise && i < m.nt && t.nblob > 1; ++i)
                 if (t.node_blob[t.node_off[b] + i] != b) fail("node blob", (long long)t.node_off[b] + i);
         }
         for (size_t k = 0; k < t.fold_mid.size(); ++k) {
@@ -781,6 +892,31 @@ void validate(const Tables &t) {
             for (int q : m3)
                 if (q < -1 || q >= nn) fail("fold record node", (long long)k);
         }
+    }
+    if (t.have_sf) {
+        const long long nf = t.sf_nfaces, ns = t.sf_nsets, nn = t.nnodes;
+        const long long nent = t.sf_entry_face.size();
+        if (t.nblob != 1 || nf <= 0 || ns <= 0 || (long long)t.sf_rows.size() != 6 * nf ||
+            (long long)t.sf_force.size() != 3 * ns || (long long)t.sf_set_ptr.size() != ns + 1 || t.sf_set_ptr[0] != 0 ||
+            t.sf_set_ptr[ns] != nent || (long long)t.sf_entry_set.size() != nent ||
+            (long long)t.sf_fe_ptr.size() != nf + 1 || t.sf_fe_ptr[0] != 0 || t.sf_fe_ptr[nf] != nent ||
+            (long long)t.sf_fe.size() != nent || (long long)t.sf_nf_ptr.size() != nn + 1 || t.sf_nf_ptr[0] != 0 ||
+            (long long)t.sf_nf.size() != t.sf_nf_ptr[nn] || !t.dyn_fold)
+            fail("surface force table size", nf);
+        for (long long k = 0; k < 6 * nf; ++k)
+            if (t.sf_rows[k] < 0 || t.sf_rows[k] >= nrows) fail("surface face row", k);
+        for (long long i = 0; i < ns; ++i)
+            if (t.sf_set_ptr[i + 1] < t.sf_set_ptr[i]) fail("surface set order", i);
+        for (long long k = 0; k < nent; ++k)
+            if (t.sf_entry_face[k] < 0 || t.sf_entry_face[k] >= nf || t.sf_entry_set[k] < 0 || t.sf_entry_set[k] >= ns ||
+                t.sf_fe[k] < 0 || t.sf_fe[k] >= nent)
+                fail("surface entry", k);
+        for (long long f = 0; f < nf; ++f)
+            if (t.sf_fe_ptr[f + 1] < t.sf_fe_ptr[f]) fail("surface face entries", f);
+        for (long long v = 0; v < nn; ++v)
+            if (t.sf_nf_ptr[v + 1] < t.sf_nf_ptr[v]) fail("surface node faces", v);
+        for (size_t k = 0; k < t.sf_nf.size(); ++k)
+            if (t.sf_nf[k] < 0 || t.sf_nf[k] >= nf) fail("surface node face", (long long)k);
     }
 }
 
@@ -824,6 +960,12 @@ struct ffea_gpu_blob {
     DevBuf<int> node_blob, node_off, row_node;
     DevBuf<int3> fold_mid;
     DevBuf<mrg::Jump> node_jump, njump_group, njump_lane;
+    bool dyn_fold = false;  // midside folds formed per step (Stokes noise, surface forces)
+    // surface constant forces (single blob)
+    bool have_sf = false;
+    int sf_nfaces = 0, sf_nsets = 0, sf_nent = 0;
+    DevBuf<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
+    DevBuf<double> sf_force, sf_area, sf_total, sf_face_force;
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
     int max_slice_w = 1;  // widest slice: the assembly grid's second dimension
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
@@ -1089,18 +1231,39 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
         drag.alloc(t.drag.size(), &dev_bytes);
         drag.upload(t.drag, stream);
     }
-    if (stokes_noise) {
-        grid_nodes = (nnodes + kBlock - 1) / kBlock;
-        node_sc.alloc(t.sc.size(), &dev_bytes); node_sc.upload(t.sc, stream);
+    dyn_fold = t.dyn_fold;
+    if (dyn_fold) {
         node_off.alloc(t.node_off.size(), &dev_bytes); node_off.upload(t.node_off, stream);
         row_node.alloc(t.row_node.size(), &dev_bytes); row_node.upload(t.row_node, stream);
         fold_mid.alloc(t.fold_mid.size(), &dev_bytes); fold_mid.upload(t.fold_mid, stream);
+        if (have_ctf) { node_ctf.alloc(t.node_ctf.size(), &dev_bytes); node_ctf.upload(t.node_ctf, stream); }
+    }
+    if (stokes_noise) {
+        grid_nodes = (nnodes + kBlock - 1) / kBlock;
+        node_sc.alloc(t.sc.size(), &dev_bytes); node_sc.upload(t.sc, stream);
         node_jump.alloc(t.node_jump.size(), &dev_bytes); node_jump.upload(t.node_jump, stream);
         njump_group.alloc(t.njump_group.size(), &dev_bytes); njump_group.upload(t.njump_group, stream);
         njump_lane.alloc(t.njump_lane.size(), &dev_bytes); njump_lane.upload(t.njump_lane, stream);
         noise.alloc((size_t)3 * nnodes, &dev_bytes); noise.zero(stream);
         if (nblob > 1) { node_blob.alloc(t.node_blob.size(), &dev_bytes); node_blob.upload(t.node_blob, stream); }
-        if (have_ctf) { node_ctf.alloc(t.node_ctf.size(), &dev_bytes); node_ctf.upload(t.node_ctf, stream); }
+    }
+    have_sf = t.have_sf;
+    if (have_sf) {
+        sf_nfaces = t.sf_nfaces;
+        sf_nsets = t.sf_nsets;
+        sf_nent = (int)t.sf_entry_face.size();
+        sf_rows.alloc(t.sf_rows.size(), &dev_bytes); sf_rows.upload(t.sf_rows, stream);
+        sf_force.alloc(t.sf_force.size(), &dev_bytes); sf_force.upload(t.sf_force, stream);
+        sf_set_ptr.alloc(t.sf_set_ptr.size(), &dev_bytes); sf_set_ptr.upload(t.sf_set_ptr, stream);
+        sf_entry_face.alloc(t.sf_entry_face.size(), &dev_bytes); sf_entry_face.upload(t.sf_entry_face, stream);
+        sf_entry_set.alloc(t.sf_entry_set.size(), &dev_bytes); sf_entry_set.upload(t.sf_entry_set, stream);
+        sf_fe_ptr.alloc(t.sf_fe_ptr.size(), &dev_bytes); sf_fe_ptr.upload(t.sf_fe_ptr, stream);
+        sf_fe.alloc(t.sf_fe.size(), &dev_bytes); sf_fe.upload(t.sf_fe, stream);
+        sf_nf_ptr.alloc(t.sf_nf_ptr.size(), &dev_bytes); sf_nf_ptr.upload(t.sf_nf_ptr, stream);
+        sf_nf.alloc(t.sf_nf.size(), &dev_bytes); sf_nf.upload(t.sf_nf, stream);
+        sf_area.alloc(sf_nent, &dev_bytes);
+        sf_total.alloc(sf_nsets, &dev_bytes);
+        sf_face_force.alloc((size_t)3 * sf_nfaces, &dev_bytes); sf_face_force.zero(stream);
     }
     jump_group.alloc(t.jump_group.size(), &dev_bytes); jump_group.upload(t.jump_group, stream);
     jump_lane.alloc(t.jump_lane.size(), &dev_bytes); jump_lane.upload(t.jump_lane, stream);
@@ -1142,9 +1305,12 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     for (const void *ptr : need)
         if (!ptr) throw Invalid("internal error: a device buffer was not allocated");
     if ((nblob > 1 && !elem_blob.p) || (have_ctf && !ctf.p) || (have_fold && (!fold_ptr.p || !fold.p)) ||
-        (stokes && !drag.p) ||
-        (stokes_noise && (!node_sc.p || !node_off.p || !row_node.p || !fold_mid.p || !node_jump.p || !njump_group.p ||
-                          !njump_lane.p || !noise.p || (nblob > 1 && !node_blob.p) || (have_ctf && !node_ctf.p))))
+        (stokes && !drag.p) || (dyn_fold && (!node_off.p || !row_node.p || !fold_mid.p || (have_ctf && !node_ctf.p))) ||
+        (stokes_noise && (!node_sc.p || !node_jump.p || !njump_group.p || !njump_lane.p || !noise.p ||
+                          (nblob > 1 && !node_blob.p))) ||
+        (have_sf && (!sf_rows.p || !sf_force.p || !sf_set_ptr.p || !sf_entry_face.p || !sf_entry_set.p ||
+                     !sf_fe_ptr.p || !sf_fe.p || !sf_nf_ptr.p || !sf_nf.p || !sf_area.p || !sf_total.p ||
+                     !sf_face_force.p)))
         throw Invalid("internal error: a device buffer was not allocated");
     // row-kernel grid: one block per chunk up to 1024 blocks (tunable, result-neutral)
     set_row_blocks(1024);
@@ -1208,11 +1374,17 @@ int ffea_gpu_blob::check_status_after_forces() {
 
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_gather(const T *force) {
+    if (have_sf) {  // Blob::apply_ctforces CTF 3 at this step's positions
+        SurfArgs sa{sf_nfaces, sf_nsets, sf_nent, sf_rows.p, pos.p, sf_force.p, sf_set_ptr.p, sf_entry_face.p,
+                    sf_entry_set.p, sf_fe_ptr.p, sf_fe.p, sf_area.p, sf_total.p, sf_face_force.p};
+        hipLaunchKernelGGL(k_surface_forces, dim3(1), dim3(kRowBlock), 0, stream, sa);
+    }
     GatherArgs<T> g{row_args(), inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
                     have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr,
-                    stokes_noise ? noise.p : nullptr, stokes_noise ? row_node.p : nullptr,
-                    stokes_noise ? fold_mid.p : nullptr, stokes_noise && have_ctf ? node_ctf.p : nullptr, pinned.p,
-                    dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
+                    stokes_noise ? noise.p : nullptr, dyn_fold ? row_node.p : nullptr,
+                    dyn_fold ? fold_mid.p : nullptr, dyn_fold && have_ctf ? node_ctf.p : nullptr,
+                    have_sf ? sf_nf_ptr.p : nullptr, have_sf ? sf_nf.p : nullptr, have_sf ? sf_face_force.p : nullptr,
+                    pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
@@ -1997,14 +2169,23 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          int num_pinned, const int *pinned, int num_ctf, const int *ctf_node,
                          const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out) {
+    return ffea_gpu_blob_create_ex(params, num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf,
+                                   ctf_node, ctf_force, stokes_drag, rng_state, nullptr, out);
+}
+
+int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements, const double *pos,
+                            const int *top, const double *A, const double *B, const double *G, const double *E,
+                            int num_pinned, const int *pinned, int num_ctf, const int *ctf_node,
+                            const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
+                            const ffea_gpu_surface_forces *surface_forces, ffea_gpu_blob **out) {
     if (out) *out = nullptr;
     if (!params || !out) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
     }
     // the drag is used only with calc_stokes = 1 (NoMassCGSolver.cpp:119-127, Blob.cpp:3172)
-    const BlobInput bi{num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
-                       ctf_force, rng_state, params->calc_stokes ? stokes_drag : nullptr};
+    const BlobInput bi{num_nodes,      num_elements, pos,       top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
+                       ctf_force,      rng_state,    params->calc_stokes ? stokes_drag : nullptr, surface_forces};
     if (!valid_blob_input(bi)) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;

@@ -259,6 +259,9 @@ struct BlobData {
     std::vector<int> ctf_node;
     std::vector<double> ctf_force;
     int num_surface_faces = 0;
+    std::vector<int> surf_nodes;        // [faces][3] (.surf order, Face::n[0..2])
+    std::vector<double> sf_force;       // linear surface constant forces: [sets][3] meso
+    std::vector<int> sf_ptr{0}, sf_faces;
     double scale = 1.0;  // meso per file unit
     bool set_centroid = false;
     double centroid[3] = {0, 0, 0};
@@ -328,7 +331,8 @@ void load_topology(BlobData &B, const std::string &f) {
     }
 }
 
-// Blob::load_surface (Blob.cpp:2090-2170): faces are validated, not used on this path
+// Blob::load_surface (Blob.cpp:2090-2170): faces are validated and kept for surface
+// constant forces
 void load_surface(BlobData &B, const std::string &f) {
     FILE *in = open_or_fail(f);
     std::unique_ptr<FILE, int (*)(FILE *)> guard(in, fclose);
@@ -367,6 +371,7 @@ void load_surface(BlobData &B, const std::string &f) {
         }
         std::sort(slots.begin(), slots.end());
         if (std::find(ok.begin(), ok.end(), slots) == ok.end()) fail("Error: Could not find the opposite node.");
+        B.surf_nodes.insert(B.surf_nodes.end(), {n[0], n[1], n[2]});
     }
     B.num_surface_faces = nf;
 }
@@ -449,7 +454,8 @@ std::vector<std::string> split_ws(const std::string &s) {
     return out;
 }
 
-// Blob::load_ctforces, linear node forces (Blob.cpp:2478-2623)
+// Blob::load_ctforces, linear node forces and linear surface forces (Blob.cpp:2478-2623,
+// 2730-2806); rotational forces are refused
 void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
     std::vector<std::string> L = file_to_lines(f);
     if (L.size() < 4) fail("ctforces_fname '%s' is too short, something is wrong\n", f.c_str());
@@ -468,11 +474,13 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
     if (n_all != n_lin + n_rot + n_surf || n_all == 0 || calc_lines != L.size())
         fail("--- ABORTING. Something went wrong when parsing the header of the ctforces file\n"
              "--- Check it or turn calc_ctforces to 0 in the FFEA input file");
-    if (n_rot + n_surf > 0) fail("rotational and surface constant forces are not supported by this build");
-    if (L[now].compare(0, 14, "linear forces:") == 0) ++now;
-    else if (n_lin > 0)
-        fail("Error reading the ctforces file; it should announce the start of linear ctforces, but instead read: %s\n",
-             L[now].c_str());
+    if (n_rot > 0) fail("rotational constant forces are not supported by this build");
+    if (n_lin > 0) {
+        if (L[now].compare(0, 14, "linear forces:") == 0) ++now;
+        else
+            fail("Error reading the ctforces file; it should announce the start of linear ctforces, but instead "
+                 "read: %s\n", L[now].c_str());
+    }
     for (int i = 0; i < n_lin; ++i) {
         auto w = split_ws(L[now + i]);
         if (w.size() != 7) fail("Invalid line in the ctforces file:\n \t %s\n", L[now + i].c_str());
@@ -492,6 +500,31 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
             if (k < 0 || k >= B.nt) fail("constant force on an out of bounds node index %d", k);
             B.ctf_node.push_back(k);
             B.ctf_force.insert(B.ctf_force.end(), {fx, fy, fz});
+        }
+    }
+    now += n_lin;
+    if (n_surf > 0) {  // Blob.cpp:2730-2806
+        if (now >= L.size() || L[now].compare(0, 22, "linear surface forces:") != 0)
+            fail("Wrong header; it should announce the start of the linear surface ctforces, but instead read: %s\n",
+                 now < L.size() ? L[now].c_str() : "");
+        ++now;
+        for (int i = 0; i < n_surf; ++i) {
+            auto w = split_ws(L[now + i]);
+            if (w.size() < 7) fail("Invalid line in the ctforces file:\n \t %s\n", L[now + i].c_str());
+            if (atoi(w[4].c_str()) != blob_index || atoi(w[5].c_str()) != 0) continue;
+            if (w[6] == "all") fail("linear surface forces on 'all' faces: the reference's loader throws here");
+            const double F = atof(w[0].c_str());
+            double d[3] = {atof(w[1].c_str()), atof(w[2].c_str()), atof(w[3].c_str())};
+            const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            for (double &x : d) x /= nrm;
+            B.sf_force.insert(B.sf_force.end(), {F * d[0] / meso::force, F * d[1] / meso::force, F * d[2] / meso::force});
+            for (size_t j = 6; j < w.size(); ++j) {
+                const int face = atoi(w[j].c_str());
+                if (face < 0 || face >= B.num_surface_faces)
+                    fail("linear surface force on an out of bounds face index %d", face);
+                B.sf_faces.push_back(face);
+            }
+            B.sf_ptr.push_back((int)B.sf_faces.size());
         }
     }
 }
@@ -748,6 +781,23 @@ int ffea_script_blob_arrays(const ffea_script *s, int b, double *pos, int *top, 
     return FFEA_GPU_OK;
 }
 
+int ffea_script_blob_surface_forces(const ffea_script *s, int b, int *num_faces, int *num_sets, int *num_entries,
+                                    int *face_nodes, double *set_force, int *set_ptr, int *set_faces) {
+    if (!s || b < 0 || b >= (int)s->blobs.size()) return FFEA_GPU_ERR_INVALID;
+    const BlobData &B = s->blobs[b];
+    if (num_faces) *num_faces = B.num_surface_faces;
+    if (num_sets) *num_sets = (int)B.sf_ptr.size() - 1;
+    if (num_entries) *num_entries = (int)B.sf_faces.size();
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    cp(face_nodes, B.surf_nodes);
+    cp(set_force, B.sf_force);
+    cp(set_ptr, B.sf_ptr);
+    cp(set_faces, B.sf_faces);
+    return FFEA_GPU_OK;
+}
+
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]) {
     if (!s || !state) return FFEA_GPU_ERR_INVALID;
     std::copy(s->rng_state, s->rng_state + 6, state);
@@ -783,11 +833,22 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
         drag.resize(B.nt);
         for (int i = 0; i < B.nt; ++i) drag[i] = 6.0 * pi * p.stokes_visc * B.stokes_radius[i];
     }
-    int rc = ffea_gpu_blob_create(&gp, B.nt, B.ne, B.pos.data(), B.top.data(), B.A.data(), B.B.data(), B.G.data(),
-                                B.E.data(), (int)B.pinned.size(), B.pinned.empty() ? nullptr : B.pinned.data(),
-                                (int)B.ctf_node.size(), B.ctf_node.empty() ? nullptr : B.ctf_node.data(),
-                                B.ctf_force.empty() ? nullptr : B.ctf_force.data(),
-                                p.calc_stokes ? drag.data() : nullptr, s->rng_state, out);
+    ffea_gpu_surface_forces sf{};
+    const int nsets = (int)B.sf_ptr.size() - 1;
+    if (nsets > 0) {
+        sf.num_faces = B.num_surface_faces;
+        sf.face_nodes = B.surf_nodes.data();
+        sf.num_sets = nsets;
+        sf.set_force = B.sf_force.data();
+        sf.set_ptr = B.sf_ptr.data();
+        sf.set_faces = B.sf_faces.data();
+    }
+    int rc = ffea_gpu_blob_create_ex(&gp, B.nt, B.ne, B.pos.data(), B.top.data(), B.A.data(), B.B.data(), B.G.data(),
+                                     B.E.data(), (int)B.pinned.size(), B.pinned.empty() ? nullptr : B.pinned.data(),
+                                     (int)B.ctf_node.size(), B.ctf_node.empty() ? nullptr : B.ctf_node.data(),
+                                     B.ctf_force.empty() ? nullptr : B.ctf_force.data(),
+                                     p.calc_stokes ? drag.data() : nullptr, s->rng_state, nsets > 0 ? &sf : nullptr,
+                                     out);
     if (rc != FFEA_GPU_OK || s->blobs.size() == 1) return rc;
     // several blobs draw from the one stream in turn (World.cpp:2031-2058): all element
     // draws first, blob after blob, then (Stokes noise) all node draws, blob after blob

@@ -559,6 +559,66 @@ struct RowArgs {
     int *num_done;         // blobs whose solve stopped
 };
 
+// ---------------------------------------------------------------- surface constant forces
+// Blob::apply_ctforces CTF 3 (Blob.cpp:1721-1740) for a single blob, one workgroup: each
+// set's face areas at the step's positions (Face::get_area, Face.cpp:169-187; a midside
+// node sits at .5 (a + b) of its edge ends, tetra_element_linear.cpp:553-577), their sum in
+// face order, and per face the sum over its set entries (set order) of
+// (area / (3 total)) F: the force Face::add_force_to_node puts on each of the face's nodes.
+struct SurfArgs {
+    int nfaces, nsets, nent;
+    const int *rows;        // [faces][3][2] corner rows of each face node
+    const double *pos;      // [rows][3]
+    const double *force;    // [sets][3]
+    const int *set_ptr;     // [sets+1]
+    const int *entry_face, *entry_set;
+    const int *fe_ptr, *fe; // per face: its entries
+    double *area, *total;   // scratch [entries], [sets]
+    double *face_force;     // [faces][3]
+};
+__global__ __launch_bounds__(kRowBlock) void k_surface_forces(SurfArgs a) {
+#pragma clang fp contract(off)
+    for (int e = threadIdx.x; e < a.nent; e += kRowBlock) {
+        const int *r = a.rows + 6 * a.entry_face[e];
+        double p[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) p[i][c] = .5 * (a.pos[3 * r[2 * i] + c] + a.pos[3 * r[2 * i + 1] + c]);
+        double u[3], v[3], t[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            u[c] = p[1][c] - p[0][c];
+            v[c] = p[2][c] - p[0][c];
+        }
+        t[0] = u[1] * v[2] - u[2] * v[1];
+        t[1] = u[2] * v[0] - u[0] * v[2];
+        t[2] = u[0] * v[1] - u[1] * v[0];
+        double m = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m += t[c] * t[c];
+        a.area[e] = .5 * sqrt(m);
+    }
+    __syncthreads();
+    for (int st = threadIdx.x; st < a.nsets; st += kRowBlock) {  // serial sum per set, as the reference
+        double tot = 0.0;
+        for (int k = a.set_ptr[st]; k < a.set_ptr[st + 1]; ++k) tot += a.area[k];
+        a.total[st] = tot;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < a.nfaces; f += kRowBlock) {
+        double ff[3] = {0.0, 0.0, 0.0};
+        for (int k = a.fe_ptr[f]; k < a.fe_ptr[f + 1]; ++k) {
+            const int e = a.fe[k], st = a.entry_set[e];
+            const double sc = a.area[e] / (3 * a.total[st]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ff[c] += sc * a.force[3 * st + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.face_force[3 * f + c] = ff[c];
+    }
+}
+
 // ---------------------------------------------------------------- gather + CG start
 template <typename T>
 struct GatherArgs {
@@ -571,8 +631,11 @@ struct GatherArgs {
     const double *fold;         // [..][3]: linearise_force increments in element order (static)
     const double *noise;        // [nodes][3] Stokes noise subtrahends, or null
     const int *row_node;        // [nrows] node (noise index) of each corner row (with noise)
-    const int3 *fold_mid;       // [..]: midside nodes folded onto the row per record (with noise)
-    const double *node_ctf;     // [nodes][3] constant forces per node (with noise), or null
+    const int3 *fold_mid;       // [..]: midside nodes folded onto the row per record (per-step folds), or null
+    const double *node_ctf;     // [nodes][3] constant forces per node (per-step folds), or null
+    const int *sf_nf_ptr;       // [nodes+1] surface constant forces: faces holding each node, or null
+    const int *sf_nf;           // [..] used faces, face order
+    const double *sf_face_force;// [faces][3] force on each of a face's nodes this step
     const unsigned char *pinned;
     const double *dinv;
     const double *pos;          // [nrows][3] (rigid modes)
@@ -610,13 +673,25 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
                 f[1] += (double)src[1];
                 f[2] += (double)src[2];
             }
-            if (a.noise) {
+            if (a.sf_nf_ptr) {  // surface forces after the element forces, face order (Blob.cpp:3160-3170)
+                const int g = a.row_node[n];
+                for (int k = a.sf_nf_ptr[g]; k < a.sf_nf_ptr[g + 1]; ++k) {
+                    const double *ff = a.sf_face_force + 3 * a.sf_nf[k];
+                    f[0] += ff[0];
+                    f[1] += ff[1];
+                    f[2] += ff[2];
+                }
+            }
+            if (a.fold_mid) {
                 // the node's own noise, then Blob::linearise_force (Blob.cpp:781-799): each
                 // record is one (element, corner) in element order; its midside nodes not
-                // folded by an earlier element contribute half of (constant force - noise)
-                const int g = a.row_node[n];
+                // folded by an earlier element contribute half of (constant force + surface
+                // forces - noise)
+                if (a.noise) {
+                    const int g = a.row_node[n];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) f[k] -= a.noise[3 * g + k];
+                    for (int k = 0; k < 3; ++k) f[k] -= a.noise[3 * g + k];
+                }
                 for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
                     const int3 mm = a.fold_mid[k];
                     const int m3[3] = {mm.x, mm.y, mm.z};
@@ -626,10 +701,16 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
                         double val[3] = {0.0, 0.0, 0.0};
                         if (m3[t] >= 0) {
 #pragma unroll
-                            for (int c = 0; c < 3; ++c) {
-                                const double cf = a.node_ctf ? a.node_ctf[3 * m3[t] + c] : 0.0;
-                                val[c] = cf - a.noise[3 * m3[t] + c];
-                            }
+                            for (int c = 0; c < 3; ++c) val[c] = a.node_ctf ? a.node_ctf[3 * m3[t] + c] : 0.0;
+                            if (a.sf_nf_ptr)
+                                for (int q = a.sf_nf_ptr[m3[t]]; q < a.sf_nf_ptr[m3[t] + 1]; ++q) {
+                                    const double *ff = a.sf_face_force + 3 * a.sf_nf[q];
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) val[c] += ff[c];
+                                }
+                            if (a.noise)
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) val[c] -= a.noise[3 * m3[t] + c];
                         }
 #pragma unroll
                         for (int c = 0; c < 3; ++c) s3[c] = t == 0 ? val[c] : s3[c] + val[c];

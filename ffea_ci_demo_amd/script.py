@@ -32,6 +32,7 @@ def _bind(L):
     L.ffea_script_blob_sizes.argtypes = [vp, C.c_int, ip, ip, ip, ip, ip]
     L.ffea_script_blob_arrays.argtypes = [vp, C.c_int] + [vp] * 11
     L.ffea_script_rng_state.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
+    L.ffea_script_blob_surface_forces.argtypes = [vp, C.c_int, ip, ip, ip, vp, vp, vp, vp]
     L.ffea_gpu_blob_create_from_script.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L._script_bound = True
     return L
@@ -88,6 +89,24 @@ class Script:
         a["num_surface_faces"] = nf.value
         return a
 
+    def surface_forces(self, b: int = 0):
+        """blob b's linear surface constant forces as (face_nodes, set_force, set_ptr,
+        set_faces), or None"""
+        L = _native.lib()
+        nf, ns, nent = C.c_int(), C.c_int(), C.c_int()
+        if L.ffea_script_blob_surface_forces(self.h, b, C.byref(nf), C.byref(ns), C.byref(nent), None, None, None,
+                                             None) != _native.OK:
+            raise ScriptError(f"no blob {b}")
+        if ns.value <= 0:
+            return None
+        fn = np.zeros((nf.value, 3), np.int32)
+        sf = np.zeros((ns.value, 3))
+        sp = np.zeros(ns.value + 1, np.int32)
+        sfc = np.zeros(max(1, nent.value), np.int32)
+        L.ffea_script_blob_surface_forces(self.h, b, None, None, None, fn.ctypes.data, sf.ctypes.data, sp.ctypes.data,
+                                          sfc.ctypes.data)
+        return fn, sf, sp, sfc[:nent.value]
+
     def blob_spec(self, b: int = 0) -> BlobSpec:
         a = self.blob_arrays(b)
         p = self.params()
@@ -97,7 +116,8 @@ class Script:
         return BlobSpec(pos=a["pos"], top=a["top"], A=a["A"], B=a["B"], G=a["G"], E=a["E"], dt=p["dt"], kT=p["kT"],
                         epsilon2=p["epsilon2"], max_iterations_cg=p["max_iterations_cg"], calc_noise=p["calc_noise"],
                         rng_state=self.rng_state(), pinned=a["pinned"], ctf_node=a["ctf_node"],
-                        ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path)
+                        ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path,
+                        surface_forces=self.surface_forces(b))
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

@@ -40,6 +40,8 @@ class BlobSpec:
     stokes_drag: Optional[np.ndarray] = None
     mesh: Optional[meshgen.Mesh] = None
     name: str = ""
+    # linear surface constant forces: (face_nodes [nf][3], set_force [ns][3] meso, set_ptr, set_faces)
+    surface_forces: Optional[tuple] = None
 
     @property
     def num_elements(self):
@@ -53,7 +55,7 @@ class BlobSpec:
         return dict(dt=self.dt, kT=self.kT, epsilon2=self.epsilon2, max_iterations_cg=self.max_iterations_cg,
                     calc_noise=self.calc_noise, calc_stokes=self.calc_stokes, pinned=self.pinned,
                     ctf_node=self.ctf_node, ctf_force=self.ctf_force, stokes_drag=self.stokes_drag,
-                    rng_state=self.rng_state)
+                    rng_state=self.rng_state, surface_forces=self.surface_forces)
 
 
 def default_rng_state(seed: int) -> np.ndarray:
@@ -116,6 +118,21 @@ def config3(n: int = 50, *, moduli_seed: int = 11, sigma: float = 0.5, **kw) -> 
     spec.ctf_node = top_lin
     spec.ctf_force = np.tile(np.array([0.0, 0.0, units.force_to_meso(1e-12)]), (top_lin.size, 1))
     return spec
+
+
+def surface_force_sets(spec: BlobSpec, sets) -> tuple:
+    """Linear surface constant forces on a generated cube blob: `sets` is a list of
+    (force in N (3,), face predicate on the faces' node centroids (F,3) in meso units ->
+    bool (F,)); faces are the .surf sub-triangles in file order."""
+    faces = spec.mesh.surf_faces[:, 1:4].astype(np.int32)
+    cen = spec.pos[faces].mean(axis=1)
+    forces, ptr, idx = [], [0], []
+    for f, pred in sets:
+        sel = np.nonzero(pred(cen))[0]
+        idx.extend(sel.tolist())
+        ptr.append(len(idx))
+        forces.append(np.asarray(f, np.float64) / units.FORCE)
+    return (faces, np.array(forces), np.array(ptr, np.int32), np.array(idx, np.int32))
 
 
 def config4(n: int = 40, **kw) -> BlobSpec:

@@ -100,6 +100,31 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out);
 
+/* Linear surface constant forces of a blob (Blob::load_ctforces "linear surface forces",
+ * Blob.cpp:2730-2806; applied by Blob::apply_ctforces, Blob.cpp:1721-1740): set s spreads
+ * set_force[s] over its faces in proportion to their areas at the step's positions, a
+ * third of each face's share on each of its three nodes, aggregated after the element
+ * forces in face order (Blob.cpp:3160-3170); midside shares are folded by linearise_force.
+ *   face_nodes  [num_faces][3] node indices of the .surf faces, in file order (Face::n[0..2])
+ *   set_force   [num_sets][3] F * unit direction / force unit (meso units)
+ *   set_ptr     [num_sets+1], set_faces [set_ptr[num_sets]]: each set's faces, in order */
+typedef struct {
+    int num_faces;
+    const int *face_nodes;
+    int num_sets;
+    const double *set_force;
+    const int *set_ptr;
+    const int *set_faces;
+} ffea_gpu_surface_forces;
+
+/* ffea_gpu_blob_create with surface constant forces (surface_forces may be NULL: none) */
+int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements,
+                            const double *pos, const int *top, const double *A, const double *B,
+                            const double *G, const double *E, int num_pinned, const int *pinned,
+                            int num_ctf, const int *ctf_node, const double *ctf_force,
+                            const double *stokes_drag, const uint32_t rng_state[6],
+                            const ffea_gpu_surface_forces *surface_forces, ffea_gpu_blob **out);
+
 /* An ensemble of independent blobs in one handle (no counterpart in the reference, which
  * steps the blobs of a World one after another, World.cpp:2031-2059). Blob b is given as
  * for ffea_gpu_blob_create by element b of each array: num_nodes[b], pos[b], top[b], ...;

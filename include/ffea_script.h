@@ -55,6 +55,12 @@ int ffea_script_blob_arrays(const ffea_script *s, int blob, double *pos, int *to
                             double *G, double *E, double *rho, int *pinned, int *ctf_node, double *ctf_force,
                             double *stokes_radius);
 
+/* blob b's linear surface constant forces (.ctforces "linear surface forces", meso units),
+ * as ffea_gpu_surface_forces lays them out: counts first, then the arrays (any pointer
+ * may be NULL) */
+int ffea_script_blob_surface_forces(const ffea_script *s, int blob, int *num_faces, int *num_sets, int *num_entries,
+                                    int *face_nodes, double *set_force, int *set_ptr, int *set_faces);
+
 /* the noise stream state of the single-thread run (rng[0] after SetPackageSeed) */
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]);
 

@@ -461,6 +461,12 @@ struct orc_blob {
     double *r, *pv, *z, *q, *f;
     int last_iters;
     int all_pinned;
+    /* linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806) */
+    int nsf_faces, nsf_sets;
+    int *sf_face_nodes;        /* [nsf_faces][3] Face::n[0..2] */
+    double *sf_set_force;      /* [nsf_sets][3] */
+    int *sf_set_ptr, *sf_set_faces;
+    double *sf_face_force;     /* [nsf_faces][3] Face::force[0..2] of this step */
 };
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
@@ -712,17 +718,21 @@ void orc_blob_destroy(orc_blob *b) {
                    b->val, b->inv_diag, b->r, b->pv, b->z, b->q, b->f};
     for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); ++i) free(d[i]);
     free(b->top); free(b->is_pinned); free(b->pinned); free(b->ctf_node);
+    free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
 }
 
+static void surface_ctforces(orc_blob *b);
+
 /* Blob::update_internal_forces (Blob.cpp:400-491), single thread (rng[0]) */
 int orc_blob_update_internal_forces(orc_blob *b) {
     if (b->all_pinned) return 0;
-    /* apply_ctforces linear part (Blob.cpp:1679-1683) */
+    /* apply_ctforces linear part (Blob.cpp:1679-1683), surface part (:1721-1740) */
     for (int i = 0; i < b->nctf; ++i)
         for (int c = 0; c < 3; ++c) b->force[3 * b->ctf_node[i] + c] += b->ctf_force[3 * i + c];
+    if (b->nsf_sets) surface_ctforces(b);
     int inversions = 0;
     for (int e = 0; e < b->ne; ++e) {
         double J[3][3], s[3][3] = {{0}}, F[3][3], Ji0[3][3];
@@ -846,6 +856,69 @@ static void linearise_force(orc_blob *b) {
     }
 }
 
+int orc_blob_set_surface_forces(orc_blob *b, int nfaces, const int *face_nodes, int nsets, const double *set_force,
+                                const int *set_ptr, const int *set_faces) {
+    if (nfaces < 0 || nsets < 0 || (nsets && (!set_ptr || set_ptr[0] != 0))) return 1;
+    for (int i = 0; i < 3 * nfaces; ++i)
+        if (face_nodes[i] < 0 || face_nodes[i] >= b->nt) return 1;
+    for (int s = 0; s < nsets; ++s)
+        if (set_ptr[s + 1] < set_ptr[s]) return 1;
+    for (int k = 0; nsets && k < set_ptr[nsets]; ++k)
+        if (set_faces[k] < 0 || set_faces[k] >= nfaces) return 1;
+    const int ne = nsets ? set_ptr[nsets] : 0;
+    free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
+    b->nsf_faces = nfaces;
+    b->nsf_sets = nsets;
+    b->sf_face_nodes = xcalloc((size_t)3 * nfaces, sizeof(int));
+    memcpy(b->sf_face_nodes, face_nodes, sizeof(int) * 3 * (size_t)nfaces);
+    b->sf_set_force = xcalloc((size_t)3 * nsets, sizeof(double));
+    memcpy(b->sf_set_force, set_force, sizeof(double) * 3 * (size_t)nsets);
+    b->sf_set_ptr = xcalloc((size_t)nsets + 1, sizeof(int));
+    if (nsets) memcpy(b->sf_set_ptr, set_ptr, sizeof(int) * ((size_t)nsets + 1));
+    b->sf_set_faces = xcalloc((size_t)ne, sizeof(int));
+    memcpy(b->sf_set_faces, set_faces, sizeof(int) * (size_t)ne);
+    b->sf_face_force = xcalloc((size_t)3 * nfaces, sizeof(double));
+    return 0;
+}
+
+/* Face::get_area (Face.cpp:169-187) */
+static double face_area(const orc_blob *b, int f) {
+    const int *n = b->sf_face_nodes + 3 * f;
+    double a[3], c[3], t[3];
+    for (int i = 0; i < 3; ++i) {
+        a[i] = b->pos[3 * n[1] + i] - b->pos[3 * n[0] + i];
+        c[i] = b->pos[3 * n[2] + i] - b->pos[3 * n[0] + i];
+    }
+    t[0] = a[1] * c[2] - a[2] * c[1];
+    t[1] = a[2] * c[0] - a[0] * c[2];
+    t[2] = a[0] * c[1] - a[1] * c[0];
+    double m = 0.0;
+    for (int i = 0; i < 3; ++i) m += t[i] * t[i];
+    return .5 * sqrt(m);
+}
+
+/* Blob::apply_ctforces CTF 3 (Blob.cpp:1721-1740): each set's force is spread over its
+ * faces by area, a third of each face's share on each of the face's three nodes
+ * (Face::add_force_to_node, Face.cpp:238-241), at the positions of the step */
+static void surface_ctforces(orc_blob *b) {
+    memset(b->sf_face_force, 0, sizeof(double) * 3 * (size_t)b->nsf_faces);
+    for (int s = 0; s < b->nsf_sets; ++s) {
+        const int k0 = b->sf_set_ptr[s], k1 = b->sf_set_ptr[s + 1];
+        double total = 0;
+        double *areas = xcalloc((size_t)(k1 - k0), sizeof(double));
+        for (int k = k0; k < k1; ++k) {
+            areas[k - k0] = face_area(b, b->sf_set_faces[k]);
+            total += areas[k - k0];
+        }
+        for (int k = k0; k < k1; ++k) {
+            const double sc = areas[k - k0] / (3 * total);
+            double *ff = b->sf_face_force + 3 * b->sf_set_faces[k];
+            for (int c = 0; c < 3; ++c) ff[c] += sc * b->sf_set_force[3 * s + c];
+        }
+        free(areas);
+    }
+}
+
 /* Blob::update_positions -> aggregate_forces_and_solve (Blob.cpp:3147-3244),
  * euler_integrate (:3249-3263), linearise_element loop (:505-507) */
 int orc_blob_update_positions(orc_blob *b) {
@@ -857,6 +930,13 @@ int orc_blob_update_positions(orc_blob *b) {
             F[3 * n] += src[0];
             F[3 * n + 1] += src[1];
             F[3 * n + 2] += src[2];
+        }
+    /* surface forces onto nodes, face order (Blob.cpp:3160-3170; the opposite node, i = 3,
+     * receives no constant force) */
+    for (int f = 0; f < b->nsf_faces; ++f)
+        for (int i = 0; i < 3; ++i) {
+            const int n = b->sf_face_nodes[3 * f + i];
+            for (int c = 0; c < 3; ++c) F[3 * n + c] += b->sf_face_force[3 * f + c];
         }
     if (b->p.calc_stokes == 1 && b->p.calc_noise == 1) {
         for (int i = 0; i < nt; ++i) {

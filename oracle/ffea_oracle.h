@@ -76,6 +76,12 @@ orc_blob *orc_blob_create(int nt, int ne, const double *pos, const int *top,
                           const orc_params *params, const uint32_t rng_state[6]);
 void orc_blob_destroy(orc_blob *b);
 
+/* linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806): faces as
+ * [nfaces][3] node indices (.surf order, Face::n[0..2]); set s applies set_force[s]
+ * (meso units) over faces set_faces[set_ptr[s] .. set_ptr[s+1]). Returns 0 or 1 (bad input). */
+int orc_blob_set_surface_forces(orc_blob *b, int nfaces, const int *face_nodes, int nsets, const double *set_force,
+                                const int *set_ptr, const int *set_faces);
+
 /* Blob.cpp:400-491 ; returns the number of inverted elements */
 int  orc_blob_update_internal_forces(orc_blob *b);
 /* Blob.cpp:493-508 ; returns 0, or 1 when CG does not converge */

@@ -80,6 +80,8 @@ def lib():
                                       C.c_int, _ip, _dp, C.c_void_p, P(OrcParams), _up]
         L.orc_blob_create.restype = C.c_void_p
         L.orc_blob_destroy.argtypes = [C.c_void_p]
+        L.orc_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _dp, _ip, _ip]
+        L.orc_blob_set_surface_forces.restype = C.c_int
         for f in ("update_internal_forces", "update_positions", "step", "num_nodes", "num_elements",
                   "last_cg_iterations", "matrix_nnz"):
             getattr(L, "orc_blob_" + f).argtypes = [C.c_void_p]
@@ -197,7 +199,10 @@ class Blob:
     """The oracle's blob, created from arrays in meso units."""
 
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=10000, calc_noise=1,
-                 calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None):
+                 calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
+                 surface_forces=None):
+        """surface_forces: None or (face_nodes [nf][3], set_force [ns][3] meso, set_ptr [ns+1],
+        set_faces), the linear surface constant forces of Blob::load_ctforces"""
         L = lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
@@ -235,6 +240,16 @@ class Blob:
         self.nt = pos.shape[0]
         self.ne = ne
         self._top = top
+        if surface_forces is not None:
+            fn, sf, sp, sfc = surface_forces
+            fn = np.ascontiguousarray(fn, np.int32).reshape(-1)
+            sf = np.ascontiguousarray(sf, np.float64).reshape(-1)
+            sp = np.ascontiguousarray(sp, np.int32)
+            sfc = np.ascontiguousarray(sfc, np.int32)
+            if L.orc_blob_set_surface_forces(self.h, fn.size // 3, fn if fn.size else np.zeros(3, np.int32),
+                                             sp.size - 1, sf if sf.size else np.zeros(3), sp,
+                                             sfc if sfc.size else np.zeros(1, np.int32)) != 0:
+                raise ValueError("oracle surface forces rejected")
 
     def __del__(self):
         if getattr(self, "h", None):
