@@ -274,6 +274,16 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
     _w(path, p)
 
 
+def write_surface_ctforces(path, sets, blob_index=0):
+    """A .ctforces file of linear surface forces: rows (F [N], direction x y z, face
+    indices of the .surf file), the layout Blob::load_ctforces reads (Blob.cpp:2730-2806)."""
+    lines = ["ffea ctforces file", f"num_ctforces {len(sets)}", f"num_linear_surface_forces {len(sets)}",
+             "linear surface forces:"]
+    for F, d, faces in sets:
+        lines.append(f"{F!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 " + " ".join(str(int(f)) for f in faces))
+    _w(path, lines)
+
+
 def write_ctforces(path, node_forces, blob_index=0):
     """A .ctforces file of linear node forces: rows (F [N], direction x y z, node)
     (format of docs/2-UserManual/7-FFEAFiles.md:210-272)."""

@@ -11,7 +11,8 @@
  *   .stokes .pin       Blob.cpp:2297-2358, 3024-3076
  *   .surf              Blob.cpp:2090-2170 (faces must be midside sub-triangles,
  *                      tetra_element_linear.cpp:629-671)
- *   .ctforces          Blob.cpp:2478-2640, linear node forces only
+ *   .ctforces          Blob.cpp:2478-2640, 2730-2806: linear node forces and linear surface
+ *                      forces (rotational forces are refused)
  *   rng_seed           World.cpp:259-295 (glibc srand/rand -> 6 seeds; stream 0)
  * Positions are placed as World::init leaves them: blob centroid moved to the
  * `centroid` tag if given (World.cpp:2748-2768), then the system centroid moved to

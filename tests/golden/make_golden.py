@@ -1,10 +1,15 @@
 """Writes the golden fixtures of tests/golden from the reference's own translation
-units built under oracle/_ref (RngStream.cpp). Run where /root/reference exists:
+units built under oracle/_ref (RngStream.cpp), and copies the data files of the
+reference's cylinder physics tests (tests/physics/cyl_160_fine mesh, gzipped; the end
+nodes calcE.py measures). Run where /root/reference exists:
 
     make -C oracle ref && python tests/golden/make_golden.py
 """
+import gzip
 import json
 import os
+import re
+import shutil
 import sys
 
 import numpy as np
@@ -31,5 +36,20 @@ def main():
         json.dump({"source": "reference RngStream.cpp via oracle/_ref", "streams": streams}, f, indent=1)
 
 
+def cylinder(ref="/root/reference/tests/physics"):
+    out = os.path.join(HERE, "cyl_160_fine")
+    os.makedirs(out, exist_ok=True)
+    for ext in ("node", "top", "surf", "pin"):
+        with open(os.path.join(ref, "cyl_160_fine", f"cyl_160_fine.{ext}"), "rb") as fi, \
+                open(os.path.join(out, f"cyl_160_fine.{ext}.gz"), "wb") as raw, \
+                gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as fo:
+            shutil.copyfileobj(fi, fo)
+    src = open(os.path.join(ref, "E_cyl_160_fine", "calcE.py")).read()
+    end = [int(x) for x in re.search(r"ENDNODES = \[([^\]]*)\]", src).group(1).split(",")]
+    with open(os.path.join(out, "end_nodes.json"), "w") as f:
+        json.dump({"source": "tests/physics/E_cyl_160_fine/calcE.py ENDNODES", "end_nodes": end}, f)
+
+
 if __name__ == "__main__":
     main()
+    cylinder()
