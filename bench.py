@@ -377,25 +377,32 @@ def main_adjoint(args):
         return float((d ** 2).sum()), 2.0 * d
 
     forms, grads = {}, {}
-    for form in ("fp64", "mixed"):
-        with GpuBlob.from_spec(spec, precision=form) as g:
+    form_list = [f for f in args.forms.split(",") if f]
+    if "fp64" not in form_list:
+        form_list.insert(0, "fp64")
+    for form in form_list:
+        prec, pc = split_form(form)
+        with GpuBlob.from_spec(spec, precision=prec) as g:
+            if pc == "multigrid":
+                g.set_preconditioner("multigrid")
             g.step(2)  # warm-up (kernels loaded, iteration count learned)
             g.set_positions(spec.pos)
             g.synchronize()
             t0 = time.perf_counter()
             g.run_checkpointed(K, interval)
             t_fwd = time.perf_counter() - t0
-            _, it_total0 = g.cg_iterations()
+            it_last = g.cg_iterations()[0]
             L, seed = loss(g.positions())
             t0 = time.perf_counter()
             dA, dB, dG, dE, _ = g.adjoint(seed)
             t_bwd = time.perf_counter() - t0
             grads[form] = {"A": dA, "B": dB, "G": dG, "E": dE}
             forms[form] = {"loss": L, "forward_ms_per_step": t_fwd / K * 1e3, "reverse_ms_per_step": t_bwd / K * 1e3,
-                           "gradient_element_steps_per_s": spec.num_elements * K / (t_fwd + t_bwd)}
+                           "gradient_element_steps_per_s": spec.num_elements * K / (t_fwd + t_bwd),
+                           "cg_iterations_last_forward_step": it_last}
             print(f"[bench] adjoint {form}: forward {t_fwd:.2f}s reverse {t_bwd:.2f}s loss {L:.6e}",
                   file=sys.stderr, flush=True)
-            if form == "fp64":
+            if prec == "fp64":
                 r = np.random.default_rng(7)
                 d = r.uniform(-1.0, 1.0, spec.num_elements)
                 h = 1e-4
@@ -408,17 +415,22 @@ def main_adjoint(args):
                 fd = (vals[0] - vals[1]) / (2 * h)
                 ad = float((dG * spec.G * d).sum())
                 forms[form]["fd_check_G_direction_rel_err"] = abs(fd - ad) / abs(fd)
-    for name in "ABGE":
-        a, b = grads["mixed"][name], grads["fp64"][name]
-        forms["mixed"][f"grad_{name}_rel_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    h = forms["fp64"]
+    for f in form_list:
+        if f == "fp64":
+            continue
+        forms[f]["loss_rel_err_vs_fp64"] = abs(forms[f]["loss"] - forms["fp64"]["loss"]) / abs(forms["fp64"]["loss"])
+        for name in "ABGE":
+            a, b = grads[f][name], grads["fp64"][name]
+            forms[f][f"grad_{name}_rel_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    head = args.headline or ("fp64+mg" if "fp64+mg" in forms else "fp64")
+    h = forms[head]
     out = _line(round(h["gradient_element_steps_per_s"], 1), "element-steps/s", round(h["forward_ms_per_step"] +
-                h["reverse_ms_per_step"], 3), K, 2, "f64",
+                h["reverse_ms_per_step"], 3), K, 2, "f64" if split_form(head)[0] == "fp64" else "f32+f64",
                 {"workload": f"differentiation blob: {n}^3 jittered cubes, {spec.num_elements} tets, log-normal G/E "
                              "(median 3.6e8/2.2e9 Pa, sigma 0.5), bottom face pinned, +1 pN z on top-face corner "
                              f"nodes, noise off; loss vs the positions of a second moduli seed after K={K} steps; "
                              f"adjoint with checkpoints every {interval} steps; value = (forward + reverse) "
-                             "element-steps/s of the FP64 gradient", "form": "fp64", "parallelism": "single GPU"},
+                             f"element-steps/s of the {head} gradient", "form": head, "parallelism": "single GPU"},
                 {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
                  for f, r in forms.items()})
     print(json.dumps(out), flush=True)

@@ -104,6 +104,13 @@ public:
                                     dL_dpos0 ? dL_dpos0->data() : nullptr));
     }
     void use_graph(bool on, int iterations = 0) { check(ffea_gpu_blob_use_graph(b_, on ? 1 : 0, iterations)); }
+    // FFEA_GPU_PRECOND_JACOBI (the reference's) or FFEA_GPU_PRECOND_MULTIGRID
+    ffea_gpu_multigrid_info set_preconditioner(int kind) {
+        check(ffea_gpu_blob_set_preconditioner(b_, kind));
+        ffea_gpu_multigrid_info mi{};
+        check(ffea_gpu_blob_get_multigrid_info(b_, &mi));
+        return mi;
+    }
     int autotune(int iterations = 20) {
         int chosen = 0;
         check(ffea_gpu_blob_autotune(b_, iterations, &chosen));
