@@ -59,6 +59,11 @@ class SurfaceForces(C.Structure):
                 ("set_force", C.c_void_p), ("set_ptr", C.c_void_p), ("set_faces", C.c_void_p)]
 
 
+class RotationalForces(C.Structure):
+    _fields_ = [("num", C.c_int), ("node", C.c_void_p), ("force", C.c_void_p), ("axis", C.c_void_p),
+                ("type", C.c_void_p)]
+
+
 class MultigridInfo(C.Structure):
     _fields_ = [("num_levels", C.c_int), ("rows", C.c_int * MG_MAX_LEVELS), ("block", C.c_int * MG_MAX_LEVELS),
                 ("blocks", C.c_longlong * MG_MAX_LEVELS), ("transfer_blocks", C.c_longlong * MG_MAX_LEVELS),
@@ -93,7 +98,7 @@ def lib():
                                        C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(vp)]
     L.ffea_gpu_blob_create_ex.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
                                           C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(SurfaceForces),
-                                          C.POINTER(vp)]
+                                          C.POINTER(RotationalForces), C.POINTER(vp)]
     L.ffea_gpu_blob_destroy.argtypes = [vp]
     L.ffea_gpu_blob_destroy.restype = None
     for f in ("update_internal_forces", "update_positions", "synchronize"):

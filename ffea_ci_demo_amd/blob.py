@@ -129,9 +129,11 @@ def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precis
 class GpuBlob(_Handle):
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=1000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
-                 surface_forces=None, precision="fp64", device=0):
+                 surface_forces=None, rotational_forces=None, precision="fp64", device=0):
         """surface_forces: None or (face_nodes [nf][3], set_force [ns][3] meso, set_ptr [ns+1],
-        set_faces): linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806)"""
+        set_faces): linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806);
+        rotational_forces: None or (node [n], force [n] meso, axis [n][6] meso point and unit
+        direction, type [n] of "pf" / "pt"): rotational constant forces (Blob.cpp:2650-2726)"""
         L = _native.lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
@@ -157,8 +159,20 @@ class GpuBlob(_Handle):
                     np.ascontiguousarray(sp, np.int32), np.ascontiguousarray(sfc, np.int32)]
             sf = _native.SurfaceForces(keep[0].size // 3, keep[0].ctypes.data, keep[2].size - 1, keep[1].ctypes.data,
                                        keep[2].ctypes.data, keep[3].ctypes.data)
+        rf = None
+        if rotational_forces is not None:
+            rn, rfo, rax, rty = rotational_forces
+            rkeep = [np.ascontiguousarray(rn, np.int32).ravel(), np.ascontiguousarray(rfo, np.float64).ravel(),
+                     np.ascontiguousarray(rax, np.float64).reshape(-1),
+                     np.frombuffer("".join(str(t) for t in rty).encode("ascii"), np.uint8).copy()]
+            n = rkeep[0].size
+            if rkeep[1].size != n or rkeep[2].size != 6 * n or rkeep[3].size != 2 * n:
+                raise FFEAException(_native.ERR_INVALID, "rotational forces: node, force, axis and type sizes differ")
+            rf = _native.RotationalForces(n, rkeep[0].ctypes.data, rkeep[1].ctypes.data, rkeep[2].ctypes.data,
+                                          rkeep[3].ctypes.data)
         rc = L.ffea_gpu_blob_create_ex(C.byref(p), pos.shape[0], ne, pos, top, *mats, npin, pin_p, nctf, cn_p, cf_p,
-                                       drag_p, st, None if sf is None else C.byref(sf), C.byref(h))
+                                       drag_p, st, None if sf is None else C.byref(sf),
+                                       None if rf is None else C.byref(rf), C.byref(h))
         if rc != _native.OK:
             raise FFEAException(rc, L.ffea_gpu_last_error().decode())
         self.h = h
@@ -172,7 +186,8 @@ class GpuBlob(_Handle):
                    epsilon2=spec.epsilon2, max_iterations_cg=spec.max_iterations_cg, calc_noise=spec.calc_noise,
                    calc_stokes=spec.calc_stokes, pinned=spec.pinned, ctf_node=spec.ctf_node,
                    ctf_force=spec.ctf_force, stokes_drag=spec.stokes_drag, rng_state=spec.rng_state,
-                   surface_forces=getattr(spec, "surface_forces", None), precision=precision, **kw)
+                   surface_forces=getattr(spec, "surface_forces", None),
+                   rotational_forces=getattr(spec, "rotational_forces", None), precision=precision, **kw)
 
     # -- the reference's per-step interface
     def update_internal_forces(self):

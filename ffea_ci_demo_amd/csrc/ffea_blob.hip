@@ -122,6 +122,7 @@ struct BlobInput {
     const uint32_t *rng = nullptr;
     const double *drag = nullptr;  // [nt] Stokes drag per node (calc_stokes), else null
     const ffea_gpu_surface_forces *sf = nullptr;  // surface constant forces, or null
+    const ffea_gpu_rotational_forces *rf = nullptr;  // rotational constant forces, or null
 };
 
 // closed-form rest state (Blob::calc_rest_state_info, Blob.cpp:3080-3143;
@@ -185,6 +186,16 @@ struct BlobPlan {
     std::vector<int> sf_entry_set;  // [entries]
     std::vector<int> sf_fe_ptr, sf_fe;  // per used face: its entries in set order
     std::vector<int> sf_nf_ptr, sf_nf;  // per node of the .node file: used faces holding it, face order
+    // rotational constant forces: per node that carries one (ascending node index)
+    bool has_rf = false;
+    std::vector<int> rf_node;       // [nrn] node
+    std::vector<int> rf_row;        // [nrn] its corner row, or -1 (midside)
+    std::vector<int> rf_ends;       // [nrn][2] corner rows whose midpoint is its position
+    std::vector<double> rf_lin;     // [nrn][3] its linear constant forces (list-order sum)
+    std::vector<int> rf_ptr;        // [nrn+1] its entries, list order
+    std::vector<double> rf_axis;    // [entries][6]
+    std::vector<double> rf_F;       // [entries]
+    std::vector<int> rf_torque;     // [entries] 1: 't'
     double centre[3] = {0, 0, 0};
     long long nblocks = 0;
     long long nslots() const { return sb.empty() ? 0 : sb.back(); }
@@ -474,8 +485,57 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
                 for (int i = 0; i < 3; ++i) P.sf_nf[fill[F.face_nodes[3 * used[u] + i]]++] = u;
         }
     }
+    // ---- rotational constant forces (Blob::load_ctforces, Blob.cpp:2650-2726)
+    if (in.rf && in.rf->num > 0) {
+        const ffea_gpu_rotational_forces &R = *in.rf;
+        if (!R.node || !R.force || !R.axis || !R.type) throw Invalid(who + "rotational constant forces: invalid argument");
+        std::vector<int> count(nt + 1, 0);
+        for (int i = 0; i < R.num; ++i) {
+            const int v = R.node[i];
+            if (v < 0 || v >= nt)
+                throw Invalid(who + fmt("rotational force %d references an out of bounds node index", i));
+            if (R.type[2 * i] != 'p')
+                throw Invalid(who + fmt("rotational force %d: only axes given by point and direction ('p') are "
+                                        "applied (the reference leaves 'n' axes unformed)", i));
+            if (R.type[2 * i + 1] != 'f' && R.type[2 * i + 1] != 't')
+                throw Invalid(who + fmt("rotational force %d: type must be 'f' or 't'", i));
+            if (P.lin_of[v] < 0 && P.mid_a[v] < 0)
+                throw Invalid(who + fmt("rotational force %d acts on node %d, which belongs to no element", i, v));
+            count[v + 1]++;
+        }
+        for (int v = 0; v < nt; ++v) count[v + 1] += count[v];
+        std::vector<int> order(R.num);
+        {
+            std::vector<int> fill(count.begin(), count.end() - 1);
+            for (int i = 0; i < R.num; ++i) order[fill[R.node[i]]++] = i;  // list order within a node
+        }
+        P.has_rf = true;
+        P.rf_ptr.push_back(0);
+        for (int v = 0; v < nt; ++v) {
+            if (count[v + 1] == count[v]) continue;
+            P.rf_node.push_back(v);
+            if (P.lin_of[v] >= 0) {
+                P.rf_row.push_back(P.lin_of[v]);
+                P.rf_ends.push_back(P.lin_of[v]);
+                P.rf_ends.push_back(P.lin_of[v]);
+            } else {
+                P.rf_row.push_back(-1);
+                P.rf_ends.push_back(P.lin_of[P.mid_a[v]]);
+                P.rf_ends.push_back(P.lin_of[P.mid_b[v]]);
+            }
+            for (int c = 0; c < 3; ++c) P.rf_lin.push_back(ctf_total[3 * v + c]);
+            for (int k = count[v]; k < count[v + 1]; ++k) {
+                const int i = order[k];
+                P.rf_axis.insert(P.rf_axis.end(), R.axis + 6 * (size_t)i, R.axis + 6 * (size_t)i + 6);
+                P.rf_F.push_back(R.force[i]);
+                P.rf_torque.push_back(R.type[2 * i + 1] == 't' ? 1 : 0);
+            }
+            P.rf_ptr.push_back((int)P.rf_F.size());
+        }
+        P.have_ctf = true;  // the per-step totals live in the constant-force tables
+    }
     P.fp.assign(nl + 1, 0);
-    if (stokes_noise || P.has_sf) {
+    if (stokes_noise || P.has_sf || P.has_rf) {
         // every node carries noise: the folds are formed each step from the records
         std::vector<std::vector<std::array<int, 3>>> rec(nl);
         std::vector<unsigned char> folded(nt, 0);
@@ -557,6 +617,10 @@ struct Tables {
     int sf_nfaces = 0, sf_nsets = 0;
     std::vector<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
     std::vector<double> sf_force;
+    // rotational constant forces (single-blob handles): per node that carries one
+    bool have_rf = false;
+    std::vector<int> rf_node, rf_row, rf_ends, rf_ptr, rf_torque;
+    std::vector<double> rf_lin, rf_axis, rf_F;
 };
 
 Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_params &prm) {
@@ -590,7 +654,9 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
     }
     for (int b = 0; b < nblob; ++b) t.have_sf |= plans[b].has_sf;
     if (t.have_sf && nblob > 1) throw Invalid("surface constant forces need a single-blob handle");
-    t.dyn_fold = t.stokes_noise || t.have_sf;
+    for (int b = 0; b < nblob; ++b) t.have_rf |= plans[b].has_rf;
+    if (t.have_rf && nblob > 1) throw Invalid("rotational constant forces need a single-blob handle");
+    t.dyn_fold = t.stokes_noise || t.have_sf || t.have_rf;
     // ---- concatenation offsets: every blob starts on a 64-row slice boundary
     t.meta.resize(nblob);
     long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0, nodes = 0;
@@ -707,6 +773,16 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
             t.sf_fe = std::move(P.sf_fe);
             t.sf_nf_ptr = std::move(P.sf_nf_ptr);
             t.sf_nf = std::move(P.sf_nf);
+        }
+        if (P.has_rf) {  // single blob: rows and nodes need no offsets
+            t.rf_node = std::move(P.rf_node);
+            t.rf_row = std::move(P.rf_row);
+            t.rf_ends = std::move(P.rf_ends);
+            t.rf_lin = std::move(P.rf_lin);
+            t.rf_ptr = std::move(P.rf_ptr);
+            t.rf_axis = std::move(P.rf_axis);
+            t.rf_F = std::move(P.rf_F);
+            t.rf_torque = std::move(P.rf_torque);
         }
         const BlobMeta &m = t.meta[b];
         const int ro = m.row_off, eo = m.elem_off;
@@ -918,6 +994,22 @@ void validate(const Tables &t) {
         for (size_t k = 0; k < t.sf_nf.size(); ++k)
             if (t.sf_nf[k] < 0 || t.sf_nf[k] >= nf) fail("surface node face", (long long)k);
     }
+    if (t.have_rf) {
+        const long long n = t.rf_node.size(), nn = t.nnodes;
+        if (t.nblob != 1 || n <= 0 || (long long)t.rf_row.size() != n || (long long)t.rf_ends.size() != 2 * n ||
+            (long long)t.rf_lin.size() != 3 * n || (long long)t.rf_ptr.size() != n + 1 || t.rf_ptr[0] != 0 ||
+            !t.dyn_fold || !t.have_ctf || (long long)t.ctf.size() != 3 * nrows || (long long)t.node_ctf.size() != 3 * nn)
+            fail("rotational force table size", n);
+        const long long nent = t.rf_ptr[n];
+        if ((long long)t.rf_axis.size() != 6 * nent || (long long)t.rf_F.size() != nent ||
+            (long long)t.rf_torque.size() != nent)
+            fail("rotational force entries", nent);
+        for (long long k = 0; k < n; ++k)
+            if (t.rf_node[k] < 0 || t.rf_node[k] >= nn || t.rf_row[k] < -1 || t.rf_row[k] >= nrows ||
+                t.rf_ends[2 * k] < 0 || t.rf_ends[2 * k] >= nrows || t.rf_ends[2 * k + 1] < 0 ||
+                t.rf_ends[2 * k + 1] >= nrows || t.rf_ptr[k + 1] < t.rf_ptr[k])
+                fail("rotational force node", k);
+    }
 }
 
 }  // namespace
@@ -960,12 +1052,17 @@ struct ffea_gpu_blob {
     DevBuf<int> node_blob, node_off, row_node;
     DevBuf<int3> fold_mid;
     DevBuf<mrg::Jump> node_jump, njump_group, njump_lane;
-    bool dyn_fold = false;  // midside folds formed per step (Stokes noise, surface forces)
+    bool dyn_fold = false;  // midside folds formed per step (Stokes noise, surface or rotational forces)
     // surface constant forces (single blob)
     bool have_sf = false;
     int sf_nfaces = 0, sf_nsets = 0, sf_nent = 0;
     DevBuf<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
     DevBuf<double> sf_force, sf_area, sf_total, sf_face_force;
+    // rotational constant forces (single blob)
+    bool have_rf = false;
+    int rf_n = 0;
+    DevBuf<int> rf_node, rf_row, rf_ends, rf_ptr, rf_torque;
+    DevBuf<double> rf_lin, rf_axis, rf_F;
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
     int max_slice_w = 1;  // widest slice: the assembly grid's second dimension
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
@@ -1265,6 +1362,18 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
         sf_total.alloc(sf_nsets, &dev_bytes);
         sf_face_force.alloc((size_t)3 * sf_nfaces, &dev_bytes); sf_face_force.zero(stream);
     }
+    have_rf = t.have_rf;
+    if (have_rf) {
+        rf_n = (int)t.rf_node.size();
+        rf_node.alloc(t.rf_node.size(), &dev_bytes); rf_node.upload(t.rf_node, stream);
+        rf_row.alloc(t.rf_row.size(), &dev_bytes); rf_row.upload(t.rf_row, stream);
+        rf_ends.alloc(t.rf_ends.size(), &dev_bytes); rf_ends.upload(t.rf_ends, stream);
+        rf_ptr.alloc(t.rf_ptr.size(), &dev_bytes); rf_ptr.upload(t.rf_ptr, stream);
+        rf_torque.alloc(t.rf_torque.size(), &dev_bytes); rf_torque.upload(t.rf_torque, stream);
+        rf_lin.alloc(t.rf_lin.size(), &dev_bytes); rf_lin.upload(t.rf_lin, stream);
+        rf_axis.alloc(t.rf_axis.size(), &dev_bytes); rf_axis.upload(t.rf_axis, stream);
+        rf_F.alloc(t.rf_F.size(), &dev_bytes); rf_F.upload(t.rf_F, stream);
+    }
     jump_group.alloc(t.jump_group.size(), &dev_bytes); jump_group.upload(t.jump_group, stream);
     jump_lane.alloc(t.jump_lane.size(), &dev_bytes); jump_lane.upload(t.jump_lane, stream);
     step_jump.alloc(t.step_jump.size(), &dev_bytes); step_jump.upload(t.step_jump, stream);
@@ -1310,7 +1419,9 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
                           (nblob > 1 && !node_blob.p))) ||
         (have_sf && (!sf_rows.p || !sf_force.p || !sf_set_ptr.p || !sf_entry_face.p || !sf_entry_set.p ||
                      !sf_fe_ptr.p || !sf_fe.p || !sf_nf_ptr.p || !sf_nf.p || !sf_area.p || !sf_total.p ||
-                     !sf_face_force.p)))
+                     !sf_face_force.p)) ||
+        (have_rf && (!rf_node.p || !rf_row.p || !rf_ends.p || !rf_ptr.p || !rf_torque.p || !rf_lin.p || !rf_axis.p ||
+                     !rf_F.p || !ctf.p || !node_ctf.p)))
         throw Invalid("internal error: a device buffer was not allocated");
     // row-kernel grid: one block per chunk up to 1024 blocks (tunable, result-neutral)
     set_row_blocks(1024);
@@ -1374,6 +1485,11 @@ int ffea_gpu_blob::check_status_after_forces() {
 
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_gather(const T *force) {
+    if (have_rf) {  // Blob::apply_ctforces CTF 1 + 2 at this step's positions
+        RotArgs ra{rf_n, rf_node.p, rf_row.p, rf_ends.p, rf_lin.p, rf_ptr.p, rf_axis.p, rf_F.p, rf_torque.p, pos.p,
+                   node_ctf.p, ctf.p};
+        hipLaunchKernelGGL(k_rotational_forces, dim3((rf_n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ra);
+    }
     if (have_sf) {  // Blob::apply_ctforces CTF 3 at this step's positions
         SurfArgs sa{sf_nfaces, sf_nsets, sf_nent, sf_rows.p, pos.p, sf_force.p, sf_set_ptr.p, sf_entry_face.p,
                     sf_entry_set.p, sf_fe_ptr.p, sf_fe.p, sf_area.p, sf_total.p, sf_face_force.p};
@@ -1925,6 +2041,9 @@ static void ensure(DevBuf<double> &b, size_t n, size_t *total) {
 int ffea_gpu_blob::forward_checkpointed(int nsteps, int interval) {
     if (nblob != 1 || prm.calc_noise || meta[0].all_pinned)
         throw Invalid("a checkpointed run needs a single blob, calc_noise = 0 and unpinned nodes");
+    if (have_sf || have_rf)
+        throw Invalid("a checkpointed run needs forces that do not depend on positions: the adjoint does not "
+                      "differentiate surface or rotational constant forces");
     const size_t vec = (size_t)3 * nrows;
     const int nck = (nsteps + interval - 1) / interval;
     ensure(ckpt, vec * std::max(1, nck), &dev_bytes);
@@ -2170,14 +2289,15 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out) {
     return ffea_gpu_blob_create_ex(params, num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf,
-                                   ctf_node, ctf_force, stokes_drag, rng_state, nullptr, out);
+                                   ctf_node, ctf_force, stokes_drag, rng_state, nullptr, nullptr, out);
 }
 
 int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements, const double *pos,
                             const int *top, const double *A, const double *B, const double *G, const double *E,
                             int num_pinned, const int *pinned, int num_ctf, const int *ctf_node,
                             const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
-                            const ffea_gpu_surface_forces *surface_forces, ffea_gpu_blob **out) {
+                            const ffea_gpu_surface_forces *surface_forces,
+                            const ffea_gpu_rotational_forces *rotational_forces, ffea_gpu_blob **out) {
     if (out) *out = nullptr;
     if (!params || !out) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
@@ -2185,7 +2305,8 @@ int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int nu
     }
     // the drag is used only with calc_stokes = 1 (NoMassCGSolver.cpp:119-127, Blob.cpp:3172)
     const BlobInput bi{num_nodes,      num_elements, pos,       top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
-                       ctf_force,      rng_state,    params->calc_stokes ? stokes_drag : nullptr, surface_forces};
+                       ctf_force,      rng_state,    params->calc_stokes ? stokes_drag : nullptr, surface_forces,
+                       rotational_forces};
     if (!valid_blob_input(bi)) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;

@@ -262,6 +262,10 @@ struct BlobData {
     std::vector<int> surf_nodes;        // [faces][3] (.surf order, Face::n[0..2])
     std::vector<double> sf_force;       // linear surface constant forces: [sets][3] meso
     std::vector<int> sf_ptr{0}, sf_faces;
+    std::vector<int> rf_node;           // rotational constant forces, list order ("all" expanded)
+    std::vector<double> rf_force;       // [n] meso
+    std::vector<double> rf_axis;        // [n][6] meso point, unit direction
+    std::vector<char> rf_type;          // [n][2]
     double scale = 1.0;  // meso per file unit
     bool set_centroid = false;
     double centroid[3] = {0, 0, 0};
@@ -474,7 +478,6 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
     if (n_all != n_lin + n_rot + n_surf || n_all == 0 || calc_lines != L.size())
         fail("--- ABORTING. Something went wrong when parsing the header of the ctforces file\n"
              "--- Check it or turn calc_ctforces to 0 in the FFEA input file");
-    if (n_rot > 0) fail("rotational constant forces are not supported by this build");
     if (n_lin > 0) {
         if (L[now].compare(0, 14, "linear forces:") == 0) ++now;
         else
@@ -488,7 +491,8 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
         const double F = atof(w[0].c_str());
         double d[3] = {atof(w[1].c_str()), atof(w[2].c_str()), atof(w[3].c_str())};
         const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        for (double &x : d) x /= nrm;  // normalize (mat_vec_fns_II.h)
+        if (nrm == 0) fail("Unable to normalize a vector with length zero");
+        for (double &x : d) x /= nrm;  // normalize (mat_vec_fns_II.h:188-198)
         const double fx = F * d[0] / meso::force, fy = F * d[1] / meso::force, fz = F * d[2] / meso::force;
         if (w[6] == "all") {
             for (int j = 0; j < B.nt; ++j) {
@@ -503,6 +507,47 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
         }
     }
     now += n_lin;
+    if (n_rot > 0) {  // Blob.cpp:2630-2726
+        if (now >= L.size() || L[now].compare(0, 18, "rotational forces:") != 0)
+            fail("Wrong header; it should announce the start of rotational ctforces, but instead read: %s\n",
+                 now < L.size() ? L[now].c_str() : "");
+        ++now;
+        const double mdfm1 = 1. / meso::force, mdlm1 = 1. / meso::length;
+        for (int i = 0; i < n_rot; ++i) {
+            auto w = split_ws(L[now + i]);
+            if (w.size() != 11) fail("Invalid line in the ctforces file:\n \t %s\n", L[now + i].c_str());
+            if (atoi(w[8].c_str()) != blob_index || atoi(w[9].c_str()) != 0) continue;
+            const double F = atof(w[0].c_str()) * mdfm1;
+            const std::string &type = w[1];
+            double pt[3] = {atof(w[2].c_str()), atof(w[3].c_str()), atof(w[4].c_str())};
+            double d[3] = {atof(w[5].c_str()), atof(w[6].c_str()), atof(w[7].c_str())};
+            if (type.size() < 2 || (type[1] != 'f' && type[1] != 't'))
+                fail("Invalid rotational force type: %s, in line read: %s\n", type.c_str(), L[now + i].c_str());
+            if (type[0] == 'n')
+                fail("rotational force with an axis given by two nodes ('n'): the reference's force loop never "
+                     "forms r for it (Blob.cpp:1697-1706); give the axis by point and direction ('p')");
+            if (type[0] != 'p')
+                fail("Invalid rotational force: %s, in line read: %s\n", type.c_str(), L[now + i].c_str());
+            const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (nrm == 0) fail("Unable to normalize a vector with length zero");
+            for (double &x : d) x /= nrm;
+            for (double &x : pt) x = mdlm1 * x;  // resize(mdlm1, ctf_p)
+            auto add = [&](int node) {
+                B.rf_node.push_back(node);
+                B.rf_force.push_back(F);
+                B.rf_axis.insert(B.rf_axis.end(), {pt[0], pt[1], pt[2], d[0], d[1], d[2]});
+                B.rf_type.insert(B.rf_type.end(), {type[0], type[1]});
+            };
+            if (w[10] == "all") {
+                for (int j = 0; j < B.nt; ++j) add(j);
+            } else {
+                const int k = atoi(w[10].c_str());
+                if (k < 0 || k >= B.nt) fail("rotational force on an out of bounds node index %d", k);
+                add(k);
+            }
+        }
+        now += n_rot;
+    }
     if (n_surf > 0) {  // Blob.cpp:2730-2806
         if (now >= L.size() || L[now].compare(0, 22, "linear surface forces:") != 0)
             fail("Wrong header; it should announce the start of the linear surface ctforces, but instead read: %s\n",
@@ -516,6 +561,7 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
             const double F = atof(w[0].c_str());
             double d[3] = {atof(w[1].c_str()), atof(w[2].c_str()), atof(w[3].c_str())};
             const double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (nrm == 0) fail("Unable to normalize a vector with length zero");
             for (double &x : d) x /= nrm;
             B.sf_force.insert(B.sf_force.end(), {F * d[0] / meso::force, F * d[1] / meso::force, F * d[2] / meso::force});
             for (size_t j = 6; j < w.size(); ++j) {
@@ -798,6 +844,21 @@ int ffea_script_blob_surface_forces(const ffea_script *s, int b, int *num_faces,
     return FFEA_GPU_OK;
 }
 
+int ffea_script_blob_rotational_forces(const ffea_script *s, int b, int *num, int *node, double *force, double *axis,
+                                       char *type) {
+    if (!s || b < 0 || b >= (int)s->blobs.size()) return FFEA_GPU_ERR_INVALID;
+    const BlobData &B = s->blobs[b];
+    if (num) *num = (int)B.rf_node.size();
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    cp(node, B.rf_node);
+    cp(force, B.rf_force);
+    cp(axis, B.rf_axis);
+    cp(type, B.rf_type);
+    return FFEA_GPU_OK;
+}
+
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]) {
     if (!s || !state) return FFEA_GPU_ERR_INVALID;
     std::copy(s->rng_state, s->rng_state + 6, state);
@@ -843,12 +904,14 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
         sf.set_ptr = B.sf_ptr.data();
         sf.set_faces = B.sf_faces.data();
     }
+    ffea_gpu_rotational_forces rf{(int)B.rf_node.size(), B.rf_node.data(), B.rf_force.data(), B.rf_axis.data(),
+                                  B.rf_type.data()};
     int rc = ffea_gpu_blob_create_ex(&gp, B.nt, B.ne, B.pos.data(), B.top.data(), B.A.data(), B.B.data(), B.G.data(),
                                      B.E.data(), (int)B.pinned.size(), B.pinned.empty() ? nullptr : B.pinned.data(),
                                      (int)B.ctf_node.size(), B.ctf_node.empty() ? nullptr : B.ctf_node.data(),
                                      B.ctf_force.empty() ? nullptr : B.ctf_force.data(),
                                      p.calc_stokes ? drag.data() : nullptr, s->rng_state, nsets > 0 ? &sf : nullptr,
-                                     out);
+                                     B.rf_node.empty() ? nullptr : &rf, out);
     if (rc != FFEA_GPU_OK || s->blobs.size() == 1) return rc;
     // several blobs draw from the one stream in turn (World.cpp:2031-2058): all element
     // draws first, blob after blob, then (Stokes noise) all node draws, blob after blob

@@ -619,6 +619,100 @@ __global__ __launch_bounds__(kRowBlock) void k_surface_forces(SurfArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- rotational constant forces
+// Blob::apply_ctforces CTF 1 and 2 (Blob.cpp:1679-1719) for the nodes that carry a
+// rotational constant force, one lane per node: the node's linear constant forces (summed in
+// list order on the host) plus, in list order, each entry's pull round its axis at the
+// node's position this step (a midside node sits at .5 (a + b) of its edge ends,
+// tetra_element_linear.cpp:553-577). The total goes to the node's entry of the per-node
+// table read by the midside folds and, for a corner node, to its row's constant force.
+struct RotArgs {
+    int n;
+    const int *node;        // [n] node
+    const int *row;         // [n] its corner row, or -1
+    const int *ends;        // [n][2] corner rows whose midpoint is its position
+    const double *lin;      // [n][3]
+    const int *ptr;         // [n+1] its entries
+    const double *axis;     // [entries][6] point, unit direction
+    const double *F;        // [entries]
+    const int *torque;      // [entries] 1: constant torque ('t'), 0: 'f'
+    const double *pos;      // [rows][3]
+    double *node_ctf;       // [nodes][3]
+    double *ctf;            // [rows][3]
+};
+
+__device__ inline void cross3(const double *u, const double *v, double *w) {
+#pragma clang fp contract(off)
+    w[0] = u[1] * v[2] - v[1] * u[2];
+    w[1] = -u[0] * v[2] + v[0] * u[2];
+    w[2] = u[0] * v[1] - v[0] * u[1];
+}
+
+// one entry (Blob.cpp:1693-1714): r runs from the node x to its foot on the axis, formed as
+// intersectingPointToLine does (mat_vec_fns_II.h:755-777: v2 = (x - a) x d; a node with
+// |v2| < 3 eps lies on the axis and gets r = 0; else foot = x + c (d x v2) with
+// c = det[(x - a); d; d x (d x v2)] / |d x (d x v2)|^2, mat_vec_fns_II.h:379-386); the
+// force is F (r x d), or (F / |r|) (r x d) for a constant torque
+__device__ inline void rotational_force(const double *x, const double *ax, double F, int torque, double *f) {
+#pragma clang fp contract(off)
+    const double *a = ax, *d = ax + 3;
+    double pa[3], v2[3], cp[3], w[3], r[3], g[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pa[c] = x[c] - a[c];
+    cross3(pa, d, v2);
+    double m = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m += v2[c] * v2[c];
+    if (sqrt(m) < 3.0 * __DBL_EPSILON__) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = x[c] - x[c];
+    } else {
+        cross3(d, v2, cp);
+        cross3(d, cp, w);
+        double det = pa[0] * (d[1] * w[2] - d[2] * w[1]);
+        det += pa[1] * (d[2] * w[0] - d[0] * w[2]);
+        det += pa[2] * (d[0] * w[1] - d[1] * w[0]);
+        const double cc = det / (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = (x[c] + cc * cp[c]) - x[c];
+    }
+    cross3(r, d, g);
+    double s = F;
+    if (torque) {
+        double rr = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rr += r[c] * r[c];
+        s = F / sqrt(rr);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = s * g[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rotational_forces(RotArgs a) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= a.n) return;
+    const int ra = a.ends[2 * k], rb = a.ends[2 * k + 1];
+    double x[3], f[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        x[c] = ra == rb ? a.pos[3 * ra + c] : .5 * (a.pos[3 * ra + c] + a.pos[3 * rb + c]);
+        f[c] = a.lin[3 * k + c];
+    }
+    for (int e = a.ptr[k]; e < a.ptr[k + 1]; ++e) {
+        double g[3];
+        rotational_force(x, a.axis + 6 * e, a.F[e], a.torque[e], g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[c] += g[c];
+    }
+    const int v = a.node[k], row = a.row[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.node_ctf[3 * v + c] = f[c];
+        if (row >= 0) a.ctf[3 * row + c] = f[c];
+    }
+}
+
 // ---------------------------------------------------------------- gather + CG start
 template <typename T>
 struct GatherArgs {

@@ -292,3 +292,27 @@ def write_ctforces(path, node_forces, blob_index=0):
     for F, d, node in node_forces:
         lines.append(f"{F!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 {node}")
     _w(path, lines)
+
+
+def write_full_ctforces(path, node_forces=(), rot_forces=(), surface_sets=(), blob_index=0):
+    """A .ctforces file with any of its three sections (Blob::load_ctforces,
+    Blob.cpp:2478-2806): linear node forces (F [N], direction, node or "all"), rotational
+    forces (F [N], type "pf"/"pt", axis point [m], axis direction, node or "all") and linear
+    surface forces (F [N], direction, face indices)."""
+    n = len(node_forces) + len(rot_forces) + len(surface_sets)
+    lines = ["ffea ctforces file", f"num_ctforces {n}", f"num_linear_forces {len(node_forces)}",
+             f"num_rot_forces {len(rot_forces)}", f"num_linear_surface_forces {len(surface_sets)}"]
+    if node_forces:
+        lines.append("linear forces:")
+        for F, d, node in node_forces:
+            lines.append(f"{F!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 {node}")
+    if rot_forces:
+        lines.append("rotational forces:")
+        for F, typ, pt, d, node in rot_forces:
+            lines.append(f"{F!r} {typ} {pt[0]!r} {pt[1]!r} {pt[2]!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 {node}")
+    if surface_sets:
+        lines.append("linear surface forces:")
+        for F, d, faces in surface_sets:
+            lines.append(f"{F!r} {d[0]!r} {d[1]!r} {d[2]!r} {blob_index} 0 " + " ".join(str(int(f)) for f in faces))
+    _w(path, lines)
+

@@ -33,6 +33,7 @@ def _bind(L):
     L.ffea_script_blob_arrays.argtypes = [vp, C.c_int] + [vp] * 11
     L.ffea_script_rng_state.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
     L.ffea_script_blob_surface_forces.argtypes = [vp, C.c_int, ip, ip, ip, vp, vp, vp, vp]
+    L.ffea_script_blob_rotational_forces.argtypes = [vp, C.c_int, ip, vp, vp, vp, vp]
     L.ffea_gpu_blob_create_from_script.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L._script_bound = True
     return L
@@ -107,6 +108,23 @@ class Script:
                                           sfc.ctypes.data)
         return fn, sf, sp, sfc[:nent.value]
 
+    def rotational_forces(self, b: int = 0):
+        """blob b's rotational constant forces as (node, force, axis, type), or None"""
+        L = _native.lib()
+        n = C.c_int()
+        if L.ffea_script_blob_rotational_forces(self.h, b, C.byref(n), None, None, None, None) != _native.OK:
+            raise ScriptError(f"no blob {b}")
+        if n.value <= 0:
+            return None
+        node = np.zeros(n.value, np.int32)
+        force = np.zeros(n.value)
+        axis = np.zeros((n.value, 6))
+        typ = C.create_string_buffer(2 * n.value)
+        L.ffea_script_blob_rotational_forces(self.h, b, None, node.ctypes.data, force.ctypes.data, axis.ctypes.data,
+                                             typ)
+        raw = typ.raw.decode("ascii")
+        return node, force, axis, [raw[2 * i:2 * i + 2] for i in range(n.value)]
+
     def blob_spec(self, b: int = 0) -> BlobSpec:
         a = self.blob_arrays(b)
         p = self.params()
@@ -117,7 +135,7 @@ class Script:
                         epsilon2=p["epsilon2"], max_iterations_cg=p["max_iterations_cg"], calc_noise=p["calc_noise"],
                         rng_state=self.rng_state(), pinned=a["pinned"], ctf_node=a["ctf_node"],
                         ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path,
-                        surface_forces=self.surface_forces(b))
+                        surface_forces=self.surface_forces(b), rotational_forces=self.rotational_forces(b))
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

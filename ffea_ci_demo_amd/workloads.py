@@ -42,6 +42,7 @@ class BlobSpec:
     name: str = ""
     # linear surface constant forces: (face_nodes [nf][3], set_force [ns][3] meso, set_ptr, set_faces)
     surface_forces: Optional[tuple] = None
+    rotational_forces: Optional[tuple] = None  # (node [n], force [n] meso, axis [n][6] meso, type [n] "pf"/"pt")
 
     @property
     def num_elements(self):
@@ -55,7 +56,8 @@ class BlobSpec:
         return dict(dt=self.dt, kT=self.kT, epsilon2=self.epsilon2, max_iterations_cg=self.max_iterations_cg,
                     calc_noise=self.calc_noise, calc_stokes=self.calc_stokes, pinned=self.pinned,
                     ctf_node=self.ctf_node, ctf_force=self.ctf_force, stokes_drag=self.stokes_drag,
-                    rng_state=self.rng_state, surface_forces=self.surface_forces)
+                    rng_state=self.rng_state, surface_forces=self.surface_forces,
+                    rotational_forces=self.rotational_forces)
 
 
 def default_rng_state(seed: int) -> np.ndarray:

@@ -117,13 +117,37 @@ typedef struct {
     const int *set_faces;
 } ffea_gpu_surface_forces;
 
-/* ffea_gpu_blob_create with surface constant forces (surface_forces may be NULL: none) */
+/* Rotational constant forces of a blob (Blob::load_ctforces "rotational forces",
+ * Blob.cpp:2650-2726: the members ctf_r_nodes, ctf_r_forces, ctf_r_axis, ctf_r_type;
+ * applied by Blob::apply_ctforces CTF 2, Blob.cpp:1684-1719, after the linear constant
+ * forces): entry i acts on node[i] at the step's positions. With r the vector from the
+ * node to its foot on the axis through axis[6i..6i+2] along axis[6i+3..6i+5], the node
+ * receives force[i] (r x d) for type 'f' or (force[i] / |r|) (r x d) for type 't'; midside
+ * shares are folded by linearise_force.
+ *   node   [num] node indices ("all" expanded to every node, in order)
+ *   force  [num] F / force unit
+ *   axis   [num][6] point / length unit, then the unit direction
+ *   type   [num][2] 'p' (axis by point and direction), then 'f' or 't'. The reference
+ *          loader also accepts 'n' (axis by two nodes) but its force loop never forms r
+ *          for it (Blob.cpp:1697-1706); such entries are refused here. */
+typedef struct {
+    int num;
+    const int *node;
+    const double *force;
+    const double *axis;
+    const char *type;
+} ffea_gpu_rotational_forces;
+
+/* ffea_gpu_blob_create with surface and rotational constant forces (either may be NULL:
+ * none). Position-dependent constant forces make a handle refuse
+ * ffea_gpu_blob_run_checkpointed / ffea_gpu_blob_adjoint. */
 int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements,
                             const double *pos, const int *top, const double *A, const double *B,
                             const double *G, const double *E, int num_pinned, const int *pinned,
                             int num_ctf, const int *ctf_node, const double *ctf_force,
                             const double *stokes_drag, const uint32_t rng_state[6],
-                            const ffea_gpu_surface_forces *surface_forces, ffea_gpu_blob **out);
+                            const ffea_gpu_surface_forces *surface_forces,
+                            const ffea_gpu_rotational_forces *rotational_forces, ffea_gpu_blob **out);
 
 /* An ensemble of independent blobs in one handle (no counterpart in the reference, which
  * steps the blobs of a World one after another, World.cpp:2031-2059). Blob b is given as
@@ -232,7 +256,7 @@ int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *blob, int index, const uint32_t
 int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
 
 /* Differentiation of a blob trajectory (single-blob handles, calc_noise = 0, some nodes
- * pinned). No counterpart in the reference.
+ * pinned, no surface or rotational constant forces). No counterpart in the reference.
  *   ffea_gpu_blob_set_material   replaces the per-element A, B, G, E (meso units, .top order)
  *   ffea_gpu_blob_run_checkpointed  nsteps steps of ffea_gpu_blob_step from the current
  *                                state, keeping the positions of every interval-th step on

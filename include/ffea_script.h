@@ -11,8 +11,9 @@
  *   .stokes .pin       Blob.cpp:2297-2358, 3024-3076
  *   .surf              Blob.cpp:2090-2170 (faces must be midside sub-triangles,
  *                      tetra_element_linear.cpp:629-671)
- *   .ctforces          Blob.cpp:2478-2640, 2730-2806: linear node forces and linear surface
- *                      forces (rotational forces are refused)
+ *   .ctforces          Blob.cpp:2478-2806: linear node forces, rotational forces (axes
+ *                      by point and direction; 'n' axes are refused) and linear surface
+ *                      forces
  *   rng_seed           World.cpp:259-295 (glibc srand/rand -> 6 seeds; stream 0)
  * Positions are placed as World::init leaves them: blob centroid moved to the
  * `centroid` tag if given (World.cpp:2748-2768), then the system centroid moved to
@@ -61,6 +62,12 @@ int ffea_script_blob_arrays(const ffea_script *s, int blob, double *pos, int *to
  * may be NULL) */
 int ffea_script_blob_surface_forces(const ffea_script *s, int blob, int *num_faces, int *num_sets, int *num_entries,
                                     int *face_nodes, double *set_force, int *set_ptr, int *set_faces);
+
+/* blob b's rotational constant forces (.ctforces "rotational forces", meso units), as
+ * ffea_gpu_rotational_forces lays them out: the count first, then the arrays (any pointer
+ * may be NULL; type holds 2 chars per entry) */
+int ffea_script_blob_rotational_forces(const ffea_script *s, int blob, int *num, int *node, double *force,
+                                       double *axis, char *type);
 
 /* the noise stream state of the single-thread run (rng[0] after SetPackageSeed) */
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]);

@@ -36,6 +36,7 @@
  */
 #include "ffea_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -467,6 +468,12 @@ struct orc_blob {
     double *sf_set_force;      /* [nsf_sets][3] */
     int *sf_set_ptr, *sf_set_faces;
     double *sf_face_force;     /* [nsf_faces][3] Face::force[0..2] of this step */
+    /* rotational constant forces (Blob::load_ctforces, Blob.cpp:2650-2726) */
+    int nrf;
+    int *rf_node;
+    double *rf_force;          /* [nrf] F / force unit */
+    double *rf_axis;           /* [nrf][6] point / length unit, unit direction */
+    int *rf_torque;            /* [nrf] 1: 't' (constant torque), 0: 'f' */
 };
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
@@ -719,19 +726,23 @@ void orc_blob_destroy(orc_blob *b) {
     for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); ++i) free(d[i]);
     free(b->top); free(b->is_pinned); free(b->pinned); free(b->ctf_node);
     free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
+    free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
 }
 
 static void surface_ctforces(orc_blob *b);
+static void rotational_ctforces(orc_blob *b);
 
 /* Blob::update_internal_forces (Blob.cpp:400-491), single thread (rng[0]) */
 int orc_blob_update_internal_forces(orc_blob *b) {
     if (b->all_pinned) return 0;
-    /* apply_ctforces linear part (Blob.cpp:1679-1683), surface part (:1721-1740) */
+    /* apply_ctforces linear part (Blob.cpp:1679-1683), rotational part (:1684-1719),
+     * surface part (:1721-1740) */
     for (int i = 0; i < b->nctf; ++i)
         for (int c = 0; c < 3; ++c) b->force[3 * b->ctf_node[i] + c] += b->ctf_force[3 * i + c];
+    if (b->nrf) rotational_ctforces(b);
     if (b->nsf_sets) surface_ctforces(b);
     int inversions = 0;
     for (int e = 0; e < b->ne; ++e) {
@@ -917,6 +928,76 @@ static void surface_ctforces(orc_blob *b) {
         }
         free(areas);
     }
+}
+
+/* u x v as arr3arr3VectorProduct (mat_vec_fns_II.h:369-373) */
+static void cross3(const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - v[1] * u[2];
+    w[1] = -u[0] * v[2] + v[0] * u[2];
+    w[2] = u[0] * v[1] - v[0] * u[1];
+}
+
+/* Blob::apply_ctforces CTF 2 (Blob.cpp:1684-1719), axes given by point and direction: r
+ * runs from the node to its foot on the axis (intersectingPointToLine,
+ * mat_vec_fns_II.h:755-777: foot = p + c (d x ((p - a) x d)), c from detByRows,
+ * mat_vec_fns_II.h:379-386; a node on the axis, |(p - a) x d| < 3 eps, gets r = 0), and
+ * the node receives F (r x d) ('f') or (F / |r|) (r x d) ('t'), in list order */
+void orc_rotational_force(const double *x, const double *axis, double F, int torque, double *f) {
+    const double three_eps = 3.0 * DBL_EPSILON;
+    const double *a = axis, *d = axis + 3;
+    double pa[3], v2[3], cp[3], v1v2[3], foot[3], r[3], w[3];
+    for (int c = 0; c < 3; ++c) pa[c] = x[c] - a[c];
+    cross3(pa, d, v2);
+    double m = 0.0;
+    for (int c = 0; c < 3; ++c) m += v2[c] * v2[c];
+    if (sqrt(m) < three_eps) {
+        for (int c = 0; c < 3; ++c) foot[c] = x[c];
+    } else {
+        cross3(d, v2, cp);
+        cross3(d, cp, v1v2);
+        double det = pa[0] * (d[1] * v1v2[2] - d[2] * v1v2[1]);
+        det += pa[1] * (d[2] * v1v2[0] - d[0] * v1v2[2]);
+        det += pa[2] * (d[0] * v1v2[1] - d[1] * v1v2[0]);
+        const double cc = det / (v1v2[0] * v1v2[0] + v1v2[1] * v1v2[1] + v1v2[2] * v1v2[2]);
+        for (int c = 0; c < 3; ++c) foot[c] = x[c] + cc * cp[c];
+    }
+    for (int c = 0; c < 3; ++c) r[c] = foot[c] - x[c];
+    cross3(r, d, w);
+    double s = F;
+    if (torque) {
+        double rr = 0.0;
+        for (int c = 0; c < 3; ++c) rr += r[c] * r[c];
+        s = F / sqrt(rr);
+    }
+    for (int c = 0; c < 3; ++c) f[c] = s * w[c];
+}
+
+static void rotational_ctforces(orc_blob *b) {
+    for (int i = 0; i < b->nrf; ++i) {
+        double f[3];
+        orc_rotational_force(b->pos + 3 * b->rf_node[i], b->rf_axis + 6 * i, b->rf_force[i], b->rf_torque[i], f);
+        for (int c = 0; c < 3; ++c) b->force[3 * b->rf_node[i] + c] += f[c];
+    }
+}
+
+int orc_blob_set_rotational_forces(orc_blob *b, int num, const int *node, const double *force, const double *axis,
+                                   const int *torque) {
+    if (num < 0) return 1;
+    for (int i = 0; i < num; ++i)
+        if (node[i] < 0 || node[i] >= b->nt || (torque[i] != 0 && torque[i] != 1)) return 1;
+    free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
+    b->nrf = num;
+    b->rf_node = xcalloc((size_t)num, sizeof(int));
+    b->rf_force = xcalloc((size_t)num, sizeof(double));
+    b->rf_axis = xcalloc((size_t)6 * num, sizeof(double));
+    b->rf_torque = xcalloc((size_t)num, sizeof(int));
+    if (num) {
+        memcpy(b->rf_node, node, sizeof(int) * (size_t)num);
+        memcpy(b->rf_force, force, sizeof(double) * (size_t)num);
+        memcpy(b->rf_axis, axis, sizeof(double) * 6 * (size_t)num);
+        memcpy(b->rf_torque, torque, sizeof(int) * (size_t)num);
+    }
+    return 0;
 }
 
 /* Blob::update_positions -> aggregate_forces_and_solve (Blob.cpp:3147-3244),

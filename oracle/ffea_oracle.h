@@ -82,6 +82,16 @@ void orc_blob_destroy(orc_blob *b);
 int orc_blob_set_surface_forces(orc_blob *b, int nfaces, const int *face_nodes, int nsets, const double *set_force,
                                 const int *set_ptr, const int *set_faces);
 
+/* rotational constant forces (Blob::load_ctforces, Blob.cpp:2650-2726), axes given by a
+ * point and a direction ('p' forces; Blob.cpp:1697-1706 leaves node-pair axes unwritten):
+ * entry i pulls node[i] round the axis through axis[6i..6i+2] (meso units) along the unit
+ * direction axis[6i+3..6i+5] with force[i] (torque[i] = 0, 'f': |f| = force |r|) or
+ * constant torque (torque[i] = 1, 't': |f| = force). Returns 0 or 1 (bad input). */
+int orc_blob_set_rotational_forces(orc_blob *b, int num, const int *node, const double *force, const double *axis,
+                                   const int *torque);
+/* the force of one such entry on a node at x (Blob.cpp:1693-1714) */
+void orc_rotational_force(const double *x, const double *axis, double F, int torque, double *f);
+
 /* Blob.cpp:400-491 ; returns the number of inverted elements */
 int  orc_blob_update_internal_forces(orc_blob *b);
 /* Blob.cpp:493-508 ; returns 0, or 1 when CG does not converge */

@@ -82,6 +82,9 @@ def lib():
         L.orc_blob_destroy.argtypes = [C.c_void_p]
         L.orc_blob_set_surface_forces.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int, _dp, _ip, _ip]
         L.orc_blob_set_surface_forces.restype = C.c_int
+        L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
+        L.orc_blob_set_rotational_forces.restype = C.c_int
+        L.orc_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         for f in ("update_internal_forces", "update_positions", "step", "num_nodes", "num_elements",
                   "last_cg_iterations", "matrix_nnz"):
             getattr(L, "orc_blob_" + f).argtypes = [C.c_void_p]
@@ -129,8 +132,19 @@ def ref_lib():
         L.ref_nomass_matrix.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_nomass_matrix.restype = C.c_int
+        L.ref_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         _ref = L
     return _ref
+
+
+def rotational_force(x, axis, F, torque, use_ref=False):
+    """Force of one rotational constant force on a node at x (Blob.cpp:1693-1714): the
+    oracle's restatement, or with use_ref the reference's own functions (oracle/_ref)."""
+    L = ref_lib() if use_ref else lib()
+    out = np.zeros(3)
+    fn = L.ref_rotational_force if use_ref else L.orc_rotational_force
+    fn(np.ascontiguousarray(x, np.float64), np.ascontiguousarray(axis, np.float64), float(F), int(torque), out)
+    return out
 
 
 class Rng:
@@ -200,9 +214,11 @@ class Blob:
 
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=10000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
-                 surface_forces=None):
+                 surface_forces=None, rotational_forces=None):
         """surface_forces: None or (face_nodes [nf][3], set_force [ns][3] meso, set_ptr [ns+1],
-        set_faces), the linear surface constant forces of Blob::load_ctforces"""
+        set_faces), the linear surface constant forces of Blob::load_ctforces;
+        rotational_forces: None or (node [n], force [n] meso, axis [n][6] meso, type [n] of
+        "pf" / "pt"), its rotational forces"""
         L = lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
@@ -250,6 +266,17 @@ class Blob:
                                              sp.size - 1, sf if sf.size else np.zeros(3), sp,
                                              sfc if sfc.size else np.zeros(1, np.int32)) != 0:
                 raise ValueError("oracle surface forces rejected")
+        if rotational_forces is not None:
+            rn, rf, ra, rt = rotational_forces
+            rn = np.ascontiguousarray(rn, np.int32).ravel()
+            rf = np.ascontiguousarray(rf, np.float64).ravel()
+            ra = np.ascontiguousarray(ra, np.float64).reshape(-1)
+            tq = np.array([1 if str(t)[1:2] == "t" else (0 if str(t)[1:2] == "f" else 2) for t in rt], np.int32)
+            if rn.size and (rf.size != rn.size or ra.size != 6 * rn.size or tq.size != rn.size
+                            or any(str(t)[:1] != "p" for t in rt)):
+                raise ValueError("oracle rotational forces rejected")
+            if rn.size and L.orc_blob_set_rotational_forces(self.h, rn.size, rn, rf, ra, tq) != 0:
+                raise ValueError("oracle rotational forces rejected")
 
     def __del__(self):
         if getattr(self, "h", None):

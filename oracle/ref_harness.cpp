@@ -2,7 +2,7 @@
 //
 // C entry points over the reference's own translation units (RngStream.cpp,
 // SparsityPattern.cpp, SparseMatrixFixedPattern.cpp, SparseMatrixTypes.cpp,
-// mat_vec_fns.cpp), compiled from the reference tree into oracle/_ref by
+// mat_vec_fns.cpp, and the templates of mat_vec_fns_II.h), compiled from the reference tree into oracle/_ref by
 // oracle/Makefile. Tests use it to check the C restatement in ffea_oracle.c
 // draw-for-draw and bit-for-bit. Nothing here ships or runs on the GPU path.
 #include <cstdint>
@@ -14,6 +14,7 @@
 #include "SparsityPattern.h"
 #include "SparseMatrixFixedPattern.h"
 #include "mat_vec_fns.h"
+#include "mat_vec_fns_II.h"
 
 extern "C" {
 
@@ -111,6 +112,25 @@ int ref_nomass_matrix(int nt, int ne, const int *top, const int *is_pinned, cons
             for (int c = 0; c < 3; ++c) y_out[3 * n + c] = yout[n][c];
     }
     return nnz;
+}
+
+// One rotational constant force as Blob::apply_ctforces CTF 2 forms it
+// (Blob.cpp:1693-1714, axis given by point and direction): the reference's
+// intersectingPointToLine, sub, arr3arr3VectorProduct, magnitude and resize,
+// called in that order. torque: 0 'f', 1 't'.
+void ref_rotational_force(const double *pos3, const double *axis6, double F, int torque, double *f3) {
+    arr3 p{pos3[0], pos3[1], pos3[2]}, a{axis6[0], axis6[1], axis6[2]}, d{axis6[3], axis6[4], axis6[5]};
+    arr3 r, f;
+    intersectingPointToLine(p, a, d, r);
+    sub(r, p, r);
+    arr3arr3VectorProduct(r, d, f);
+    if (torque) {
+        scalar rsize = magnitude(r);
+        resize(F / rsize, f);
+    } else {
+        resize(F, f);
+    }
+    for (int c = 0; c < 3; ++c) f3[c] = f[c];
 }
 
 }  // extern "C"

@@ -74,6 +74,45 @@ def test_ctforces_and_pins(tmp_path):
     assert np.allclose(a["ctf_force"], [[0, 0, 1e-12 / units.FORCE], [2e-12 / units.FORCE, 0, 0]], rtol=1e-15)
 
 
+def test_rotational_ctforces(tmp_path):
+    """the rotational section (Blob.cpp:2630-2726): F / force unit as F * (1 / force unit),
+    the point in length units, the direction normalised, "all" expanded in node order,
+    between the linear and surface sections, with an XML comment on the line"""
+    mesh = meshgen.kuhn_mesh(2, seed=1)
+    prefix = str(tmp_path / "r")
+    meshgen.write_blob_files(mesh, prefix)
+    ctf = str(tmp_path / "r.ctforces")
+    meshgen.write_full_ctforces(ctf, node_forces=[(1e-12, (0, 0, 2.0), 5)],
+                                rot_forces=[(1e-10, "pt", (-1.9e-9, -1.4313e-9, 8.911e-10), (0, 2, 0), "all"),
+                                            (3e-11, "pf", (1e-9, 0, 0), (1, 1, 0), 4)])
+    txt = open(ctf).read().replace(" all\n", " all  <!-- every node -->\n")
+    open(ctf, "w").write(txt)
+    script = str(tmp_path / "r.ffea")
+    meshgen.write_ffea_script(script, prefix, calc_ctforces=1, ctforces_fname="r.ctforces")
+    S = Script(script)
+    node, F, axis, typ = S.rotational_forces(0)
+    nt = mesh.pos.shape[0]
+    assert node.tolist() == list(range(nt)) + [4]
+    assert typ == ["pt"] * nt + ["pf"]
+    assert F[0] == 1e-10 * (1.0 / units.FORCE) and F[-1] == 3e-11 * (1.0 / units.FORCE)
+    np.testing.assert_array_equal(axis[0], [(1.0 / units.LENGTH) * -1.9e-9, (1.0 / units.LENGTH) * -1.4313e-9,
+                                            (1.0 / units.LENGTH) * 8.911e-10, 0.0, 1.0, 0.0])
+    s2 = 1.0 / np.sqrt(2.0)
+    np.testing.assert_allclose(axis[-1], [1e-9 / units.LENGTH, 0, 0, s2, s2, 0], rtol=1e-15)
+    assert S.blob_arrays(0)["ctf_node"].tolist() == [5]
+    assert S.blob_spec(0).rotational_forces is not None
+    # node-pair axes: loaded by the reference but never formed by its force loop
+    open(ctf, "w").write(txt.replace(" pf ", " nf "))
+    with pytest.raises(ScriptError, match="two nodes"):
+        Script(script)
+    open(ctf, "w").write(txt.replace(" pf ", " px "))
+    with pytest.raises(ScriptError, match="Invalid rotational force type"):
+        Script(script)
+    open(ctf, "w").write(txt.replace("rotational forces:", "rotating forces:"))
+    with pytest.raises(ScriptError, match="announce the start of rotational ctforces"):
+        Script(script)
+
+
 def test_errors_carry_reference_messages(tmp_path):
     mesh, script = write_cube(tmp_path)
     txt = open(script).read()
