@@ -33,7 +33,8 @@ EXPORTED = [
     "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint", "ffea_gpu_batch_strain_energy",
     "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
-    "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex",
+    "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
+    "ffea_gpu_blob_get_node_positions",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -98,7 +99,9 @@ def lib():
                                        C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(vp)]
     L.ffea_gpu_blob_create_ex.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
                                           C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(SurfaceForces),
-                                          C.POINTER(RotationalForces), C.POINTER(vp)]
+                                          C.POINTER(RotationalForces), C.c_int, vp, C.POINTER(vp)]
+    L.ffea_gpu_blob_add_node_forces.argtypes = [vp, C.c_int, _ip, _dp]
+    L.ffea_gpu_blob_get_node_positions.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_destroy.argtypes = [vp]
     L.ffea_gpu_blob_destroy.restype = None
     for f in ("update_internal_forces", "update_positions", "synchronize"):

@@ -129,11 +129,12 @@ def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precis
 class GpuBlob(_Handle):
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=1000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
-                 surface_forces=None, rotational_forces=None, precision="fp64", device=0):
+                 surface_forces=None, rotational_forces=None, external_nodes=(), precision="fp64", device=0):
         """surface_forces: None or (face_nodes [nf][3], set_force [ns][3] meso, set_ptr [ns+1],
         set_faces): linear surface constant forces (Blob::load_ctforces, Blob.cpp:2730-2806);
         rotational_forces: None or (node [n], force [n] meso, axis [n][6] meso point and unit
-        direction, type [n] of "pf" / "pt"): rotational constant forces (Blob.cpp:2650-2726)"""
+        direction, type [n] of "pf" / "pt"): rotational constant forces (Blob.cpp:2650-2726);
+        external_nodes: the nodes add_node_forces may load (World::apply_springs)"""
         L = _native.lib()
         pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
         top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
@@ -170,9 +171,11 @@ class GpuBlob(_Handle):
                 raise FFEAException(_native.ERR_INVALID, "rotational forces: node, force, axis and type sizes differ")
             rf = _native.RotationalForces(n, rkeep[0].ctypes.data, rkeep[1].ctypes.data, rkeep[2].ctypes.data,
                                           rkeep[3].ctypes.data)
+        ext_p, ext = _opt(external_nodes, np.int32)
         rc = L.ffea_gpu_blob_create_ex(C.byref(p), pos.shape[0], ne, pos, top, *mats, npin, pin_p, nctf, cn_p, cf_p,
                                        drag_p, st, None if sf is None else C.byref(sf),
-                                       None if rf is None else C.byref(rf), C.byref(h))
+                                       None if rf is None else C.byref(rf), 0 if ext is None else ext.size, ext_p,
+                                       C.byref(h))
         if rc != _native.OK:
             raise FFEAException(rc, L.ffea_gpu_last_error().decode())
         self.h = h
@@ -187,7 +190,8 @@ class GpuBlob(_Handle):
                    calc_stokes=spec.calc_stokes, pinned=spec.pinned, ctf_node=spec.ctf_node,
                    ctf_force=spec.ctf_force, stokes_drag=spec.stokes_drag, rng_state=spec.rng_state,
                    surface_forces=getattr(spec, "surface_forces", None),
-                   rotational_forces=getattr(spec, "rotational_forces", None), precision=precision, **kw)
+                   rotational_forces=getattr(spec, "rotational_forces", None),
+                   external_nodes=getattr(spec, "external_nodes", ()), precision=precision, **kw)
 
     # -- the reference's per-step interface
     def update_internal_forces(self):
@@ -207,6 +211,23 @@ class GpuBlob(_Handle):
 
     def positions(self):
         return self._get("ffea_gpu_blob_get_positions", (self.nt, 3))
+
+    def add_node_forces(self, nodes, forces):
+        """Blob::add_force_to_node for the next step (meso units, list order)"""
+        n = np.ascontiguousarray(nodes, np.int32).ravel()
+        f = np.ascontiguousarray(forces, np.float64).reshape(-1)
+        if f.size != 3 * n.size:
+            raise ValueError("one 3-vector of force per node")
+        self._check(_native.lib().ffea_gpu_blob_add_node_forces(self.h, n.size, n if n.size else np.zeros(1, np.int32),
+                                                               f if f.size else np.zeros(3)))
+
+    def node_positions(self, nodes):
+        """Blob::get_node for each of the given nodes, [n][3]"""
+        n = np.ascontiguousarray(nodes, np.int32).ravel()
+        out = np.zeros((n.size, 3))
+        if n.size:
+            self._check(_native.lib().ffea_gpu_blob_get_node_positions(self.h, n.size, n, out.reshape(-1)))
+        return out
 
     def set_positions(self, p):
         self._check(_native.lib().ffea_gpu_blob_set_positions(self.h, np.ascontiguousarray(p, np.float64).reshape(-1)))

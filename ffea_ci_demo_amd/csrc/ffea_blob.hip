@@ -10,7 +10,7 @@
 //   order; blobs are concatenated, each starting on a 64-row boundary;
 //   positions and velocities are FP64 [rows][3]; CG vectors are FP64, tiled by 64 rows;
 //   element data are structure-of-arrays over the elements of all blobs;
-//   element nodal forces are [Ne][12], shape data [10][Ne];
+//   element nodal forces are [Ne][12]; shape records [Ne][16] (dpsi, vol, A, B, padding);
 //   the viscosity matrix is 3x3 blocks in a sliced-ELL layout: 64 rows per slice, slice
 //   width W = its longest row, values [slot_base*9 + (j*9 + k)*64 + lane] and block
 //   columns [slot_base + j*64 + lane], so lane-per-row loads of one (j, k) are 512
@@ -123,6 +123,8 @@ struct BlobInput {
     const double *drag = nullptr;  // [nt] Stokes drag per node (calc_stokes), else null
     const ffea_gpu_surface_forces *sf = nullptr;  // surface constant forces, or null
     const ffea_gpu_rotational_forces *rf = nullptr;  // rotational constant forces, or null
+    int next = 0;
+    const int *ext = nullptr;  // nodes that may receive per-step external forces
 };
 
 // closed-form rest state (Blob::calc_rest_state_info, Blob.cpp:3080-3143;
@@ -186,13 +188,18 @@ struct BlobPlan {
     std::vector<int> sf_entry_set;  // [entries]
     std::vector<int> sf_fe_ptr, sf_fe;  // per used face: its entries in set order
     std::vector<int> sf_nf_ptr, sf_nf;  // per node of the .node file: used faces holding it, face order
-    // rotational constant forces: per node that carries one (ascending node index)
-    bool has_rf = false;
-    std::vector<int> rf_node;       // [nrn] node
-    std::vector<int> rf_row;        // [nrn] its corner row, or -1 (midside)
-    std::vector<int> rf_ends;       // [nrn][2] corner rows whose midpoint is its position
-    std::vector<double> rf_lin;     // [nrn][3] its linear constant forces (list-order sum)
-    std::vector<int> rf_ptr;        // [nrn+1] its entries, list order
+    // nodes whose force is formed each step (rotational constant forces, external forces),
+    // ascending node index
+    bool has_dn = false, has_rf = false;
+    bool dn_mid = false;            // some are midside nodes: per-step folds
+    std::vector<int> ext_nodes;     // [next] external force nodes, in the caller's order (first occurrence)
+    std::vector<int> dn_node;       // [ndn] node
+    std::vector<int> dn_ext;        // [ndn] its external force slot, or -1
+    std::vector<int> dn_row;        // [ndn] its corner row, or -1 (midside)
+    std::vector<int> dn_ends;       // [ndn][2] corner rows whose midpoint is its position
+    std::vector<int> dn_lptr;       // [ndn+1] its linear constant forces, list order
+    std::vector<double> dn_lval;    // [..][3]
+    std::vector<int> dn_rptr;       // [ndn+1] its rotational forces, list order
     std::vector<double> rf_axis;    // [entries][6]
     std::vector<double> rf_F;       // [entries]
     std::vector<int> rf_torque;     // [entries] 1: 't'
@@ -485,57 +492,94 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
                 for (int i = 0; i < 3; ++i) P.sf_nf[fill[F.face_nodes[3 * used[u] + i]]++] = u;
         }
     }
-    // ---- rotational constant forces (Blob::load_ctforces, Blob.cpp:2650-2726)
-    if (in.rf && in.rf->num > 0) {
-        const ffea_gpu_rotational_forces &R = *in.rf;
-        if (!R.node || !R.force || !R.axis || !R.type) throw Invalid(who + "rotational constant forces: invalid argument");
-        std::vector<int> count(nt + 1, 0);
-        for (int i = 0; i < R.num; ++i) {
-            const int v = R.node[i];
-            if (v < 0 || v >= nt)
-                throw Invalid(who + fmt("rotational force %d references an out of bounds node index", i));
-            if (R.type[2 * i] != 'p')
-                throw Invalid(who + fmt("rotational force %d: only axes given by point and direction ('p') are "
-                                        "applied (the reference leaves 'n' axes unformed)", i));
-            if (R.type[2 * i + 1] != 'f' && R.type[2 * i + 1] != 't')
-                throw Invalid(who + fmt("rotational force %d: type must be 'f' or 't'", i));
-            if (P.lin_of[v] < 0 && P.mid_a[v] < 0)
-                throw Invalid(who + fmt("rotational force %d acts on node %d, which belongs to no element", i, v));
-            count[v + 1]++;
+    // ---- nodes whose force is formed each step: rotational constant forces
+    // (Blob::load_ctforces, Blob.cpp:2650-2726) and external forces (the nodes
+    // World::apply_springs pulls). Per node: its external force, then its linear constant
+    // forces one by one, then its rotational ones, each in list order, as
+    // Blob::add_force_to_node and Blob::apply_ctforces add them (Blob.cpp:1676-1719).
+    const int nrot = (in.rf && in.rf->num > 0) ? in.rf->num : 0;
+    if (nrot > 0 || in.next > 0) {
+        std::vector<int> rcount(nt + 1, 0), lcount(nt + 1, 0);
+        std::vector<int> ext_of(nt, -1);
+        auto on_element = [&](int v) { return P.lin_of[v] >= 0 || P.mid_a[v] >= 0; };
+        if (nrot > 0) {
+            const ffea_gpu_rotational_forces &R = *in.rf;
+            if (!R.node || !R.force || !R.axis || !R.type)
+                throw Invalid(who + "rotational constant forces: invalid argument");
+            for (int i = 0; i < R.num; ++i) {
+                const int v = R.node[i];
+                if (v < 0 || v >= nt)
+                    throw Invalid(who + fmt("rotational force %d references an out of bounds node index", i));
+                if (R.type[2 * i] != 'p')
+                    throw Invalid(who + fmt("rotational force %d: only axes given by point and direction ('p') are "
+                                            "applied (the reference leaves 'n' axes unformed)", i));
+                if (R.type[2 * i + 1] != 'f' && R.type[2 * i + 1] != 't')
+                    throw Invalid(who + fmt("rotational force %d: type must be 'f' or 't'", i));
+                if (!on_element(v))
+                    throw Invalid(who + fmt("rotational force %d acts on node %d, which belongs to no element", i, v));
+                rcount[v + 1]++;
+            }
         }
-        for (int v = 0; v < nt; ++v) count[v + 1] += count[v];
-        std::vector<int> order(R.num);
-        {
-            std::vector<int> fill(count.begin(), count.end() - 1);
-            for (int i = 0; i < R.num; ++i) order[fill[R.node[i]]++] = i;  // list order within a node
+        if (in.next > 0) {
+            if (!in.ext) throw Invalid(who + "external force nodes: invalid argument");
+            for (int i = 0; i < in.next; ++i) {
+                const int v = in.ext[i];
+                if (v < 0 || v >= nt)
+                    throw Invalid(who + fmt("external force node %d is an out of bounds node index", i));
+                if (!on_element(v))
+                    throw Invalid(who + fmt("external force node %d belongs to no element", v));
+                if (ext_of[v] < 0) {
+                    ext_of[v] = (int)P.ext_nodes.size();
+                    P.ext_nodes.push_back(v);
+                }
+            }
         }
-        P.has_rf = true;
-        P.rf_ptr.push_back(0);
+        for (int i = 0; i < in.nctf; ++i) lcount[in.ctf_node[i] + 1]++;
         for (int v = 0; v < nt; ++v) {
-            if (count[v + 1] == count[v]) continue;
-            P.rf_node.push_back(v);
+            rcount[v + 1] += rcount[v];
+            lcount[v + 1] += lcount[v];
+        }
+        std::vector<int> rorder(nrot), lorder(in.nctf);
+        {
+            std::vector<int> fill(rcount.begin(), rcount.end() - 1);
+            for (int i = 0; i < nrot; ++i) rorder[fill[in.rf->node[i]]++] = i;  // list order within a node
+            std::vector<int> lfill(lcount.begin(), lcount.end() - 1);
+            for (int i = 0; i < in.nctf; ++i) lorder[lfill[in.ctf_node[i]]++] = i;
+        }
+        P.has_rf = nrot > 0;
+        P.has_dn = true;
+        P.dn_lptr.push_back(0);
+        P.dn_rptr.push_back(0);
+        for (int v = 0; v < nt; ++v) {
+            if (rcount[v + 1] == rcount[v] && ext_of[v] < 0) continue;
+            P.dn_node.push_back(v);
+            P.dn_ext.push_back(ext_of[v]);
             if (P.lin_of[v] >= 0) {
-                P.rf_row.push_back(P.lin_of[v]);
-                P.rf_ends.push_back(P.lin_of[v]);
-                P.rf_ends.push_back(P.lin_of[v]);
+                P.dn_row.push_back(P.lin_of[v]);
+                P.dn_ends.push_back(P.lin_of[v]);
+                P.dn_ends.push_back(P.lin_of[v]);
             } else {
-                P.rf_row.push_back(-1);
-                P.rf_ends.push_back(P.lin_of[P.mid_a[v]]);
-                P.rf_ends.push_back(P.lin_of[P.mid_b[v]]);
+                P.dn_mid = true;
+                P.dn_row.push_back(-1);
+                P.dn_ends.push_back(P.lin_of[P.mid_a[v]]);
+                P.dn_ends.push_back(P.lin_of[P.mid_b[v]]);
             }
-            for (int c = 0; c < 3; ++c) P.rf_lin.push_back(ctf_total[3 * v + c]);
-            for (int k = count[v]; k < count[v + 1]; ++k) {
-                const int i = order[k];
-                P.rf_axis.insert(P.rf_axis.end(), R.axis + 6 * (size_t)i, R.axis + 6 * (size_t)i + 6);
-                P.rf_F.push_back(R.force[i]);
-                P.rf_torque.push_back(R.type[2 * i + 1] == 't' ? 1 : 0);
+            for (int k = lcount[v]; k < lcount[v + 1]; ++k)
+                P.dn_lval.insert(P.dn_lval.end(), in.ctf_force + 3 * (size_t)lorder[k],
+                                 in.ctf_force + 3 * (size_t)lorder[k] + 3);
+            P.dn_lptr.push_back((int)(P.dn_lval.size() / 3));
+            for (int k = rcount[v]; k < rcount[v + 1]; ++k) {
+                const int i = rorder[k];
+                P.rf_axis.insert(P.rf_axis.end(), in.rf->axis + 6 * (size_t)i, in.rf->axis + 6 * (size_t)i + 6);
+                P.rf_F.push_back(in.rf->force[i]);
+                P.rf_torque.push_back(in.rf->type[2 * i + 1] == 't' ? 1 : 0);
             }
-            P.rf_ptr.push_back((int)P.rf_F.size());
+            P.dn_rptr.push_back((int)P.rf_F.size());
         }
         P.have_ctf = true;  // the per-step totals live in the constant-force tables
     }
     P.fp.assign(nl + 1, 0);
-    if (stokes_noise || P.has_sf || P.has_rf) {
+    if (stokes_noise || P.has_sf || P.dn_mid) {
         // every node carries noise: the folds are formed each step from the records
         std::vector<std::vector<std::array<int, 3>>> rec(nl);
         std::vector<unsigned char> folded(nt, 0);
@@ -617,10 +661,11 @@ struct Tables {
     int sf_nfaces = 0, sf_nsets = 0;
     std::vector<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
     std::vector<double> sf_force;
-    // rotational constant forces (single-blob handles): per node that carries one
-    bool have_rf = false;
-    std::vector<int> rf_node, rf_row, rf_ends, rf_ptr, rf_torque;
-    std::vector<double> rf_lin, rf_axis, rf_F;
+    // nodes whose force is formed each step (single-blob handles)
+    bool have_dn = false, have_rf = false;
+    int next = 0;
+    std::vector<int> ext_nodes, dn_node, dn_ext, dn_row, dn_ends, dn_lptr, dn_rptr, rf_torque;
+    std::vector<double> dn_lval, rf_axis, rf_F;
 };
 
 Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_params &prm) {
@@ -654,9 +699,15 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
     }
     for (int b = 0; b < nblob; ++b) t.have_sf |= plans[b].has_sf;
     if (t.have_sf && nblob > 1) throw Invalid("surface constant forces need a single-blob handle");
-    for (int b = 0; b < nblob; ++b) t.have_rf |= plans[b].has_rf;
-    if (t.have_rf && nblob > 1) throw Invalid("rotational constant forces need a single-blob handle");
-    t.dyn_fold = t.stokes_noise || t.have_sf || t.have_rf;
+    bool dn_mid = false;
+    for (int b = 0; b < nblob; ++b) {
+        t.have_dn |= plans[b].has_dn;
+        t.have_rf |= plans[b].has_rf;
+        dn_mid |= plans[b].dn_mid;
+    }
+    if (t.have_dn && nblob > 1)
+        throw Invalid("rotational constant forces and external forces need a single-blob handle");
+    t.dyn_fold = t.stokes_noise || t.have_sf || dn_mid;
     // ---- concatenation offsets: every blob starts on a 64-row slice boundary
     t.meta.resize(nblob);
     long long rows = 0, elems = 0, slots = 0, slices = 0, contribs = 0, nodes = 0;
@@ -774,12 +825,16 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
             t.sf_nf_ptr = std::move(P.sf_nf_ptr);
             t.sf_nf = std::move(P.sf_nf);
         }
-        if (P.has_rf) {  // single blob: rows and nodes need no offsets
-            t.rf_node = std::move(P.rf_node);
-            t.rf_row = std::move(P.rf_row);
-            t.rf_ends = std::move(P.rf_ends);
-            t.rf_lin = std::move(P.rf_lin);
-            t.rf_ptr = std::move(P.rf_ptr);
+        if (P.has_dn) {  // single blob: rows and nodes need no offsets
+            t.next = (int)P.ext_nodes.size();
+            t.ext_nodes = std::move(P.ext_nodes);
+            t.dn_node = std::move(P.dn_node);
+            t.dn_ext = std::move(P.dn_ext);
+            t.dn_row = std::move(P.dn_row);
+            t.dn_ends = std::move(P.dn_ends);
+            t.dn_lptr = std::move(P.dn_lptr);
+            t.dn_lval = std::move(P.dn_lval);
+            t.dn_rptr = std::move(P.dn_rptr);
             t.rf_axis = std::move(P.rf_axis);
             t.rf_F = std::move(P.rf_F);
             t.rf_torque = std::move(P.rf_torque);
@@ -994,21 +1049,26 @@ void validate(const Tables &t) {
         for (size_t k = 0; k < t.sf_nf.size(); ++k)
             if (t.sf_nf[k] < 0 || t.sf_nf[k] >= nf) fail("surface node face", (long long)k);
     }
-    if (t.have_rf) {
-        const long long n = t.rf_node.size(), nn = t.nnodes;
-        if (t.nblob != 1 || n <= 0 || (long long)t.rf_row.size() != n || (long long)t.rf_ends.size() != 2 * n ||
-            (long long)t.rf_lin.size() != 3 * n || (long long)t.rf_ptr.size() != n + 1 || t.rf_ptr[0] != 0 ||
-            !t.dyn_fold || !t.have_ctf || (long long)t.ctf.size() != 3 * nrows || (long long)t.node_ctf.size() != 3 * nn)
-            fail("rotational force table size", n);
-        const long long nent = t.rf_ptr[n];
-        if ((long long)t.rf_axis.size() != 6 * nent || (long long)t.rf_F.size() != nent ||
-            (long long)t.rf_torque.size() != nent)
-            fail("rotational force entries", nent);
-        for (long long k = 0; k < n; ++k)
-            if (t.rf_node[k] < 0 || t.rf_node[k] >= nn || t.rf_row[k] < -1 || t.rf_row[k] >= nrows ||
-                t.rf_ends[2 * k] < 0 || t.rf_ends[2 * k] >= nrows || t.rf_ends[2 * k + 1] < 0 ||
-                t.rf_ends[2 * k + 1] >= nrows || t.rf_ptr[k + 1] < t.rf_ptr[k])
-                fail("rotational force node", k);
+    if (t.have_dn) {
+        const long long n = t.dn_node.size(), nn = t.nnodes;
+        if (t.nblob != 1 || n <= 0 || (long long)t.dn_ext.size() != n || (long long)t.dn_row.size() != n ||
+            (long long)t.dn_ends.size() != 2 * n || (long long)t.dn_lptr.size() != n + 1 || t.dn_lptr[0] != 0 ||
+            (long long)t.dn_rptr.size() != n + 1 || t.dn_rptr[0] != 0 || !t.have_ctf ||
+            (long long)t.ctf.size() != 3 * nrows || (long long)t.ext_nodes.size() != t.next ||
+            (t.dyn_fold && (long long)t.node_ctf.size() != 3 * nn))
+            fail("dynamic node table size", n);
+        const long long nl = t.dn_lptr[n], nr = t.dn_rptr[n];
+        if ((long long)t.dn_lval.size() != 3 * nl || (long long)t.rf_axis.size() != 6 * nr ||
+            (long long)t.rf_F.size() != nr || (long long)t.rf_torque.size() != nr)
+            fail("dynamic node entries", nr);
+        for (long long k = 0; k < n; ++k) {
+            if (t.dn_node[k] < 0 || t.dn_node[k] >= nn || t.dn_row[k] < -1 || t.dn_row[k] >= nrows ||
+                t.dn_ends[2 * k] < 0 || t.dn_ends[2 * k] >= nrows || t.dn_ends[2 * k + 1] < 0 ||
+                t.dn_ends[2 * k + 1] >= nrows || t.dn_lptr[k + 1] < t.dn_lptr[k] || t.dn_rptr[k + 1] < t.dn_rptr[k] ||
+                t.dn_ext[k] < -1 || t.dn_ext[k] >= t.next)
+                fail("dynamic node", k);
+            if (t.dn_row[k] < 0 && !t.dyn_fold) fail("dynamic midside node without per-step folds", k);
+        }
     }
 }
 
@@ -1058,13 +1118,19 @@ struct ffea_gpu_blob {
     int sf_nfaces = 0, sf_nsets = 0, sf_nent = 0;
     DevBuf<int> sf_rows, sf_set_ptr, sf_entry_face, sf_entry_set, sf_fe_ptr, sf_fe, sf_nf_ptr, sf_nf;
     DevBuf<double> sf_force, sf_area, sf_total, sf_face_force;
-    // rotational constant forces (single blob)
-    bool have_rf = false;
-    int rf_n = 0;
-    DevBuf<int> rf_node, rf_row, rf_ends, rf_ptr, rf_torque;
-    DevBuf<double> rf_lin, rf_axis, rf_F;
+    // nodes whose force is formed each step (single blob): rotational constant forces,
+    // external forces (staged on the host by ffea_gpu_blob_add_node_forces)
+    bool have_dn = false, have_rf = false;
+    int dn_n = 0, next = 0;
+    DevBuf<int> dn_node, dn_ext, dn_row, dn_ends, dn_lptr, dn_rptr, rf_torque;
+    DevBuf<double> dn_lval, rf_axis, rf_F, ext_force;
+    std::vector<int> ext_slot;        // [num_nodes] external force slot of each node, or -1
+    std::vector<double> ext_staged;   // [next][3] forces added since the last force formation
+    bool ext_dirty = false;           // ext_staged holds forces not yet uploaded
+    bool ext_on_device = false;       // ext_force holds forces of the coming step (to clear after it)
+    DevBuf<int> take_rows;            // ffea_gpu_blob_get_node_positions scratch
+    DevBuf<double> take_out;
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
-    int max_slice_w = 1;  // widest slice: the assembly grid's second dimension
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
     bool fused_cg = false;
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
@@ -1195,10 +1261,13 @@ struct ffea_gpu_blob {
     RowArgs row_args() { return RowArgs{nchunks, cpb, chunks.p, seg.p, S.p, num_done.p}; }
     // assembly grid: a workgroup per slot and row block for small meshes, per row block
     // (its lanes walking their rows' slots) from 1024 row blocks on
-    dim3 asm_grid() const { return grid_rows >= 1024 ? dim3(grid_rows, 1) : dim3(grid_rows, max_slice_w); }
+    // lanes per row of the assembly: 1 once the rows alone fill 1024 workgroups, else 16
+    int asm_lanes() const { return grid_rows >= 1024 ? 1 : 16; }
+    dim3 asm_grid() const { return dim3((unsigned)(((long long)nrows * asm_lanes() + kBlock - 1) / kBlock)); }
     void launch_element(bool advance = true);
     void launch_assemble();
     void launch_assemble_gather();
+    void prepare_ext();
     void mg_setup();
     void launch_mg_vcycle(double *Pout);
     void launch_mg_gauge();
@@ -1252,9 +1321,14 @@ void ffea_gpu_blob::upload_elements(const Tables &t) {
     upload_converted(*bE, t.E, stream, &dev_bytes);
     bf->alloc((size_t)12 * ne, &dev_bytes);
     bf->zero(stream);
-    // shape data: the volume row is read for the previous volume of an inverted element
-    std::vector<double> g0((size_t)10 * ne, 0.0);
-    for (int e = 0; e < ne; ++e) g0[(size_t)10 * e + 9] = t.vol0[e];
+    // shape records: the volume is read for the previous volume of an inverted element;
+    // A and B ride in each record for the assembly
+    std::vector<double> g0((size_t)kGeo * ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
+        g0[(size_t)kGeo * e + 9] = t.vol0[e];
+        g0[(size_t)kGeo * e + 10] = t.A[e];
+        g0[(size_t)kGeo * e + 11] = t.B[e];
+    }
     upload_converted(*bg, g0, stream, &dev_bytes);
     bval->alloc((size_t)nslots * 9, &dev_bytes);
     bval->zero(stream);
@@ -1276,7 +1350,6 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     any_project = t.any_project;
     meta = std::move(t.meta);
     grid_rows = (nrows + kBlock - 1) / kBlock;
-    for (int w : t.sw) max_slice_w = std::max(max_slice_w, w);
     fused_cg = nblob > 0;
     for (const BlobMeta &m : meta) fused_cg = fused_cg && m.nl <= kFusedMaxRows;
     grid_elems = (ne + kBlock - 1) / kBlock;
@@ -1362,17 +1435,30 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
         sf_total.alloc(sf_nsets, &dev_bytes);
         sf_face_force.alloc((size_t)3 * sf_nfaces, &dev_bytes); sf_face_force.zero(stream);
     }
+    have_dn = t.have_dn;
     have_rf = t.have_rf;
-    if (have_rf) {
-        rf_n = (int)t.rf_node.size();
-        rf_node.alloc(t.rf_node.size(), &dev_bytes); rf_node.upload(t.rf_node, stream);
-        rf_row.alloc(t.rf_row.size(), &dev_bytes); rf_row.upload(t.rf_row, stream);
-        rf_ends.alloc(t.rf_ends.size(), &dev_bytes); rf_ends.upload(t.rf_ends, stream);
-        rf_ptr.alloc(t.rf_ptr.size(), &dev_bytes); rf_ptr.upload(t.rf_ptr, stream);
-        rf_torque.alloc(t.rf_torque.size(), &dev_bytes); rf_torque.upload(t.rf_torque, stream);
-        rf_lin.alloc(t.rf_lin.size(), &dev_bytes); rf_lin.upload(t.rf_lin, stream);
-        rf_axis.alloc(t.rf_axis.size(), &dev_bytes); rf_axis.upload(t.rf_axis, stream);
-        rf_F.alloc(t.rf_F.size(), &dev_bytes); rf_F.upload(t.rf_F, stream);
+    if (have_dn) {
+        dn_n = (int)t.dn_node.size();
+        next = t.next;
+        dn_node.alloc(t.dn_node.size(), &dev_bytes); dn_node.upload(t.dn_node, stream);
+        dn_ext.alloc(t.dn_ext.size(), &dev_bytes); dn_ext.upload(t.dn_ext, stream);
+        dn_row.alloc(t.dn_row.size(), &dev_bytes); dn_row.upload(t.dn_row, stream);
+        dn_ends.alloc(t.dn_ends.size(), &dev_bytes); dn_ends.upload(t.dn_ends, stream);
+        dn_lptr.alloc(t.dn_lptr.size(), &dev_bytes); dn_lptr.upload(t.dn_lptr, stream);
+        dn_rptr.alloc(t.dn_rptr.size(), &dev_bytes); dn_rptr.upload(t.dn_rptr, stream);
+        dn_lval.alloc(std::max<size_t>(1, t.dn_lval.size()), &dev_bytes);
+        if (!t.dn_lval.empty()) dn_lval.upload(t.dn_lval, stream);
+        rf_torque.alloc(std::max<size_t>(1, t.rf_torque.size()), &dev_bytes);
+        if (!t.rf_torque.empty()) rf_torque.upload(t.rf_torque, stream);
+        rf_axis.alloc(std::max<size_t>(1, t.rf_axis.size()), &dev_bytes);
+        if (!t.rf_axis.empty()) rf_axis.upload(t.rf_axis, stream);
+        rf_F.alloc(std::max<size_t>(1, t.rf_F.size()), &dev_bytes);
+        if (!t.rf_F.empty()) rf_F.upload(t.rf_F, stream);
+        ext_force.alloc((size_t)3 * std::max(1, next), &dev_bytes);
+        ext_force.zero(stream);
+        ext_slot.assign(meta[0].nt, -1);
+        for (int i = 0; i < next; ++i) ext_slot[t.ext_nodes[i]] = i;
+        ext_staged.assign((size_t)3 * next, 0.0);
     }
     jump_group.alloc(t.jump_group.size(), &dev_bytes); jump_group.upload(t.jump_group, stream);
     jump_lane.alloc(t.jump_lane.size(), &dev_bytes); jump_lane.upload(t.jump_lane, stream);
@@ -1420,8 +1506,8 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
         (have_sf && (!sf_rows.p || !sf_force.p || !sf_set_ptr.p || !sf_entry_face.p || !sf_entry_set.p ||
                      !sf_fe_ptr.p || !sf_fe.p || !sf_nf_ptr.p || !sf_nf.p || !sf_area.p || !sf_total.p ||
                      !sf_face_force.p)) ||
-        (have_rf && (!rf_node.p || !rf_row.p || !rf_ends.p || !rf_ptr.p || !rf_torque.p || !rf_lin.p || !rf_axis.p ||
-                     !rf_F.p || !ctf.p || !node_ctf.p)))
+        (have_dn && (!dn_node.p || !dn_ext.p || !dn_row.p || !dn_ends.p || !dn_lptr.p || !dn_rptr.p || !dn_lval.p ||
+                     !rf_torque.p || !rf_axis.p || !rf_F.p || !ext_force.p || !ctf.p || (dyn_fold && !node_ctf.p))))
         throw Invalid("internal error: a device buffer was not allocated");
     // row-kernel grid: one block per chunk up to 1024 blocks (tunable, result-neutral)
     set_row_blocks(1024);
@@ -1485,10 +1571,10 @@ int ffea_gpu_blob::check_status_after_forces() {
 
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_gather(const T *force) {
-    if (have_rf) {  // Blob::apply_ctforces CTF 1 + 2 at this step's positions
-        RotArgs ra{rf_n, rf_node.p, rf_row.p, rf_ends.p, rf_lin.p, rf_ptr.p, rf_axis.p, rf_F.p, rf_torque.p, pos.p,
-                   node_ctf.p, ctf.p};
-        hipLaunchKernelGGL(k_rotational_forces, dim3((rf_n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ra);
+    if (have_dn) {  // external forces, then Blob::apply_ctforces CTF 1 + 2 at this step's positions
+        DynArgs da{dn_n, dn_node.p, dn_row.p, dn_ends.p, dn_ext.p, ext_force.p, dn_lptr.p, dn_lval.p, dn_rptr.p,
+                   rf_axis.p, rf_F.p, rf_torque.p, pos.p, dyn_fold ? node_ctf.p : nullptr, ctf.p};
+        hipLaunchKernelGGL(k_node_forces, dim3((dn_n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, da);
     }
     if (have_sf) {  // Blob::apply_ctforces CTF 3 at this step's positions
         SurfArgs sa{sf_nfaces, sf_nsets, sf_nent, sf_rows.p, pos.p, sf_force.p, sf_set_ptr.p, sf_entry_face.p,
@@ -1527,13 +1613,15 @@ void ffea_gpu_blob::launch_assemble() {
     hipEvent_t t0;
     t_begin(1, &t0);
     if (mixed) {
-        AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_A.p, f_B.p,
+        AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p,
                          f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
-        hipLaunchKernelGGL(k_assemble<float>, asm_grid(), dim3(kBlock), 0, stream, a);
+        if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<float, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        else hipLaunchKernelGGL((k_assemble<float, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     } else {
-        AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_A.p, d_B.p,
+        AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p,
                           d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
-        hipLaunchKernelGGL(k_assemble<double>, asm_grid(), dim3(kBlock), 0, stream, a);
+        if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<double, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        else hipLaunchKernelGGL((k_assemble<double, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
     t_end(1, t0);
@@ -1981,8 +2069,26 @@ void ffea_gpu_blob::capture_graph(int iters) {
     graph_iters = iters;
 }
 
+// External forces staged since the last force formation go to the device for the coming
+// step; those of an earlier step are cleared (Blob::set_forces_to_zero each step,
+// World.cpp:1862). Runs outside any graph.
+void ffea_gpu_blob::prepare_ext() {
+    if (!have_dn || next == 0) return;
+    if (ext_dirty) {
+        ext_force.upload(ext_staged, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));  // the copy has read ext_staged
+        std::fill(ext_staged.begin(), ext_staged.end(), 0.0);
+        ext_dirty = false;
+        ext_on_device = true;
+    } else if (ext_on_device) {
+        ext_force.zero(stream);
+        ext_on_device = false;
+    }
+}
+
 int ffea_gpu_blob::step_once() {
     if (active_blobs == 0) return FFEA_GPU_OK;
+    prepare_ext();
     if (use_graph && expect_iters > 0) {
         // enough iterations to cover the last step's count with a margin, in steps of 16
         int want = std::min(prm.max_iterations_cg, ((expect_iters + expect_iters / 8 + 8 + 15) / 16) * 16);
@@ -2024,6 +2130,11 @@ void ffea_gpu_blob::set_material_t(const double *A, const double *B, const doubl
         it.first->upload(t, stream);
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
+    T *geo = std::is_same<T, double>::value ? (T *)d_geo.p : (T *)f_geo.p;
+    hipLaunchKernelGGL(k_geo_material<T>, dim3((ne0 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, ne0,
+                       (const T *)bA->p, (const T *)bB->p, geo);
+    FFEA_HIP_TRY(hipGetLastError());
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
 }
 
 static void ensure(DevBuf<double> &b, size_t n, size_t *total) {
@@ -2041,9 +2152,9 @@ static void ensure(DevBuf<double> &b, size_t n, size_t *total) {
 int ffea_gpu_blob::forward_checkpointed(int nsteps, int interval) {
     if (nblob != 1 || prm.calc_noise || meta[0].all_pinned)
         throw Invalid("a checkpointed run needs a single blob, calc_noise = 0 and unpinned nodes");
-    if (have_sf || have_rf)
+    if (have_sf || have_dn)
         throw Invalid("a checkpointed run needs forces that do not depend on positions: the adjoint does not "
-                      "differentiate surface or rotational constant forces");
+                      "differentiate surface, rotational or external forces");
     const size_t vec = (size_t)3 * nrows;
     const int nck = (nsteps + interval - 1) / interval;
     ensure(ckpt, vec * std::max(1, nck), &dev_bytes);
@@ -2289,7 +2400,7 @@ int ffea_gpu_blob_create(const ffea_gpu_params *params, int num_nodes, int num_e
                          const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                          ffea_gpu_blob **out) {
     return ffea_gpu_blob_create_ex(params, num_nodes, num_elements, pos, top, A, B, G, E, num_pinned, pinned, num_ctf,
-                                   ctf_node, ctf_force, stokes_drag, rng_state, nullptr, nullptr, out);
+                                   ctf_node, ctf_force, stokes_drag, rng_state, nullptr, nullptr, 0, nullptr, out);
 }
 
 int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements, const double *pos,
@@ -2297,8 +2408,13 @@ int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int nu
                             int num_pinned, const int *pinned, int num_ctf, const int *ctf_node,
                             const double *ctf_force, const double *stokes_drag, const uint32_t rng_state[6],
                             const ffea_gpu_surface_forces *surface_forces,
-                            const ffea_gpu_rotational_forces *rotational_forces, ffea_gpu_blob **out) {
+                            const ffea_gpu_rotational_forces *rotational_forces, int num_external_nodes,
+                            const int *external_nodes, ffea_gpu_blob **out) {
     if (out) *out = nullptr;
+    if (num_external_nodes < 0 || (num_external_nodes > 0 && !external_nodes)) {
+        g_last_error = "ffea_gpu_blob_create: invalid argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
     if (!params || !out) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
@@ -2306,7 +2422,7 @@ int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int nu
     // the drag is used only with calc_stokes = 1 (NoMassCGSolver.cpp:119-127, Blob.cpp:3172)
     const BlobInput bi{num_nodes,      num_elements, pos,       top, A, B, G, E, num_pinned, pinned, num_ctf, ctf_node,
                        ctf_force,      rng_state,    params->calc_stokes ? stokes_drag : nullptr, surface_forces,
-                       rotational_forces};
+                       rotational_forces, num_external_nodes, external_nodes};
     if (!valid_blob_input(bi)) {
         g_last_error = "ffea_gpu_blob_create: invalid argument";
         return FFEA_GPU_ERR_INVALID;
@@ -2398,6 +2514,7 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
+        b->prepare_ext();
         b->launch_assemble_gather();
         b->run_cg_eager(0);
         b->launch_update();
@@ -2412,6 +2529,69 @@ int ffea_gpu_blob_step(ffea_gpu_blob *b, int nsteps) {
         for (int s = 0; s < nsteps; ++s) {
             const int rc = b->step_once();
             if (rc != FFEA_GPU_OK) return rc;
+        }
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_add_node_forces(ffea_gpu_blob *b, int num, const int *node, const double *force) {
+    if (!b || num < 0 || (num > 0 && (!node || !force))) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        if (num == 0) return (int)FFEA_GPU_OK;
+        if (!b->have_dn || b->next == 0)
+            throw Invalid("ffea_gpu_blob_add_node_forces: the handle was created without external force nodes");
+        for (int i = 0; i < num; ++i)
+            if (node[i] < 0 || node[i] >= (int)b->ext_slot.size() || b->ext_slot[node[i]] < 0)
+                throw Invalid(fmt("ffea_gpu_blob_add_node_forces: node %d is not an external force node of this handle",
+                                  node[i]));
+        for (int i = 0; i < num; ++i) {  // Blob::add_force_to_node, in call order
+            double *f = &b->ext_staged[3 * (size_t)b->ext_slot[node[i]]];
+            for (int c = 0; c < 3; ++c) f[c] += force[3 * i + c];
+        }
+        b->ext_dirty = true;
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_get_node_positions(ffea_gpu_blob *b, int num, const int *node, double *out) {
+    if (!b || num < 0 || (num > 0 && (!node || !out))) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        if (num == 0) return (int)FFEA_GPU_OK;
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        const BlobMeta &m = b->meta[0];
+        std::vector<int> rows((size_t)2 * num);
+        for (int i = 0; i < num; ++i) {
+            const int v = node[i];
+            if (v < 0 || v >= m.nt) throw Invalid(fmt("ffea_gpu_blob_get_node_positions: node %d is out of bounds", v));
+            int ra = -1, rb = -1;
+            if (m.lin_of[v] >= 0) {
+                ra = rb = m.row_off + m.lin_of[v];
+            } else if (m.mid_a[v] >= 0) {
+                ra = m.row_off + m.lin_of[m.mid_a[v]];
+                rb = m.row_off + m.lin_of[m.mid_b[v]];
+            }
+            rows[2 * i] = ra;
+            rows[2 * i + 1] = rb;
+        }
+        const size_t nr = rows.size();
+        if (!b->take_rows.p || b->take_rows.n < nr) {
+            b->take_rows.reset();
+            b->take_rows.alloc(nr, &b->dev_bytes);
+            b->take_out.reset();
+            b->take_out.alloc(3 * nr, &b->dev_bytes);
+        }
+        b->take_rows.upload(rows, b->stream);
+        hipLaunchKernelGGL(k_take_rows, dim3((unsigned)((nr + kBlock - 1) / kBlock)), dim3(kBlock), 0, b->stream,
+                           (int)nr, b->take_rows.p, b->pos.p, b->take_out.p);
+        FFEA_HIP_TRY(hipGetLastError());
+        std::vector<double> g(3 * nr);
+        FFEA_HIP_TRY(hipMemcpyAsync(g.data(), b->take_out.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost,
+                                    b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        for (int i = 0; i < num; ++i) {
+            const double *pa = &g[6 * (size_t)i], *pb = &g[6 * (size_t)i + 3];
+            for (int c = 0; c < 3; ++c)  // tetra_element_linear::linearise_element for midside nodes
+                out[3 * i + c] = rows[2 * i] == rows[2 * i + 1] ? pa[c] : .5 * (pa[c] + pb[c]);
         }
         return (int)FFEA_GPU_OK;
     });

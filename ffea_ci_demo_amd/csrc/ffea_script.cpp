@@ -28,6 +28,7 @@ namespace meso {
 constexpr double length = 1.7e-10;
 constexpr double Energy = 4.141945559999999e-21;
 constexpr double mass = 1.994307387553024e-26;
+constexpr double area = 2.89e-20;
 constexpr double volume = 4.913e-30;
 constexpr double force = 2.4364705882352941e-11;
 constexpr double time = 3.7302915560886126e-13;
@@ -577,10 +578,18 @@ void load_ctforces(BlobData &B, int blob_index, const std::string &f) {
 
 }  // namespace
 
+// one spring of the springs file (Spring.h; World::load_springs, World.cpp:3543-3643)
+struct SpringData {
+    double k = 0, l = 0;  // meso units
+    int blob[2] = {0, 0}, conf[2] = {0, 0}, node[2] = {0, 0};
+    bool active = false;  // World::activate_springs: both ends in the active (first) conformation
+};
+
 struct ffea_script {
     std::string path;
     Params P;
     std::vector<BlobData> blobs;
+    std::vector<SpringData> springs;
     uint32_t rng_state[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -593,6 +602,79 @@ void parse_vec3(const std::string &s, double *v) {
     std::istringstream is(t);
     is >> v[0] >> v[1] >> v[2];
     if (!is) fail("could not read a 3-vector from '%s'", s.c_str());
+}
+
+// World::load_springs (World.cpp:3543-3643), read with the reference's fscanf formats
+void load_springs(ffea_script &S, const std::string &fname) {
+    FILE *in = fopen(fname.c_str(), "rb");
+    if (!in) fail("Failed to open springs file: '%s'", fname.c_str());
+    char line[50] = {0};
+    if (!fgets(line, sizeof line, in)) {
+        fclose(in);
+        fail("Error reading first line of spring file");
+    }
+    if (strcmp(line, "ffea springs file\n") != 0) {
+        fclose(in);
+        fail("This is not a 'ffea spring file' (read '%s')", line);
+    }
+    int num = 0;
+    if (fscanf(in, "num_springs %d\n", &num) != 1) {
+        fclose(in);
+        fail("Error reading number of springs");
+    }
+    const int header = fscanf(in, "springs:\n");  // unchecked in the reference: the line may be absent
+    (void)header;
+    const int nb = (int)S.blobs.size();
+    for (int i = 0; i < num; ++i) {
+        SpringData sp;
+        if (fscanf(in, "%lf %lf %d %d %d %d %d %d\n", &sp.k, &sp.l, &sp.blob[0], &sp.blob[1], &sp.conf[0], &sp.conf[1],
+                   &sp.node[0], &sp.node[1]) != 8) {
+            fclose(in);
+            fail("Problem reading spring data from %s.Format is : \n\nffea spring file\nnum_springs ? \n"
+                 "k l blob_index_0 blob_index 1 conformation_index_0 conformation_index_1 node_index_0 node_index_1\n",
+                 fname.c_str());
+        }
+        sp.k *= meso::area / meso::Energy;
+        sp.l /= meso::length;
+        for (int j = 0; j < 2; ++j) {
+            if (sp.blob[j] >= nb || sp.blob[j] < 0) {
+                fclose(in);
+                fail("In spring %d, blob index %d is out of bounds given the number of blobs defined (%d). Please fix. "
+                     "Remember, indexing starts at ZERO!\n", i, j, nb);
+            }
+            if (sp.conf[j] >= 1 || sp.conf[j] < 0)
+                printf("In spring %d, conformation index %d is out of bounds given the number of conformations defined "
+                       "in blob %d (%d). Please fix. Remember, indexing starts at ZERO!\n", i, j, sp.blob[j], 1);
+            if (sp.node[j] >= S.blobs[sp.blob[j]].nt || sp.node[j] < 0) {
+                fclose(in);
+                fail("In spring %d, node index %d is out of bounds given the number of nodes defined in blob %d, "
+                     "conformation %d (%d). Please fix. Remember, indexing starts at ZERO!\n", i, j, sp.blob[j],
+                     sp.conf[j], S.blobs[sp.blob[j]].nt);
+            }
+            if (sp.k < 0) {
+                fclose(in);
+                fail("In spring %d, spring constant, %e, < 0. This is not going to end well for you...\n", i, sp.k);
+            }
+            if (sp.l < 0) {
+                fclose(in);
+                fail("In spring %d, spring equilibrium length, %e, < 0. Reverse node definitions for consistency.\n",
+                     i, sp.l);
+            }
+        }
+        if (sp.blob[0] == sp.blob[1] && sp.conf[0] == sp.conf[1] && sp.node[0] == sp.node[1]) {
+            fclose(in);
+            fail("In spring %d, spring is connected to same node on same blob on same conformation. Will probably cause "
+                 "an force calculation error.\n", i);
+        }
+        if (sp.blob[0] == sp.blob[1] && sp.conf[0] != sp.conf[1]) {
+            fclose(in);
+            fail("In spring %d, spring is connected two conformations of the same blob (blob_index = %d). This cannot "
+                 "happen as conformations are mutually exclusive.\n", i, sp.blob[0]);
+        }
+        sp.active = sp.conf[0] == 0 && sp.conf[1] == 0;
+        S.springs.push_back(sp);
+    }
+    fclose(in);
 }
 
 void load_script(ffea_script &S) {
@@ -626,6 +708,19 @@ void load_script(ffea_script &S) {
         if (lv != "ctforces_fname")
             fail("ctforces_fname not provided in <ctforces> block, while calc_ctforces set to 1.");
         ctf_file = join(dir, rv);
+    }
+    std::string springs_file;
+    bool springs_given = false;
+    if (p.calc_springs == 1) {  // World.cpp:2291-2305
+        std::vector<std::string> inter = extract_block("interactions", 0, system);
+        std::vector<std::string> spr = extract_block("springs", 0, inter);
+        if (spr.size() > 1) fail("'Spring' block should only have 1 file.");
+        if (spr.size() == 1) {
+            std::string lv, rv;
+            parse_tag(spr[0], lv, rv);
+            springs_file = join(dir, rv);
+            springs_given = true;
+        }
     }
     for (int b = 0; b < p.num_blobs; ++b) {
         std::vector<std::string> blob = extract_block("blob", b, system);
@@ -707,6 +802,10 @@ void load_script(ffea_script &S) {
                 for (int k = 0; k < 3; ++k) B.pos[3 * i + k] += B.centroid[k] * B.scale - c[k];
         }
         S.blobs.push_back(std::move(B));
+    }
+    if (p.calc_springs == 1) {  // World.cpp:2891-2896, after the blobs
+        if (!springs_given) fail("Failed to open springs file: '' (no <springs> block in <interactions>)");
+        load_springs(S, springs_file);
     }
     // World.cpp:442-474: box from the system size when no grid is set, then the system
     // centroid moved to the box centre
@@ -859,6 +958,24 @@ int ffea_script_blob_rotational_forces(const ffea_script *s, int b, int *num, in
     return FFEA_GPU_OK;
 }
 
+int ffea_script_springs(const ffea_script *s, int *num, double *k, double *l, int *blob, int *conf, int *node,
+                        int *active) {
+    if (!s) return FFEA_GPU_ERR_INVALID;
+    if (num) *num = (int)s->springs.size();
+    for (size_t i = 0; i < s->springs.size(); ++i) {
+        const SpringData &sp = s->springs[i];
+        if (k) k[i] = sp.k;
+        if (l) l[i] = sp.l;
+        for (int j = 0; j < 2; ++j) {
+            if (blob) blob[2 * i + j] = sp.blob[j];
+            if (conf) conf[2 * i + j] = sp.conf[j];
+            if (node) node[2 * i + j] = sp.node[j];
+        }
+        if (active) active[i] = sp.active ? 1 : 0;
+    }
+    return FFEA_GPU_OK;
+}
+
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]) {
     if (!s || !state) return FFEA_GPU_ERR_INVALID;
     std::copy(s->rng_state, s->rng_state + 6, state);
@@ -872,10 +989,10 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
     }
     const BlobData &B = s->blobs[b];
     const ffea_script_params &p = s->P.p;
-    if (p.calc_ssint || p.calc_steric || p.calc_es || p.calc_springs || p.calc_preComp || p.calc_kinetics) {
-        ffea_internal_set_error("surface interactions, electrostatics, springs, pre-computed potentials and "
-                                "kinetics are not part of this build; set calc_ssint, calc_steric, calc_es, "
-                                "calc_springs, calc_preComp and calc_kinetics to 0");
+    if (p.calc_ssint || p.calc_steric || p.calc_es || p.calc_preComp || p.calc_kinetics) {
+        ffea_internal_set_error("surface interactions, electrostatics, pre-computed potentials and kinetics are not "
+                                "part of this build; set calc_ssint, calc_steric, calc_es, calc_preComp and "
+                                "calc_kinetics to 0");
         return FFEA_GPU_ERR_INVALID;
     }
     ffea_gpu_params gp{};
@@ -906,12 +1023,17 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
     }
     ffea_gpu_rotational_forces rf{(int)B.rf_node.size(), B.rf_node.data(), B.rf_force.data(), B.rf_axis.data(),
                                   B.rf_type.data()};
+    std::vector<int> ext;  // the nodes of this blob that active springs pull (World::apply_springs)
+    for (const SpringData &sp : s->springs)
+        for (int j = 0; j < 2; ++j)
+            if (sp.active && sp.blob[j] == b) ext.push_back(sp.node[j]);
     int rc = ffea_gpu_blob_create_ex(&gp, B.nt, B.ne, B.pos.data(), B.top.data(), B.A.data(), B.B.data(), B.G.data(),
                                      B.E.data(), (int)B.pinned.size(), B.pinned.empty() ? nullptr : B.pinned.data(),
                                      (int)B.ctf_node.size(), B.ctf_node.empty() ? nullptr : B.ctf_node.data(),
                                      B.ctf_force.empty() ? nullptr : B.ctf_force.data(),
                                      p.calc_stokes ? drag.data() : nullptr, s->rng_state, nsets > 0 ? &sf : nullptr,
-                                     B.rf_node.empty() ? nullptr : &rf, out);
+                                     B.rf_node.empty() ? nullptr : &rf, (int)ext.size(),
+                                     ext.empty() ? nullptr : ext.data(), out);
     if (rc != FFEA_GPU_OK || s->blobs.size() == 1) return rc;
     // several blobs draw from the one stream in turn (World.cpp:2031-2058): all element
     // draws first, blob after blob, then (Stokes noise) all node draws, blob after blob

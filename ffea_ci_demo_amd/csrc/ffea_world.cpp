@@ -95,6 +95,13 @@ struct ffea_world {
     long long step = 0, step_initial = 0;
     std::vector<double> host;
     std::string err;
+    // springs (World::load_springs, World.cpp:3543-3643), file order
+    struct Spring {
+        double k = 0, l = 0;
+        int blob[2] = {0, 0}, node[2] = {0, 0};
+        bool active = false;
+    };
+    std::vector<Spring> springs;
 
     ~ffea_world() {
         for (FILE *f : {traj, meas, ckpt})
@@ -107,6 +114,81 @@ struct ffea_world {
         const char *m = blob >= 0 ? ffea_gpu_blob_last_error(blobs[blob]) : ffea_gpu_last_error();
         fail(rc, std::string(what) + (blob >= 0 ? " (blob " + std::to_string(blob) + ")" : std::string()) + ": " +
                      (m ? m : ""));
+    }
+
+    // positions [n][3] of the given nodes of blob b
+    void node_positions(int b, const std::vector<int> &nodes, std::vector<double> &x) {
+        x.resize(3 * nodes.size());
+        if (!nodes.empty())
+            check_rc(ffea_gpu_blob_get_node_positions(blobs[b], (int)nodes.size(), nodes.data(), x.data()), b,
+                     "reading spring node positions");
+    }
+
+    // World::apply_springs (World.cpp:4063-4108): each active spring pulls its two nodes with
+    // k (|x1 - x0| - l) along (x1 - x0)/|x1 - x0| (normalize and magnitude,
+    // mat_vec_fns_II.h:188-198, 326-332), in spring order; the forces reach the blobs through
+    // Blob::add_force_to_node before their update_internal_forces
+    void apply_springs() {
+        if (springs.empty()) return;
+        const int nb = (int)blobs.size();
+        std::vector<std::vector<int>> want(nb);
+        for (const Spring &sp : springs)
+            if (sp.active)
+                for (int j = 0; j < 2; ++j) want[sp.blob[j]].push_back(sp.node[j]);
+        std::vector<std::vector<double>> x(nb);
+        for (int b = 0; b < nb; ++b) node_positions(b, want[b], x[b]);
+        std::vector<std::vector<int>> fn(nb);
+        std::vector<std::vector<double>> ff(nb);
+        std::vector<size_t> used(nb, 0);
+        for (const Spring &sp : springs) {
+            if (!sp.active) continue;
+            // the ends were listed in spring order, end 0 first
+            const double *n0 = &x[sp.blob[0]][3 * used[sp.blob[0]]++];
+            const double *n1 = &x[sp.blob[1]][3 * used[sp.blob[1]]++];
+            double sep[3] = {n1[0] - n0[0], n1[1] - n0[1], n1[2] - n0[2]}, sep_norm[3];
+            double sum = 0;
+            for (int c = 0; c < 3; ++c) sum += sep[c] * sep[c];
+            // a zero separation: normalize throws, and World::apply_springs's catch (int) lets it through
+            if (sum == 0) fail(FFEA_GPU_ERR_INVALID, "Unable to normalize a vector with length zero");
+            sum = std::sqrt(sum);
+            for (int c = 0; c < 3; ++c) sep_norm[c] = sep[c] / sum;
+            double mag = 0.0;
+            for (int c = 0; c < 3; ++c) mag += sep[c] * sep[c];
+            const double force_mag = sp.k * (std::sqrt(mag) - sp.l);
+            fn[sp.blob[0]].push_back(sp.node[0]);
+            for (int c = 0; c < 3; ++c) ff[sp.blob[0]].push_back(force_mag * sep_norm[c]);
+            fn[sp.blob[1]].push_back(sp.node[1]);
+            for (int c = 0; c < 3; ++c) ff[sp.blob[1]].push_back(-1 * force_mag * sep_norm[c]);
+        }
+        for (int b = 0; b < nb; ++b)
+            if (!fn[b].empty())
+                check_rc(ffea_gpu_blob_add_node_forces(blobs[b], (int)fn[b].size(), fn[b].data(), ff[b].data()), b,
+                         "applying springs");
+    }
+
+    // the spring part of World::make_measurements (World.cpp:4632-4652): every spring, active
+    // or not, adds 0.5 k (|x0 - x1| - l)^2 to its (blob 0, blob 1) cell; the cells are summed
+    // row by row (World::get_spring_field_energy, World.cpp:4110-4123)
+    double spring_energy() {
+        const int nb = (int)blobs.size();
+        std::vector<std::vector<int>> want(nb);
+        for (const Spring &sp : springs)
+            for (int j = 0; j < 2; ++j) want[sp.blob[j]].push_back(sp.node[j]);
+        std::vector<std::vector<double>> x(nb);
+        for (int b = 0; b < nb; ++b) node_positions(b, want[b], x[b]);
+        std::vector<double> cell((size_t)nb * nb, 0.0);
+        std::vector<size_t> used(nb, 0);
+        for (const Spring &sp : springs) {
+            const double *a = &x[sp.blob[0]][3 * used[sp.blob[0]]++];
+            const double *b = &x[sp.blob[1]][3 * used[sp.blob[1]]++];
+            double m = 0.0;
+            for (int c = 0; c < 3; ++c) m += (a[c] - b[c]) * (a[c] - b[c]);
+            m = std::sqrt(m);
+            cell[(size_t)sp.blob[0] * nb + sp.blob[1]] += 0.5 * sp.k * (m - sp.l) * (m - sp.l);
+        }
+        double e = 0.0;
+        for (double v : cell) e += v;
+        return e;
     }
 
     // World::init headers (World.cpp:527-603)
@@ -143,7 +225,8 @@ struct ffea_world {
         fprintf(meas, "\tnum_rods = %d\n", 0);
         fprintf(meas, "\n\tCalculations enabled:\n");
         fprintf(meas, "\tcalc_noise = %d\n\tcalc_stokes = %d\n", p.calc_noise, p.calc_stokes);
-        fprintf(meas, "\tcalc_ssint = %d\n\tcalc_steric = %d\n\tcalc_preComp = %d\n\tcalc_springs = %d\n", 0, 0, 0, 0);
+        fprintf(meas, "\tcalc_ssint = %d\n\tcalc_steric = %d\n\tcalc_preComp = %d\n\tcalc_springs = %d\n", 0, 0, 0,
+                p.calc_springs);
         fprintf(meas, "\tcalc_ctforces = %d\n\tcalc_kinetics = %d\n\tcalc_es = %d\n", p.calc_ctforces, 0, 0);
         fprintf(meas, "\n\tSimulation box configuration:\n\tmove_into_box = %d\n", p.move_into_box);
         fprintf(meas, "\n\n");
@@ -151,6 +234,7 @@ struct ffea_world {
         fprintf(meas, "%-14s", "Time");
         fprintf(meas, "%-14s", "StrainEnergy");
         fprintf(meas, "%-14s%-14s%-14s%-14s", "Centroid[0]", "Centroid[1]", "Centroid[2]", "RMSD");
+        if (p.calc_springs != 0) fprintf(meas, "%-14s", "SpringEnergy");
         fprintf(meas, "\n");
         fflush(meas);
     }
@@ -209,6 +293,7 @@ struct ffea_world {
         fprintf(meas, "%-14.6e", strain * kEnergy);
         fprintf(meas, "%-14.6e%-14.6e%-14.6e", cog[0] * kLength, cog[1] * kLength, cog[2] * kLength);
         fprintf(meas, "%-14.6e", rmsd * kLength);
+        if (p.calc_springs != 0) fprintf(meas, "%-14.6e", spring_energy() * kEnergy);
         fprintf(meas, "\n");
         fflush(meas);
     }
@@ -380,6 +465,24 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
             w->draws_before.push_back(before);
             before += 7ll * ne;
         }
+        if (p.calc_springs == 1) {
+            int ns = 0;
+            ffea_script_springs(s, &ns, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            std::vector<double> k(ns), l(ns);
+            std::vector<int> blob(2 * (size_t)ns), conf(2 * (size_t)ns), node(2 * (size_t)ns), active(ns);
+            ffea_script_springs(s, &ns, k.data(), l.data(), blob.data(), conf.data(), node.data(), active.data());
+            for (int i = 0; i < ns; ++i) {
+                ffea_world::Spring sp;
+                sp.k = k[i];
+                sp.l = l[i];
+                for (int j = 0; j < 2; ++j) {
+                    sp.blob[j] = blob[2 * i + j];
+                    sp.node[j] = node[2 * i + j];
+                }
+                sp.active = active[i] != 0;
+                w->springs.push_back(sp);
+            }
+        }
         ffea_script_free(s);
         s = nullptr;
         // one blob: whole steps between outputs are replayed as graphs (results identical to
@@ -428,6 +531,7 @@ int ffea_world_run(ffea_world *w, long long max_steps) {
         for (; w->step < p.num_steps + 1 && (max_steps < 0 || done < max_steps); ++w->step, ++done) {
             const bool out = w->step % p.check == 0;
             if (out) w->print_checkpoint();
+            w->apply_springs();
             if (!out && p.num_blobs == 1) {
                 // a step with no output: forces and positions in one call (one graph launch
                 // once the iteration count is known); an inversion stops it before the update

@@ -28,6 +28,9 @@ constexpr int kMixedBlock = 256;     // ... of the FP32-mixed form's: its projec
 template <typename T>
 constexpr int row_block = std::is_same<T, double>::value ? kRowBlock : kMixedBlock;
 constexpr int kSlice = 64;           // rows per ELL slice (one wavefront)
+// T per element shape record: dpsi 1,2,3,5,6,7,9,10,11, vol, A, B and 4 padding, so that a
+// record is 64 (FP32) or 128 (FP64) aligned bytes that the assembly reads in one piece
+constexpr int kGeo = 16;
 constexpr int kChunkRows = 1024;     // rows per reduction chunk (one per thread of a row workgroup)
 constexpr uint32_t kIdentity = 0xFFFFFFFFu;
 constexpr int kFusedMaxRows = 1024;  // blobs up to this many rows solve in one launch (k_cg_fused)
@@ -204,7 +207,7 @@ struct ElemArgs {
     const T *vol0, *G, *E, *A, *B;
     double *last_det;             // [ne]
     T *force;                     // [ne][12]
-    T *geo;                       // [ne][10]: dpsi 1,2,3,5,6,7,9,10,11 then vol
+    T *geo;                       // [ne][kGeo]: dpsi 1,2,3,5,6,7,9,10,11, vol (A, B left as set)
     int calc_noise;
     double kT, dt;
     const int *elem_blob;         // [ne] blob of each element (null: one blob)
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
     T vol;
     if (ld * (double)det < 0) {
         __hip_atomic_fetch_add(&a.S[blob].inverted, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vol = a.geo[(size_t)10 * e + 9];  // the reference keeps the previous volume
+        vol = a.geo[(size_t)kGeo * e + 9];  // the reference keeps the previous volume
     } else {
         a.last_det[e] = (double)det;
         vol = (T)(1.0 / 6.0) * (T)fabs(det);
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) fo[k] = out[k];
     const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
-    T *go = a.geo + (size_t)10 * e;
+    T *go = a.geo + (size_t)kGeo * e;
 #pragma unroll
     for (int k = 0; k < 9; ++k) go[k] = d[gi[k]];
     go[9] = vol;
@@ -453,14 +456,42 @@ struct AsmArgs {
     const uint32_t *cptr;        // [nslots+1]
     const uint32_t *contrib;     // (e<<4)|(a<<2)|b, element order; kIdentity = identity block
     const unsigned char *diag_j; // [nrows] slot index j of the diagonal block
-    const T *geo;                // [ne][10]
-    const T *A, *B;
+    const T *geo;                // [ne][kGeo]
     T *val;                      // [nslots*9]
     double *dinv;                // tiled [nrows][3]
     const double *drag;          // [nrows] Stokes drag of unpinned rows, or null
     _Float16 *hval;              // [nslots*9] the multigrid preconditioner's scaled copy, or null
     double hscale;               // its scale
 };
+
+// the first 12 T of a shape record in 16-byte loads
+template <typename T>
+__device__ __forceinline__ void load_geo(const T *gr, T (&g)[12]) {
+    if constexpr (std::is_same<T, float>::value) {
+        const float4 *v = reinterpret_cast<const float4 *>(gr);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float4 x = v[i];
+            g[4 * i] = x.x; g[4 * i + 1] = x.y; g[4 * i + 2] = x.z; g[4 * i + 3] = x.w;
+        }
+    } else {
+        const double2 *v = reinterpret_cast<const double2 *>(gr);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double2 x = v[i];
+            g[2 * i] = x.x; g[2 * i + 1] = x.y;
+        }
+    }
+}
+
+// A and B of every element into its shape record (set up, and after new materials)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_geo_material(int ne, const T *A, const T *B, T *geo) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    geo[(size_t)kGeo * e + 10] = A[e];
+    geo[(size_t)kGeo * e + 11] = B[e];
+}
 
 // Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
 // reference's create_viscosity_matrix does (tetra_element_linear.cpp:298-327, macros
@@ -488,23 +519,21 @@ __device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int 
 
 // SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
 // 305-312): every block entry is the sum of its element sources in element order.
-// Grid (row blocks, widest slice): blockIdx.y is the slot j, so each lane follows the
-// source chain of one block only (the rows' slots are independent) - the shape for small
-// meshes. With a grid of height 1 each lane takes all slots of its row in turn: a
-// workgroup then reads the shape records of its rows' elements while they are cached,
-// the shape for large meshes (the elements' 16 sources otherwise fall in 16 passes over
-// the whole mesh). Either way each block is the same sum.
-template <typename T>
+// L lanes share a row, lane s taking the row's slots s, s + L, ...: L = 1 for large
+// meshes (a workgroup reads the shape records of its rows' elements while they are
+// cached; measured at 100^3 against L = 4 and 16, which read them together across lanes
+// but idle on the short off-diagonal chains), L = 16 for small ones (more lanes in
+// flight). Each block is the same sum whatever the mapping.
+template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma clang fp contract(off)
-    const int n = blockIdx.x * kBlock + threadIdx.x;
+    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int n = (int)(t / L), s = (int)(t % L);
     if (n >= a.nrows) return;
     const int sl = n / kSlice, lane = n % kSlice;
     const int W = a.slice_w[sl];
     const long long base = a.slice_base[sl];
-    const int j0 = gridDim.y > 1 ? (int)blockIdx.y : 0;
-    const int j1 = gridDim.y > 1 ? min(j0 + 1, W) : W;
-    for (int j = j0; j < j1; ++j) {
+    for (int j = s; j < W; j += L) {
         const long long slot = base + (long long)j * kSlice + lane;
         const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
         double acc[9];
@@ -517,8 +546,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
                 continue;
             }
             const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
-            T d[12];
-            const T *gr = a.geo + (size_t)10 * e;  // the element's shape record
+            T d[12], gr[12];
+            load_geo(a.geo + (size_t)kGeo * e, gr);  // the element's shape record
             d[1] = gr[0]; d[2] = gr[1]; d[3] = gr[2];
             d[5] = gr[3]; d[6] = gr[4]; d[7] = gr[5];
             d[9] = gr[6]; d[10] = gr[7]; d[11] = gr[8];
@@ -526,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
             d[0] = -(d[1] + d[2] + d[3]);
             d[4] = -(d[5] + d[6] + d[7]);
             d[8] = -(d[9] + d[10] + d[11]);
-            const T A = a.A[e], B = a.B[e];
+            const T A = gr[10], B = gr[11];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -619,25 +648,30 @@ __global__ __launch_bounds__(kRowBlock) void k_surface_forces(SurfArgs a) {
     }
 }
 
-// ---------------------------------------------------------------- rotational constant forces
-// Blob::apply_ctforces CTF 1 and 2 (Blob.cpp:1679-1719) for the nodes that carry a
-// rotational constant force, one lane per node: the node's linear constant forces (summed in
-// list order on the host) plus, in list order, each entry's pull round its axis at the
-// node's position this step (a midside node sits at .5 (a + b) of its edge ends,
-// tetra_element_linear.cpp:553-577). The total goes to the node's entry of the per-node
-// table read by the midside folds and, for a corner node, to its row's constant force.
-struct RotArgs {
+// ---------------------------------------------------------------- per-step node forces
+// The force of each node whose force is formed each step, one lane per node: its external
+// force (Blob::add_force_to_node calls before the step, Blob.cpp:1770-1774, summed in call
+// order on the host), then its linear constant forces one by one and then each rotational
+// one's pull round its axis at the node's position this step (Blob::apply_ctforces CTF 1
+// and 2, Blob.cpp:1679-1719), in list order. A midside node sits at .5 (a + b) of its edge
+// ends (tetra_element_linear.cpp:553-577). The total goes to the node's entry of the
+// per-node table read by the midside folds (when the handle forms folds each step) and,
+// for a corner node, to its row's constant force.
+struct DynArgs {
     int n;
     const int *node;        // [n] node
     const int *row;         // [n] its corner row, or -1
     const int *ends;        // [n][2] corner rows whose midpoint is its position
-    const double *lin;      // [n][3]
-    const int *ptr;         // [n+1] its entries
+    const int *ext;         // [n] its external force slot, or -1
+    const double *ext_force;// [slots][3]
+    const int *lptr;        // [n+1] its linear constant forces
+    const double *lval;     // [..][3]
+    const int *rptr;        // [n+1] its rotational forces
     const double *axis;     // [entries][6] point, unit direction
     const double *F;        // [entries]
     const int *torque;      // [entries] 1: constant torque ('t'), 0: 'f'
     const double *pos;      // [rows][3]
-    double *node_ctf;       // [nodes][3]
+    double *node_ctf;       // [nodes][3], or null
     double *ctf;            // [rows][3]
 };
 
@@ -688,18 +722,21 @@ __device__ inline void rotational_force(const double *x, const double *ax, doubl
     for (int c = 0; c < 3; ++c) f[c] = s * g[c];
 }
 
-__global__ __launch_bounds__(kBlock) void k_rotational_forces(RotArgs a) {
+__global__ __launch_bounds__(kBlock) void k_node_forces(DynArgs a) {
 #pragma clang fp contract(off)
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= a.n) return;
-    const int ra = a.ends[2 * k], rb = a.ends[2 * k + 1];
+    const int ra = a.ends[2 * k], rb = a.ends[2 * k + 1], x_slot = a.ext[k];
     double x[3], f[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         x[c] = ra == rb ? a.pos[3 * ra + c] : .5 * (a.pos[3 * ra + c] + a.pos[3 * rb + c]);
-        f[c] = a.lin[3 * k + c];
+        f[c] = x_slot >= 0 ? a.ext_force[3 * x_slot + c] : 0.0;
     }
-    for (int e = a.ptr[k]; e < a.ptr[k + 1]; ++e) {
+    for (int e = a.lptr[k]; e < a.lptr[k + 1]; ++e)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[c] += a.lval[3 * e + c];
+    for (int e = a.rptr[k]; e < a.rptr[k + 1]; ++e) {
         double g[3];
         rotational_force(x, a.axis + 6 * e, a.F[e], a.torque[e], g);
 #pragma unroll
@@ -708,9 +745,18 @@ __global__ __launch_bounds__(kBlock) void k_rotational_forces(RotArgs a) {
     const int v = a.node[k], row = a.row[k];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a.node_ctf[3 * v + c] = f[c];
+        if (a.node_ctf) a.node_ctf[3 * v + c] = f[c];
         if (row >= 0) a.ctf[3 * row + c] = f[c];
     }
+}
+
+// positions of selected rows (-1: none, zeros)
+__global__ __launch_bounds__(kBlock) void k_take_rows(int n, const int *rows, const double *pos, double *out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int r = rows[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * i + c] = r >= 0 ? pos[3 * r + c] : 0.0;
 }
 
 // ---------------------------------------------------------------- gather + CG start

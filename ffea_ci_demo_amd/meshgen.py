@@ -245,9 +245,11 @@ def write_blob_files(mesh: Mesh, prefix: str, material=(1500.0, 1e-3, 1e-3, 3.6e
 def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, num_steps=1000,
                       check=1000, rng_seed=2024, epsilon=1e-10, max_iterations_cg=10000,
                       calc_noise=1, calc_stokes=0, stokes_visc=1e-3, scale=1e-9, calc_ctforces=0,
-                      ctforces_fname=None, centroid=None):
+                      ctforces_fname=None, centroid=None, num_blobs=1, centroids=None, calc_springs=0,
+                      springs_fname=None):
     """A `.ffea` script in the reference's <param>/<system> layout
-    (FFEA_input_reader.cpp:116-196; keys of SimulationParams.cpp:239-776)."""
+    (FFEA_input_reader.cpp:116-196; keys of SimulationParams.cpp:239-776). With num_blobs > 1
+    every blob uses the same files, blob i at centroids[i] when given."""
     base = os.path.basename(blob_prefix)
     p = ["<param>", "\t<restart = 0>", f"\t<dt = {dt!r}>", f"\t<kT = {kT!r}>", f"\t<check = {check}>",
          f"\t<num_steps = {num_steps}>", f"\t<rng_seed = {rng_seed}>",
@@ -257,21 +259,38 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
          f"\t<calc_stokes = {calc_stokes}>", f"\t<stokes_visc = {stokes_visc!r}>",
          "\t<calc_ssint = 0>", "\t<calc_steric = 0>", f"\t<calc_noise = {calc_noise}>",
          "\t<calc_kinetics = 0>", "\t<calc_es = 0>", "\t<es_N_x = 0>", "\t<es_N_y = 0>", "\t<es_N_z = 0>",
-         f"\t<calc_ctforces = {calc_ctforces}>"]
-    p += ["\t<num_blobs = 1>", "\t<num_conformations = (1)>", "\t<num_states = (1)>", "</param>", "",
-          "<system>", "\t<blob>", "\t\t<conformation>", "\t\t\t<motion_state = DYNAMIC>",
-          f"\t\t\t<nodes = {base}.node>", f"\t\t\t<topology = {base}.top>", f"\t\t\t<surface = {base}.surf>",
-          f"\t\t\t<material = {base}.mat>", f"\t\t\t<stokes = {base}.stokes>", f"\t\t\t<vdw = {base}.vdw>",
-          f"\t\t\t<pin = {base}.pin>", "\t\t</conformation>", "\t\t<solver = CG_nomass>",
-          f"\t\t<scale = {scale!r}>"]
-    if centroid is not None:
-        p.append("\t\t<centroid = (%r,%r,%r)>" % tuple(centroid))
-    p += ["\t</blob>"]
-    if ctforces_fname:
-        p += ["\t<interactions>", "\t\t<ctforces>", f"\t\t\t<ctforces_fname = {ctforces_fname}>",
-              "\t\t</ctforces>", "\t</interactions>"]
+         f"\t<calc_ctforces = {calc_ctforces}>", f"\t<calc_springs = {calc_springs}>"]
+    ones = ",".join(["1"] * num_blobs)
+    p += [f"\t<num_blobs = {num_blobs}>", f"\t<num_conformations = ({ones})>", f"\t<num_states = ({ones})>",
+          "</param>", "", "<system>"]
+    for b in range(num_blobs):
+        p += ["\t<blob>", "\t\t<conformation>", "\t\t\t<motion_state = DYNAMIC>",
+              f"\t\t\t<nodes = {base}.node>", f"\t\t\t<topology = {base}.top>", f"\t\t\t<surface = {base}.surf>",
+              f"\t\t\t<material = {base}.mat>", f"\t\t\t<stokes = {base}.stokes>", f"\t\t\t<vdw = {base}.vdw>",
+              f"\t\t\t<pin = {base}.pin>", "\t\t</conformation>", "\t\t<solver = CG_nomass>",
+              f"\t\t<scale = {scale!r}>"]
+        c = centroids[b] if centroids is not None else (centroid if b == 0 else None)
+        if c is not None:
+            p.append("\t\t<centroid = (%r,%r,%r)>" % tuple(c))
+        p += ["\t</blob>"]
+    if ctforces_fname or springs_fname:
+        p += ["\t<interactions>"]
+        if ctforces_fname:
+            p += ["\t\t<ctforces>", f"\t\t\t<ctforces_fname = {ctforces_fname}>", "\t\t</ctforces>"]
+        if springs_fname:
+            p += ["\t\t<springs>", f"\t\t\t<springs_fname = {springs_fname}>", "\t\t</springs>"]
+        p += ["\t</interactions>"]
     p += ["</system>"]
     _w(path, p)
+
+
+def write_springs(path, springs):
+    """A springs file (World::load_springs, World.cpp:3543-3643): rows (k [N/m], l [m],
+    blob 0, blob 1, conformation 0, conformation 1, node 0, node 1)."""
+    lines = ["ffea springs file", f"num_springs {len(springs)}", "springs:"]
+    for k, l, b0, b1, n0, n1 in springs:
+        lines.append(f"{k!r} {l!r} {b0} {b1} 0 0 {n0} {n1}")
+    _w(path, lines)
 
 
 def write_surface_ctforces(path, sets, blob_index=0):

@@ -34,6 +34,7 @@ def _bind(L):
     L.ffea_script_rng_state.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
     L.ffea_script_blob_surface_forces.argtypes = [vp, C.c_int, ip, ip, ip, vp, vp, vp, vp]
     L.ffea_script_blob_rotational_forces.argtypes = [vp, C.c_int, ip, vp, vp, vp, vp]
+    L.ffea_script_springs.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp]
     L.ffea_gpu_blob_create_from_script.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L._script_bound = True
     return L
@@ -125,6 +126,21 @@ class Script:
         raw = typ.raw.decode("ascii")
         return node, force, axis, [raw[2 * i:2 * i + 2] for i in range(n.value)]
 
+    def springs(self):
+        """the script's springs, file order: (k, l, blob0, blob1, node0, node1, active), k and l
+        in meso units"""
+        L = _native.lib()
+        n = C.c_int()
+        L.ffea_script_springs(self.h, C.byref(n), None, None, None, None, None, None)
+        m = n.value
+        k, l = np.zeros(max(1, m)), np.zeros(max(1, m))
+        blob, conf, node = (np.zeros(max(2, 2 * m), np.int32) for _ in range(3))
+        act = np.zeros(max(1, m), np.int32)
+        L.ffea_script_springs(self.h, None, k.ctypes.data, l.ctypes.data, blob.ctypes.data, conf.ctypes.data,
+                              node.ctypes.data, act.ctypes.data)
+        return [(float(k[i]), float(l[i]), int(blob[2 * i]), int(blob[2 * i + 1]), int(node[2 * i]),
+                 int(node[2 * i + 1]), bool(act[i])) for i in range(m)]
+
     def blob_spec(self, b: int = 0) -> BlobSpec:
         a = self.blob_arrays(b)
         p = self.params()
@@ -135,7 +151,9 @@ class Script:
                         epsilon2=p["epsilon2"], max_iterations_cg=p["max_iterations_cg"], calc_noise=p["calc_noise"],
                         rng_state=self.rng_state(), pinned=a["pinned"], ctf_node=a["ctf_node"],
                         ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path,
-                        surface_forces=self.surface_forces(b), rotational_forces=self.rotational_forces(b))
+                        surface_forces=self.surface_forces(b), rotational_forces=self.rotational_forces(b),
+                        external_nodes=tuple(n for sp in self.springs() if sp[6]
+                                             for bb, n in ((sp[2], sp[4]), (sp[3], sp[5])) if bb == b))
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

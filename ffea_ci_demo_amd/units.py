@@ -10,6 +10,7 @@ import math
 LENGTH = 1.7e-10
 ENERGY = 4.141945559999999e-21
 MASS = 1.994307387553024e-26
+AREA = 2.89e-20
 VOLUME = 4.913e-30
 FORCE = 2.4364705882352941e-11
 TIME = 3.7302915560886126e-13

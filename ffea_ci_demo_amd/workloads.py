@@ -43,6 +43,7 @@ class BlobSpec:
     # linear surface constant forces: (face_nodes [nf][3], set_force [ns][3] meso, set_ptr, set_faces)
     surface_forces: Optional[tuple] = None
     rotational_forces: Optional[tuple] = None  # (node [n], force [n] meso, axis [n][6] meso, type [n] "pf"/"pt")
+    external_nodes: tuple = ()  # nodes that receive per-step external forces (springs)
 
     @property
     def num_elements(self):

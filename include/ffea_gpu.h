@@ -139,15 +139,18 @@ typedef struct {
 } ffea_gpu_rotational_forces;
 
 /* ffea_gpu_blob_create with surface and rotational constant forces (either may be NULL:
- * none). Position-dependent constant forces make a handle refuse
- * ffea_gpu_blob_run_checkpointed / ffea_gpu_blob_adjoint. */
+ * none) and the nodes that may receive per-step external forces through
+ * ffea_gpu_blob_add_node_forces (external_nodes [num_external_nodes], may be 0 / NULL; the
+ * nodes World::apply_springs pulls, World.cpp:4063-4108). Position-dependent constant
+ * forces make a handle refuse ffea_gpu_blob_run_checkpointed / ffea_gpu_blob_adjoint. */
 int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int num_elements,
                             const double *pos, const int *top, const double *A, const double *B,
                             const double *G, const double *E, int num_pinned, const int *pinned,
                             int num_ctf, const int *ctf_node, const double *ctf_force,
                             const double *stokes_drag, const uint32_t rng_state[6],
                             const ffea_gpu_surface_forces *surface_forces,
-                            const ffea_gpu_rotational_forces *rotational_forces, ffea_gpu_blob **out);
+                            const ffea_gpu_rotational_forces *rotational_forces, int num_external_nodes,
+                            const int *external_nodes, ffea_gpu_blob **out);
 
 /* An ensemble of independent blobs in one handle (no counterpart in the reference, which
  * steps the blobs of a World one after another, World.cpp:2031-2059). Blob b is given as
@@ -194,6 +197,17 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *blob);
 /* nsteps x { zero forces; update_internal_forces; update_positions } — the blob's part
  * of World::run (World.cpp:1786-2059). Stops at the first failing step. */
 int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
+
+/* Blob::add_force_to_node (Blob.cpp:1770-1774) for the next step only: force[i] (meso
+ * units) is added to node[i], in list order and across calls in call order, before the
+ * constant forces of the next step's force formation (World.cpp:1862-1910: forces are
+ * zeroed, the springs are applied, then update_internal_forces adds the constant forces).
+ * The nodes must be among the external_nodes the handle was created with. */
+int ffea_gpu_blob_add_node_forces(ffea_gpu_blob *blob, int num, const int *node, const double *force);
+
+/* Blob::get_node (Blob.cpp:1762-1764): positions [num][3] of the given nodes (midside
+ * nodes at their edge midpoints) */
+int ffea_gpu_blob_get_node_positions(ffea_gpu_blob *blob, int num, const int *node, double *pos);
 
 /* node state, [num_nodes][3] (midside nodes at their edge midpoints) */
 int ffea_gpu_blob_get_positions(ffea_gpu_blob *blob, double *pos);
@@ -256,7 +270,7 @@ int ffea_gpu_batch_set_rng_stream(ffea_gpu_blob *blob, int index, const uint32_t
 int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
 
 /* Differentiation of a blob trajectory (single-blob handles, calc_noise = 0, some nodes
- * pinned, no surface or rotational constant forces). No counterpart in the reference.
+ * pinned, no surface, rotational or external forces). No counterpart in the reference.
  *   ffea_gpu_blob_set_material   replaces the per-element A, B, G, E (meso units, .top order)
  *   ffea_gpu_blob_run_checkpointed  nsteps steps of ffea_gpu_blob_step from the current
  *                                state, keeping the positions of every interval-th step on

@@ -14,6 +14,7 @@
  *   .ctforces          Blob.cpp:2478-2806: linear node forces, rotational forces (axes
  *                      by point and direction; 'n' axes are refused) and linear surface
  *                      forces
+ *   .springs           World.cpp:2291-2305, 3543-3643 (springs between nodes of any blobs)
  *   rng_seed           World.cpp:259-295 (glibc srand/rand -> 6 seeds; stream 0)
  * Positions are placed as World::init leaves them: blob centroid moved to the
  * `centroid` tag if given (World.cpp:2748-2768), then the system centroid moved to
@@ -69,10 +70,18 @@ int ffea_script_blob_surface_forces(const ffea_script *s, int blob, int *num_fac
 int ffea_script_blob_rotational_forces(const ffea_script *s, int blob, int *num, int *node, double *force,
                                        double *axis, char *type);
 
+/* the springs of the script (calc_springs = 1), file order: k and l in meso units
+ * (k * area / Energy, l / length), blob / conf / node [num][2] for the two ends, active [num]
+ * (World::activate_springs: both ends in the first conformation). Any pointer may be NULL. */
+int ffea_script_springs(const ffea_script *s, int *num, double *k, double *l, int *blob, int *conf, int *node,
+                        int *active);
+
 /* the noise stream state of the single-thread run (rng[0] after SetPackageSeed) */
 int ffea_script_rng_state(const ffea_script *s, uint32_t state[6]);
 
-/* blob b of the script on the GPU (ffea_gpu_blob_create with the script's params) */
+/* blob b of the script on the GPU (ffea_gpu_blob_create with the script's params). The
+ * nodes active springs pull are the handle's external force nodes; the spring forces
+ * themselves are the World's to apply (ffea_world_run does, World::apply_springs). */
 int ffea_gpu_blob_create_from_script(const ffea_script *s, int blob, int precision, int device,
                                      ffea_gpu_blob **out);
 

@@ -474,6 +474,10 @@ struct orc_blob {
     double *rf_force;          /* [nrf] F / force unit */
     double *rf_axis;           /* [nrf][6] point / length unit, unit direction */
     int *rf_torque;            /* [nrf] 1: 't' (constant torque), 0: 'f' */
+    /* external forces for the next step (Blob::add_force_to_node) */
+    int next, next_cap;
+    int *ext_node;
+    double *ext_force;
 };
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
@@ -727,6 +731,7 @@ void orc_blob_destroy(orc_blob *b) {
     free(b->top); free(b->is_pinned); free(b->pinned); free(b->ctf_node);
     free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
     free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
+    free(b->ext_node); free(b->ext_force);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
@@ -1043,9 +1048,56 @@ int orc_blob_update_positions(orc_blob *b) {
 }
 
 /* one World::run iteration for one blob (World.cpp:1786-1866, 2031-2059) */
+int orc_blob_add_node_forces(orc_blob *b, int num, const int *node, const double *force) {
+    for (int i = 0; i < num; ++i)
+        if (node[i] < 0 || node[i] >= b->nt) return 1;
+    if (b->next + num > b->next_cap) {
+        const int cap = 2 * (b->next + num);
+        b->ext_node = realloc(b->ext_node, sizeof(int) * (size_t)cap);
+        b->ext_force = realloc(b->ext_force, sizeof(double) * 3 * (size_t)cap);
+        b->next_cap = cap;
+    }
+    for (int i = 0; i < num; ++i) {
+        b->ext_node[b->next + i] = node[i];
+        for (int c = 0; c < 3; ++c) b->ext_force[3 * (b->next + i) + c] = force[3 * i + c];
+    }
+    b->next += num;
+    return 0;
+}
+
+/* World::apply_springs for one spring (World.cpp:4070-4106) */
+int orc_spring_forces(const double *x0, const double *x1, double k, double l, double *f0, double *f1) {
+    double sep[3], sep_norm[3], sum = 0.0, mag = 0.0;
+    for (int c = 0; c < 3; ++c) sep[c] = x1[c] - x0[c];
+    for (int c = 0; c < 3; ++c) sum += sep[c] * sep[c];  /* normalize, mat_vec_fns_II.h:188-198 */
+    if (sum == 0) return 1;
+    sum = sqrt(sum);
+    for (int c = 0; c < 3; ++c) sep_norm[c] = sep[c] / sum;
+    for (int c = 0; c < 3; ++c) mag += sep[c] * sep[c];  /* magnitude, mat_vec_fns_II.h:326-332 */
+    const double force_mag = k * (sqrt(mag) - l);
+    for (int c = 0; c < 3; ++c) {
+        f0[c] = force_mag * sep_norm[c];
+        f1[c] = -1 * force_mag * sep_norm[c];
+    }
+    return 0;
+}
+
+/* World::make_measurements, one spring's energy (World.cpp:4645-4648) */
+double orc_spring_energy(const double *x0, const double *x1, double k, double l) {
+    double c[3], m = 0.0;
+    for (int i = 0; i < 3; ++i) c[i] = x0[i] - x1[i];
+    for (int i = 0; i < 3; ++i) m += c[i] * c[i];
+    m = sqrt(m);
+    return 0.5 * k * (m - l) * (m - l);
+}
+
 int orc_blob_step(orc_blob *b) {
     memset(b->node_force, 0, sizeof(double) * 30 * (size_t)b->ne);
     memset(b->force, 0, sizeof(double) * 3 * (size_t)b->nt);
+    /* forces added before the step (World::apply_springs, World.cpp:1910), in call order */
+    for (int i = 0; i < b->next; ++i)
+        for (int c = 0; c < 3; ++c) b->force[3 * b->ext_node[i] + c] += b->ext_force[3 * i + c];
+    b->next = 0;
     if (orc_blob_update_internal_forces(b) != 0) return 1;
     if (b->all_pinned) return 0;
     return orc_blob_update_positions(b) ? 2 : 0;

@@ -92,6 +92,16 @@ int orc_blob_set_rotational_forces(orc_blob *b, int num, const int *node, const 
 /* the force of one such entry on a node at x (Blob.cpp:1693-1714) */
 void orc_rotational_force(const double *x, const double *axis, double F, int torque, double *f);
 
+/* Blob::add_force_to_node (Blob.cpp:1770-1774) for the next orc_blob_step only, applied
+ * after its forces are zeroed and before the constant forces. Returns 0 or 1 (bad node). */
+int orc_blob_add_node_forces(orc_blob *b, int num, const int *node, const double *force);
+/* one spring of World::apply_springs (World.cpp:4063-4108): forces on its two nodes at x0
+ * and x1 for constant k and length l (meso units); returns 1 for coincident nodes (the
+ * reference's normalize throws there), else 0 */
+int orc_spring_forces(const double *x0, const double *x1, double k, double l, double *f0, double *f1);
+/* its energy 0.5 k (|x0 - x1| - l)^2 (World.cpp:4645-4648) */
+double orc_spring_energy(const double *x0, const double *x1, double k, double l);
+
 /* Blob.cpp:400-491 ; returns the number of inverted elements */
 int  orc_blob_update_internal_forces(orc_blob *b);
 /* Blob.cpp:493-508 ; returns 0, or 1 when CG does not converge */

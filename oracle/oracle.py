@@ -85,6 +85,12 @@ def lib():
         L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
         L.orc_blob_set_rotational_forces.restype = C.c_int
         L.orc_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_blob_add_node_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+        L.orc_blob_add_node_forces.restype = C.c_int
+        L.orc_spring_forces.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp, _dp]
+        L.orc_spring_forces.restype = C.c_int
+        L.orc_spring_energy.argtypes = [_dp, _dp, C.c_double, C.c_double]
+        L.orc_spring_energy.restype = C.c_double
         for f in ("update_internal_forces", "update_positions", "step", "num_nodes", "num_elements",
                   "last_cg_iterations", "matrix_nnz"):
             getattr(L, "orc_blob_" + f).argtypes = [C.c_void_p]
@@ -133,6 +139,10 @@ def ref_lib():
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_nomass_matrix.restype = C.c_int
         L.ref_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
+        L.ref_spring_forces.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp, _dp]
+        L.ref_spring_forces.restype = C.c_int
+        L.ref_spring_energy.argtypes = [_dp, _dp, C.c_double, C.c_double]
+        L.ref_spring_energy.restype = C.c_double
         _ref = L
     return _ref
 
@@ -145,6 +155,53 @@ def rotational_force(x, axis, F, torque, use_ref=False):
     fn = L.ref_rotational_force if use_ref else L.orc_rotational_force
     fn(np.ascontiguousarray(x, np.float64), np.ascontiguousarray(axis, np.float64), float(F), int(torque), out)
     return out
+
+
+def spring_forces(x0, x1, k, l, use_ref=False):
+    """Forces (f0, f1) of one spring on its nodes at x0 and x1 (World.cpp:4070-4106), or
+    None for coincident nodes: the oracle's restatement, or with use_ref the reference's
+    own vector functions (oracle/_ref)."""
+    L = ref_lib() if use_ref else lib()
+    f0, f1 = np.zeros(3), np.zeros(3)
+    fn = L.ref_spring_forces if use_ref else L.orc_spring_forces
+    if fn(np.ascontiguousarray(x0, np.float64), np.ascontiguousarray(x1, np.float64), float(k), float(l), f0, f1):
+        return None
+    return f0, f1
+
+
+def spring_energy(x0, x1, k, l, use_ref=False):
+    """0.5 k (|x0 - x1| - l)^2 (World.cpp:4645-4648)"""
+    L = ref_lib() if use_ref else lib()
+    fn = L.ref_spring_energy if use_ref else L.orc_spring_energy
+    return fn(np.ascontiguousarray(x0, np.float64), np.ascontiguousarray(x1, np.float64), float(k), float(l))
+
+
+def apply_springs(blobs, springs):
+    """World::apply_springs (World.cpp:4063-4108) over oracle blobs: springs is a list of
+    (k, l, blob0, blob1, node0, node1, active) in file order; each active spring's forces go
+    to its two blobs with Blob::add_force_to_node before their next step."""
+    for k, l, b0, b1, n0, n1, active in springs:
+        if not active:
+            continue
+        r = spring_forces(blobs[b0].positions()[n0], blobs[b1].positions()[n1], k, l)
+        if r is None:
+            raise ValueError("Unable to normalize a vector with length zero")
+        blobs[b0].add_node_forces([n0], r[0])
+        blobs[b1].add_node_forces([n1], r[1])
+
+
+def springs_energy(blobs, springs):
+    """World::make_measurements' spring energy: per (blob0, blob1) cell in file order, then
+    the cells row by row (World.cpp:4632-4652, 4110-4123)"""
+    nb = len(blobs)
+    cell = [[0.0] * nb for _ in range(nb)]
+    for k, l, b0, b1, n0, n1, _ in springs:
+        cell[b0][b1] += spring_energy(blobs[b0].positions()[n0], blobs[b1].positions()[n1], k, l)
+    e = 0.0
+    for i in range(nb):
+        for j in range(nb):
+            e += cell[i][j]
+    return e
 
 
 class Rng:
@@ -285,6 +342,13 @@ class Blob:
 
     def step(self):
         return lib().orc_blob_step(self.h)
+
+    def add_node_forces(self, nodes, forces):
+        """Blob::add_force_to_node for the next step (meso units, list order)"""
+        n = np.ascontiguousarray(nodes, np.int32).ravel()
+        f = np.ascontiguousarray(forces, np.float64).reshape(-1)
+        if f.size != 3 * n.size or lib().orc_blob_add_node_forces(self.h, n.size, n, f) != 0:
+            raise ValueError("oracle node forces rejected")
 
     def update_internal_forces(self):
         return lib().orc_blob_update_internal_forces(self.h)

@@ -133,4 +133,31 @@ void ref_rotational_force(const double *pos3, const double *axis6, double F, int
     for (int c = 0; c < 3; ++c) f3[c] = f[c];
 }
 
+// One spring as World::apply_springs forms it (World.cpp:4070-4106) with the reference's
+// normalize (throws for a zero vector: returns 1 here) and magnitude.
+int ref_spring_forces(const double *x0, const double *x1, double k, double l, double *f0, double *f1) {
+    arr3 n0{x0[0], x0[1], x0[2]}, n1{x1[0], x1[1], x1[2]}, sep, sep_norm;
+    sep[0] = n1[0] - n0[0];
+    sep[1] = n1[1] - n0[1];
+    sep[2] = n1[2] - n0[2];
+    try {
+        normalize(sep, sep_norm);
+    } catch (const FFEAException &) {
+        return 1;
+    }
+    const scalar force_mag = k * (magnitude(sep) - l);
+    for (int c = 0; c < 3; ++c) {
+        f0[c] = force_mag * sep_norm[c];
+        f1[c] = -1 * force_mag * sep_norm[c];
+    }
+    return 0;
+}
+
+// World::make_measurements' term for one spring (World.cpp:4645-4648)
+double ref_spring_energy(const double *x0, const double *x1, double k, double l) {
+    arr3 a{x0[0], x0[1], x0[2]}, b{x1[0], x1[1], x1[2]}, c;
+    sub(a, b, c);
+    return 0.5 * k * (magnitude(c) - l) * (magnitude(c) - l);
+}
+
 }  // extern "C"
