@@ -20,6 +20,7 @@ reference algorithm, single thread) on a bounded sample of the same workload.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--forms fp64,mixed]
   python bench.py --workload ensemble [--blobs 512]   # many blobs in one handle,
                                                       # graph-captured steps
+  python bench.py --workload masslumped               # the lumped-mass (inertial) solver
 """
 from __future__ import annotations
 
@@ -203,7 +204,8 @@ def main():
     ap.add_argument("--forms", default=DEFAULT_FORMS,
                     help="fp64 / mixed: the reference's Jacobi-PCG; +mg: multigrid-preconditioned CG")
     ap.add_argument("--headline", default=None, help="form reported as value (default: last of --forms)")
-    ap.add_argument("--workload", default="cube", choices=("cube", "ensemble", "adjoint", "illcond", "drift"))
+    ap.add_argument("--workload", default="cube",
+                    choices=("cube", "ensemble", "adjoint", "illcond", "drift", "masslumped"))
     ap.add_argument("--run-steps", type=int, default=None,
                     help="adjoint: trajectory length K (200); illcond/drift: steps compared (2000/1000)")
     ap.add_argument("--blobs", type=int, default=512, help="ensemble: number of blobs")
@@ -222,6 +224,8 @@ def main():
         return main_adjoint(args)
     if args.workload in ("illcond", "drift"):
         return main_drift(args)
+    if args.workload == "masslumped":
+        return main_masslumped(args)
     t0 = time.perf_counter()
     spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
     print(f"[bench] mesh n={args.n} built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
@@ -287,6 +291,85 @@ def main():
         "forms": {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items() if k != "info"}
                   for f, r in results.items()},
     }
+    print(json.dumps(out), flush=True)
+
+
+def element_pass_bytes(info, w, inertial):
+    """algorithmic bytes of one element-pass launch: per element the corner indices (16 B),
+    J^-1_0, vol_0, G, E, A, B (14 w), last_det read and written (16 B), forces (12 w) and the
+    shape record (16 w) written; per corner node its position (24 B) and, for the inertial
+    solver, its velocity (24 B), each read once"""
+    ne, nl = info["num_elements"], info["num_linear_nodes"]
+    return ne * (32 + 14 * w + 12 * w + 16 * w) + nl * (48 if inertial else 24)
+
+
+def main_masslumped(args):
+    """the lumped-mass solver (MassLumpedSolver) on the 6.0M-tet blob: explicit inertial
+    steps (element pass with the viscous term V v, gather, a = f/m, v += a dt, x += v dt)"""
+    from ffea_ci_demo_amd import _native, units, workloads
+    from ffea_ci_demo_amd.blob import GpuBlob
+    spec = workloads.cube_blob(args.n, name=f"cube{args.n}_{6 * args.n ** 3}tets")
+    rho = 1500.0 * units.VOLUME / units.MASS
+    forms = [f for f in args.forms.split(",") if f] if args.forms != DEFAULT_FORMS else ["fp64", "mixed"]
+    results, pos = {}, {}
+    for form in forms:
+        g = GpuBlob.from_spec(spec, precision=form)
+        g.set_solver("masslumped", rho)
+        info = g.info()
+        for _ in range(args.warmup):
+            g.step(1)
+        g.set_timing(("element",))
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        el_ms, el_n = g.timing()["element"]
+        w = 8 if form == "fp64" else 4
+        el_s = el_ms / max(el_n, 1) / 1e3
+        results[form] = {"element_steps_per_s": spec.num_elements * args.steps / wall,
+                         "ms_per_step": wall / args.steps * 1e3, "element_pass_avg_us": el_s * 1e6,
+                         "element_pass_alg_bytes": element_pass_bytes(info, w, True),
+                         "element_pass_achieved_GBs": element_pass_bytes(info, w, True) / el_s / 1e9}
+        pos[form] = g.positions()
+        print(f"[bench] masslumped form={form} {results[form]['ms_per_step']:.3f} ms/step, element pass "
+              f"{el_s * 1e6:.0f} us = {results[form]['element_pass_achieved_GBs']:.0f} GB/s", file=sys.stderr,
+              flush=True)
+        g.close()
+    if "fp64" in pos:
+        for f in forms:
+            if f != "fp64":
+                results[f]["position_rel_err_vs_fp64"] = float(np.abs(pos[f] - pos["fp64"]).max() /
+                                                               np.abs(pos["fp64"]).max())
+    head = args.headline or forms[-1]
+    h = results[head]
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        o = oracle.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+        o.set_mass_lumped(np.full(spec.num_elements, rho))
+        t0 = time.perf_counter()
+        assert o.step() == 0
+        t_step = time.perf_counter() - t0
+        del o
+        cpu = {"value": spec.num_elements / t_step, "unit": "element-steps/s", "cores": 1, "kind": "port",
+               "sample": f"one lumped-mass step of the same blob on the oracle (FP64 C restatement, 1 thread): "
+                         f"{t_step:.2f}s"}
+    out = _line(round(h["element_steps_per_s"], 1), "element-steps/s", round(h["ms_per_step"], 4), args.steps,
+                args.warmup, "f64" if head == "fp64" else "f32+f64",
+                {"workload": f"{spec.name}: {args.n}^3 jittered unit cubes x 6 quadratic tets, {spec.num_elements} "
+                             "tets, density 1500 kg/m^3, thermal noise at 300 K, dt 1e-14 s; lumped-mass solver "
+                             "(explicit inertial step, no linear solve)", "form": head,
+                 "parallelism": "single GPU"},
+                {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
+                 for f, r in results.items()})
+    out["roofline"] = {"bound": "hbm", "kernel": "k_element", "achieved": round(h["element_pass_achieved_GBs"], 1),
+                       "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": round(h["element_pass_achieved_GBs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                       "alg_bytes_per_launch": h["element_pass_alg_bytes"],
+                       "avg_launch_us": round(h["element_pass_avg_us"], 2)}
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

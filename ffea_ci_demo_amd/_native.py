@@ -34,10 +34,11 @@ EXPORTED = [
     "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
-    "ffea_gpu_blob_get_node_positions",
+    "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+SOLVER_MASSLUMPED, SOLVER_NOMASS_CG = 2, 3
 MG_MAX_LEVELS = 8
 
 
@@ -102,6 +103,8 @@ def lib():
                                           C.POINTER(RotationalForces), C.c_int, vp, C.POINTER(vp)]
     L.ffea_gpu_blob_add_node_forces.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_get_node_positions.argtypes = [vp, C.c_int, _ip, _dp]
+    L.ffea_gpu_blob_set_solver.argtypes = [vp, C.c_int, vp]
+    L.ffea_gpu_blob_set_velocities.argtypes = [vp, _dp]
     L.ffea_gpu_blob_destroy.argtypes = [vp]
     L.ffea_gpu_blob_destroy.restype = None
     for f in ("update_internal_forces", "update_positions", "synchronize"):

@@ -119,6 +119,19 @@ class _Handle:
     def synchronize(self):
         self._check(_native.lib().ffea_gpu_blob_synchronize(self.h))
 
+    def set_solver(self, kind, rho=None):
+        """\"masslumped\" (MassLumpedSolver, rho per element in meso units) or \"nomass\"
+        (the default mass-less CG)"""
+        L = _native.lib()
+        if kind == "masslumped":
+            r = np.asarray(rho, np.float64)
+            r = np.ascontiguousarray(np.broadcast_to(r, (self.ne,)) if r.ndim == 0 else r.ravel())
+            self._check(L.ffea_gpu_blob_set_solver(self.h, _native.SOLVER_MASSLUMPED, r.ctypes.data))
+        elif kind == "nomass":
+            self._check(L.ffea_gpu_blob_set_solver(self.h, _native.SOLVER_NOMASS_CG, None))
+        else:
+            raise ValueError(f"unknown solver {kind!r}")
+
 
 def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precision, device):
     prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
@@ -185,6 +198,17 @@ class GpuBlob(_Handle):
 
     @classmethod
     def from_spec(cls, spec, precision="fp64", **kw):
+        g = cls._from_spec(spec, precision, **kw)
+        if getattr(spec, "solver", "nomass") == "masslumped":
+            try:
+                g.set_solver("masslumped", spec.rho)
+            except Exception:
+                g.close()
+                raise
+        return g
+
+    @classmethod
+    def _from_spec(cls, spec, precision="fp64", **kw):
         return cls(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, dt=spec.dt, kT=spec.kT,
                    epsilon2=spec.epsilon2, max_iterations_cg=spec.max_iterations_cg, calc_noise=spec.calc_noise,
                    calc_stokes=spec.calc_stokes, pinned=spec.pinned, ctf_node=spec.ctf_node,
@@ -211,6 +235,9 @@ class GpuBlob(_Handle):
 
     def positions(self):
         return self._get("ffea_gpu_blob_get_positions", (self.nt, 3))
+
+    def set_velocities(self, v):
+        self._check(_native.lib().ffea_gpu_blob_set_velocities(self.h, np.ascontiguousarray(v, np.float64).reshape(-1)))
 
     def add_node_forces(self, nodes, forces):
         """Blob::add_force_to_node for the next step (meso units, list order)"""

@@ -639,6 +639,7 @@ struct Tables {
     std::vector<int> conn, sw, col, eblob, eoff, proj;
     std::vector<long long> sb;
     std::vector<uint32_t> cnt, contrib, ip, ic, fp, rng;
+    std::vector<uint32_t> inc_pos;       // [4][ne]: position of (element, corner) in the incidence lists
     std::vector<unsigned char> dj, pin;
     std::vector<double> pos, det, ctf, fv;
     std::vector<double> jinv0, vol0, A, B, G, E;  // [9][ne] and [ne], element order
@@ -903,6 +904,9 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         t.ip[n + 1] += t.ip[n];
         t.fp[n + 1] += t.fp[n];
     }
+    // the element pass writes each corner's force where the gather of its row reads it
+    t.inc_pos.assign((size_t)4 * ne, 0xFFFFFFFFu);
+    for (size_t k = 0; k < t.ic.size(); ++k) t.inc_pos[(size_t)(t.ic[k] & 3) * ne + (t.ic[k] >> 2)] = (uint32_t)k;
     t.nchunks = (int)t.chunks.size();
     // ---- noise jump tables: A^(7*64*g) per group of 64 elements, A^(7*l) per lane
     const int ngroups = std::max(1, (t.max_blob_ne + 63) / 64);
@@ -959,6 +963,10 @@ void validate(const Tables &t) {
         if (t.ip[n + 1] < t.ip[n]) fail("incidence order", n);
     for (size_t i = 0; i < t.ic.size(); ++i)
         if ((long long)(t.ic[i] >> 2) >= ne) fail("incidence element", (long long)i);
+    if ((long long)t.inc_pos.size() != 4 * ne) fail("incidence positions", (long long)t.inc_pos.size());
+    for (size_t i = 0; i < t.inc_pos.size(); ++i)  // each (element, corner) once in the lists
+        if ((long long)t.inc_pos[i] >= 4 * ne || t.ic[t.inc_pos[i]] != (uint32_t)(4 * (i % ne) + i / ne))
+            fail("incidence position", (long long)i);
     if (t.have_fold && !t.dyn_fold) {
         if ((long long)t.fp.size() != nrows + 1 || t.fp[0] != 0 || 3 * (size_t)t.fp[nrows] != t.fv.size())
             fail("fold ends", nrows);
@@ -1091,7 +1099,7 @@ struct ffea_gpu_blob {
     // device arrays
     DevBuf<int> conn, slice_w, col, elem_blob, elem_off, project;
     DevBuf<long long> slice_base;
-    DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, fold_ptr, rng;
+    DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, inc_pos, fold_ptr, rng;
     DevBuf<unsigned char> diag_j, pinned;
     DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
@@ -1130,6 +1138,9 @@ struct ffea_gpu_blob {
     bool ext_on_device = false;       // ext_force holds forces of the coming step (to clear after it)
     DevBuf<int> take_rows;            // ffea_gpu_blob_get_node_positions scratch
     DevBuf<double> take_out;
+    // MassLumpedSolver (ffea_gpu_blob_set_solver): explicit inertial steps
+    bool mass = false;
+    DevBuf<double> inv_m, rho_dev;     // [rows] 1 / lumped mass, [ne] densities
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
     bool fused_cg = false;
@@ -1268,6 +1279,8 @@ struct ffea_gpu_blob {
     void launch_assemble();
     void launch_assemble_gather();
     void prepare_ext();
+    int step_mass();
+    void set_solver(int kind, const double *rho);
     void mg_setup();
     void launch_mg_vcycle(double *Pout);
     void launch_mg_gauge();
@@ -1361,6 +1374,7 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     last_det.alloc(t.det.size(), &dev_bytes); last_det.upload(t.det, stream);
     inc_ptr.alloc(t.ip.size(), &dev_bytes); inc_ptr.upload(t.ip, stream);
     inc.alloc(t.ic.size(), &dev_bytes); inc.upload(t.ic, stream);
+    inc_pos.alloc(t.inc_pos.size(), &dev_bytes); inc_pos.upload(t.inc_pos, stream);
     pinned.alloc(t.pin.size(), &dev_bytes); pinned.upload(t.pin, stream);
     slice_w.alloc(t.sw.size(), &dev_bytes); slice_w.upload(t.sw, stream);
     slice_base.alloc(t.sb.size(), &dev_bytes); slice_base.upload(t.sb, stream);
@@ -1529,12 +1543,12 @@ void ffea_gpu_blob::launch_element(bool advance) {
     if (mixed) {
         ElemArgs<float> a{ne, conn.p, pos.p, f_jinv0.p, f_vol0.p, f_G.p, f_E.p, f_A.p, f_B.p, last_det.p, f_force.p,
                           f_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
-                          rng.p, S.p, rest.p};
+                          rng.p, S.p, rest.p, mass ? vel.p : nullptr, inc_pos.p};
         hipLaunchKernelGGL(k_element<float>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     } else {
         ElemArgs<double> a{ne, conn.p, pos.p, d_jinv0.p, d_vol0.p, d_G.p, d_E.p, d_A.p, d_B.p, last_det.p, d_force.p,
                            d_geo.p, prm.calc_noise, prm.kT, prm.dt, eb, elem_off.p, jump_group.p, jump_lane.p,
-                           rng.p, S.p, nullptr};  // FP64: the reference's arithmetic
+                           rng.p, S.p, nullptr, mass ? vel.p : nullptr, inc_pos.p};  // FP64: the reference's arithmetic
         hipLaunchKernelGGL(k_element<double>, dim3(grid_elems), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
@@ -1581,12 +1595,13 @@ void ffea_gpu_blob::launch_gather(const T *force) {
                     sf_entry_set.p, sf_fe_ptr.p, sf_fe.p, sf_area.p, sf_total.p, sf_face_force.p};
         hipLaunchKernelGGL(k_surface_forces, dim3(1), dim3(kRowBlock), 0, stream, sa);
     }
-    GatherArgs<T> g{row_args(), inc_ptr.p, inc.p, force, have_ctf ? ctf.p : nullptr,
+    GatherArgs<T> g{row_args(), inc_ptr.p, force, have_ctf ? ctf.p : nullptr,
                     have_fold ? fold_ptr.p : nullptr, have_fold ? fold.p : nullptr,
                     stokes_noise ? noise.p : nullptr, dyn_fold ? row_node.p : nullptr,
                     dyn_fold ? fold_mid.p : nullptr, dyn_fold && have_ctf ? node_ctf.p : nullptr,
                     have_sf ? sf_nf_ptr.p : nullptr, have_sf ? sf_nf.p : nullptr, have_sf ? sf_face_force.p : nullptr,
-                    pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p};
+                    pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p,
+                    mass && stokes ? vel.p : nullptr, mass && stokes ? drag.p : nullptr};
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
@@ -2086,8 +2101,65 @@ void ffea_gpu_blob::prepare_ext() {
     }
 }
 
+// One inertial step of the lumped-mass solver: element pass (with the viscous term),
+// gather (drag under calc_stokes, folds, pins), then a = f / m, v += a dt, x += v dt.
+int ffea_gpu_blob::step_mass() {
+    prepare_ext();
+    if (mixed) launch_gather<float, false>(f_force.p);
+    else launch_gather<double, false>(d_force.p);
+    hipLaunchKernelGGL(k_mass_update, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)rhs.p,
+                       (const double *)inv_m.p, vel.p, pos.p, prm.dt);
+    hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars) * nblob, hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    for (int b = 0; b < nblob; ++b) {
+        if (meta[b].all_pinned || h_S[b].inverted_step <= 0) continue;
+        err = (nblob > 1 ? fmt("blob %d: ", b) : std::string()) + inversion_message(h_S[b].inverted_step);
+        return FFEA_GPU_ERR_INVERSION;
+    }
+    return FFEA_GPU_OK;
+}
+
+void ffea_gpu_blob::set_solver(int kind, const double *rho_h) {
+    if (kind == FFEA_GPU_SOLVER_NOMASS_CG) {
+        mass = false;
+        inv_m.reset();
+        rho_dev.reset();
+        return;
+    }
+    if (kind != FFEA_GPU_SOLVER_MASSLUMPED)
+        throw Invalid(fmt("ffea_gpu_blob_set_solver: solver %d is not one this library runs (2 = MASSLUMPED, 3 = "
+                          "NOMASS_CG)", kind));
+    if (nblob != 1) throw Invalid("ffea_gpu_blob_set_solver: the lumped-mass solver needs a single-blob handle");
+    if (!rho_h) throw Invalid("ffea_gpu_blob_set_solver: the lumped-mass solver needs the element densities");
+    if (mg_on()) throw Invalid("ffea_gpu_blob_set_solver: switch the multigrid preconditioner off first");
+    for (int e = 0; e < ne; ++e)
+        if (!(rho_h[e] > 0.0) || !std::isfinite(rho_h[e]))
+            throw Invalid(fmt("ffea_gpu_blob_set_solver: element %d has density %g", e, rho_h[e]));
+    drop_graph();
+    rho_dev.reset();
+    rho_dev.alloc(ne, &dev_bytes);
+    rho_dev.upload(rho_h, ne, stream);
+    if (!inv_m.p) inv_m.alloc(nrows, &dev_bytes);
+    if (mixed)
+        hipLaunchKernelGGL(k_lumped_mass<float>, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), inc_ptr.p, inc.p,
+                           (const double *)rho_dev.p, (const float *)f_vol0.p, pinned.p, inv_m.p);
+    else
+        hipLaunchKernelGGL(k_lumped_mass<double>, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), inc_ptr.p, inc.p,
+                           (const double *)rho_dev.p, (const double *)d_vol0.p, pinned.p, inv_m.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    FFEA_HIP_TRY(hipMemsetAsync(vel.p, 0, sizeof(double) * vel.n, stream));  // nodes start at rest
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    mass = true;
+}
+
 int ffea_gpu_blob::step_once() {
     if (active_blobs == 0) return FFEA_GPU_OK;
+    if (mass) {
+        launch_element();
+        return step_mass();
+    }
     prepare_ext();
     if (use_graph && expect_iters > 0) {
         // enough iterations to cover the last step's count with a margin, in steps of 16
@@ -2155,6 +2227,7 @@ int ffea_gpu_blob::forward_checkpointed(int nsteps, int interval) {
     if (have_sf || have_dn)
         throw Invalid("a checkpointed run needs forces that do not depend on positions: the adjoint does not "
                       "differentiate surface, rotational or external forces");
+    if (mass) throw Invalid("a checkpointed run needs the mass-less solver");
     const size_t vec = (size_t)3 * nrows;
     const int nck = (nsteps + interval - 1) / interval;
     ensure(ckpt, vec * std::max(1, nck), &dev_bytes);
@@ -2514,6 +2587,7 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
+        if (b->mass) return b->step_mass();
         b->prepare_ext();
         b->launch_assemble_gather();
         b->run_cg_eager(0);
@@ -2530,6 +2604,15 @@ int ffea_gpu_blob_step(ffea_gpu_blob *b, int nsteps) {
             const int rc = b->step_once();
             if (rc != FFEA_GPU_OK) return rc;
         }
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_set_solver(ffea_gpu_blob *b, int solver, const double *rho) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        b->set_solver(solver, rho);
         return (int)FFEA_GPU_OK;
     });
 }
@@ -2637,6 +2720,21 @@ int ffea_gpu_batch_set_positions(ffea_gpu_blob *b, int blob, const double *in) {
     });
 }
 
+int ffea_gpu_blob_set_velocities(ffea_gpu_blob *b, const double *in) {
+    if (!b || !in) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        const BlobMeta &m = b->meta[0];
+        std::vector<double> v((size_t)3 * m.nl);
+        for (int n = 0; n < m.nl; ++n)
+            for (int c = 0; c < 3; ++c) v[3 * n + c] = in[3 * m.orig_of_lin[n] + c];
+        FFEA_HIP_TRY(hipMemcpyAsync(b->vel.p + 3 * (size_t)m.row_off, v.data(), sizeof(double) * v.size(),
+                                    hipMemcpyHostToDevice, b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
 static int get_node_vec(ffea_gpu_blob *b, int blob, const double *dev, double *out, bool tiled) {
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
@@ -2667,18 +2765,32 @@ int ffea_gpu_batch_get_element_forces(ffea_gpu_blob *b, int blob, double *out) {
     if (bad_blob(b, blob) || !out) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
+        // the blob's corner forces fill incidence positions [4 elem_off, 4 (elem_off + ne)) in
+        // row order; inc_pos says where each (element, corner) sits
         const BlobMeta &m = b->meta[blob];
         const size_t n = (size_t)12 * m.ne, off = (size_t)12 * m.elem_off;
+        std::vector<double> f(n);
         if (b->mixed) {
-            std::vector<float> f(n);
-            FFEA_HIP_TRY(hipMemcpyAsync(f.data(), b->f_force.p + off, sizeof(float) * n, hipMemcpyDeviceToHost,
+            std::vector<float> ff(n);
+            FFEA_HIP_TRY(hipMemcpyAsync(ff.data(), b->f_force.p + off, sizeof(float) * n, hipMemcpyDeviceToHost,
                                         b->stream));
             FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
-            for (size_t i = 0; i < n; ++i) out[i] = f[i];
+            for (size_t i = 0; i < n; ++i) f[i] = ff[i];
         } else {
-            FFEA_HIP_TRY(hipMemcpyAsync(out, b->d_force.p + off, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
-            FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+            FFEA_HIP_TRY(hipMemcpyAsync(f.data(), b->d_force.p + off, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                        b->stream));
         }
+        std::vector<uint32_t> ip((size_t)4 * m.ne);
+        for (int q = 0; q < 4; ++q)
+            FFEA_HIP_TRY(hipMemcpyAsync(ip.data() + (size_t)q * m.ne, b->inc_pos.p + (size_t)q * b->ne + m.elem_off,
+                                        sizeof(uint32_t) * m.ne, hipMemcpyDeviceToHost, b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        const size_t base = (size_t)4 * m.elem_off;
+        for (int e = 0; e < m.ne; ++e)
+            for (int q = 0; q < 4; ++q) {
+                const size_t k = ip[(size_t)q * m.ne + e] - base;
+                for (int i = 0; i < 3; ++i) out[12 * (size_t)e + 3 * q + i] = f[3 * k + i];
+            }
         return (int)FFEA_GPU_OK;
     });
 }
@@ -2968,6 +3080,7 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->mg_info = ffea_gpu_multigrid_info{};
             return (int)FFEA_GPU_OK;
         }
+        if (b->mass) throw Invalid("the multigrid preconditioner is for the mass-less solver");
         b->precond = FFEA_GPU_PRECOND_JACOBI;  // until the hierarchy is complete
         b->mg_setup();
         b->precond = FFEA_GPU_PRECOND_MULTIGRID;

@@ -267,6 +267,7 @@ struct BlobData {
     std::vector<double> rf_force;       // [n] meso
     std::vector<double> rf_axis;        // [n][6] meso point, unit direction
     std::vector<char> rf_type;          // [n][2]
+    bool masslumped = false;            // <solver = masslumped>: MassLumpedSolver
     double scale = 1.0;  // meso per file unit
     bool set_centroid = false;
     double centroid[3] = {0, 0, 0};
@@ -775,8 +776,12 @@ void load_script(ffea_script &S) {
                  "must be set.",
                  b);
         if (!set_scale) fail("Blob %d, scale not set.", b);
-        if (solver != "CG_nomass") fail("Blob %d: solver '%s' is not the mass-less CG solver of this build.", b,
-                                        solver.c_str());
+        // World.cpp:2600-2625
+        if (solver != "CG_nomass" && solver != "masslumped" && solver != "CG" && solver != "direct")
+            fail("In blob %d, unrecognised solver type.\nRecognised solvers:CG, CG_nomass, direct, masslumped.", b);
+        if (solver == "CG" || solver == "direct")
+            fail("Blob %d: solver '%s' is not part of this build (CG_nomass and masslumped are).", b, solver.c_str());
+        B.masslumped = solver == "masslumped";
         load_nodes(B, f_nodes);
         load_topology(B, f_top);
         load_surface(B, f_surf);
@@ -958,6 +963,12 @@ int ffea_script_blob_rotational_forces(const ffea_script *s, int b, int *num, in
     return FFEA_GPU_OK;
 }
 
+int ffea_script_blob_solver(const ffea_script *s, int b, int *solver) {
+    if (!s || !solver || b < 0 || b >= (int)s->blobs.size()) return FFEA_GPU_ERR_INVALID;
+    *solver = s->blobs[b].masslumped ? FFEA_GPU_SOLVER_MASSLUMPED : FFEA_GPU_SOLVER_NOMASS_CG;
+    return FFEA_GPU_OK;
+}
+
 int ffea_script_springs(const ffea_script *s, int *num, double *k, double *l, int *blob, int *conf, int *node,
                         int *active) {
     if (!s) return FFEA_GPU_ERR_INVALID;
@@ -1034,6 +1045,15 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
                                      p.calc_stokes ? drag.data() : nullptr, s->rng_state, nsets > 0 ? &sf : nullptr,
                                      B.rf_node.empty() ? nullptr : &rf, (int)ext.size(),
                                      ext.empty() ? nullptr : ext.data(), out);
+    if (rc == FFEA_GPU_OK && B.masslumped) {
+        rc = ffea_gpu_blob_set_solver(*out, FFEA_GPU_SOLVER_MASSLUMPED, B.rho.data());
+        if (rc != FFEA_GPU_OK) {
+            ffea_internal_set_error(ffea_gpu_blob_last_error(*out));
+            ffea_gpu_blob_destroy(*out);
+            *out = nullptr;
+            return rc;
+        }
+    }
     if (rc != FFEA_GPU_OK || s->blobs.size() == 1) return rc;
     // several blobs draw from the one stream in turn (World.cpp:2031-2058): all element
     // draws first, blob after blob, then (Stokes noise) all node draws, blob after blob

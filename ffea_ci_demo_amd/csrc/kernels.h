@@ -197,6 +197,30 @@ __device__ inline void ztz_from_moments(const double *m, double *M) {
         for (int l = 0; l < 3; ++l) M[6 * (3 + k) + 3 + l] = R[k][l];
 }
 
+// Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
+// reference's create_viscosity_matrix does (tetra_element_linear.cpp:298-327, macros
+// tetra_element_linear.h:56-117, del2 :585-622): upper triangle computed, the lower
+// one mirrored, everything times vol at the end.
+template <typename T>
+__device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
+#pragma clang fp contract(off)
+    int I = 4 * i, Jc = 4 * j, p = a, q = b;
+    if (I > Jc || (I == Jc && p > q)) {  // mirror to the upper triangle
+        int t = I; I = Jc; Jc = t;
+        t = p; p = q; q = t;
+    }
+    T v;
+    if (I == Jc) {
+        const T del2 = (d[p] * d[q] + d[4 + p] * d[4 + q] + d[8 + p] * d[8 + q]);
+        v = d[I + q] * d[I + p] * (A + B);
+        v += del2 * A;
+    } else {
+        if (p == q) v = (B + A) * (d[I + p] * d[Jc + p]);
+        else v = B * (d[I + p] * d[Jc + q]) + A * (d[I + q] * d[Jc + p]);
+    }
+    return v * vol;
+}
+
 // ---------------------------------------------------------------- element pass
 template <typename T>
 struct ElemArgs {
@@ -206,7 +230,7 @@ struct ElemArgs {
     const T *jinv0;               // [9][ne]
     const T *vol0, *G, *E, *A, *B;
     double *last_det;             // [ne]
-    T *force;                     // [ne][12]
+    T *force;                     // [4 ne][3]: each corner's force at its place in the incidence lists
     T *geo;                       // [ne][kGeo]: dpsi 1,2,3,5,6,7,9,10,11, vol (A, B left as set)
     int calc_noise;
     double kT, dt;
@@ -217,6 +241,8 @@ struct ElemArgs {
     const uint32_t *rng;          // [nblob][6]: stream state at the start of this pass
     CgScalars *S;                 // [nblob]
     const double *rest;           // [nl][3] rest positions (mixed form: cancellation-free stress)
+    const double *vel = nullptr;  // [nl][3] node velocities: inertial solvers' viscous term, or null
+    const uint32_t *inc_pos = nullptr;  // [4][ne] where corner q of element e goes in force
 };
 
 // Elastic stress without cancellation, for FP32: with H = F - I formed from displacements
@@ -385,18 +411,39 @@ __global__ __launch_bounds__(kBlock) void k_element(ElemArgs<T> a) {
         s[2][0] = s[0][2];
         s[2][1] = s[1][2];
     }
+    // inertial solvers: du = V v_e, the element viscosity matrix times its corner
+    // velocities (get_element_velocity_vector, :448-463; mat12_apply, mat_vec_fns.cpp:28-36)
+    T visc[12];
+#pragma unroll
+    for (int r = 0; r < 12; ++r) visc[r] = (T)0;
+    if (a.vel) {
+        const T A = a.A[e], B = a.B[e];
+        T v[12];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[4 * i + q] = (T)a.vel[3 * n[q] + i];
+        for (int r = 0; r < 12; ++r) {
+            T acc = (T)0;
+            for (int j = 0; j < 12; ++j) acc += visc_entry<T>(d, A, B, vol, r >> 2, r & 3, j >> 2, j & 3) * v[j];
+            visc[r] = acc;
+        }
+    }
     // apply_stress_tensor (:436-443) then node_force -= du (:468-483)
     T out[12];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const T du = (T)0 + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
+            const T du = visc[4 * i + q] + vol * (d[q] * s[i][0] + d[4 + q] * s[i][1] + d[8 + q] * s[i][2]);
             out[3 * q + i] = (T)0 - du;
         }
-    T *fo = a.force + (size_t)12 * e;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) fo[k] = out[k];
+    for (int q = 0; q < 4; ++q) {
+        T *fo = a.force + 3 * (size_t)a.inc_pos[(size_t)q * ne + e];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) fo[i] = out[3 * q + i];
+    }
     const int gi[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
     T *go = a.geo + (size_t)kGeo * e;
 #pragma unroll
@@ -493,29 +540,6 @@ __global__ __launch_bounds__(kBlock) void k_geo_material(int ne, const T *A, con
     geo[(size_t)kGeo * e + 11] = B[e];
 }
 
-// Entry (4i+a, 4j+b) of an element's 12x12 viscosity matrix, evaluated as the
-// reference's create_viscosity_matrix does (tetra_element_linear.cpp:298-327, macros
-// tetra_element_linear.h:56-117, del2 :585-622): upper triangle computed, the lower
-// one mirrored, everything times vol at the end.
-template <typename T>
-__device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
-#pragma clang fp contract(off)
-    int I = 4 * i, Jc = 4 * j, p = a, q = b;
-    if (I > Jc || (I == Jc && p > q)) {  // mirror to the upper triangle
-        int t = I; I = Jc; Jc = t;
-        t = p; p = q; q = t;
-    }
-    T v;
-    if (I == Jc) {
-        const T del2 = (d[p] * d[q] + d[4 + p] * d[4 + q] + d[8 + p] * d[8 + q]);
-        v = d[I + q] * d[I + p] * (A + B);
-        v += del2 * A;
-    } else {
-        if (p == q) v = (B + A) * (d[I + p] * d[Jc + p]);
-        else v = B * (d[I + p] * d[Jc + q]) + A * (d[I + q] * d[Jc + p]);
-    }
-    return v * vol;
-}
 
 // SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
 // 305-312): every block entry is the sum of its element sources in element order.
@@ -764,8 +788,7 @@ template <typename T>
 struct GatherArgs {
     RowArgs ra;
     const uint32_t *inc_ptr;    // [nrows+1]
-    const uint32_t *inc;        // e*4 + slot, element order
-    const T *force;             // [ne][12]
+    const T *force;             // [4 ne][3]: the corner forces of each row's elements, element order
     const double *ctf;          // [nrows][3] or null: constant forces summed in list order
     const uint32_t *fold_ptr;   // [nrows+1] or null
     const double *fold;         // [..][3]: linearise_force increments in element order (static)
@@ -782,6 +805,8 @@ struct GatherArgs {
     const double3 *centre;      // [nblob]
     double *rhs, *r, *z, *x;
     double *P;                  // chunk partials
+    const double *mass_vel = nullptr;   // inertial solvers under calc_stokes: [rows][3] velocities
+    const double *mass_drag = nullptr;  // and [rows] drags, for the drag force -drag v
 };
 
 // Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
@@ -807,8 +832,7 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
                 f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
             }
             for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
-                const uint32_t es = a.inc[k];
-                const T *src = a.force + (size_t)12 * (es >> 2) + 3 * (es & 3);
+                const T *src = a.force + 3 * (size_t)k;
                 f[0] += (double)src[0];
                 f[1] += (double)src[1];
                 f[2] += (double)src[2];
@@ -821,6 +845,10 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
                     f[1] += ff[1];
                     f[2] += ff[2];
                 }
+            }
+            if (a.mass_vel) {  // inertial solvers: -drag v before the noise (Blob.cpp:3185-3190)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[k] -= a.mass_vel[3 * n + k] * a.mass_drag[n];
             }
             if (a.fold_mid) {
                 // the node's own noise, then Blob::linearise_force (Blob.cpp:781-799): each
@@ -1322,6 +1350,48 @@ __global__ __launch_bounds__(kRowBlock) void k_update(RowArgs ra, const double *
                 vel[3 * n + k] = v;
                 pos[3 * n + k] += v * dt;
             }
+    }
+}
+
+// MassLumpedSolver::solve and the inertial Blob::euler_integrate (MassLumpedSolver.cpp:66-73,
+// Blob.cpp:3266-3280): a = f / m, v += a dt, x += v dt, for blobs without inverted elements
+__global__ __launch_bounds__(kRowBlock) void k_mass_update(RowArgs ra, const double *rhs, const double *inv_m,
+                                                        double *vel, double *pos, double dt) {
+#pragma clang fp contract(off)
+    const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = ra.chunks[c];
+        if (ra.S[ch.blob].inverted_step > 0) continue;
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double acc = rhs[vi(n, k)] * inv_m[n];
+                const double v = vel[3 * n + k] + acc * dt;
+                vel[3 * n + k] = v;
+                pos[3 * n + k] += v * dt;
+            }
+    }
+}
+
+// MassLumpedSolver::init (MassLumpedSolver.cpp:27-63): 1 / (sum over the row's elements, in
+// element order, of .25 rho vol_0); pinned rows 1
+template <typename T>
+__global__ __launch_bounds__(kRowBlock) void k_lumped_mass(RowArgs ra, const uint32_t *inc_ptr, const uint32_t *inc,
+                                                        const double *rho, const T *vol0, const unsigned char *pinned,
+                                                        double *inv_m) {
+#pragma clang fp contract(off)
+    const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = ra.chunks[c];
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
+            double m = 0.0;
+            for (uint32_t k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+                const uint32_t e = inc[k] >> 2;
+                m += .25 * rho[e] * (double)vol0[e];
+            }
+            if (pinned[n]) m = 1.0;
+            inv_m[n] = 1.0 / m;
+        }
     }
 }
 

@@ -246,7 +246,7 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
                       check=1000, rng_seed=2024, epsilon=1e-10, max_iterations_cg=10000,
                       calc_noise=1, calc_stokes=0, stokes_visc=1e-3, scale=1e-9, calc_ctforces=0,
                       ctforces_fname=None, centroid=None, num_blobs=1, centroids=None, calc_springs=0,
-                      springs_fname=None):
+                      springs_fname=None, solver="CG_nomass"):
     """A `.ffea` script in the reference's <param>/<system> layout
     (FFEA_input_reader.cpp:116-196; keys of SimulationParams.cpp:239-776). With num_blobs > 1
     every blob uses the same files, blob i at centroids[i] when given."""
@@ -267,7 +267,7 @@ def write_ffea_script(path, blob_prefix, *, dt=1e-14, kT=4.141945559999999e-21, 
         p += ["\t<blob>", "\t\t<conformation>", "\t\t\t<motion_state = DYNAMIC>",
               f"\t\t\t<nodes = {base}.node>", f"\t\t\t<topology = {base}.top>", f"\t\t\t<surface = {base}.surf>",
               f"\t\t\t<material = {base}.mat>", f"\t\t\t<stokes = {base}.stokes>", f"\t\t\t<vdw = {base}.vdw>",
-              f"\t\t\t<pin = {base}.pin>", "\t\t</conformation>", "\t\t<solver = CG_nomass>",
+              f"\t\t\t<pin = {base}.pin>", "\t\t</conformation>", f"\t\t<solver = {solver}>",
               f"\t\t<scale = {scale!r}>"]
         c = centroids[b] if centroids is not None else (centroid if b == 0 else None)
         if c is not None:

@@ -35,6 +35,7 @@ def _bind(L):
     L.ffea_script_blob_surface_forces.argtypes = [vp, C.c_int, ip, ip, ip, vp, vp, vp, vp]
     L.ffea_script_blob_rotational_forces.argtypes = [vp, C.c_int, ip, vp, vp, vp, vp]
     L.ffea_script_springs.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp]
+    L.ffea_script_blob_solver.argtypes = [vp, C.c_int, ip]
     L.ffea_gpu_blob_create_from_script.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L._script_bound = True
     return L
@@ -153,7 +154,15 @@ class Script:
                         ctf_force=a["ctf_force"], calc_stokes=p["calc_stokes"], stokes_drag=drag, name=self.path,
                         surface_forces=self.surface_forces(b), rotational_forces=self.rotational_forces(b),
                         external_nodes=tuple(n for sp in self.springs() if sp[6]
-                                             for bb, n in ((sp[2], sp[4]), (sp[3], sp[5])) if bb == b))
+                                             for bb, n in ((sp[2], sp[4]), (sp[3], sp[5])) if bb == b),
+                        solver=self.solver(b), rho=a["rho"])
+
+    def solver(self, b: int = 0) -> str:
+        L = _native.lib()
+        k = C.c_int()
+        if L.ffea_script_blob_solver(self.h, b, C.byref(k)) != _native.OK:
+            raise ScriptError(f"no blob {b}")
+        return "masslumped" if k.value == _native.SOLVER_MASSLUMPED else "nomass"
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

@@ -198,6 +198,23 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *blob);
  * of World::run (World.cpp:1786-2059). Stops at the first failing step. */
 int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
 
+/* The reference's solvers behind its Solver interface (Blob.cpp:228-243,
+ * FFEA_return_codes.h:35-38) that this library runs */
+#define FFEA_GPU_SOLVER_MASSLUMPED 2
+#define FFEA_GPU_SOLVER_NOMASS_CG 3
+
+/* Chooses the blob's solver (single-blob handles). NOMASS_CG (the default) is the
+ * mass-less viscous solve. MASSLUMPED is MassLumpedSolver (MassLumpedSolver.cpp:27-73)
+ * with the inertial time step it implies: each corner node's mass is the sum over its
+ * elements, in element order, of rho vol_0 / 4 (rho [num_elements] in meso units, .mat
+ * order); midside and pinned nodes have mass 1. A step then forms the element forces with
+ * the viscous term V v_e from the node velocities (Blob.cpp:460-466), subtracts drag v from
+ * every node under calc_stokes (Blob.cpp:3172-3196), and updates a = f / m, v += a dt,
+ * x += v dt (Blob.cpp:3249-3280): no linear solve. Switching sets the velocities to zero;
+ * rho may be NULL for NOMASS_CG. The multigrid preconditioner, graphs and the adjoint are
+ * for NOMASS_CG only. */
+int ffea_gpu_blob_set_solver(ffea_gpu_blob *blob, int solver, const double *rho);
+
 /* Blob::add_force_to_node (Blob.cpp:1770-1774) for the next step only: force[i] (meso
  * units) is added to node[i], in list order and across calls in call order, before the
  * constant forces of the next step's force formation (World.cpp:1862-1910: forces are
@@ -213,6 +230,9 @@ int ffea_gpu_blob_get_node_positions(ffea_gpu_blob *blob, int num, const int *no
 int ffea_gpu_blob_get_positions(ffea_gpu_blob *blob, double *pos);
 int ffea_gpu_blob_set_positions(ffea_gpu_blob *blob, const double *pos);
 int ffea_gpu_blob_get_velocities(ffea_gpu_blob *blob, double *vel);
+/* node velocities [num_nodes][3] (corner nodes; the inertial solver's state, read back by a
+ * restart, Blob.cpp:1002-1010) */
+int ffea_gpu_blob_set_velocities(ffea_gpu_blob *blob, const double *vel);
 
 /* solver right-hand side of the last update_positions ([num_nodes][3], after gather,
  * linearise_force and pin zeroing — the `force` vector Blob.cpp:3243 hands to solve) */

@@ -70,6 +70,9 @@ int ffea_script_blob_surface_forces(const ffea_script *s, int blob, int *num_fac
 int ffea_script_blob_rotational_forces(const ffea_script *s, int blob, int *num, int *node, double *force,
                                        double *axis, char *type);
 
+/* blob b's solver (World.cpp:2600-2625): FFEA_GPU_SOLVER_NOMASS_CG or _MASSLUMPED */
+int ffea_script_blob_solver(const ffea_script *s, int blob, int *solver);
+
 /* the springs of the script (calc_springs = 1), file order: k and l in meso units
  * (k * area / Energy, l / length), blob / conf / node [num][2] for the two ends, active [num]
  * (World::activate_springs: both ends in the first conformation). Any pointer may be NULL. */

@@ -425,10 +425,12 @@ static void elem_noise_stress(double vol, double A, double B, double kT, double 
 
 /* tetra_element_linear.cpp:436-443 then add_element_force_vector :468-483
  * (du starts at zero for the mass-less solver, Blob.cpp:462-467) */
+static double vol_term(const double d[12], double vol, const double s[3][3], int i, int a) {
+    return vol * (d[a] * s[i][0] + d[4 + a] * s[i][1] + d[8 + a] * s[i][2]);
+}
 static void elem_apply_stress(const double d[12], double vol, const double s[3][3], double du[12]) {
     for (int i = 0; i < 3; ++i)
-        for (int a = 0; a < 4; ++a)
-            du[4 * i + a] = 0 + vol * (d[a] * s[i][0] + d[4 + a] * s[i][1] + d[8 + a] * s[i][2]);
+        for (int a = 0; a < 4; ++a) du[4 * i + a] = 0 + vol_term(d, vol, s, i, a);
 }
 
 /* ======================================================================
@@ -474,6 +476,9 @@ struct orc_blob {
     double *rf_force;          /* [nrf] F / force unit */
     double *rf_axis;           /* [nrf][6] point / length unit, unit direction */
     int *rf_torque;            /* [nrf] 1: 't' (constant torque), 0: 'f' */
+    /* MassLumpedSolver (Blob.cpp:232-234): inertial steps */
+    int mass;
+    double *inv_m;             /* [nt] */
     /* external forces for the next step (Blob::add_force_to_node) */
     int next, next_cap;
     int *ext_node;
@@ -731,7 +736,7 @@ void orc_blob_destroy(orc_blob *b) {
     free(b->top); free(b->is_pinned); free(b->pinned); free(b->ctf_node);
     free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
     free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
-    free(b->ext_node); free(b->ext_force);
+    free(b->ext_node); free(b->ext_force); free(b->inv_m);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
@@ -766,6 +771,23 @@ int orc_blob_update_internal_forces(orc_blob *b) {
         }
         double du[12];
         elem_apply_stress(d, b->vol[e], s, du);
+        if (b->mass) {
+            /* Blob.cpp:460-466 with mass: du = V v_e (get_element_velocity_vector,
+             * tetra_element_linear.cpp:448-463; mat12_apply, mat_vec_fns.cpp:28-36), then the
+             * stress terms are added to it */
+            const int *nn = b->top + 10 * e;
+            const double(*V)[12] = (const double(*)[12])(b->V + 144 * e);
+            double v[12], tv[12];
+            for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 3; ++c) v[4 * c + a] = b->vel[3 * nn[a] + c];
+            for (int i = 0; i < 12; ++i) {
+                tv[i] = 0;
+                for (int j = 0; j < 12; ++j) tv[i] += V[i][j] * v[j];
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int a = 0; a < 4; ++a)
+                    du[4 * i + a] = tv[4 * i + a] + vol_term(d, b->vol[e], s, i, a);
+        }
         double *out = b->node_force + 30 * e;
         for (int a = 0; a < 4; ++a)
             for (int c = 0; c < 3; ++c) out[3 * a + c] -= du[4 * c + a];
@@ -1024,7 +1046,16 @@ int orc_blob_update_positions(orc_blob *b) {
             const int n = b->sf_face_nodes[3 * f + i];
             for (int c = 0; c < 3; ++c) F[3 * n + c] += b->sf_face_force[3 * f + c];
         }
-    if (b->p.calc_stokes == 1 && b->p.calc_noise == 1) {
+    if (b->p.calc_stokes == 1 && b->mass) {  /* Blob.cpp:3172-3196 */
+        for (int i = 0; i < nt; ++i) {
+            for (int c = 0; c < 3; ++c) F[3 * i + c] -= b->vel[3 * i + c] * b->stokes_drag[i];
+            if (b->p.calc_noise == 1) {
+                const double sc = sqrt((24 * b->p.kT * b->stokes_drag[i]) / (b->p.dt));
+                for (int c = 0; c < 3; ++c)
+                    F[3 * i + c] -= ((-.5) + ((.5) - (-.5)) * orc_rng_rand_u01(&b->rng)) * sc;
+            }
+        }
+    } else if (b->p.calc_stokes == 1 && b->p.calc_noise == 1) {
         for (int i = 0; i < nt; ++i) {
             const double sc = sqrt((24 * b->p.kT * b->stokes_drag[i]) / (b->p.dt));
             for (int c = 0; c < 3; ++c) F[3 * i + c] -= ((-.5) + ((.5) - (-.5)) * orc_rng_rand_u01(&b->rng)) * sc;
@@ -1034,6 +1065,19 @@ int orc_blob_update_positions(orc_blob *b) {
     for (int k = 0; k < b->npin; ++k)
         for (int c = 0; c < 3; ++c) F[3 * b->pinned[k] + c] = 0;
     memcpy(b->rhs, F, sizeof(double) * 3 * nt);
+    if (b->mass) {
+        /* MassLumpedSolver::solve (MassLumpedSolver.cpp:66-73), then the inertial
+         * euler_integrate (Blob.cpp:3266-3280) */
+        for (int i = 0; i < nt; ++i)
+            for (int c = 0; c < 3; ++c) F[3 * i + c] = F[3 * i + c] * b->inv_m[i];
+        for (int i = 0; i < nt; ++i)
+            for (int c = 0; c < 3; ++c) {
+                b->vel[3 * i + c] += F[3 * i + c] * b->p.dt;
+                b->pos[3 * i + c] += b->vel[3 * i + c] * b->p.dt;
+            }
+        for (int e = 0; e < b->ne; ++e) linearise_element(b, e);
+        return 0;
+    }
     /* solver->solve(force): build V from the element matrices of this step */
     matrix_build(b);
     int it = cg_solve(b, F);
@@ -1048,6 +1092,27 @@ int orc_blob_update_positions(orc_blob *b) {
 }
 
 /* one World::run iteration for one blob (World.cpp:1786-1866, 2031-2059) */
+/* MassLumpedSolver::init (MassLumpedSolver.cpp:27-63) */
+int orc_blob_set_mass_lumped(orc_blob *b, const double *rho) {
+    for (int e = 0; e < b->ne; ++e)
+        if (!(rho[e] > 0)) return 1;
+    free(b->inv_m);
+    b->inv_m = xcalloc((size_t)b->nt, sizeof(double));
+    for (int i = 0; i < b->nt; ++i) b->inv_m[i] = 0;
+    for (int e = 0; e < b->ne; ++e) {
+        const int *n = b->top + 10 * e;
+        for (int i = 0; i < 10; ++i) {
+            if (i < 4) b->inv_m[n[i]] += .25 * rho[e] * b->vol0[e];
+            else b->inv_m[n[i]] = 1;
+        }
+    }
+    for (int k = 0; k < b->npin; ++k) b->inv_m[b->pinned[k]] = 1.0;
+    for (int i = 0; i < b->nt; ++i) b->inv_m[i] = 1.0 / b->inv_m[i];
+    memset(b->vel, 0, sizeof(double) * 3 * (size_t)b->nt);
+    b->mass = 1;
+    return 0;
+}
+
 int orc_blob_add_node_forces(orc_blob *b, int num, const int *node, const double *force) {
     for (int i = 0; i < num; ++i)
         if (node[i] < 0 || node[i] >= b->nt) return 1;
@@ -1108,6 +1173,7 @@ int orc_blob_num_elements(const orc_blob *b) { return b->ne; }
 void orc_blob_get_positions(const orc_blob *b, double *out) { memcpy(out, b->pos, sizeof(double) * 3 * b->nt); }
 void orc_blob_set_positions(orc_blob *b, const double *in) { memcpy(b->pos, in, sizeof(double) * 3 * b->nt); }
 void orc_blob_get_velocities(const orc_blob *b, double *out) { memcpy(out, b->vel, sizeof(double) * 3 * b->nt); }
+void orc_blob_set_velocities(orc_blob *b, const double *in) { memcpy(b->vel, in, sizeof(double) * 3 * b->nt); }
 void orc_blob_get_rhs(const orc_blob *b, double *out) { memcpy(out, b->rhs, sizeof(double) * 3 * b->nt); }
 void orc_blob_get_element_forces(const orc_blob *b, double *out) {
     for (int e = 0; e < b->ne; ++e) memcpy(out + 12 * e, b->node_force + 30 * e, sizeof(double) * 12);

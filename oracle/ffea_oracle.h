@@ -92,6 +92,10 @@ int orc_blob_set_rotational_forces(orc_blob *b, int num, const int *node, const 
 /* the force of one such entry on a node at x (Blob.cpp:1693-1714) */
 void orc_rotational_force(const double *x, const double *axis, double F, int torque, double *f);
 
+/* MassLumpedSolver (MassLumpedSolver.cpp:27-73) in place of the mass-less CG: rho [ne]
+ * densities (meso units); velocities are set to zero. Returns 0 or 1 (bad density). */
+int orc_blob_set_mass_lumped(orc_blob *b, const double *rho);
+
 /* Blob::add_force_to_node (Blob.cpp:1770-1774) for the next orc_blob_step only, applied
  * after its forces are zeroed and before the constant forces. Returns 0 or 1 (bad node). */
 int orc_blob_add_node_forces(orc_blob *b, int num, const int *node, const double *force);
@@ -115,6 +119,7 @@ int  orc_blob_num_elements(const orc_blob *b);
 void orc_blob_get_positions(const orc_blob *b, double *out);      /* [nt][3] */
 void orc_blob_set_positions(orc_blob *b, const double *in);       /* [nt][3] */
 void orc_blob_get_velocities(const orc_blob *b, double *out);     /* [nt][3] */
+void orc_blob_set_velocities(orc_blob *b, const double *in);
 /* the solver right-hand side of the last update_positions (after gather,
  * stokes noise, linearise_force and pin zeroing) */
 void orc_blob_get_rhs(const orc_blob *b, double *out);            /* [nt][3] */

@@ -85,6 +85,8 @@ def lib():
         L.orc_blob_set_rotational_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, _ip]
         L.orc_blob_set_rotational_forces.restype = C.c_int
         L.orc_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
+        L.orc_blob_set_mass_lumped.argtypes = [C.c_void_p, _dp]
+        L.orc_blob_set_mass_lumped.restype = C.c_int
         L.orc_blob_add_node_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.orc_blob_add_node_forces.restype = C.c_int
         L.orc_spring_forces.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -95,7 +97,7 @@ def lib():
                   "last_cg_iterations", "matrix_nnz"):
             getattr(L, "orc_blob_" + f).argtypes = [C.c_void_p]
             getattr(L, "orc_blob_" + f).restype = C.c_int
-        for f in ("get_positions", "set_positions", "get_velocities", "get_rhs", "get_element_forces",
+        for f in ("get_positions", "set_positions", "get_velocities", "set_velocities", "get_rhs", "get_element_forces",
                   "get_inverse_diagonal"):
             getattr(L, "orc_blob_" + f).argtypes = [C.c_void_p, _dp]
         L.orc_blob_get_rng_state.argtypes = [C.c_void_p, _up]
@@ -343,6 +345,12 @@ class Blob:
     def step(self):
         return lib().orc_blob_step(self.h)
 
+    def set_mass_lumped(self, rho):
+        """MassLumpedSolver in place of the mass-less CG (rho per element, meso units)"""
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (self.ne,)))
+        if lib().orc_blob_set_mass_lumped(self.h, r) != 0:
+            raise ValueError("oracle densities rejected")
+
     def add_node_forces(self, nodes, forces):
         """Blob::add_force_to_node for the next step (meso units, list order)"""
         n = np.ascontiguousarray(nodes, np.int32).ravel()
@@ -366,6 +374,9 @@ class Blob:
 
     def set_positions(self, p):
         lib().orc_blob_set_positions(self.h, np.ascontiguousarray(p, np.float64).reshape(-1))
+
+    def set_velocities(self, v):
+        lib().orc_blob_set_velocities(self.h, np.ascontiguousarray(v, np.float64).reshape(-1))
 
     def velocities(self):
         return self._get("orc_blob_get_velocities", (self.nt, 3))

@@ -1,0 +1,87 @@
+"""The oracle's lumped-mass solver (MassLumpedSolver.cpp:27-73, with the inertial parts of
+Blob::update_internal_forces, aggregate_forces_and_solve and euler_integrate:
+Blob.cpp:460-466, 3172-3196, 3266-3280), pinned by what the mechanics fixes:
+- momentum: internal elastic, viscous and noise forces sum to zero per element, so with a
+  constant external load F on a free blob the momentum sum_i m_i v_i grows by N dt F over
+  N steps, with m_i the lumped masses (sum of rho vol_0 / 4 over a node's elements)
+  computed here from the mesh;
+- a rigid rotation at rest positions carries no viscous force (V kills rigid modes), a
+  uniform shear rate does;
+- dissipation: from a stretched state with noise off the total energy (kinetic with the
+  lumped masses, plus strain) falls, step after step: a viscous term of the wrong sign
+  would feed it.
+"""
+import numpy as np
+
+from ffea_ci_demo_amd import workloads
+
+
+def free_spec(n=3, noise=0):
+    spec = workloads.cube_blob(n, calc_noise=noise, jitter=0.1, name=f"cube{n}_mass")
+    spec.pinned = np.zeros(0, np.int32)
+    rho = np.full(spec.num_elements, 1500.0 * 4.913e-30 / 1.994307387553024e-26)
+    return spec, rho
+
+
+def test_momentum_under_a_constant_load(oracle_mod):
+    spec, rho = free_spec()
+    nt = spec.pos.shape[0]
+    F = np.array([2e-3, -1e-3, 3e-3])
+    spec.ctf_node = np.arange(nt, dtype=np.int32)
+    spec.ctf_force = np.tile(F, (nt, 1))
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_lumped(rho)
+    _, vol0 = o.rest_state()
+    m = np.zeros(nt)
+    for e in range(spec.num_elements):
+        for a in range(4):
+            m[spec.top[e, a]] += 0.25 * rho[e] * vol0[e]
+    N = 20
+    for _ in range(N):
+        assert o.step() == 0
+    P = (m[:, None] * o.velocities()).sum(axis=0)
+    want = N * spec.dt * nt * F
+    assert np.abs(P - want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_rigid_rotation_is_force_free(oracle_mod):
+    spec, rho = free_spec()
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_lumped(rho)
+    x = o.positions()
+    c = x.mean(axis=0)
+    w = np.array([0.3, -0.2, 0.5])
+    o.set_velocities(np.cross(w, x - c))
+    assert o.step() == 0
+    rot = np.abs(o.rhs()).max()
+    o2 = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o2.set_mass_lumped(rho)
+    L = np.array([[0.0, 0.3, 0.0], [0.3, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    o2.set_velocities((x - c) @ L.T)
+    assert o2.step() == 0
+    shear = np.abs(o2.rhs()).max()
+    assert shear > 0
+    assert rot <= 1e-12 * shear * (np.abs(w).max() / 0.3)
+
+
+def test_viscous_dissipation(oracle_mod):
+    spec, rho = free_spec()
+    spec.pos = spec.pos * np.array([1.02, 0.99, 1.0])  # stretched: the rest state stays the original
+    o = oracle_mod.Blob(workloads.cube_blob(3, calc_noise=0, jitter=0.1).pos, spec.top, spec.A, spec.B, spec.G,
+                        spec.E, **spec.oracle_kwargs())
+    o.set_mass_lumped(rho)
+    o.set_positions(spec.pos)
+    _, vol0 = o.rest_state()
+    nt = spec.pos.shape[0]
+    m = np.zeros(nt)
+    for e in range(spec.num_elements):
+        for a in range(4):
+            m[spec.top[e, a]] += 0.25 * rho[e] * vol0[e]
+    energies = []
+    for _ in range(40):
+        v = o.velocities()
+        energies.append(0.5 * (m[:, None] * v * v).sum() + o.strain_energy())
+        assert o.step() == 0
+    energies = np.array(energies)
+    assert energies[-1] < 0.99 * energies[0]
+    assert np.all(np.diff(energies) <= 1e-9 * energies[0])
