@@ -35,6 +35,7 @@ EXPORTED = [
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
+    "ffea_gpu_blob_kinetic_energy",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -105,6 +106,7 @@ def lib():
     L.ffea_gpu_blob_get_node_positions.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_set_solver.argtypes = [vp, C.c_int, vp]
     L.ffea_gpu_blob_set_velocities.argtypes = [vp, _dp]
+    L.ffea_gpu_blob_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
     L.ffea_gpu_blob_destroy.argtypes = [vp]
     L.ffea_gpu_blob_destroy.restype = None
     for f in ("update_internal_forces", "update_positions", "synchronize"):

@@ -236,6 +236,11 @@ class GpuBlob(_Handle):
     def positions(self):
         return self._get("ffea_gpu_blob_get_positions", (self.nt, 3))
 
+    def kinetic_energy(self):
+        e = C.c_double()
+        self._check(_native.lib().ffea_gpu_blob_kinetic_energy(self.h, C.byref(e)))
+        return e.value
+
     def set_velocities(self, v):
         self._check(_native.lib().ffea_gpu_blob_set_velocities(self.h, np.ascontiguousarray(v, np.float64).reshape(-1)))
 

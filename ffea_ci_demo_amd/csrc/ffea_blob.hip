@@ -2850,6 +2850,37 @@ int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *b, int blob, long long of
     });
 }
 
+int ffea_gpu_blob_kinetic_energy(ffea_gpu_blob *b, double *energy) {
+    if (!b || !energy) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        *energy = 0.0;
+        if (!b->mass) return (int)FFEA_GPU_OK;  // the mass-less solver has none (Blob.cpp:1079)
+        if (!b->energy_w.p) {
+            b->energy_w.alloc(b->ne, &b->dev_bytes);
+            b->energy_out.alloc(b->nblob, &b->dev_bytes);
+            std::vector<int> cnt(b->nblob);
+            for (int i = 0; i < b->nblob; ++i) cnt[i] = b->meta[i].ne;
+            b->elem_cnt.alloc(b->nblob, &b->dev_bytes);
+            b->elem_cnt.upload(cnt, b->stream);
+        }
+        if (b->mixed)
+            hipLaunchKernelGGL(k_kinetic_energy<float>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->vel.p, (const double *)b->rho_dev.p,
+                               (const float *)b->f_vol0.p, b->energy_w.p);
+        else
+            hipLaunchKernelGGL(k_kinetic_energy<double>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->vel.p, (const double *)b->rho_dev.p,
+                               (const double *)b->d_vol0.p, b->energy_w.p);
+        hipLaunchKernelGGL(k_blob_energy, dim3(b->nblob), dim3(kRowBlock), 0, b->stream, (const int *)b->elem_off.p,
+                           (const int *)b->elem_cnt.p, (const double *)b->energy_w.p, b->energy_out.p);
+        FFEA_HIP_TRY(hipGetLastError());
+        FFEA_HIP_TRY(hipMemcpyAsync(energy, b->energy_out.p, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
 int ffea_gpu_batch_strain_energy(ffea_gpu_blob *b, double *energy) {
     if (!b || !energy) return FFEA_GPU_ERR_INVALID;
     return guarded(b, [&]() {

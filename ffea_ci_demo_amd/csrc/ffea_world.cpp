@@ -27,6 +27,44 @@ constexpr double kLength = 1.7e-10;
 constexpr double kEnergy = 4.141945559999999e-21;
 constexpr double kTime = 3.7302915560886126e-13;
 constexpr double kPressure = 8.4305832688784826e8;
+constexpr double kVelocity = 4.5572845297447225e2;
+constexpr double kForce = 2.4364705882352941e-11;
+
+// one rotational constant force on a node at x (Blob::apply_ctforces CTF 2,
+// Blob.cpp:1693-1714; intersectingPointToLine, mat_vec_fns_II.h:755-777), for the force
+// column of an inertial blob's trajectory frame
+void rotational_force(const double *x, const double *ax, double F, bool torque, double *f) {
+    const double *a = ax, *d = ax + 3;
+    auto cross = [](const double *u, const double *v, double *w) {
+        w[0] = u[1] * v[2] - v[1] * u[2];
+        w[1] = -u[0] * v[2] + v[0] * u[2];
+        w[2] = u[0] * v[1] - v[0] * u[1];
+    };
+    double pa[3], v2[3], cp[3], w[3], r[3], g[3];
+    for (int c = 0; c < 3; ++c) pa[c] = x[c] - a[c];
+    cross(pa, d, v2);
+    double m = 0.0;
+    for (int c = 0; c < 3; ++c) m += v2[c] * v2[c];
+    if (std::sqrt(m) < 3.0 * 2.220446049250313e-16) {
+        for (int c = 0; c < 3; ++c) r[c] = x[c] - x[c];
+    } else {
+        cross(d, v2, cp);
+        cross(d, cp, w);
+        double det = pa[0] * (d[1] * w[2] - d[2] * w[1]);
+        det += pa[1] * (d[2] * w[0] - d[0] * w[2]);
+        det += pa[2] * (d[0] * w[1] - d[1] * w[0]);
+        const double cc = det / (w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        for (int c = 0; c < 3; ++c) r[c] = (x[c] + cc * cp[c]) - x[c];
+    }
+    cross(r, d, g);
+    double s = F;
+    if (torque) {
+        double rr = 0.0;
+        for (int c = 0; c < 3; ++c) rr += r[c] * r[c];
+        s = F / std::sqrt(rr);
+    }
+    for (int c = 0; c < 3; ++c) f[c] = s * g[c];
+}
 
 struct WorldError {
     int code;
@@ -102,6 +140,18 @@ struct ffea_world {
         bool active = false;
     };
     std::vector<Spring> springs;
+    // inertial blobs (<solver = masslumped>): the trajectory carries velocities and the
+    // node forces of the print, the measurements the kinetic energy
+    std::vector<char> inertial;
+    bool mass_in_system = false;
+    struct NodeForces {  // the constant forces of a blob, list order
+        std::vector<int> lin_node, rot_node;
+        std::vector<double> lin_force, rot_force, rot_axis;
+        std::vector<char> rot_type;
+    };
+    std::vector<NodeForces> ctf;
+    std::vector<std::vector<int>> spring_node;       // this step's spring forces per blob, spring order
+    std::vector<std::vector<double>> spring_force;
 
     ~ffea_world() {
         for (FILE *f : {traj, meas, ckpt})
@@ -164,6 +214,27 @@ struct ffea_world {
             if (!fn[b].empty())
                 check_rc(ffea_gpu_blob_add_node_forces(blobs[b], (int)fn[b].size(), fn[b].data(), ff[b].data()), b,
                          "applying springs");
+        spring_node = std::move(fn);
+        spring_force = std::move(ff);
+    }
+
+    // Blob::force at the print (World.cpp:2045-2049), an inertial blob's force column: zeroed
+    // (World.cpp:1862), the springs' pulls (:1910), then the linear and rotational constant
+    // forces (Blob::apply_ctforces, Blob.cpp:1679-1719) at the frame's positions
+    void print_forces(int b, const std::vector<double> &x, std::vector<double> &f) {
+        f.assign(3 * (size_t)nt[b], 0.0);
+        if (b < (int)spring_node.size())
+            for (size_t i = 0; i < spring_node[b].size(); ++i)
+                for (int c = 0; c < 3; ++c) f[3 * (size_t)spring_node[b][i] + c] += spring_force[b][3 * i + c];
+        const NodeForces &F = ctf[b];
+        for (size_t i = 0; i < F.lin_node.size(); ++i)
+            for (int c = 0; c < 3; ++c) f[3 * (size_t)F.lin_node[i] + c] += F.lin_force[3 * i + c];
+        for (size_t i = 0; i < F.rot_node.size(); ++i) {
+            double g[3];
+            rotational_force(&x[3 * (size_t)F.rot_node[i]], &F.rot_axis[6 * i], F.rot_force[i],
+                             F.rot_type[2 * i + 1] == 't', g);
+            for (int c = 0; c < 3; ++c) f[3 * (size_t)F.rot_node[i] + c] += g[c];
+        }
     }
 
     // the spring part of World::make_measurements (World.cpp:4632-4652): every spring, active
@@ -232,6 +303,7 @@ struct ffea_world {
         fprintf(meas, "\n\n");
         fprintf(meas, "Measurements:\n");
         fprintf(meas, "%-14s", "Time");
+        if (mass_in_system) fprintf(meas, "%-14s", "KineticEnergy");
         fprintf(meas, "%-14s", "StrainEnergy");
         fprintf(meas, "%-14s%-14s%-14s%-14s", "Centroid[0]", "Centroid[1]", "Centroid[2]", "RMSD");
         if (p.calc_springs != 0) fprintf(meas, "%-14s", "SpringEnergy");
@@ -255,18 +327,34 @@ struct ffea_world {
     // and global measurements (World::make_measurements, write_measurements_to_file,
     // World.cpp:4598-4692; Blob::make_measurements Blob.cpp:1066-1202)
     void print_outputs(long long s) {
-        double strain = 0.0, cog[3] = {0, 0, 0}, rmsd2 = 0.0;
+        double strain = 0.0, kinetic = 0.0, cog[3] = {0, 0, 0}, rmsd2 = 0.0;
         long long nodes = 0;
+        std::vector<double> vel, frc;
         for (int b = 0; b < p.num_blobs; ++b) {
             host.resize(3 * (size_t)nt[b]);
             check_rc(ffea_gpu_blob_get_positions(blobs[b], host.data()), b, "reading positions");
+            if (inertial[b]) {
+                vel.resize(3 * (size_t)nt[b]);
+                check_rc(ffea_gpu_blob_get_velocities(blobs[b], vel.data()), b, "reading velocities");
+                print_forces(b, host, frc);
+                double ke = 0.0;
+                check_rc(ffea_gpu_blob_kinetic_energy(blobs[b], &ke), b, "kinetic energy");
+                kinetic += ke;
+            }
             fprintf(traj, "Blob %d, Conformation %d, step %d\n", b, 0, (int)s);
             fprintf(traj, "DYNAMIC\n");
             double c[3] = {0, 0, 0}, r = 0.0;
             for (int i = 0; i < nt[b]; ++i) {
                 const double *x = &host[3 * (size_t)i];
-                fprintf(traj, "%e %e %e %e %e %e %e %e %e %e\n", x[0] * kLength, x[1] * kLength, x[2] * kLength, 0., 0.,
-                        0., 0., 0., 0., 0.);
+                if (inertial[b]) {  // Blob::write_nodes_to_file (Blob.cpp:866-873)
+                    const double *v = &vel[3 * (size_t)i], *f = &frc[3 * (size_t)i];
+                    fprintf(traj, "%e %e %e %e %e %e %e %e %e %e\n", x[0] * kLength, x[1] * kLength, x[2] * kLength,
+                            v[0] * kVelocity, v[1] * kVelocity, v[2] * kVelocity, 0., f[0] * kForce, f[1] * kForce,
+                            f[2] * kForce);
+                } else {
+                    fprintf(traj, "%e %e %e %e %e %e %e %e %e %e\n", x[0] * kLength, x[1] * kLength, x[2] * kLength, 0.,
+                            0., 0., 0., 0., 0., 0.);
+                }
                 for (int k = 0; k < 3; ++k) {
                     c[k] += x[k];
                     const double d = x[k] - pos0[b][3 * (size_t)i + k];
@@ -290,6 +378,7 @@ struct ffea_world {
         const double rmsd = std::sqrt(rmsd2 / nodes);
         for (double &c : cog) c *= 1.0 / nodes;
         fprintf(meas, "%-14.6e", (int)s * p.dt * kTime);
+        if (mass_in_system) fprintf(meas, "%-14.6e", kinetic * kEnergy);
         fprintf(meas, "%-14.6e", strain * kEnergy);
         fprintf(meas, "%-14.6e%-14.6e%-14.6e", cog[0] * kLength, cog[1] * kLength, cog[2] * kLength);
         fprintf(meas, "%-14.6e", rmsd * kLength);
@@ -391,6 +480,7 @@ struct ffea_world {
             cur += used;
             skip_line();  // DYNAMIC
             host.resize(3 * (size_t)nt[b]);
+            std::vector<double> rvel(inertial[b] ? 3 * (size_t)nt[b] : 0);
             for (int i = 0; i < nt[b]; ++i) {
                 double v[10];
                 char *end = const_cast<char *>(cur);
@@ -404,8 +494,11 @@ struct ffea_world {
                 cur = end;
                 while (*cur == ' ' || *cur == '\n' || *cur == '\t' || *cur == '\r') ++cur;
                 for (int k = 0; k < 3; ++k) host[3 * (size_t)i + k] = v[k] / kLength;
+                if (inertial[b])  // Blob::read_nodes_from_file (Blob.cpp:1002-1010)
+                    for (int k = 0; k < 3; ++k) rvel[3 * (size_t)i + k] = v[3 + k] / kVelocity;
             }
             check_rc(ffea_gpu_blob_set_positions(blobs[b], host.data()), b, "setting restart positions");
+            if (inertial[b]) check_rc(ffea_gpu_blob_set_velocities(blobs[b], rvel.data()), b, "setting restart velocities");
         }
         step_initial = rstep;
         truncate_to(files.trajectory, cut);
@@ -464,6 +557,26 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
             w->check_rc(ffea_gpu_blob_get_positions(h, w->pos0.back().data()), b, "reading initial positions");
             w->draws_before.push_back(before);
             before += 7ll * ne;
+            int solver = FFEA_GPU_SOLVER_NOMASS_CG;
+            ffea_script_blob_solver(s, b, &solver);
+            w->inertial.push_back(solver != FFEA_GPU_SOLVER_NOMASS_CG);
+            w->mass_in_system |= w->inertial.back() != 0;
+            ffea_world::NodeForces F;
+            if (w->inertial.back()) {
+                F.lin_node.resize(nctf);
+                F.lin_force.resize(3 * (size_t)nctf);
+                ffea_script_blob_arrays(s, b, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        F.lin_node.data(), F.lin_force.data(), nullptr);
+                int nr = 0;
+                ffea_script_blob_rotational_forces(s, b, &nr, nullptr, nullptr, nullptr, nullptr);
+                F.rot_node.resize(nr);
+                F.rot_force.resize(nr);
+                F.rot_axis.resize(6 * (size_t)nr);
+                F.rot_type.resize(2 * (size_t)nr);
+                ffea_script_blob_rotational_forces(s, b, nullptr, F.rot_node.data(), F.rot_force.data(),
+                                                   F.rot_axis.data(), F.rot_type.data());
+            }
+            w->ctf.push_back(std::move(F));
         }
         if (p.calc_springs == 1) {
             int ns = 0;

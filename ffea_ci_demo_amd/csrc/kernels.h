@@ -1698,6 +1698,39 @@ __global__ __launch_bounds__(kBlock) void k_strain_energy(int ne, const int *con
 }
 
 // per blob (one workgroup each): half the sum of its elements' W
+// Blob::make_measurements, one element's kinetic energy term (Blob.cpp:1079-1102):
+// v_e . (M v_e) with the element mass matrix of apply_element_mass_matrix
+// (tetra_element_linear.cpp:524-542: .1 on the diagonal, .05 off it, per component, times
+// vol_0 rho)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_kinetic_energy(int ne, const int *conn, const double *vel,
+                                                        const double *rho, const T *vol0, double *w) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    double v[12], t[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int n = conn[a * ne + e];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[4 * c + a] = vel[3 * n + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k += 4)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc += (i == j ? .1 : .05) * v[j + k];
+            t[i + k] = acc;
+        }
+    const double s = (double)vol0[e] * rho[e];
+    double ke = v[0] * (t[0] * s);
+#pragma unroll
+    for (int i = 1; i < 12; ++i) ke += v[i] * (t[i] * s);
+    w[e] = ke;
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_blob_energy(const int *elem_off, const int *elem_cnt, const double *w,
                                                         double *out) {
     const int b = blockIdx.x;

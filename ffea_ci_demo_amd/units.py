@@ -13,6 +13,7 @@ MASS = 1.994307387553024e-26
 AREA = 2.89e-20
 VOLUME = 4.913e-30
 FORCE = 2.4364705882352941e-11
+VELOCITY = 4.5572845297447225e2
 TIME = 3.7302915560886126e-13
 PRESSURE = 8.4305832688784826e8
 PI = 3.141592653589793

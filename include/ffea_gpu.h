@@ -215,6 +215,11 @@ int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
  * for NOMASS_CG only. */
 int ffea_gpu_blob_set_solver(ffea_gpu_blob *blob, int solver, const double *rho);
 
+/* Blob::make_measurements' kinetic energy (Blob.cpp:1076-1118): half the sum over elements
+ * of v_e . (M_e v_e) with the element mass matrix (tetra_element_linear.cpp:524-542); 0
+ * for the mass-less solver */
+int ffea_gpu_blob_kinetic_energy(ffea_gpu_blob *blob, double *energy);
+
 /* Blob::add_force_to_node (Blob.cpp:1770-1774) for the next step only: force[i] (meso
  * units) is added to node[i], in list order and across calls in call order, before the
  * constant forces of the next step's force formation (World.cpp:1862-1910: forces are

@@ -479,6 +479,7 @@ struct orc_blob {
     /* MassLumpedSolver (Blob.cpp:232-234): inertial steps */
     int mass;
     double *inv_m;             /* [nt] */
+    double *rho;               /* [ne] */
     /* external forces for the next step (Blob::add_force_to_node) */
     int next, next_cap;
     int *ext_node;
@@ -736,7 +737,7 @@ void orc_blob_destroy(orc_blob *b) {
     free(b->top); free(b->is_pinned); free(b->pinned); free(b->ctf_node);
     free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
     free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
-    free(b->ext_node); free(b->ext_force); free(b->inv_m);
+    free(b->ext_node); free(b->ext_force); free(b->inv_m); free(b->rho);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
@@ -1109,8 +1110,34 @@ int orc_blob_set_mass_lumped(orc_blob *b, const double *rho) {
     for (int k = 0; k < b->npin; ++k) b->inv_m[b->pinned[k]] = 1.0;
     for (int i = 0; i < b->nt; ++i) b->inv_m[i] = 1.0 / b->inv_m[i];
     memset(b->vel, 0, sizeof(double) * 3 * (size_t)b->nt);
+    free(b->rho);
+    b->rho = xcalloc((size_t)b->ne, sizeof(double));
+    memcpy(b->rho, rho, sizeof(double) * (size_t)b->ne);
     b->mass = 1;
     return 0;
+}
+
+/* Blob::make_measurements, kinetic energy (Blob.cpp:1076-1118; apply_element_mass_matrix,
+ * tetra_element_linear.cpp:524-542); 0 without mass */
+double orc_blob_kinetic_energy(const orc_blob *b) {
+    static const double M[4][4] = {{.1, .05, .05, .05}, {.05, .1, .05, .05}, {.05, .05, .1, .05}, {.05, .05, .05, .1}};
+    if (!b->mass) return 0.0;
+    double kenergy = 0.0;
+    for (int e = 0; e < b->ne; ++e) {
+        const int *n = b->top + 10 * e;
+        double vec[12], tmp[12] = {0};
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 4; ++a) vec[4 * c + a] = b->vel[3 * n[a] + c];
+        for (int k = 0; k < 12; k += 4)
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) tmp[i + k] += M[i][j] * vec[j + k];
+        double du[12];
+        for (int i = 0; i < 12; ++i) du[i] = tmp[i] * (b->vol0[e] * b->rho[e]);
+        double t = 0.0;
+        for (int i = 0; i < 12; ++i) t = i == 0 ? vec[0] * du[0] : t + vec[i] * du[i];
+        kenergy += t;
+    }
+    return kenergy * 0.5;
 }
 
 int orc_blob_add_node_forces(orc_blob *b, int num, const int *node, const double *force) {

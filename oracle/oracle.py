@@ -87,6 +87,8 @@ def lib():
         L.orc_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_blob_set_mass_lumped.argtypes = [C.c_void_p, _dp]
         L.orc_blob_set_mass_lumped.restype = C.c_int
+        L.orc_blob_kinetic_energy.argtypes = [C.c_void_p]
+        L.orc_blob_kinetic_energy.restype = C.c_double
         L.orc_blob_add_node_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
         L.orc_blob_add_node_forces.restype = C.c_int
         L.orc_spring_forces.argtypes = [_dp, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -350,6 +352,9 @@ class Blob:
         r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (self.ne,)))
         if lib().orc_blob_set_mass_lumped(self.h, r) != 0:
             raise ValueError("oracle densities rejected")
+
+    def kinetic_energy(self):
+        return lib().orc_blob_kinetic_energy(self.h)
 
     def add_node_forces(self, nodes, forces):
         """Blob::add_force_to_node for the next step (meso units, list order)"""

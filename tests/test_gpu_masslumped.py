@@ -122,3 +122,52 @@ def test_refusals():
     with GpuBatch([spec, spec]) as b:
         with pytest.raises(FFEAException, match="single-blob"):
             b.set_solver("masslumped", np.full(spec.num_elements, RHO))
+
+
+def test_world_files_of_an_inertial_blob(oracle_mod, tmp_path):
+    """a <solver = masslumped> script with linear and rotational constant forces through
+    the World loop: every trajectory frame's positions, velocities and node forces, the
+    KineticEnergy column, and the final state against the oracle stepped alongside"""
+    import oracle
+    from test_gpu_world import PRINT_RTOL, read_measurements, read_trajectory
+    from ffea_ci_demo_amd.world import World
+    mesh = meshgen.kuhn_mesh(3, seed=4, jitter=0.1)
+    prefix = str(tmp_path / "w")
+    z = mesh.pos[:, 2]
+    meshgen.write_blob_files(mesh, prefix, pinned=np.nonzero(z <= z.min() + 1e-9)[0])
+    meshgen.write_full_ctforces(str(tmp_path / "w.ctforces"), node_forces=[(3e-12, (0, 0, 1.0), 20)],
+                                rot_forces=[(2e-12, "pf", (1.3e-9, 1.1e-9, 0.0), (0, 0.3, 1), "all")])
+    script = str(tmp_path / "w.ffea")
+    meshgen.write_ffea_script(script, prefix, solver="masslumped", calc_ctforces=1, ctforces_fname="w.ctforces",
+                              num_steps=6, check=2)
+    spec = Script(script).blob_spec(0)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_lumped(spec.rho)
+    print("[case] world inertial blob", flush=True)
+    with World(script) as w:
+        w.run()
+        got_pos, got_vel = w.blob(0).positions(), w.blob(0).velocities()
+    frames = read_trajectory(str(tmp_path / "w_trajectory.out"))
+    head, meas = read_measurements(str(tmp_path / "w_measurement.out"))
+    assert head[:3] == ["Time", "KineticEnergy", "StrainEnergy"]
+    rn, rF, rax, rty = spec.rotational_forces
+    for n in range(7):
+        if n % 2 == 0:
+            x = o.positions()
+            f = np.zeros_like(x)
+            for node, fl in zip(spec.ctf_node, spec.ctf_force):
+                f[node] += fl
+            for i in range(len(rn)):
+                f[rn[i]] += oracle.rotational_force(x[rn[i]], rax[i], rF[i], rty[i][1] == "t")
+            rows = frames[n // 2][1][0]
+            sc = np.abs(x).max() * units.LENGTH
+            assert np.all(np.abs(rows[:, :3] - x * units.LENGTH) <= PRINT_RTOL * np.abs(x * units.LENGTH) + 1e-12 * sc)
+            v = o.velocities() * units.VELOCITY
+            assert np.all(np.abs(rows[:, 3:6] - v) <= PRINT_RTOL * np.abs(v) + 1e-12 * max(np.abs(v).max(), 1e-30))
+            fs = f * units.FORCE
+            assert np.all(np.abs(rows[:, 7:10] - fs) <= PRINT_RTOL * np.abs(fs) + 1e-12 * np.abs(fs).max())
+            ke = o.kinetic_energy() * units.ENERGY
+            assert abs(meas[n // 2][1] - ke) <= PRINT_RTOL * abs(ke) + 1e-30, (n, meas[n // 2][1], ke)
+        assert o.step() == 0
+    assert rel_err(got_pos, o.positions()) <= POS_TOL
+    assert rel_err(got_vel, o.velocities()) <= POS_TOL * 100

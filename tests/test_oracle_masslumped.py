@@ -85,3 +85,19 @@ def test_viscous_dissipation(oracle_mod):
     energies = np.array(energies)
     assert energies[-1] < 0.99 * energies[0]
     assert np.all(np.diff(energies) <= 1e-9 * energies[0])
+
+
+def test_kinetic_energy_of_a_uniform_motion(oracle_mod):
+    """the element mass matrix's entries sum to rho vol_0 per component, so a uniform
+    velocity v gives 0.5 |v|^2 times the total mass"""
+    spec, rho = free_spec()
+    rho = rho * np.linspace(0.5, 1.5, spec.num_elements)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    assert o.kinetic_energy() == 0.0
+    o.set_mass_lumped(rho)
+    _, vol0 = o.rest_state()
+    v = np.array([0.3, -0.4, 1.2])
+    o.set_velocities(np.tile(v, (spec.pos.shape[0], 1)))
+    want = 0.5 * (rho * vol0).sum() * (v @ v)
+    assert abs(o.kinetic_energy() - want) <= 1e-12 * want
+
