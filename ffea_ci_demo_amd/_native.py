@@ -39,7 +39,7 @@ EXPORTED = [
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
-SOLVER_MASSLUMPED, SOLVER_NOMASS_CG = 2, 3
+SOLVER_ITERATIVE, SOLVER_MASSLUMPED, SOLVER_NOMASS_CG = 1, 2, 3
 MG_MAX_LEVELS = 8
 
 

@@ -120,13 +120,14 @@ class _Handle:
         self._check(_native.lib().ffea_gpu_blob_synchronize(self.h))
 
     def set_solver(self, kind, rho=None):
-        """\"masslumped\" (MassLumpedSolver, rho per element in meso units) or \"nomass\"
-        (the default mass-less CG)"""
+        """\"masslumped\" (MassLumpedSolver) or \"cg\" (ConjugateGradientSolver, consistent
+        mass), both with rho per element in meso units, or \"nomass\" (the default mass-less CG)"""
         L = _native.lib()
-        if kind == "masslumped":
+        if kind in ("masslumped", "cg"):
             r = np.asarray(rho, np.float64)
             r = np.ascontiguousarray(np.broadcast_to(r, (self.ne,)) if r.ndim == 0 else r.ravel())
-            self._check(L.ffea_gpu_blob_set_solver(self.h, _native.SOLVER_MASSLUMPED, r.ctypes.data))
+            code = _native.SOLVER_MASSLUMPED if kind == "masslumped" else _native.SOLVER_ITERATIVE
+            self._check(L.ffea_gpu_blob_set_solver(self.h, code, r.ctypes.data))
         elif kind == "nomass":
             self._check(L.ffea_gpu_blob_set_solver(self.h, _native.SOLVER_NOMASS_CG, None))
         else:
@@ -199,9 +200,9 @@ class GpuBlob(_Handle):
     @classmethod
     def from_spec(cls, spec, precision="fp64", **kw):
         g = cls._from_spec(spec, precision, **kw)
-        if getattr(spec, "solver", "nomass") == "masslumped":
+        if getattr(spec, "solver", "nomass") != "nomass":
             try:
-                g.set_solver("masslumped", spec.rho)
+                g.set_solver(spec.solver, spec.rho)
             except Exception:
                 g.close()
                 raise

@@ -1138,8 +1138,10 @@ struct ffea_gpu_blob {
     bool ext_on_device = false;       // ext_force holds forces of the coming step (to clear after it)
     DevBuf<int> take_rows;            // ffea_gpu_blob_get_node_positions scratch
     DevBuf<double> take_out;
-    // MassLumpedSolver (ffea_gpu_blob_set_solver): explicit inertial steps
-    bool mass = false;
+    // inertial solvers (ffea_gpu_blob_set_solver): MassLumpedSolver's explicit steps, or
+    // ConjugateGradientSolver's (mass_cg: the consistent mass matrix in the CG's slots)
+    bool mass = false, mass_cg = false;
+    bool project_saved = false;        // any_project of the mass-less solver, off for the mass CG
     DevBuf<double> inv_m, rho_dev;     // [rows] 1 / lumped mass, [ne] densities
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
@@ -2107,6 +2109,18 @@ int ffea_gpu_blob::step_mass() {
     prepare_ext();
     if (mixed) launch_gather<float, false>(f_force.p);
     else launch_gather<double, false>(d_force.p);
+    if (mass_cg) {  // ConjugateGradientSolver::solve with the constant mass matrix, then the update
+        // its convergence test comes before each update: a zero right-hand side stops it at once
+        // with a = 0 (ConjugateGradientSolver.cpp:194-200, 319-334)
+        zero_rhs_solved = 1;
+        run_cg_eager(0);
+        zero_rhs_solved = 0;
+        hipLaunchKernelGGL(k_mass_update_cg, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)x.p,
+                           vel.p, pos.p, prm.dt);
+        hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
+        FFEA_HIP_TRY(hipGetLastError());
+        return finish_step();
+    }
     hipLaunchKernelGGL(k_mass_update, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)rhs.p,
                        (const double *)inv_m.p, vel.p, pos.p, prm.dt);
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
@@ -2122,17 +2136,18 @@ int ffea_gpu_blob::step_mass() {
 }
 
 void ffea_gpu_blob::set_solver(int kind, const double *rho_h) {
+    if (mass_cg) any_project = project_saved;
     if (kind == FFEA_GPU_SOLVER_NOMASS_CG) {
-        mass = false;
+        mass = mass_cg = false;
         inv_m.reset();
         rho_dev.reset();
         return;
     }
-    if (kind != FFEA_GPU_SOLVER_MASSLUMPED)
-        throw Invalid(fmt("ffea_gpu_blob_set_solver: solver %d is not one this library runs (2 = MASSLUMPED, 3 = "
-                          "NOMASS_CG)", kind));
-    if (nblob != 1) throw Invalid("ffea_gpu_blob_set_solver: the lumped-mass solver needs a single-blob handle");
-    if (!rho_h) throw Invalid("ffea_gpu_blob_set_solver: the lumped-mass solver needs the element densities");
+    if (kind != FFEA_GPU_SOLVER_MASSLUMPED && kind != FFEA_GPU_SOLVER_ITERATIVE)
+        throw Invalid(fmt("ffea_gpu_blob_set_solver: solver %d is not one this library runs (1 = ITERATIVE, 2 = "
+                          "MASSLUMPED, 3 = NOMASS_CG)", kind));
+    if (nblob != 1) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need a single-blob handle");
+    if (!rho_h) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need the element densities");
     if (mg_on()) throw Invalid("ffea_gpu_blob_set_solver: switch the multigrid preconditioner off first");
     for (int e = 0; e < ne; ++e)
         if (!(rho_h[e] > 0.0) || !std::isfinite(rho_h[e]))
@@ -2149,6 +2164,23 @@ void ffea_gpu_blob::set_solver(int kind, const double *rho_h) {
         hipLaunchKernelGGL(k_lumped_mass<double>, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), inc_ptr.p, inc.p,
                            (const double *)rho_dev.p, (const double *)d_vol0.p, pinned.p, inv_m.p);
     FFEA_HIP_TRY(hipGetLastError());
+    mass_cg = kind == FFEA_GPU_SOLVER_ITERATIVE;
+    if (mass_cg) {  // the consistent mass matrix, once: it depends on the rest state only
+        if (mixed) {
+            AsmArgs<float> am{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p, f_val.p, dinv.p,
+                              nullptr, nullptr, 1.0};
+            hipLaunchKernelGGL(k_assemble_mass<float>, dim3(grid_rows), dim3(kBlock), 0, stream, am,
+                               (const double *)rho_dev.p, (const float *)f_vol0.p);
+        } else {
+            AsmArgs<double> am{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p, d_val.p,
+                               dinv.p, nullptr, nullptr, 1.0};
+            hipLaunchKernelGGL(k_assemble_mass<double>, dim3(grid_rows), dim3(kBlock), 0, stream, am,
+                               (const double *)rho_dev.p, (const double *)d_vol0.p);
+        }
+        FFEA_HIP_TRY(hipGetLastError());
+        project_saved = any_project;
+        any_project = false;  // the mass matrix is nonsingular: no rigid-mode projection
+    }
     FFEA_HIP_TRY(hipMemsetAsync(vel.p, 0, sizeof(double) * vel.n, stream));  // nodes start at rest
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     mass = true;

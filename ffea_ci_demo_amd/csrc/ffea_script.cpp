@@ -267,7 +267,7 @@ struct BlobData {
     std::vector<double> rf_force;       // [n] meso
     std::vector<double> rf_axis;        // [n][6] meso point, unit direction
     std::vector<char> rf_type;          // [n][2]
-    bool masslumped = false;            // <solver = masslumped>: MassLumpedSolver
+    int solver = FFEA_GPU_SOLVER_NOMASS_CG;  // <solver = ...> (World.cpp:2600-2625)
     double scale = 1.0;  // meso per file unit
     bool set_centroid = false;
     double centroid[3] = {0, 0, 0};
@@ -779,9 +779,11 @@ void load_script(ffea_script &S) {
         // World.cpp:2600-2625
         if (solver != "CG_nomass" && solver != "masslumped" && solver != "CG" && solver != "direct")
             fail("In blob %d, unrecognised solver type.\nRecognised solvers:CG, CG_nomass, direct, masslumped.", b);
-        if (solver == "CG" || solver == "direct")
-            fail("Blob %d: solver '%s' is not part of this build (CG_nomass and masslumped are).", b, solver.c_str());
-        B.masslumped = solver == "masslumped";
+        if (solver == "direct")
+            fail("Blob %d: solver 'direct' is not part of this build (CG_nomass, CG and masslumped are).", b);
+        B.solver = solver == "masslumped" ? FFEA_GPU_SOLVER_MASSLUMPED
+                   : solver == "CG"       ? FFEA_GPU_SOLVER_ITERATIVE
+                                          : FFEA_GPU_SOLVER_NOMASS_CG;
         load_nodes(B, f_nodes);
         load_topology(B, f_top);
         load_surface(B, f_surf);
@@ -965,7 +967,7 @@ int ffea_script_blob_rotational_forces(const ffea_script *s, int b, int *num, in
 
 int ffea_script_blob_solver(const ffea_script *s, int b, int *solver) {
     if (!s || !solver || b < 0 || b >= (int)s->blobs.size()) return FFEA_GPU_ERR_INVALID;
-    *solver = s->blobs[b].masslumped ? FFEA_GPU_SOLVER_MASSLUMPED : FFEA_GPU_SOLVER_NOMASS_CG;
+    *solver = s->blobs[b].solver;
     return FFEA_GPU_OK;
 }
 
@@ -1045,8 +1047,8 @@ int ffea_gpu_blob_create_from_script(const ffea_script *s, int b, int precision,
                                      p.calc_stokes ? drag.data() : nullptr, s->rng_state, nsets > 0 ? &sf : nullptr,
                                      B.rf_node.empty() ? nullptr : &rf, (int)ext.size(),
                                      ext.empty() ? nullptr : ext.data(), out);
-    if (rc == FFEA_GPU_OK && B.masslumped) {
-        rc = ffea_gpu_blob_set_solver(*out, FFEA_GPU_SOLVER_MASSLUMPED, B.rho.data());
+    if (rc == FFEA_GPU_OK && B.solver != FFEA_GPU_SOLVER_NOMASS_CG) {
+        rc = ffea_gpu_blob_set_solver(*out, B.solver, B.rho.data());
         if (rc != FFEA_GPU_OK) {
             ffea_internal_set_error(ffea_gpu_blob_last_error(*out));
             ffea_gpu_blob_destroy(*out);

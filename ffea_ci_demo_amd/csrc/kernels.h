@@ -603,6 +603,40 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
     }
 }
 
+// ConjugateGradientSolver::init (ConjugateGradientSolver.cpp:51-190): the linear tets'
+// consistent mass matrix in the viscosity matrix's slots, m I per 3x3 block with m the sum
+// over the block's element sources, in element order, of .1 rho vol_0 (a corner with
+// itself) or .05 rho vol_0 (two corners); identity sources (pinned rows) give 1. D^-1 from
+// the diagonal. One lane per row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_assemble_mass(AsmArgs<T> a, const double *rho, const T *vol0) {
+#pragma clang fp contract(off)
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= a.nrows) return;
+    const int sl = n / kSlice, lane = n % kSlice;
+    const int W = a.slice_w[sl];
+    const long long base = a.slice_base[sl];
+    for (int j = 0; j < W; ++j) {
+        const long long slot = base + (long long)j * kSlice + lane;
+        double m = 0.0;
+        for (uint32_t c = a.cptr[slot]; c < a.cptr[slot + 1]; ++c) {
+            const uint32_t code = a.contrib[c];
+            if (code == kIdentity) {
+                m = 1.0;
+                continue;
+            }
+            const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
+            m += (pa == pb ? .1 : .05) * rho[e] * (double)vol0[e];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)(k % 4 == 0 ? m : 0.0);
+        if (j == a.diag_j[n]) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)m;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- shared row-kernel arguments
 struct RowArgs {
     int nchunks, cpb;
@@ -1367,6 +1401,26 @@ __global__ __launch_bounds__(kRowBlock) void k_mass_update(RowArgs ra, const dou
             for (int k = 0; k < 3; ++k) {
                 const double acc = rhs[vi(n, k)] * inv_m[n];
                 const double v = vel[3 * n + k] + acc * dt;
+                vel[3 * n + k] = v;
+                pos[3 * n + k] += v * dt;
+            }
+    }
+}
+
+// the inertial Blob::euler_integrate after ConjugateGradientSolver::solve (Blob.cpp:3266-3280):
+// v += a dt, x += v dt with a the CG solution, for blobs whose solve converged
+__global__ __launch_bounds__(kRowBlock) void k_mass_update_cg(RowArgs ra, const double *acc, double *vel, double *pos,
+                                                           double dt) {
+#pragma clang fp contract(off)
+    const ChunkRange cr = my_chunks(ra.nchunks, ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = ra.chunks[c];
+        const CgScalars &S = ra.S[ch.blob];
+        if (!S.converged || S.updated) continue;
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double v = vel[3 * n + k] + acc[vi(n, k)] * dt;
                 vel[3 * n + k] = v;
                 pos[3 * n + k] += v * dt;
             }

@@ -162,7 +162,7 @@ class Script:
         k = C.c_int()
         if L.ffea_script_blob_solver(self.h, b, C.byref(k)) != _native.OK:
             raise ScriptError(f"no blob {b}")
-        return "masslumped" if k.value == _native.SOLVER_MASSLUMPED else "nomass"
+        return {_native.SOLVER_MASSLUMPED: "masslumped", _native.SOLVER_ITERATIVE: "cg"}.get(k.value, "nomass")
 
     def gpu_blob(self, b: int = 0, precision: str = "fp64", device: int = 0):
         from .blob import FFEAException, GpuBlob

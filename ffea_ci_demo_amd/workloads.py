@@ -44,7 +44,7 @@ class BlobSpec:
     surface_forces: Optional[tuple] = None
     rotational_forces: Optional[tuple] = None  # (node [n], force [n] meso, axis [n][6] meso, type [n] "pf"/"pt")
     external_nodes: tuple = ()  # nodes that receive per-step external forces (springs)
-    solver: str = "nomass"      # or "masslumped" (MassLumpedSolver, needs rho)
+    solver: str = "nomass"      # or "masslumped" / "cg" (the inertial solvers, need rho)
     rho: Optional[np.ndarray] = None  # (Ne,) densities, meso units
 
     @property

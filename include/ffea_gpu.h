@@ -200,11 +200,17 @@ int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
 
 /* The reference's solvers behind its Solver interface (Blob.cpp:228-243,
  * FFEA_return_codes.h:35-38) that this library runs */
+#define FFEA_GPU_SOLVER_ITERATIVE 1
 #define FFEA_GPU_SOLVER_MASSLUMPED 2
 #define FFEA_GPU_SOLVER_NOMASS_CG 3
 
 /* Chooses the blob's solver (single-blob handles). NOMASS_CG (the default) is the
- * mass-less viscous solve. MASSLUMPED is MassLumpedSolver (MassLumpedSolver.cpp:27-73)
+ * mass-less viscous solve. ITERATIVE is ConjugateGradientSolver
+ * (ConjugateGradientSolver.cpp:51-335): the linear tets' consistent mass matrix (.1 rho
+ * vol_0 on a corner's own entry, .05 rho vol_0 between two corners of an element, element
+ * order; pinned rows identity), built once, and a Jacobi-PCG solve of M a = f each step
+ * with the mass-less CG's stopping rule, then the same inertial update as MASSLUMPED.
+ * MASSLUMPED is MassLumpedSolver (MassLumpedSolver.cpp:27-73)
  * with the inertial time step it implies: each corner node's mass is the sum over its
  * elements, in element order, of rho vol_0 / 4 (rho [num_elements] in meso units, .mat
  * order); midside and pinned nodes have mass 1. A step then forms the element forces with

@@ -477,9 +477,11 @@ struct orc_blob {
     double *rf_axis;           /* [nrf][6] point / length unit, unit direction */
     int *rf_torque;            /* [nrf] 1: 't' (constant torque), 0: 'f' */
     /* MassLumpedSolver (Blob.cpp:232-234): inertial steps */
-    int mass;
+    int mass;                  /* 1: MassLumpedSolver, 2: ConjugateGradientSolver */
     double *inv_m;             /* [nt] */
     double *rho;               /* [ne] */
+    int *m_ptr, *m_col;        /* ConjugateGradientSolver's mass matrix, scalar CSR over nt rows */
+    double *m_val, *m_pre;
     /* external forces for the next step (Blob::add_force_to_node) */
     int next, next_cap;
     int *ext_node;
@@ -738,6 +740,7 @@ void orc_blob_destroy(orc_blob *b) {
     free(b->sf_face_nodes); free(b->sf_set_force); free(b->sf_set_ptr); free(b->sf_set_faces); free(b->sf_face_force);
     free(b->rf_node); free(b->rf_force); free(b->rf_axis); free(b->rf_torque);
     free(b->ext_node); free(b->ext_force); free(b->inv_m); free(b->rho);
+    free(b->m_ptr); free(b->m_col); free(b->m_val); free(b->m_pre);
     free(b->contrib_ptr); free(b->contrib); free(b->rowptr); free(b->col); free(b->diag);
     free(b->src_ptr); free(b->src);
     free(b);
@@ -745,6 +748,7 @@ void orc_blob_destroy(orc_blob *b) {
 
 static void surface_ctforces(orc_blob *b);
 static void rotational_ctforces(orc_blob *b);
+static int mass_cg_solve(orc_blob *b, double *x);
 
 /* Blob::update_internal_forces (Blob.cpp:400-491), single thread (rng[0]) */
 int orc_blob_update_internal_forces(orc_blob *b) {
@@ -1067,10 +1071,14 @@ int orc_blob_update_positions(orc_blob *b) {
         for (int c = 0; c < 3; ++c) F[3 * b->pinned[k] + c] = 0;
     memcpy(b->rhs, F, sizeof(double) * 3 * nt);
     if (b->mass) {
-        /* MassLumpedSolver::solve (MassLumpedSolver.cpp:66-73), then the inertial
-         * euler_integrate (Blob.cpp:3266-3280) */
-        for (int i = 0; i < nt; ++i)
-            for (int c = 0; c < 3; ++c) F[3 * i + c] = F[3 * i + c] * b->inv_m[i];
+        /* MassLumpedSolver::solve (MassLumpedSolver.cpp:66-73) or ConjugateGradientSolver's,
+         * then the inertial euler_integrate (Blob.cpp:3266-3280) */
+        if (b->mass == 2) {
+            if (mass_cg_solve(b, F) < 0) return 1;
+        } else {
+            for (int i = 0; i < nt; ++i)
+                for (int c = 0; c < 3; ++c) F[3 * i + c] = F[3 * i + c] * b->inv_m[i];
+        }
         for (int i = 0; i < nt; ++i)
             for (int c = 0; c < 3; ++c) {
                 b->vel[3 * i + c] += F[3 * i + c] * b->p.dt;
@@ -1115,6 +1123,132 @@ int orc_blob_set_mass_lumped(orc_blob *b, const double *rho) {
     memcpy(b->rho, rho, sizeof(double) * (size_t)b->ne);
     b->mass = 1;
     return 0;
+}
+
+/* ConjugateGradientSolver::init (ConjugateGradientSolver.cpp:51-190): the linear tets'
+ * consistent mass matrix over the corner nodes, .1 rho vol_0 on a node's own entry and .05
+ * rho vol_0 between two corners of an element, added in element order; a pair with a pinned
+ * node gets nothing except a 1 on a pinned node's diagonal; midside nodes are identity
+ * rows. Entries that stay exactly 0 are dropped; columns ascend. Jacobi preconditioner. */
+static int cmp_int(const void *a, const void *b) { return *(const int *)a - *(const int *)b; }
+static int find_entry(const orc_blob *b, int i, int j) {
+    int lo = b->m_ptr[i], hi = b->m_ptr[i + 1] - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (b->m_col[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+int orc_blob_set_mass_consistent(orc_blob *b, const double *rho) {
+    if (orc_blob_set_mass_lumped(b, rho)) return 1;  /* densities, zero velocities */
+    const int nt = b->nt, ne = b->ne;
+    /* pattern: each node's neighbours through elements (corners), or itself */
+    int *cnt = xcalloc((size_t)nt + 1, sizeof(int));
+    int *nb = xcalloc((size_t)nt * 64, sizeof(int));  /* <= 64 distinct neighbours per node */
+    for (int e = 0; e < ne; ++e) {
+        const int *n = b->top + 10 * e;
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) {
+                if (!((i < 4 && j < 4) || i == j)) continue;
+                const int ni = n[i], nj = n[j];
+                int found = 0;
+                for (int k = 0; k < cnt[ni]; ++k)
+                    if (nb[64 * ni + k] == nj) { found = 1; break; }
+                if (!found) {
+                    if (cnt[ni] >= 64) { free(cnt); free(nb); return 1; }
+                    nb[64 * ni + cnt[ni]++] = nj;
+                }
+            }
+    }
+    free(b->m_ptr); free(b->m_col); free(b->m_val); free(b->m_pre);
+    b->m_ptr = xcalloc((size_t)nt + 1, sizeof(int));
+    for (int i = 0; i < nt; ++i) {
+        qsort(nb + 64 * i, (size_t)cnt[i], sizeof(int), cmp_int);
+        b->m_ptr[i + 1] = b->m_ptr[i] + cnt[i];
+    }
+    b->m_col = xcalloc((size_t)b->m_ptr[nt], sizeof(int));
+    b->m_val = xcalloc((size_t)b->m_ptr[nt], sizeof(double));
+    for (int i = 0; i < nt; ++i)
+        for (int k = 0; k < cnt[i]; ++k) b->m_col[b->m_ptr[i] + k] = nb[64 * i + k];
+    free(cnt);
+    free(nb);
+    /* the entry (ni, nj) in the CSR */
+    #define MENTRY(ni, nj) find_entry(b, ni, nj)
+    for (int e = 0; e < ne; ++e) {
+        const int *n = b->top + 10 * e;
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) {
+                if (i < 4 && j < 4) {
+                    const int ni = n[i], nj = n[j];
+                    double *v = b->m_val + MENTRY(ni, nj);
+                    if (!b->is_pinned[ni] && !b->is_pinned[nj]) {
+                        if (i == j) *v += .1 * rho[e] * b->vol0[e];
+                        else *v += .05 * rho[e] * b->vol0[e];
+                    } else if (i == j) {
+                        *v = 1;
+                    }
+                } else if (i == j) {
+                    b->m_val[MENTRY(n[i], n[j])] = 1;
+                }
+            }
+    }
+    #undef MENTRY
+    b->m_pre = xcalloc((size_t)nt, sizeof(double));
+    for (int i = 0; i < nt; ++i) b->m_pre[i] = 1.0 / b->m_val[find_entry(b, i, i)];
+    b->mass = 2;
+    return 0;
+}
+
+/* ConjugateGradientSolver::solve (ConjugateGradientSolver.cpp:192-215, 226-335): Jacobi-PCG
+ * on M a = f, x_0 = 0, stop when sum r^2 / sum f^2 < eps^2 at the start of an iteration.
+ * Returns the iterations, or -1 without convergence. Dropped zero entries change no sum. */
+static int mass_cg_solve(orc_blob *b, double *x) {
+    const int nn = b->nt;
+    double *r = b->r, *d = b->pv, *s = b->z, *q = b->q, *f = b->f;
+    const double *P = b->m_pre;
+    double delta_new = 0;
+    for (int i = 0; i < nn; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            r[3 * i + c] = x[3 * i + c];
+            f[3 * i + c] = x[3 * i + c];
+            x[3 * i + c] = 0;
+            d[3 * i + c] = P[i] * r[3 * i + c];
+        }
+        delta_new += r[3 * i] * d[3 * i] + r[3 * i + 1] * d[3 * i + 1] + r[3 * i + 2] * d[3 * i + 2];
+    }
+    for (int it = 0; it < b->p.max_iterations_cg; ++it) {
+        double r2 = 0, f2 = 0;
+        for (int i = 0; i < nn; ++i) {
+            r2 += r[3 * i] * r[3 * i] + r[3 * i + 1] * r[3 * i + 1] + r[3 * i + 2] * r[3 * i + 2];
+            f2 += f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2];
+        }
+        if ((f2 == 0.0 ? 0.0 : r2 / f2) < b->p.epsilon2) {
+            b->last_iters = it;
+            return it;
+        }
+        double dTq = 0;
+        for (int i = 0; i < nn; ++i) {
+            q[3 * i] = q[3 * i + 1] = q[3 * i + 2] = 0;
+            for (int k = b->m_ptr[i]; k < b->m_ptr[i + 1]; ++k) {
+                const int j = b->m_col[k];
+                for (int c = 0; c < 3; ++c) q[3 * i + c] += b->m_val[k] * d[3 * j + c];
+            }
+            dTq += d[3 * i] * q[3 * i] + d[3 * i + 1] * q[3 * i + 1] + d[3 * i + 2] * q[3 * i + 2];
+        }
+        const double alpha = delta_new / dTq;
+        for (int i = 0; i < 3 * nn; ++i) x[i] += alpha * d[i];
+        for (int i = 0; i < 3 * nn; ++i) r[i] += -alpha * q[i];
+        const double delta_old = delta_new;
+        delta_new = 0;
+        for (int i = 0; i < nn; ++i) {
+            for (int c = 0; c < 3; ++c) s[3 * i + c] = P[i] * r[3 * i + c];
+            delta_new += r[3 * i] * s[3 * i] + r[3 * i + 1] * s[3 * i + 1] + r[3 * i + 2] * s[3 * i + 2];
+        }
+        for (int i = 0; i < 3 * nn; ++i) d[i] = s[i] + (delta_new / delta_old) * d[i];
+    }
+    return -1;
 }
 
 /* Blob::make_measurements, kinetic energy (Blob.cpp:1076-1118; apply_element_mass_matrix,

@@ -95,6 +95,10 @@ void orc_rotational_force(const double *x, const double *axis, double F, int tor
 /* MassLumpedSolver (MassLumpedSolver.cpp:27-73) in place of the mass-less CG: rho [ne]
  * densities (meso units); velocities are set to zero. Returns 0 or 1 (bad density). */
 int orc_blob_set_mass_lumped(orc_blob *b, const double *rho);
+/* ConjugateGradientSolver (ConjugateGradientSolver.cpp:51-335) in place of the mass-less
+ * CG: the consistent mass matrix of the linear tets, solved by Jacobi-PCG each step;
+ * velocities are set to zero. Returns 0 or 1. */
+int orc_blob_set_mass_consistent(orc_blob *b, const double *rho);
 /* the kinetic energy of Blob::make_measurements (Blob.cpp:1076-1118), 0 without mass */
 double orc_blob_kinetic_energy(const orc_blob *b);
 

@@ -87,6 +87,8 @@ def lib():
         L.orc_rotational_force.argtypes = [_dp, _dp, C.c_double, C.c_int, _dp]
         L.orc_blob_set_mass_lumped.argtypes = [C.c_void_p, _dp]
         L.orc_blob_set_mass_lumped.restype = C.c_int
+        L.orc_blob_set_mass_consistent.argtypes = [C.c_void_p, _dp]
+        L.orc_blob_set_mass_consistent.restype = C.c_int
         L.orc_blob_kinetic_energy.argtypes = [C.c_void_p]
         L.orc_blob_kinetic_energy.restype = C.c_double
         L.orc_blob_add_node_forces.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
@@ -351,6 +353,12 @@ class Blob:
         """MassLumpedSolver in place of the mass-less CG (rho per element, meso units)"""
         r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (self.ne,)))
         if lib().orc_blob_set_mass_lumped(self.h, r) != 0:
+            raise ValueError("oracle densities rejected")
+
+    def set_mass_consistent(self, rho):
+        """ConjugateGradientSolver (consistent mass, CG each step) in place of the mass-less CG"""
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (self.ne,)))
+        if lib().orc_blob_set_mass_consistent(self.h, r) != 0:
             raise ValueError("oracle densities rejected")
 
     def kinetic_energy(self):

@@ -171,3 +171,65 @@ def test_world_files_of_an_inertial_blob(oracle_mod, tmp_path):
         assert o.step() == 0
     assert rel_err(got_pos, o.positions()) <= POS_TOL
     assert rel_err(got_vel, o.velocities()) <= POS_TOL * 100
+
+
+def cg_pair(oracle_mod, spec, precision="fp64"):
+    print(f"[case] consistent mass {spec.name} ne={spec.num_elements} {precision}", flush=True)
+    rho = np.full(spec.num_elements, RHO) * np.linspace(0.8, 1.2, spec.num_elements)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_consistent(rho)
+    g = GpuBlob.from_spec(spec, precision=precision)
+    g.set_solver("cg", rho)
+    return o, g
+
+
+def test_consistent_mass_pinned_blob_with_noise(oracle_mod):
+    """ConjugateGradientSolver: per-step right-hand side within 1e-12, positions within
+    1e-10 after 30 steps, iteration counts alike"""
+    spec = workloads.config3(n=5, epsilon=1e-12)
+    spec.calc_noise = 1
+    o, g = cg_pair(oracle_mod, spec)
+    lockstep(o, g, 4)
+    free_run(o, g, 30)
+    assert abs(g.cg_iterations()[0] - o.cg_iterations()) <= 1
+    g.close()
+
+
+def test_consistent_mass_free_blob_stokes_and_mixed(oracle_mod):
+    spec = stokes_spec(4, seed=8)
+    o, g = cg_pair(oracle_mod, spec)
+    free_run(o, g, 25)
+    g.close()
+    o, gm = cg_pair(oracle_mod, workloads.config3(n=5), "mixed")
+    for _ in range(15):
+        assert o.step() == 0
+    gm.step(15)
+    assert rel_err(gm.positions(), o.positions()) <= 1e-8
+    gm.close()
+
+
+def test_consistent_mass_script_and_switch_back(oracle_mod, tmp_path):
+    mesh = meshgen.kuhn_mesh(3, seed=5, jitter=0.1)
+    prefix = str(tmp_path / "c")
+    z = mesh.pos[:, 2]
+    meshgen.write_blob_files(mesh, prefix, pinned=np.nonzero(z <= z.min() + 1e-9)[0])
+    script = str(tmp_path / "c.ffea")
+    meshgen.write_ffea_script(script, prefix, solver="CG")
+    S = Script(script)
+    spec = S.blob_spec(0)
+    assert spec.solver == "cg"
+    print("[case] consistent mass from script", flush=True)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_consistent(spec.rho)
+    g = S.gpu_blob(0)
+    free_run(o, g, 12)
+    # back to the mass-less solver: the viscosity matrix is assembled again each step
+    g.set_solver("nomass")
+    o2 = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o2.set_positions(g.positions())
+    g.set_rng_state(o2.rng_state())
+    for _ in range(3):
+        assert o2.step() == 0
+    g.step(3)
+    assert rel_err(g.positions(), o2.positions()) <= POS_TOL
+    g.close()

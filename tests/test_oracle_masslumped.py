@@ -101,3 +101,42 @@ def test_kinetic_energy_of_a_uniform_motion(oracle_mod):
     want = 0.5 * (rho * vol0).sum() * (v @ v)
     assert abs(o.kinetic_energy() - want) <= 1e-12 * want
 
+
+
+def test_consistent_mass_momentum_and_solve(oracle_mod):
+    """ConjugateGradientSolver: its mass matrix's entries add up to the total mass (the
+    reference's own "Mass Solver Sums" check, ConjugateGradientSolver.cpp:165-171), so the
+    momentum 1^T M v grows by N dt F under a constant load F, to the CG tolerance; and the
+    accelerations differ from the lumped ones"""
+    spec, rho = free_spec()
+    spec.epsilon2 = 1e-24
+    nt = spec.pos.shape[0]
+    F = np.array([2e-3, -1e-3, 3e-3])
+    spec.ctf_node = np.arange(nt, dtype=np.int32)
+    spec.ctf_force = np.tile(F, (nt, 1))
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    o.set_mass_consistent(rho)
+    ol = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    ol.set_mass_lumped(rho)
+    _, vol0 = o.rest_state()
+    N = 10
+    for _ in range(N):
+        assert o.step() == 0
+        assert ol.step() == 0
+    v = o.velocities()
+    # 1^T M v with M assembled here from the element rule
+    Mv = np.zeros_like(v)
+    for e in range(spec.num_elements):
+        n = spec.top[e, :4]
+        for i in range(4):
+            for j in range(4):
+                Mv[n[i]] += (0.1 if i == j else 0.05) * rho[e] * vol0[e] * v[n[j]]
+    P = Mv.sum(axis=0)
+    want = N * spec.dt * nt * F
+    assert np.abs(P - want).max() <= 1e-9 * np.abs(want).max()
+    assert o.cg_iterations() > 1
+    assert rel_err_local(v, ol.velocities()) > 1e-6
+
+
+def rel_err_local(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
