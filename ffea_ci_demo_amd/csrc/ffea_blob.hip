@@ -2136,22 +2136,24 @@ int ffea_gpu_blob::step_mass() {
 }
 
 void ffea_gpu_blob::set_solver(int kind, const double *rho_h) {
-    if (mass_cg) any_project = project_saved;
+    if (kind != FFEA_GPU_SOLVER_NOMASS_CG) {  // every check before any state changes
+        if (kind != FFEA_GPU_SOLVER_MASSLUMPED && kind != FFEA_GPU_SOLVER_ITERATIVE)
+            throw Invalid(fmt("ffea_gpu_blob_set_solver: solver %d is not one this library runs (1 = ITERATIVE, 2 = "
+                              "MASSLUMPED, 3 = NOMASS_CG)", kind));
+        if (nblob != 1) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need a single-blob handle");
+        if (!rho_h) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need the element densities");
+        if (mg_on()) throw Invalid("ffea_gpu_blob_set_solver: switch the multigrid preconditioner off first");
+        for (int e = 0; e < ne; ++e)
+            if (!(rho_h[e] > 0.0) || !std::isfinite(rho_h[e]))
+                throw Invalid(fmt("ffea_gpu_blob_set_solver: element %d has density %g", e, rho_h[e]));
+    }
+    if (mass_cg) any_project = project_saved;  // leaving the mass CG: the mass-less solver's projection again
     if (kind == FFEA_GPU_SOLVER_NOMASS_CG) {
         mass = mass_cg = false;
         inv_m.reset();
         rho_dev.reset();
         return;
     }
-    if (kind != FFEA_GPU_SOLVER_MASSLUMPED && kind != FFEA_GPU_SOLVER_ITERATIVE)
-        throw Invalid(fmt("ffea_gpu_blob_set_solver: solver %d is not one this library runs (1 = ITERATIVE, 2 = "
-                          "MASSLUMPED, 3 = NOMASS_CG)", kind));
-    if (nblob != 1) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need a single-blob handle");
-    if (!rho_h) throw Invalid("ffea_gpu_blob_set_solver: the inertial solvers need the element densities");
-    if (mg_on()) throw Invalid("ffea_gpu_blob_set_solver: switch the multigrid preconditioner off first");
-    for (int e = 0; e < ne; ++e)
-        if (!(rho_h[e] > 0.0) || !std::isfinite(rho_h[e]))
-            throw Invalid(fmt("ffea_gpu_blob_set_solver: element %d has density %g", e, rho_h[e]));
     drop_graph();
     rho_dev.reset();
     rho_dev.alloc(ne, &dev_bytes);

@@ -233,3 +233,28 @@ def test_consistent_mass_script_and_switch_back(oracle_mod, tmp_path):
     g.step(3)
     assert rel_err(g.positions(), o2.positions()) <= POS_TOL
     g.close()
+
+
+def test_refused_solver_change_keeps_the_consistent_mass_solver():
+    """a refused ffea_gpu_blob_set_solver call changes nothing: a free mixed blob (whose
+    mass-less solve projects rigid modes out) on the consistent-mass CG steps bit for bit
+    like a twin that never saw the refused call"""
+    from ffea_ci_demo_amd import _native
+    spec = workloads.cube_blob(4, calc_noise=0, jitter=0.1, name="cube4_free")
+    spec.pinned = np.zeros(0, np.int32)
+    nt = spec.pos.shape[0]
+    spec.ctf_node = np.arange(nt, dtype=np.int32)
+    spec.ctf_force = np.tile([1e-3, -2e-3, 0.5e-3], (nt, 1))
+    rho = np.full(spec.num_elements, RHO)
+    print("[case] refused solver change, free mixed blob on the consistent-mass CG", flush=True)
+    ga, gb = GpuBlob.from_spec(spec, precision="mixed"), GpuBlob.from_spec(spec, precision="mixed")
+    ga.set_solver("cg", rho)
+    gb.set_solver("cg", rho)
+    with pytest.raises(FFEAException, match="not one this library runs"):
+        ga._check(_native.lib().ffea_gpu_blob_set_solver(ga.h, 99, rho.ctypes.data))
+    ga.step(5)
+    gb.step(5)
+    np.testing.assert_array_equal(ga.positions(), gb.positions())
+    assert np.abs(gb.velocities()).max() > 0
+    ga.close()
+    gb.close()
