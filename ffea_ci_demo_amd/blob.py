@@ -134,6 +134,14 @@ class _Handle:
             raise ValueError(f"unknown solver {kind!r}")
 
 
+def _node_vectors(a, nt):
+    """one FP64 3-vector per node, flattened; the library reads exactly 3 nt values"""
+    v = np.ascontiguousarray(a, np.float64).reshape(-1)
+    if v.size != 3 * nt:
+        raise ValueError(f"expected {nt} node 3-vectors ({3 * nt} values), got {v.size} values")
+    return v
+
+
 def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precision, device):
     prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
     return _native.Params(float(dt), float(kT), float(epsilon2), int(max_iterations_cg), int(calc_noise),
@@ -243,7 +251,7 @@ class GpuBlob(_Handle):
         return e.value
 
     def set_velocities(self, v):
-        self._check(_native.lib().ffea_gpu_blob_set_velocities(self.h, np.ascontiguousarray(v, np.float64).reshape(-1)))
+        self._check(_native.lib().ffea_gpu_blob_set_velocities(self.h, _node_vectors(v, self.nt)))
 
     def add_node_forces(self, nodes, forces):
         """Blob::add_force_to_node for the next step (meso units, list order)"""
@@ -263,7 +271,7 @@ class GpuBlob(_Handle):
         return out
 
     def set_positions(self, p):
-        self._check(_native.lib().ffea_gpu_blob_set_positions(self.h, np.ascontiguousarray(p, np.float64).reshape(-1)))
+        self._check(_native.lib().ffea_gpu_blob_set_positions(self.h, _node_vectors(p, self.nt)))
 
     def velocities(self):
         return self._get("ffea_gpu_blob_get_velocities", (self.nt, 3))
@@ -305,7 +313,7 @@ class GpuBlob(_Handle):
     def adjoint(self, dL_dpos):
         """reverse of the last checkpointed run: dL/dpos at its end -> (dL/dA, dL/dB, dL/dG,
         dL/dE per element, dL/dpos at its start)"""
-        g = np.ascontiguousarray(dL_dpos, np.float64).reshape(-1)
+        g = _node_vectors(dL_dpos, self.nt)
         outs = [np.zeros(self.ne) for _ in range(4)]
         d0 = np.zeros((self.nt, 3))
         self._check(_native.lib().ffea_gpu_blob_adjoint(self.h, g, *[o.ctypes.data for o in outs], d0.ctypes.data))
@@ -352,8 +360,9 @@ class GpuBatch(_Handle):
         return self._get("ffea_gpu_batch_get_positions", b, (self.nt[b], 3))
 
     def set_positions(self, b, p):
-        self._check(_native.lib().ffea_gpu_batch_set_positions(self.h, int(b),
-                                                               np.ascontiguousarray(p, np.float64).reshape(-1)))
+        if not 0 <= int(b) < len(self.nt):
+            raise IndexError(f"blob {b} is not in this ensemble of {len(self.nt)}")
+        self._check(_native.lib().ffea_gpu_batch_set_positions(self.h, int(b), _node_vectors(p, self.nt[b])))
 
     def velocities(self, b):
         return self._get("ffea_gpu_batch_get_velocities", b, (self.nt[b], 3))

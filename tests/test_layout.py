@@ -87,3 +87,15 @@ def test_stokes_layout_has_fold_records():
     specs.append(pinned)
     info = check_layout(specs)
     assert info["num_blobs"] == 3
+
+
+def test_node_vector_arguments_are_sized():
+    """positions, velocities and adjoint seeds reach the library as exactly 3 values per
+    node: a wrong-sized array is refused on the host before the library reads it"""
+    from ffea_ci_demo_amd.blob import _node_vectors
+    v = _node_vectors(np.arange(12.0).reshape(4, 3), 4)
+    assert v.dtype == np.float64 and v.shape == (12,) and v.flags.c_contiguous
+    np.testing.assert_array_equal(_node_vectors(np.ones((3, 4), np.float32).T, 4), np.ones(12))
+    for bad in (np.zeros((3, 3)), np.zeros((5, 3)), np.zeros(13)):
+        with pytest.raises(ValueError, match="node 3-vectors"):
+            _node_vectors(bad, 4)
