@@ -218,7 +218,8 @@ int ffea_gpu_blob_step(ffea_gpu_blob *blob, int nsteps);
  * every node under calc_stokes (Blob.cpp:3172-3196), and updates a = f / m, v += a dt,
  * x += v dt (Blob.cpp:3249-3280): no linear solve. Switching sets the velocities to zero;
  * rho may be NULL for NOMASS_CG. The multigrid preconditioner, graphs and the adjoint are
- * for NOMASS_CG only. */
+ * for NOMASS_CG only. A refused call (unknown solver, a batch handle, missing or
+ * non-positive densities, multigrid on) returns an error and leaves the handle as it was. */
 int ffea_gpu_blob_set_solver(ffea_gpu_blob *blob, int solver, const double *rho);
 
 /* Blob::make_measurements' kinetic energy (Blob.cpp:1076-1118): half the sum over elements
