@@ -35,7 +35,7 @@ EXPORTED = [
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
-    "ffea_gpu_blob_kinetic_energy",
+    "ffea_gpu_blob_kinetic_energy", "ffea_gpu_solver_create", "ffea_gpu_solver_solve", "ffea_gpu_solver_apply_matrix",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -102,6 +102,10 @@ def lib():
     L.ffea_gpu_blob_create_ex.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, _dp,
                                           C.c_int, vp, C.c_int, vp, vp, vp, _up, C.POINTER(SurfaceForces),
                                           C.POINTER(RotationalForces), C.c_int, vp, C.POINTER(vp)]
+    L.ffea_gpu_solver_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, _dp, _ip, _dp, _dp, C.c_int, vp, vp,
+                                         C.POINTER(vp)]
+    L.ffea_gpu_solver_solve.argtypes = [vp, _dp, _dp]
+    L.ffea_gpu_solver_apply_matrix.argtypes = [vp, _dp, _dp, _dp]
     L.ffea_gpu_blob_add_node_forces.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_get_node_positions.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_set_solver.argtypes = [vp, C.c_int, vp]

@@ -28,10 +28,17 @@ class BlobTrajectory(torch.autograd.Function):
         blob.set_positions(np.asarray(pos0, np.float64))
         blob.run_checkpointed(int(nsteps), int(interval))
         ctx.blob = blob
+        ctx.run_generation = blob.run_generation
         return torch.from_numpy(blob.positions())
 
     @staticmethod
     def backward(ctx, grad_pos):
+        # the library reverses the run the blob keeps, with its current material: refuse
+        # when another trajectory or a material change has replaced this forward's run
+        if ctx.blob.run_generation != ctx.run_generation:
+            raise RuntimeError("BlobTrajectory.backward: the blob has run another trajectory or changed its "
+                               "material since this forward; run each differentiated trajectory on its own blob "
+                               "or call backward before the next one")
         dA, dB, dG, dE, _ = ctx.blob.adjoint(grad_pos.detach().contiguous().numpy())
         t = torch.from_numpy
         return t(dG), t(dE), t(dA), t(dB), None, None, None, None

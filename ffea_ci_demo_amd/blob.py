@@ -127,6 +127,7 @@ class _Handle:
             r = np.asarray(rho, np.float64)
             r = np.ascontiguousarray(np.broadcast_to(r, (self.ne,)) if r.ndim == 0 else r.ravel())
             code = _native.SOLVER_MASSLUMPED if kind == "masslumped" else _native.SOLVER_ITERATIVE
+            self.run_generation = getattr(self, "run_generation", 0) + 1  # the library drops a kept run
             self._check(L.ffea_gpu_blob_set_solver(self.h, code, r.ctypes.data))
         elif kind == "nomass":
             self._check(L.ffea_gpu_blob_set_solver(self.h, _native.SOLVER_NOMASS_CG, None))
@@ -149,6 +150,8 @@ def _params(dt, kT, epsilon2, max_iterations_cg, calc_noise, calc_stokes, precis
 
 
 class GpuBlob(_Handle):
+    run_generation = 0  # bumped by every checkpointed run and material change
+
     def __init__(self, pos, top, A, B, G, E, *, dt, kT, epsilon2, max_iterations_cg=1000, calc_noise=1,
                  calc_stokes=0, pinned=(), ctf_node=(), ctf_force=(), stokes_drag=None, rng_state=None,
                  surface_forces=None, rotational_forces=None, external_nodes=(), precision="fp64", device=0):
@@ -304,10 +307,13 @@ class GpuBlob(_Handle):
     def set_material(self, A, B, G, E):
         """per-element A, B, G, E in meso units (scalars broadcast)"""
         arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (self.ne,))) for v in (A, B, G, E)]
+        self.run_generation += 1  # the library drops a kept run made with the earlier material
         self._check(_native.lib().ffea_gpu_blob_set_material(self.h, *arrs))
 
     def run_checkpointed(self, nsteps, interval=20):
-        """nsteps steps from the current state, positions kept every `interval` steps"""
+        """nsteps steps from the current state, positions kept every `interval` steps;
+        `run_generation` names the run kept (it changes with every run and material)"""
+        self.run_generation += 1
         self._check(_native.lib().ffea_gpu_blob_run_checkpointed(self.h, int(nsteps), int(interval)))
 
     def adjoint(self, dL_dpos):
@@ -318,6 +324,64 @@ class GpuBlob(_Handle):
         d0 = np.zeros((self.nt, 3))
         self._check(_native.lib().ffea_gpu_blob_adjoint(self.h, g, *[o.ctypes.data for o in outs], d0.ctypes.data))
         return outs[0], outs[1], outs[2], outs[3], d0
+
+
+class GpuSolver(_Handle):
+    """The mass-less viscosity solver alone, the reference's NoMassCGSolver behind its Solver
+    interface (Solver.h:31-52): created from the mesh, the per-element viscosities A, B, the
+    pins and (calc_stokes) the drag (NoMassCGSolver::init, NoMassCGSolver.cpp:56-153);
+    solve(pos, x) is NoMassCGSolver::solve (NoMassCGSolver.cpp:156-188) with the viscosity
+    matrix built at the node positions pos: x, the force vector after linearise_force and
+    pin zeroing, in; the velocities out. The caller's element loop forms the forces."""
+
+    def __init__(self, pos, top, A, B, *, dt, epsilon2, max_iterations_cg=1000, calc_stokes=0, pinned=(),
+                 stokes_drag=None, precision="fp64", device=0, kT=0.0):
+        L = _native.lib()
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        top = np.ascontiguousarray(top, np.int32).reshape(-1, 10)
+        self.nt, self.ne = pos.shape[0], top.shape[0]
+        A = np.ascontiguousarray(np.broadcast_to(np.asarray(A, np.float64), (self.ne,)))
+        B = np.ascontiguousarray(np.broadcast_to(np.asarray(B, np.float64), (self.ne,)))
+        pin_p, pin = _opt(pinned, np.int32)
+        npin = 0 if pin is None else pin.size
+        drag = None
+        if calc_stokes:
+            drag = np.ascontiguousarray(stokes_drag, np.float64).reshape(-1)
+            if drag.size != self.nt:
+                raise ValueError("stokes_drag needs one value per node")
+        prm = _params(dt, kT, epsilon2, max_iterations_cg, 0, calc_stokes, precision, device)
+        h = C.c_void_p()
+        rc = L.ffea_gpu_solver_create(C.byref(prm), self.nt, self.ne, pos.reshape(-1), top.reshape(-1), A, B,
+                                      npin, pin_p, drag.ctypes.data if drag is not None else None, C.byref(h))
+        if rc != _native.OK:
+            raise FFEAException(rc, L.ffea_gpu_last_error().decode())
+        self.h = h
+
+    @classmethod
+    def from_spec(cls, spec, precision="fp64", **kw):
+        return cls(spec.pos, spec.top, spec.A, spec.B, dt=spec.dt, epsilon2=spec.epsilon2,
+                   max_iterations_cg=spec.max_iterations_cg, calc_stokes=spec.calc_stokes, pinned=spec.pinned,
+                   stokes_drag=spec.stokes_drag, precision=precision, kT=spec.kT, **kw)
+
+    def solve(self, pos, rhs):
+        """the solution of V(pos) v = rhs, [num_nodes][3] (rhs is not modified)"""
+        p = _node_vectors(pos, self.nt)
+        x = _node_vectors(rhs, self.nt).copy()
+        self._check(_native.lib().ffea_gpu_solver_solve(self.h, p, x))
+        return x.reshape(self.nt, 3)
+
+    def apply_matrix(self, pos, v):
+        """V(pos) v over every row of the reference's matrix, [num_nodes][3]"""
+        out = np.zeros(3 * self.nt)
+        self._check(_native.lib().ffea_gpu_solver_apply_matrix(self.h, _node_vectors(pos, self.nt),
+                                                               _node_vectors(v, self.nt), out))
+        return out.reshape(self.nt, 3)
+
+    def cg_iterations(self):
+        last = C.c_int()
+        total = C.c_longlong()
+        self._check(_native.lib().ffea_gpu_blob_get_cg_iterations(self.h, C.byref(last), C.byref(total)))
+        return last.value, total.value
 
 
 class GpuBatch(_Handle):

@@ -1144,6 +1144,13 @@ struct ffea_gpu_blob {
     bool project_saved = false;        // any_project of the mass-less solver, off for the mass CG
     DevBuf<double> inv_m, rho_dev;     // [rows] 1 / lumped mass, [ne] densities
     int grid_rows = 0, grid_elems = 0, grid_blobs = 0;
+    // solver-only handle (ffea_gpu_solver_create): the mass-less viscosity solve of a caller
+    // whose Blob keeps its own element loop; no element forces, steps or adjoint
+    bool solver_only = false;
+    const double *rhs_in = nullptr;    // set while a handed-over right-hand side is gathered
+    std::vector<double> mid_diag;      // [num_nodes] diagonal of each midside node's identity row
+                                       // (1, plus the drag of an unpinned node under calc_stokes)
+    std::vector<double> h_vec;         // tiled host staging of one node vector
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
     bool fused_cg = false;
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
@@ -1306,6 +1313,9 @@ struct ffea_gpu_blob {
     int finish_step();
     int step_once();
     void capture_graph(int iters);
+    void solver_matrix_at(const double *pos_nodes);
+    int solver_solve(const double *pos_nodes, double *x);
+    void solver_apply(const double *pos_nodes, const double *in, double *out);
     int check_status_after_forces();
 };
 
@@ -1603,7 +1613,7 @@ void ffea_gpu_blob::launch_gather(const T *force) {
                     dyn_fold ? fold_mid.p : nullptr, dyn_fold && have_ctf ? node_ctf.p : nullptr,
                     have_sf ? sf_nf_ptr.p : nullptr, have_sf ? sf_nf.p : nullptr, have_sf ? sf_face_force.p : nullptr,
                     pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p,
-                    mass && stokes ? vel.p : nullptr, mass && stokes ? drag.p : nullptr};
+                    mass && stokes ? vel.p : nullptr, mass && stokes ? drag.p : nullptr, rhs_in};
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
@@ -2155,6 +2165,7 @@ void ffea_gpu_blob::set_solver(int kind, const double *rho_h) {
         return;
     }
     drop_graph();
+    run_steps = 0;  // the adjoint reverses mass-less runs only
     rho_dev.reset();
     rho_dev.alloc(ne, &dev_bytes);
     rho_dev.upload(rho_h, ne, stream);
@@ -2217,6 +2228,99 @@ int ffea_gpu_blob::step_once() {
     run_cg_eager(0);
     launch_update();
     return finish_step();
+}
+
+// ---------------------------------------------------------------- solver-only handles
+// The viscosity matrix at the caller's node positions: NoMassCGSolver::solve builds it from
+// the element matrices that Blob::update_internal_forces has just formed from those
+// positions (NoMassCGSolver.cpp:158-159, tetra_element_linear.cpp:222-327), so the handle
+// forms shape data and volumes from the same positions and assembles.
+void ffea_gpu_blob::solver_matrix_at(const double *pos_nodes) {
+    const BlobMeta &m = meta[0];
+    std::vector<double> p((size_t)3 * m.nl);
+    for (int n = 0; n < m.nl; ++n)
+        for (int c = 0; c < 3; ++c) p[3 * n + c] = pos_nodes[3 * m.orig_of_lin[n] + c];
+    FFEA_HIP_TRY(hipMemcpyAsync(pos.p, p.data(), sizeof(double) * p.size(), hipMemcpyHostToDevice, stream));
+    launch_element(false);
+    launch_assemble();
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));  // p goes out of scope
+}
+
+// NoMassCGSolver::solve (NoMassCGSolver.cpp:156-188): x holds the right-hand side on entry
+// and the solution on return, [num_nodes][3]. Corner rows go through the handle's CG (its
+// preconditioner, the reference's start and stopping test); pinned rows are identity rows
+// there. Midside rows are identity rows that Blob::linearise_force hands over zero
+// (Blob.cpp:781-799): a nonzero one is refused, and they return zero.
+int ffea_gpu_blob::solver_solve(const double *pos_nodes, double *xio) {
+    const BlobMeta &m = meta[0];
+    for (int v = 0; v < m.nt; ++v)
+        if (m.lin_of[v] < 0)
+            for (int c = 0; c < 3; ++c)
+                if (xio[3 * v + c] != 0.0)
+                    throw Invalid(fmt("ffea_gpu_solver_solve: node %d is a midside node with a nonzero right-hand "
+                                      "side (linearise_force leaves midside entries zero)", v));
+    h_vec.assign((size_t)nslices * 3 * kSlice, 0.0);
+    for (int n = 0; n < m.nl; ++n)
+        for (int c = 0; c < 3; ++c) h_vec[vi(n, c)] = xio[3 * m.orig_of_lin[n] + c];
+    FFEA_HIP_TRY(hipMemcpyAsync(r.p, h_vec.data(), sizeof(double) * h_vec.size(), hipMemcpyHostToDevice, stream));
+    solver_matrix_at(pos_nodes);
+    // r holds the right-hand side; the gather copies it into rhs, then starts the CG on it
+    rhs_in = r.p;
+    try {
+        hipEvent_t t0;
+        t_begin(2, &t0);
+        if (mixed) {
+            if (any_project) launch_gather<float, true>(f_force.p);
+            else launch_gather<float, false>(f_force.p);
+        } else {
+            launch_gather<double, false>(d_force.p);
+        }
+        FFEA_HIP_TRY(hipGetLastError());
+        if (mg_on()) launch_mg_vcycle(Pi.p);
+        t_end(2, t0);
+    } catch (...) {
+        rhs_in = nullptr;
+        throw;
+    }
+    rhs_in = nullptr;
+    run_cg_eager(0);
+    if (mg_on() && m.free_blob) launch_mg_gauge();  // Jacobi-PCG's rigid-mode gauge
+    hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    FFEA_HIP_TRY(hipMemcpyAsync(h_vec.data(), x.p, sizeof(double) * h_vec.size(), hipMemcpyDeviceToHost, stream));
+    const int rc = finish_step();
+    if (rc != FFEA_GPU_OK) return rc;  // finish_step synchronised the stream
+    std::fill(xio, xio + (size_t)3 * m.nt, 0.0);
+    for (int n = 0; n < m.nl; ++n)
+        for (int c = 0; c < 3; ++c) xio[3 * m.orig_of_lin[n] + c] = h_vec[vi(n, c)];
+    return FFEA_GPU_OK;
+}
+
+// out = V in with the viscosity matrix solver_solve would use at these positions, every
+// row of the reference's matrix included: corner rows from the block matrix, midside rows
+// identity rows (with the drag of an unpinned node under calc_stokes,
+// NoMassCGSolver.cpp:97-127)
+void ffea_gpu_blob::solver_apply(const double *pos_nodes, const double *in, double *out) {
+    const BlobMeta &m = meta[0];
+    h_vec.assign((size_t)nslices * 3 * kSlice, 0.0);
+    for (int n = 0; n < m.nl; ++n)
+        for (int c = 0; c < 3; ++c) h_vec[vi(n, c)] = in[3 * m.orig_of_lin[n] + c];
+    FFEA_HIP_TRY(hipMemcpyAsync(z.p, h_vec.data(), sizeof(double) * h_vec.size(), hipMemcpyHostToDevice, stream));
+    solver_matrix_at(pos_nodes);
+    if (mixed)
+        hipLaunchKernelGGL(k_apply_matrix<float>, dim3(grid_rows), dim3(kBlock), 0, stream, nrows,
+                           (const int *)slice_w.p, (const long long *)slice_base.p, (const int *)col.p,
+                           (const float *)f_val.p, (const double *)z.p, q.p);
+    else
+        hipLaunchKernelGGL(k_apply_matrix<double>, dim3(grid_rows), dim3(kBlock), 0, stream, nrows,
+                           (const int *)slice_w.p, (const long long *)slice_base.p, (const int *)col.p,
+                           (const double *)d_val.p, (const double *)z.p, q.p);
+    hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
+    FFEA_HIP_TRY(hipGetLastError());
+    FFEA_HIP_TRY(hipMemcpyAsync(h_vec.data(), q.p, sizeof(double) * h_vec.size(), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    for (int v = 0; v < m.nt; ++v)
+        for (int c = 0; c < 3; ++c) out[3 * v + c] = m.lin_of[v] >= 0 ? h_vec[vi(m.lin_of[v], c)] : mid_diag[v] * in[3 * v + c];
 }
 
 // ---------------------------------------------------------------- checkpointed run and reverse
@@ -2545,6 +2649,77 @@ int ffea_gpu_blob_create_ex(const ffea_gpu_params *params, int num_nodes, int nu
     return create_handle("ffea_gpu_blob_create", params, {bi}, out);
 }
 
+int ffea_gpu_solver_create(const ffea_gpu_params *params, int num_nodes, int num_elements, const double *pos,
+                           const int *top, const double *A, const double *B, int num_pinned, const int *pinned,
+                           const double *stokes_drag, ffea_gpu_blob **out) {
+    if (out) *out = nullptr;
+    if (!params || !out || num_nodes <= 0 || num_elements <= 0 || !pos || !top || !A || !B || num_pinned < 0 ||
+        (num_pinned > 0 && !pinned)) {
+        g_last_error = "ffea_gpu_solver_create: invalid argument";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    if (params->calc_stokes && !stokes_drag) {
+        g_last_error = "ffea_gpu_solver_create: calc_stokes = 1 needs the per-node Stokes drag";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    for (int i = 0; i < num_pinned; ++i)
+        if (pinned[i] < 0 || pinned[i] >= num_nodes) {
+            g_last_error = fmt("Error: Pinned node %d references an out of bounds node index.", i);
+            return FFEA_GPU_ERR_INVALID;
+        }
+    // the solver needs the mesh, the viscosities, the pins and the drag only: no moduli, no
+    // noise (the caller's element loop forms the forces)
+    ffea_gpu_params prm = *params;
+    prm.calc_noise = 0;
+    const std::vector<double> zero((size_t)num_elements, 0.0);
+    static const uint32_t kAnyState[6] = {12345u, 12345u, 12345u, 12345u, 12345u, 12345u};
+    const BlobInput bi{num_nodes, num_elements, pos,     top,       A,       B,       zero.data(),
+                       zero.data(), num_pinned, pinned,  0,         nullptr, nullptr, kAnyState,
+                       prm.calc_stokes ? stokes_drag : nullptr};
+    const int rc = create_handle("ffea_gpu_solver_create", &prm, {bi}, out);
+    if (rc != FFEA_GPU_OK) return rc;
+    ffea_gpu_blob *b = *out;
+    b->solver_only = true;
+    std::vector<unsigned char> pin(num_nodes, 0);
+    for (int i = 0; i < num_pinned; ++i) pin[pinned[i]] = 1;
+    b->mid_diag.assign(num_nodes, 1.0);
+    for (int v = 0; v < num_nodes; ++v)
+        if (prm.calc_stokes && !pin[v]) b->mid_diag[v] = 1.0 + stokes_drag[v];  // &one, then &stokes_drag
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_solver_solve(ffea_gpu_blob *b, const double *pos, double *x) {
+    if (!b || !pos || !x) return FFEA_GPU_ERR_INVALID;
+    if (!b->solver_only) {
+        b->err = "ffea_gpu_solver_solve: not a handle from ffea_gpu_solver_create";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        if (b->active_blobs == 0) {  // every node pinned: identity rows only
+            const BlobMeta &m = b->meta[0];
+            for (int v = 0; v < m.nt; ++v)
+                if (m.lin_of[v] < 0)
+                    for (int c = 0; c < 3; ++c) x[3 * v + c] = 0.0;
+            return (int)FFEA_GPU_OK;
+        }
+        return b->solver_solve(pos, x);
+    });
+}
+
+int ffea_gpu_solver_apply_matrix(ffea_gpu_blob *b, const double *pos, const double *in, double *out) {
+    if (!b || !pos || !in || !out) return FFEA_GPU_ERR_INVALID;
+    if (!b->solver_only) {
+        b->err = "ffea_gpu_solver_apply_matrix: not a handle from ffea_gpu_solver_create";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        b->solver_apply(pos, in, out);
+        return (int)FFEA_GPU_OK;
+    });
+}
+
 int ffea_gpu_batch_create(const ffea_gpu_params *params, int num_blobs, const int *num_nodes,
                           const int *num_elements, const double *const *pos, const int *const *top,
                           const double *const *A, const double *const *B, const double *const *G,
@@ -2606,8 +2781,17 @@ void ffea_gpu_blob_destroy(ffea_gpu_blob *blob) {
     delete blob;
 }
 
+namespace {
+// a solver-only handle holds the viscosity solve only
+int solver_refuses(ffea_gpu_blob *b, const char *who) {
+    b->err = std::string(who) + ": a handle from ffea_gpu_solver_create only solves (ffea_gpu_solver_solve)";
+    return FFEA_GPU_ERR_INVALID;
+}
+}  // namespace
+
 int ffea_gpu_blob_update_internal_forces(ffea_gpu_blob *b) {
     if (!b) return FFEA_GPU_ERR_INVALID;
+    if (b->solver_only) return solver_refuses(b, "ffea_gpu_blob_update_internal_forces");
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
@@ -2618,6 +2802,7 @@ int ffea_gpu_blob_update_internal_forces(ffea_gpu_blob *b) {
 
 int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
     if (!b) return FFEA_GPU_ERR_INVALID;
+    if (b->solver_only) return solver_refuses(b, "ffea_gpu_blob_update_positions");
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         if (b->active_blobs == 0) return (int)FFEA_GPU_OK;
@@ -2632,6 +2817,7 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
 
 int ffea_gpu_blob_step(ffea_gpu_blob *b, int nsteps) {
     if (!b || nsteps < 0) return FFEA_GPU_ERR_INVALID;
+    if (b->solver_only) return solver_refuses(b, "ffea_gpu_blob_step");
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         for (int s = 0; s < nsteps; ++s) {
@@ -2644,6 +2830,7 @@ int ffea_gpu_blob_step(ffea_gpu_blob *b, int nsteps) {
 
 int ffea_gpu_blob_set_solver(ffea_gpu_blob *b, int solver, const double *rho) {
     if (!b) return FFEA_GPU_ERR_INVALID;
+    if (b->solver_only) return solver_refuses(b, "ffea_gpu_blob_set_solver");
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         b->set_solver(solver, rho);
@@ -3096,12 +3283,14 @@ int ffea_gpu_blob_set_material(ffea_gpu_blob *b, const double *A, const double *
         FFEA_HIP_TRY(hipSetDevice(b->device));
         if (b->mixed) b->set_material_t<float>(A, B, G, E);
         else b->set_material_t<double>(A, B, G, E);
+        b->run_steps = 0;  // a stored run was made with the earlier material: nothing left to reverse
         return (int)FFEA_GPU_OK;
     });
 }
 
 int ffea_gpu_blob_run_checkpointed(ffea_gpu_blob *b, int nsteps, int interval) {
     if (!b || nsteps <= 0 || interval <= 0) return FFEA_GPU_ERR_INVALID;
+    if (b->solver_only) return solver_refuses(b, "ffea_gpu_blob_run_checkpointed");
     return guarded(b, [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         return b->forward_checkpointed(nsteps, interval);

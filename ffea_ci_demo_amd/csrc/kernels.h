@@ -841,6 +841,8 @@ struct GatherArgs {
     double *P;                  // chunk partials
     const double *mass_vel = nullptr;   // inertial solvers under calc_stokes: [rows][3] velocities
     const double *mass_drag = nullptr;  // and [rows] drags, for the drag force -drag v
+    const double *rhs_in = nullptr;     // tiled [rows][3]: a right-hand side handed over whole
+                                        // (the solver's own entry point), or null: formed here
 };
 
 // Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
@@ -862,72 +864,77 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
         const double3 cen = a.centre[ch.blob];
         for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
             double f[3] = {0.0, 0.0, 0.0};
-            if (a.ctf) {
-                f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
-            }
-            for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
-                const T *src = a.force + 3 * (size_t)k;
-                f[0] += (double)src[0];
-                f[1] += (double)src[1];
-                f[2] += (double)src[2];
-            }
-            if (a.sf_nf_ptr) {  // surface forces after the element forces, face order (Blob.cpp:3160-3170)
-                const int g = a.row_node[n];
-                for (int k = a.sf_nf_ptr[g]; k < a.sf_nf_ptr[g + 1]; ++k) {
-                    const double *ff = a.sf_face_force + 3 * a.sf_nf[k];
-                    f[0] += ff[0];
-                    f[1] += ff[1];
-                    f[2] += ff[2];
-                }
-            }
-            if (a.mass_vel) {  // inertial solvers: -drag v before the noise (Blob.cpp:3185-3190)
+            if (a.rhs_in) {  // NoMassCGSolver::solve's x on entry: pinned rows are identity rows
 #pragma unroll
-                for (int k = 0; k < 3; ++k) f[k] -= a.mass_vel[3 * n + k] * a.mass_drag[n];
-            }
-            if (a.fold_mid) {
-                // the node's own noise, then Blob::linearise_force (Blob.cpp:781-799): each
-                // record is one (element, corner) in element order; its midside nodes not
-                // folded by an earlier element contribute half of (constant force + surface
-                // forces - noise)
-                if (a.noise) {
+                for (int k = 0; k < 3; ++k) f[k] = a.rhs_in[vi(n, k)];
+            } else {
+                if (a.ctf) {
+                    f[0] = a.ctf[3 * n]; f[1] = a.ctf[3 * n + 1]; f[2] = a.ctf[3 * n + 2];
+                }
+                for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k) {
+                    const T *src = a.force + 3 * (size_t)k;
+                    f[0] += (double)src[0];
+                    f[1] += (double)src[1];
+                    f[2] += (double)src[2];
+                }
+                if (a.sf_nf_ptr) {  // surface forces after the element forces, face order (Blob.cpp:3160-3170)
                     const int g = a.row_node[n];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) f[k] -= a.noise[3 * g + k];
+                    for (int k = a.sf_nf_ptr[g]; k < a.sf_nf_ptr[g + 1]; ++k) {
+                        const double *ff = a.sf_face_force + 3 * a.sf_nf[k];
+                        f[0] += ff[0];
+                        f[1] += ff[1];
+                        f[2] += ff[2];
+                    }
                 }
-                for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
-                    const int3 mm = a.fold_mid[k];
-                    const int m3[3] = {mm.x, mm.y, mm.z};
-                    double s3[3] = {0.0, 0.0, 0.0};
+                if (a.mass_vel) {  // inertial solvers: -drag v before the noise (Blob.cpp:3185-3190)
 #pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        double val[3] = {0.0, 0.0, 0.0};
-                        if (m3[t] >= 0) {
+                    for (int k = 0; k < 3; ++k) f[k] -= a.mass_vel[3 * n + k] * a.mass_drag[n];
+                }
+                if (a.fold_mid) {
+                    // the node's own noise, then Blob::linearise_force (Blob.cpp:781-799): each
+                    // record is one (element, corner) in element order; its midside nodes not
+                    // folded by an earlier element contribute half of (constant force + surface
+                    // forces - noise)
+                    if (a.noise) {
+                        const int g = a.row_node[n];
 #pragma unroll
-                            for (int c = 0; c < 3; ++c) val[c] = a.node_ctf ? a.node_ctf[3 * m3[t] + c] : 0.0;
-                            if (a.sf_nf_ptr)
-                                for (int q = a.sf_nf_ptr[m3[t]]; q < a.sf_nf_ptr[m3[t] + 1]; ++q) {
-                                    const double *ff = a.sf_face_force + 3 * a.sf_nf[q];
+                        for (int k = 0; k < 3; ++k) f[k] -= a.noise[3 * g + k];
+                    }
+                    for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
+                        const int3 mm = a.fold_mid[k];
+                        const int m3[3] = {mm.x, mm.y, mm.z};
+                        double s3[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-                                    for (int c = 0; c < 3; ++c) val[c] += ff[c];
-                                }
-                            if (a.noise)
+                        for (int t = 0; t < 3; ++t) {
+                            double val[3] = {0.0, 0.0, 0.0};
+                            if (m3[t] >= 0) {
 #pragma unroll
-                                for (int c = 0; c < 3; ++c) val[c] -= a.noise[3 * m3[t] + c];
+                                for (int c = 0; c < 3; ++c) val[c] = a.node_ctf ? a.node_ctf[3 * m3[t] + c] : 0.0;
+                                if (a.sf_nf_ptr)
+                                    for (int q = a.sf_nf_ptr[m3[t]]; q < a.sf_nf_ptr[m3[t] + 1]; ++q) {
+                                        const double *ff = a.sf_face_force + 3 * a.sf_nf[q];
+#pragma unroll
+                                        for (int c = 0; c < 3; ++c) val[c] += ff[c];
+                                    }
+                                if (a.noise)
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) val[c] -= a.noise[3 * m3[t] + c];
+                            }
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) s3[c] = t == 0 ? val[c] : s3[c] + val[c];
                         }
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) s3[c] = t == 0 ? val[c] : s3[c] + val[c];
+                        for (int c = 0; c < 3; ++c) f[c] += 0.5 * s3[c];
                     }
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) f[c] += 0.5 * s3[c];
+                } else if (a.fold_ptr) {
+                    for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
+                        f[0] += a.fold[3 * k];
+                        f[1] += a.fold[3 * k + 1];
+                        f[2] += a.fold[3 * k + 2];
+                    }
                 }
-            } else if (a.fold_ptr) {
-                for (uint32_t k = a.fold_ptr[n]; k < a.fold_ptr[n + 1]; ++k) {
-                    f[0] += a.fold[3 * k];
-                    f[1] += a.fold[3 * k + 1];
-                    f[2] += a.fold[3 * k + 2];
-                }
+                if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
             }
-            if (a.pinned[n]) f[0] = f[1] = f[2] = 0.0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) a.rhs[vi(n, k)] = f[k];
             if constexpr (!PROJ) {
@@ -1237,6 +1244,31 @@ template <>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
     SpmvArgs<float> a) {
     cg_spmv<float, true>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
+}
+
+// out = V in over all rows (tiled vectors), one lane per row, blocks in slot order:
+// SparseMatrixFixedPattern::apply (SparseMatrixFixedPattern.cpp:148-187) on the block
+// matrix; padding slots hold zero blocks
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply_matrix(int nrows, const int *slice_w, const long long *slice_base,
+                                                         const int *col, const T *val, const double *in,
+                                                         double *out) {
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= nrows) return;
+    const int sl = n / kSlice, lane = n % kSlice;
+    const int W = slice_w[sl];
+    const long long base = slice_base[sl];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < W; ++j) {
+        const int cc = col[base + (long long)j * kSlice + lane];
+        T w[9];
+        load_block<T>(val + (size_t)base * 9 + (size_t)j * 9 * kSlice, lane, w);
+        const double v0 = in[vi(cc, 0)], v1 = in[vi(cc, 1)], v2 = in[vi(cc, 2)];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) acc[r] += (double)w[3 * r] * v0 + (double)w[3 * r + 1] * v1 + (double)w[3 * r + 2] * v2;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[vi(n, r)] = acc[r];
 }
 
 struct VecArgs {

@@ -67,8 +67,8 @@ def kuhn_mesh(n: int, jitter: float = 0.1, seed: int = 0, flatten_frac: float = 
     """n^3 cubes, 6n^3 quadratic tets, interior corner nodes jittered uniformly by
     +-jitter (per axis, in edge units) from `seed`; boundary nodes not jittered.
 
-    flatten_frac > 0 squashes a layer of corner nodes so that about that fraction
-    of tets get aspect ratio ~flatten_aspect (see flatten_layers)."""
+    flatten_frac > 0 squashes about that fraction of the cubes to height
+    1/flatten_aspect (see flatten_layers)."""
     n = int(n)
     N1 = n + 1
     nl = N1 ** 3
@@ -162,31 +162,42 @@ def kuhn_mesh(n: int, jitter: float = 0.1, seed: int = 0, flatten_frac: float = 
 
 
 def flatten_layers(pos_lin, grid, n, frac, aspect, rng):
-    """Squash tets to aspect ~`aspect`: for randomly chosen interior z-layers of
-    corner nodes, move the layer towards the layer below so the cubes between them
-    have height 1/aspect. About frac*6n^3 tets are affected (whole cube layers
-    within a random x-y window), and no tet inverts since z order is kept."""
+    """Squash about frac*n^3 cubes to height 1/aspect (edge units).
+
+    The squashed cubes form a square window of node columns in one or more interior
+    cube layers L (never two adjacent layers; 1 <= L <= n-2). Inside the window node
+    plane L+1 is put on its grid point lowered to z = L + 1/aspect, and node plane L on
+    its grid points (z = L) inside the window and one column around it: these nodes
+    carry no jitter, so no tet of a squashed cube or of the wedges round it changes its
+    orientation. A squashed cube is a regular unit square prism of height 1/aspect
+    split into 6 tets, each of volume 1/(6 aspect) (no tet is close to degenerate), with
+    longest edge over smallest height between 1.4 and 2 times `aspect`. Cubes bordering
+    the window become wedges; the cubes above it become about 2 - 1/aspect tall."""
     pos = pos_lin.copy()
-    N1 = n + 1
-    target = frac * n ** 3  # cubes to flatten
+    target = frac * n ** 3  # cubes to squash
+    n_in = max(1, n - 2)     # interior layers 1..n-2
     per_layer = n * n
-    nlayers = max(1, int(round(target / per_layer)))
-    if nlayers * per_layer > target * 1.5:
-        # a partial window inside one layer
-        w = max(1, int(round(np.sqrt(target))))
-        layers = [int(rng.integers(1, n - 1))]
-        x0 = int(rng.integers(0, max(1, n - w)))
-        y0 = int(rng.integers(0, max(1, n - w)))
-        win = (x0, x0 + w, y0, y0 + w)
-    else:
-        layers = sorted(rng.choice(np.arange(1, n - 1), size=min(nlayers, n - 2), replace=False).tolist())
-        win = (0, n, 0, n)
+    m = int(min(max(1, np.ceil(target / per_layer)), (n_in + 1) // 2))
+    w = int(min(n, max(1, round(np.sqrt(target / m)))))
+    # m layers from 1..n-2, pairwise non-adjacent
+    choices = np.arange(1, n - 1)
+    layers = []
+    for L in rng.permutation(choices):
+        if all(abs(int(L) - q) > 1 for q in layers):
+            layers.append(int(L))
+        if len(layers) == m:
+            break
+    x0 = int(rng.integers(0, n - w + 1))
+    y0 = int(rng.integers(0, n - w + 1))
     h = 1.0 / aspect
-    for L in layers:
-        sel = (grid[:, 2] == L + 1) & (grid[:, 0] >= win[0]) & (grid[:, 0] <= win[1]) & \
-              (grid[:, 1] >= win[2]) & (grid[:, 1] <= win[3]) & (grid[:, 2] < n)
-        below = ((grid[sel, 0] * N1 + grid[sel, 1]) * N1 + L).astype(np.int64)
-        pos[sel, 2] = pos[below, 2] + h
+    inwin = (grid[:, 0] >= x0) & (grid[:, 0] <= x0 + w) & (grid[:, 1] >= y0) & (grid[:, 1] <= y0 + w)
+    rim = (grid[:, 0] >= x0 - 1) & (grid[:, 0] <= x0 + w + 1) & (grid[:, 1] >= y0 - 1) & (grid[:, 1] <= y0 + w + 1)
+    for L in sorted(layers):
+        bot = rim & (grid[:, 2] == L)
+        top = inwin & (grid[:, 2] == L + 1)
+        pos[bot] = grid[bot]
+        pos[top] = grid[top]
+        pos[top, 2] = float(L) + h
     return pos
 
 

@@ -183,6 +183,41 @@ int ffea_gpu_check_layout(const ffea_gpu_params *params, int num_blobs, const in
                           const double *const *ctf_force, const double *const *stokes_drag,
                           const uint32_t *rng_states, ffea_gpu_blob_info *info);
 
+/* The mass-less viscosity solver on its own, for a caller whose Blob keeps its CPU element
+ * loop and hands the solve to the GPU through the reference's Solver interface
+ * (Solver.h:31-52; Blob creates its solver by enum, Blob.cpp:228-247, and calls
+ * solver->solve(force) in Blob::aggregate_forces_and_solve, Blob.cpp:3243).
+ *   ffea_gpu_solver_create   NoMassCGSolver::init (NoMassCGSolver.cpp:56-153): the mesh
+ *                            (pos, top as for ffea_gpu_blob_create), the per-element A, B,
+ *                            the pinned nodes and, with params->calc_stokes, the per-node
+ *                            drag. The moduli and the noise belong to the caller's element
+ *                            loop and are not taken. The handle is an ffea_gpu_blob for
+ *                            ffea_gpu_blob_destroy, _last_error, _get_info,
+ *                            _get_cg_iterations, _set_preconditioner, _autotune and
+ *                            _set_timing; it refuses the step, force, solver and adjoint calls.
+ *   ffea_gpu_solver_solve    NoMassCGSolver::solve (NoMassCGSolver.cpp:156-188), in place: x
+ *                            [num_nodes][3] holds the right-hand side (the force vector after
+ *                            linearise_force and pin zeroing) on entry and the solution on
+ *                            return. pos [num_nodes][3] are the node positions the caller's
+ *                            element loop used for this step: the viscosity matrix is built at
+ *                            them (the element matrices, tetra_element_linear.cpp:222-327, and
+ *                            SparseMatrixFixedPattern::build, SparseMatrixFixedPattern.cpp:86-94).
+ *                            Jacobi-PCG from x = 0 with the reference's stopping test (or the
+ *                            multigrid preconditioner when set); FFEA_GPU_ERR_CG_NO_CONVERGE
+ *                            carries the reference's message. Pinned rows are identity rows.
+ *                            Midside rows are identity rows that linearise_force leaves zero:
+ *                            a nonzero midside entry is refused, and they return zero.
+ *   ffea_gpu_solver_apply_matrix  out = V in [num_nodes][3] with the matrix solve builds at
+ *                            pos, midside and pinned rows as the identity rows of the
+ *                            reference's matrix (with the drag of an unpinned node under
+ *                            calc_stokes). The reference's NoMassCGSolver::apply_matrix is an
+ *                            empty stub (NoMassCGSolver.h:103). */
+int ffea_gpu_solver_create(const ffea_gpu_params *params, int num_nodes, int num_elements, const double *pos,
+                           const int *top, const double *A, const double *B, int num_pinned, const int *pinned,
+                           const double *stokes_drag, ffea_gpu_blob **out);
+int ffea_gpu_solver_solve(ffea_gpu_blob *solver, const double *pos, double *x);
+int ffea_gpu_solver_apply_matrix(ffea_gpu_blob *solver, const double *pos, const double *in, double *out);
+
 /* releases device memory and the stream; NULL is accepted */
 void ffea_gpu_blob_destroy(ffea_gpu_blob *blob);
 
@@ -303,7 +338,10 @@ int ffea_gpu_blob_get_info(ffea_gpu_blob *blob, ffea_gpu_blob_info *info);
 
 /* Differentiation of a blob trajectory (single-blob handles, calc_noise = 0, some nodes
  * pinned, no surface, rotational or external forces). No counterpart in the reference.
- *   ffea_gpu_blob_set_material   replaces the per-element A, B, G, E (meso units, .top order)
+ *   ffea_gpu_blob_set_material   replaces the per-element A, B, G, E (meso units, .top order);
+ *                                a run kept by ffea_gpu_blob_run_checkpointed before it is
+ *                                dropped (it was made with the earlier material), as it is
+ *                                by ffea_gpu_blob_set_solver with an inertial solver
  *   ffea_gpu_blob_run_checkpointed  nsteps steps of ffea_gpu_blob_step from the current
  *                                state, keeping the positions of every interval-th step on
  *                                the device

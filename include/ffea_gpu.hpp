@@ -130,6 +130,48 @@ private:
     int num_nodes_ = 0;
 };
 
+// The reference's NoMassCGSolver on the GPU behind the shape of its Solver interface
+// (Solver.h:31-52), for a Blob that keeps its CPU element loop: constructed where
+// NoMassCGSolver::init runs (Blob.cpp:247) and asked to solve in place where Blob calls
+// solver->solve(force) (Blob.cpp:3243). pos are the node positions the element loop used
+// for the step ([num_nodes][3]); x holds the right-hand side on entry, the velocities on
+// return.
+class NoMassSolver {
+public:
+    NoMassSolver(const ffea_gpu_params &p, int num_nodes, int num_elements, const double *pos, const int *top,
+                 const double *A, const double *B, int num_pinned, const int *pinned,
+                 const double *stokes_drag = nullptr)
+        : num_nodes_(num_nodes) {
+        if (int rc = ffea_gpu_solver_create(&p, num_nodes, num_elements, pos, top, A, B, num_pinned, pinned,
+                                            stokes_drag, &b_))
+            throw Exception(rc, ffea_gpu_last_error());
+    }
+    ~NoMassSolver() { ffea_gpu_blob_destroy(b_); }
+    NoMassSolver(const NoMassSolver &) = delete;
+    NoMassSolver &operator=(const NoMassSolver &) = delete;
+
+    void solve(const double *pos, double *x) { check(ffea_gpu_solver_solve(b_, pos, x)); }
+    void apply_matrix(const double *pos, const double *in, double *out) {
+        check(ffea_gpu_solver_apply_matrix(b_, pos, in, out));
+    }
+    // FFEA_GPU_PRECOND_JACOBI (the reference's) or FFEA_GPU_PRECOND_MULTIGRID
+    void set_preconditioner(int kind) { check(ffea_gpu_blob_set_preconditioner(b_, kind)); }
+    int cg_iterations() {
+        int last = 0;
+        check(ffea_gpu_blob_get_cg_iterations(b_, &last, nullptr));
+        return last;
+    }
+    int num_nodes() const { return num_nodes_; }
+    ffea_gpu_blob *handle() { return b_; }
+
+private:
+    void check(int rc) {
+        if (rc) throw Exception(rc, ffea_gpu_blob_last_error(b_));
+    }
+    ffea_gpu_blob *b_ = nullptr;
+    int num_nodes_ = 0;
+};
+
 // Every blob of a script with the reference's output files: World::init / World::run
 // (World.cpp:155-930, 1747-2060), restart included.
 class World {

@@ -61,13 +61,14 @@ def test_cpp_wrapper_and_example_build(tmp_path):
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("no g++")
-    out = str(tmp_path / "run_script")
-    cmd = [gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "ffea_ci_demo_amd", "examples", "run_script.cpp"),
-           "-L", os.path.join(ROOT, "ffea_ci_demo_amd"), "-lffea_gpu",
-           "-Wl,-rpath," + os.path.join(ROOT, "ffea_ci_demo_amd"), "-o", out]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr
+    for name in ("run_script", "solver_drop_in"):
+        out = str(tmp_path / name)
+        cmd = [gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"),
+               os.path.join(ROOT, "ffea_ci_demo_amd", "examples", name + ".cpp"),
+               "-L", os.path.join(ROOT, "ffea_ci_demo_amd"), "-lffea_gpu",
+               "-Wl,-rpath," + os.path.join(ROOT, "ffea_ci_demo_amd"), "-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
 
 
 def test_rng_stream_state_matches_oracle_streams(oracle_mod):

@@ -133,3 +133,24 @@ def test_autograd_function_matches_adjoint():
         fd = (f(1.0) - f(-1.0)) / (2 * h)
     ad = float((G.grad * G.detach() * d).sum())
     assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+
+
+def test_stale_run_refused():
+    """a second trajectory on the same blob before backward, and a material change after a
+    kept run, are refused instead of reversing another run"""
+    import torch
+    from ffea_ci_demo_amd.adjoint import trajectory
+    from ffea_ci_demo_amd.blob import FFEAException
+    spec, pos0, corner, target = deformed_case(n=2, seed=7, epsilon=1e-12)
+    G = torch.tensor(spec.G, dtype=torch.float64, requires_grad=True)
+    E, A, B = (torch.tensor(v, dtype=torch.float64) for v in (spec.E, spec.A, spec.B))
+    with GpuBlob.from_spec(spec) as g:
+        x1 = trajectory(g, G, E, A, B, pos0, 4, 2)
+        x2 = trajectory(g, G * 1.1, E, A, B, pos0, 4, 2)
+        with pytest.raises(RuntimeError, match="another trajectory"):
+            x1.sum().backward()
+        x2.sum().backward()  # the latest run is still reversible
+        g.run_checkpointed(3, 2)
+        g.set_material(spec.A, spec.B, spec.G * 1.2, spec.E)
+        with pytest.raises(FFEAException, match="no checkpointed run"):
+            g.adjoint(np.ones((spec.num_nodes, 3)))
