@@ -27,6 +27,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -60,11 +61,25 @@ def iter_alg_bytes(info, w):
     return info["num_blocks"] * (9 * w + 4) + 216 * info["num_linear_nodes"]
 
 
+def step_alg_bytes(info, w, k):
+    """the whole-step model B_const + k B_iter: per element its record (32 + 58 w: corner
+    indices, rest state and material read, forces and shape data written and read back),
+    per corner node 120 B (positions, right-hand side, velocity), the block values written
+    (9 w per block), and k CG iterations; a multigrid V-cycle's own bytes are not in it"""
+    ne, nl, nb = info["num_elements"], info["num_linear_nodes"], info["num_blocks"]
+    return ne * (32 + 58 * w) + 120 * nl + 9 * w * nb + k * iter_alg_bytes(info, w)
+
+
 def split_form(form):
     """"fp64" / "mixed" (the reference's Jacobi-PCG), "fp64+mg" / "mixed+mg" (multigrid-
-    preconditioned CG): (precision, preconditioner)"""
-    prec, _, pc = form.partition("+")
-    return prec, ("multigrid" if pc == "mg" else "jacobi")
+    preconditioned CG); a "+ref" suffix adds FP64 refinement of the mixed form's solves:
+    (precision, preconditioner)"""
+    parts = form.split("+")
+    return parts[0], ("multigrid" if "mg" in parts[1:] else "jacobi")
+
+
+def refined(form):
+    return "ref" in form.split("+")[1:]
 
 
 def run_form(spec, form, steps, warmup):
@@ -74,8 +89,11 @@ def run_form(spec, form, steps, warmup):
     g = GpuBlob.from_spec(spec, precision=prec)
     t_create = time.perf_counter() - t0
     mg = None
+    t_mg = 0.0
     if pc == "multigrid":
+        t0 = time.perf_counter()
         mg = g.set_preconditioner("multigrid")
+        t_mg = time.perf_counter() - t0
         print(f"[bench] multigrid: levels {mg['rows']} (blocks {mg['block']}), coarsest {mg['coarsest_unknowns']} "
               f"unknowns, set-up {mg['setup_seconds']:.1f}s", file=sys.stderr, flush=True)
     info = g.info()
@@ -110,6 +128,11 @@ def run_form(spec, form, steps, warmup):
         "spmv_alg_bytes": spmv_alg_bytes(info, w, proj),
         "spmv_achieved_GBs": spmv_alg_bytes(info, w, proj) / avg_spmv_s / 1e9,
         "iteration_model_GBs": iter_alg_bytes(info, w) * iters / wall / 1e9,
+        "step_model_bytes": step_alg_bytes(info, w, iters / steps),
+        "step_model_GBs": step_alg_bytes(info, w, iters / steps) * steps / wall / 1e9,
+        "step_model_frac": step_alg_bytes(info, w, iters / steps) * steps / wall / 1e9 / PEAK_HBM_GBS,
+        "setup_s": t_create + t_mg,
+        "element_steps_per_s_1000_steps_with_setup": ne * 1000 / (1000 * wall / steps + t_create + t_mg),
         "info": info,
     }
     if mg is not None:
@@ -180,9 +203,12 @@ def run_ensemble(specs, form, steps, warmup, tune_iters=20):
     return res
 
 
-def cpu_baseline(spec, k_per_step, cg_iters):
-    """oracle (FP64, 1 thread) on one step of the same blob with the CG capped at
-    cg_iters iterations; a full step is element pass + gather/build + k iterations"""
+def cpu_baseline(spec, k_per_step, cg_iters, n=None):
+    """The oracle (the FP64 C restatement of the reference's algorithm) on this host's CPU:
+    whole steps of the same blob on every core the process may use (OpenMP build with the
+    reference's Release flags, oracle/cpu_baseline.py, in a child process), and the
+    single-thread figure of one step with the CG capped at cg_iters iterations (a full
+    step is element pass + gather/build + k iterations)"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     t0 = time.perf_counter()
@@ -193,14 +219,36 @@ def cpu_baseline(spec, k_per_step, cg_iters):
     t_step = te + tb + k_per_step * ti
     print(f"[bench] cpu oracle: create {t_create:.1f}s element {te:.2f}s build {tb:.2f}s cg iter {ti*1e3:.1f}ms",
           file=sys.stderr, flush=True)
+    single = {"value": spec.num_elements / t_step, "unit": "element-steps/s", "cores": 1,
+              "sample": (f"one step on 1 thread (-O2, strict IEEE): element pass {te:.2f}s + gather/matrix build "
+                         f"{tb:.2f}s + {cg_iters} CG iterations at {ti*1e3:.1f} ms each, scaled to the "
+                         f"{k_per_step:.0f} iterations/step the GPU FP64 run needed")}
+    omp = None
+    if n is not None:
+        try:
+            r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
+                                "--steps", "1"], capture_output=True, text=True, timeout=600)
+            if r.returncode == 0:
+                omp = json.loads(r.stdout.strip().splitlines()[-1])
+            else:
+                print(f"[bench] OpenMP CPU baseline failed: {r.stderr[-500:]}", file=sys.stderr, flush=True)
+        except (subprocess.TimeoutExpired, ValueError) as ex:
+            print(f"[bench] OpenMP CPU baseline failed: {ex}", file=sys.stderr, flush=True)
+    if omp is None:
+        return dict(single, kind="port")
+    print(f"[bench] cpu oracle OpenMP: {omp['threads']} threads, {omp['s_per_step']:.2f} s/step "
+          f"({omp['cg_iterations']} CG iterations)", file=sys.stderr, flush=True)
     return {
-        "value": spec.num_elements / t_step,
+        "value": omp["element_steps_per_s"],
         "unit": "element-steps/s",
-        "cores": 1,
+        "cores": omp["threads"],
         "kind": "port",
-        "sample": (f"one step of the same blob on the oracle (FP64 C restatement, 1 thread): element pass "
-                   f"{te:.2f}s + gather/matrix build {tb:.2f}s + {cg_iters} CG iterations at {ti*1e3:.1f} ms each, "
-                   f"scaled to the {k_per_step:.0f} iterations/step the GPU FP64 run needed"),
+        "sample": (f"one whole step of the same blob ({omp['cg_iterations'][0]} Jacobi-PCG iterations, measured, "
+                   f"{omp['s_per_step']:.1f} s) on the oracle's OpenMP build (-O3 -ffast-math -march=native "
+                   f"-fopenmp, the reference's Release flags), {omp['threads']} threads of a "
+                   f"{omp['host_cpus']}-CPU host"),
+        "cpu_model": omp["cpu_model"],
+        "single_thread": single,
     }
 
 
@@ -218,6 +266,8 @@ def main():
     ap.add_argument("--run-steps", type=int, default=None,
                     help="adjoint: trajectory length K (200); illcond/drift: steps compared (2000/1000)")
     ap.add_argument("--blobs", type=int, default=512, help="ensemble: number of blobs")
+    ap.add_argument("--fd-forms", default="fp64",
+                    help="adjoint: forms whose gradient is checked by central differences (CG epsilon 1e-13)")
     ap.add_argument("--cpu-cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -273,7 +323,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         k_ref = results["fp64"]["cg_iterations_per_step"] if "fp64" in results else h["cg_iterations_per_step"]
-        cpu = cpu_baseline(spec, k_ref, args.cpu_cg_iters)
+        cpu = cpu_baseline(spec, k_ref, args.cpu_cg_iters, n=args.n)
+    ratios = {}
+    for a, b in (("fp64", "mixed"), ("fp64+mg", "mixed+mg"), ("fp64", "mixed+mg"), ("fp64", "fp64+mg")):
+        if a in results and b in results:
+            ratios[f"{b}_vs_{a}"] = round(results[a]["ms_per_step"] / results[b]["ms_per_step"], 3)
     out = {
         "metric": METRIC,
         "value": round(h["element_steps_per_s"], 1),
@@ -293,8 +347,20 @@ def main():
                         "thermal noise at 300 K, dt 1e-14 s, CG epsilon 1e-10, no pins",
             "form": head,
             "cg_iterations_per_step": round(h["cg_iterations_per_step"], 1),
+            "precision": ("element algebra, shape data and viscosity matrix FP32; CG vectors, inner products, "
+                          "residual norms and positions FP64" if split_form(head)[0] == "mixed" else "FP64") +
+                         ("; multigrid preconditioner matrices FP16 with a power-of-two scale per matrix, FP64 "
+                          "accumulation" if split_form(head)[1] == "multigrid" else ""),
             "parallelism": "single GPU",
         },
+        "speedup_ratios": ratios,
+        "step_model": {"bytes_per_step": h["step_model_bytes"], "achieved_GBs": round(h["step_model_GBs"], 1),
+                       "frac": round(h["step_model_frac"], 4),
+                       "note": "B_const + k B_iter per step over the step time; a V-cycle's own bytes are not "
+                               "counted"},
+        "setup": {"setup_s": round(h["setup_s"], 2),
+                  "element_steps_per_s_1000_steps_with_setup": round(h["element_steps_per_s_1000_steps_with_setup"],
+                                                                     1)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items() if k != "info"}
@@ -416,6 +482,10 @@ def main_ensemble(args):
         mid = sorted(specs, key=lambda s: s.num_elements)[len(specs) // 2]
         cpu = cpu_baseline(mid, h["cg_iterations_per_step_median"], args.cpu_cg_iters)
         cpu["sample"] = f"median blob ({mid.name}, {mid.num_elements} tets): " + cpu["sample"]
+    ratios = {}
+    for a, b in (("fp64", "mixed"), ("fp64+mg", "mixed+mg"), ("fp64", "mixed+mg"), ("fp64", "fp64+mg")):
+        if a in results and b in results:
+            ratios[f"{b}_vs_{a}"] = round(results[a]["ms_per_step"] / results[b]["ms_per_step"], 3)
     out = {
         "metric": METRIC,
         "value": round(h["element_steps_per_s"], 1),
@@ -455,8 +525,10 @@ def main_adjoint(args):
     bottom face pinned, +1 pN z on the top-face corner nodes, noise off. Target field: the
     positions after K steps with a second moduli seed. Loss: sum over corner nodes of
     |pos_K - target|^2. Gradient in G, E, A, B per element by the adjoint with checkpoints
-    every 20 steps, FP64 and mixed; a directional central difference of the FP64 forward
-    checks the FP64 gradient."""
+    every 20 steps in every form of --forms (+ref: FP64 refinement of the mixed form's
+    solves to 1e-10), each against the FP64 form's. Directional central differences of the
+    loss check the FP64 gradient, one random direction in each of G, E, A and B, with the
+    CG tolerance at 1e-13 for both the differences and the adjoint they check."""
     from ffea_ci_demo_amd import workloads
     from ffea_ci_demo_amd.blob import GpuBlob
     n = args.n if args.n != 100 else 50
@@ -466,6 +538,7 @@ def main_adjoint(args):
     tgt_spec = workloads.config3(n=n, moduli_seed=12)
     corner = spec.mesh.is_linear
     with GpuBlob.from_spec(tgt_spec) as g:
+        g.set_preconditioner("multigrid")
         g.step(K)
         target = g.positions()
 
@@ -473,15 +546,21 @@ def main_adjoint(args):
         d = (pos - target) * corner[:, None]
         return float((d ** 2).sum()), 2.0 * d
 
+    def make(form, s=spec):
+        prec, pc = split_form(form)
+        g = GpuBlob.from_spec(s, precision=prec)
+        if pc == "multigrid":
+            g.set_preconditioner("multigrid")
+        if refined(form):
+            g.set_refinement(1e-10)
+        return g
+
     forms, grads = {}, {}
     form_list = [f for f in args.forms.split(",") if f]
-    if "fp64" not in form_list:
-        form_list.insert(0, "fp64")
+    if "fp64" not in form_list and "fp64+mg" not in form_list:
+        form_list.insert(0, "fp64+mg")
     for form in form_list:
-        prec, pc = split_form(form)
-        with GpuBlob.from_spec(spec, precision=prec) as g:
-            if pc == "multigrid":
-                g.set_preconditioner("multigrid")
+        with make(form) as g:
             g.step(2)  # warm-up (kernels loaded, iteration count learned)
             g.set_positions(spec.pos)
             g.synchronize()
@@ -497,29 +576,47 @@ def main_adjoint(args):
             forms[form] = {"loss": L, "forward_ms_per_step": t_fwd / K * 1e3, "reverse_ms_per_step": t_bwd / K * 1e3,
                            "gradient_element_steps_per_s": spec.num_elements * K / (t_fwd + t_bwd),
                            "cg_iterations_last_forward_step": it_last}
+            if refined(form):
+                forms[form]["refinement_correction_solves"] = g.refinement_sweeps()[1]
             print(f"[bench] adjoint {form}: forward {t_fwd:.2f}s reverse {t_bwd:.2f}s loss {L:.6e}",
                   file=sys.stderr, flush=True)
-            if prec == "fp64":
-                r = np.random.default_rng(7)
+    ref_form = "fp64" if "fp64" in grads else "fp64+mg"
+    for f in form_list:
+        if f == ref_form:
+            continue
+        forms[f]["loss_rel_err_vs_fp64"] = abs(forms[f]["loss"] - forms[ref_form]["loss"]) / abs(forms[ref_form]["loss"])
+        for name in "ABGE":
+            a, b = grads[f][name], grads[ref_form][name]
+            forms[f][f"grad_{name}_rel_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    # finite-difference checks of the FP64 forms' adjoints at CG tolerance 1e-13
+    fd = {}
+    for form in [f for f in args.fd_forms.split(",") if f]:
+        s13 = workloads.config3(n=n, epsilon=1e-13)
+        with make(form, s13) as g:
+            g.run_checkpointed(K, interval)
+            _, seed = loss(g.positions())
+            ga = dict(zip("ABGE", g.adjoint(seed)[:4]))
+            r = np.random.default_rng(7)
+            h = 1e-4
+            errs = {}
+            for name in "GEAB":
                 d = r.uniform(-1.0, 1.0, spec.num_elements)
-                h = 1e-4
+                base = {"A": s13.A, "B": s13.B, "G": s13.G, "E": s13.E}
                 vals = []
                 for sgn in (1.0, -1.0):
-                    g.set_material(spec.A, spec.B, spec.G * (1 + sgn * h * d), spec.E)
-                    g.set_positions(spec.pos)
+                    m = dict(base)
+                    m[name] = base[name] * (1 + sgn * h * d)
+                    g.set_material(m["A"], m["B"], m["G"], m["E"])
+                    g.set_positions(s13.pos)
                     g.step(K)
                     vals.append(loss(g.positions())[0])
-                fd = (vals[0] - vals[1]) / (2 * h)
-                ad = float((dG * spec.G * d).sum())
-                forms[form]["fd_check_G_direction_rel_err"] = abs(fd - ad) / abs(fd)
-    for f in form_list:
-        if f == "fp64":
-            continue
-        forms[f]["loss_rel_err_vs_fp64"] = abs(forms[f]["loss"] - forms["fp64"]["loss"]) / abs(forms["fp64"]["loss"])
-        for name in "ABGE":
-            a, b = grads[f][name], grads["fp64"][name]
-            forms[f][f"grad_{name}_rel_err_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    head = args.headline or ("fp64+mg" if "fp64+mg" in forms else "fp64")
+                fdv = (vals[0] - vals[1]) / (2 * h)
+                ad = float((ga[name] * base[name] * d).sum())
+                errs[name] = abs(fdv - ad) / abs(fdv)
+                print(f"[bench] fd check {form} d{name}: fd {fdv:.9e} adjoint {ad:.9e} rel err {errs[name]:.2e}",
+                      file=sys.stderr, flush=True)
+            fd[form] = {"epsilon": 1e-13, "h": h, "rel_err": errs, "max_rel_err": max(errs.values())}
+    head = args.headline or ("fp64+mg" if "fp64+mg" in forms else form_list[0])
     h = forms[head]
     out = _line(round(h["gradient_element_steps_per_s"], 1), "element-steps/s", round(h["forward_ms_per_step"] +
                 h["reverse_ms_per_step"], 3), K, 2, "f64" if split_form(head)[0] == "fp64" else "f32+f64",
@@ -530,6 +627,7 @@ def main_adjoint(args):
                              f"element-steps/s of the {head} gradient", "form": head, "parallelism": "single GPU"},
                 {f: {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()}
                  for f, r in forms.items()})
+    out["fd_check"] = fd
     print(json.dumps(out), flush=True)
 
 

@@ -36,6 +36,7 @@ EXPORTED = [
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
     "ffea_gpu_blob_kinetic_energy", "ffea_gpu_solver_create", "ffea_gpu_solver_solve", "ffea_gpu_solver_apply_matrix",
+    "ffea_gpu_blob_set_refinement", "ffea_gpu_blob_get_refinement",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -106,6 +107,8 @@ def lib():
                                          C.POINTER(vp)]
     L.ffea_gpu_solver_solve.argtypes = [vp, _dp, _dp]
     L.ffea_gpu_solver_apply_matrix.argtypes = [vp, _dp, _dp, _dp]
+    L.ffea_gpu_blob_set_refinement.argtypes = [vp, C.c_double]
+    L.ffea_gpu_blob_get_refinement.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     L.ffea_gpu_blob_add_node_forces.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_get_node_positions.argtypes = [vp, C.c_int, _ip, _dp]
     L.ffea_gpu_blob_set_solver.argtypes = [vp, C.c_int, vp]

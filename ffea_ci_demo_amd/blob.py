@@ -89,6 +89,16 @@ class _Handle:
         iterations (0: a little over the last step's count)"""
         self._check(_native.lib().ffea_gpu_blob_use_graph(self.h, 1 if enable else 0, int(iterations)))
 
+    def set_refinement(self, tolerance):
+        """FP64 refinement of the mixed form's solves to |b - V v| <= tolerance |b| (0: off)"""
+        self._check(_native.lib().ffea_gpu_blob_set_refinement(self.h, float(tolerance)))
+
+    def refinement_sweeps(self):
+        """(correction solves of the last solve, in all)"""
+        last, total = C.c_int(), C.c_longlong()
+        self._check(_native.lib().ffea_gpu_blob_get_refinement(self.h, C.byref(last), C.byref(total)))
+        return last.value, total.value
+
     def set_preconditioner(self, kind="jacobi"):
         """CG preconditioner: "jacobi" (the reference's) or "multigrid" (a smoothed-
         aggregation V-cycle built from the matrix at the current positions); returns the

@@ -1151,6 +1151,15 @@ struct ffea_gpu_blob {
     std::vector<double> mid_diag;      // [num_nodes] diagonal of each midside node's identity row
                                        // (1, plus the drag of an unpinned node under calc_stokes)
     std::vector<double> h_vec;         // tiled host staging of one node vector
+    // FP64 refinement of the solves (ffea_gpu_blob_set_refinement): the true residual by an
+    // FP64 matrix-free product, FP32 correction solves, until |b - V x| <= refine_tol |b|
+    double refine_tol = 0.0;           // 0: off
+    double eps2_override = 0.0;        // > 0: the CG's stopping threshold for a correction solve
+    int refine_sweeps_last = 0;
+    long long refine_sweeps_total = 0;
+    DevBuf<double> w64, xacc, bsave, res64, Pr, A64, B64;
+    std::vector<double> h_pr;
+    double cg_eps2() const { return eps2_override > 0.0 ? eps2_override : prm.epsilon2; }
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
     bool fused_cg = false;
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
@@ -1313,6 +1322,8 @@ struct ffea_gpu_blob {
     int finish_step();
     int step_once();
     void capture_graph(int iters);
+    void set_refinement(double tol);
+    void refine_solution();
     void solver_matrix_at(const double *pos_nodes);
     int solver_solve(const double *pos_nodes, double *x);
     void solver_apply(const double *pos_nodes, const double *in, double *out);
@@ -1337,6 +1348,12 @@ void ffea_gpu_blob::upload_elements(const Tables &t) {
     } else {
         bj = &f_jinv0; bv = &f_vol0; bG = &f_G; bE = &f_E; bA = &f_A; bB = &f_B; bf = &f_force; bg = &f_geo;
         bval = &f_val;
+    }
+    if constexpr (!std::is_same<T, double>::value) {  // FP64 viscosities for the refinement's product
+        A64.alloc(ne, &dev_bytes);
+        A64.upload(t.A, stream);
+        B64.alloc(ne, &dev_bytes);
+        B64.upload(t.B, stream);
     }
     upload_converted(*bj, t.jinv0, stream, &dev_bytes);
     upload_converted(*bv, t.vol0, stream, &dev_bytes);
@@ -1658,7 +1675,7 @@ template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) {
     hipEvent_t ts = nullptr;
     if (t_cg(3, it)) t_begin(3, &ts);
-    SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
+    SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
     hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, sa);
     t_end(3, ts, it);
@@ -1694,7 +1711,7 @@ void ffea_gpu_blob::launch_cg_iteration(int it, bool spmv_only) {
 
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_cg_fused_t(const T *val) {
-    SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, prm.epsilon2, zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
+    SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
     VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
     hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(row_block<T>), 0, stream, sa, va, (const double *)Pi.p);
@@ -2206,7 +2223,7 @@ int ffea_gpu_blob::step_once() {
         return step_mass();
     }
     prepare_ext();
-    if (use_graph && expect_iters > 0) {
+    if (use_graph && expect_iters > 0 && refine_tol == 0.0) {
         // enough iterations to cover the last step's count with a margin, in steps of 16
         int want = std::min(prm.max_iterations_cg, ((expect_iters + expect_iters / 8 + 8 + 15) / 16) * 16);
         if (graph_fixed_iters > 0) want = std::min(prm.max_iterations_cg, graph_fixed_iters);
@@ -2226,8 +2243,105 @@ int ffea_gpu_blob::step_once() {
     launch_element();
     launch_assemble_gather();
     run_cg_eager(0);
+    if (refine_tol > 0.0) refine_solution();
     launch_update();
     return finish_step();
+}
+
+// ---------------------------------------------------------------- FP64 refinement
+// tol > 0 turns refinement on (FP32-mixed single-blob handles whose viscosity matrix is
+// nonsingular: some nodes pinned or Stokes drag)
+void ffea_gpu_blob::set_refinement(double tol) {
+    if (!(tol >= 0.0) || !std::isfinite(tol)) throw Invalid("ffea_gpu_blob_set_refinement: tolerance must be >= 0");
+    if (tol == 0.0) {
+        refine_tol = 0.0;
+        return;
+    }
+    if (!mixed) throw Invalid("ffea_gpu_blob_set_refinement: the FP64 form needs no refinement");
+    if (nblob != 1) throw Invalid("ffea_gpu_blob_set_refinement: single-blob handles only");
+    if (mass) throw Invalid("ffea_gpu_blob_set_refinement: the mass-less solver only");
+    if (meta[0].free_blob || any_project)
+        throw Invalid("ffea_gpu_blob_set_refinement: a free blob's viscosity matrix is singular; refinement needs "
+                      "pinned nodes or Stokes drag");
+    if (have_sf || have_dn)
+        throw Invalid("ffea_gpu_blob_set_refinement: not with surface, rotational or external forces");
+    const size_t vlen = (size_t)nslices * 3 * kSlice;
+    if (!w64.p) w64.alloc((size_t)12 * ne, &dev_bytes);
+    for (DevBuf<double> *bf : {&xacc, &bsave, &res64})
+        if (!bf->p) bf->alloc(vlen, &dev_bytes);
+    if (!Pr.p) Pr.alloc((size_t)2 * std::max(1, nchunks), &dev_bytes);
+    drop_graph();  // refined steps run eagerly
+    refine_tol = tol;
+}
+
+// After a converged solve of V x = b (x in x, b in rhs): sweeps of r = b - V64 x in FP64,
+// an FP32 correction solve V d = r (the CG's own kernels, stopping at a fraction of r that
+// still meets the tolerance overall), x += d, until |r| <= refine_tol |b|. Leaves x and rhs
+// as a plain solve would, the step's CG iteration count including the correction solves.
+void ffea_gpu_blob::refine_solution() {
+    refine_sweeps_last = 0;
+    FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    if (!h_S[0].converged || h_S[0].inverted_step > 0) return;  // reported by the caller
+    const CgScalars main = h_S[0];
+    const size_t vlen = (size_t)nslices * 3 * kSlice;
+    FFEA_HIP_TRY(hipMemcpyAsync(xacc.p, x.p, sizeof(double) * vlen, hipMemcpyDeviceToDevice, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(bsave.p, rhs.p, sizeof(double) * vlen, hipMemcpyDeviceToDevice, stream));
+    const int nc = std::max(1, nchunks);
+    h_pr.resize((size_t)2 * nc);
+    int extra_iters = 0;
+    constexpr int kMaxSweeps = 30;
+    constexpr double kInner2 = 1e-6;  // a correction solve reduces its residual by 1e3 at least
+    for (int sweep = 0;; ++sweep) {
+        ViscApplyArgs va{ne, conn.p, pos.p, A64.p, B64.p, xacc.p, inc_pos.p, w64.p};
+        hipLaunchKernelGGL(k_visc_apply64, dim3(grid_elems), dim3(kBlock), 0, stream, va);
+        ResidArgs ra{row_args(), inc_ptr.p, w64.p, stokes ? drag.p : nullptr, pinned.p, bsave.p, xacc.p, res64.p, Pr.p};
+        hipLaunchKernelGGL(k_resid64, dim3(nb), dim3(kRowBlock), 0, stream, ra);
+        FFEA_HIP_TRY(hipGetLastError());
+        FFEA_HIP_TRY(hipMemcpyAsync(h_pr.data(), Pr.p, sizeof(double) * h_pr.size(), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+        double r2 = 0.0, b2 = 0.0;
+        for (int c = 0; c < nchunks; ++c) {
+            r2 += h_pr[c];
+            b2 += h_pr[(size_t)nchunks + c];
+        }
+        if (!std::isfinite(r2)) throw Invalid("FP64 refinement: the residual is not finite");
+        if (r2 <= refine_tol * refine_tol * b2) break;
+        if (sweep == kMaxSweeps)
+            throw Invalid(fmt("FP64 refinement: |b - V x| / |b| = %.3e after %d correction solves (tolerance %.3e)",
+                              std::sqrt(r2 / b2), kMaxSweeps, refine_tol));
+        eps2_override = std::max(0.25 * refine_tol * refine_tol * b2 / r2, kInner2);
+        rhs_in = res64.p;
+        try {
+            launch_gather<float, false>(f_force.p);  // the correction's start on r
+            FFEA_HIP_TRY(hipGetLastError());
+            if (mg_on()) launch_mg_vcycle(Pi.p);
+            run_cg_eager(0);
+        } catch (...) {
+            rhs_in = nullptr;
+            eps2_override = 0.0;
+            throw;
+        }
+        rhs_in = nullptr;
+        eps2_override = 0.0;
+        FFEA_HIP_TRY(hipMemcpyAsync(h_S, S.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+        if (!h_S[0].converged)
+            throw Invalid(fmt("FP64 refinement: a correction solve could not converge after %d iterations",
+                              prm.max_iterations_cg));
+        extra_iters += h_S[0].iter;
+        const long long nv = (long long)vlen;
+        hipLaunchKernelGGL(k_add_vec, dim3((unsigned)((nv + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, nv,
+                           (const double *)x.p, xacc.p);
+        ++refine_sweeps_last;
+    }
+    refine_sweeps_total += refine_sweeps_last;
+    FFEA_HIP_TRY(hipMemcpyAsync(x.p, xacc.p, sizeof(double) * vlen, hipMemcpyDeviceToDevice, stream));
+    FFEA_HIP_TRY(hipMemcpyAsync(rhs.p, bsave.p, sizeof(double) * vlen, hipMemcpyDeviceToDevice, stream));
+    h_S[0] = main;
+    h_S[0].iter = main.iter + extra_iters;
+    FFEA_HIP_TRY(hipMemcpyAsync(S.p, h_S, sizeof(CgScalars), hipMemcpyHostToDevice, stream));
+    FFEA_HIP_TRY(hipStreamSynchronize(stream));
 }
 
 // ---------------------------------------------------------------- solver-only handles
@@ -2284,6 +2398,7 @@ int ffea_gpu_blob::solver_solve(const double *pos_nodes, double *xio) {
     }
     rhs_in = nullptr;
     run_cg_eager(0);
+    if (refine_tol > 0.0) refine_solution();
     if (mg_on() && m.free_blob) launch_mg_gauge();  // Jacobi-PCG's rigid-mode gauge
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
     FFEA_HIP_TRY(hipGetLastError());
@@ -2338,6 +2453,11 @@ void ffea_gpu_blob::set_material_t(const double *A, const double *B, const doubl
     for (const auto &it : items) {
         for (int e = 0; e < ne0; ++e) t[e] = (T)it.second[e];
         it.first->upload(t, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    if constexpr (!std::is_same<T, double>::value) {
+        A64.upload(A, ne0, stream);
+        B64.upload(B, ne0, stream);
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
     T *geo = std::is_same<T, double>::value ? (T *)d_geo.p : (T *)f_geo.p;
@@ -2412,6 +2532,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     if (!h_S[0].converged)
         throw Invalid(fmt("adjoint solve: conjugate gradient could not converge after %d iterations",
                           prm.max_iterations_cg));
+    if (refine_tol > 0.0) refine_solution();
     AdjArgs<T> a{ne, conn.p, pos.p, rest.p, x.p, vel_k, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, prm.dt, w,
                  grad.p};
     if constexpr (std::is_same<T, double>::value) {
@@ -2810,6 +2931,7 @@ int ffea_gpu_blob_update_positions(ffea_gpu_blob *b) {
         b->prepare_ext();
         b->launch_assemble_gather();
         b->run_cg_eager(0);
+        if (b->refine_tol > 0.0) b->refine_solution();
         b->launch_update();
         return b->finish_step();
     });
@@ -3304,6 +3426,22 @@ int ffea_gpu_blob_adjoint(ffea_gpu_blob *b, const double *dL_dpos, double *dL_dA
         FFEA_HIP_TRY(hipSetDevice(b->device));
         return b->adjoint(dL_dpos, dL_dA, dL_dB, dL_dG, dL_dE, dL_dpos0);
     });
+}
+
+int ffea_gpu_blob_set_refinement(ffea_gpu_blob *b, double tolerance) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        b->set_refinement(tolerance);
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+int ffea_gpu_blob_get_refinement(ffea_gpu_blob *b, int *last_sweeps, long long *total_sweeps) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    if (last_sweeps) *last_sweeps = b->refine_sweeps_last;
+    if (total_sweeps) *total_sweeps = b->refine_sweeps_total;
+    return FFEA_GPU_OK;
 }
 
 int ffea_gpu_blob_use_graph(ffea_gpu_blob *b, int enable, int iterations) {

@@ -1489,6 +1489,133 @@ __global__ void k_mark_updated(int nblob, CgScalars *S) {
     S[b].inverted = 0;
 }
 
+// ---------------------------------------------------------------- FP64 refinement of a solve
+// The FP32-mixed form solves with an FP32 viscosity matrix, whose rounding (~1e-7 of each
+// entry) the solve amplifies in the smooth, low-eigenvalue part of a solution. Refinement
+// forms the true residual b - V x with an FP64 matrix-free product (shape derivatives and
+// volumes recomputed in FP64 from the positions, FP64 viscosities) and solves the FP32
+// system again for the correction, until that residual meets its tolerance.
+
+// Each element's product with v (tiled node vector): (V_e v)_a = vol sigma(v) grad psi_a with
+// sigma = A (L + L^T) + B tr(L) I and L = sum_b v_b (x) grad psi_b, the action of the 12x12
+// matrix of tetra_element_linear.cpp:298-327; written where the row gather reads it.
+struct ViscApplyArgs {
+    int ne;
+    const int *conn;           // [4][ne]
+    const double *pos;         // [rows][3]
+    const double *A, *B;       // [ne] FP64
+    const double *v;           // tiled [rows][3]
+    const uint32_t *inc_pos;   // [4][ne]
+    double *w;                 // [4 ne][3]
+};
+__global__ __launch_bounds__(kBlock) void k_visc_apply64(ViscApplyArgs a) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= a.ne) return;
+    const int ne = a.ne;
+    int n[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) n[k] = a.conn[k * ne + e];
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[r][c] = a.pos[3 * n[r + 1] + c] - a.pos[3 * n[0] + c];
+    double d[12];
+    d[1] = J[2][2] * J[1][1] - J[2][1] * J[1][2];
+    d[2] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+    d[3] = J[1][2] * J[0][1] - J[1][1] * J[0][2];
+    d[5] = J[2][0] * J[1][2] - J[2][2] * J[1][0];
+    d[6] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+    d[7] = J[1][0] * J[0][2] - J[1][2] * J[0][0];
+    d[9] = J[2][1] * J[1][0] - J[2][0] * J[1][1];
+    d[10] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+    d[11] = J[1][1] * J[0][0] - J[1][0] * J[0][1];
+    const double det = J[0][0] * d[1] + J[1][0] * d[2] + J[2][0] * d[3];
+    const double rdet = 1.0 / det, vol = fabs(det) / 6.0;
+#pragma unroll
+    for (int k = 1; k < 12; ++k)
+        if (k & 3) d[k] *= rdet;
+    d[0] = -(d[1] + d[2] + d[3]);
+    d[4] = -(d[5] + d[6] + d[7]);
+    d[8] = -(d[9] + d[10] + d[11]);
+    double L[3][3];  // L[i][j] = sum_b v_b[i] d_j psi_b, d_j psi_b = d[4 j + b]
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L[i][j] = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double vb[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vb[i] = a.v[vi(n[b], i)];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) L[i][j] += vb[i] * d[4 * j + b];
+    }
+    const double A = a.A[e], B = a.B[e], tr = L[0][0] + L[1][1] + L[2][2];
+    double sg[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sg[i][j] = A * (L[i][j] + L[j][i]) + (i == j ? B * tr : 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double *wo = a.w + 3 * (size_t)a.inc_pos[(size_t)q * ne + e];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            wo[i] = vol * (sg[i][0] * d[q] + sg[i][1] * d[4 + q] + sg[i][2] * d[8 + q]);
+    }
+}
+
+// res = b - V x per corner row (V x from the element products in element order, the drag
+// on unpinned rows; pinned rows are identity rows), with chunk partials |res|^2 and |b|^2
+struct ResidArgs {
+    RowArgs ra;
+    const uint32_t *inc_ptr;
+    const double *w;             // [4 ne][3] from k_visc_apply64
+    const double *drag;          // [rows] or null
+    const unsigned char *pinned;
+    const double *b, *x;         // tiled
+    double *res;                 // tiled
+    double *P;                   // [2][nchunks]
+};
+__global__ __launch_bounds__(kRowBlock) void k_resid64(ResidArgs a) {
+    const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
+    for (int c = cr.c0; c < cr.c1; ++c) {
+        const Chunk ch = a.ra.chunks[c];
+        double v[2] = {0.0, 0.0};
+        for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
+            double kx[3] = {0.0, 0.0, 0.0};
+            if (a.pinned[n]) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) kx[i] = a.x[vi(n, i)];
+            } else {
+                for (uint32_t k = a.inc_ptr[n]; k < a.inc_ptr[n + 1]; ++k)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) kx[i] += a.w[3 * (size_t)k + i];
+                if (a.drag)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) kx[i] += a.drag[n] * a.x[vi(n, i)];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double bi = a.b[vi(n, i)], ri = bi - kx[i];
+                a.res[vi(n, i)] = ri;
+                v[0] += ri * ri;
+                v[1] += bi * bi;
+            }
+        }
+        store_partials<2, kRowBlock>(v, a.P, a.ra.nchunks, c);
+    }
+}
+
+// y += x over a whole tiled vector
+__global__ __launch_bounds__(kBlock) void k_add_vec(long long n, const double *x, double *y) {
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
 // ---------------------------------------------------------------- adjoint (reverse of a step)
 // One mass-less step is x' = x + dt v with K(x) v = P f(x) (P zeroes pinned rows). For
 // lambda' = dL/dx' the reverse is: solve K mu = P lambda'; then

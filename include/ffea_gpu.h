@@ -359,6 +359,22 @@ int ffea_gpu_blob_run_checkpointed(ffea_gpu_blob *blob, int nsteps, int interval
 int ffea_gpu_blob_adjoint(ffea_gpu_blob *blob, const double *dL_dpos, double *dL_dA, double *dL_dB,
                           double *dL_dG, double *dL_dE, double *dL_dpos0);
 
+/* FP64 refinement of the FP32-mixed form's solves (no counterpart in the reference, whose
+ * solve is FP64). The FP32 viscosity matrix's rounding is amplified by the solve in the
+ * smooth, slowly relaxing part of a solution (a static load on a pinned blob; the adjoint's
+ * solves). With tolerance > 0, after each converged mass-less solve V v = b (the step's,
+ * ffea_gpu_solver_solve's and the adjoint's) the true residual b - V v is formed in FP64
+ * (matrix-free: shape derivatives and volumes recomputed in FP64 from the positions, FP64
+ * viscosities), an FP32 correction solve with the CG's own kernels and preconditioner is
+ * added, and this repeats until |b - V v| <= tolerance |b| (at most 30 correction solves;
+ * FFEA_GPU_ERR_INVALID otherwise). The correction solves' iterations count in the step's
+ * CG iterations; refined steps run eagerly (no graph). 0 turns it off (the default).
+ * FP32-mixed single-blob mass-less handles with a nonsingular matrix (pinned nodes or
+ * Stokes drag), without surface, rotational or external forces.
+ * ffea_gpu_blob_get_refinement: correction solves of the last solve and in all. */
+int ffea_gpu_blob_set_refinement(ffea_gpu_blob *blob, double tolerance);
+int ffea_gpu_blob_get_refinement(ffea_gpu_blob *blob, int *last_sweeps, long long *total_sweeps);
+
 /* Launch shape of the row kernels. Totals are summed per reduction chunk in a fixed
  * order, so the shape never changes results. ffea_gpu_blob_set_row_blocks sets the
  * number of workgroups (rounded to whole chunks per workgroup and a multiple of 8, one

@@ -36,6 +36,17 @@
  */
 #include "ffea_oracle.h"
 
+/* OMP(...) parallelises the step's loops in the timing build only (libffea_oracle_omp.so,
+ * -fopenmp with the reference's Release flags, as the reference's OpenMP loops,
+ * Blob.cpp:419-481, NoMassCGSolver.cpp:199-281): that build times the CPU baseline and
+ * nothing else. The default build has no OpenMP and runs every loop in order. */
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP(x) _Pragma(#x)
+#else
+#define OMP(x)
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -760,6 +771,17 @@ int orc_blob_update_internal_forces(orc_blob *b) {
     if (b->nrf) rotational_ctforces(b);
     if (b->nsf_sets) surface_ctforces(b);
     int inversions = 0;
+#ifdef _OPENMP
+    /* each thread draws from its own stream, the reference's rng[omp_get_thread_num()]
+     * (Blob.cpp:424-428): substreams of the blob's stream here */
+    const int nthr = omp_get_max_threads();
+    orc_rng *trng = (orc_rng *)malloc(sizeof(orc_rng) * (size_t)nthr);
+    for (int t = 0; t < nthr; ++t) {
+        trng[t] = b->rng;
+        for (int k = 0; k < t; ++k) orc_rng_reset_next_substream(&trng[t]);
+    }
+#endif
+    OMP(omp parallel for schedule(guided) reduction(+ : inversions))
     for (int e = 0; e < b->ne; ++e) {
         double J[3][3], s[3][3] = {{0}}, F[3][3], Ji0[3][3];
         double *d = b->dpsi + 12 * e;
@@ -771,7 +793,11 @@ int orc_blob_update_internal_forces(orc_blob *b) {
         elem_bulk_stress(b->G[e], b->E[e], b->vol0[e], b->vol[e], s);
         if (b->p.calc_noise == 1) {
             double u[7];
+#ifdef _OPENMP
+            for (int k = 0; k < 7; ++k) u[k] = orc_rng_rand_u01(&trng[omp_get_thread_num()]);
+#else
             for (int k = 0; k < 7; ++k) u[k] = orc_rng_rand_u01(&b->rng);
+#endif
             elem_noise_stress(b->vol[e], b->A[e], b->B[e], b->p.kT, b->p.dt, u, s);
         }
         double du[12];
@@ -797,12 +823,17 @@ int orc_blob_update_internal_forces(orc_blob *b) {
         for (int a = 0; a < 4; ++a)
             for (int c = 0; c < 3; ++c) out[3 * a + c] -= du[4 * c + a];
     }
+#ifdef _OPENMP
+    b->rng = trng[0];
+    free(trng);
+#endif
     return inversions;
 }
 
 /* SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
  * 305-312 ; SparseMatrixTypes.cpp:40-46) */
 static void matrix_build(orc_blob *b) {
+    OMP(omp parallel for schedule(static))
     for (int k = 0; k < b->nnz; ++k) {
         double sum = 0.0;
         for (int64_t t = b->src_ptr[k]; t < b->src_ptr[k + 1]; ++t) {
@@ -813,6 +844,7 @@ static void matrix_build(orc_blob *b) {
         }
         b->val[k] = sum;
     }
+    OMP(omp parallel for schedule(static))
     for (int r = 0; r < b->nrows; ++r) b->inv_diag[r] = 1.0 / b->val[b->diag[r]];
 }
 
@@ -828,6 +860,7 @@ void orc_blob_build_matrix(orc_blob *b) {
 
 /* SparseMatrixFixedPattern::apply(arr3) (:148-187) */
 void orc_blob_apply_matrix(const orc_blob *b, const double *in, double *out) {
+    OMP(omp parallel for schedule(static))
     for (int r = 0; r < b->nrows; ++r) {
         double acc = 0.0;
         for (int k = b->rowptr[r]; k < b->rowptr[r + 1]; ++k) acc += b->val[k] * in[b->col[k]];
@@ -840,6 +873,7 @@ static int cg_solve(orc_blob *b, double *x) {
     const int nn = b->nt;
     double *r = b->r, *p = b->pv, *z = b->z, *q = b->q, *f = b->f, *D = b->inv_diag;
     double delta_new = 0;
+    OMP(omp parallel for schedule(static) reduction(+ : delta_new))
     for (int i = 0; i < nn; ++i) {
         for (int c = 0; c < 3; ++c) {
             r[3 * i + c] = x[3 * i + c];
@@ -853,12 +887,17 @@ static int cg_solve(orc_blob *b, double *x) {
     for (int it = 0; it < b->p.max_iterations_cg; ++it) {
         orc_blob_apply_matrix(b, p, q);
         double pTq = 0;
+        OMP(omp parallel for schedule(static) reduction(+ : pTq))
         for (int i = 0; i < nn; ++i)
             pTq += p[3 * i] * q[3 * i] + p[3 * i + 1] * q[3 * i + 1] + p[3 * i + 2] * q[3 * i + 2];
         const double alpha = delta_new / pTq;
-        for (int k = 0; k < 3 * nn; ++k) x[k] += alpha * p[k];
-        for (int k = 0; k < 3 * nn; ++k) r[k] += -alpha * q[k];
+        OMP(omp parallel for schedule(static))
+        for (int k = 0; k < 3 * nn; ++k) {
+            x[k] += alpha * p[k];
+            r[k] += -alpha * q[k];
+        }
         double r2 = 0, f2 = 0;
+        OMP(omp parallel for schedule(static) reduction(+ : r2, f2))
         for (int i = 0; i < nn; ++i) {
             r2 += r[3 * i] * r[3 * i] + r[3 * i + 1] * r[3 * i + 1] + r[3 * i + 2] * r[3 * i + 2];
             f2 += f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2];
@@ -870,11 +909,13 @@ static int cg_solve(orc_blob *b, double *x) {
         }
         const double delta_old = delta_new;
         delta_new = 0;
+        OMP(omp parallel for schedule(static) reduction(+ : delta_new))
         for (int i = 0; i < nn; ++i) {
             for (int c = 0; c < 3; ++c) z[3 * i + c] = D[3 * i + c] * r[3 * i + c];
             delta_new += r[3 * i] * z[3 * i] + r[3 * i + 1] * z[3 * i + 1] + r[3 * i + 2] * z[3 * i + 2];
         }
         const double beta = delta_new / delta_old;
+        OMP(omp parallel for schedule(static))
         for (int k = 0; k < 3 * nn; ++k) p[k] = beta * p[k] + z[k];
     }
     b->last_iters = b->p.max_iterations_cg;
@@ -1037,6 +1078,7 @@ int orc_blob_set_rotational_forces(orc_blob *b, int num, const int *node, const 
 int orc_blob_update_positions(orc_blob *b) {
     const int nt = b->nt;
     double *F = b->force;
+    OMP(omp parallel for schedule(static))
     for (int n = 0; n < nt; ++n)
         for (int m = b->contrib_ptr[n]; m < b->contrib_ptr[n + 1]; ++m) {
             const double *src = b->node_force + 3 * b->contrib[m];
@@ -1091,11 +1133,13 @@ int orc_blob_update_positions(orc_blob *b) {
     matrix_build(b);
     int it = cg_solve(b, F);
     if (it < 0) return 1;
+    OMP(omp parallel for schedule(static))
     for (int i = 0; i < nt; ++i)
         for (int c = 0; c < 3; ++c) {
             b->vel[3 * i + c] = F[3 * i + c];
             b->pos[3 * i + c] += F[3 * i + c] * b->p.dt;
         }
+    OMP(omp parallel for schedule(static))
     for (int e = 0; e < b->ne; ++e) linearise_element(b, e);
     return 0;
 }
@@ -1209,6 +1253,7 @@ static int mass_cg_solve(orc_blob *b, double *x) {
     double *r = b->r, *d = b->pv, *s = b->z, *q = b->q, *f = b->f;
     const double *P = b->m_pre;
     double delta_new = 0;
+    OMP(omp parallel for schedule(static) reduction(+ : delta_new))
     for (int i = 0; i < nn; ++i) {
         for (int c = 0; c < 3; ++c) {
             r[3 * i + c] = x[3 * i + c];

@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libffea_oracle.so")
+# FFEA_ORACLE_LIB names another build of the same source (the OpenMP timing build that
+# oracle/cpu_baseline.py loads); the default is the single-threaded checker
+LIB_PATH = os.environ.get("FFEA_ORACLE_LIB") or os.path.join(HERE, "libffea_oracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libffea_ref.so")
 REF_TEST_RNG = os.path.join(HERE, "_ref", "testRngStream")
 

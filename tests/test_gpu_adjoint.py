@@ -154,3 +154,77 @@ def test_stale_run_refused():
         g.set_material(spec.A, spec.B, spec.G * 1.2, spec.E)
         with pytest.raises(FFEAException, match="no checkpointed run"):
             g.adjoint(np.ones((spec.num_nodes, 3)))
+
+
+def _static_case(n, K, interval, seed=4):
+    """config-3-type: pinned bottom, +1 pN on the top face, log-normal moduli; loss against
+    the positions of a second moduli seed"""
+    spec = workloads.config3(n=n)
+    tgt = workloads.config3(n=n, moduli_seed=12)
+    with GpuBlob.from_spec(tgt) as g:
+        g.step(K)
+        target = g.positions()
+    return spec, target
+
+
+def _grads(spec, target, form, K, interval, refine=0.0):
+    corner = spec.mesh.is_linear
+    prec, _, pc = form.partition("+")
+    with GpuBlob.from_spec(spec, precision=prec) as g:
+        if pc == "mg":
+            g.set_preconditioner("multigrid")
+        if refine:
+            g.set_refinement(refine)
+        g.run_checkpointed(K, interval)
+        L, seed = loss_and_seed(g.positions(), corner, target)
+        gr = g.adjoint(seed)[:4]
+        sweeps = g.refinement_sweeps()[1]
+    return L, gr, sweeps
+
+
+def test_mixed_refined_adjoint_matches_fp64():
+    """on a 16^3 static-load blob the unrefined mixed gradients are off the FP64 ones by
+    percents; with FP64 refinement of every solve (tolerance 1e-10) the loss agrees within
+    2e-6 and dL/dA, dB, dG, dE within 2e-5 (normwise), the FP32 element forces' rounding
+    being what is left (measured 3.2e-7 and 4.0e-6)"""
+    n, K, interval = 16, 40, 10
+    spec, target = _static_case(n, K, interval)
+    print(f"[case] {spec.name} ne={spec.num_elements} K={K}", flush=True)
+    L64, g64, _ = _grads(spec, target, "fp64+mg", K, interval)
+    Lm, gm, _ = _grads(spec, target, "mixed+mg", K, interval)
+    Lr, gr, sweeps = _grads(spec, target, "mixed+mg", K, interval, refine=1e-10)
+    err_plain = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(gm, g64))
+    err_ref = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(gr, g64)]
+    print(f"[refine] loss rel err plain {abs(Lm - L64) / abs(L64):.2e} refined {abs(Lr - L64) / abs(L64):.2e}; "
+          f"gradient rel err plain {err_plain:.2e} refined {max(err_ref):.2e}; correction solves {sweeps}",
+          flush=True)
+    assert abs(Lr - L64) <= 2e-6 * abs(L64)
+    assert max(err_ref) <= 2e-5, err_ref
+    assert err_plain >= 100 * max(err_ref)
+    assert sweeps > 0
+
+
+def test_refined_steps_match_oracle(oracle_mod):
+    """refined mixed steps of a pinned, loaded blob against the oracle's FP64 positions"""
+    spec = workloads.config3(n=6)
+    o = oracle_mod.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+    with GpuBlob.from_spec(spec, precision="mixed") as g:
+        g.set_refinement(1e-10)
+        for _ in range(20):
+            assert o.step() == 0
+        g.step(20)
+        e = np.abs(g.positions() - o.positions()).max() / np.abs(o.positions()).max()
+        print(f"[refine] 6^3 pinned blob, 20 refined mixed steps: positions rel err {e:.2e}, "
+              f"sweeps {g.refinement_sweeps()}", flush=True)
+        assert e <= 1e-9
+
+
+def test_refinement_refusals():
+    from ffea_ci_demo_amd.blob import FFEAException
+    with GpuBlob.from_spec(workloads.config0(), precision="mixed") as g:  # free blob: singular
+        with pytest.raises(FFEAException, match="singular"):
+            g.set_refinement(1e-10)
+    with GpuBlob.from_spec(workloads.config3(n=3)) as g:  # FP64
+        with pytest.raises(FFEAException, match="needs no refinement"):
+            g.set_refinement(1e-10)
+        g.set_refinement(0.0)
