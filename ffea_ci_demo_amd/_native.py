@@ -36,7 +36,7 @@ EXPORTED = [
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
     "ffea_gpu_blob_kinetic_energy", "ffea_gpu_solver_create", "ffea_gpu_solver_solve", "ffea_gpu_solver_apply_matrix",
-    "ffea_gpu_blob_set_refinement", "ffea_gpu_blob_get_refinement",
+    "ffea_gpu_blob_set_refinement", "ffea_gpu_blob_get_refinement", "ffea_gpu_batch_strain_energy_noalloc",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -149,6 +149,7 @@ def lib():
     L.ffea_gpu_blob_run_checkpointed.argtypes = [vp, C.c_int, C.c_int]
     L.ffea_gpu_blob_adjoint.argtypes = [vp, _dp, vp, vp, vp, vp, vp]
     L.ffea_gpu_batch_strain_energy.argtypes = [vp, _dp]
+    L.ffea_gpu_batch_strain_energy_noalloc.argtypes = [vp, _dp]
     L.ffea_gpu_blob_set_preconditioner.argtypes = [vp, C.c_int]
     L.ffea_gpu_blob_get_multigrid_info.argtypes = [vp, C.POINTER(MultigridInfo)]
     L.ffea_gpu_multigrid_inspect.argtypes = [C.c_int, _ip, _ip, _dp, _dp, vp, C.c_int, C.POINTER(MultigridInfo),

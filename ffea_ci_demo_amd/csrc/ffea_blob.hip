@@ -31,6 +31,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -44,6 +45,13 @@
 #include "mrg32k3a.h"
 
 using namespace ffk;
+
+// Scratch of the allocation-free entry point (ffea_gpu_batch_strain_energy_noalloc): 64 MiB
+// per GPU, held by every GPU the library runs on until the process ends. Host code never
+// uses this name as an address: each device's copy is found once with hipGetSymbolAddress
+// (scratch_on below) and that pointer is what kernels and copies receive.
+constexpr size_t kScratchDoubles = ((size_t)64 << 20) / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -71,6 +79,24 @@ std::string fmt(const char *f, ...) {
     vsnprintf(buf, sizeof buf, f, ap);
     va_end(ap);
     return buf;
+}
+
+// the device address of `scratch` on `device` (which must be the current device), looked
+// up on the first call for that device and kept
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+double *scratch_on(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) throw Invalid("device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (double *)ptr;
+    }
+    return g_scratch_ptr[device];
 }
 
 template <typename T>
@@ -2679,6 +2705,7 @@ int create_handle(const char *who, const ffea_gpu_params *params, const std::vec
     const int rc = guarded(b.get(), [&]() {
         FFEA_HIP_TRY(hipSetDevice(b->device));
         FFEA_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        (void)scratch_on(b->device);  // set-up: this device's scratch address, once per device
         b->build(in);
         return (int)FFEA_GPU_OK;
     });
@@ -3251,6 +3278,39 @@ int ffea_gpu_batch_strain_energy(ffea_gpu_blob *b, double *energy) {
         FFEA_HIP_TRY(hipGetLastError());
         FFEA_HIP_TRY(hipMemcpyAsync(energy, b->energy_out.p, sizeof(double) * b->nblob, hipMemcpyDeviceToHost,
                                     b->stream));
+        FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
+        return (int)FFEA_GPU_OK;
+    });
+}
+
+// ffea_gpu_batch_strain_energy without its lazily allocated buffers: the per-element
+// energies and the per-blob sums live in this device's `scratch` (through its looked-up
+// address), every entry of which the kernels write before they read it
+int ffea_gpu_batch_strain_energy_noalloc(ffea_gpu_blob *b, double *energy) {
+    if (!b || !energy) return FFEA_GPU_ERR_INVALID;
+    return guarded(b, [&]() {
+        FFEA_HIP_TRY(hipSetDevice(b->device));
+        const size_t need = (size_t)b->ne + (size_t)b->nblob;  // energies, then the per-blob sums
+        if (need > kScratchDoubles)
+            throw Invalid(fmt("ffea_gpu_batch_strain_energy_noalloc: %d elements and %d blobs need %zu doubles of "
+                              "scratch, more than its %zu (64 MiB); use ffea_gpu_batch_strain_energy",
+                              b->ne, b->nblob, need, kScratchDoubles));
+        double *w = scratch_on(b->device);
+        double *sums = w + b->ne;
+        if (b->mixed)
+            hipLaunchKernelGGL(k_strain_energy<float>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->pos.p, (const double *)b->rest.p,
+                               (const float *)b->f_jinv0.p, (const float *)b->f_vol0.p, (const float *)b->f_G.p,
+                               (const float *)b->f_E.p, w);
+        else
+            hipLaunchKernelGGL(k_strain_energy<double>, dim3(b->grid_elems), dim3(kBlock), 0, b->stream, b->ne,
+                               (const int *)b->conn.p, (const double *)b->pos.p, (const double *)b->rest.p,
+                               (const double *)b->d_jinv0.p, (const double *)b->d_vol0.p, (const double *)b->d_G.p,
+                               (const double *)b->d_E.p, w);
+        hipLaunchKernelGGL(k_blob_energy_range, dim3(b->nblob), dim3(kRowBlock), 0, b->stream, b->nblob, b->ne,
+                           (const int *)b->elem_off.p, (const double *)w, sums);
+        FFEA_HIP_TRY(hipGetLastError());
+        FFEA_HIP_TRY(hipMemcpyAsync(energy, sums, sizeof(double) * b->nblob, hipMemcpyDeviceToHost, b->stream));
         FFEA_HIP_TRY(hipStreamSynchronize(b->stream));
         return (int)FFEA_GPU_OK;
     });

@@ -1953,6 +1953,19 @@ __global__ __launch_bounds__(kRowBlock) void k_blob_energy(const int *elem_off, 
     if (threadIdx.x == 0) out[b] = 0.5 * v[0];
 }
 
+// per-blob sums of per-element energies whose blob b holds elements [elem_off[b],
+// elem_off[b+1]) (the last blob ends at ne): the allocation-free strain-energy call, which
+// has no per-blob count table
+__global__ __launch_bounds__(kRowBlock) void k_blob_energy_range(int nblob, int ne, const int *elem_off, const double *w,
+                                                              double *out) {
+    const int b = blockIdx.x;
+    const int e0 = elem_off[b], e1 = b + 1 < nblob ? elem_off[b + 1] : ne;
+    double v[1] = {0.0};
+    for (int e = e0 + threadIdx.x; e < e1; e += kRowBlock) v[0] += w[e];
+    block_sum<1, kRowBlock>(v);
+    if (threadIdx.x == 0) out[b] = 0.5 * v[0];
+}
+
 // ---------------------------------------------------------------- copy probe
 // y = x over n2 16-byte vectors, grid-stride: reads n2*16 and writes n2*16 bytes
 __global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {

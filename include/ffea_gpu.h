@@ -313,6 +313,16 @@ int ffea_gpu_batch_set_node_draw_offset(ffea_gpu_blob *blob, int index, long lon
  * (Blob::calculate_strain_energy, Blob.cpp:1052-1064; the measurement file's StrainEnergy) */
 int ffea_gpu_batch_strain_energy(ffea_gpu_blob *blob, double *energy);
 
+/* ffea_gpu_batch_strain_energy that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Its per-element energies and per-blob sums live in
+ * a fixed 64 MiB scratch array per GPU, held until the process ends, so the handle's
+ * elements plus blobs may not exceed 8,388,608 (FFEA_GPU_ERR_INVALID otherwise; the
+ * 6.0M-tet blob fits). Results equal ffea_gpu_batch_strain_energy's bit for bit. Every call on
+ * one GPU uses that GPU's one array: calls that overlap on one GPU (handles of the same device
+ * used from different threads at once) race and must not be made. */
+int ffea_gpu_batch_strain_energy_noalloc(ffea_gpu_blob *blob, double *energy);
+
 /* state after `ndraws` draws from `state` (exact jump-ahead; host only) */
 int ffea_gpu_advance_rng_state(const uint32_t state[6], long long ndraws, uint32_t out[6]);
 
