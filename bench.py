@@ -4,23 +4,27 @@ Workload (default): the 100x100x100 jittered-cube blob — 6.0M quadratic tets,
 1.03M corner nodes — with the reference material (shear/bulk modulus 3.6e8 /
 2.2e9 Pa, viscosities 1e-3 Pa s), thermal noise on at 300 K, dt = 1e-14 s, CG
 tolerance 1e-10, steric/electrostatics off. One step = one full blob update:
-element forces, gather, viscosity-matrix assembly, Jacobi-PCG to tolerance, Euler
+element forces, gather, viscosity-matrix assembly, the CG to tolerance, Euler
 update. Inputs are resident in HBM before the timed region, which is bracketed by a
 synchronisation of the library's HIP stream on both sides (all work runs on it; one
 GPU, so there is no barrier across ranks). torch is not used: it bundles a HIP runtime
 of its own that must not share the process with the library's.
 
-Prints one JSON line (rank 0). `value` is the headline form's element-steps/s
-(default: FP32-mixed); `forms` holds every form measured (default FP64 and mixed,
-with the mixed form's position error against FP64). `roofline` is for the dominant
+Prints one JSON line (rank 0). `value` is the headline form's element-steps/s (default:
+FP32-mixed with the multigrid preconditioner); `forms` holds every form measured (FP64
+and mixed with the reference's Jacobi-PCG and with multigrid, each autotuned before its
+timed steps, with its position error against FP64). `roofline` is for the dominant
 kernel (the CG SpMV step), from HIP events recorded on the library's stream over the
-timed region. `cpu_baseline` times the oracle (the FP64 C restatement of the
-reference algorithm, single thread) on a bounded sample of the same workload.
+timed region. `cpu_baseline` times whole steps of the oracle (the FP64 C restatement of
+the reference algorithm) on the host's cores, OpenMP build, beside a single-thread
+figure.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 100] [--forms fp64,mixed]
-  python bench.py --workload ensemble [--blobs 512]   # many blobs in one handle,
-                                                      # graph-captured steps
-  python bench.py --workload masslumped               # the lumped-mass (inertial) solver
+  python bench.py --workload ensemble [--blobs 512] [--forms mixed+mg] [--no-graph]
+                                              # many blobs in one handle
+  python bench.py --workload adjoint          # gradients (config-3 blob), +ref forms
+  python bench.py --workload illcond          # the ill-conditioned blob, 2000 steps
+  python bench.py --workload masslumped       # the lumped-mass (inertial) solver
 """
 from __future__ import annotations
 
@@ -82,7 +86,7 @@ def refined(form):
     return "ref" in form.split("+")[1:]
 
 
-def run_form(spec, form, steps, warmup):
+def run_form(spec, form, steps, warmup, tune=True):
     from ffea_ci_demo_amd.blob import GpuBlob
     prec, pc = split_form(form)
     t0 = time.perf_counter()
@@ -102,8 +106,12 @@ def run_form(spec, form, steps, warmup):
           file=sys.stderr, flush=True)
     for _ in range(warmup):
         g.step(1)
+    if tune:
+        g.autotune(10)  # row grid, SpMV row unroll, assembly lanes: result-neutral
+        g.step(1)
+    shape = g.launch_shape()
     _, tot0 = g.cg_iterations()
-    g.set_timing(("cg_spmv", "cg_vector"))
+    g.set_timing(("element", "assemble", "gather_cg_start", "cg_spmv", "cg_vector"))
     g.synchronize()
     t0 = time.perf_counter()
     for s in range(steps):
@@ -119,7 +127,10 @@ def run_form(spec, form, steps, warmup):
     vec_ms, vec_n = tim["cg_vector"]
     avg_spmv_s = spmv_ms / max(spmv_n, 1) / 1e3
     ne = info["num_elements"]
+    per_step = {k: round(tim[k][0] / max(tim[k][1], 1), 4) for k in ("element", "assemble", "gather_cg_start")}
     res = {
+        "launch_shape": shape,
+        "phase_ms_per_launch": per_step,
         "element_steps_per_s": ne * steps / wall,
         "ms_per_step": wall / steps * 1e3,
         "cg_iterations_per_step": iters / steps,
@@ -147,25 +158,35 @@ def run_form(spec, form, steps, warmup):
     return res, positions
 
 
-def run_ensemble(specs, form, steps, warmup, tune_iters=20):
-    """all blobs in one handle: autotuned row grid, one eager timed step for the SpMV
-    average, then graph-captured steps timed end to end"""
+def run_ensemble(specs, form, steps, warmup, tune_iters=20, graph=True):
+    """all blobs in one handle (+mg: one multigrid hierarchy per blob): autotuned row grid,
+    one eager timed step for the SpMV average, then graph-captured (or, graph=False, eager)
+    steps timed end to end"""
     from ffea_ci_demo_amd.blob import GpuBatch
+    prec, pc = split_form(form)
     t0 = time.perf_counter()
-    g = GpuBatch(specs, precision=form)
+    g = GpuBatch(specs, precision=prec)
     t_create = time.perf_counter() - t0
     info = g.info()
     print(f"[bench] ensemble of {len(specs)} blobs form={form} created in {t_create:.1f}s: Ne={info['num_elements']} "
           f"Nl={info['num_linear_nodes']} chunks={info['num_chunks']} device_bytes={info['device_bytes']/1e9:.2f}GB",
           file=sys.stderr, flush=True)
+    mg = None
+    if pc == "multigrid":
+        t0 = time.perf_counter()
+        mg = g.set_preconditioner("multigrid")
+        print(f"[bench] ensemble multigrid: {mg['num_levels']} levels, rows {mg['rows']}, coarsest "
+              f"{mg['coarsest_unknowns']} unknowns, set-up {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
     g.step(1)
     chosen = g.autotune(tune_iters)
-    print(f"[bench] autotuned row grid: {chosen} workgroups", file=sys.stderr, flush=True)
+    shape = g.launch_shape()
+    print(f"[bench] autotuned launch shape: {shape}", file=sys.stderr, flush=True)
     g.set_timing(("cg_spmv", "cg_vector"))
     g.step(1)
     tim = g.timing()
     g.set_timing(())
-    g.use_graph(True)
+    g.use_graph(graph)
     for _ in range(warmup):
         g.step(1)
     last0, tot0 = g.cg_iterations()
@@ -177,8 +198,8 @@ def run_ensemble(specs, form, steps, warmup, tune_iters=20):
     last1, tot1 = g.cg_iterations()
     k = (tot1 - tot0) / steps
     info = g.info()
-    w = 8 if form == "fp64" else 4
-    proj = projects(specs, form)
+    w = 8 if prec == "fp64" else 4
+    proj = projects(specs, prec)
     spmv_ms, spmv_n = tim["cg_spmv"]
     avg_spmv_s = spmv_ms / max(spmv_n, 1) / 1e3
     ne = info["num_elements"]
@@ -192,9 +213,13 @@ def run_ensemble(specs, form, steps, warmup, tune_iters=20):
         "spmv_alg_bytes": spmv_alg_bytes(info, w, proj),
         "spmv_achieved_GBs": spmv_alg_bytes(info, w, proj) / avg_spmv_s / 1e9,
         "row_workgroups": chosen,
+        "launch_shape": shape,
         "graph_cg_iterations": info["graph_iterations"],
+        "graph": graph,
         "info": info,
     }
+    if mg is not None:
+        res["multigrid"] = {k: mg[k] for k in ("num_levels", "rows", "coarsest_unknowns", "setup_seconds")}
     print(f"[bench] ensemble form={form} {res['ms_per_step']:.1f} ms/step, k max/median "
           f"{res['cg_iterations_per_step_max']:.0f}/{res['cg_iterations_per_step_median']:.0f}, "
           f"spmv {res['spmv_avg_us_all_blobs_running']:.1f} us = {res['spmv_achieved_GBs']:.0f} GB/s",
@@ -252,6 +277,33 @@ def cpu_baseline(spec, k_per_step, cg_iters, n=None):
     }
 
 
+def cpu_baseline_ensemble(specs, stride=64):
+    """The oracle on one host core over a bounded sample of the ensemble: every stride-th
+    blob by size (so the sample spans the size range), one whole step each, the
+    reference's Jacobi-PCG run to convergence; element-steps/s = sampled tets / time"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    order = sorted(specs, key=lambda s: s.num_elements)
+    sample = order[stride // 2::stride]
+    ne, secs, iters = 0, 0.0, []
+    for spec in sample:
+        o = oracle.Blob(spec.pos, spec.top, spec.A, spec.B, spec.G, spec.E, **spec.oracle_kwargs())
+        t0 = time.perf_counter()
+        if o.step() != 0:
+            raise RuntimeError(f"oracle step failed on {spec.name}")
+        secs += time.perf_counter() - t0
+        ne += spec.num_elements
+        iters.append(o.cg_iterations())
+        del o
+    print(f"[bench] cpu oracle: {len(sample)} blobs, {ne} tets, {secs:.1f} s, Jacobi-PCG iterations "
+          f"{min(iters)}..{max(iters)}", file=sys.stderr, flush=True)
+    return {"value": ne / secs, "unit": "element-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"one whole step (element pass, matrix build, Jacobi-PCG to convergence: "
+                       f"{min(iters)}..{max(iters)} iterations, median {int(np.median(iters))}) of every "
+                       f"{stride}th blob by size, {len(sample)} blobs, {ne} tets, {secs:.1f} s on 1 thread "
+                       f"(-O2, strict IEEE)")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +322,7 @@ def main():
                     help="adjoint: forms whose gradient is checked by central differences (CG epsilon 1e-13)")
     ap.add_argument("--cpu-cg-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="ensemble: eager steps instead of graph-captured ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -462,7 +515,7 @@ def main_ensemble(args):
     print(f"[bench] {len(specs)} meshes ({ne_tot} tets, n {min(ns)}..{max(ns)}) built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     form = (args.headline or "mixed") if args.forms == DEFAULT_FORMS else args.forms.split(",")[-1]
-    h = run_ensemble(specs, form, args.steps, args.warmup)
+    h = run_ensemble(specs, form, args.steps, args.warmup, graph=not args.no_graph)
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
     roof = {
@@ -479,13 +532,7 @@ def main_ensemble(args):
     }
     cpu = None
     if not args.no_cpu_baseline:
-        mid = sorted(specs, key=lambda s: s.num_elements)[len(specs) // 2]
-        cpu = cpu_baseline(mid, h["cg_iterations_per_step_median"], args.cpu_cg_iters)
-        cpu["sample"] = f"median blob ({mid.name}, {mid.num_elements} tets): " + cpu["sample"]
-    ratios = {}
-    for a, b in (("fp64", "mixed"), ("fp64+mg", "mixed+mg"), ("fp64", "mixed+mg"), ("fp64", "fp64+mg")):
-        if a in results and b in results:
-            ratios[f"{b}_vs_{a}"] = round(results[a]["ms_per_step"] / results[b]["ms_per_step"], 3)
+        cpu = cpu_baseline_ensemble(specs)
     out = {
         "metric": METRIC,
         "value": round(h["element_steps_per_s"], 1),
@@ -497,13 +544,14 @@ def main_ensemble(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if form == "fp64" else "f32+f64",
+        "dtype": "f64" if split_form(form)[0] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"ensemble of {len(specs)} independent jittered-cube blobs, n^3 cubes with n = "
                         f"round(exp(U(ln 8, ln 40))) ({min(ns)}..{max(ns)}), {ne_tot} tets in all; shear and bulk "
                         "moduli each log-uniform in [1e7, 1e9] Pa; one MRG32k3a stream per blob; noise at 300 K, "
-                        "dt 1e-14 s, CG epsilon 1e-10; one handle, graph-captured steps",
+                        "dt 1e-14 s, CG epsilon 1e-10; one handle, " +
+                        ("graph-captured steps" if not args.no_graph else "eager steps"),
             "form": form,
             "parallelism": "single GPU",
         },

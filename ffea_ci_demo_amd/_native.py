@@ -37,6 +37,7 @@ EXPORTED = [
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
     "ffea_gpu_blob_kinetic_energy", "ffea_gpu_solver_create", "ffea_gpu_solver_solve", "ffea_gpu_solver_apply_matrix",
     "ffea_gpu_blob_set_refinement", "ffea_gpu_blob_get_refinement", "ffea_gpu_batch_strain_energy_noalloc",
+    "ffea_gpu_blob_get_launch_shape", "ffea_gpu_blob_set_launch_shape",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -144,6 +145,8 @@ def lib():
     L.ffea_gpu_advance_rng_state.argtypes = [_up, C.c_longlong, _up]
     L.ffea_gpu_blob_set_row_blocks.argtypes = [vp, C.c_int]
     L.ffea_gpu_blob_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.ffea_gpu_blob_get_launch_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.ffea_gpu_blob_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.ffea_gpu_blob_use_graph.argtypes = [vp, C.c_int, C.c_int]
     L.ffea_gpu_blob_set_material.argtypes = [vp, _dp, _dp, _dp, _dp]
     L.ffea_gpu_blob_run_checkpointed.argtypes = [vp, C.c_int, C.c_int]

@@ -84,6 +84,17 @@ class _Handle:
         self._check(_native.lib().ffea_gpu_blob_autotune(self.h, int(iterations), C.byref(chosen)))
         return chosen.value
 
+    def launch_shape(self):
+        """{"row_blocks", "spmv_unroll", "assembly_lanes"}: the launch-shape knobs in force"""
+        v = [C.c_int() for _ in range(3)]
+        self._check(_native.lib().ffea_gpu_blob_get_launch_shape(self.h, *[C.byref(x) for x in v]))
+        return {"row_blocks": v[0].value, "spmv_unroll": v[1].value, "assembly_lanes": v[2].value}
+
+    def set_launch_shape(self, row_blocks=0, spmv_unroll=0, assembly_lanes=0):
+        """sets the launch-shape knobs (0: unchanged); results do not depend on them"""
+        self._check(_native.lib().ffea_gpu_blob_set_launch_shape(self.h, int(row_blocks), int(spmv_unroll),
+                                                                 int(assembly_lanes)))
+
     def use_graph(self, enable=True, iterations=0):
         """run each step() as one captured HIP graph launch holding `iterations` CG
         iterations (0: a little over the last step's count)"""

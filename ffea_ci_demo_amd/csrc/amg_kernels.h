@@ -1,6 +1,9 @@
 // Device code of the multigrid preconditioner (MI355X, gfx950). Included by ffea_blob.hip
-// after kernels.h. Single-blob handles only: every kernel returns at once when blob 0's
-// solve has stopped.
+// after kernels.h. A single blob's kernels return at once when its solve has stopped. An
+// ensemble's levels are the blobs' own levels side by side (block-diagonal operators);
+// its coarse-level kernels run for every blob (a stopped blob's CG no longer reads z),
+// its level-0 kernels skip the chunks of stopped blobs, and the smoother weights are the
+// blobs' own: per row on level 0, folded into the stored diagonal above.
 //
 // One V-cycle z = M r, with level 0 the corner-node viscosity matrix K (3x3 blocks, the
 // CG's own sliced-ELL storage) and levels 1.. the Galerkin operators over aggregates
@@ -21,6 +24,17 @@ namespace ffk {
 
 constexpr int kMgBlock = 256;  // threads per workgroup of the coarse-level kernels
 
+// a coarse-level kernel has nothing to do once the solve it serves has stopped: blob 0's
+// (S) for a single blob, every blob's (num_done of active) for an ensemble
+struct MgStop {
+    const CgScalars *S;
+    const int *num_done;
+    int active;
+    __device__ __forceinline__ bool stopped() const {
+        return S ? S->done != 0 : (num_done && *num_done >= active);
+    }
+};
+
 // vector layouts: level 0 tiled by 64 rows (vi), coarse levels plain [rows][6]
 template <int B>
 struct VecLayout;
@@ -36,7 +50,7 @@ struct VecLayout<6> {
 enum MgMode { MG_RESID = 0, MG_RESTRICT = 1, MG_PROLONG = 2, MG_POST = 3 };
 
 struct MgArgs {
-    const CgScalars *S;     // blob 0
+    MgStop stop;
     int n;                  // block rows
     const int *sw;          // [slices] slot widths
     const long long *sb;    // [slices] slot bases
@@ -49,6 +63,7 @@ struct MgArgs {
     const double *dinv;     // RESTRICT: the next level's D^-1; POST: this level's (row layout)
     double omega, bscale;
     double *y, *y2;         // RESTRICT: y = A x, y2 = omega D^-1 y (if not null)
+    const double *bscale_row = nullptr;  // PROLONG on level 0 of an ensemble: bscale per row
 };
 
 // y = A x and the mode's row update, one thread per block row:
@@ -58,7 +73,7 @@ struct MgArgs {
 //   POST      y = x + omega D^-1 (b - A x) (the post-smoothing step; A square)
 template <int BR, int BC, int MODE>
 __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
-    if (a.S->done) return;
+    if (a.stop.stopped()) return;
     const int row = blockIdx.x * kMgBlock + threadIdx.x;
     if (row >= a.n) return;
     const int sl = row >> 6, lane = row & 63;
@@ -90,7 +105,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
             a.y[o] = acc[p];
             if (a.y2) a.y2[o] = a.omega * a.dinv[o] * acc[p];
         } else if constexpr (MODE == MG_PROLONG) {
-            a.y[o] = a.bscale * a.base[o] + acc[p];
+            a.y[o] = (a.bscale_row ? a.bscale_row[row] : a.bscale) * a.base[o] + acc[p];
         } else {
             a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - acc[p]);
         }
@@ -105,7 +120,7 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
 // 2 for 6x3), vector t of block j is at (rp[row] * BR*BC) + (t W + j) V, so one vector
 // load of a wavefront reads consecutive blocks' vectors (contiguous bytes).
 struct MgCsrArgs {
-    const CgScalars *S;
+    MgStop stop;
     int n;
     const int *rp, *ci;
     const _Float16 *val;    // [nnzb][BR*BC], scaled
@@ -117,7 +132,7 @@ struct MgCsrArgs {
 
 template <int BR, int BC, int MODE>
 __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
-    if (a.S->done) return;
+    if (a.stop.stopped()) return;
     const int row = blockIdx.x * (kMgBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= a.n) return;  // whole wavefronts
     double acc[BR];
@@ -171,16 +186,34 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
 }
 
 // coarsest level: e = C b with the dense inverse C (n x n, FP32, symmetric: row `row`
-// is read as contiguous bytes), one wavefront per row
-__global__ __launch_bounds__(kMgBlock) void k_mg_dense(const CgScalars *S, int n, const float *C, const double *b,
-                                                    double *e) {
-    if (S->done) return;
+// is read as contiguous bytes), one wavefront per row. An ensemble has one inverse per
+// blob (row_blob non-null): row `row` belongs to blob q = row_blob[row], whose rows are
+// [off[q], off[q] + m[q]) and whose inverse starts at C + coff[q].
+struct MgDenseArgs {
+    MgStop stop;
+    int n;
+    const float *C;
+    const double *b;
+    double *e;
+    const int *row_blob, *off, *m;
+    const long long *coff;
+};
+__global__ __launch_bounds__(kMgBlock) void k_mg_dense(MgDenseArgs a) {
+    if (a.stop.stopped()) return;
     const int row = blockIdx.x * (kMgBlock / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (row >= n) return;
+    if (row >= a.n) return;
+    int o = 0, m = a.n;
+    const float *C = a.C;
+    if (a.row_blob) {
+        const int q = a.row_blob[row];
+        o = a.off[q];
+        m = a.m[q];
+        C += a.coff[q];
+    }
     double acc = 0.0;
-    for (int k = lane; k < n; k += 64) acc += (double)C[(size_t)row * n + k] * b[k];
+    for (int k = lane; k < m; k += 64) acc += (double)C[(size_t)(row - o) * m + k] * a.b[o + k];
     acc = wave_sum(acc);
-    if (lane == 0) e[row] = acc;
+    if (lane == 0) a.e[row] = acc;
 }
 
 // Level-0 steps on the CG's own matrix storage, one chunk of rows per workgroup
@@ -192,7 +225,7 @@ constexpr int MG_FINE_RESID = 0, MG_FINE_POST = 1;
 
 template <typename T>
 struct MgFineArgs {
-    const CgScalars *S;
+    const CgScalars *S;    // [nblob]: a chunk of a stopped blob is skipped
     const Chunk *chunks;
     int nchunks;
     const int *slice_w;
@@ -205,16 +238,19 @@ struct MgFineArgs {
     double omega;
     double *y;             // RESID: residual; POST: z
     double *P;             // POST: partials row (P[c])
+    const double *omega_row = nullptr;  // an ensemble's smoother weight per row (else omega)
 };
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(kRowBlock) void k_mg_fine(MgFineArgs<T> a) {
-    if (a.S->done) return;
+// BS threads per chunk: kRowBlock for chunks of up to 1024 rows, one row a thread; an
+// ensemble's mostly 256-row chunks take kMgBlock, looping over longer chunks
+template <typename T, int MODE, int BS = kRowBlock>
+__global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
     const int c = blockIdx.x;
     const Chunk ch = a.chunks[c];
-    const int n = ch.begin + threadIdx.x;
+    if (a.S[ch.blob].done) return;
     double v[1] = {0.0};
-    if (n < ch.end) {
+    for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
+        const double omega = a.omega_row ? a.omega_row[n] : a.omega;
         const int sl = n / kSlice, lane = n % kSlice;
         const int W = a.slice_w[sl];
         const long long base = a.slice_base[sl];
@@ -236,15 +272,15 @@ __global__ __launch_bounds__(kRowBlock) void k_mg_fine(MgFineArgs<T> a) {
             const size_t o = vi(n, k);
             const double rk = a.r[o];
             if constexpr (MODE == MG_FINE_RESID) {
-                a.y[o] = rk - a.omega * acc[k];
+                a.y[o] = rk - omega * acc[k];
             } else {
-                const double zk = a.x[o] + a.omega * a.dinv[o] * (rk - acc[k]);
+                const double zk = a.x[o] + omega * a.dinv[o] * (rk - acc[k]);
                 a.y[o] = zk;
                 v[0] += rk * zk;
             }
         }
     }
-    if constexpr (MODE == MG_FINE_POST) store_partials<1, kRowBlock>(v, a.P, a.nchunks, c);
+    if constexpr (MODE == MG_FINE_POST) store_partials<1, BS>(v, a.P, a.nchunks, c);
 }
 
 // D-weighted rigid-mode gauge of a free blob's solution. Jacobi-PCG from x = 0 returns
@@ -253,14 +289,16 @@ __global__ __launch_bounds__(kRowBlock) void k_mg_fine(MgFineArgs<T> a) {
 // with D the matrix diagonal. Partials per chunk: Z^T D x (6) and the upper triangle
 // of Z^T D Z (21).
 struct MgRigidArgs {
-    const CgScalars *S;
+    const CgScalars *S;     // [nblob]
     const Chunk *chunks;
     int nchunks;
     const double *x, *dinv, *pos;
-    const double3 *centre;  // blob 0's rigid-mode centre
+    const double3 *centre;  // [nblob] rigid-mode centres
     double *P;      // [27][nchunks]
-    double *coef;   // [6]
+    double *coef;   // [nblob][6]
     double *xout;   // x (corrected in place)
+    const int2 *seg;        // [nblob] chunk range of each blob
+    const int *free_blob;   // [nblob] 1: a free blob, whose solution is put in the gauge
 };
 
 __device__ __forceinline__ void rigid_basis(const double *d, double (&Z)[3][6]) {
@@ -277,11 +315,11 @@ __device__ __forceinline__ void rigid_basis(const double *d, double (&Z)[3][6]) 
 
 // 256 threads per chunk (4 rows each): the 27 sums stay in registers
 __global__ __launch_bounds__(kMgBlock) void k_mg_rigid_sums(MgRigidArgs a) {
-    const CgScalars &S = *a.S;
-    if (!S.converged || S.updated) return;
     const int c = blockIdx.x;
     const Chunk ch = a.chunks[c];
-    const double3 cen = a.centre[0];
+    const CgScalars &S = a.S[ch.blob];
+    if (!S.converged || S.updated || !a.free_blob[ch.blob]) return;
+    const double3 cen = a.centre[ch.blob];
     double v[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) v[k] = 0.0;
@@ -307,11 +345,13 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_rigid_sums(MgRigidArgs a) {
     store_partials<27, kMgBlock>(v, a.P, a.nchunks, c);
 }
 
+// one workgroup per blob
 __global__ __launch_bounds__(kMgBlock) void k_mg_rigid_coef(MgRigidArgs a) {
-    const CgScalars &S = *a.S;
-    if (!S.converged || S.updated) return;
+    const int q = blockIdx.x;
+    const CgScalars &S = a.S[q];
+    if (!S.converged || S.updated || !a.free_blob[q]) return;
     double t[27];
-    sum_partials<27, kMgBlock>(a.P, a.nchunks, 0, a.nchunks, t);
+    sum_partials<27, kMgBlock>(a.P, a.nchunks, a.seg[q].x, a.seg[q].y, t);
     if (threadIdx.x == 0) {
         double M[36], Mi[36];
         int u = 6;
@@ -324,22 +364,22 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_rigid_coef(MgRigidArgs a) {
         for (int k = 0; k < 6; ++k) {
             double acc = 0.0;
             for (int l = 0; l < 6; ++l) acc += Mi[6 * k + l] * t[l];
-            a.coef[k] = acc;
+            a.coef[6 * q + k] = acc;
         }
     }
 }
 
 __global__ __launch_bounds__(kRowBlock) void k_mg_rigid_fix(MgRigidArgs a) {
-    const CgScalars &S = *a.S;
-    if (!S.converged || S.updated) return;
     const Chunk ch = a.chunks[blockIdx.x];
+    const CgScalars &S = a.S[ch.blob];
+    if (!S.converged || S.updated || !a.free_blob[ch.blob]) return;
     const int n = ch.begin + threadIdx.x;
     if (n >= ch.end) return;
-    const double3 cen = a.centre[0];
-        const double d[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
+    const double3 cen = a.centre[ch.blob];
+    const double d[3] = {a.pos[3 * n] - cen.x, a.pos[3 * n + 1] - cen.y, a.pos[3 * n + 2] - cen.z};
     double cf[6], zc[3];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cf[k] = a.coef[k];
+    for (int k = 0; k < 6; ++k) cf[k] = a.coef[6 * ch.blob + k];
     z_times(d, cf, zc);
 #pragma unroll
     for (int i = 0; i < 3; ++i) a.xout[vi(n, i)] -= zc[i];

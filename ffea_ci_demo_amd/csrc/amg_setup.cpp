@@ -492,6 +492,7 @@ Hierarchy build(const BCsr &fine, const double *pos, const unsigned char *pinned
         std::vector<int> agg;
         const int na = aggregate(L.A, active, agg);
         if (na == 0 || 6 * na >= (int)(0.8 * unknowns)) break;  // no coarsening progress
+        if (na < opt.min_aggregates) break;
         BCsr Pt;
         std::vector<double> Bc;
         if (l == 0) {

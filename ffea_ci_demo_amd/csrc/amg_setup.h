@@ -40,6 +40,8 @@ struct Options {
     int max_coarse = 1200;     // unknowns at which coarsening stops
     int max_levels = 8;
     double smoother_safety = 1.3;  // the fine operator drifts from the set-up one over a run
+    int min_aggregates = 1;    // a coarsening to fewer aggregates stops instead: a level of one
+                               // aggregate of a free blob is its null space and nothing else
     int threads = 16;
 };
 

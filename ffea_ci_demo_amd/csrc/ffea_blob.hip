@@ -695,6 +695,13 @@ struct Tables {
     std::vector<double> dn_lval, rf_axis, rf_F;
 };
 
+// Rows per reduction chunk of a blob, a function of its own size only (so a blob sums alike
+// alone and in an ensemble): blobs of a million rows take kChunkRows, one chunk per
+// 1024-thread workgroup, and so do blobs that solve in one workgroup (k_cg_fused, one chunk);
+// those between take 256, so that a blob of some 10^5 rows still spreads over the 256 CUs
+// (69k rows: 270 chunks rather than 68)
+int chunk_rows(int nl) { return (nl <= kFusedMaxRows || nl >= (1 << 18)) ? kChunkRows : kChunkRows / 4; }
+
 Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_params &prm) {
     Tables t;
     const int nblob = t.nblob = (int)in.size();
@@ -905,12 +912,12 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         }
         for (size_t k = 0; k < P.ic.size(); ++k) t.ic[(size_t)4 * eo + k] = P.ic[k] + 4u * (uint32_t)eo;
         t.fv.insert(t.fv.end(), P.fv.begin(), P.fv.end());
-        // reduction chunks: up to kChunkRows rows of one blob each
+        // reduction chunks: up to chunk_rows(nl) rows of one blob each
         t.seg[b].x = (int)t.chunks.size();
         if (!m.all_pinned) {
-            const int first = (int)t.chunks.size();
-            for (int r0 = 0; r0 < P.nl; r0 += kChunkRows)
-                t.chunks.push_back(Chunk{b, ro + r0, ro + std::min(P.nl, r0 + kChunkRows), first});
+            const int first = (int)t.chunks.size(), cr = chunk_rows(P.nl);
+            for (int r0 = 0; r0 < P.nl; r0 += cr)
+                t.chunks.push_back(Chunk{b, ro + r0, ro + std::min(P.nl, r0 + cr), first});
         }
         t.seg[b].y = (int)t.chunks.size();
         t.centre[b] = make_double3(P.centre[0], P.centre[1], P.centre[2]);
@@ -1188,6 +1195,7 @@ struct ffea_gpu_blob {
     double cg_eps2() const { return eps2_override > 0.0 ? eps2_override : prm.epsilon2; }
     // every blob has at most kFusedMaxRows rows: each CG solve is one k_cg_fused launch
     bool fused_cg = false;
+    bool small_chunks = false;  // no chunk has more than kMixedBlock rows
     int nb = 0, cpb = 0;  // row-kernel grid and reduction chunks per block
 
     // host-visible status (pinned)
@@ -1239,6 +1247,10 @@ struct ffea_gpu_blob {
     DevBuf<_Float16> mg_hval;  // scaled FP16 copy of the viscosity matrix, written by the assembly
     double mg_hscale = 1.0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
+    DevBuf<double> mg_omega0;                           // ensemble: level-0 smoother weight per row
+    DevBuf<int> mg_crow_blob, mg_coff_row, mg_cm, mg_free;  // ensemble coarsest tables; free blobs
+    DevBuf<long long> mg_ccoff;
+    bool mg_any_free = false;
     ffea_gpu_multigrid_info mg_info{};
     bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID; }
 
@@ -1317,8 +1329,15 @@ struct ffea_gpu_blob {
     // assembly grid: a workgroup per slot and row block for small meshes, per row block
     // (its lanes walking their rows' slots) from 1024 row blocks on
     // lanes per row of the assembly: 1 once the rows alone fill 1024 workgroups, else 16
-    int asm_lanes() const { return grid_rows >= 1024 ? 1 : 16; }
-    dim3 asm_grid() const { return dim3((unsigned)(((long long)nrows * asm_lanes() + kBlock - 1) / kBlock)); }
+    int asm_lanes() const { return asm_lanes_set ? asm_lanes_set : (grid_rows >= 1024 ? 1 : 16); }
+    // launch-shape knobs the autotuner sets (results do not depend on them)
+    int asm_lanes_set = 0;              // lanes per row of the assembly (1, 4 or 16); 0: by size
+    int spmv_unroll = kMaxUnrolledRow;  // blocks per row the SpMV issues together (16 or 8)
+    dim3 asm_grid() const {  // a multiple of 8 workgroups: each XCD takes one contiguous eighth of the rows
+        long long g = ((long long)nrows * asm_lanes() + kBlock - 1) / kBlock;
+        if (g >= 8 && g % 8) g += 8 - g % 8;
+        return dim3((unsigned)g);
+    }
     void launch_element(bool advance = true);
     void launch_assemble();
     void launch_assemble_gather();
@@ -1422,6 +1441,8 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     for (const BlobMeta &m : meta) fused_cg = fused_cg && m.nl <= kFusedMaxRows;
     grid_elems = (ne + kBlock - 1) / kBlock;
     grid_blobs = (nblob + kBlock - 1) / kBlock;
+    small_chunks = !t.chunks.empty();
+    for (const Chunk &c : t.chunks) small_chunks = small_chunks && c.end - c.begin <= kMixedBlock;
     conn.alloc(t.conn.size(), &dev_bytes); conn.upload(t.conn, stream);
     pos.alloc(t.pos.size(), &dev_bytes); pos.upload(t.pos, stream);
     rest.alloc(t.pos.size(), &dev_bytes); rest.upload(t.pos, stream);
@@ -1686,11 +1707,13 @@ void ffea_gpu_blob::launch_assemble() {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p,
                          f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
         if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<float, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        else if (asm_lanes() == 4) hipLaunchKernelGGL((k_assemble<float, 4>), asm_grid(), dim3(kBlock), 0, stream, a);
         else hipLaunchKernelGGL((k_assemble<float, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p,
                           d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
         if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<double, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        else if (asm_lanes() == 4) hipLaunchKernelGGL((k_assemble<double, 4>), asm_grid(), dim3(kBlock), 0, stream, a);
         else hipLaunchKernelGGL((k_assemble<double, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     }
     FFEA_HIP_TRY(hipGetLastError());
@@ -1703,7 +1726,15 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
     if (t_cg(3, it)) t_begin(3, &ts);
     SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
-    hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, sa);
+    constexpr int U2 = kMaxUnrolledRow / 2;
+    const bool half = spmv_unroll != kMaxUnrolledRow;
+    if (std::is_same<T, double>::value && small_chunks) {
+        if (half) hipLaunchKernelGGL((k_cg_spmv<T, PROJ, kMixedBlock, U2>), dim3(nb), dim3(kMixedBlock), 0, stream, sa);
+        else hipLaunchKernelGGL((k_cg_spmv<T, PROJ, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, sa);
+    } else {
+        if (half) hipLaunchKernelGGL((k_cg_spmv<T, PROJ, row_block<T>, U2>), dim3(nb), dim3(row_block<T>), 0, stream, sa);
+        else hipLaunchKernelGGL((k_cg_spmv<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, sa);
+    }
     t_end(3, ts, it);
     if (!spmv_only) launch_cg_vec(it);
 }
@@ -1717,7 +1748,7 @@ void ffea_gpu_blob::launch_cg_vec(int it) {
                            (const double *)Gmat.p, coef.p);
         hipLaunchKernelGGL((k_cg_vec<true, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
     } else {
-        if (mixed)
+        if (mixed || small_chunks)
             hipLaunchKernelGGL((k_cg_vec<false, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
         else
             hipLaunchKernelGGL((k_cg_vec<false, kRowBlock>), dim3(nb), dim3(kRowBlock), 0, stream, va);
@@ -1759,7 +1790,7 @@ void ffea_gpu_blob::launch_cg_fused() {
 void ffea_gpu_blob::launch_update() {
     hipEvent_t t0;
     t_begin(5, &t0);
-    if (mg_on() && meta[0].free_blob) launch_mg_gauge();
+    if (mg_on() && mg_any_free) launch_mg_gauge();
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(kRowBlock), 0, stream, row_args(), (const double *)x.p, vel.p, pos.p,
                        prm.dt);
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
@@ -1816,15 +1847,68 @@ EllHost to_ell(const amg::BCsr &A, double scale) {
 }
 }  // namespace
 
+// An identity transfer between two copies of a coarse level (6x6 blocks): a blob whose own
+// hierarchy is shallower than the ensemble's repeats its coarsest level, whose exact solve
+// the padded levels pass straight through
+static amg::BCsr identity6(int n) {
+    amg::BCsr I;
+    I.n = I.ncols = n;
+    I.br = I.bc = 6;
+    I.rp.resize(n + 1);
+    I.ci.resize(n);
+    I.v.assign((size_t)n * 36, 0.0);
+    for (int i = 0; i < n; ++i) {
+        I.rp[i] = i;
+        I.ci[i] = i;
+        for (int k = 0; k < 6; ++k) I.v[(size_t)i * 36 + 7 * k] = 1.0;
+    }
+    I.rp[n] = n;
+    return I;
+}
+
+// blobs' matrices side by side: part q's rows at row_off[q], its columns shifted by
+// col_off[q]; rows no part covers are empty
+static amg::BCsr side_by_side(const std::vector<const amg::BCsr *> &part, const std::vector<int> &row_off,
+                              const std::vector<int> &col_off, int nrows, int ncols) {
+    amg::BCsr C;
+    C.n = nrows;
+    C.ncols = ncols;
+    C.br = part[0]->br;
+    C.bc = part[0]->bc;
+    const int bb = C.br * C.bc;
+    std::vector<int> cnt(nrows, 0);
+    for (size_t q = 0; q < part.size(); ++q)
+        for (int i = 0; i < part[q]->n; ++i) cnt[row_off[q] + i] = part[q]->rp[i + 1] - part[q]->rp[i];
+    C.rp.assign(nrows + 1, 0);
+    for (int i = 0; i < nrows; ++i) C.rp[i + 1] = C.rp[i] + cnt[i];
+    C.ci.resize(C.rp[nrows]);
+    C.v.resize((size_t)C.rp[nrows] * bb);
+    for (size_t q = 0; q < part.size(); ++q) {
+        const amg::BCsr &P = *part[q];
+        for (int i = 0; i < P.n; ++i) {
+            const int r = row_off[q] + i;
+            for (int k = P.rp[i]; k < P.rp[i + 1]; ++k) {
+                const int d = C.rp[r] + (k - P.rp[i]);
+                C.ci[d] = P.ci[k] + col_off[q];
+                std::copy(P.v.begin() + (size_t)k * bb, P.v.begin() + (size_t)(k + 1) * bb, C.v.begin() + (size_t)d * bb);
+            }
+        }
+    }
+    return C;
+}
+
 // Builds the multigrid hierarchy from the viscosity matrix at the current positions: an
 // element pass (no stream advance) and an assembly on the device, the matrix read back,
-// the set-up on the host (amg::build), the levels uploaded. The hierarchy is kept for
-// the rest of the run; each step's CG still applies that step's own matrix and diagonal
-// on level 0.
+// the set-up on the host (amg::build; an ensemble's blobs each their own, on up to 16
+// threads), the levels uploaded. An ensemble's levels hold its blobs' levels side by side:
+// a blob with fewer levels repeats its coarsest one behind identity transfers; each
+// blob's smoother weights are folded into its rows of the stored diagonal (per row on
+// level 0); each blob's coarsest level has its own dense inverse. The hierarchy is kept
+// for the rest of the run; each step's CG still applies that step's own matrix and
+// diagonal on level 0.
 void ffea_gpu_blob::mg_setup() {
-    if (nblob != 1) throw Invalid("the multigrid preconditioner takes single-blob handles");
-    const BlobMeta &m = meta[0];
-    if (m.all_pinned) throw Invalid("multigrid preconditioner: every node is pinned, nothing is solved");
+    for (const BlobMeta &m : meta)
+        if (m.all_pinned) throw Invalid("multigrid preconditioner: a blob has every node pinned, nothing is solved");
     if (fused_cg)
         throw Invalid("multigrid preconditioner: a blob of up to 1024 corner nodes solves in one launch; "
                       "the preconditioner is for larger blobs");
@@ -1833,7 +1917,6 @@ void ffea_gpu_blob::mg_setup() {
     launch_assemble();
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);  // pass's inversion count
     FFEA_HIP_TRY(hipGetLastError());
-    const int nl = m.nl;
     std::vector<int> h_sw(nslices), h_col(nslots);
     std::vector<long long> h_sb(nslices);
     std::vector<unsigned char> h_pin(nrows);
@@ -1853,51 +1936,120 @@ void ffea_gpu_blob::mg_setup() {
         FFEA_HIP_TRY(hipMemcpyAsync(h_val.data(), d_val.p, sizeof(double) * h_val.size(), hipMemcpyDeviceToHost, stream));
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
-    amg::BCsr fine;
-    fine.n = fine.ncols = nl;
-    fine.br = fine.bc = 3;
-    fine.rp.assign(nl + 1, 0);
-    for (int n = 0; n < nl; ++n) {
-        const int s = n / kSlice, lane = n % kSlice;
-        std::vector<std::pair<int, int>> ent;  // (column, slot j)
-        for (int j = 0; j < h_sw[s]; ++j) {
-            const int c = h_col[h_sb[s] + (long long)j * kSlice + lane];
-            bool nz = false;
-            for (int k = 0; k < 9; ++k)
-                nz |= h_val[mixed ? val_index<float>(h_sb[s], j, lane, k) : val_index<double>(h_sb[s], j, lane, k)] != 0.0;
-            if (!nz) continue;  // slice padding
-            if (c < 0 || c >= nl) throw Invalid("internal error: matrix column outside the blob");
-            ent.push_back({c, j});
+    // blob b's corner-node matrix in block CSR over its own rows
+    auto fine_of = [&](int b) {
+        const BlobMeta &m = meta[b];
+        const int nl = m.nl, ro = m.row_off;
+        amg::BCsr fine;
+        fine.n = fine.ncols = nl;
+        fine.br = fine.bc = 3;
+        fine.rp.assign(nl + 1, 0);
+        std::vector<std::pair<int, int>> ent;  // (local column, slot j)
+        for (int n = 0; n < nl; ++n) {
+            const int g = ro + n, s = g / kSlice, lane = g % kSlice;
+            ent.clear();
+            for (int j = 0; j < h_sw[s]; ++j) {
+                const int c = h_col[h_sb[s] + (long long)j * kSlice + lane];
+                bool nz = false;
+                for (int k = 0; k < 9; ++k)
+                    nz |= h_val[mixed ? val_index<float>(h_sb[s], j, lane, k) : val_index<double>(h_sb[s], j, lane, k)] !=
+                          0.0;
+                if (!nz) continue;  // slice padding
+                if (c < ro || c >= ro + nl) throw Invalid("internal error: matrix column outside the blob");
+                ent.push_back({c - ro, j});
+            }
+            std::sort(ent.begin(), ent.end());
+            for (size_t t = 1; t < ent.size(); ++t)
+                if (ent[t].first == ent[t - 1].first) throw Invalid("internal error: repeated matrix column");
+            fine.rp[n + 1] = fine.rp[n] + (int)ent.size();
+            for (auto &e : ent) {
+                fine.ci.push_back(e.first);
+                for (int k = 0; k < 9; ++k)
+                    fine.v.push_back(h_val[mixed ? val_index<float>(h_sb[s], e.second, lane, k)
+                                                 : val_index<double>(h_sb[s], e.second, lane, k)]);
+            }
         }
-        std::sort(ent.begin(), ent.end());
-        for (size_t t = 1; t < ent.size(); ++t)
-            if (ent[t].first == ent[t - 1].first) throw Invalid("internal error: repeated matrix column");
-        fine.rp[n + 1] = fine.rp[n] + (int)ent.size();
-        for (auto &e : ent) {
-            fine.ci.push_back(e.first);
-            for (int k = 0; k < 9; ++k)
-                fine.v.push_back(h_val[mixed ? val_index<float>(h_sb[s], e.second, lane, k)
-                                             : val_index<double>(h_sb[s], e.second, lane, k)]);
-        }
+        return fine;
+    };
+    const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<amg::Hierarchy> H(nblob);
+    double hscale_max = 0.0;  // the FP16 copy of the whole level-0 matrix shares one scale
+    {
+        std::vector<std::string> errors(nblob);
+        std::vector<double> vmax(nblob, 0.0);
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int b = next++; b < nblob; b = next++) {
+                try {
+                    const amg::BCsr fine = fine_of(b);
+                    vmax[b] = max_abs(fine.v);
+                    amg::Options opt;
+                    opt.threads = nblob == 1 ? hw : 1;
+                    // an ensemble's blobs coarsen at least once, so that every blob has a coarse
+                    // level, and never to fewer than 4 aggregates (a one-aggregate level of a free
+                    // blob is pure null space: with the FP32 matrix its rounding is not positive)
+                    if (nblob > 1) {
+                        opt.max_coarse = std::min(opt.max_coarse, std::max(6, 3 * meta[b].nl / 8));
+                        opt.min_aggregates = 4;
+                    }
+                    const int ro = meta[b].row_off;
+                    H[b] = amg::build(fine, h_pos.data() + 3 * (size_t)ro, h_pin.data() + ro, meta[b].free_blob, opt);
+                } catch (const std::runtime_error &e) {
+                    errors[b] = (nblob > 1 ? fmt("blob %d: ", b) : std::string()) + e.what();
+                } catch (const Invalid &e) {
+                    errors[b] = e.msg;
+                } catch (const std::bad_alloc &) {
+                    errors[b] = "host allocation failed";
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < std::min(hw, nblob); ++t) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        for (const auto &e : errors)
+            if (!e.empty()) throw Invalid(e);
+        for (double v : vmax) hscale_max = std::max(hscale_max, v);
     }
     h_val.clear();
     h_val.shrink_to_fit();
-    amg::Options opt;
-    opt.threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-    amg::Hierarchy H;
-    try {
-        H = amg::build(fine, h_pos.data(), h_pin.data(), m.free_blob, opt);
-    } catch (const std::runtime_error &e) {
-        throw Invalid(e.what());
-    }
-    const int L = (int)H.lv.size();
+    int L = 0;
+    for (const auto &h : H) L = std::max(L, (int)h.lv.size());
+    for (int b = 0; b < nblob; ++b)
+        if ((int)H[b].lv.size() < 2 && L > 1)
+            throw Invalid(fmt("multigrid preconditioner: blob %d does not coarsen", b));
     if (L < 2 || L > 8) throw Invalid(fmt("multigrid preconditioner: %d levels (needs 2 to 8)", L));
+    for (auto &h : H)  // pad shallower hierarchies: the coarsest level repeated behind identity transfers
+        while ((int)h.lv.size() < L) {
+            amg::Level &last = h.lv.back();
+            const int n = last.A.n;
+            amg::Level copy;
+            copy.A = last.A;
+            copy.dinv = last.dinv;
+            copy.lmax = last.lmax;
+            copy.omega = last.omega;
+            last.P = identity6(n);
+            last.R = identity6(n);
+            h.lv.push_back(std::move(copy));
+        }
+    // row offsets of each blob on each level: level 0 is the handle's own rows
+    std::vector<std::vector<int>> off(L, std::vector<int>(nblob, 0));
+    std::vector<int> nlev(L, 0);
+    nlev[0] = nrows;
+    for (int b = 0; b < nblob; ++b) off[0][b] = meta[b].row_off;
+    for (int l = 1; l < L; ++l)
+        for (int b = 0; b < nblob; ++b) {
+            off[l][b] = nlev[l];
+            nlev[l] += H[b].lv[l].A.n;
+        }
     // upload
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     mg.clear();
     mg_cinv.reset();
     mg_hval.reset();
-    for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef}) bf->reset();
+    for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef, &mg_omega0}) bf->reset();
+    for (DevBuf<int> *bf : {&mg_crow_blob, &mg_coff_row, &mg_cm, &mg_free}) bf->reset();
+    mg_ccoff.reset();
     auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
         const double sc = fp16_scale(max_abs(A.v));
         const EllHost E = to_ell(A, sc);
@@ -1932,60 +2084,119 @@ void ffea_gpu_blob::mg_setup() {
         D.val.alloc(v.size(), &dev_bytes); D.val.upload(v, stream);
         FFEA_HIP_TRY(hipStreamSynchronize(stream));  // v goes out of scope
     };
+    // the blob whose smoother weights the info reports: the largest
+    int big = 0, total_nl = 0;
+    for (const BlobMeta &m : meta) total_nl += m.nl;
+    for (int b = 1; b < nblob; ++b)
+        if (meta[b].nl > meta[big].nl) big = b;
     mg_info = ffea_gpu_multigrid_info{};
     mg_info.num_levels = L;
     for (int l = 0; l < L; ++l) {
-        const amg::Level &hl = H.lv[l];
         auto lev = std::make_unique<MgLevel>();
-        lev->n = hl.A.n;
-        lev->omega = hl.omega;
+        lev->n = nlev[l];
+        lev->omega = l == 0 ? H[0].lv[0].omega : 1.0;  // levels above 0: folded into dinv
+        long long blocks = 0;
+        for (int b = 0; b < nblob; ++b) blocks += H[b].lv[l].A.nnzb();
         if (l >= 1) {
-            if (hl.A.br != 6 || hl.A.bc != 6) throw Invalid("internal error: coarse blocks are not 6x6");
-            if (l < L - 1) up_csr(lev->A, hl.A);
-            const size_t nv = (size_t)hl.A.n * 6;
-            std::vector<double> dv(hl.dinv);
+            for (int b = 0; b < nblob; ++b)
+                if (H[b].lv[l].A.br != 6 || H[b].lv[l].A.bc != 6)
+                    throw Invalid("internal error: coarse blocks are not 6x6");
+            std::vector<const amg::BCsr *> parts;
+            for (int b = 0; b < nblob; ++b) parts.push_back(&H[b].lv[l].A);
+            if (l < L - 1) up_csr(lev->A, side_by_side(parts, off[l], off[l], nlev[l], nlev[l]));
+            const size_t nv = (size_t)nlev[l] * 6;
+            std::vector<double> dv(nv, 0.0);
+            for (int b = 0; b < nblob; ++b) {
+                const amg::Level &hl = H[b].lv[l];
+                for (size_t i = 0; i < hl.dinv.size(); ++i) dv[(size_t)off[l][b] * 6 + i] = hl.omega * hl.dinv[i];
+            }
             lev->dinv.alloc(nv, &dev_bytes);
             lev->dinv.upload(dv, stream);
+            FFEA_HIP_TRY(hipStreamSynchronize(stream));
             for (DevBuf<double> *bf : {&lev->b, &lev->xa, &lev->res, &lev->xb, &lev->e}) {
                 bf->alloc(nv, &dev_bytes);
                 bf->zero(stream);
             }
         }
         if (l < L - 1) {
-            const amg::BCsr &P = hl.P, &R = hl.R;
-            if (P.n != hl.A.n || P.ncols != H.lv[l + 1].A.n || R.n != P.ncols || R.ncols != P.n)
-                throw Invalid("internal error: multigrid transfer shapes");
-            up_ell(lev->P, P);
-            up_csr(lev->R, R);
-            mg_info.transfer_blocks[l] = P.nnzb();
+            std::vector<const amg::BCsr *> pp, rr;
+            long long tb = 0;
+            for (int b = 0; b < nblob; ++b) {
+                const amg::Level &hl = H[b].lv[l];
+                if (hl.P.n != hl.A.n || hl.P.ncols != H[b].lv[l + 1].A.n || hl.R.n != hl.P.ncols ||
+                    hl.R.ncols != hl.P.n)
+                    throw Invalid("internal error: multigrid transfer shapes");
+                pp.push_back(&hl.P);
+                rr.push_back(&hl.R);
+                tb += hl.P.nnzb();
+            }
+            up_ell(lev->P, side_by_side(pp, off[l], off[l + 1], nlev[l], nlev[l + 1]));
+            up_csr(lev->R, side_by_side(rr, off[l + 1], off[l], nlev[l + 1], nlev[l]));
+            mg_info.transfer_blocks[l] = tb;
         }
-        mg_info.rows[l] = hl.A.n;
-        mg_info.block[l] = hl.A.br;
-        mg_info.blocks[l] = hl.A.nnzb();
-        mg_info.omega[l] = hl.omega;
-        mg_info.lmax[l] = hl.lmax;
+        mg_info.rows[l] = l == 0 ? total_nl : nlev[l];
+        mg_info.block[l] = l == 0 ? 3 : 6;
+        mg_info.blocks[l] = blocks;
+        mg_info.omega[l] = H[big].lv[l].omega;
+        mg_info.lmax[l] = H[big].lv[l].lmax;
         mg.push_back(std::move(lev));
     }
-    mg_nc = H.nc;
-    if (mg_nc != H.lv[L - 1].A.n * 6 || (long long)H.cinv.size() != (long long)mg_nc * mg_nc)
-        throw Invalid("internal error: coarsest inverse size");
-    mg_info.coarsest_unknowns = mg_nc;
+    // the coarsest level: each blob's dense inverse
+    mg_nc = nlev[L - 1] * 6;
     {
-        std::vector<float> ci(H.cinv.begin(), H.cinv.end());
+        std::vector<float> ci;
+        std::vector<int> row_blob, coff_row(nblob), cm(nblob);
+        std::vector<long long> ccoff(nblob);
+        for (int b = 0; b < nblob; ++b) {
+            const int m = H[b].nc;
+            if (m != H[b].lv[L - 1].A.n * 6 || (long long)H[b].cinv.size() != (long long)m * m)
+                throw Invalid("internal error: coarsest inverse size");
+            coff_row[b] = off[L - 1][b] * 6;
+            cm[b] = m;
+            ccoff[b] = (long long)ci.size();
+            ci.insert(ci.end(), H[b].cinv.begin(), H[b].cinv.end());
+            row_blob.insert(row_blob.end(), m, b);
+        }
         mg_cinv.alloc(ci.size(), &dev_bytes);
         mg_cinv.upload(ci, stream);
+        if (nblob > 1) {
+            mg_crow_blob.alloc(row_blob.size(), &dev_bytes); mg_crow_blob.upload(row_blob, stream);
+            mg_coff_row.alloc(nblob, &dev_bytes); mg_coff_row.upload(coff_row, stream);
+            mg_cm.alloc(nblob, &dev_bytes); mg_cm.upload(cm, stream);
+            mg_ccoff.alloc(nblob, &dev_bytes); mg_ccoff.upload(ccoff, stream);
+        }
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
-    mg_hscale = fp16_scale(max_abs(fine.v));
+    mg_info.coarsest_unknowns = mg_nc;
+    // level 0: the FP16 copy of K (the assembly writes it), per-row smoother weights of an
+    // ensemble, the rigid-mode gauge of free blobs
+    mg_hscale = fp16_scale(hscale_max);
     mg_hval.alloc((size_t)nslots * 9, &dev_bytes);
     launch_assemble();  // fills the FP16 copy at the current positions
+    if (nblob > 1) {
+        std::vector<double> om(nrows, 0.0);
+        for (int b = 0; b < nblob; ++b)
+            for (int n = 0; n < meta[b].nl; ++n) om[meta[b].row_off + n] = H[b].lv[0].omega;
+        mg_omega0.alloc(nrows, &dev_bytes);
+        mg_omega0.upload(om, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    mg_any_free = false;
+    {
+        std::vector<int> fr(nblob, 0);
+        for (int b = 0; b < nblob; ++b) {
+            fr[b] = meta[b].free_blob ? 1 : 0;
+            mg_any_free |= meta[b].free_blob;
+        }
+        mg_free.alloc(nblob, &dev_bytes);
+        mg_free.upload(fr, stream);
+    }
     const size_t vlen = (size_t)nslices * 3 * kSlice;
     mg_res0.alloc(vlen, &dev_bytes); mg_res0.zero(stream);
     mg_xb0.alloc(vlen, &dev_bytes); mg_xb0.zero(stream);
     mg_rp.alloc((size_t)27 * std::max(1, nchunks), &dev_bytes); mg_rp.zero(stream);
-    mg_coef.alloc(8, &dev_bytes); mg_coef.zero(stream);
+    mg_coef.alloc((size_t)6 * nblob, &dev_bytes); mg_coef.zero(stream);
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
-    if (mg[0]->n != nl) throw Invalid("internal error: multigrid level 0 rows");
     mg_info.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1996,49 +2207,57 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     auto grid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock - 1) / kMgBlock)); };
     MgLevel &f = *mg[0];
     const double hms = 1.0 / mg_hscale;
+    // coarse kernels stop with blob 0's solve, an ensemble's with every blob's
+    const MgStop S0{nblob == 1 ? (const CgScalars *)S.p : nullptr, num_done.p, active_blobs};
+    const bool ens = nblob > 1;  // an ensemble's mostly 256-row chunks: 256-thread level-0 workgroups
+    const double *om0 = nblob == 1 ? nullptr : (const double *)mg_omega0.p;
     MgFineArgs<_Float16> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, z.p, r.p,
-                            dinv.p, f.omega, mg_res0.p, nullptr};
-    hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+                            dinv.p, f.omega, mg_res0.p, nullptr, om0};
+    if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fa);
+    else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
     auto wgrid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock / 64 - 1) / (kMgBlock / 64))); };
     for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
         const bool coarsest = l + 1 == L - 1;
-        MgCsrArgs ra{S.p, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, cur.R.mscale, l == 0 ? mg_res0.p : cur.res.p,
+        MgCsrArgs ra{S0, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, cur.R.mscale, l == 0 ? mg_res0.p : cur.res.p,
                      nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p};
         if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         if (!coarsest) {
-            MgCsrArgs rs{S.p, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.A.mscale, nx.xa.p, nx.b.p, nullptr, 0.0,
+            MgCsrArgs rs{S0, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.A.mscale, nx.xa.p, nx.b.p, nullptr, 0.0,
                          nx.res.p, nullptr};
             hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESID>), wgrid(nx.n), dim3(kMgBlock), 0, stream, rs);
         }
     }
     MgLevel &c = *mg[L - 1];
-    hipLaunchKernelGGL(k_mg_dense, wgrid(mg_nc), dim3(kMgBlock), 0, stream, (const CgScalars *)S.p, mg_nc,
-                       (const float *)mg_cinv.p, (const double *)c.b.p, c.e.p);
+    MgDenseArgs da{S0, mg_nc, mg_cinv.p, c.b.p, c.e.p, nblob == 1 ? nullptr : (const int *)mg_crow_blob.p,
+                   nblob == 1 ? nullptr : (const int *)mg_coff_row.p, nblob == 1 ? nullptr : (const int *)mg_cm.p,
+                   nblob == 1 ? nullptr : (const long long *)mg_ccoff.p};
+    hipLaunchKernelGGL(k_mg_dense, wgrid(mg_nc), dim3(kMgBlock), 0, stream, da);
     for (int l = L - 2; l >= 1; --l) {  // up: prolong, post-smooth
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
-        MgArgs pa{S.p, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, cur.P.mscale, nx.e.p, nullptr,
+        MgArgs pa{S0, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, cur.P.mscale, nx.e.p, nullptr,
                   cur.xa.p, nullptr, 0.0, 1.0, cur.xb.p, nullptr};
         hipLaunchKernelGGL((k_mg_apply<6, 6, MG_PROLONG>), grid(cur.n), dim3(kMgBlock), 0, stream, pa);
-        MgCsrArgs po{S.p, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.A.mscale, cur.xb.p, cur.b.p, cur.dinv.p,
+        MgCsrArgs po{S0, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.A.mscale, cur.xb.p, cur.b.p, cur.dinv.p,
                      cur.omega, cur.e.p, nullptr};
         hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(cur.n), dim3(kMgBlock), 0, stream, po);
     }
-    MgArgs p0{S.p, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, mg[1]->e.p, nullptr, z.p, nullptr,
-              0.0, f.omega, mg_xb0.p, nullptr};
+    MgArgs p0{S0, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, mg[1]->e.p, nullptr, z.p, nullptr,
+              0.0, f.omega, mg_xb0.p, nullptr, om0};
     hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
     MgFineArgs<_Float16> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, mg_xb0.p, r.p,
-                            dinv.p, f.omega, z.p, Pout};
-    hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+                            dinv.p, f.omega, z.p, Pout, om0};
+    if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fp);
+    else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
     FFEA_HIP_TRY(hipGetLastError());
 }
 
-// the free blob's solution in the Jacobi-PCG gauge, Z^T D x = 0, before the update
+// free blobs' solutions in the Jacobi-PCG gauge, Z^T D x = 0, before the update
 void ffea_gpu_blob::launch_mg_gauge() {
-    MgRigidArgs ra{S.p, chunks.p, nchunks, x.p, dinv.p, pos.p, centre.p, mg_rp.p, mg_coef.p, x.p};
+    MgRigidArgs ra{S.p, chunks.p, nchunks, x.p, dinv.p, pos.p, centre.p, mg_rp.p, mg_coef.p, x.p, seg.p, mg_free.p};
     hipLaunchKernelGGL(k_mg_rigid_sums, dim3(nchunks), dim3(kMgBlock), 0, stream, ra);
-    hipLaunchKernelGGL(k_mg_rigid_coef, dim3(1), dim3(kMgBlock), 0, stream, ra);
+    hipLaunchKernelGGL(k_mg_rigid_coef, dim3(nblob), dim3(kMgBlock), 0, stream, ra);
     hipLaunchKernelGGL(k_mg_rigid_fix, dim3(nchunks), dim3(kRowBlock), 0, stream, ra);
     FFEA_HIP_TRY(hipGetLastError());
 }
@@ -2425,7 +2644,7 @@ int ffea_gpu_blob::solver_solve(const double *pos_nodes, double *xio) {
     rhs_in = nullptr;
     run_cg_eager(0);
     if (refine_tol > 0.0) refine_solution();
-    if (mg_on() && m.free_blob) launch_mg_gauge();  // Jacobi-PCG's rigid-mode gauge
+    if (mg_on() && mg_any_free) launch_mg_gauge();  // Jacobi-PCG's rigid-mode gauge
     hipLaunchKernelGGL(k_mark_updated, dim3(grid_blobs), dim3(kBlock), 0, stream, nblob, S.p);
     FFEA_HIP_TRY(hipGetLastError());
     FFEA_HIP_TRY(hipMemcpyAsync(h_vec.data(), x.p, sizeof(double) * h_vec.size(), hipMemcpyDeviceToHost, stream));
@@ -3436,11 +3655,57 @@ int ffea_gpu_blob_autotune(ffea_gpu_blob *b, int iterations, int *chosen_blocks)
             if (e1) (void)hipEventDestroy(e1);
             throw;
         }
+        b->set_row_blocks(best_blocks);
+        // then the SpMV's row unroll and the assembly's lanes per row at that grid
+        try {
+            double best_u = 1e300;
+            int unroll = b->spmv_unroll;
+            for (int u : {kMaxUnrolledRow, kMaxUnrolledRow / 2}) {  // the SpMV step alone, twice as often
+                b->spmv_unroll = u;
+                b->S.upload(run, b->stream);
+                for (int it = 1; it <= 2; ++it) b->launch_cg_iteration(it, true);
+                FFEA_HIP_TRY(hipEventRecord(e0, b->stream));
+                for (int it = 3; it < 3 + 2 * iterations; ++it) b->launch_cg_iteration(it, true);
+                FFEA_HIP_TRY(hipEventRecord(e1, b->stream));
+                FFEA_HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                FFEA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_u) {
+                    best_u = ms;
+                    unroll = u;
+                }
+            }
+            b->spmv_unroll = unroll;
+            // the assembly rewrites the matrix of the last element pass, the same values each time
+            double best_a = 1e300;
+            int lanes = b->asm_lanes();
+            for (int l : {1, 4, 16}) {
+                b->asm_lanes_set = l;
+                b->launch_assemble();
+                FFEA_HIP_TRY(hipEventRecord(e0, b->stream));
+                for (int r = 0; r < 3; ++r) b->launch_assemble();
+                FFEA_HIP_TRY(hipEventRecord(e1, b->stream));
+                FFEA_HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                FFEA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best_a) {
+                    best_a = ms;
+                    lanes = l;
+                }
+            }
+            b->asm_lanes_set = lanes;
+        } catch (...) {
+            b->prm = keep;
+            b->timing = keep_timing;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         b->prm = keep;
         b->timing = keep_timing;
-        b->set_row_blocks(best_blocks);
+        b->drop_graph();  // a captured step holds the old launch shape
         // the CG vectors are rebuilt by the next gather; restore the solver scalars
         std::vector<CgScalars> s0(b->nblob);
         for (int i = 0; i < b->nblob; ++i) {
@@ -3504,6 +3769,31 @@ int ffea_gpu_blob_get_refinement(ffea_gpu_blob *b, int *last_sweeps, long long *
     return FFEA_GPU_OK;
 }
 
+int ffea_gpu_blob_get_launch_shape(ffea_gpu_blob *b, int *row_blocks, int *spmv_unroll, int *assembly_lanes) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    if (row_blocks) *row_blocks = b->nb;
+    if (spmv_unroll) *spmv_unroll = b->spmv_unroll;
+    if (assembly_lanes) *assembly_lanes = b->asm_lanes();
+    return FFEA_GPU_OK;
+}
+
+int ffea_gpu_blob_set_launch_shape(ffea_gpu_blob *b, int row_blocks, int spmv_unroll, int assembly_lanes) {
+    if (!b) return FFEA_GPU_ERR_INVALID;
+    if ((spmv_unroll && spmv_unroll != kMaxUnrolledRow && spmv_unroll != kMaxUnrolledRow / 2) ||
+        (assembly_lanes && assembly_lanes != 1 && assembly_lanes != 4 && assembly_lanes != 16) || row_blocks < 0) {
+        b->err = "ffea_gpu_blob_set_launch_shape: row_blocks >= 0, spmv_unroll 16 or 8, assembly_lanes 1, 4 or 16 "
+                 "(0: unchanged)";
+        return FFEA_GPU_ERR_INVALID;
+    }
+    return guarded(b, [&]() {
+        if (row_blocks) b->set_row_blocks(row_blocks);
+        if (spmv_unroll) b->spmv_unroll = spmv_unroll;
+        if (assembly_lanes) b->asm_lanes_set = assembly_lanes;
+        b->drop_graph();
+        return (int)FFEA_GPU_OK;
+    });
+}
+
 int ffea_gpu_blob_use_graph(ffea_gpu_blob *b, int enable, int iterations) {
     if (!b) return FFEA_GPU_ERR_INVALID;
     b->use_graph = enable != 0;
@@ -3528,7 +3818,10 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->mg.clear();
             b->mg_cinv.reset();
             b->mg_hval.reset();
-            for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef}) bf->reset();
+            for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef, &b->mg_omega0}) bf->reset();
+            for (auto *bf : {&b->mg_crow_blob, &b->mg_coff_row, &b->mg_cm, &b->mg_free}) bf->reset();
+            b->mg_ccoff.reset();
+            b->mg_any_free = false;
             b->mg_info = ffea_gpu_multigrid_info{};
             return (int)FFEA_GPU_OK;
         }

@@ -551,7 +551,12 @@ __global__ __launch_bounds__(kBlock) void k_geo_material(int ne, const T *A, con
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
 #pragma clang fp contract(off)
-    const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+    // XCD-aware order: workgroups reach the 8 XCDs round-robin, so workgroup b takes the
+    // (b % 8)-th eighth of the rows; the rows an XCD works on at once are then neighbours
+    // in the mesh, whose shape records its L2 shares
+    const int nbk = gridDim.x, bk = blockIdx.x;
+    const int lb = (nbk % 8 == 0) ? (bk % 8) * (nbk / 8) + bk / 8 : bk;
+    const long long t = (long long)lb * kBlock + threadIdx.x;
     const int n = (int)(t / L), s = (int)(t % L);
     if (n >= a.nrows) return;
     const int sl = n / kSlice, lane = n % kSlice;
@@ -1102,9 +1107,10 @@ __device__ __forceinline__ void load_block<float>(const float *slot, int lane, f
     w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
 }
 
-template <typename T, bool PROJ>
+// U: rows with up to U blocks issue all their block loads together (the unrolled path);
+// longer rows loop. Either way every row sums its blocks in slot order.
+template <typename T, bool PROJ, int BS = row_block<T>, int U = kMaxUnrolledRow>
 __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange cr) {
-    constexpr int BS = row_block<T>;
     const int i = a.iter;
     const bool first = (i == 0);
     int cur = -1;
@@ -1188,15 +1194,15 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
                 acc1 += (double)w[3] * z0; acc1 += (double)w[4] * z1; acc1 += (double)w[5] * z2;
                 acc2 += (double)w[6] * z0; acc2 += (double)w[7] * z1; acc2 += (double)w[8] * z2;
             };
-            if (W <= kMaxUnrolledRow) {
+            if (W <= U) {
                 // all block columns of the row first, so the gathers and value loads of every
                 // block can be in flight together
-                int cc[kMaxUnrolledRow];
+                int cc[U];
 #pragma unroll
-                for (int j = 0; j < kMaxUnrolledRow; ++j)
+                for (int j = 0; j < U; ++j)
                     if (j < W) cc[j] = __builtin_nontemporal_load(colp + (size_t)j * kSlice);
 #pragma unroll
-                for (int j = 0; j < kMaxUnrolledRow; ++j)
+                for (int j = 0; j < U; ++j)
                     if (j < W) block(j, cc[j]);
             } else {
 #pragma unroll 2
@@ -1234,16 +1240,28 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
     }
 }
 
-template <typename T, bool PROJ>
-__global__ __launch_bounds__(row_block<T>) void k_cg_spmv(SpmvArgs<T> a) {
-    cg_spmv<T, PROJ>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
+// BS: threads per workgroup; the FP64 form takes kRowBlock, or kMixedBlock when no chunk
+// has more than 256 rows (either way each thread owns at most one row of a chunk, so the
+// sums are the same)
+template <typename T, bool PROJ, int BS = row_block<T>, int U = kMaxUnrolledRow>
+__global__ __launch_bounds__(BS) void k_cg_spmv(SpmvArgs<T> a) {
+    cg_spmv<T, PROJ, BS, U>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
+
 // the projected FP32 step is held to 128 VGPRs (4 waves per SIMD): one 1024-row chunk
 // per workgroup then fits the 6M-tet blob's 1007 chunks on the 256 CUs in one round
+#ifndef FFEA_SPMV_PROJ_WAVES
+#define FFEA_SPMV_PROJ_WAVES 4
+#endif
 template <>
-__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cg_spmv<float, true>(
-    SpmvArgs<float> a) {
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(FFEA_SPMV_PROJ_WAVES))) void
+k_cg_spmv<float, true, kMixedBlock, kMaxUnrolledRow>(SpmvArgs<float> a) {
     cg_spmv<float, true>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
+}
+template <>
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(FFEA_SPMV_PROJ_WAVES))) void
+k_cg_spmv<float, true, kMixedBlock, kMaxUnrolledRow / 2>(SpmvArgs<float> a) {
+    cg_spmv<float, true, kMixedBlock, kMaxUnrolledRow / 2>(a, my_chunks(a.ra.nchunks, a.ra.cpb));
 }
 
 // out = V in over all rows (tiled vectors), one lane per row, blocks in slot order:

@@ -385,15 +385,22 @@ int ffea_gpu_blob_adjoint(ffea_gpu_blob *blob, const double *dL_dpos, double *dL
 int ffea_gpu_blob_set_refinement(ffea_gpu_blob *blob, double tolerance);
 int ffea_gpu_blob_get_refinement(ffea_gpu_blob *blob, int *last_sweeps, long long *total_sweeps);
 
-/* Launch shape of the row kernels. Totals are summed per reduction chunk in a fixed
- * order, so the shape never changes results. ffea_gpu_blob_set_row_blocks sets the
- * number of workgroups (rounded to whole chunks per workgroup and a multiple of 8, one
- * share per XCD). ffea_gpu_blob_autotune times `iterations` CG iterations of the current
- * matrix at each candidate shape, keeps the fastest and returns it in *chosen (may be
- * NULL); it leaves positions, velocities and noise streams untouched. No counterpart in
- * the reference. */
+/* Launch shape. Totals are summed per reduction chunk in a fixed order and every row and
+ * block entry in a fixed order, so the shape never changes results. The knobs: the number
+ * of workgroups of the row kernels (rounded to whole chunks per workgroup and a multiple
+ * of 8, one share per XCD; ffea_gpu_blob_set_row_blocks), the SpMV's row unroll (rows of
+ * up to 16 or up to 8 blocks issue all their block loads together) and the assembly's
+ * lanes per row (1, 4 or 16). ffea_gpu_blob_autotune times `iterations` CG iterations of
+ * the current matrix at each candidate workgroup count, then both unrolls at the fastest
+ * count, then three assemblies at each lane count, keeps the fastest of each and returns
+ * the workgroup count in *chosen (may be NULL); it leaves positions, velocities and noise
+ * streams untouched. ffea_gpu_blob_get_launch_shape reads the three knobs;
+ * ffea_gpu_blob_set_launch_shape sets them (0: unchanged). No counterpart in the
+ * reference. */
 int ffea_gpu_blob_set_row_blocks(ffea_gpu_blob *blob, int blocks);
 int ffea_gpu_blob_autotune(ffea_gpu_blob *blob, int iterations, int *chosen);
+int ffea_gpu_blob_get_launch_shape(ffea_gpu_blob *blob, int *row_blocks, int *spmv_unroll, int *assembly_lanes);
+int ffea_gpu_blob_set_launch_shape(ffea_gpu_blob *blob, int row_blocks, int spmv_unroll, int assembly_lanes);
 
 /* enable != 0: ffea_gpu_blob_step runs each step as one HIP graph launch (element pass,
  * stream advance, assembly, gather, CG iterations, update), continuing eagerly only if a

@@ -126,12 +126,35 @@ def test_row_grid_and_autotune_are_result_neutral():
             g.set_row_blocks(blocks)
             g.step(3)
             assert np.array_equal(g.positions(), want), blocks
+    for unroll, lanes in ((8, 1), (16, 4), (8, 16)):  # SpMV row unroll, assembly lanes per row
+        with GpuBlob.from_spec(spec) as g:
+            g.set_launch_shape(spmv_unroll=unroll, assembly_lanes=lanes)
+            assert g.launch_shape()["spmv_unroll"] == unroll and g.launch_shape()["assembly_lanes"] == lanes
+            g.step(3)
+            assert np.array_equal(g.positions(), want), (unroll, lanes)
     with GpuBlob.from_spec(spec) as g:
         g.step(1)
         chosen = g.autotune(iterations=10)
-        assert chosen == g.info()["row_blocks"] and chosen >= 1
+        shape = g.launch_shape()
+        print(f"[autotune] {spec.name}: {shape}", flush=True)
+        assert chosen == g.info()["row_blocks"] == shape["row_blocks"] and chosen >= 1
+        assert shape["spmv_unroll"] in (8, 16) and shape["assembly_lanes"] in (1, 4, 16)
         g.step(2)
         assert np.array_equal(g.positions(), want)
+
+
+def test_launch_shape_is_result_neutral_in_projected_mixed_form():
+    """the mixed form of a free blob projects rigid modes out of every SpMV step: either row
+    unroll and every assembly lane count give the same positions bit for bit"""
+    spec = workloads.cube_blob(24, mesh_seed=5, rng_seed=9)
+    got = {}
+    for unroll, lanes in ((16, 1), (8, 1), (16, 4), (8, 16)):
+        with GpuBlob.from_spec(spec, precision="mixed") as g:
+            g.set_launch_shape(spmv_unroll=unroll, assembly_lanes=lanes)
+            g.step(3)
+            got[unroll, lanes] = g.positions()
+    for key, pos in got.items():
+        assert np.array_equal(pos, got[16, 1]), key
 
 
 def test_mixed_batch_close_to_fp64():

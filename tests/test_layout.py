@@ -9,11 +9,17 @@ from ffea_ci_demo_amd import _native, workloads
 from ffea_ci_demo_amd.blob import FFEAException, check_layout
 
 
+def chunk_rows(nl):
+    """rows per reduction chunk: 1024 for blobs that solve in one workgroup (up to 1024
+    rows) and from 2^18 rows on, 256 between"""
+    return 1024 if nl <= 1024 or nl >= 1 << 18 else 256
+
+
 def expected_chunks(spec):
     nl = int(spec.mesh.is_linear.sum()) if spec.mesh is not None else None
     if spec.pinned is not None and len(spec.pinned) == spec.num_nodes:
         return 0
-    return -(-nl // 1024)
+    return -(-nl // chunk_rows(nl))
 
 
 def mixed_specs():
@@ -46,7 +52,7 @@ def test_single_large_blob_layout():
     spec = workloads.cube_blob(40)
     info = check_layout([spec])
     nl = 41 ** 3
-    assert info["num_linear_nodes"] == nl and info["num_chunks"] == -(-nl // 1024)
+    assert info["num_linear_nodes"] == nl and info["num_chunks"] == -(-nl // 256)
     # interior rows of a Kuhn-split cube mesh couple to 14 neighbours and themselves
     assert info["num_blocks"] <= 15 * nl
 
