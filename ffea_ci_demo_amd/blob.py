@@ -91,7 +91,8 @@ class _Handle:
         return {"row_blocks": v[0].value, "spmv_unroll": v[1].value, "assembly_lanes": v[2].value}
 
     def set_launch_shape(self, row_blocks=0, spmv_unroll=0, assembly_lanes=0):
-        """sets the launch-shape knobs (0: unchanged); results do not depend on them"""
+        """sets the launch-shape knobs (0: unchanged); results do not depend on them.
+        assembly_lanes: 1, 4 or 16 lanes per row, or -1 (shape records staged in LDS)"""
         self._check(_native.lib().ffea_gpu_blob_set_launch_shape(self.h, int(row_blocks), int(spmv_unroll),
                                                                  int(assembly_lanes)))
 

@@ -963,6 +963,76 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
     return t;
 }
 
+// Tables of the LDS-staged assembly (k_assemble_staged): rows in groups of G, each group's
+// elements (every element with a corner among its rows, from the rows' incidence lists)
+// ascending in gel[gptr[g] .. gptr[g+1]), and for each block source its element's place in
+// the list of its row's group, cloc = (place<<4)|(a<<2)|b (kLocalIdentity for an identity
+// source). Groups with more than cap elements keep cloc unset; the kernel reads their
+// records from memory. Built on up to 16 host threads.
+struct StageTables {
+    std::vector<uint32_t> gptr, gel;
+    std::vector<uint16_t> cloc;
+    int overflow = 0;
+};
+
+StageTables stage_tables(const Tables &t, int G, int cap) {
+    StageTables st;
+    if (t.nrows % G != 0) throw Invalid("internal layout error: rows are not whole assembly groups");
+    const int ngroups = t.nrows / G;
+    std::vector<std::vector<uint32_t>> lists(ngroups);
+    st.cloc.assign(t.contrib.size(), kLocalIdentity);
+    std::atomic<int> next{0}, bad{0}, over{0};
+    auto work = [&]() {
+        constexpr int kBatch = 256;
+        for (int g0 = (next += kBatch) - kBatch; g0 < ngroups; g0 = (next += kBatch) - kBatch) {
+            for (int g = g0; g < std::min(ngroups, g0 + kBatch); ++g) {
+                std::vector<uint32_t> &L = lists[g];
+                for (int n = g * G; n < (g + 1) * G; ++n)
+                    for (uint32_t k = t.ip[n]; k < t.ip[n + 1]; ++k) L.push_back(t.ic[k] >> 2);
+                std::sort(L.begin(), L.end());
+                L.erase(std::unique(L.begin(), L.end()), L.end());
+                if ((int)L.size() > cap) {
+                    ++over;
+                    continue;
+                }
+                for (int n = g * G; n < (g + 1) * G; ++n) {
+                    const int sl = n / kSlice, lane = n % kSlice;
+                    for (int j = 0; j < t.sw[sl]; ++j) {
+                        const long long slot = t.sb[sl] + (long long)j * kSlice + lane;
+                        for (uint32_t c = t.cnt[slot]; c < t.cnt[slot + 1]; ++c) {
+                            const uint32_t code = t.contrib[c];
+                            if (code == kIdentity) continue;
+                            const auto it = std::lower_bound(L.begin(), L.end(), code >> 4);
+                            if (it == L.end() || *it != (code >> 4)) {
+                                ++bad;
+                                continue;
+                            }
+                            st.cloc[c] = (uint16_t)(((uint32_t)(it - L.begin()) << 4) | (code & 15u));
+                        }
+                    }
+                }
+            }
+        }
+    };
+    const int nthreads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nthreads; ++k) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (bad) throw Invalid("internal layout error: a block source outside its row's element list");
+    st.overflow = over;
+    st.gptr.assign(ngroups + 1, 0);
+    for (int g = 0; g < ngroups; ++g) st.gptr[g + 1] = st.gptr[g] + (uint32_t)lists[g].size();
+    st.gel.reserve(st.gptr[ngroups]);
+    for (auto &L : lists) {
+        st.gel.insert(st.gel.end(), L.begin(), L.end());
+        std::vector<uint32_t>().swap(L);
+    }
+    for (uint32_t e : st.gel)
+        if ((long long)e >= t.ne) throw Invalid("internal layout error: assembly group element");
+    return st;
+}
+
 // Every index a kernel forms from the tables lies inside the buffer it addresses
 // (the kernels themselves do not check). Throws Invalid naming the first violation.
 void validate(const Tables &t) {
@@ -1133,6 +1203,8 @@ struct ffea_gpu_blob {
     DevBuf<int> conn, slice_w, col, elem_blob, elem_off, project;
     DevBuf<long long> slice_base;
     DevBuf<uint32_t> cptr, contrib, inc_ptr, inc, inc_pos, fold_ptr, rng;
+    DevBuf<uint32_t> st_ptr, st_el;     // LDS-staged assembly: group element lists
+    DevBuf<uint16_t> st_loc;            // ... and each source's place in its group's list
     DevBuf<unsigned char> diag_j, pinned;
     DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
@@ -1329,10 +1401,12 @@ struct ffea_gpu_blob {
     // assembly grid: a workgroup per slot and row block for small meshes, per row block
     // (its lanes walking their rows' slots) from 1024 row blocks on
     // lanes per row of the assembly: 1 once the rows alone fill 1024 workgroups, else 16
-    int asm_lanes() const { return asm_lanes_set ? asm_lanes_set : (grid_rows >= 1024 ? 1 : 16); }
+    int asm_lanes() const { return asm_lanes_set ? asm_lanes_set : (nrows > kFusedMaxRows ? kAsmStaged : 16); }
+    int st_groups = 0;                  // groups of the LDS-staged assembly
     // launch-shape knobs the autotuner sets (results do not depend on them)
-    int asm_lanes_set = 0;              // lanes per row of the assembly (1, 4 or 16); 0: by size
+    int asm_lanes_set = 0;              // lanes per row of the assembly (1, 4 or 16), kAsmStaged; 0: by size
     int spmv_unroll = kMaxUnrolledRow;  // blocks per row the SpMV issues together (16 or 8)
+    dim3 st_grid() const { return dim3((unsigned)((st_groups + 7) / 8 * 8)); }
     dim3 asm_grid() const {  // a multiple of 8 workgroups: each XCD takes one contiguous eighth of the rows
         long long g = ((long long)nrows * asm_lanes() + kBlock - 1) / kBlock;
         if (g >= 8 && g % 8) g += 8 - g % 8;
@@ -1457,6 +1531,15 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     col.alloc(t.col.size(), &dev_bytes); col.upload(t.col, stream);
     cptr.alloc(t.cnt.size(), &dev_bytes); cptr.upload(t.cnt, stream);
     contrib.alloc(t.contrib.size(), &dev_bytes); contrib.upload(t.contrib, stream);
+    {
+        const StageTables st = mixed ? stage_tables(t, kStageRows<float>, kStageCap<float>)
+                                     : stage_tables(t, kStageRows<double>, kStageCap<double>);
+        st_groups = (int)st.gptr.size() - 1;
+        st_ptr.alloc(st.gptr.size(), &dev_bytes); st_ptr.upload(st.gptr, stream);
+        st_el.alloc(st.gel.size(), &dev_bytes); st_el.upload(st.gel, stream);
+        st_loc.alloc(st.cloc.size(), &dev_bytes); st_loc.upload(st.cloc, stream);
+        FFEA_HIP_TRY(hipStreamSynchronize(stream));  // the host tables go out of scope
+    }
     diag_j.alloc(t.dj.size(), &dev_bytes); diag_j.upload(t.dj, stream);
     if (nblob > 1) {
         elem_blob.alloc(t.eblob.size(), &dev_bytes);
@@ -1706,13 +1789,18 @@ void ffea_gpu_blob::launch_assemble() {
     if (mixed) {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p,
                          f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
-        if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<float, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        if (asm_lanes() == kAsmStaged)
+            hipLaunchKernelGGL(k_assemble_staged<float>, st_grid(), dim3(kSlice), 0, stream, a, st_ptr.p, st_el.p, st_loc.p);
+        else if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<float, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
         else if (asm_lanes() == 4) hipLaunchKernelGGL((k_assemble<float, 4>), asm_grid(), dim3(kBlock), 0, stream, a);
         else hipLaunchKernelGGL((k_assemble<float, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p,
                           d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
-        if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<double, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
+        if (asm_lanes() == kAsmStaged)
+            hipLaunchKernelGGL(k_assemble_staged<double>, st_grid(), dim3(kSlice), 0, stream, a, st_ptr.p, st_el.p,
+                               st_loc.p);
+        else if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<double, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
         else if (asm_lanes() == 4) hipLaunchKernelGGL((k_assemble<double, 4>), asm_grid(), dim3(kBlock), 0, stream, a);
         else hipLaunchKernelGGL((k_assemble<double, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     }
@@ -3125,6 +3213,8 @@ int ffea_gpu_check_layout(const ffea_gpu_params *params, int num_blobs, const in
     return guarded(nullptr, [&]() {
         const Tables t = make_tables(in, precision == FFEA_GPU_MIXED, *params);
         validate(t);
+        if (precision == FFEA_GPU_MIXED) (void)stage_tables(t, kStageRows<float>, kStageCap<float>);
+        else (void)stage_tables(t, kStageRows<double>, kStageCap<double>);
         if (info) {
             *info = ffea_gpu_blob_info{};
             for (const auto &m : t.meta) {
@@ -3679,7 +3769,7 @@ int ffea_gpu_blob_autotune(ffea_gpu_blob *b, int iterations, int *chosen_blocks)
             // the assembly rewrites the matrix of the last element pass, the same values each time
             double best_a = 1e300;
             int lanes = b->asm_lanes();
-            for (int l : {1, 4, 16}) {
+            for (int l : {1, 4, 16, kAsmStaged}) {
                 b->asm_lanes_set = l;
                 b->launch_assemble();
                 FFEA_HIP_TRY(hipEventRecord(e0, b->stream));
@@ -3780,9 +3870,11 @@ int ffea_gpu_blob_get_launch_shape(ffea_gpu_blob *b, int *row_blocks, int *spmv_
 int ffea_gpu_blob_set_launch_shape(ffea_gpu_blob *b, int row_blocks, int spmv_unroll, int assembly_lanes) {
     if (!b) return FFEA_GPU_ERR_INVALID;
     if ((spmv_unroll && spmv_unroll != kMaxUnrolledRow && spmv_unroll != kMaxUnrolledRow / 2) ||
-        (assembly_lanes && assembly_lanes != 1 && assembly_lanes != 4 && assembly_lanes != 16) || row_blocks < 0) {
-        b->err = "ffea_gpu_blob_set_launch_shape: row_blocks >= 0, spmv_unroll 16 or 8, assembly_lanes 1, 4 or 16 "
-                 "(0: unchanged)";
+        (assembly_lanes && assembly_lanes != 1 && assembly_lanes != 4 && assembly_lanes != 16 &&
+         assembly_lanes != kAsmStaged) ||
+        row_blocks < 0) {
+        b->err = "ffea_gpu_blob_set_launch_shape: row_blocks >= 0, spmv_unroll 16 or 8, assembly_lanes 1, 4, 16 or -1 "
+                 "(records staged in LDS) (0: unchanged)";
         return FFEA_GPU_ERR_INVALID;
     }
     return guarded(b, [&]() {

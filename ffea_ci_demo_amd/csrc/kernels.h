@@ -202,7 +202,7 @@ __device__ inline void ztz_from_moments(const double *m, double *M) {
 // tetra_element_linear.h:56-117, del2 :585-622): upper triangle computed, the lower
 // one mirrored, everything times vol at the end.
 template <typename T>
-__device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
+__host__ __device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int a, int j, int b) {
 #pragma clang fp contract(off)
     int I = 4 * i, Jc = 4 * j, p = a, q = b;
     if (I > Jc || (I == Jc && p > q)) {  // mirror to the upper triangle
@@ -219,6 +219,43 @@ __device__ __forceinline__ T visc_entry(const T *d, T A, T B, T vol, int i, int 
         else v = B * (d[I + p] * d[Jc + q]) + A * (d[I + q] * d[Jc + p]);
     }
     return v * vol;
+}
+
+// The 3x3 block (a, b) of the same matrix for corners known only at run time, entry (i, j)
+// equal to visc_entry(d, A, B, vol, i, a, j, b) bit for bit (the same products and sums:
+// IEEE products and sums of two are commutative). The two corners' shape gradients
+// ga = d[4c + a], gb = d[4c + b] are picked once; indexing d with a run-time corner in every
+// entry instead costs a chain of selects per access.
+template <typename T>
+__host__ __device__ __forceinline__ void visc_block_g(const T (&ga)[3], const T (&gb)[3], bool same, T A, T B, T vol,
+                                                      T (&out)[9]) {
+#pragma clang fp contract(off)
+    const T del2 = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T v;
+            if (i == j) {
+                v = ga[i] * gb[i] * (A + B);
+                v += del2 * A;
+            } else {
+                const T vs = (B + A) * (ga[i] * ga[j]);
+                const T vd = B * (ga[i] * gb[j]) + A * (gb[i] * ga[j]);
+                v = same ? vs : vd;
+            }
+            out[3 * i + j] = v * vol;
+        }
+}
+template <typename T>
+__host__ __device__ __forceinline__ void visc_block(const T (&d)[12], T A, T B, T vol, int a, int b, T (&out)[9]) {
+    T ga[3], gb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ga[c] = a == 0 ? d[4 * c] : a == 1 ? d[4 * c + 1] : a == 2 ? d[4 * c + 2] : d[4 * c + 3];
+        gb[c] = b == 0 ? d[4 * c] : b == 1 ? d[4 * c + 1] : b == 2 ? d[4 * c + 2] : d[4 * c + 3];
+    }
+    visc_block_g<T>(ga, gb, a == b, A, B, vol, out);
 }
 
 // ---------------------------------------------------------------- element pass
@@ -541,6 +578,100 @@ __global__ __launch_bounds__(kBlock) void k_geo_material(int ne, const T *A, con
 }
 
 
+// the 12 used values of a shape record as the 3x4 shape-derivative array d (column 0 from
+// the other three), its volume and A, B
+template <typename T>
+__device__ __forceinline__ void unpack_geo(const T (&gr)[12], T (&d)[12], T &vol, T &A, T &B) {
+#pragma clang fp contract(off)
+    d[1] = gr[0]; d[2] = gr[1]; d[3] = gr[2];
+    d[5] = gr[3]; d[6] = gr[4]; d[7] = gr[5];
+    d[9] = gr[6]; d[10] = gr[7]; d[11] = gr[8];
+    vol = gr[9];
+    d[0] = -(d[1] + d[2] + d[3]);
+    d[4] = -(d[5] + d[6] + d[7]);
+    d[8] = -(d[9] + d[10] + d[11]);
+    A = gr[10];
+    B = gr[11];
+}
+
+// a summed block (+ the row's drag on its diagonal) into the matrix, its FP16 copy and D^-1
+template <typename T>
+__device__ __forceinline__ void store_block(const AsmArgs<T> &a, int n, int j, int lane, long long base,
+                                            double (&acc)[9]) {
+#pragma clang fp contract(off)
+    if (a.drag && j == a.diag_j[n]) {  // NoMassCGSolver.cpp:119-127: after the element sources
+        const double dr = a.drag[n];
+        acc[0] += dr;
+        acc[4] += dr;
+        acc[8] += dr;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)acc[k];
+    if (a.hval)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a.hval[val_index<_Float16>(base, j, lane, k)] = (_Float16)(acc[k] * a.hscale);
+    if (j == a.diag_j[n]) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
+    }
+}
+
+// The slot a row takes at step p: its diagonal block first (about 24 sources in a tet mesh,
+// against 1-6 for an off-diagonal one), then the others in slot order. Rows of a wavefront
+// have their diagonals in different slots; taken in slot order, every step would wait for
+// the lane at its diagonal. The order of the blocks does not change any block's sum.
+__device__ __forceinline__ int diag_first(int p, int dj) {
+    if (dj == 0xFF) return p;
+    return p == 0 ? dj : (p <= dj ? p - 1 : p);
+}
+
+// one row's slots, steps s, s + L, ... of diag_first: each block the sum of its element
+// sources in element order (+ drag on the diagonal), written to the matrix, its FP16 copy
+// and D^-1. source(c, gr) returns the record of source c in gr, or false for an identity
+// source.
+template <typename T, int L, typename Source>
+__device__ __forceinline__ void assemble_row(const AsmArgs<T> &a, int n, int s, Source source) {
+#pragma clang fp contract(off)
+    const int sl = n / kSlice, lane = n % kSlice;
+    const int W = a.slice_w[sl];
+    const long long base = a.slice_base[sl];
+    const int dj = a.diag_j[n];
+    for (int p = s; p < W; p += L) {
+        const int j = diag_first(p, dj);
+        const long long slot = base + (long long)j * kSlice + lane;
+        const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
+        double acc[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+        for (uint32_t c = c0; c < c1; ++c) {
+            T gr[12];
+            int pa, pb;
+            if (!source(c, gr, pa, pb)) {
+                acc[0] += 1.0; acc[4] += 1.0; acc[8] += 1.0;
+                continue;
+            }
+            T d[12], vol, A, B, blk[9];
+            unpack_geo<T>(gr, d, vol, A, B);
+            visc_block<T>(d, A, B, vol, pa, pb, blk);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k] += (double)blk[k];
+        }
+        store_block<T>(a, n, j, lane, base, acc);
+    }
+}
+
+// the global source code (e<<4)|(a<<2)|b: the record from the element's shape record
+template <typename T>
+__device__ __forceinline__ bool global_source(const AsmArgs<T> &a, uint32_t c, T (&gr)[12], int &pa, int &pb) {
+    const uint32_t code = a.contrib[c];
+    if (code == kIdentity) return false;
+    const int e = (int)(code >> 4);
+    pa = (code >> 2) & 3;
+    pb = code & 3;
+    load_geo(a.geo + (size_t)kGeo * e, gr);
+    return true;
+}
+
 // SparseMatrixFixedPattern::build + calc_inverse_diagonal (SparseMatrixFixedPattern.cpp:86-94,
 // 305-312): every block entry is the sum of its element sources in element order.
 // L lanes share a row, lane s taking the row's slots s, s + L, ...: L = 1 for large
@@ -550,7 +681,6 @@ __global__ __launch_bounds__(kBlock) void k_geo_material(int ne, const T *A, con
 // flight). Each block is the same sum whatever the mapping.
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
-#pragma clang fp contract(off)
     // XCD-aware order: workgroups reach the 8 XCDs round-robin, so workgroup b takes the
     // (b % 8)-th eighth of the rows; the rows an XCD works on at once are then neighbours
     // in the mesh, whose shape records its L2 shares
@@ -559,52 +689,116 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs<T> a) {
     const long long t = (long long)lb * kBlock + threadIdx.x;
     const int n = (int)(t / L), s = (int)(t % L);
     if (n >= a.nrows) return;
+    assemble_row<T, L>(a, n, s, [&](uint32_t c, T(&gr)[12], int &pa, int &pb) {
+        return global_source<T>(a, c, gr, pa, pb);
+    });
+}
+
+// The same blocks with the shape records staged in LDS. One wavefront per group of
+// kStageRows<T> consecutive rows, kStageLanes<T> lanes per row. The group's elements (every
+// element with a corner among its rows, ascending) are listed in gel[gptr[g] .. gptr[g+1]);
+// the wavefront copies their records to LDS once, as the four corners' shape gradients
+// (d[4c + a] at 3a + c, column 0 formed here as unpack_geo forms it) then vol, A, B, so a
+// source reads its two corners' gradients at run-time offsets with no selects. Each source
+// c names its record by its place in that list, cloc[c] = (place<<4)|(a<<2)|b (0xFFFF:
+// identity). A 6M-tet blob's record is thus fetched about 3 times (once per group it
+// touches) instead of 16 times (once per block source). A group with more than
+// kStageCap<T> elements reads the records from memory instead. Same sums in the same order
+// as k_assemble.
+#ifndef FFEA_STAGE_ROWS_F32
+#define FFEA_STAGE_ROWS_F32 8
+#endif
+#ifndef FFEA_STAGE_ROWS_F64
+#define FFEA_STAGE_ROWS_F64 8
+#endif
+template <typename T> constexpr int kStageRows = std::is_same<T, double>::value ? FFEA_STAGE_ROWS_F64 : FFEA_STAGE_ROWS_F32;
+template <typename T> constexpr int kStageLanes = kSlice / kStageRows<T>;
+// a group of a regular tet mesh has about 18.4 elements per row; at 8 rows its records take
+// 9.6 KB of LDS in FP32 and 19.2 KB in FP64
+template <typename T> constexpr int kStageCap = 20 * kStageRows<T>;
+constexpr int kStageRec = 15;  // values per staged record
+constexpr uint16_t kLocalIdentity = 0xFFFF;
+constexpr int kAsmStaged = -1;  // the assembly launch-shape value that selects k_assemble_staged
+
+template <typename T>
+__global__ __launch_bounds__(kSlice) void k_assemble_staged(AsmArgs<T> a, const uint32_t *gptr, const uint32_t *gel,
+                                                            const uint16_t *cloc) {
+#pragma clang fp contract(off)
+    constexpr int G = kStageRows<T>, L = kStageLanes<T>, CAP = kStageCap<T>;
+    constexpr int RPT = (CAP + kSlice - 1) / kSlice;  // records per thread
+    constexpr int kBatch = 8;                          // sources whose codes load together
+    __shared__ T rec[CAP * kStageRec];
+    const int nbk = gridDim.x, bk = blockIdx.x;
+    const int g = (nbk % 8 == 0) ? (bk % 8) * (nbk / 8) + bk / 8 : bk;
+    const int r0 = g * G;
+    if (r0 >= a.nrows) return;  // the same for the whole workgroup
+    const uint32_t e0 = gptr[g], cnt = gptr[g + 1] - e0;
+    const bool staged = cnt <= (uint32_t)CAP;
+    if (staged && cnt > 0) {
+        // every record first, then the LDS copies, so that all loads are in flight together
+        // (addresses past the list repeat its last entry; those copies are not kept)
+        T gr[RPT][12];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            load_geo(a.geo + (size_t)kGeo * gel[e0 + min((uint32_t)(threadIdx.x + k * kSlice), cnt - 1)], gr[k]);
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int r = threadIdx.x + k * kSlice;
+            if (r < (int)cnt) {
+                T d[12], vol, A, B;
+                unpack_geo<T>(gr[k], d, vol, A, B);
+                T *o = rec + r * kStageRec;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) o[3 * q + c] = d[4 * c + q];
+                o[12] = vol;
+                o[13] = A;
+                o[14] = B;
+            }
+        }
+    }
+    __syncthreads();
+    const int n = r0 + threadIdx.x / L, s = threadIdx.x % L;
+    if (!staged) {
+        assemble_row<T, L>(a, n, s, [&](uint32_t c, T(&gr)[12], int &pa, int &pb) {
+            return global_source<T>(a, c, gr, pa, pb);
+        });
+        return;
+    }
     const int sl = n / kSlice, lane = n % kSlice;
     const int W = a.slice_w[sl];
     const long long base = a.slice_base[sl];
-    for (int j = s; j < W; j += L) {
+    const int dj = a.diag_j[n];
+    for (int p = s; p < W; p += L) {
+        const int j = diag_first(p, dj);
         const long long slot = base + (long long)j * kSlice + lane;
         const uint32_t c0 = a.cptr[slot], c1 = a.cptr[slot + 1];
         double acc[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-        for (uint32_t c = c0; c < c1; ++c) {
-            const uint32_t code = a.contrib[c];
-            if (code == kIdentity) {
-                acc[0] += 1.0; acc[4] += 1.0; acc[8] += 1.0;
-                continue;
+        for (uint32_t cb = c0; cb < c1; cb += kBatch) {
+            uint16_t code[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) code[k] = cloc[min(cb + k, c1 - 1)];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                if (cb + k >= c1) break;
+                if (code[k] == kLocalIdentity) {
+                    acc[0] += 1.0; acc[4] += 1.0; acc[8] += 1.0;
+                    continue;
+                }
+                const int pa = (code[k] >> 2) & 3, pb = code[k] & 3;
+                const T *rp = rec + (code[k] >> 4) * kStageRec;
+                const T ga[3] = {rp[3 * pa], rp[3 * pa + 1], rp[3 * pa + 2]};
+                const T gb[3] = {rp[3 * pb], rp[3 * pb + 1], rp[3 * pb + 2]};
+                T blk[9];
+                visc_block_g<T>(ga, gb, pa == pb, rp[13], rp[14], rp[12], blk);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) acc[q] += (double)blk[q];
             }
-            const int e = (int)(code >> 4), pa = (code >> 2) & 3, pb = code & 3;
-            T d[12], gr[12];
-            load_geo(a.geo + (size_t)kGeo * e, gr);  // the element's shape record
-            d[1] = gr[0]; d[2] = gr[1]; d[3] = gr[2];
-            d[5] = gr[3]; d[6] = gr[4]; d[7] = gr[5];
-            d[9] = gr[6]; d[10] = gr[7]; d[11] = gr[8];
-            const T vol = gr[9];
-            d[0] = -(d[1] + d[2] + d[3]);
-            d[4] = -(d[5] + d[6] + d[7]);
-            d[8] = -(d[9] + d[10] + d[11]);
-            const T A = gr[10], B = gr[11];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int jj = 0; jj < 3; ++jj) acc[3 * i + jj] += (double)visc_entry<T>(d, A, B, vol, i, pa, jj, pb);
         }
-        if (a.drag && j == a.diag_j[n]) {  // NoMassCGSolver.cpp:119-127: after the element sources
-            const double dr = a.drag[n];
-            acc[0] += dr;
-            acc[4] += dr;
-            acc[8] += dr;
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) a.val[val_index<T>(base, j, lane, k)] = (T)acc[k];
-        if (a.hval)
-#pragma unroll
-            for (int k = 0; k < 9; ++k) a.hval[val_index<_Float16>(base, j, lane, k)] = (_Float16)(acc[k] * a.hscale);
-        if (j == a.diag_j[n]) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
-        }
+        store_block<T>(a, n, j, lane, base, acc);
     }
 }
 

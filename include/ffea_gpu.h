@@ -390,9 +390,11 @@ int ffea_gpu_blob_get_refinement(ffea_gpu_blob *blob, int *last_sweeps, long lon
  * of workgroups of the row kernels (rounded to whole chunks per workgroup and a multiple
  * of 8, one share per XCD; ffea_gpu_blob_set_row_blocks), the SpMV's row unroll (rows of
  * up to 16 or up to 8 blocks issue all their block loads together) and the assembly's
- * lanes per row (1, 4 or 16). ffea_gpu_blob_autotune times `iterations` CG iterations of
+ * mapping: 1, 4 or 16 lanes per row reading each block source's shape record from memory,
+ * or -1: one wavefront per group of 8 rows that first copies the records of its rows'
+ * elements to LDS. ffea_gpu_blob_autotune times `iterations` CG iterations of
  * the current matrix at each candidate workgroup count, then both unrolls at the fastest
- * count, then three assemblies at each lane count, keeps the fastest of each and returns
+ * count, then three assemblies with each mapping, keeps the fastest of each and returns
  * the workgroup count in *chosen (may be NULL); it leaves positions, velocities and noise
  * streams untouched. ffea_gpu_blob_get_launch_shape reads the three knobs;
  * ffea_gpu_blob_set_launch_shape sets them (0: unchanged). No counterpart in the

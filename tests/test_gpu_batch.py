@@ -138,9 +138,53 @@ def test_row_grid_and_autotune_are_result_neutral():
         shape = g.launch_shape()
         print(f"[autotune] {spec.name}: {shape}", flush=True)
         assert chosen == g.info()["row_blocks"] == shape["row_blocks"] and chosen >= 1
-        assert shape["spmv_unroll"] in (8, 16) and shape["assembly_lanes"] in (1, 4, 16)
+        assert shape["spmv_unroll"] in (8, 16) and shape["assembly_lanes"] in (1, 4, 16, -1)
         g.step(2)
         assert np.array_equal(g.positions(), want)
+
+
+def renumbered(spec, seed=0):
+    """the same blob with its nodes in random order: consecutive rows then share no elements,
+    so most assembly groups list more elements than the LDS holds"""
+    perm = np.random.default_rng(seed).permutation(spec.num_nodes).astype(np.int32)
+    pos = np.empty_like(spec.pos)
+    pos[perm] = spec.pos
+    spec.pos = pos
+    spec.top = perm[spec.top]
+    spec.mesh = None
+    return spec
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("mesh", ["ordered", "renumbered"])
+def test_assembly_mappings_are_result_neutral(precision, mesh):
+    """lanes per row reading shape records from memory, or records staged in LDS per group
+    of rows (falling back to memory for groups with too many elements): the same matrix,
+    so the same positions bit for bit"""
+    spec = workloads.cube_blob(14, mesh_seed=6, rng_seed=3)
+    if mesh == "renumbered":
+        spec = renumbered(spec)
+    got = {}
+    for lanes in (1, 4, 16, -1):
+        with GpuBlob.from_spec(spec, precision=precision) as g:
+            g.set_launch_shape(assembly_lanes=lanes)
+            assert g.launch_shape()["assembly_lanes"] == lanes
+            g.step(3)
+            got[lanes] = g.positions()
+    for lanes, pos in got.items():
+        assert np.array_equal(pos, got[1]), lanes
+
+
+def test_staged_assembly_in_ensembles_and_stokes():
+    """the staged assembly over groups that straddle blobs, pinned rows and Stokes drag"""
+    from test_layout import stokes_spec
+    for specs in (small_ensemble(big=True), [stokes_spec(5), stokes_spec(4, seed=2)]):
+        with GpuBatch(specs) as a, GpuBatch(specs) as b:
+            b.set_launch_shape(assembly_lanes=-1)
+            a.step(3)
+            b.step(3)
+            for k in range(len(specs)):
+                assert np.array_equal(a.positions(k), b.positions(k)), k
 
 
 def test_launch_shape_is_result_neutral_in_projected_mixed_form():
@@ -148,7 +192,7 @@ def test_launch_shape_is_result_neutral_in_projected_mixed_form():
     unroll and every assembly lane count give the same positions bit for bit"""
     spec = workloads.cube_blob(24, mesh_seed=5, rng_seed=9)
     got = {}
-    for unroll, lanes in ((16, 1), (8, 1), (16, 4), (8, 16)):
+    for unroll, lanes in ((16, 1), (8, 1), (16, 4), (8, 16), (16, -1)):
         with GpuBlob.from_spec(spec, precision="mixed") as g:
             g.set_launch_shape(spmv_unroll=unroll, assembly_lanes=lanes)
             g.step(3)
