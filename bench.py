@@ -44,6 +44,19 @@ METRIC = "element-steps/s on 1 MI355X (FP32-mixed vs FP64) + HBM roofline fracti
 PEAK_HBM_GBS = 8000.0
 DEFAULT_FORMS = "fp64,mixed,fp64+mg,mixed+mg"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_spmv_cube100_traffic.json")
+ENSEMBLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ensemble512_traffic.json")
+
+
+def profiled_traffic(path, workload, form):
+    """HBM bytes per SpMV launch that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this
+    workload and form recorded (scaled by a 1 GiB copy), or None"""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    if t.get("workload") != workload or form not in t.get("forms", {}):
+        return None
+    return t["forms"][form]["hbm_bytes_per_launch"]
 
 
 def spmv_alg_bytes(info, w, projected=False):
@@ -277,7 +290,7 @@ def cpu_baseline(spec, k_per_step, cg_iters, n=None):
     }
 
 
-def cpu_baseline_ensemble(specs, stride=64):
+def cpu_baseline_ensemble(specs, stride=32):
     """The oracle on one host core over a bounded sample of the ensemble: every stride-th
     blob by size (so the sample spans the size range), one whole step each, the
     reference's Jacobi-PCG run to convergence; element-steps/s = sampled tets / time"""
@@ -355,12 +368,7 @@ def main():
     w = 8 if split_form(head)[0] == "fp64" else 4
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as fh:
-            t = json.load(fh)
-        if t.get("workload") == spec.name and head in t.get("forms", {}):
-            traffic = t["forms"][head]["hbm_bytes_per_launch"]
+    traffic = profiled_traffic(TRAFFIC_FILE, spec.name, head)
     roof = {
         "bound": "hbm",
         "kernel": "k_cg_spmv",
@@ -514,7 +522,7 @@ def main_ensemble(args):
     ns = [round((s.num_elements / 6) ** (1 / 3)) for s in specs]
     print(f"[bench] {len(specs)} meshes ({ne_tot} tets, n {min(ns)}..{max(ns)}) built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    form = (args.headline or "mixed") if args.forms == DEFAULT_FORMS else args.forms.split(",")[-1]
+    form = (args.headline or "mixed+mg") if args.forms == DEFAULT_FORMS else args.forms.split(",")[-1]
     h = run_ensemble(specs, form, args.steps, args.warmup, graph=not args.no_graph)
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
@@ -525,7 +533,7 @@ def main_ensemble(args):
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(h["spmv_achieved_GBs"] / PEAK_HBM_GBS, 4),
-        "traffic": None,
+        "traffic": profiled_traffic(ENSEMBLE_TRAFFIC_FILE, f"ensemble{len(specs)}", form),
         "alg_bytes_per_launch": h["spmv_alg_bytes"],
         "avg_launch_us": round(h["spmv_avg_us_all_blobs_running"], 2),
         "copy_probe_GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
