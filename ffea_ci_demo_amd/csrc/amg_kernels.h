@@ -241,6 +241,11 @@ struct MgFineArgs {
     const double *omega_row = nullptr;  // an ensemble's smoother weight per row (else omega)
 };
 
+// blocks whose loads one row issues together (slice_row_times)
+#ifndef FFEA_MG_FINE_GROUP
+#define FFEA_MG_FINE_GROUP 2
+#endif
+
 // BS threads per chunk: kRowBlock for chunks of up to 1024 rows, one row a thread; an
 // ensemble's mostly 256-row chunks take kMgBlock, looping over longer chunks
 template <typename T, int MODE, int BS = kRowBlock>
@@ -252,21 +257,12 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
     for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
         const double omega = a.omega_row ? a.omega_row[n] : a.omega;
         const int sl = n / kSlice, lane = n % kSlice;
-        const int W = a.slice_w[sl];
+        const int W = __builtin_amdgcn_readfirstlane(a.slice_w[sl]);  // wave-uniform
         const long long base = a.slice_base[sl];
-        const int *colp = a.col + base + lane;
-        const T *vp = a.val + (size_t)base * 9;
-        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-        for (int j = 0; j < W; ++j) {
-            const int cc = __builtin_nontemporal_load(colp + (size_t)j * kSlice);
-            const double x0 = a.x[vi(cc, 0)], x1 = a.x[vi(cc, 1)], x2 = a.x[vi(cc, 2)];
-            T w[9];
-            load_block<T>(vp + (size_t)j * 9 * kSlice, lane, w);
-            acc0 += (double)w[0] * x0; acc0 += (double)w[1] * x1; acc0 += (double)w[2] * x2;
-            acc1 += (double)w[3] * x0; acc1 += (double)w[4] * x1; acc1 += (double)w[5] * x2;
-            acc2 += (double)w[6] * x0; acc2 += (double)w[7] * x1; acc2 += (double)w[8] * x2;
-        }
-        const double acc[3] = {acc0 * a.mscale, acc1 * a.mscale, acc2 * a.mscale};
+        double acc[3] = {0.0, 0.0, 0.0};
+        slice_row_times<T, 0, FFEA_MG_FINE_GROUP>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.x, acc);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] *= a.mscale;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const size_t o = vi(n, k);

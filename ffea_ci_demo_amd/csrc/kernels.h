@@ -1301,8 +1301,67 @@ __device__ __forceinline__ void load_block<float>(const float *slot, int lane, f
     w[8] = __builtin_nontemporal_load(slot + 8 * kSlice + lane);
 }
 
-// U: rows with up to U blocks issue all their block loads together (the unrolled path);
-// longer rows loop. Either way every row sums its blocks in slot order.
+// acc += (one row of V) z, the row being lane `lane` of a slice whose W slots (W
+// wave-uniform: padding slots hold zero blocks) start at colp / vp; blocks are summed in
+// slot order (SparseMatrixFixedPattern.cpp:148-187 on the block matrix).
+// Loads are issued in groups of G blocks, each group's value loads and z gathers before
+// any of them is used (a scheduling barrier keeps them together), so G blocks are in
+// flight at once; one branch per slot instead made every block wait for the one before.
+// Rows of up to U slots (U > 0) load all their block columns first. Slots past W in a row's last
+// group repeat slot W-1 with a zero z, which adds +-0 and leaves the sums unchanged.
+// The matrix streams through once per product: non-temporal loads keep it from evicting
+// the gathered z and the row vectors that the next kernel reads.
+#ifndef FFEA_SPMV_GROUP_F64
+#define FFEA_SPMV_GROUP_F64 1
+#endif
+template <typename T, int U, int G = sizeof(T) == 8 ? FFEA_SPMV_GROUP_F64 : 4>
+__device__ __forceinline__ void slice_row_times(const int *colp, const T *vp, int lane, int W, const double *z,
+                                                double (&acc)[3]) {
+    auto group = [&](int g, const int (&cc)[G]) {
+        T w[G][9];
+        double zz[G][3];
+#pragma unroll
+        for (int h = 0; h < G; ++h) {
+            const int j = g + h < W ? g + h : W - 1;
+            load_block<T>(vp + (size_t)j * 9 * kSlice, lane, w[h]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) zz[h][k] = z[vi(cc[h], k)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int h = 0; h < G; ++h) {
+            const bool live = g + h < W;
+            const double z0 = live ? zz[h][0] : 0.0, z1 = live ? zz[h][1] : 0.0, z2 = live ? zz[h][2] : 0.0;
+            acc[0] += (double)w[h][0] * z0; acc[0] += (double)w[h][1] * z1; acc[0] += (double)w[h][2] * z2;
+            acc[1] += (double)w[h][3] * z0; acc[1] += (double)w[h][4] * z1; acc[1] += (double)w[h][5] * z2;
+            acc[2] += (double)w[h][6] * z0; acc[2] += (double)w[h][7] * z1; acc[2] += (double)w[h][8] * z2;
+        }
+    };
+    if (U > 0 && W <= U) {
+        int ca[U > 0 ? U : 1];
+#pragma unroll
+        for (int j = 0; j < U; ++j) ca[j] = __builtin_nontemporal_load(colp + (size_t)(j < W ? j : W - 1) * kSlice);
+#pragma unroll
+        for (int g = 0; g < U; g += G) {
+            if (g < W) {
+                int cc[G];
+#pragma unroll
+                for (int h = 0; h < G; ++h) cc[h] = ca[g + h];
+                group(g, cc);
+            }
+        }
+    } else {
+        for (int g = 0; g < W; g += G) {
+            int cc[G];
+#pragma unroll
+            for (int h = 0; h < G; ++h)
+                cc[h] = __builtin_nontemporal_load(colp + (size_t)(g + h < W ? g + h : W - 1) * kSlice);
+            group(g, cc);
+        }
+    }
+}
+
+// U: rows with up to U blocks load all their block columns first (slice_row_times)
 template <typename T, bool PROJ, int BS = row_block<T>, int U = kMaxUnrolledRow>
 __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange cr) {
     const int i = a.iter;
@@ -1372,37 +1431,12 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
         for (int k = 0; k < NV; ++k) v[k] = 0.0;
         for (int n = ch.begin + threadIdx.x; n < ch.end; n += BS) {
             const int sl = n / kSlice, lane = n % kSlice;
-            const int W = a.slice_w[sl];
+            // wave-uniform: a slice is one wavefront's 64 rows
+            const int W = __builtin_amdgcn_readfirstlane(a.slice_w[sl]);
             const long long base = a.slice_base[sl];
-            const int *colp = a.col + base + lane;
-            const T *vp = a.val + (size_t)base * 9;
-            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-            auto block = [&](int j, int cc) {
-                const T *vv = vp + (size_t)j * 9 * kSlice;
-                const double z0 = a.z[vi(cc, 0)], z1 = a.z[vi(cc, 1)], z2 = a.z[vi(cc, 2)];
-                // the matrix streams through once per iteration: non-temporal loads keep it
-                // from evicting the gathered z and the p, q the vector step reads next
-                T w[9];
-                load_block<T>(vv, lane, w);
-                acc0 += (double)w[0] * z0; acc0 += (double)w[1] * z1; acc0 += (double)w[2] * z2;
-                acc1 += (double)w[3] * z0; acc1 += (double)w[4] * z1; acc1 += (double)w[5] * z2;
-                acc2 += (double)w[6] * z0; acc2 += (double)w[7] * z1; acc2 += (double)w[8] * z2;
-            };
-            if (W <= U) {
-                // all block columns of the row first, so the gathers and value loads of every
-                // block can be in flight together
-                int cc[U];
-#pragma unroll
-                for (int j = 0; j < U; ++j)
-                    if (j < W) cc[j] = __builtin_nontemporal_load(colp + (size_t)j * kSlice);
-#pragma unroll
-                for (int j = 0; j < U; ++j)
-                    if (j < W) block(j, cc[j]);
-            } else {
-#pragma unroll 2
-                for (int j = 0; j < W; ++j) block(j, __builtin_nontemporal_load(colp + (size_t)j * kSlice));
-            }
-            double pn[3], qn[3] = {acc0, acc1, acc2};
+            double acc[3] = {0.0, 0.0, 0.0};
+            slice_row_times<T, U>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.z, acc);
+            double pn[3], qn[3] = {acc[0], acc[1], acc[2]};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const double zk = a.z[vi(n, k)];
