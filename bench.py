@@ -61,10 +61,12 @@ def profiled_traffic(path, workload, form):
 
 def spmv_alg_bytes(info, w, projected=False):
     """algorithmic bytes of one CG SpMV-step launch: the viscosity matrix (9 values of
-    w bytes + a 4-byte block column per 3x3 block) and five FP64 node 3-vectors (z
-    gathered once, p and q read and written); the projected mixed form of a free blob
-    also reads the FP64 positions for its rigid-mode sums"""
-    return info["num_blocks"] * (9 * w + 4) + (6 if projected else 5) * 24 * info["num_linear_nodes"]
+    w bytes + a 4-byte block column per 3x3 block) and five FP64 node 3-vectors (z read
+    once, p and q read and written); the mixed form (w = 4) gathers z's packed FP32 copy
+    (12 B per row, read once), and its projected form of a free blob also reads the FP64
+    positions for its rigid-mode sums"""
+    nl = info["num_linear_nodes"]
+    return info["num_blocks"] * (9 * w + 4) + (6 if projected else 5) * 24 * nl + (12 * nl if w == 4 else 0)
 
 
 def projects(specs, prec):
@@ -74,8 +76,11 @@ def projects(specs, prec):
 
 
 def iter_alg_bytes(info, w):
-    """the per-CG-iteration model: Nb (9w + 4) + 216 Nl (matrix + 9 FP64 vector passes)"""
-    return info["num_blocks"] * (9 * w + 4) + 216 * info["num_linear_nodes"]
+    """the per-CG-iteration model: Nb (9w + 4) + 216 Nl (matrix + 9 FP64 vector passes);
+    the mixed form adds z's FP32 copy, written by the vector step and read by the SpMV
+    (24 Nl)"""
+    nl = info["num_linear_nodes"]
+    return info["num_blocks"] * (9 * w + 4) + 216 * nl + (24 * nl if w == 4 else 0)
 
 
 def step_alg_bytes(info, w, k):
@@ -408,7 +413,8 @@ def main():
                         "thermal noise at 300 K, dt 1e-14 s, CG epsilon 1e-10, no pins",
             "form": head,
             "cg_iterations_per_step": round(h["cg_iterations_per_step"], 1),
-            "precision": ("element algebra, shape data and viscosity matrix FP32; CG vectors, inner products, "
+            "precision": ("element algebra, shape data and viscosity matrix FP32; the matrix products gather "
+                          "an FP32 copy of their input vector and accumulate in FP64; CG vectors, inner products, "
                           "residual norms and positions FP64" if split_form(head)[0] == "mixed" else "FP64") +
                          ("; multigrid preconditioner matrices FP16 with a power-of-two scale per matrix, FP64 "
                           "accumulation" if split_form(head)[1] == "multigrid" else ""),

@@ -64,6 +64,7 @@ struct MgArgs {
     double omega, bscale;
     double *y, *y2;         // RESTRICT: y = A x, y2 = omega D^-1 y (if not null)
     const double *bscale_row = nullptr;  // PROLONG on level 0 of an ensemble: bscale per row
+    float *y32 = nullptr;   // PROLONG on level 0 of the FP32-mixed form: y's packed FP32 copy
 };
 
 // y = A x and the mode's row update, one thread per block row:
@@ -105,11 +106,15 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
             a.y[o] = acc[p];
             if (a.y2) a.y2[o] = a.omega * a.dinv[o] * acc[p];
         } else if constexpr (MODE == MG_PROLONG) {
-            a.y[o] = (a.bscale_row ? a.bscale_row[row] : a.bscale) * a.base[o] + acc[p];
+            const double yv = (a.bscale_row ? a.bscale_row[row] : a.bscale) * a.base[o] + acc[p];
+            a.y[o] = yv;
+            acc[p] = yv;  // for the FP32 copy
         } else {
             a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - acc[p]);
         }
     }
+    if constexpr (MODE == MG_PROLONG && BR == 3)
+        if (a.y32) store_f32x3(a.y32, row, acc);
 }
 
 // The same operations for matrices with few, long rows (the coarse operators, 6x6 blocks
@@ -233,12 +238,14 @@ struct MgFineArgs {
     const int *col;
     const T *val;          // the assembly's scaled FP16 copy of K (the CG's slots)
     double mscale;         // inverse of its scale
-    const double *x;       // gathered: z (RESID) or xb (POST)
+    const double *x;       // z (RESID) or xb (POST): gathered (FP64 form), own row
     const double *r, *dinv;
     double omega;
     double *y;             // RESID: residual; POST: z
     double *P;             // POST: partials row (P[c])
     const double *omega_row = nullptr;  // an ensemble's smoother weight per row (else omega)
+    const float *x32 = nullptr;         // FP32-mixed form: x's packed FP32 copy, gathered
+    float *y32 = nullptr;               // POST of the FP32-mixed form: z's packed FP32 copy
 };
 
 // blocks whose loads one row issues together (slice_row_times)
@@ -248,7 +255,8 @@ struct MgFineArgs {
 
 // BS threads per chunk: kRowBlock for chunks of up to 1024 rows, one row a thread; an
 // ensemble's mostly 256-row chunks take kMgBlock, looping over longer chunks
-template <typename T, int MODE, int BS = kRowBlock>
+// V: the gathered vector's type, double (FP64 form: x) or float (FP32-mixed form: x32)
+template <typename T, int MODE, int BS = kRowBlock, typename V = double>
 __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
     const int c = blockIdx.x;
     const Chunk ch = a.chunks[c];
@@ -260,9 +268,13 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
         const int W = __builtin_amdgcn_readfirstlane(a.slice_w[sl]);  // wave-uniform
         const long long base = a.slice_base[sl];
         double acc[3] = {0.0, 0.0, 0.0};
-        slice_row_times<T, 0, FFEA_MG_FINE_GROUP>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.x, acc);
+        if constexpr (std::is_same<V, float>::value)
+            slice_row_times<T, 0, FFEA_MG_FINE_GROUP>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.x32, acc);
+        else
+            slice_row_times<T, 0, FFEA_MG_FINE_GROUP>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.x, acc);
 #pragma unroll
         for (int k = 0; k < 3; ++k) acc[k] *= a.mscale;
+        double z3[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const size_t o = vi(n, k);
@@ -272,9 +284,12 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
             } else {
                 const double zk = a.x[o] + omega * a.dinv[o] * (rk - acc[k]);
                 a.y[o] = zk;
+                z3[k] = zk;
                 v[0] += rk * zk;
             }
         }
+        if constexpr (MODE == MG_FINE_POST)
+            if (a.y32) store_f32x3(a.y32, n, z3);
     }
     if constexpr (MODE == MG_FINE_POST) store_partials<1, BS>(v, a.P, a.nchunks, c);
 }

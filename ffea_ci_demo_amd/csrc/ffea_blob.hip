@@ -1208,6 +1208,7 @@ struct ffea_gpu_blob {
     DevBuf<unsigned char> diag_j, pinned;
     DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
+    DevBuf<float> z32;              // FP32-mixed form: packed FP32 copy of z, gathered by the row products
     DevBuf<double> coef;            // [nblob][8]: projected-form step coefficients
     DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
     DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
@@ -1319,6 +1320,7 @@ struct ffea_gpu_blob {
     DevBuf<_Float16> mg_hval;  // scaled FP16 copy of the viscosity matrix, written by the assembly
     double mg_hscale = 1.0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
+    DevBuf<float> mg_xb32;          // FP32-mixed form: packed FP32 copy of mg_xb0
     DevBuf<double> mg_omega0;                           // ensemble: level-0 smoother weight per row
     DevBuf<int> mg_crow_blob, mg_coff_row, mg_cm, mg_free;  // ensemble coarsest tables; free blobs
     DevBuf<long long> mg_ccoff;
@@ -1548,6 +1550,10 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     elem_off.alloc(t.eoff.size(), &dev_bytes); elem_off.upload(t.eoff, stream);
     project.alloc(t.proj.size(), &dev_bytes); project.upload(t.proj, stream);
     const size_t vlen = (size_t)nslices * 3 * kSlice;  // tiled node vectors (vi)
+    if (mixed) {
+        z32.alloc((size_t)nslices * 3 * kSlice, &dev_bytes);
+        z32.zero(stream);
+    }
     for (DevBuf<double> *buf : {&dinv, &rhs, &r, &z, &x, &q, &p}) {
         buf->alloc(vlen, &dev_bytes);
         buf->zero(stream);
@@ -1761,9 +1767,11 @@ void ffea_gpu_blob::launch_gather(const T *force) {
                     have_sf ? sf_nf_ptr.p : nullptr, have_sf ? sf_nf.p : nullptr, have_sf ? sf_face_force.p : nullptr,
                     pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p,
                     mass && stokes ? vel.p : nullptr, mass && stokes ? drag.p : nullptr, rhs_in};
+    g.z32 = z32.p;
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
+        sa.z32 = z32.p;
         hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kMixedBlock), 0, stream, sa);
     }
 }
@@ -1813,7 +1821,7 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
     hipEvent_t ts = nullptr;
     if (t_cg(3, it)) t_begin(3, &ts);
     SpmvArgs<T> sa{row_args(), it, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
-                   pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
+                   z32.p, pos.p, centre.p, p.p, q.p, it == 0 ? Pi.p : P2.p, P1.p};
     constexpr int U2 = kMaxUnrolledRow / 2;
     const bool half = spmv_unroll != kMaxUnrolledRow;
     if (std::is_same<T, double>::value && small_chunks) {
@@ -1830,7 +1838,7 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
 void ffea_gpu_blob::launch_cg_vec(int it) {
     hipEvent_t tv = nullptr;
     if (t_cg(4, it)) t_begin(4, &tv);
-    VecArgs va{row_args(), it, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
+    VecArgs va{row_args(), it, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
     if (mixed && any_project) {
         hipLaunchKernelGGL(k_cg_coef, dim3(nblob), dim3(kMixedBlock), 0, stream, row_args(), it, (const double *)P1.p,
                            (const double *)Gmat.p, coef.p);
@@ -1857,8 +1865,8 @@ void ffea_gpu_blob::launch_cg_iteration(int it, bool spmv_only) {
 template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_cg_fused_t(const T *val) {
     SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
-                   pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
-    VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p};
+                   z32.p, pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
+    VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
     hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(row_block<T>), 0, stream, sa, va, (const double *)Pi.p);
     FFEA_HIP_TRY(hipGetLastError());
 }
@@ -2136,6 +2144,7 @@ void ffea_gpu_blob::mg_setup() {
     mg_cinv.reset();
     mg_hval.reset();
     for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef, &mg_omega0}) bf->reset();
+    mg_xb32.reset();
     for (DevBuf<int> *bf : {&mg_crow_blob, &mg_coff_row, &mg_cm, &mg_free}) bf->reset();
     mg_ccoff.reset();
     auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
@@ -2282,6 +2291,10 @@ void ffea_gpu_blob::mg_setup() {
     const size_t vlen = (size_t)nslices * 3 * kSlice;
     mg_res0.alloc(vlen, &dev_bytes); mg_res0.zero(stream);
     mg_xb0.alloc(vlen, &dev_bytes); mg_xb0.zero(stream);
+    if (mixed) {
+        mg_xb32.alloc(vlen, &dev_bytes);
+        mg_xb32.zero(stream);
+    }
     mg_rp.alloc((size_t)27 * std::max(1, nchunks), &dev_bytes); mg_rp.zero(stream);
     mg_coef.alloc((size_t)6 * nblob, &dev_bytes); mg_coef.zero(stream);
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
@@ -2299,10 +2312,16 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     const MgStop S0{nblob == 1 ? (const CgScalars *)S.p : nullptr, num_done.p, active_blobs};
     const bool ens = nblob > 1;  // an ensemble's mostly 256-row chunks: 256-thread level-0 workgroups
     const double *om0 = nblob == 1 ? nullptr : (const double *)mg_omega0.p;
+    // the FP32-mixed form's level-0 products gather the packed FP32 copies (z32, mg_xb32)
     MgFineArgs<_Float16> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, z.p, r.p,
-                            dinv.p, f.omega, mg_res0.p, nullptr, om0};
-    if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fa);
-    else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+                            dinv.p, f.omega, mg_res0.p, nullptr, om0, z32.p};
+    if (mixed) {
+        if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kMgBlock, float>), dim3(nchunks), dim3(kMgBlock), 0, stream, fa);
+        else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kRowBlock, float>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+    } else {
+        if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fa);
+        else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
+    }
     auto wgrid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock / 64 - 1) / (kMgBlock / 64))); };
     for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
@@ -2332,12 +2351,17 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
         hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(cur.n), dim3(kMgBlock), 0, stream, po);
     }
     MgArgs p0{S0, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, mg[1]->e.p, nullptr, z.p, nullptr,
-              0.0, f.omega, mg_xb0.p, nullptr, om0};
+              0.0, f.omega, mg_xb0.p, nullptr, om0, mg_xb32.p};
     hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
     MgFineArgs<_Float16> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, mg_xb0.p, r.p,
-                            dinv.p, f.omega, z.p, Pout, om0};
-    if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fp);
-    else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+                            dinv.p, f.omega, z.p, Pout, om0, mg_xb32.p, z32.p};
+    if (mixed) {
+        if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kMgBlock, float>), dim3(nchunks), dim3(kMgBlock), 0, stream, fp);
+        else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kRowBlock, float>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+    } else {
+        if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kMgBlock>), dim3(nchunks), dim3(kMgBlock), 0, stream, fp);
+        else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
+    }
     FFEA_HIP_TRY(hipGetLastError());
 }
 
@@ -2848,7 +2872,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     FFEA_HIP_TRY(hipMemcpyAsync(pos.p, pos_k, vec * sizeof(double), hipMemcpyDeviceToDevice, stream));
     launch_element();  // shape data of x_k for the operator
     launch_assemble();
-    AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p};
+    AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p, z32.p};
     hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kRowBlock), 0, stream, st);
     FFEA_HIP_TRY(hipGetLastError());
     if (mg_on()) launch_mg_vcycle(Pi.p);  // z = M r of the start; its r.z replaces row 0
@@ -3911,6 +3935,7 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->mg_cinv.reset();
             b->mg_hval.reset();
             for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef, &b->mg_omega0}) bf->reset();
+            b->mg_xb32.reset();
             for (auto *bf : {&b->mg_crow_blob, &b->mg_coff_row, &b->mg_cm, &b->mg_free}) bf->reset();
             b->mg_ccoff.reset();
             b->mg_any_free = false;

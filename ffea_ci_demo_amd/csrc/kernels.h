@@ -43,6 +43,14 @@ __host__ __device__ __forceinline__ size_t vi(int n, int k) {
     return ((size_t)(n >> 6) * 3 + (size_t)k) * 64 + (size_t)(n & 63);
 }
 
+// the FP32-mixed form's packed FP32 copy of a gathered vector, row n: three contiguous
+// floats, so a gather of one row is one 12-byte load
+__device__ __forceinline__ void store_f32x3(float *y, int n, const double (&v)[3]) {
+    float *o = y + 3 * (size_t)n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = (float)v[k];
+}
+
 // Block values of one slot (64 rows x 9 entries) are ordered so that a lane reads its 9
 // entries as 16-byte vectors: FP64 as pairs (0,1) (2,3) (4,5) (6,7) then 8, FP32 as
 // quads (0..3) (4..7) then 8. Each wave load is one contiguous 1 KiB span.
@@ -1042,6 +1050,7 @@ struct GatherArgs {
     const double *mass_drag = nullptr;  // and [rows] drags, for the drag force -drag v
     const double *rhs_in = nullptr;     // tiled [rows][3]: a right-hand side handed over whole
                                         // (the solver's own entry point), or null: formed here
+    float *z32 = nullptr;               // FP32-mixed form: z's packed FP32 copy
 };
 
 // Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
@@ -1145,6 +1154,7 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
                     zz[k] = a.dinv[vi(n, k)] * f[k];
                     a.z[vi(n, k)] = zz[k];
                 }
+                if (a.z32) store_f32x3(a.z32, n, zz);
                 v[0] += f[0] * zz[0] + f[1] * zz[1] + f[2] * zz[2];
                 v[1] += f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
             } else {
@@ -1180,6 +1190,7 @@ struct StartArgs {
     const double3 *centre;
     const int *project;         // [nblob]: 1 = free blob, project; 0 = leave r = f
     double *r, *z, *x, *P, *G;  // G: [nblob][36] ((Z^T Z)^-1, zero for unprojected blobs)
+    float *z32 = nullptr;       // FP32-mixed form: z's packed FP32 copy
 };
 __global__ __launch_bounds__(kMixedBlock) void k_cg_start(StartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
@@ -1229,6 +1240,7 @@ __global__ __launch_bounds__(kMixedBlock) void k_cg_start(StartArgs a) {
                 zl[k] = a.dinv[vi(n, k)] * rl[k];
                 a.z[vi(n, k)] = zl[k];
             }
+            if (a.z32) store_f32x3(a.z32, n, zl);
             v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
         }
         store_partials<1, kMixedBlock>(v, a.P, a.ra.nchunks, c);  // its block sum also orders the sGi reads before reuse
@@ -1252,6 +1264,7 @@ struct SpmvArgs {
     const int *col;        // [nslots]
     const T *val;          // [nslots*9]
     const double *z;
+    const float *z32;      // FP32-mixed form: z's packed FP32 copy [rows][3], which the row products gather
     const double *pos;
     const double3 *centre;
     double *p, *q;         // updated in place
@@ -1311,11 +1324,27 @@ __device__ __forceinline__ void load_block<float>(const float *slot, int lane, f
 // group repeat slot W-1 with a zero z, which adds +-0 and leaves the sums unchanged.
 // The matrix streams through once per product: non-temporal loads keep it from evicting
 // the gathered z and the row vectors that the next kernel reads.
+// blocks whose loads one row issues together (slice_row_times), FP64 and FP32 values
 #ifndef FFEA_SPMV_GROUP_F64
 #define FFEA_SPMV_GROUP_F64 1
 #endif
-template <typename T, int U, int G = sizeof(T) == 8 ? FFEA_SPMV_GROUP_F64 : 4>
-__device__ __forceinline__ void slice_row_times(const int *colp, const T *vp, int lane, int W, const double *z,
+#ifndef FFEA_SPMV_GROUP_F32
+#define FFEA_SPMV_GROUP_F32 4
+#endif
+// The gathered vector: FP64 tiled by 64 rows (vi), or the FP32-mixed form's packed FP32
+// copy [rows][3], whose three values a lane reads in one 12-byte load
+__device__ __forceinline__ void gather3(const double *z, int c, double (&o)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = z[vi(c, k)];
+}
+__device__ __forceinline__ void gather3(const float *z, int c, double (&o)[3]) {
+    const float *g = z + 3 * (size_t)c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = (double)g[k];
+}
+
+template <typename T, int U, int G = sizeof(T) == 8 ? FFEA_SPMV_GROUP_F64 : FFEA_SPMV_GROUP_F32, typename V = double>
+__device__ __forceinline__ void slice_row_times(const int *colp, const T *vp, int lane, int W, const V *z,
                                                 double (&acc)[3]) {
     auto group = [&](int g, const int (&cc)[G]) {
         T w[G][9];
@@ -1324,8 +1353,7 @@ __device__ __forceinline__ void slice_row_times(const int *colp, const T *vp, in
         for (int h = 0; h < G; ++h) {
             const int j = g + h < W ? g + h : W - 1;
             load_block<T>(vp + (size_t)j * 9 * kSlice, lane, w[h]);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) zz[h][k] = z[vi(cc[h], k)];
+            gather3(z, cc[h], zz[h]);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1435,7 +1463,10 @@ __device__ __forceinline__ void cg_spmv(const SpmvArgs<T> &a, const ChunkRange c
             const int W = __builtin_amdgcn_readfirstlane(a.slice_w[sl]);
             const long long base = a.slice_base[sl];
             double acc[3] = {0.0, 0.0, 0.0};
-            slice_row_times<T, U>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.z, acc);
+            if constexpr (std::is_same<T, float>::value)
+                slice_row_times<T, U>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.z32, acc);
+            else
+                slice_row_times<T, U>(a.col + base + lane, a.val + (size_t)base * 9, lane, W, a.z, acc);
             double pn[3], qn[3] = {acc[0], acc[1], acc[2]};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -1528,6 +1559,7 @@ struct VecArgs {
     double *x, *r, *z;
     const double *P1;  // chunk partials of the SpMV step
     double *P2;        // r.r, r.z partials of this step
+    float *z32 = nullptr;  // FP32-mixed form: z's packed FP32 copy
 };
 
 // alpha = delta / p.q, x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197),
@@ -1607,6 +1639,7 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
                 z3[k] = a.dinv[vi(n, k)] * r3[k];
                 a.z[vi(n, k)] = z3[k];
             }
+            if (a.z32) store_f32x3(a.z32, n, z3);
             v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
             v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
         }
@@ -2061,6 +2094,7 @@ struct AdjStartArgs {
     const unsigned char *pinned;
     const double *dinv;
     double *rhs, *r, *z, *x, *P;
+    float *z32 = nullptr;         // FP32-mixed form: z's packed FP32 copy
 };
 __global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
@@ -2070,6 +2104,7 @@ __global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
         double v[2] = {0.0, 0.0};
         for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
             const bool pin = a.pinned[n] != 0;
+            double z3[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const double f = pin ? 0.0 : a.lam[vi(n, k)];
@@ -2078,9 +2113,11 @@ __global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
                 a.r[vi(n, k)] = f;
                 a.x[vi(n, k)] = 0.0;
                 a.z[vi(n, k)] = zz;
+                z3[k] = zz;
                 v[0] += f * zz;
                 v[1] += f * f;
             }
+            if (a.z32) store_f32x3(a.z32, n, z3);
         }
         store_partials<2, kRowBlock>(v, a.P, a.ra.nchunks, c);
         if (c == ch.first && threadIdx.x == 0) {
