@@ -133,6 +133,7 @@ struct MgCsrArgs {
     const double *x, *b, *dinv;
     double omega;
     double *y, *y2;
+    const float *x32 = nullptr;  // level-0 restriction of the FP32-mixed form: x's packed FP32 copy, gathered
 };
 
 template <int BR, int BC, int MODE>
@@ -148,8 +149,13 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
     for (int j = lane; j < W; j += 64) {
         const int c = a.ci[r0 + j];
         double xv[BC];
+        if constexpr (BC == 3) {
+            if (a.x32) gather3(a.x32, c, xv);
+            else gather3(a.x, c, xv);
+        } else {
 #pragma unroll
-        for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+            for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+        }
         float w[BR * BC];
         if constexpr ((BR * BC) % 4 == 0) {
             typedef _Float16 vh4 __attribute__((ext_vector_type(4)));
@@ -245,7 +251,7 @@ struct MgFineArgs {
     double *P;             // POST: partials row (P[c])
     const double *omega_row = nullptr;  // an ensemble's smoother weight per row (else omega)
     const float *x32 = nullptr;         // FP32-mixed form: x's packed FP32 copy, gathered
-    float *y32 = nullptr;               // POST of the FP32-mixed form: z's packed FP32 copy
+    float *y32 = nullptr;               // FP32-mixed form: y's packed FP32 copy (the residual, z)
 };
 
 // blocks whose loads one row issues together (slice_row_times)
@@ -280,7 +286,8 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
             const size_t o = vi(n, k);
             const double rk = a.r[o];
             if constexpr (MODE == MG_FINE_RESID) {
-                a.y[o] = rk - omega * acc[k];
+                z3[k] = rk - omega * acc[k];
+                a.y[o] = z3[k];
             } else {
                 const double zk = a.x[o] + omega * a.dinv[o] * (rk - acc[k]);
                 a.y[o] = zk;
@@ -288,8 +295,7 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
                 v[0] += rk * zk;
             }
         }
-        if constexpr (MODE == MG_FINE_POST)
-            if (a.y32) store_f32x3(a.y32, n, z3);
+        if (a.y32) store_f32x3(a.y32, n, z3);
     }
     if constexpr (MODE == MG_FINE_POST) store_partials<1, BS>(v, a.P, a.nchunks, c);
 }

@@ -1320,7 +1320,7 @@ struct ffea_gpu_blob {
     DevBuf<_Float16> mg_hval;  // scaled FP16 copy of the viscosity matrix, written by the assembly
     double mg_hscale = 1.0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
-    DevBuf<float> mg_xb32;          // FP32-mixed form: packed FP32 copy of mg_xb0
+    DevBuf<float> mg_xb32, mg_res32;  // FP32-mixed form: packed FP32 copies of mg_xb0, mg_res0
     DevBuf<double> mg_omega0;                           // ensemble: level-0 smoother weight per row
     DevBuf<int> mg_crow_blob, mg_coff_row, mg_cm, mg_free;  // ensemble coarsest tables; free blobs
     DevBuf<long long> mg_ccoff;
@@ -2145,6 +2145,7 @@ void ffea_gpu_blob::mg_setup() {
     mg_hval.reset();
     for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef, &mg_omega0}) bf->reset();
     mg_xb32.reset();
+    mg_res32.reset();
     for (DevBuf<int> *bf : {&mg_crow_blob, &mg_coff_row, &mg_cm, &mg_free}) bf->reset();
     mg_ccoff.reset();
     auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
@@ -2294,6 +2295,8 @@ void ffea_gpu_blob::mg_setup() {
     if (mixed) {
         mg_xb32.alloc(vlen, &dev_bytes);
         mg_xb32.zero(stream);
+        mg_res32.alloc(vlen, &dev_bytes);
+        mg_res32.zero(stream);
     }
     mg_rp.alloc((size_t)27 * std::max(1, nchunks), &dev_bytes); mg_rp.zero(stream);
     mg_coef.alloc((size_t)6 * nblob, &dev_bytes); mg_coef.zero(stream);
@@ -2312,9 +2315,10 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     const MgStop S0{nblob == 1 ? (const CgScalars *)S.p : nullptr, num_done.p, active_blobs};
     const bool ens = nblob > 1;  // an ensemble's mostly 256-row chunks: 256-thread level-0 workgroups
     const double *om0 = nblob == 1 ? nullptr : (const double *)mg_omega0.p;
-    // the FP32-mixed form's level-0 products gather the packed FP32 copies (z32, mg_xb32)
+    // the FP32-mixed form's level-0 products gather the packed FP32 copies (z32, mg_xb32),
+    // its first restriction mg_res32
     MgFineArgs<_Float16> fa{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, z.p, r.p,
-                            dinv.p, f.omega, mg_res0.p, nullptr, om0, z32.p};
+                            dinv.p, f.omega, mg_res0.p, nullptr, om0, z32.p, mg_res32.p};
     if (mixed) {
         if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kMgBlock, float>), dim3(nchunks), dim3(kMgBlock), 0, stream, fa);
         else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID, kRowBlock, float>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
@@ -2327,7 +2331,8 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
         const bool coarsest = l + 1 == L - 1;
         MgCsrArgs ra{S0, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, cur.R.mscale, l == 0 ? mg_res0.p : cur.res.p,
-                     nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p};
+                     nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p,
+                     l == 0 ? mg_res32.p : nullptr};
         if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         if (!coarsest) {
@@ -3936,6 +3941,7 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->mg_hval.reset();
             for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef, &b->mg_omega0}) bf->reset();
             b->mg_xb32.reset();
+            b->mg_res32.reset();
             for (auto *bf : {&b->mg_crow_blob, &b->mg_coff_row, &b->mg_cm, &b->mg_free}) bf->reset();
             b->mg_ccoff.reset();
             b->mg_any_free = false;
