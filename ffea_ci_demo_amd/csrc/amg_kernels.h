@@ -15,8 +15,9 @@
 // multiplied back by the exact inverse), with FP64 accumulation: M only has to be a
 // fixed symmetric positive operator, and half the bytes make the V-cycle cheaper.
 // Prolongations (few blocks per row) are in a sliced ELL: 64 block rows per slice, slot
-// width W = the slice's longest row, value k of the block in slot j of lane l at
-// (base + j*64) * BR*BC + k*64 + l, block column at base + j*64 + l (-1: padding).
+// width W = the slice's longest row, block column at base + j*64 + l (-1: padding); a
+// slot's values are lane-major vectors (ell_val_index), so a lane reads its block's
+// BR*BC FP16 values in 16-byte loads.
 #pragma once
 #include "kernels.h"
 
@@ -48,6 +49,42 @@ struct VecLayout<6> {
 };
 
 enum MgMode { MG_RESID = 0, MG_RESTRICT = 1, MG_PROLONG = 2, MG_POST = 3 };
+
+// Value k of the BB-value block of lane l in the ELL slot starting at `slot` (= base +
+// j*64): the first BB/8*8 values as vectors of 8 (vector t of lanes 0..63 contiguous),
+// the rest (BB % 8) as one vector per lane after them. Each vector load of a wavefront
+// is one contiguous span.
+__host__ __device__ __forceinline__ size_t ell_val_index(long long slot, int bb, int lane, int k) {
+    const size_t b = (size_t)slot * bb;
+    const int full = bb / 8 * 8, rem = bb - full;
+    return k < full ? b + (size_t)(k / 8) * 8 * 64 + (size_t)lane * 8 + (k % 8)
+                    : b + (size_t)full * 64 + (size_t)lane * rem + (k - full);
+}
+
+template <int BB>
+__device__ __forceinline__ void load_ell_block(const _Float16 *val, long long slot, int lane, float (&w)[BB]) {
+    typedef _Float16 vh8 __attribute__((ext_vector_type(8)));
+    constexpr int full = BB / 8 * 8, rem = BB - full;
+    static_assert(rem == 0 || rem == 2 || rem == 4, "ELL blocks: a remainder of 2 or 4 values");
+    const _Float16 *b = val + (size_t)slot * BB;
+#pragma unroll
+    for (int t = 0; t < BB / 8; ++t) {
+        const vh8 u = __builtin_nontemporal_load((const vh8 *)(b + (size_t)t * 8 * 64) + lane);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[8 * t + i] = (float)u[i];
+    }
+    if constexpr (rem == 4) {
+        typedef _Float16 vh4 __attribute__((ext_vector_type(4)));
+        const vh4 u = __builtin_nontemporal_load((const vh4 *)(b + (size_t)full * 64) + lane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[full + i] = (float)u[i];
+    } else if constexpr (rem == 2) {
+        typedef _Float16 vh2 __attribute__((ext_vector_type(2)));
+        const vh2 u = __builtin_nontemporal_load((const vh2 *)(b + (size_t)full * 64) + lane);
+        w[full] = (float)u.x;
+        w[full + 1] = (float)u.y;
+    }
+}
 
 struct MgArgs {
     MgStop stop;
@@ -88,13 +125,23 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
         const int c = a.col[slot + lane];
         if (c < 0) continue;
         double xv[BC];
+        if constexpr (BC == 6) {  // a coarse row [n][6]: three 16-byte loads
 #pragma unroll
-        for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
-        const _Float16 *v = a.val + (size_t)slot * (BR * BC) + lane;
+            for (int h = 0; h < 3; ++h) {
+                const vd2 u = ((const vd2 *)(a.x + (size_t)c * 6))[h];
+                xv[2 * h] = u.x;
+                xv[2 * h + 1] = u.y;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+        }
+        float w[BR * BC];
+        load_ell_block<BR * BC>(a.val, slot, lane, w);
 #pragma unroll
         for (int p = 0; p < BR; ++p)
 #pragma unroll
-            for (int q = 0; q < BC; ++q) acc[p] += (double)__builtin_nontemporal_load(v + (p * BC + q) * 64) * xv[q];
+            for (int q = 0; q < BC; ++q) acc[p] += (double)w[p * BC + q] * xv[q];
     }
 #pragma unroll
     for (int p = 0; p < BR; ++p) {

@@ -1936,7 +1936,7 @@ EllHost to_ell(const amg::BCsr &A, double scale) {
             if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
             E.col[slot + lane] = A.ci[k];
             for (int e = 0; e < bb; ++e)
-                E.val[(size_t)slot * bb + (size_t)e * 64 + lane] = (_Float16)(A.v[(size_t)k * bb + e] * scale);
+                E.val[ell_val_index(slot, bb, lane, e)] = (_Float16)(A.v[(size_t)k * bb + e] * scale);
         }
     }
     return E;
