@@ -115,33 +115,50 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_apply(MgArgs a) {
     const int row = blockIdx.x * kMgBlock + threadIdx.x;
     if (row >= a.n) return;
     const int sl = row >> 6, lane = row & 63;
-    const int W = a.sw[sl];
+    const int W = __builtin_amdgcn_readfirstlane(a.sw[sl]);  // wave-uniform: a slice is one wavefront's rows
     const long long base = a.sb[sl];
     double acc[BR];
 #pragma unroll
     for (int p = 0; p < BR; ++p) acc[p] = 0.0;
-    for (int j = 0; j < W; ++j) {
-        const long long slot = base + (long long)j * 64;
-        const int c = a.col[slot + lane];
-        if (c < 0) continue;
-        double xv[BC];
-        if constexpr (BC == 6) {  // a coarse row [n][6]: three 16-byte loads
+    // groups of G slots issue all their loads before any product; padding slots (column -1)
+    // hold zero blocks and gather row 0, slots past W repeat slot W-1 with a zero x: both add
+    // +-0 and leave the sums unchanged
+    constexpr int G = BR * BC <= 18 ? 4 : 2;
+    for (int g = 0; g < W; g += G) {
+        int c[G];
 #pragma unroll
-            for (int h = 0; h < 3; ++h) {
-                const vd2 u = ((const vd2 *)(a.x + (size_t)c * 6))[h];
-                xv[2 * h] = u.x;
-                xv[2 * h + 1] = u.y;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < BC; ++q) xv[q] = a.x[VecLayout<BC>::at(c, q)];
+        for (int h = 0; h < G; ++h) {
+            const int j = g + h < W ? g + h : W - 1;
+            const int cj = a.col[base + (long long)j * 64 + lane];
+            c[h] = cj < 0 ? 0 : cj;
         }
-        float w[BR * BC];
-        load_ell_block<BR * BC>(a.val, slot, lane, w);
+        double xv[G][BC];
+        float w[G][BR * BC];
 #pragma unroll
-        for (int p = 0; p < BR; ++p)
+        for (int h = 0; h < G; ++h) {
+            const int j = g + h < W ? g + h : W - 1;
+            if constexpr (BC == 6) {  // a coarse row [n][6]: three 16-byte loads
 #pragma unroll
-            for (int q = 0; q < BC; ++q) acc[p] += (double)w[p * BC + q] * xv[q];
+                for (int t = 0; t < 3; ++t) {
+                    const vd2 u = ((const vd2 *)(a.x + (size_t)c[h] * 6))[t];
+                    xv[h][2 * t] = u.x;
+                    xv[h][2 * t + 1] = u.y;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < BC; ++q) xv[h][q] = a.x[VecLayout<BC>::at(c[h], q)];
+            }
+            load_ell_block<BR * BC>(a.val, base + (long long)j * 64, lane, w[h]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int h = 0; h < G; ++h) {
+            const bool live = g + h < W;
+#pragma unroll
+            for (int p = 0; p < BR; ++p)
+#pragma unroll
+                for (int q = 0; q < BC; ++q) acc[p] += (double)w[h][p * BC + q] * (live ? xv[h][q] : 0.0);
+        }
     }
 #pragma unroll
     for (int p = 0; p < BR; ++p) {

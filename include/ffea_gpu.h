@@ -44,8 +44,10 @@ typedef enum {
 
 typedef enum {
     FFEA_GPU_FP64 = 0,   /* all arithmetic in FP64, the reference's algorithm and orders */
-    FFEA_GPU_MIXED = 1   /* element algebra and the viscosity matrix in FP32; CG vectors,
-                            inner products, residual norms and positions in FP64 */
+    FFEA_GPU_MIXED = 1   /* element algebra and the viscosity matrix in FP32, matrix products
+                            gathering an FP32 copy of their input vector and accumulating in
+                            FP64; CG vectors, inner products, residual norms and positions in
+                            FP64 */
 } ffea_gpu_precision;
 
 /* Run parameters in FFEA's internal units (SimulationParams after assign/validate). */
