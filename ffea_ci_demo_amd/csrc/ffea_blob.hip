@@ -1209,7 +1209,6 @@ struct ffea_gpu_blob {
     DevBuf<double> pos, rest, vel, last_det, dinv, rhs, r, z, x, q, p, ctf, fold, Gmat;
     DevBuf<double> Pg, Pi, P1, P2;  // chunk partials: gather, CG start, SpMV step, vector step
     DevBuf<float> z32;              // FP32-mixed form: packed FP32 copy of z, gathered by the row products
-    DevBuf<double> coef;            // [nblob][8]: projected-form step coefficients
     DevBuf<double> d_jinv0, d_vol0, d_G, d_E, d_A, d_B, d_force, d_geo, d_val;  // FP64 form
     DevBuf<float> f_jinv0, f_vol0, f_G, f_E, f_A, f_B, f_force, f_geo, f_val;   // mixed form
     DevBuf<mrg::Jump> jump_group, jump_lane, step_jump;
@@ -1653,7 +1652,6 @@ void ffea_gpu_blob::build(const std::vector<BlobInput> &in) {
     seg.alloc(t.seg.size(), &dev_bytes); seg.upload(t.seg, stream);
     centre.alloc(t.centre.size(), &dev_bytes); centre.upload(t.centre, stream);
     Gmat.alloc((size_t)36 * nblob, &dev_bytes); Gmat.zero(stream);
-    coef.alloc((size_t)8 * nblob, &dev_bytes); coef.zero(stream);
     num_done.alloc(1, &dev_bytes); num_done.zero(stream);
     FFEA_HIP_TRY(hipStreamSynchronize(stream));
     if (mixed) upload_elements<float>(t);
@@ -1838,10 +1836,8 @@ void ffea_gpu_blob::launch_cg_iteration_t(int it, bool spmv_only, const T *val) 
 void ffea_gpu_blob::launch_cg_vec(int it) {
     hipEvent_t tv = nullptr;
     if (t_cg(4, it)) t_begin(4, &tv);
-    VecArgs va{row_args(), it, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
+    VecArgs va{row_args(), it, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
     if (mixed && any_project) {
-        hipLaunchKernelGGL(k_cg_coef, dim3(nblob), dim3(kMixedBlock), 0, stream, row_args(), it, (const double *)P1.p,
-                           (const double *)Gmat.p, coef.p);
         hipLaunchKernelGGL((k_cg_vec<true, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
     } else {
         if (mixed || small_chunks)
@@ -1866,7 +1862,7 @@ template <typename T, bool PROJ>
 void ffea_gpu_blob::launch_cg_fused_t(const T *val) {
     SpmvArgs<T> sa{row_args(), 0, prm.max_iterations_cg, cg_eps2(), zero_rhs_solved, slice_w.p, slice_base.p, col.p, val, z.p,
                    z32.p, pos.p, centre.p, p.p, q.p, Pi.p, P1.p};
-    VecArgs va{row_args(), 0, coef.p, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
+    VecArgs va{row_args(), 0, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
     hipLaunchKernelGGL((k_cg_fused<T, PROJ>), dim3(nblob), dim3(row_block<T>), 0, stream, sa, va, (const double *)Pi.p);
     FFEA_HIP_TRY(hipGetLastError());
 }

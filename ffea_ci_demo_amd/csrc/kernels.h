@@ -1551,7 +1551,6 @@ __global__ __launch_bounds__(kBlock) void k_apply_matrix(int nrows, const int *s
 struct VecArgs {
     RowArgs ra;
     int iter;
-    const double *coef;  // [nblob][8]: alpha, (Z^T Z)^-1 Z^T q of the projected form
     const double *p, *q, *dinv;
     const double *pos;
     const double3 *centre;
@@ -1566,30 +1565,22 @@ struct VecArgs {
 // z = D^-1 r (NoMassCGSolver.cpp:268-281), partials of r.r and r.z.
 // With PROJ, q is first projected onto the range of the operator:
 // q <- q - Z c, c = (Z^T Z)^-1 Z^T q, and p.q is corrected to match.
-// projected form: per blob (one workgroup each) the 13 sums of the SpMV step, the
-// projection coefficients c = (Z^T Z)^-1 Z^T q and alpha = delta / (p.q - (Z^T p).c)
-__device__ __forceinline__ void cg_coef(const RowArgs &ra, int iter, const double *P1, const double *G,
-                                        double *coef, int b) {
-    const CgScalars *S = ra.S + b;
-    if (S->done) return;
-    const int2 sg = ra.seg[b];
-    double t[13];
-    sum_partials<13, kMixedBlock>(P1, ra.nchunks, sg.x, sg.y, t);
-    if (threadIdx.x == 0) {
-        double cq[6], pq = t[0];
-        for (int k = 0; k < 6; ++k) {
-            double acc = 0.0;
-            for (int j = 0; j < 6; ++j) acc += G[36 * b + 6 * k + j] * t[1 + j];
-            cq[k] = acc;
-        }
-        for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
-        coef[8 * b] = S->delta[(iter - 1) & 1] / pq;
-        for (int k = 0; k < 6; ++k) coef[8 * b + 1 + k] = cq[k];
+// the projected step's coefficients from the SpMV step's totals t (p.q, Z^T q, Z^T p):
+// cq = (Z^T Z)^-1 Z^T q (Gi = (Z^T Z)^-1) and alpha = r.z / (p.q - (Z^T p).cq), the
+// projected q being q - Z cq
+__device__ __forceinline__ void proj_coef(const double (&t)[13], const double *Gi, double rz, double &alpha,
+                                          double (&cq)[6]) {
+    double pq = t[0];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += Gi[6 * k + j] * t[1 + j];
+        cq[k] = acc;
     }
-}
-__global__ __launch_bounds__(kMixedBlock) void k_cg_coef(RowArgs ra, int iter, const double *P1, const double *G,
-                                                    double *coef) {
-    cg_coef(ra, iter, P1, G, coef, blockIdx.x);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pq -= t[7 + k] * cq[k];
+    alpha = rz / pq;
 }
 
 template <bool PROJ, int BS>
@@ -1606,9 +1597,12 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
             skip = S->done != 0;
             if (!skip) {
                 if constexpr (PROJ) {
-                    alpha = a.coef[8 * cur];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) cq[k] = a.coef[8 * cur + 1 + k];
+                    // every workgroup forms the blob's coefficients from the SpMV step's
+                    // totals itself (identical in every block): no kernel between the steps
+                    const int2 sg = a.ra.seg[cur];
+                    double t[13];
+                    sum_partials<13, BS>(a.P1, a.ra.nchunks, sg.x, sg.y, t);
+                    proj_coef(t, a.G + 36 * cur, S->delta[(a.iter - 1) & 1], alpha, cq);
                 } else {
                     const int2 sg = a.ra.seg[cur];
                     double t[1];
@@ -1652,8 +1646,7 @@ __global__ __launch_bounds__(BS) void k_cg_vec(VecArgs a) {
 }
 
 // Small blobs: the whole solve in one launch, one workgroup per blob running the SpMV
-// step, (projected form) the coefficient step and the vector step of each iteration
-// back to back over the blob's chunks. Partials, scalars and every sum are those of
+// step and the vector step of each iteration back to back over the blob's chunks. Partials, scalars and every sum are those of
 // the separate kernels, so the results are identical; a barrier with a workgroup fence
 // between the steps stands in for the kernel boundary.
 template <typename T, bool PROJ>
@@ -1670,10 +1663,6 @@ __global__ __launch_bounds__(row_block<T>) void k_cg_fused(SpmvArgs<T> sa, VecAr
         __syncthreads();
         if (S->done || it == sa.max_iter) break;  // the step of max_iter only tests max_iter - 1
         va.iter = it;
-        if constexpr (PROJ) {
-            cg_coef(va.ra, it, va.P1, va.G, const_cast<double *>(va.coef), b);
-            __syncthreads();
-        }
         cg_vec<PROJ, row_block<T>>(va, cr);
         __syncthreads();
     }
