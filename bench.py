@@ -373,6 +373,7 @@ def main():
     w = 8 if split_form(head)[0] == "fp64" else 4
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
+    read_ms = _native.read_probe(copy_bytes, iters=20)
     traffic = profiled_traffic(TRAFFIC_FILE, spec.name, head)
     roof = {
         "bound": "hbm",
@@ -385,6 +386,7 @@ def main():
         "alg_bytes_per_launch": h["spmv_alg_bytes"],
         "avg_launch_us": round(h["spmv_avg_us"], 2),
         "copy_probe_GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
+        "read_probe_GBs": round(copy_bytes / (read_ms / 1e3) / 1e9, 1),
     }
     cpu = None
     if not args.no_cpu_baseline:
@@ -532,6 +534,7 @@ def main_ensemble(args):
     h = run_ensemble(specs, form, args.steps, args.warmup, graph=not args.no_graph)
     copy_bytes = 1 << 30
     copy_ms = _native.copy_probe(copy_bytes, iters=20)
+    read_ms = _native.read_probe(copy_bytes, iters=20)
     roof = {
         "bound": "hbm",
         "kernel": "k_cg_spmv",
@@ -543,6 +546,7 @@ def main_ensemble(args):
         "alg_bytes_per_launch": h["spmv_alg_bytes"],
         "avg_launch_us": round(h["spmv_avg_us_all_blobs_running"], 2),
         "copy_probe_GBs": round(2 * copy_bytes / (copy_ms / 1e3) / 1e9, 1),
+        "read_probe_GBs": round(copy_bytes / (read_ms / 1e3) / 1e9, 1),
     }
     cpu = None
     if not args.no_cpu_baseline:

@@ -37,7 +37,7 @@ EXPORTED = [
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
     "ffea_gpu_blob_kinetic_energy", "ffea_gpu_solver_create", "ffea_gpu_solver_solve", "ffea_gpu_solver_apply_matrix",
     "ffea_gpu_blob_set_refinement", "ffea_gpu_blob_get_refinement", "ffea_gpu_batch_strain_energy_noalloc",
-    "ffea_gpu_blob_get_launch_shape", "ffea_gpu_blob_set_launch_shape",
+    "ffea_gpu_blob_get_launch_shape", "ffea_gpu_blob_set_launch_shape", "ffea_gpu_read_probe",
 ]
 
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
@@ -128,6 +128,7 @@ def lib():
     L.ffea_gpu_blob_get_info.argtypes = [vp, C.POINTER(Info)]
     L.ffea_gpu_blob_set_timing.argtypes = [vp, C.c_uint]
     L.ffea_gpu_copy_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_double)]
+    L.ffea_gpu_read_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.ffea_gpu_blob_get_timing.argtypes = [vp, _dp, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")]
     pp = C.POINTER(C.c_void_p)
     L.ffea_gpu_batch_create.argtypes = [C.POINTER(Params), C.c_int, _ip, _ip, pp, pp, pp, pp, pp, pp, vp, vp, vp, vp,
@@ -209,6 +210,15 @@ def copy_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:
     """ms per copy of nbytes (read nbytes + write nbytes)"""
     ms = C.c_double()
     rc = lib().ffea_gpu_copy_probe(device, int(nbytes), int(iters), C.byref(ms))
+    if rc != OK:
+        raise RuntimeError(lib().ffea_gpu_last_error().decode())
+    return ms.value
+
+
+def read_probe(nbytes: int, iters: int = 20, device: int = 0) -> float:
+    """ms per streaming read of nbytes"""
+    ms = C.c_double()
+    rc = lib().ffea_gpu_read_probe(device, int(nbytes), int(iters), C.byref(ms))
     if rc != OK:
         raise RuntimeError(lib().ffea_gpu_last_error().decode())
     return ms.value

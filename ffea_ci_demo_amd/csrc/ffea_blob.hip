@@ -4050,33 +4050,49 @@ int ffea_gpu_blob_synchronize(ffea_gpu_blob *b) {
     });
 }
 
-int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_copy) {
-    if (bytes <= 0 || iters <= 0 || !ms_per_copy) return FFEA_GPU_ERR_INVALID;
+// mean time of `iters` launches of a streaming probe kernel over `bytes` (read = 0: a copy
+// of `bytes`; read = 1: `bytes` read and summed)
+static int stream_probe(int device, long long bytes, int iters, int read, double *ms_per_launch) {
+    if (bytes < 16 || iters <= 0 || !ms_per_launch) return FFEA_GPU_ERR_INVALID;
     return guarded(nullptr, [&]() {
         FFEA_HIP_TRY(hipSetDevice(device));
         const size_t n2 = (size_t)bytes / 16;
-        double2 *xx = nullptr, *yy = nullptr;
-        FFEA_HIP_TRY(hipMalloc(&xx, n2 * 16));
-        FFEA_HIP_TRY(hipMalloc(&yy, n2 * 16));
-        FFEA_HIP_TRY(hipMemset(xx, 0, n2 * 16));
+        const unsigned grid = (unsigned)((n2 + (size_t)kProbeVecs * kBlock - 1) / ((size_t)kProbeVecs * kBlock));
+        DevBuf<vd2> xx, yy;
+        DevBuf<double> part;
+        size_t total = 0;
+        xx.alloc(n2, &total);
+        if (read) part.alloc(grid, &total);
+        else yy.alloc(n2, &total);
+        FFEA_HIP_TRY(hipMemset(xx.p, 0, n2 * 16));
         hipEvent_t e0, e1;
         FFEA_HIP_TRY(hipEventCreate(&e0));
         FFEA_HIP_TRY(hipEventCreate(&e1));
-        const int grid = 256 * 32;
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, xx, yy, n2);  // warm-up
+        auto launch = [&]() {
+            if (read) hipLaunchKernelGGL(k_read, dim3(grid), dim3(kBlock), 0, 0, (const vd2 *)xx.p, n2, part.p);
+            else hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, (const vd2 *)xx.p, yy.p, n2);
+        };
+        launch();  // warm-up
         FFEA_HIP_TRY(hipEventRecord(e0, 0));
-        for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, 0, xx, yy, n2);
+        for (int i = 0; i < iters; ++i) launch();
         FFEA_HIP_TRY(hipEventRecord(e1, 0));
         FFEA_HIP_TRY(hipEventSynchronize(e1));
+        FFEA_HIP_TRY(hipGetLastError());
         float ms = 0;
         FFEA_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        *ms_per_copy = ms / iters;
+        *ms_per_launch = ms / iters;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        (void)hipFree(xx);
-        (void)hipFree(yy);
         return (int)FFEA_GPU_OK;
     });
+}
+
+int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_copy) {
+    return stream_probe(device, bytes, iters, 0, ms_per_copy);
+}
+
+int ffea_gpu_read_probe(int device, long long bytes, int iters, double *ms_per_read) {
+    return stream_probe(device, bytes, iters, 1, ms_per_read);
 }
 
 const char *ffea_gpu_blob_last_error(const ffea_gpu_blob *b) { return b ? b->err.c_str() : g_last_error.c_str(); }

@@ -2240,9 +2240,36 @@ __global__ __launch_bounds__(kRowBlock) void k_blob_energy_range(int nblob, int 
 
 // ---------------------------------------------------------------- copy probe
 // y = x over n2 16-byte vectors, grid-stride: reads n2*16 and writes n2*16 bytes
-__global__ __launch_bounds__(kBlock) void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, size_t n2) {
+// Streaming probes of the achievable HBM rate. A thread moves kProbeVecs 16-byte vectors
+// a grid stride apart (each instruction of a wavefront covers 1 KiB of contiguous bytes),
+// all loads issued before any use, non-temporal. k_copy: y = x (the calibration kernel of
+// the hardware byte counters); k_read: every vector read once, one sum per workgroup.
+constexpr int kProbeVecs = 4;
+__global__ __launch_bounds__(kBlock) void k_copy(const vd2 *__restrict__ x, vd2 *__restrict__ y, size_t n2) {
     const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += stride) y[i] = x[i];
+    for (size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x; i0 < n2; i0 += kProbeVecs * stride) {
+        vd2 v[kProbeVecs];
+#pragma unroll
+        for (int u = 0; u < kProbeVecs; ++u)
+            if (i0 + u * stride < n2) v[u] = __builtin_nontemporal_load(x + i0 + u * stride);
+#pragma unroll
+        for (int u = 0; u < kProbeVecs; ++u)
+            if (i0 + u * stride < n2) __builtin_nontemporal_store(v[u], y + i0 + u * stride);
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_read(const vd2 *__restrict__ x, size_t n2, double *__restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    double s[1] = {0.0};
+    for (size_t i0 = (size_t)blockIdx.x * kBlock + threadIdx.x; i0 < n2; i0 += kProbeVecs * stride) {
+        vd2 v[kProbeVecs];
+#pragma unroll
+        for (int u = 0; u < kProbeVecs; ++u)
+            v[u] = i0 + u * stride < n2 ? __builtin_nontemporal_load(x + i0 + u * stride) : vd2{0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kProbeVecs; ++u) s[0] += v[u].x + v[u].y;
+    }
+    block_sum<1, kBlock>(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 }  // namespace ffk

@@ -474,10 +474,12 @@ int ffea_gpu_blob_set_timing(ffea_gpu_blob *blob, unsigned phase_mask);
 int ffea_gpu_blob_get_timing(ffea_gpu_blob *blob, double ms[FFEA_GPU_NUM_PHASES],
                              long long launches[FFEA_GPU_NUM_PHASES]);
 
-/* Achievable-bandwidth probe: mean time of a device-to-device copy kernel moving
- * `bytes` in and `bytes` out, over `iters` launches. Also the calibration kernel for
- * hardware byte counters. No counterpart in the reference. */
+/* Achievable-bandwidth probes (no counterpart in the reference): mean time over `iters`
+ * launches of a streaming kernel that copies `bytes` device to device (`bytes` in and
+ * `bytes` out; also the calibration kernel of the hardware byte counters), or that reads
+ * `bytes` once (`ffea_gpu_read_probe`). */
 int ffea_gpu_copy_probe(int device, long long bytes, int iters, double *ms_per_copy);
+int ffea_gpu_read_probe(int device, long long bytes, int iters, double *ms_per_read);
 
 /* waits for all work queued on the handle's stream */
 int ffea_gpu_blob_synchronize(ffea_gpu_blob *blob);
