@@ -46,6 +46,15 @@
 
 using namespace ffk;
 
+// damped-Jacobi sweeps of the multigrid V-cycle on level 1 and on the coarser levels
+// (2 .. L-2), before and after the correction from the level below
+#ifndef FFEA_MG_SWEEPS_L1
+#define FFEA_MG_SWEEPS_L1 1
+#endif
+#ifndef FFEA_MG_SWEEPS_COARSER
+#define FFEA_MG_SWEEPS_COARSER 4
+#endif
+
 // Scratch of the allocation-free entry point (ffea_gpu_batch_strain_energy_noalloc): 64 MiB
 // per GPU, held by every GPU the library runs on until the process ends. Host code never
 // uses this name as an address: each device's copy is found once with hipGetSymbolAddress
@@ -2323,6 +2332,17 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
         else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_RESID>), dim3(nchunks), dim3(kRowBlock), 0, stream, fa);
     }
     auto wgrid = [](int n) { return dim3((unsigned)std::max(1, (n + kMgBlock / 64 - 1) / (kMgBlock / 64))); };
+    // levels 1 .. L-2 take sweeps(l) damped-Jacobi sweeps before and after their coarse
+    // correction (the first pre-sweep from zero is the restriction's y2); the sweeps
+    // alternate between two of the level's vectors, and pre[l] / out[l] record where the
+    // pre-smoothed iterate and the level's correction end up
+    auto sweeps = [](int l) { return l == 1 ? FFEA_MG_SWEEPS_L1 : FFEA_MG_SWEEPS_COARSER; };
+    std::vector<double *> pre(L, nullptr), out(L, nullptr);
+    auto post_sweep = [&](MgLevel &lv, const double *x, double *y) {
+        MgCsrArgs po{S0, lv.n, lv.A.rp.p, lv.A.ci.p, lv.A.val.p, lv.A.mscale, x, lv.b.p, lv.dinv.p, lv.omega, y,
+                     nullptr};
+        hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(lv.n), dim3(kMgBlock), 0, stream, po);
+    };
     for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
         const bool coarsest = l + 1 == L - 1;
@@ -2332,7 +2352,14 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
         if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         if (!coarsest) {
-            MgCsrArgs rs{S0, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.A.mscale, nx.xa.p, nx.b.p, nullptr, 0.0,
+            double *xp = nx.xa.p;
+            for (int k = 1; k < sweeps(l + 1); ++k) {
+                double *to = xp == nx.xa.p ? nx.e.p : nx.xa.p;  // e is free until the way up
+                post_sweep(nx, xp, to);
+                xp = to;
+            }
+            pre[l + 1] = xp;
+            MgCsrArgs rs{S0, nx.n, nx.A.rp.p, nx.A.ci.p, nx.A.val.p, nx.A.mscale, xp, nx.b.p, nullptr, 0.0,
                          nx.res.p, nullptr};
             hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESID>), wgrid(nx.n), dim3(kMgBlock), 0, stream, rs);
         }
@@ -2342,16 +2369,21 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
                    nblob == 1 ? nullptr : (const int *)mg_coff_row.p, nblob == 1 ? nullptr : (const int *)mg_cm.p,
                    nblob == 1 ? nullptr : (const long long *)mg_ccoff.p};
     hipLaunchKernelGGL(k_mg_dense, wgrid(mg_nc), dim3(kMgBlock), 0, stream, da);
+    out[L - 1] = c.e.p;
     for (int l = L - 2; l >= 1; --l) {  // up: prolong, post-smooth
-        MgLevel &cur = *mg[l], &nx = *mg[l + 1];
-        MgArgs pa{S0, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, cur.P.mscale, nx.e.p, nullptr,
-                  cur.xa.p, nullptr, 0.0, 1.0, cur.xb.p, nullptr};
+        MgLevel &cur = *mg[l];
+        MgArgs pa{S0, cur.n, cur.P.sw.p, cur.P.sb.p, cur.P.col.p, cur.P.val.p, cur.P.mscale, out[l + 1], nullptr,
+                  pre[l], nullptr, 0.0, 1.0, cur.xb.p, nullptr};
         hipLaunchKernelGGL((k_mg_apply<6, 6, MG_PROLONG>), grid(cur.n), dim3(kMgBlock), 0, stream, pa);
-        MgCsrArgs po{S0, cur.n, cur.A.rp.p, cur.A.ci.p, cur.A.val.p, cur.A.mscale, cur.xb.p, cur.b.p, cur.dinv.p,
-                     cur.omega, cur.e.p, nullptr};
-        hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(cur.n), dim3(kMgBlock), 0, stream, po);
+        double *xp = cur.xb.p;
+        for (int k = 0; k < sweeps(l); ++k) {
+            double *to = xp == cur.xb.p ? cur.e.p : cur.xb.p;
+            post_sweep(cur, xp, to);
+            xp = to;
+        }
+        out[l] = xp;
     }
-    MgArgs p0{S0, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, mg[1]->e.p, nullptr, z.p, nullptr,
+    MgArgs p0{S0, f.n, f.P.sw.p, f.P.sb.p, f.P.col.p, f.P.val.p, f.P.mscale, out[1], nullptr, z.p, nullptr,
               0.0, f.omega, mg_xb0.p, nullptr, om0, mg_xb32.p};
     hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
     MgFineArgs<_Float16> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, mg_xb0.p, r.p,
