@@ -260,10 +260,12 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
     }
 }
 
-// coarsest level: e = C b with the dense inverse C (n x n, FP32, symmetric: row `row`
-// is read as contiguous bytes), one wavefront per row. An ensemble has one inverse per
-// blob (row_blob non-null): row `row` belongs to blob q = row_blob[row], whose rows are
-// [off[q], off[q] + m[q]) and whose inverse starts at C + coff[q].
+// coarsest level: e = C b with the dense inverse C (m x m, FP32, symmetric: row `row`
+// is read as contiguous bytes, rows padded with zeros to dense_ld(m), a multiple of 4
+// floats, so a lane reads 16-byte vectors), one wavefront per row. An ensemble has one
+// inverse per blob (row_blob non-null): row `row` belongs to blob q = row_blob[row],
+// whose rows are [off[q], off[q] + m[q]) and whose inverse starts at C + coff[q].
+__host__ __device__ __forceinline__ int dense_ld(int m) { return (m + 3) / 4 * 4; }
 struct MgDenseArgs {
     MgStop stop;
     int n;
@@ -285,8 +287,20 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_dense(MgDenseArgs a) {
         m = a.m[q];
         C += a.coff[q];
     }
+    const int ld = dense_ld(m);
+    const vf4 *Cr = (const vf4 *)(C + (size_t)(row - o) * ld);
+    const double *b = a.b + o;
     double acc = 0.0;
-    for (int k = lane; k < m; k += 64) acc += (double)C[(size_t)(row - o) * m + k] * a.b[o + k];
+#pragma unroll 4
+    for (int k4 = lane; k4 < ld / 4; k4 += 64) {
+        const vf4 c = __builtin_nontemporal_load(Cr + k4);
+        const int k = 4 * k4;
+        // the padding's zeros multiply nothing: past m no b is read
+        acc += (double)c.x * b[k];
+        if (k + 1 < m) acc += (double)c.y * b[k + 1];
+        if (k + 2 < m) acc += (double)c.z * b[k + 2];
+        if (k + 3 < m) acc += (double)c.w * b[k + 3];
+    }
     acc = wave_sum(acc);
     if (lane == 0) a.e[row] = acc;
 }

@@ -2,6 +2,7 @@
 #include "amg_setup.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <functional>
 #include <stdexcept>
@@ -24,6 +25,71 @@ void parallel_rows(int n, int threads, const std::function<void(int, int, int)> 
         pool.emplace_back(fn, b, e, t);
     }
     for (auto &th : pool) th.join();
+}
+
+// fn(i) for every i in [0, count) on up to `threads` threads, which take items in turn;
+// each item is computed whole by one thread in a fixed order, so results do not depend
+// on the thread count
+void parallel_items(int count, int threads, const std::function<void(int)> &fn) {
+    const int nt = std::max(1, std::min(threads, count));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next++; i < count; i = next++) fn(i);
+    };
+    if (nt == 1) {
+        work();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t) pool.emplace_back(work);
+    for (auto &th : pool) th.join();
+}
+
+// dot product of n contiguous values in four interleaved partial sums (a fixed order)
+inline double dot4(const double *a, const double *b, int n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+        s0 += a[k] * b[k];
+        s1 += a[k + 1] * b[k + 1];
+        s2 += a[k + 2] * b[k + 2];
+        s3 += a[k + 3] * b[k + 3];
+    }
+    for (; k < n; ++k) s0 += a[k] * b[k];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Cholesky M = L L^T in place (lower triangle of the row-major n x n M), right-looking
+// in blocks of kb columns: the diagonal block on one thread, then the panel rows below
+// it and the trailing rows, one row per parallel item
+void cholesky_blocked(std::vector<double> &M, int n, int threads) {
+    constexpr int kb = 64;
+    for (int k0 = 0; k0 < n; k0 += kb) {
+        const int k1 = std::min(n, k0 + kb);
+        for (int j = k0; j < k1; ++j) {
+            double *rj = &M[(size_t)j * n];
+            const double d = rj[j] - dot4(rj + k0, rj + k0, j - k0);
+            if (!(d > 0.0)) throw std::runtime_error("multigrid set-up: coarsest operator is not positive definite");
+            const double l = std::sqrt(d);
+            rj[j] = l;
+            for (int i = j + 1; i < k1; ++i) {
+                double *ri = &M[(size_t)i * n];
+                ri[j] = (ri[j] - dot4(ri + k0, rj + k0, j - k0)) / l;
+            }
+        }
+        parallel_items(n - k1, threads, [&](int t) {  // panel: L[i, k0:k1] for i >= k1
+            double *ri = &M[(size_t)(k1 + t) * n];
+            for (int j = k0; j < k1; ++j) {
+                const double *rj = &M[(size_t)j * n];
+                ri[j] = (ri[j] - dot4(ri + k0, rj + k0, j - k0)) / rj[j];
+            }
+        });
+        parallel_items(n - k1, threads, [&](int t) {  // trailing update of row i >= k1, columns k1..i
+            const int i = k1 + t;
+            double *ri = &M[(size_t)i * n];
+            for (int j = k1; j <= i; ++j) ri[j] -= dot4(ri + k0, &M[(size_t)j * n] + k0, k1 - k0);
+        });
+    }
 }
 
 // y = D^-1 A x (scalar diagonal)
@@ -409,34 +475,25 @@ std::vector<double> coarsest_inverse(const BCsr &A, const std::vector<double> &B
         for (int j = 0; j < n && zero; ++j) zero = M[(size_t)i * n + j] == 0.0;
         if (zero) M[(size_t)i * n + i] = 1.0;
     }
-    // Cholesky M = L L^T (lower triangle in place)
-    for (int j = 0; j < n; ++j) {
-        double d = M[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
-        if (!(d > 0.0)) throw std::runtime_error("multigrid set-up: coarsest operator is not positive definite");
-        const double l = std::sqrt(d);
-        M[(size_t)j * n + j] = l;
-        for (int i = j + 1; i < n; ++i) {
-            double s = M[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) s -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
-            M[(size_t)i * n + j] = s / l;
+    cholesky_blocked(M, n, threads);
+    // W = L^-1 by columns (column c solves L w = e_c, w zero above row c), then
+    // inv = W^T W: entry (i, j), j <= i, is the dot product of columns i and j from row i
+    std::vector<double> W((size_t)n * n, 0.0);
+    parallel_items(n, threads, [&](int c) {
+        double *w = &W[(size_t)c * n];
+        w[c] = 1.0 / M[(size_t)c * n + c];
+        for (int i = c + 1; i < n; ++i) {
+            const double *ri = &M[(size_t)i * n];
+            w[i] = -dot4(ri + c, w + c, i - c) / ri[i];
         }
-    }
+    });
     std::vector<double> inv((size_t)n * n, 0.0);
-    parallel_rows(n, threads, [&](int b, int e, int) {
-        std::vector<double> y(n);
-        for (int c = b; c < e; ++c) {  // column c of the inverse: L L^T y = e_c
-            for (int i = 0; i < n; ++i) {
-                double s = i == c ? 1.0 : 0.0;
-                for (int k = 0; k < i; ++k) s -= M[(size_t)i * n + k] * y[k];
-                y[i] = s / M[(size_t)i * n + i];
-            }
-            for (int i = n - 1; i >= 0; --i) {
-                double s = y[i];
-                for (int k = i + 1; k < n; ++k) s -= M[(size_t)k * n + i] * y[k];
-                y[i] = s / M[(size_t)i * n + i];
-            }
-            for (int i = 0; i < n; ++i) inv[(size_t)c * n + i] = y[i];
+    parallel_items(n, threads, [&](int i) {
+        const double *wi = &W[(size_t)i * n];
+        for (int j = 0; j <= i; ++j) {
+            const double v = dot4(wi + i, &W[(size_t)j * n] + i, n - i);
+            inv[(size_t)i * n + j] = v;
+            inv[(size_t)j * n + i] = v;
         }
     });
     return inv;

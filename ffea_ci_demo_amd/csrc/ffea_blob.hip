@@ -46,14 +46,14 @@
 
 using namespace ffk;
 
-// damped-Jacobi sweeps of the multigrid V-cycle on level 1 and on the coarser levels
-// (2 .. L-2), before and after the correction from the level below
-#ifndef FFEA_MG_SWEEPS_L1
-#define FFEA_MG_SWEEPS_L1 1
-#endif
-#ifndef FFEA_MG_SWEEPS_COARSER
-#define FFEA_MG_SWEEPS_COARSER 4
-#endif
+// The multigrid V-cycle's damped-Jacobi sweeps before and after the correction from the
+// level below: one on levels 0 and 1, except two on a single blob's level 1 of more than
+// kMgBigLevel1 aggregates, and kMgCoarserSweeps on levels 2 .. L-2
+constexpr int kMgBigLevel1 = 10000;
+constexpr int kMgCoarserSweeps = 4;
+// unknowns at which a single blob's coarsening stops: its coarsest level, up to this size,
+// is solved with a dense inverse (an ensemble's blobs keep amg::Options' default)
+constexpr int kMgMaxCoarseSingle = 4800;
 
 // Scratch of the allocation-free entry point (ffea_gpu_batch_strain_energy_noalloc): 64 MiB
 // per GPU, held by every GPU the library runs on until the process ends. Host code never
@@ -2086,6 +2086,7 @@ void ffea_gpu_blob::mg_setup() {
                     vmax[b] = max_abs(fine.v);
                     amg::Options opt;
                     opt.threads = nblob == 1 ? hw : 1;
+                    if (nblob == 1) opt.max_coarse = kMgMaxCoarseSingle;
                     // an ensemble's blobs coarsen at least once, so that every blob has a coarse
                     // level, and never to fewer than 4 aggregates (a one-aggregate level of a free
                     // blob is pure null space: with the FP32 matrix its rounding is not positive)
@@ -2256,8 +2257,12 @@ void ffea_gpu_blob::mg_setup() {
                 throw Invalid("internal error: coarsest inverse size");
             coff_row[b] = off[L - 1][b] * 6;
             cm[b] = m;
-            ccoff[b] = (long long)ci.size();
-            ci.insert(ci.end(), H[b].cinv.begin(), H[b].cinv.end());
+            ccoff[b] = (long long)ci.size();  // a multiple of 4 floats: every row starts 16-byte aligned
+            const int ld = dense_ld(m);
+            for (int i = 0; i < m; ++i) {
+                ci.insert(ci.end(), H[b].cinv.begin() + (size_t)i * m, H[b].cinv.begin() + (size_t)(i + 1) * m);
+                ci.insert(ci.end(), (size_t)(ld - m), 0.0f);
+            }
             row_blob.insert(row_blob.end(), m, b);
         }
         mg_cinv.alloc(ci.size(), &dev_bytes);
@@ -2336,7 +2341,8 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     // correction (the first pre-sweep from zero is the restriction's y2); the sweeps
     // alternate between two of the level's vectors, and pre[l] / out[l] record where the
     // pre-smoothed iterate and the level's correction end up
-    auto sweeps = [](int l) { return l == 1 ? FFEA_MG_SWEEPS_L1 : FFEA_MG_SWEEPS_COARSER; };
+    const int sweeps1 = nblob == 1 && L > 1 && mg[1]->n > kMgBigLevel1 ? 2 : 1;
+    auto sweeps = [&](int l) { return l == 1 ? sweeps1 : kMgCoarserSweeps; };
     std::vector<double *> pre(L, nullptr), out(L, nullptr);
     auto post_sweep = [&](MgLevel &lv, const double *x, double *y) {
         MgCsrArgs po{S0, lv.n, lv.A.rp.p, lv.A.ci.p, lv.A.val.p, lv.A.mscale, x, lv.b.p, lv.dinv.p, lv.omega, y,
