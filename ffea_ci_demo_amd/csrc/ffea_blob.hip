@@ -55,6 +55,21 @@ constexpr int kMgCoarserSweeps = 4;
 // is solved with a dense inverse (an ensemble's blobs keep amg::Options' default)
 constexpr int kMgMaxCoarseSingle = 4800;
 
+// Set-up options of one blob's hierarchy in a handle of nblob blobs, the blob having nl
+// corner nodes. Every blob coarsens at least once, so that it has a coarse level (a
+// single blob of up to kMgMaxCoarseSingle unknowns would otherwise stop at level 0); an
+// ensemble's blobs keep amg::Options' coarse size and never coarsen to fewer than 4
+// aggregates (a one-aggregate level of a free blob is pure null space: with the FP32
+// matrix its rounding is not positive).
+static amg::Options mg_options(int nblob, int nl, int threads) {
+    amg::Options opt;
+    opt.threads = threads;
+    if (nblob == 1) opt.max_coarse = kMgMaxCoarseSingle;
+    opt.max_coarse = std::min(opt.max_coarse, std::max(6, 3 * nl / 8));
+    if (nblob > 1) opt.min_aggregates = 4;
+    return opt;
+}
+
 // Scratch of the allocation-free entry point (ffea_gpu_batch_strain_energy_noalloc): 64 MiB
 // per GPU, held by every GPU the library runs on until the process ends. Host code never
 // uses this name as an address: each device's copy is found once with hipGetSymbolAddress
@@ -2084,16 +2099,7 @@ void ffea_gpu_blob::mg_setup() {
                 try {
                     const amg::BCsr fine = fine_of(b);
                     vmax[b] = max_abs(fine.v);
-                    amg::Options opt;
-                    opt.threads = nblob == 1 ? hw : 1;
-                    if (nblob == 1) opt.max_coarse = kMgMaxCoarseSingle;
-                    // an ensemble's blobs coarsen at least once, so that every blob has a coarse
-                    // level, and never to fewer than 4 aggregates (a one-aggregate level of a free
-                    // blob is pure null space: with the FP32 matrix its rounding is not positive)
-                    if (nblob > 1) {
-                        opt.max_coarse = std::min(opt.max_coarse, std::max(6, 3 * meta[b].nl / 8));
-                        opt.min_aggregates = 4;
-                    }
+                    const amg::Options opt = mg_options(nblob, meta[b].nl, nblob == 1 ? hw : 1);
                     const int ro = meta[b].row_off;
                     H[b] = amg::build(fine, h_pos.data() + 3 * (size_t)ro, h_pin.data() + ro, meta[b].free_blob, opt);
                 } catch (const std::runtime_error &e) {
@@ -4013,8 +4019,9 @@ int ffea_gpu_multigrid_inspect(int num_rows, const int *rowptr, const int *col, 
         A.rp.assign(rowptr, rowptr + num_rows + 1);
         A.ci.assign(col, col + rowptr[num_rows]);
         A.v.assign(val, val + (size_t)rowptr[num_rows] * 9);
-        amg::Options opt;
-        opt.threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+        // the hierarchy a single-blob handle builds for this matrix
+        const amg::Options opt =
+            mg_options(1, num_rows, std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
         const auto t0 = std::chrono::steady_clock::now();
         amg::Hierarchy H;
         try {

@@ -91,3 +91,17 @@ def test_bad_input_is_refused():
     with pytest.raises(RuntimeError):
         _native.multigrid_inspect(np.array([0, 1], np.int32), np.array([5], np.int32), np.eye(3).reshape(1, 9),
                                   np.zeros((1, 3)))
+
+
+@pytest.mark.parametrize("n", [10, 19])
+def test_single_blob_hierarchy_has_a_coarse_level(oracle_mod, n):
+    """the inspected hierarchy is the one a single-blob handle builds: a blob small enough
+    for a dense coarsest level (1331 corner nodes, 3993 unknowns) still coarsens once,
+    and a larger one (8000 nodes) coarsens until its coarsest level is at most 4800
+    unknowns"""
+    spec = workloads.config4(n=n, rng_seed=23)
+    B, x, pin, K = corner_bsr(oracle_mod, spec)
+    d, P, A1 = inspect(B, x, pin, True)
+    assert d["num_levels"] >= 2
+    assert d["coarsest_unknowns"] <= min(4800, 3 * len(x) // 8)
+    assert d["coarsest_unknowns"] == d["rows"][d["num_levels"] - 1] * d["block"][d["num_levels"] - 1]
