@@ -9,6 +9,7 @@
 // damped-Jacobi step; coarse operators are Galerkin products; the coarsest level is
 // inverted densely (regularised on the rigid modes for a free blob).
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -49,7 +50,12 @@ struct Options {
 // no couplings); pos: [nl][3] positions; pinned: [nl] flags (may be null); free_blob:
 // the operator is singular with the 6 rigid modes as null space.
 // Throws std::runtime_error on inconsistent input.
-Hierarchy build(const BCsr &fine, const double *pos, const unsigned char *pinned, bool free_blob,
-                const Options &opt);
+Hierarchy build(BCsr fine, const double *pos, const unsigned char *pinned, bool free_blob, const Options &opt);
+
+// contiguous ranges of n rows for up to `threads` threads (bounds[t] .. bounds[t+1])
+std::vector<int> split_rows(int n, int threads);
+// fn(begin, end, t) over the ranges, one thread each; fn must not throw
+void parallel_ranges(const std::vector<int> &bounds, const std::function<void(int, int, int)> &fn);
+void parallel_rows(int n, int threads, const std::function<void(int, int, int)> &fn);
 
 }  // namespace amg

@@ -1929,12 +1929,21 @@ double fp16_scale(double maxabs) {
     (void)std::frexp(maxabs, &e);
     return std::ldexp(1.0, 10 - e);
 }
-double max_abs(const std::vector<double> &v) {
+double max_abs(const std::vector<double> &v, int threads) {
+    const int nb = (int)std::min<size_t>(v.size() / 9 + 1, (size_t)1 << 30);  // ranges of whole 9-value groups
+    const std::vector<int> bounds = amg::split_rows(nb, threads);
+    std::vector<double> part(bounds.size() - 1, 0.0);
+    amg::parallel_ranges(bounds, [&](int b0, int b1, int t) {
+        double m = 0.0;
+        const size_t e = std::min(v.size(), (size_t)b1 * 9);
+        for (size_t i = (size_t)b0 * 9; i < e; ++i) m = std::max(m, std::fabs(v[i]));
+        part[t] = m;
+    });
     double m = 0.0;
-    for (double x : v) m = std::max(m, std::fabs(x));
+    for (double x : part) m = std::max(m, x);
     return m;
 }
-EllHost to_ell(const amg::BCsr &A, double scale) {
+EllHost to_ell(const amg::BCsr &A, double scale, int threads) {
     EllHost E;
     const int ns = (A.n + 63) / 64, bb = A.br * A.bc;
     E.sw.assign(std::max(1, ns), 1);
@@ -1947,18 +1956,21 @@ EllHost to_ell(const amg::BCsr &A, double scale) {
         E.sb[s] = slots;
         slots += (long long)w * 64;
     }
+    for (long long k = 0; k < A.nnzb(); ++k)
+        if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
     E.col.assign(std::max<long long>(1, slots), -1);
     E.val.assign((size_t)std::max<long long>(1, slots) * bb, (_Float16)0.0f);
-    for (int i = 0; i < A.n; ++i) {
-        const int s = i / 64, lane = i % 64;
-        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const long long slot = E.sb[s] + (long long)(k - A.rp[i]) * 64;
-            if (A.ci[k] < 0 || A.ci[k] >= A.ncols) throw Invalid("internal error: multigrid block column out of range");
-            E.col[slot + lane] = A.ci[k];
-            for (int e = 0; e < bb; ++e)
-                E.val[ell_val_index(slot, bb, lane, e)] = (_Float16)(A.v[(size_t)k * bb + e] * scale);
+    amg::parallel_rows(ns, threads, [&](int s0, int s1, int) {  // whole slices per range
+        for (int i = s0 * 64; i < std::min(A.n, s1 * 64); ++i) {
+            const int s = i / 64, lane = i % 64;
+            for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const long long slot = E.sb[s] + (long long)(k - A.rp[i]) * 64;
+                E.col[slot + lane] = A.ci[k];
+                for (int e = 0; e < bb; ++e)
+                    E.val[ell_val_index(slot, bb, lane, e)] = (_Float16)(A.v[(size_t)k * bb + e] * scale);
+            }
         }
-    }
+    });
     return E;
 }
 }  // namespace
@@ -2052,39 +2064,66 @@ void ffea_gpu_blob::mg_setup() {
         FFEA_HIP_TRY(hipMemcpyAsync(h_val.data(), d_val.p, sizeof(double) * h_val.size(), hipMemcpyDeviceToHost, stream));
         FFEA_HIP_TRY(hipStreamSynchronize(stream));
     }
-    // blob b's corner-node matrix in block CSR over its own rows
-    auto fine_of = [&](int b) {
+    // blob b's corner-node matrix in block CSR over its own rows (slice padding left out),
+    // rows in parallel ranges on `threads` threads: a count pass, then a fill pass
+    auto fine_of = [&](int b, int threads) {
         const BlobMeta &m = meta[b];
         const int nl = m.nl, ro = m.row_off;
         amg::BCsr fine;
         fine.n = fine.ncols = nl;
         fine.br = fine.bc = 3;
         fine.rp.assign(nl + 1, 0);
-        std::vector<std::pair<int, int>> ent;  // (local column, slot j)
-        for (int n = 0; n < nl; ++n) {
-            const int g = ro + n, s = g / kSlice, lane = g % kSlice;
-            ent.clear();
-            for (int j = 0; j < h_sw[s]; ++j) {
-                const int c = h_col[h_sb[s] + (long long)j * kSlice + lane];
-                bool nz = false;
-                for (int k = 0; k < 9; ++k)
-                    nz |= h_val[mixed ? val_index<float>(h_sb[s], j, lane, k) : val_index<double>(h_sb[s], j, lane, k)] !=
-                          0.0;
-                if (!nz) continue;  // slice padding
-                if (c < ro || c >= ro + nl) throw Invalid("internal error: matrix column outside the blob");
-                ent.push_back({c - ro, j});
+        auto value = [&](int s, int j, int lane, int k) {
+            return h_val[mixed ? val_index<float>(h_sb[s], j, lane, k) : val_index<double>(h_sb[s], j, lane, k)];
+        };
+        auto live = [&](int s, int j, int lane) {
+            bool nz = false;
+            for (int k = 0; k < 9; ++k) nz |= value(s, j, lane, k) != 0.0;
+            return nz;
+        };
+        const std::vector<int> bounds = amg::split_rows(nl, threads);
+        amg::parallel_ranges(bounds, [&](int n0, int n1, int) {
+            for (int n = n0; n < n1; ++n) {
+                const int g = ro + n, s = g / kSlice, lane = g % kSlice;
+                int cnt = 0;
+                for (int j = 0; j < h_sw[s]; ++j) cnt += live(s, j, lane) ? 1 : 0;
+                fine.rp[n + 1] = cnt;
             }
-            std::sort(ent.begin(), ent.end());
-            for (size_t t = 1; t < ent.size(); ++t)
-                if (ent[t].first == ent[t - 1].first) throw Invalid("internal error: repeated matrix column");
-            fine.rp[n + 1] = fine.rp[n] + (int)ent.size();
-            for (auto &e : ent) {
-                fine.ci.push_back(e.first);
-                for (int k = 0; k < 9; ++k)
-                    fine.v.push_back(h_val[mixed ? val_index<float>(h_sb[s], e.second, lane, k)
-                                                 : val_index<double>(h_sb[s], e.second, lane, k)]);
+        });
+        for (int n = 0; n < nl; ++n) fine.rp[n + 1] += fine.rp[n];
+        fine.ci.resize(fine.rp[nl]);
+        fine.v.resize((size_t)fine.rp[nl] * 9);
+        std::atomic<int> bad{0};  // 1: a column outside the blob, 2: a repeated column
+        amg::parallel_ranges(bounds, [&](int n0, int n1, int) {
+            std::vector<std::pair<int, int>> ent;  // (local column, slot j)
+            for (int n = n0; n < n1; ++n) {
+                const int g = ro + n, s = g / kSlice, lane = g % kSlice;
+                ent.clear();
+                for (int j = 0; j < h_sw[s]; ++j) {
+                    if (!live(s, j, lane)) continue;
+                    const int c = h_col[h_sb[s] + (long long)j * kSlice + lane];
+                    if (c < ro || c >= ro + nl) {
+                        bad = 1;
+                        return;
+                    }
+                    ent.push_back({c - ro, j});
+                }
+                std::sort(ent.begin(), ent.end());
+                for (size_t t = 1; t < ent.size(); ++t)
+                    if (ent[t].first == ent[t - 1].first) {
+                        bad = 2;
+                        return;
+                    }
+                int d = fine.rp[n];
+                for (auto &e : ent) {
+                    fine.ci[d] = e.first;
+                    for (int k = 0; k < 9; ++k) fine.v[(size_t)d * 9 + k] = value(s, e.second, lane, k);
+                    ++d;
+                }
             }
-        }
+        });
+        if (bad == 1) throw Invalid("internal error: matrix column outside the blob");
+        if (bad == 2) throw Invalid("internal error: repeated matrix column");
         return fine;
     };
     const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -2097,11 +2136,13 @@ void ffea_gpu_blob::mg_setup() {
         auto work = [&]() {
             for (int b = next++; b < nblob; b = next++) {
                 try {
-                    const amg::BCsr fine = fine_of(b);
-                    vmax[b] = max_abs(fine.v);
-                    const amg::Options opt = mg_options(nblob, meta[b].nl, nblob == 1 ? hw : 1);
+                    const int threads = nblob == 1 ? hw : 1;
+                    amg::BCsr fine = fine_of(b, threads);
+                    vmax[b] = max_abs(fine.v, threads);
+                    const amg::Options opt = mg_options(nblob, meta[b].nl, threads);
                     const int ro = meta[b].row_off;
-                    H[b] = amg::build(fine, h_pos.data() + 3 * (size_t)ro, h_pin.data() + ro, meta[b].free_blob, opt);
+                    H[b] = amg::build(std::move(fine), h_pos.data() + 3 * (size_t)ro, h_pin.data() + ro,
+                                      meta[b].free_blob, opt);
                 } catch (const std::runtime_error &e) {
                     errors[b] = (nblob > 1 ? fmt("blob %d: ", b) : std::string()) + e.what();
                 } catch (const Invalid &e) {
@@ -2160,9 +2201,17 @@ void ffea_gpu_blob::mg_setup() {
     mg_res32.reset();
     for (DevBuf<int> *bf : {&mg_crow_blob, &mg_coff_row, &mg_cm, &mg_free}) bf->reset();
     mg_ccoff.reset();
+    // one matrix of the handle's level: a single blob's own, or the blobs' side by side
+    auto level_matrix = [&](const std::vector<const amg::BCsr *> &part, const std::vector<int> &row_off,
+                            const std::vector<int> &col_off, int nr, int nc, amg::BCsr &keep) -> const amg::BCsr & {
+        if (part.size() == 1 && row_off[0] == 0 && col_off[0] == 0 && part[0]->n == nr && part[0]->ncols == nc)
+            return *part[0];
+        keep = side_by_side(part, row_off, col_off, nr, nc);
+        return keep;
+    };
     auto up_ell = [&](MgEll &D, const amg::BCsr &A) {
-        const double sc = fp16_scale(max_abs(A.v));
-        const EllHost E = to_ell(A, sc);
+        const double sc = fp16_scale(max_abs(A.v, hw));
+        const EllHost E = to_ell(A, sc, hw);
         D.n = A.n;
         D.mscale = 1.0 / sc;
         D.sw.alloc(E.sw.size(), &dev_bytes); D.sw.upload(E.sw, stream);
@@ -2177,16 +2226,18 @@ void ffea_gpu_blob::mg_setup() {
         // vector-major rows (k_mg_csr): vector t of block j of row i at (rp[i] BB) + (t W + j) V
         const int BB = A.br * A.bc, V = BB % 4 == 0 ? 4 : 2;
         if (BB % V) throw Invalid("internal error: multigrid block size");
-        const double sc = fp16_scale(max_abs(A.v));
+        const double sc = fp16_scale(max_abs(A.v, hw));
         std::vector<_Float16> v(A.v.size());
-        for (int i = 0; i < A.n; ++i) {
-            const int r0 = A.rp[i], W = A.rp[i + 1] - r0;
-            for (int j = 0; j < W; ++j)
-                for (int t = 0; t < BB / V; ++t)
-                    for (int u = 0; u < V; ++u)
-                        v[(size_t)r0 * BB + ((size_t)t * W + j) * V + u] =
-                            (_Float16)(A.v[((size_t)r0 + j) * BB + t * V + u] * sc);
-        }
+        amg::parallel_rows(A.n, hw, [&](int i0, int i1, int) {
+            for (int i = i0; i < i1; ++i) {
+                const int r0 = A.rp[i], W = A.rp[i + 1] - r0;
+                for (int j = 0; j < W; ++j)
+                    for (int t = 0; t < BB / V; ++t)
+                        for (int u = 0; u < V; ++u)
+                            v[(size_t)r0 * BB + ((size_t)t * W + j) * V + u] =
+                                (_Float16)(A.v[((size_t)r0 + j) * BB + t * V + u] * sc);
+            }
+        });
         D.n = A.n;
         D.mscale = 1.0 / sc;
         D.rp.alloc(A.rp.size(), &dev_bytes); D.rp.upload(A.rp, stream);
@@ -2213,7 +2264,8 @@ void ffea_gpu_blob::mg_setup() {
                     throw Invalid("internal error: coarse blocks are not 6x6");
             std::vector<const amg::BCsr *> parts;
             for (int b = 0; b < nblob; ++b) parts.push_back(&H[b].lv[l].A);
-            if (l < L - 1) up_csr(lev->A, side_by_side(parts, off[l], off[l], nlev[l], nlev[l]));
+            amg::BCsr keep;
+            if (l < L - 1) up_csr(lev->A, level_matrix(parts, off[l], off[l], nlev[l], nlev[l], keep));
             const size_t nv = (size_t)nlev[l] * 6;
             std::vector<double> dv(nv, 0.0);
             for (int b = 0; b < nblob; ++b) {
@@ -2240,8 +2292,9 @@ void ffea_gpu_blob::mg_setup() {
                 rr.push_back(&hl.R);
                 tb += hl.P.nnzb();
             }
-            up_ell(lev->P, side_by_side(pp, off[l], off[l + 1], nlev[l], nlev[l + 1]));
-            up_csr(lev->R, side_by_side(rr, off[l + 1], off[l], nlev[l + 1], nlev[l]));
+            amg::BCsr keep;
+            up_ell(lev->P, level_matrix(pp, off[l], off[l + 1], nlev[l], nlev[l + 1], keep));
+            up_csr(lev->R, level_matrix(rr, off[l + 1], off[l], nlev[l + 1], nlev[l], keep));
             mg_info.transfer_blocks[l] = tb;
         }
         mg_info.rows[l] = l == 0 ? total_nl : nlev[l];
