@@ -258,7 +258,7 @@ struct BlobPlan {
     long long nslots() const { return sb.empty() ? 0 : sb.back(); }
 };
 
-BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
+BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise, int threads) {
     BlobPlan P;
     const int nt = in.nt, ne = in.ne;
     const int *top = in.top;
@@ -361,7 +361,6 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
     };
     P.nslices = (nl + kSlice - 1) / kSlice;
     P.sw.assign(P.nslices, 0);
-    std::vector<Nb> tmp;
     auto row_entries = [&](int n, std::vector<Nb> &out) {
         out.clear();
         if (P.pin_lin[n]) return;
@@ -375,22 +374,39 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
         }
         std::stable_sort(out.begin(), out.end(), [](const Nb &x, const Nb &y) { return x.m < y.m; });
     };
-    for (int n = 0; n < nl; ++n) {
+    // three passes over the rows, each over whole slices in parallel ranges (every row
+    // writes only its own lane of its slice's slots)
+    std::vector<int> sbounds(threads + 1);
+    for (int t = 0; t <= threads; ++t) sbounds[t] = (int)((long long)P.nslices * t / threads);
+    auto over_slices = [&](const std::function<void(int, int, std::vector<Nb> &)> &row) {
+        amg::parallel_ranges(sbounds, [&](int s0, int s1, int) {
+            std::vector<Nb> tmp;
+            for (int n = s0 * kSlice; n < std::min(nl, s1 * kSlice); ++n) row(n, s1, tmp);
+        });
+    };
+    std::atomic<int> crowded{-1};  // the lowest row with more than 253 neighbours
+    over_slices([&](int n, int, std::vector<Nb> &tmp) {
         row_entries(n, tmp);
         int distinct = 0;
         for (size_t t = 0; t < tmp.size(); ++t)
             if (t == 0 || tmp[t].m != tmp[t - 1].m) ++distinct;
         if (distinct == 0) distinct = 1;  // identity diagonal
-        if (distinct > 254) throw Invalid(who + fmt("node %d has more than 253 neighbours", P.orig_of_lin[n]));
+        if (distinct > 254) {  // the lowest such row is reported
+            int cur = crowded.load();
+            while ((cur < 0 || n < cur) && !crowded.compare_exchange_weak(cur, n)) {
+            }
+        }
         P.sw[n / kSlice] = std::max(P.sw[n / kSlice], distinct);
-    }
+    });
+    if (crowded >= 0) throw Invalid(who + fmt("node %d has more than 253 neighbours", P.orig_of_lin[crowded]));
     P.sb.assign(P.nslices + 1, 0);
     for (int s = 0; s < P.nslices; ++s) P.sb[s + 1] = P.sb[s] + (long long)P.sw[s] * kSlice;
     const long long nslots = P.nslots();
     P.col.assign(nslots, 0);
     P.cnt.assign(nslots + 1, 0);
     P.dj.assign(nl, 0);
-    for (int n = 0; n < nl; ++n) {
+    std::vector<long long> row_blocks(nl, 0);
+    over_slices([&](int n, int, std::vector<Nb> &tmp) {
         const int s = n / kSlice, lane = n % kSlice;
         row_entries(n, tmp);
         const long long base = P.sb[s];
@@ -413,18 +429,19 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
                 t = t1;
             }
         }
-        P.nblocks += j;
+        row_blocks[n] = j;
         for (; j < P.sw[s]; ++j) P.col[base + (long long)j * kSlice + lane] = n;  // padding: zero block
-    }
+    });
+    for (int n = 0; n < nl; ++n) P.nblocks += row_blocks[n];
     for (long long k = 0; k < nslots; ++k) P.cnt[k + 1] += P.cnt[k];
     P.contrib.assign(P.cnt[nslots], 0);
-    for (int n = 0; n < nl; ++n) {
+    over_slices([&](int n, int, std::vector<Nb> &tmp) {
         const int s = n / kSlice, lane = n % kSlice;
         row_entries(n, tmp);
         const long long base = P.sb[s];
         if (tmp.empty()) {
             P.contrib[P.cnt[base + lane]] = kIdentity;
-            continue;
+            return;
         }
         size_t t = 0;
         int j = 0;
@@ -434,7 +451,7 @@ BlobPlan make_plan(const BlobInput &in, int blob_index, bool stokes_noise) {
             while (t < tmp.size() && tmp[t].m == P.col[slot]) P.contrib[w++] = tmp[t++].code;
             ++j;
         }
-    }
+    });
     // ---- constant forces and the linearise_force increments of midside ones
     std::vector<double> ctf_total((size_t)3 * nt, 0.0);
     std::vector<unsigned char> has_ctf(nt, 0);
@@ -740,7 +757,9 @@ Tables make_tables(const std::vector<BlobInput> &in, bool mixed, const ffea_gpu_
         auto work = [&]() {
             for (int b = next++; b < nblob; b = next++) {
                 try {
-                    plans[b] = make_plan(in[b], nblob > 1 ? b : -1, t.stokes_noise);
+                    // a single blob's plan takes the host threads itself
+                    plans[b] = make_plan(in[b], nblob > 1 ? b : -1, t.stokes_noise,
+                                         nblob > 1 ? 1 : std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
                 } catch (const Invalid &e) {
                     errors[b] = e.msg;
                 } catch (const std::bad_alloc &) {
