@@ -330,6 +330,7 @@ struct MgFineArgs {
     const double *omega_row = nullptr;  // an ensemble's smoother weight per row (else omega)
     const float *x32 = nullptr;         // FP32-mixed form: x's packed FP32 copy, gathered
     float *y32 = nullptr;               // FP32-mixed form: y's packed FP32 copy (the residual, z)
+    const float *binv = nullptr;        // POST: the rows' diagonal-block inverses (B^-1 in place of D^-1)
 };
 
 // blocks whose loads one row issues together (slice_row_times)
@@ -359,18 +360,28 @@ __global__ __launch_bounds__(BS) void k_mg_fine(MgFineArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) acc[k] *= a.mscale;
         double z3[3];
+        if constexpr (MODE == MG_FINE_RESID) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const size_t o = vi(n, k);
-            const double rk = a.r[o];
-            if constexpr (MODE == MG_FINE_RESID) {
-                z3[k] = rk - omega * acc[k];
+            for (int k = 0; k < 3; ++k) {
+                const size_t o = vi(n, k);
+                z3[k] = a.r[o] - omega * acc[k];
                 a.y[o] = z3[k];
-            } else {
-                const double zk = a.x[o] + omega * a.dinv[o] * (rk - acc[k]);
+            }
+        } else {
+            double r3[3], d3[3], s3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                r3[k] = a.r[vi(n, k)];
+                d3[k] = r3[k] - acc[k];
+            }
+            jacobi_times(a.dinv, a.binv, n, d3, s3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const size_t o = vi(n, k);
+                const double zk = a.x[o] + omega * s3[k];
                 a.y[o] = zk;
                 z3[k] = zk;
-                v[0] += rk * zk;
+                v[0] += r3[k] * zk;
             }
         }
         if (a.y32) store_f32x3(a.y32, n, z3);

@@ -132,10 +132,12 @@ void cholesky_blocked(std::vector<double> &M, int n, int threads) {
     }
 }
 
-// largest eigenvalue of D^-1 A by power iteration from a fixed start vector; the norms
+// largest eigenvalue of M^-1 A by power iteration from a fixed start vector, M the scalar
+// diagonal (dinv) or, with binv (3x3 blocks only: [n][9]), the block diagonal; the norms
 // are sums of partial sums over fixed chunks of rows, so the estimate does not depend on
 // the thread count
-double power_lmax(const BCsr &A, const std::vector<double> &dinv, int iters, int threads) {
+double power_lmax(const BCsr &A, const std::vector<double> &dinv, int iters, int threads,
+                  const std::vector<double> *binv = nullptr) {
     const int br = A.br, bc = A.bc;
     const size_t n = (size_t)A.n * br;
     constexpr int kRows = 4096;
@@ -144,9 +146,9 @@ double power_lmax(const BCsr &A, const std::vector<double> &dinv, int iters, int
     for (size_t i = 0; i < n; ++i) v[i] = dinv[i] != 0.0 ? 1.0 + 0.5 * std::sin(0.7 * (double)i) : 0.0;
     double lam = 0.0;
     for (int it = 0; it < iters; ++it) {
-        parallel_items(nchunk, threads, [&](int c) {  // w = D^-1 A v and the chunk's |v|^2, |w|^2
-            double sv = 0.0, sw = 0.0;
-            for (int i = c * kRows; i < std::min(A.n, (c + 1) * kRows); ++i)
+        parallel_items(nchunk, threads, [&](int c) {  // w = M^-1 A v and the chunk's |v|^2, |w|^2
+            double sv = 0.0, sw = 0.0, t[8];
+            for (int i = c * kRows; i < std::min(A.n, (c + 1) * kRows); ++i) {
                 for (int p = 0; p < br; ++p) {
                     double acc = 0.0;
                     for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
@@ -154,11 +156,20 @@ double power_lmax(const BCsr &A, const std::vector<double> &dinv, int iters, int
                         const double *xx = &v[(size_t)A.ci[k] * bc];
                         for (int q = 0; q < bc; ++q) acc += blk[q] * xx[q];
                     }
+                    t[p] = acc;
+                }
+                for (int p = 0; p < br; ++p) {
                     const size_t r = (size_t)i * br + p;
-                    w[r] = dinv[r] * acc;
+                    if (binv) {
+                        const double *bi = &(*binv)[(size_t)i * 9 + (size_t)p * 3];
+                        w[r] = bi[0] * t[0] + bi[1] * t[1] + bi[2] * t[2];
+                    } else {
+                        w[r] = dinv[r] * t[p];
+                    }
                     sv += v[r] * v[r];
                     sw += w[r] * w[r];
                 }
+            }
             pv[c] = sv;
             pw[c] = sw;
         });
@@ -175,6 +186,34 @@ double power_lmax(const BCsr &A, const std::vector<double> &dinv, int iters, int
         });
     }
     return lam;
+}
+
+// inverses of the 3x3 diagonal blocks, symmetrised ([n][9]; the inverse diagonal where a
+// block is not positive definite): the level-0 smoother's block diagonal
+std::vector<double> block_inverse3(const BCsr &A) {
+    std::vector<double> bi((size_t)A.n * 9, 0.0);
+    for (int i = 0; i < A.n; ++i)
+        for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            if (A.ci[k] != i) continue;
+            const double *m = &A.v[(size_t)k * 9];
+            const double a = m[0], d = m[4], f = m[8];
+            const double b = 0.5 * (m[1] + m[3]), c = 0.5 * (m[2] + m[6]), e = 0.5 * (m[5] + m[7]);
+            const double A0 = d * f - e * e, B0 = c * e - b * f, C0 = b * e - c * d;
+            const double det = a * A0 + b * B0 + c * C0;
+            double *o = &bi[(size_t)i * 9];
+            if (det > 0.0 && a > 0.0 && a * d - b * b > 0.0) {
+                const double s = 1.0 / det;
+                const double u[6] = {A0 * s, B0 * s, C0 * s, (a * f - c * c) * s, (b * c - a * e) * s, (a * d - b * b) * s};
+                o[0] = u[0]; o[1] = u[1]; o[2] = u[2];
+                o[3] = u[1]; o[4] = u[3]; o[5] = u[4];
+                o[6] = u[2]; o[7] = u[4]; o[8] = u[5];
+            } else {
+                o[0] = a > 0.0 ? 1.0 / a : 0.0;
+                o[4] = d > 0.0 ? 1.0 / d : 0.0;
+                o[8] = f > 0.0 ? 1.0 / f : 0.0;
+            }
+        }
+    return bi;
 }
 
 std::vector<double> inverse_diagonal(const BCsr &A) {
@@ -621,9 +660,15 @@ Hierarchy build(BCsr fine_in, const double *pos, const unsigned char *pinned, bo
     for (int l = 0;; ++l) {
         Level &L = H.lv[l];
         L.dinv = inverse_diagonal(L.A);
-        const double est = power_lmax(L.A, L.dinv, 30, opt.threads);
-        L.lmax = est;
-        const double lhat = est * (l == 0 ? opt.smoother_safety : 1.1);
+        const double est = power_lmax(L.A, L.dinv, 30, opt.threads);  // D^-1 A: the prolongator's smoothing
+        // level 0 smooths with the inverses of its 3x3 diagonal blocks B, its weight from B^-1 A
+        double est_s = est;
+        if (l == 0) {
+            const std::vector<double> bi = block_inverse3(L.A);
+            est_s = power_lmax(L.A, L.dinv, 30, opt.threads, &bi);
+        }
+        L.lmax = est_s;
+        const double lhat = est_s * (l == 0 ? opt.smoother_safety : 1.1);
         L.omega = 2.0 / (lhat * (1.0 + 1.0 / 30.0));
         const int unknowns = L.A.n * L.A.br;
         if (unknowns <= opt.max_coarse || l + 1 >= opt.max_levels) break;

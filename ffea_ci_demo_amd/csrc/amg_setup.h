@@ -26,8 +26,9 @@ struct BCsr {
 struct Level {
     BCsr A;                    // operator (level 0: a copy of the fine matrix)
     std::vector<double> dinv;  // inverse scalar diagonal (0 for dead unknowns)
-    double lmax = 0.0;         // power-iteration estimate of the largest eigenvalue of D^-1 A
-    double omega = 0.0;        // smoother weight: x += omega D^-1 (b - A x)
+    double lmax = 0.0;         // power-iteration estimate of the largest eigenvalue of M^-1 A, M the
+                               // smoother's diagonal: the 3x3 diagonal blocks on level 0, D above
+    double omega = 0.0;        // smoother weight: x += omega M^-1 (b - A x)
     BCsr P, R;                 // transfer to the next level and its transpose (not on the coarsest)
 };
 

@@ -1360,6 +1360,7 @@ struct ffea_gpu_blob {
     DevBuf<float> mg_cinv;
     int mg_nc = 0;
     DevBuf<_Float16> mg_hval;  // scaled FP16 copy of the viscosity matrix, written by the assembly
+    DevBuf<float> mg_binv;     // [nrows][6] level-0 smoother: inverses of the diagonal blocks, written by the assembly
     double mg_hscale = 1.0;
     DevBuf<double> mg_res0, mg_xb0, mg_rp, mg_coef;
     DevBuf<float> mg_xb32, mg_res32;  // FP32-mixed form: packed FP32 copies of mg_xb0, mg_res0
@@ -1369,6 +1370,9 @@ struct ffea_gpu_blob {
     bool mg_any_free = false;
     ffea_gpu_multigrid_info mg_info{};
     bool mg_on() const { return precond == FFEA_GPU_PRECOND_MULTIGRID; }
+    // the CG's z = B^-1 r under the multigrid preconditioner (its pre-smoothed level-0
+    // iterate, up to omega); null: Jacobi-PCG's z = D^-1 r
+    const float *mg_smoother() const { return mg_on() ? (const float *)mg_binv.p : nullptr; }
 
     // timing
     unsigned timing = 0;  // bit i: time phase i
@@ -1809,10 +1813,12 @@ void ffea_gpu_blob::launch_gather(const T *force) {
                     pinned.p, dinv.p, pos.p, centre.p, rhs.p, r.p, z.p, x.p, PROJ ? Pg.p : Pi.p,
                     mass && stokes ? vel.p : nullptr, mass && stokes ? drag.p : nullptr, rhs_in};
     g.z32 = z32.p;
+    g.binv = mg_smoother();
     hipLaunchKernelGGL((k_gather<T, PROJ>), dim3(nb), dim3(row_block<T>), 0, stream, g);
     if (PROJ) {
         StartArgs sa{row_args(), Pg.p, rhs.p, pos.p, dinv.p, centre.p, project.p, r.p, z.p, x.p, Pi.p, Gmat.p};
         sa.z32 = z32.p;
+        sa.binv = mg_smoother();
         hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(kMixedBlock), 0, stream, sa);
     }
 }
@@ -1837,7 +1843,7 @@ void ffea_gpu_blob::launch_assemble() {
     t_begin(1, &t0);
     if (mixed) {
         AsmArgs<float> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, f_geo.p,
-                         f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
+                         f_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale, mg_binv.p};
         if (asm_lanes() == kAsmStaged)
             hipLaunchKernelGGL(k_assemble_staged<float>, st_grid(), dim3(kSlice), 0, stream, a, st_ptr.p, st_el.p, st_loc.p);
         else if (asm_lanes() == 1) hipLaunchKernelGGL((k_assemble<float, 1>), asm_grid(), dim3(kBlock), 0, stream, a);
@@ -1845,7 +1851,7 @@ void ffea_gpu_blob::launch_assemble() {
         else hipLaunchKernelGGL((k_assemble<float, 16>), asm_grid(), dim3(kBlock), 0, stream, a);
     } else {
         AsmArgs<double> a{nrows, ne, slice_w.p, slice_base.p, cptr.p, contrib.p, diag_j.p, d_geo.p,
-                          d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale};
+                          d_val.p, dinv.p, stokes ? drag.p : nullptr, mg_hval.p, mg_hscale, mg_binv.p};
         if (asm_lanes() == kAsmStaged)
             hipLaunchKernelGGL(k_assemble_staged<double>, st_grid(), dim3(kSlice), 0, stream, a, st_ptr.p, st_el.p,
                                st_loc.p);
@@ -1880,6 +1886,7 @@ void ffea_gpu_blob::launch_cg_vec(int it) {
     hipEvent_t tv = nullptr;
     if (t_cg(4, it)) t_begin(4, &tv);
     VecArgs va{row_args(), it, p.p, q.p, dinv.p, pos.p, centre.p, Gmat.p, x.p, r.p, z.p, P1.p, P2.p, z32.p};
+    va.binv = mg_smoother();
     if (mixed && any_project) {
         hipLaunchKernelGGL((k_cg_vec<true, kMixedBlock>), dim3(nb), dim3(kMixedBlock), 0, stream, va);
     } else {
@@ -2215,6 +2222,7 @@ void ffea_gpu_blob::mg_setup() {
     mg.clear();
     mg_cinv.reset();
     mg_hval.reset();
+    mg_binv.reset();
     for (DevBuf<double> *bf : {&mg_res0, &mg_xb0, &mg_rp, &mg_coef, &mg_omega0}) bf->reset();
     mg_xb32.reset();
     mg_res32.reset();
@@ -2358,7 +2366,9 @@ void ffea_gpu_blob::mg_setup() {
     // ensemble, the rigid-mode gauge of free blobs
     mg_hscale = fp16_scale(hscale_max);
     mg_hval.alloc((size_t)nslots * 9, &dev_bytes);
-    launch_assemble();  // fills the FP16 copy at the current positions
+    mg_binv.alloc((size_t)nrows * 6, &dev_bytes);
+    mg_binv.zero(stream);
+    launch_assemble();  // fills the FP16 copy and the block inverses at the current positions
     if (nblob > 1) {
         std::vector<double> om(nrows, 0.0);
         for (int b = 0; b < nblob; ++b)
@@ -2471,7 +2481,7 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
               0.0, f.omega, mg_xb0.p, nullptr, om0, mg_xb32.p};
     hipLaunchKernelGGL((k_mg_apply<3, 6, MG_PROLONG>), grid(f.n), dim3(kMgBlock), 0, stream, p0);
     MgFineArgs<_Float16> fp{S.p, chunks.p, nchunks, slice_w.p, slice_base.p, col.p, mg_hval.p, hms, mg_xb0.p, r.p,
-                            dinv.p, f.omega, z.p, Pout, om0, mg_xb32.p, z32.p};
+                            dinv.p, f.omega, z.p, Pout, om0, mg_xb32.p, z32.p, mg_binv.p};
     if (mixed) {
         if (ens) hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kMgBlock, float>), dim3(nchunks), dim3(kMgBlock), 0, stream, fp);
         else hipLaunchKernelGGL((k_mg_fine<_Float16, MG_FINE_POST, kRowBlock, float>), dim3(nchunks), dim3(kRowBlock), 0, stream, fp);
@@ -2990,6 +3000,7 @@ void ffea_gpu_blob::launch_adjoint_step(const double *pos_k, const double *vel_k
     launch_element();  // shape data of x_k for the operator
     launch_assemble();
     AdjStartArgs st{row_args(), lam.p, pinned.p, dinv.p, rhs.p, r.p, z.p, x.p, Pi.p, z32.p};
+    st.binv = mg_smoother();
     hipLaunchKernelGGL(k_adjoint_start, dim3(nb), dim3(kRowBlock), 0, stream, st);
     FFEA_HIP_TRY(hipGetLastError());
     if (mg_on()) launch_mg_vcycle(Pi.p);  // z = M r of the start; its r.z replaces row 0
@@ -4051,6 +4062,7 @@ int ffea_gpu_blob_set_preconditioner(ffea_gpu_blob *b, int kind) {
             b->mg.clear();
             b->mg_cinv.reset();
             b->mg_hval.reset();
+            b->mg_binv.reset();
             for (auto *bf : {&b->mg_res0, &b->mg_xb0, &b->mg_rp, &b->mg_coef, &b->mg_omega0}) bf->reset();
             b->mg_xb32.reset();
             b->mg_res32.reset();

@@ -540,6 +540,52 @@ __global__ __launch_bounds__(kBlock) void k_node_noise(NoiseArgs a) {
 }
 
 // ---------------------------------------------------------------- matrix assembly
+// The multigrid smoother's inverse of a row's 3x3 diagonal block B: the symmetric inverse
+// (adjugate over determinant) of the symmetrised block, stored as 6 FP32 values (xx, xy,
+// xz, yy, yz, zz) per row; where B is not positive definite, the inverses of its diagonal.
+__device__ __forceinline__ void store_block_inverse(float *binv, int n, const double (&m)[9]) {
+    const double a = m[0], d = m[4], f = m[8];
+    const double b = 0.5 * (m[1] + m[3]), c = 0.5 * (m[2] + m[6]), e = 0.5 * (m[5] + m[7]);
+    const double A = d * f - e * e, B = c * e - b * f, C = b * e - c * d;
+    const double det = a * A + b * B + c * C;
+    float o[6];
+    if (det > 0.0 && a > 0.0 && a * d - b * b > 0.0 && det < 1e300) {
+        const double s = 1.0 / det;
+        o[0] = (float)(A * s);
+        o[1] = (float)(B * s);
+        o[2] = (float)(C * s);
+        o[3] = (float)((a * f - c * c) * s);
+        o[4] = (float)((b * c - a * e) * s);
+        o[5] = (float)((a * d - b * b) * s);
+    } else {
+        o[0] = (float)(1.0 / a);
+        o[1] = o[2] = o[4] = 0.0f;
+        o[3] = (float)(1.0 / d);
+        o[5] = (float)(1.0 / f);
+    }
+    float2 *dst = (float2 *)(binv + 6 * (size_t)n);
+    dst[0] = make_float2(o[0], o[1]);
+    dst[1] = make_float2(o[2], o[3]);
+    dst[2] = make_float2(o[4], o[5]);
+}
+
+// z = D^-1 r (Jacobi), or z = B^-1 r where the multigrid preconditioner's block smoother
+// is on (binv: the rows' stored block inverses, else null)
+__device__ __forceinline__ void jacobi_times(const double *dinv, const float *binv, int n, const double (&r)[3],
+                                             double (&z)[3]) {
+    if (binv) {
+        const float2 *src = (const float2 *)(binv + 6 * (size_t)n);
+        const float2 u = src[0], v = src[1], w = src[2];
+        const double xx = u.x, xy = u.y, xz = v.x, yy = v.y, yz = w.x, zz = w.y;
+        z[0] = xx * r[0] + xy * r[1] + xz * r[2];
+        z[1] = xy * r[0] + yy * r[1] + yz * r[2];
+        z[2] = xz * r[0] + yz * r[1] + zz * r[2];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) z[k] = dinv[vi(n, k)] * r[k];
+    }
+}
+
 template <typename T>
 struct AsmArgs {
     int nrows, ne;
@@ -554,6 +600,7 @@ struct AsmArgs {
     const double *drag;          // [nrows] Stokes drag of unpinned rows, or null
     _Float16 *hval;              // [nslots*9] the multigrid preconditioner's scaled copy, or null
     double hscale;               // its scale
+    float *binv = nullptr;       // [nrows][6] the multigrid smoother's diagonal-block inverses, or null
 };
 
 // the first 12 T of a shape record in 16-byte loads
@@ -621,6 +668,12 @@ __device__ __forceinline__ void store_block(const AsmArgs<T> &a, int n, int j, i
     if (j == a.diag_j[n]) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) a.dinv[vi(n, i)] = 1.0 / (double)(T)acc[4 * i];
+        if (a.binv) {
+            double m[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[k] = (double)(T)acc[k];
+            store_block_inverse(a.binv, n, m);
+        }
     }
 }
 
@@ -1051,6 +1104,7 @@ struct GatherArgs {
     const double *rhs_in = nullptr;     // tiled [rows][3]: a right-hand side handed over whole
                                         // (the solver's own entry point), or null: formed here
     float *z32 = nullptr;               // FP32-mixed form: z's packed FP32 copy
+    const float *binv = nullptr;        // multigrid block smoother: z = B^-1 r (jacobi_times)
 };
 
 // Blob::aggregate_forces_and_solve (Blob.cpp:3152-3238) for corner nodes, then the
@@ -1147,11 +1201,12 @@ __global__ __launch_bounds__(row_block<T>) void k_gather(GatherArgs<T> a) {
             for (int k = 0; k < 3; ++k) a.rhs[vi(n, k)] = f[k];
             if constexpr (!PROJ) {
                 double zz[3];
+                const double f3[3] = {f[0], f[1], f[2]};
+                jacobi_times(a.dinv, a.binv, n, f3, zz);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     a.r[vi(n, k)] = f[k];
                     a.x[vi(n, k)] = 0.0;
-                    zz[k] = a.dinv[vi(n, k)] * f[k];
                     a.z[vi(n, k)] = zz[k];
                 }
                 if (a.z32) store_f32x3(a.z32, n, zz);
@@ -1191,6 +1246,7 @@ struct StartArgs {
     const int *project;         // [nblob]: 1 = free blob, project; 0 = leave r = f
     double *r, *z, *x, *P, *G;  // G: [nblob][36] ((Z^T Z)^-1, zero for unprojected blobs)
     float *z32 = nullptr;       // FP32-mixed form: z's packed FP32 copy
+    const float *binv = nullptr;  // multigrid block smoother: z = B^-1 r (jacobi_times)
 };
 __global__ __launch_bounds__(kMixedBlock) void k_cg_start(StartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
@@ -1237,9 +1293,10 @@ __global__ __launch_bounds__(kMixedBlock) void k_cg_start(StartArgs a) {
                 rl[k] = a.rhs[vi(n, k)] - zc[k];
                 a.r[vi(n, k)] = rl[k];
                 a.x[vi(n, k)] = 0.0;
-                zl[k] = a.dinv[vi(n, k)] * rl[k];
-                a.z[vi(n, k)] = zl[k];
             }
+            jacobi_times(a.dinv, a.binv, n, rl, zl);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a.z[vi(n, k)] = zl[k];
             if (a.z32) store_f32x3(a.z32, n, zl);
             v[0] += rl[0] * zl[0] + rl[1] * zl[1] + rl[2] * zl[2];
         }
@@ -1559,6 +1616,7 @@ struct VecArgs {
     const double *P1;  // chunk partials of the SpMV step
     double *P2;        // r.r, r.z partials of this step
     float *z32 = nullptr;  // FP32-mixed form: z's packed FP32 copy
+    const float *binv = nullptr;  // multigrid block smoother: z = B^-1 r (jacobi_times)
 };
 
 // alpha = delta / p.q, x += alpha p, r -= alpha q (mat_vec_fns.cpp:187-197),
@@ -1630,9 +1688,10 @@ __device__ __forceinline__ void cg_vec(const VecArgs &a, const ChunkRange cr) {
                 a.x[vi(n, k)] += alpha * a.p[vi(n, k)];
                 r3[k] = a.r[vi(n, k)] + (-alpha) * qn[k];
                 a.r[vi(n, k)] = r3[k];
-                z3[k] = a.dinv[vi(n, k)] * r3[k];
-                a.z[vi(n, k)] = z3[k];
             }
+            jacobi_times(a.dinv, a.binv, n, r3, z3);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) a.z[vi(n, k)] = z3[k];
             if (a.z32) store_f32x3(a.z32, n, z3);
             v[0] += r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
             v[1] += r3[0] * z3[0] + r3[1] * z3[1] + r3[2] * z3[2];
@@ -2084,6 +2143,7 @@ struct AdjStartArgs {
     const double *dinv;
     double *rhs, *r, *z, *x, *P;
     float *z32 = nullptr;         // FP32-mixed form: z's packed FP32 copy
+    const float *binv = nullptr;  // multigrid block smoother: z = B^-1 r (jacobi_times)
 };
 __global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
     const ChunkRange cr = my_chunks(a.ra.nchunks, a.ra.cpb);
@@ -2093,16 +2153,17 @@ __global__ __launch_bounds__(kRowBlock) void k_adjoint_start(AdjStartArgs a) {
         double v[2] = {0.0, 0.0};
         for (int n = ch.begin + threadIdx.x; n < ch.end; n += kRowBlock) {
             const bool pin = a.pinned[n] != 0;
-            double z3[3];
+            double f3[3], z3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f3[k] = pin ? 0.0 : a.lam[vi(n, k)];
+            jacobi_times(a.dinv, a.binv, n, f3, z3);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const double f = pin ? 0.0 : a.lam[vi(n, k)];
-                const double zz = a.dinv[vi(n, k)] * f;
+                const double f = f3[k], zz = z3[k];
                 a.rhs[vi(n, k)] = f;
                 a.r[vi(n, k)] = f;
                 a.x[vi(n, k)] = 0.0;
                 a.z[vi(n, k)] = zz;
-                z3[k] = zz;
                 v[0] += f * zz;
                 v[1] += f * f;
             }

@@ -4,8 +4,9 @@ matrices the oracle assembles in the reference's summation order.
 What pins it: the coarse operator is the Galerkin product P0^T K P0 (recomputed here
 with scipy); the prolongator reproduces the 6 rigid modes exactly (smoothing keeps any
 vector K annihilates), so a free blob's coarse operator annihilates their coarse
-representation; pinned rows (identity blocks) get no coarse unknowns; the smoother
-weight follows the largest eigenvalue of D^-1 K (scipy's eigensolver).
+representation; pinned rows (identity blocks) get no coarse unknowns; level 0's smoother
+weight follows the largest eigenvalue of Bd^-1 K, Bd its 3x3 diagonal blocks (scipy's
+eigensolver).
 """
 import numpy as np
 import pytest
@@ -66,10 +67,16 @@ def test_free_blob_galerkin_and_rigid_modes(oracle_mod, n):
     Bc = np.linalg.lstsq(P.toarray(), Z, rcond=None)[0]
     assert np.abs(P @ Bc - Z).max() <= 1e-10 * np.abs(Z).max()
     assert np.abs(A1 @ Bc).max() <= 1e-10 * np.abs(A1).max() * np.abs(Bc).max()
-    # smoother weight from the largest eigenvalue of D^-1 K
-    Dm = sp.diags(1.0 / np.sqrt(K.diagonal()))
-    lam = sla.eigsh(Dm @ K @ Dm, k=1, which="LA", return_eigenvectors=False)[0]
+    # level 0 smooths with its 3x3 diagonal blocks Bd: the weight follows the largest
+    # eigenvalue of Bd^-1 K (K x = lam Bd x, scipy's eigensolver), below that of D^-1 K
+    Bd = sp.block_diag([B.data[k] for i in range(B.shape[0] // 3) for k in range(B.indptr[i], B.indptr[i + 1])
+                        if B.indices[k] == i]).tocsc()
+    Bd = 0.5 * (Bd + Bd.T)
+    lam = sla.eigsh(K, M=Bd, k=1, which="LA", return_eigenvectors=False)[0]
     assert 0.95 * lam <= d["lmax"][0] <= 1.0001 * lam
+    Dm = sp.diags(1.0 / np.sqrt(K.diagonal()))
+    lam_d = sla.eigsh(Dm @ K @ Dm, k=1, which="LA", return_eigenvectors=False)[0]
+    assert lam < lam_d
     assert d["omega"][0] * 1.3 * d["lmax"][0] * (1 + 1 / 30) == pytest.approx(2.0)
 
 
