@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <exception>
 #include <functional>
 #include <stdexcept>
 #include <thread>
@@ -19,16 +20,27 @@ std::vector<int> split_rows(int n, int threads) {
 }
 
 // fn(begin, end, t) over the ranges [bounds[t], bounds[t+1]), one thread each; results
-// that each range writes to its own rows are independent of the thread count
+// that each range writes to its own rows are independent of the thread count. An
+// exception in a range is rethrown here, on the caller's thread, once all ranges ended.
 void parallel_ranges(const std::vector<int> &bounds, const std::function<void(int, int, int)> &fn) {
     const int nt = (int)bounds.size() - 1;
     if (nt == 1) {
         fn(bounds[0], bounds[1], 0);
         return;
     }
+    std::vector<std::exception_ptr> err(nt);
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t) pool.emplace_back(fn, bounds[t], bounds[t + 1], t);
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            try {
+                fn(bounds[t], bounds[t + 1], t);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
     for (auto &th : pool) th.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // fn(begin, end, t) over contiguous row ranges on up to `threads` threads
@@ -44,16 +56,24 @@ namespace {
 void parallel_items(int count, int threads, const std::function<void(int)> &fn) {
     const int nt = std::max(1, std::min(threads, count));
     std::atomic<int> next{0};
-    auto work = [&]() {
-        for (int i = next++; i < count; i = next++) fn(i);
-    };
     if (nt == 1) {
-        work();
+        for (int i = 0; i < count; ++i) fn(i);
         return;
     }
+    std::vector<std::exception_ptr> err(nt);  // rethrown on the caller's thread
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t) pool.emplace_back(work);
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            try {
+                for (int i = next++; i < count; i = next++) fn(i);
+            } catch (...) {
+                err[t] = std::current_exception();
+                next = count;  // the other threads stop taking items
+            }
+        });
     for (auto &th : pool) th.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 typedef double v2d __attribute__((vector_size(16)));

@@ -55,7 +55,8 @@ Hierarchy build(BCsr fine, const double *pos, const unsigned char *pinned, bool 
 
 // contiguous ranges of n rows for up to `threads` threads (bounds[t] .. bounds[t+1])
 std::vector<int> split_rows(int n, int threads);
-// fn(begin, end, t) over the ranges, one thread each; fn must not throw
+// fn(begin, end, t) over the ranges, one thread each; the first exception a range throws
+// is rethrown on the caller's thread after every range has ended
 void parallel_ranges(const std::vector<int> &bounds, const std::function<void(int, int, int)> &fn);
 void parallel_rows(int n, int threads, const std::function<void(int, int, int)> &fn);
 
