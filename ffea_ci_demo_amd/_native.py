@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libffea_gpu.so")
 
 OK, ERR_INVERSION, ERR_CG_NO_CONVERGE, ERR_INVALID, ERR_HIP, ERR_NO_DEVICE, ERR_IO = range(7)
 FP64, MIXED = 0, 1
+WORLD_NO_DETAIL = 1  # ffea_world_create_ex: no detailed measurement file (FFEA_WORLD_NO_DETAIL)
 NUM_PHASES = 6
 PHASES = ("element", "assemble", "gather_cg_start", "cg_spmv", "cg_vector", "update")
 
@@ -31,7 +32,7 @@ EXPORTED = [
     "ffea_gpu_rng_stream_state", "ffea_gpu_advance_rng_state", "ffea_gpu_blob_set_row_blocks",
     "ffea_gpu_blob_autotune", "ffea_gpu_blob_use_graph", "ffea_gpu_check_layout", "ffea_gpu_blob_set_material",
     "ffea_gpu_blob_run_checkpointed", "ffea_gpu_blob_adjoint", "ffea_gpu_batch_strain_energy",
-    "ffea_world_create", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
+    "ffea_world_create", "ffea_world_create_ex", "ffea_world_run", "ffea_world_current_step", "ffea_world_blob", "ffea_world_destroy",
     "ffea_world_last_error", "ffea_gpu_blob_set_preconditioner", "ffea_gpu_blob_get_multigrid_info",
     "ffea_gpu_multigrid_inspect", "ffea_gpu_blob_create_ex", "ffea_gpu_blob_add_node_forces",
     "ffea_gpu_blob_get_node_positions", "ffea_gpu_blob_set_solver", "ffea_gpu_blob_set_velocities",
@@ -159,6 +160,7 @@ def lib():
     L.ffea_gpu_multigrid_inspect.argtypes = [C.c_int, _ip, _ip, _dp, _dp, vp, C.c_int, C.POINTER(MultigridInfo),
                                              vp, vp, vp, vp, vp, vp]
     L.ffea_world_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
+    L.ffea_world_create_ex.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.ffea_world_run.argtypes = [vp, C.c_longlong]
     L.ffea_world_current_step.argtypes = [vp]
     L.ffea_world_current_step.restype = C.c_longlong

@@ -167,7 +167,7 @@ struct Params {
     double ssint_cutoff = 3e-9 / meso::length;
     // output and checkpoint files (SimulationParams.cpp:682-749, 972-990), as written: the
     // script's directory is prepended when the runner resolves them
-    std::string traj_out, meas_out, ckpt_in, ckpt_out;
+    std::string traj_out, meas_out, det_out, ckpt_in, ckpt_out;
 };
 
 // defaults of SimulationParams::SimulationParams (SimulationParams.cpp:26-117)
@@ -229,6 +229,7 @@ void assign(Params &P, const std::string &lv, const std::string &rv) {
     else if (lv == "ssint_cutoff" || lv == "vdw_cutoff") P.ssint_cutoff = atof(r) / meso::length;
     else if (lv == "trajectory_out_fname") P.traj_out = rv;
     else if (lv == "measurement_out_fname") P.meas_out = rv;
+    else if (lv == "det_measurement_out_fname") P.det_out = rv;
     else if (lv == "checkpoint_in") P.ckpt_in = rv;
     else if (lv == "checkpoint_out") P.ckpt_out = rv;
     else {
@@ -237,7 +238,7 @@ void assign(Params &P, const std::string &lv, const std::string &rv) {
             "sticky_wall_xz", "es_h", "kappa", "dielec_ext", "epsilon_0", "calc_steric_rod", "calc_vdw_rod",
             "pbc_rod", "inc_self_ssint", "inc_self_vdw", "ssint_type", "vdw_type", "force_pbc", "steric_factor",
             "steric_dr", "flow_profile", "shear_rate", "flow_velocity", "wall_x_1", "wall_x_2", "wall_y_1",
-            "wall_y_2", "wall_z_1", "wall_z_2", "trajectory_out_fname", "det_measurement_out_fname",
+            "wall_y_2", "wall_z_1", "wall_z_2", "trajectory_out_fname",
             "measurement_out_fname", "kinetics_out_fname", "vdw_in_fname", "ssint_in_fname",
             "vdw_forcefield_params", "ssint_forcefield_params", "ljparams", "lj_params", "rod_lj_in_fname",
             "rod_vdw_forcefield_params", "rod_lj_params", "checkpoint_in", "checkpoint_out", "beads_out_fname",
@@ -863,6 +864,19 @@ bool ffea_internal_script_files(const ffea_script *s, ScriptFiles *out) {
     out->script = s->path;
     out->trajectory = P.traj_out.empty() ? dir + "/" + stem + "/.ftj" : join(dir, P.traj_out);
     out->measurement = P.meas_out.empty() ? dir + "/" + stem + "/.fm" : join(dir, P.meas_out);
+    // the detailed measurement file (SimulationParams.cpp:690-709): det_measurement_out_fname,
+    // else the measurement file's path with its extension replaced by .fdm; with neither
+    // given the reference's name stays "\n" and no detailed file is written here
+    if (!P.det_out.empty()) {
+        out->detailed = join(dir, P.det_out);
+    } else if (!P.meas_out.empty()) {
+        const std::string &m = out->measurement;
+        const size_t sl = m.find_last_of('/'), d = m.find_last_of('.');
+        const bool ext = d != std::string::npos && (sl == std::string::npos ? d > 0 : d > sl + 1);
+        out->detailed = (ext ? m.substr(0, d) : m) + ".fdm";
+    } else {
+        out->detailed.clear();
+    }
     out->checkpoint_in = P.ckpt_in.empty() ? std::string() : join(dir, P.ckpt_in);
     if (!P.ckpt_out.empty()) out->checkpoint_out = join(dir, P.ckpt_out);
     else out->checkpoint_out = (dot == std::string::npos ? s->path : s->path.substr(0, s->path.size() - (base.size() - dot))) + ".fcp";

@@ -1,6 +1,6 @@
 // The World loop of an FFEA script over GPU blob handles: trajectory (.ftj), global
-// measurement (.fm) and checkpoint (.fcp) files in the reference's formats, and restart
-// from them. One handle per blob; each blob keeps its own positions on the GPU and the
+// measurement (.fm), detailed measurement (.fdm) and checkpoint (.fcp) files in the
+// reference's formats, and restart from them. One handle per blob; each blob keeps its own positions on the GPU and the
 // host only sees them at output steps.
 #include <cerrno>
 #include <cmath>
@@ -130,6 +130,8 @@ struct ffea_world {
     std::vector<long long> draws_before;   // noise draws of the blobs before this one in a step
     std::vector<std::string> old_seeds;     // extra seed lines of an earlier multi-threaded run
     FILE *traj = nullptr, *meas = nullptr, *ckpt = nullptr;
+    FILE *det = nullptr;   // the detailed measurement file (.fdm), unless turned off (the reference's -d)
+    bool detailed = true;
     long long step = 0, step_initial = 0;
     std::vector<double> host;
     std::string err;
@@ -154,7 +156,7 @@ struct ffea_world {
     std::vector<std::vector<double>> spring_force;
 
     ~ffea_world() {
-        for (FILE *f : {traj, meas, ckpt})
+        for (FILE *f : {traj, meas, ckpt, det})
             if (f) fclose(f);
         for (ffea_gpu_blob *b : blobs) ffea_gpu_blob_destroy(b);
     }
@@ -238,9 +240,8 @@ struct ffea_world {
     }
 
     // the spring part of World::make_measurements (World.cpp:4632-4652): every spring, active
-    // or not, adds 0.5 k (|x0 - x1| - l)^2 to its (blob 0, blob 1) cell; the cells are summed
-    // row by row (World::get_spring_field_energy, World.cpp:4110-4123)
-    double spring_energy() {
+    // or not, adds 0.5 k (|x0 - x1| - l)^2 to its (blob 0, blob 1) cell [nb][nb]
+    std::vector<double> spring_cells() {
         const int nb = (int)blobs.size();
         std::vector<std::vector<int>> want(nb);
         for (const Spring &sp : springs)
@@ -257,9 +258,22 @@ struct ffea_world {
             m = std::sqrt(m);
             cell[(size_t)sp.blob[0] * nb + sp.blob[1]] += 0.5 * sp.k * (m - sp.l) * (m - sp.l);
         }
+        return cell;
+    }
+
+    // the whole field: the cells summed row by row (World::get_spring_field_energy(-1, -1),
+    // World.cpp:4110-4123)
+    static double spring_total(const std::vector<double> &cell) {
         double e = 0.0;
         for (double v : cell) e += v;
         return e;
+    }
+
+    // a blob with a spring at either end (World::load_springs, World.cpp:3596-3597)
+    bool has_springs(int b) const {
+        for (const Spring &sp : springs)
+            if (sp.blob[0] == b || sp.blob[1] == b) return true;
+        return false;
     }
 
     // World::init headers (World.cpp:527-603)
@@ -309,6 +323,30 @@ struct ffea_world {
         if (p.calc_springs != 0) fprintf(meas, "%-14s", "SpringEnergy");
         fprintf(meas, "\n");
         fflush(meas);
+        if (det) write_detailed_header();
+    }
+
+    // the detailed measurement file's header (World.cpp:606-646): per blob its kinetic energy
+    // (blobs with mass), strain energy, centroid and RMSD; per blob pair (i <= j) its spring
+    // energy where both blobs carry springs, the pair block only with springs on (surface
+    // interactions and PreComp are off on this path)
+    void write_detailed_header() {
+        fprintf(det, "FFEA Detailed Measurement File\n\nMeasurements:\n");
+        fprintf(det, "%-14s", "Time");
+        for (int i = 0; i < p.num_blobs; ++i) {
+            fprintf(det, "| B%d ", i);
+            if (inertial[i]) fprintf(det, "%-14s", "KineticEnergy");
+            fprintf(det, "%-14s", "StrainEnergy");
+            fprintf(det, "%-14s%-14s%-14s%-14s", "Centroid[0]", "Centroid[1]", "Centroid[2]", "RMSD");
+        }
+        if (p.calc_springs == 1)
+            for (int i = 0; i < p.num_blobs; ++i)
+                for (int j = i; j < p.num_blobs; ++j) {
+                    fprintf(det, "| B%dB%d ", i, j);
+                    if (has_springs(i) && has_springs(j)) fprintf(det, "%-14s", "SpringEnergy");
+                }
+        fprintf(det, "\n");
+        fflush(det);
     }
 
     // World::print_checkpoints (World.cpp:4546-4596): the noise stream's state before this
@@ -329,6 +367,9 @@ struct ffea_world {
     void print_outputs(long long s) {
         double strain = 0.0, kinetic = 0.0, cog[3] = {0, 0, 0}, rmsd2 = 0.0;
         long long nodes = 0;
+        // per blob (Blob::make_measurements): kinetic and strain energy, centroid, RMSD
+        std::vector<double> b_ke(p.num_blobs, 0.0), b_strain(p.num_blobs, 0.0), b_cog(3 * (size_t)p.num_blobs, 0.0),
+            b_rmsd(p.num_blobs, 0.0);
         std::vector<double> vel, frc;
         for (int b = 0; b < p.num_blobs; ++b) {
             host.resize(3 * (size_t)nt[b]);
@@ -340,6 +381,7 @@ struct ffea_world {
                 double ke = 0.0;
                 check_rc(ffea_gpu_blob_kinetic_energy(blobs[b], &ke), b, "kinetic energy");
                 kinetic += ke;
+                b_ke[b] = ke;
             }
             fprintf(traj, "Blob %d, Conformation %d, step %d\n", b, 0, (int)s);
             fprintf(traj, "DYNAMIC\n");
@@ -366,6 +408,9 @@ struct ffea_world {
             strain += e;
             const double rb = std::sqrt(r / nt[b]);
             rmsd2 += rb * rb * nt[b];
+            b_strain[b] = e;
+            b_rmsd[b] = rb;
+            for (int k = 0; k < 3; ++k) b_cog[3 * (size_t)b + k] = c[k] / nt[b];
             for (int k = 0; k < 3; ++k) cog[k] += c[k] / nt[b] * nt[b];
             nodes += nt[b];
         }
@@ -382,9 +427,37 @@ struct ffea_world {
         fprintf(meas, "%-14.6e", strain * kEnergy);
         fprintf(meas, "%-14.6e%-14.6e%-14.6e", cog[0] * kLength, cog[1] * kLength, cog[2] * kLength);
         fprintf(meas, "%-14.6e", rmsd * kLength);
-        if (p.calc_springs != 0) fprintf(meas, "%-14.6e", spring_energy() * kEnergy);
+        std::vector<double> cell;
+        if (p.calc_springs != 0) cell = spring_cells();
+        if (p.calc_springs != 0) fprintf(meas, "%-14.6e", spring_total(cell) * kEnergy);
         fprintf(meas, "\n");
         fflush(meas);
+        if (!det) return;
+        // the detailed line (World.cpp:4504-4537, Blob::write_measurements_to_file
+        // Blob.cpp:1204-1213, World::write_detailed_measurements_to_file World.cpp:4694-4721):
+        // time, each blob's block, then each pair's block, whose blank lead is written whether
+        // or not the header has pair columns
+        fprintf(det, "%-14.6e", (int)s * p.dt * kTime);
+        for (int b = 0; b < p.num_blobs; ++b) {
+            fprintf(det, "     ");
+            if (inertial[b]) fprintf(det, "%-14.6e", b_ke[b] * kEnergy);
+            fprintf(det, "%-14.6e", b_strain[b] * kEnergy);
+            fprintf(det, "%-14.6e%-14.6e%-14.6e%-14.6e", b_cog[3 * (size_t)b] * kLength,
+                    b_cog[3 * (size_t)b + 1] * kLength, b_cog[3 * (size_t)b + 2] * kLength, b_rmsd[b] * kLength);
+        }
+        const int nb = p.num_blobs;
+        for (int i = 0; i < nb; ++i)
+            for (int j = i; j < nb; ++j) {
+                fprintf(det, "       ");
+                if (p.calc_springs != 0 && has_springs(i) && has_springs(j)) {
+                    // World::get_spring_field_energy(i, j) (World.cpp:4127-4136)
+                    const double e = i == j ? cell[(size_t)i * nb + i]
+                                            : cell[(size_t)i * nb + j] + cell[(size_t)j * nb + i];
+                    fprintf(det, "%-14.6e", e * kEnergy);
+                }
+            }
+        fprintf(det, "\n");
+        fflush(det);
     }
 
     // restart (World.cpp:302-426): the checkpoint's first thermal stream becomes the stream;
@@ -503,24 +576,35 @@ struct ffea_world {
         step_initial = rstep;
         truncate_to(files.trajectory, cut);
 
-        // the measurement file loses its last line, the step that is repeated (World.cpp:838-866)
-        const std::string m = read_whole(files.measurement);
+        // the measurement files lose their last line, the step that is repeated
+        // (World.cpp:838-866, detailed: 867-912)
+        drop_last_line(files.measurement);
+        if (detailed) drop_last_line(files.detailed);
+    }
+
+    // from the end, back to the newline before the last character, and cut there
+    static void drop_last_line(const std::string &path) {
+        const std::string m = read_whole(path);
         long long mp = (long long)m.size();
         for (;;) {
-            if (mp - 2 < 0) fail(FFEA_GPU_ERR_IO, "Error when trying to find last frame from file " + files.measurement);
+            if (mp - 2 < 0) fail(FFEA_GPU_ERR_IO, "Error when trying to find last frame from file " + path);
             mp -= 2;
             const char c = m[(size_t)mp];
             mp += 1;
             if (c == '\n') break;
         }
-        truncate_to(files.measurement, mp);
+        truncate_to(path, mp);
     }
 };
 
 extern "C" {
 
 int ffea_world_create(const char *path, int precision, int device, ffea_world **out) {
-    if (!path || !out) {
+    return ffea_world_create_ex(path, precision, device, 0, out);
+}
+
+int ffea_world_create_ex(const char *path, int precision, int device, int flags, ffea_world **out) {
+    if (!path || !out || (flags & ~FFEA_WORLD_NO_DETAIL) != 0) {
         ffea_internal_set_error("ffea_world_create: invalid argument");
         return FFEA_GPU_ERR_INVALID;
     }
@@ -530,9 +614,11 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
     if (rc != FFEA_GPU_OK) return rc;
     ffea_world *w = new ffea_world();
     w->precision = precision;
+    w->detailed = (flags & FFEA_WORLD_NO_DETAIL) == 0;
     try {
         ffea_script_get_params(s, &w->p);
         ffea_internal_script_files(s, &w->files);
+        if (w->files.detailed.empty()) w->detailed = false;  // no name for it (see ScriptFiles)
         const ffea_script_params &p = w->p;
         // SimulationParams::validate (SimulationParams.cpp:805-1009), the file checks
         if (p.restart != 0 && p.restart != 1)
@@ -606,6 +692,7 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
         move_aside(w->files.checkpoint_out);
         if (p.restart == 0) {
             move_aside(w->files.measurement);
+            if (w->detailed) move_aside(w->files.detailed);
             move_aside(w->files.trajectory);
         }
         if (p.restart == 1) w->read_checkpoint();
@@ -616,6 +703,8 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
                 fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.trajectory);
             if (!(w->meas = fopen(w->files.measurement.c_str(), "wb")))
                 fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.measurement);
+            if (w->detailed && !(w->det = fopen(w->files.detailed.c_str(), "wb")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.detailed);
             w->write_headers(path);
         } else {
             w->restart_trajectory();
@@ -623,6 +712,8 @@ int ffea_world_create(const char *path, int precision, int device, ffea_world **
                 fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.trajectory);
             if (!(w->meas = fopen(w->files.measurement.c_str(), "ab")))
                 fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.measurement);
+            if (w->detailed && !(w->det = fopen(w->files.detailed.c_str(), "ab")))
+                fail(FFEA_GPU_ERR_IO, "failed to open " + w->files.detailed);
         }
         w->step = w->step_initial;
     } catch (const WorldError &e) {

@@ -11,6 +11,8 @@ struct ScriptFiles {
     std::string measurement;     // measurement_out_fname, or <dir>/<stem>/.fm (:977-980)
     std::string checkpoint_in;   // checkpoint_in ("" if not set)
     std::string checkpoint_out;  // checkpoint_out, or the script path with .fcp (:984-990)
+    std::string detailed;        // det_measurement_out_fname, or the measurement file with .fdm for its
+                                 // extension when measurement_out_fname is given (:690-709); "" otherwise
 };
 
 bool ffea_internal_script_files(const ffea_script *s, ScriptFiles *out);

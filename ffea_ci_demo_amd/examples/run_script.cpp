@@ -1,7 +1,8 @@
 // `ffea <script>` for the mass-less blob path (ffea.cpp:53-308): every blob of an .ffea
-// script on the GPU through the World loop, writing the script's trajectory, measurement
-// and checkpoint files (restart = 1 resumes from them).
-//   run_script <script.ffea> [max_updates] [fp64|mixed]
+// script on the GPU through the World loop, writing the script's trajectory, measurement,
+// detailed measurement and checkpoint files (restart = 1 resumes from them); -d writes no
+// detailed measurement file, as ffea -d.
+//   run_script [-d] <script.ffea> [max_updates] [fp64|mixed]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -11,14 +12,20 @@
 #include "ffea_gpu.hpp"
 
 int main(int argc, char **argv) {
-    if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <script.ffea> [max_updates] [fp64|mixed]\n", argv[0]);
+    bool detailed = true;
+    std::vector<char *> args;  // the positional arguments
+    for (int i = 1; i < argc; ++i) {
+        if (std::strcmp(argv[i], "-d") == 0 || std::strcmp(argv[i], "--no-detail") == 0) detailed = false;
+        else args.push_back(argv[i]);
+    }
+    if (args.empty()) {
+        std::fprintf(stderr, "usage: %s [-d] <script.ffea> [max_updates] [fp64|mixed]\n", argv[0]);
         return 2;
     }
     try {
-        const long long max_updates = argc > 2 ? std::atoll(argv[2]) : -1;
-        const int precision = (argc > 3 && std::strcmp(argv[3], "mixed") == 0) ? FFEA_GPU_MIXED : FFEA_GPU_FP64;
-        ffea_gpu::World world(argv[1], precision);
+        const long long max_updates = args.size() > 1 ? std::atoll(args[1]) : -1;
+        const int precision = (args.size() > 2 && std::strcmp(args[2], "mixed") == 0) ? FFEA_GPU_MIXED : FFEA_GPU_FP64;
+        ffea_gpu::World world(args[0], precision, 0, detailed);
         const long long first = world.step();
         const auto t0 = std::chrono::steady_clock::now();
         world.run(max_updates);

@@ -1,8 +1,9 @@
 """The World loop of an FFEA script on the GPU blob path (include/ffea_world.h).
 
 `World(path)` is World::init (World.cpp:155-930) for the blob path: every blob of the
-script on the GPU, the trajectory, measurement and checkpoint files opened with the
-reference's headers, or, with `restart = 1`, the noise stream read back from
+script on the GPU, the trajectory, measurement, detailed measurement (.fdm; not with
+`detailed=False`, the reference's -d) and checkpoint files opened with the reference's
+headers, or, with `restart = 1`, the noise stream read back from
 `checkpoint_in` and the run resumed from the last complete trajectory frame.
 `World.run()` is World::run (World.cpp:1747-2060). `python -m ffea_ci_demo_amd.world
 script.ffea` runs a script as the reference's `ffea script.ffea` does.
@@ -32,11 +33,12 @@ class _BlobView(GpuBlob):
 
 
 class World:
-    def __init__(self, script_path: str, precision: str = "fp64", device: int = 0):
+    def __init__(self, script_path: str, precision: str = "fp64", device: int = 0, detailed: bool = True):
         L = _native.lib()
         prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
         h = C.c_void_p()
-        rc = L.ffea_world_create(script_path.encode(), prec, int(device), C.byref(h))
+        flags = 0 if detailed else _native.WORLD_NO_DETAIL
+        rc = L.ffea_world_create_ex(script_path.encode(), prec, int(device), flags, C.byref(h))
         if rc != _native.OK:
             raise FFEAException(rc, L.ffea_gpu_last_error().decode())
         self.h = h
@@ -86,9 +88,11 @@ def main(argv=None) -> int:
     ap.add_argument("script")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("-d", "--no-detail", action="store_true",
+                    help="write no detailed measurement file (.fdm), as the reference's ffea -d")
     a = ap.parse_args(argv)
     try:
-        with World(a.script, a.precision, a.device) as w:
+        with World(a.script, a.precision, a.device, detailed=not a.no_detail) as w:
             w.run()
     except FFEAException as e:
         print(f"FFEA: {e}", file=sys.stderr)

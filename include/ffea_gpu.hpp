@@ -176,8 +176,11 @@ private:
 // (World.cpp:155-930, 1747-2060), restart included.
 class World {
 public:
-    explicit World(const std::string &script_path, int precision = FFEA_GPU_FP64, int device = 0) {
-        if (int rc = ffea_world_create(script_path.c_str(), precision, device, &w_))
+    // detailed = false: no detailed measurement file (.fdm), as the reference's ffea -d
+    explicit World(const std::string &script_path, int precision = FFEA_GPU_FP64, int device = 0,
+                   bool detailed = true) {
+        if (int rc = ffea_world_create_ex(script_path.c_str(), precision, device, detailed ? 0 : FFEA_WORLD_NO_DETAIL,
+                                          &w_))
             throw Exception(rc, ffea_gpu_last_error());
     }
     ~World() { ffea_world_destroy(w_); }

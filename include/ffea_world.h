@@ -5,8 +5,9 @@
  * reference's World drives them, with the reference's output files:
  *
  *   World::init (World.cpp:155-930)      ffea_world_create: builds the blobs, opens the
- *                                        trajectory and measurement files (headers,
- *                                        World.cpp:519-603) or, with restart = 1, reads
+ *                                        trajectory, measurement and detailed measurement
+ *                                        files (headers, World.cpp:519-646) or, with
+ *                                        restart = 1, reads
  *                                        the checkpoint's noise stream state
  *                                        (World.cpp:302-426) and the last complete
  *                                        trajectory frame, truncates the trajectory and
@@ -21,11 +22,14 @@
  *                                        frame (Blob.cpp:855-889, World.cpp:4461-4488)
  *                                        and the measurement line: time, strain energy,
  *                                        centroid, RMSD (Blob.cpp:1066-1202,
- *                                        World.cpp:4598-4692); then update_positions
+ *                                        World.cpp:4598-4692), and the detailed line: the
+ *                                        same per blob, spring energy per blob pair
+ *                                        (Blob.cpp:1204-1213, World.cpp:4504-4537,
+ *                                        4694-4721); then update_positions
  *
- * The supported subset is that of ffea_gpu_blob_create_from_script (mass-less solver,
- * DYNAMIC blobs, one conformation, no surface interactions, electrostatics, springs or
- * kinetics). Single-threaded noise semantics: one thermal stream, shared by the blobs
+ * The supported subset is that of ffea_gpu_blob_create_from_script (the mass-less, lumped-
+ * mass and consistent-mass solvers, DYNAMIC blobs, one conformation, constant forces,
+ * springs; no surface interactions, electrostatics, PreComp, kinetics or rods). Single-threaded noise semantics: one thermal stream, shared by the blobs
  * in order, as the reference's ffea with one thread.
  */
 #ifndef FFEA_WORLD_H
@@ -39,8 +43,16 @@ extern "C" {
 
 typedef struct ffea_world ffea_world;
 
-/* precision: ffea_gpu_precision; device: HIP device ordinal */
+/* precision: ffea_gpu_precision; device: HIP device ordinal. Writes the detailed measurement
+ * file as the reference's ffea does by default: det_measurement_out_fname, else the
+ * measurement file's name with the extension .fdm (SimulationParams.cpp:690-709); a script
+ * that names no measurement file gets none. */
 int ffea_world_create(const char *ffea_script_path, int precision, int device, ffea_world **out);
+
+/* ffea_world_create with flags: FFEA_WORLD_NO_DETAIL writes no detailed measurement file
+ * (the reference's -d / --no-detail, ffea.cpp:86, 166) */
+#define FFEA_WORLD_NO_DETAIL 1
+int ffea_world_create_ex(const char *ffea_script_path, int precision, int device, int flags, ffea_world **out);
 
 /* runs the loop from the current step; max_steps < 0 runs to num_steps (inclusive, as
  * the reference's `step < num_steps + 1`), otherwise at most max_steps loop iterations.

@@ -34,8 +34,15 @@ def test_cpp_script_runner(tmp_path):
     assert r.returncode == 0, r.stderr
     last = r.stdout.strip().splitlines()[-1]
     assert last.startswith("done: 10 updates")
-    for name in ("b_trajectory.out", "b_measurement.out", "b.fcp"):
+    for name in ("b_trajectory.out", "b_measurement.out", "b_measurement.fdm", "b.fcp"):
         assert os.path.getsize(tmp_path / name) > 0, name
+    # -d: no detailed measurement file, as ffea -d
+    meshgen.write_blob_files(mesh, str(tmp_path / "c"))
+    path_c = str(tmp_path / "c.ffea")
+    meshgen.write_ffea_script(path_c, str(tmp_path / "c"), num_steps=1, check=1, rng_seed=5)
+    rc = subprocess.run([exe, "-d", path_c], capture_output=True, text=True, timeout=120)
+    assert rc.returncode == 0, rc.stderr
+    assert os.path.exists(tmp_path / "c_measurement.out") and not os.path.exists(tmp_path / "c_measurement.fdm")
     g = Script(path).gpu_blob(0)
     g.step(10)
     x0 = g.positions()[0]

@@ -13,7 +13,7 @@ from ffea_ci_demo_amd import units, workloads
 from ffea_ci_demo_amd.blob import FFEAException, GpuBlob
 from ffea_ci_demo_amd.script import Script
 from ffea_ci_demo_amd.world import World
-from test_gpu_world import PRINT_RTOL, read_measurements, sphere_script
+from test_gpu_world import read_detailed, PRINT_RTOL, read_measurements, sphere_script
 
 pytestmark = pytest.mark.gpu
 
@@ -148,10 +148,20 @@ def test_world_springs_against_oracle(oracle_mod, tmp_path):
         got = [w.blob(b).positions() for b in range(2)]
     head, meas = read_measurements(str(tmp_path / "spr_measurement.out"))
     assert head[-1] == "SpringEnergy"
+    # the detailed file: per blob pair (0,0), (0,1), (1,1) the spring energy of the springs
+    # between them (World::get_spring_field_energy, World.cpp:4127-4136)
+    pairs = {(0, 0): 1, (0, 1): 1, (1, 1): 1}
+    dhead, det = read_detailed(str(tmp_path / "spr_measurement.fdm"), 2, pair_cols=pairs)
+    assert dhead.endswith("| B0B0 %-14s| B0B1 %-14s| B1B1 %-14s" % (("SpringEnergy",) * 3)), repr(dhead)
     for n in range(7):
         if n % 2 == 0:
             e = oracle.springs_energy(orcs, sp) * units.ENERGY
             assert abs(meas[n // 2][-1] - e) <= PRINT_RTOL * abs(e), (n, meas[n // 2][-1], e)
+            for c, (i, j) in enumerate(((0, 0), (0, 1), (1, 1))):
+                sub = [x for x in sp if {x[2], x[3]} == {i, j} and (i != j or x[2] == x[3])]
+                ep = oracle.springs_energy(orcs, sub) * units.ENERGY
+                got_p = det[n // 2][11 + c]
+                assert abs(got_p - ep) <= PRINT_RTOL * abs(e), (n, (i, j), got_p, ep)
         oracle.apply_springs(orcs, sp)
         for o in orcs:
             assert o.step() == 0

@@ -97,6 +97,26 @@ def read_measurements(path):
     return txt[k + 1].split(), np.array([[float(x) for x in l.split()] for l in txt[k + 2:] if l.strip()])
 
 
+def read_detailed(path, num_blobs, blob_cols=5, pair_cols=None):
+    """header line and rows of a .fdm (World.cpp:606-646, 4504-4537, 4694-4721): per row the
+    time, then each blob's fields (blob_cols of them), then each pair's (pair_cols[(i, j)] of
+    them, default none); the text of every line is checked to end with the pair blocks'
+    7-space leads"""
+    txt = open(path).read().split("\n")
+    assert txt[:3] == ["FFEA Detailed Measurement File", "", "Measurements:"], txt[:3]
+    assert txt[-1] == ""
+    pairs = [(i, j) for i in range(num_blobs) for j in range(i, num_blobs)]
+    pc = pair_cols or {}
+    rows = []
+    for line in txt[4:-1]:
+        vals = [float(x) for x in line.split()]
+        assert len(vals) == 1 + blob_cols * num_blobs + sum(pc.get(q, 0) for q in pairs), line
+        if not pc:
+            assert line.endswith(" " * 7 * len(pairs)), repr(line[-30:])
+        rows.append(vals)
+    return txt[3], np.array(rows)
+
+
 def close(a, b, rtol, atol):
     a, b = np.asarray(a, float), np.asarray(b, float)
     return bool(np.all(np.abs(a - b) <= rtol * np.abs(b) + atol))
@@ -188,6 +208,15 @@ def test_restart_matches_uninterrupted_run(tmp_path):
         d = np.where(m1 != 0, np.abs((m2 - m1) / m1), 0.0)
     print("restart DMax:", d.max())
     assert d.max() <= 5e-4
+    # the detailed files, truncated and appended the same way (World.cpp:867-930)
+    _, f1 = read_detailed(str(tmp_path / "sphere_63_120_nomass_I_measurement.fdm"), 1)
+    _, f2 = read_detailed(str(tmp_path / "sphere_63_120_nomass_II_measurement.fdm"), 1)
+    assert f1.shape == f2.shape == (11, 6)
+    assert np.array_equal(f1[:, 0], m1[:, 0]) and np.array_equal(f2[:, 0], m2[:, 0])
+    assert np.array_equal(f1[:5], f2[:5])  # before the restarted step: the same run
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d2 = np.where(f1 != 0, np.abs((f2 - f1) / f1), 0.0)
+    assert d2.max() <= 5e-4
     t1 = read_trajectory(str(tmp_path / "sphere_63_120_nomass_I_trajectory.out"))
     t2 = read_trajectory(str(tmp_path / "sphere_63_120_nomass_II_trajectory.out"))
     assert [f[0] for f in t2] == list(range(11)) == [f[0] for f in t1]
@@ -222,6 +251,11 @@ def test_two_blob_world_against_oracle(oracle_mod, tmp_path):
         got = [w.blob(b).positions() for b in range(2)]
     frames = read_trajectory(str(tmp_path / "pair_trajectory.out"), num_blobs=2)
     _, meas = read_measurements(str(tmp_path / "pair_measurement.out"))
+    dhead, det = read_detailed(str(tmp_path / "pair_measurement.fdm"), 2)
+    blob_head = "%-14s%-14s%-14s%-14s%-14s" % ("StrainEnergy", "Centroid[0]", "Centroid[1]", "Centroid[2]", "RMSD")
+    assert dhead == "%-14s" % "Time" + "| B0 " + blob_head + "| B1 " + blob_head, repr(dhead)
+    assert det.shape == (3, 11)
+    assert np.array_equal(det[:, 0], meas[:, 0])
     assert [f[0] for f in frames] == [0, 2, 4]
     pos0 = [sp.pos.copy() for sp in specs]
     scale = max(np.abs(p).max() for p in pos0) * units.LENGTH
@@ -239,7 +273,37 @@ def test_two_blob_world_against_oracle(oracle_mod, tmp_path):
             assert close(row[1], strain, 1e-5, 1e-9 * 4.11e-21), (n, row[1], strain)
             assert close(row[2:5], allp.mean(axis=0) * units.LENGTH, PRINT_RTOL, 1e-12 * scale)
             assert close(row[5], rmsd, 1e-5, 1e-12 * scale)
+            for b in range(2):  # the detailed file: each blob's own strain energy, centroid, RMSD
+                d = det[n // 2][1 + 5 * b:6 + 5 * b]
+                e_b = orcs[b].strain_energy() * units.ENERGY
+                r_b = np.sqrt(((ps[b] - pos0[b]) ** 2).sum(axis=1).mean()) * units.LENGTH
+                assert close(d[0], e_b, 1e-5, 1e-9 * 4.11e-21), (n, b, d[0], e_b)
+                assert close(d[1:4], ps[b].mean(axis=0) * units.LENGTH, PRINT_RTOL, 1e-12 * scale), (n, b)
+                assert close(d[4], r_b, 1e-5, 1e-12 * scale), (n, b, d[4], r_b)
         for o in orcs:
             assert o.step() == 0
     for b in range(2):
         assert rel_err(got[b], orcs[b].positions()) <= 1e-10, b
+
+
+def test_detailed_file_switch_and_backups(tmp_path):
+    """the detailed file is written by default (with one blob its columns are the global
+    measurement's), moved aside like the others, and not written with detailed=False (the
+    reference's ffea -d); det_measurement_out_fname names it"""
+    path = sphere_script(tmp_path, "det", num_steps=2)
+    print("[case] world detailed file", flush=True)
+    for k in range(2):
+        with World(path) as w:
+            w.run()
+    assert (tmp_path / "det_measurement.fdm").exists() and (tmp_path / "__det_measurement.fdm__bckp.1").exists()
+    _, meas = read_measurements(str(tmp_path / "det_measurement.out"))
+    _, det = read_detailed(str(tmp_path / "det_measurement.fdm"), 1)
+    assert det.shape == meas.shape and close(det, meas, PRINT_RTOL, 0.0)  # the same sums, printed
+    off = sphere_script(tmp_path, "nodet", num_steps=1)
+    with World(off, detailed=False) as w:
+        w.run()
+    assert (tmp_path / "nodet_measurement.out").exists() and not (tmp_path / "nodet_measurement.fdm").exists()
+    named = sphere_script(tmp_path, "named", num_steps=1, extra=[("det_measurement_out_fname", "named.details")])
+    with World(named) as w:
+        w.run()
+    assert (tmp_path / "named.details").exists() and not (tmp_path / "named_measurement.fdm").exists()
