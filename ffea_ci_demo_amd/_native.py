@@ -160,7 +160,7 @@ def lib():
     L.ffea_gpu_multigrid_inspect.argtypes = [C.c_int, _ip, _ip, _dp, _dp, vp, C.c_int, C.POINTER(MultigridInfo),
                                              vp, vp, vp, vp, vp, vp]
     L.ffea_world_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
-    L.ffea_world_create_ex.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.ffea_world_create_ex.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.ffea_world_run.argtypes = [vp, C.c_longlong]
     L.ffea_world_current_step.argtypes = [vp]
     L.ffea_world_current_step.restype = C.c_longlong

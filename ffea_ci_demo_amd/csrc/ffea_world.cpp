@@ -132,6 +132,7 @@ struct ffea_world {
     FILE *traj = nullptr, *meas = nullptr, *ckpt = nullptr;
     FILE *det = nullptr;   // the detailed measurement file (.fdm), unless turned off (the reference's -d)
     bool detailed = true;
+    int frames_to_delete = 0;  // restart: complete frames dropped before resuming (the reference's -l)
     long long step = 0, step_initial = 0;
     std::vector<double> host;
     std::string err;
@@ -509,7 +510,10 @@ struct ffea_world {
         long long pos = (long long)t.size();
         int found = 0;
         bool single = false;
-        while (found != 4) {
+        // 3 asterisks to the top of the last frame, 2 more per deleted frame, 1 for the
+        // conformation changes of the frame resumed from (World.cpp:711)
+        const int need = 3 + 2 * frames_to_delete + 1;
+        while (found != need) {
             if (pos - 2 < 0) {  // the file's start: only one frame; take the header's asterisk
                 single = true;
                 size_t q = t.find('*', (size_t)pos);
@@ -521,7 +525,7 @@ struct ffea_world {
             const char c = t[(size_t)pos];
             pos += 1;
             if (c == '*') ++found;
-            if (single && found != 4) fail(FFEA_GPU_ERR_IO, "no complete frame in " + files.trajectory);
+            if (single && found != need) fail(FFEA_GPU_ERR_IO, "no complete frame in " + files.trajectory);
         }
         long long cut = pos;
         if (pos < (long long)t.size() && t[(size_t)pos] == '\n') cut = ++pos;
@@ -576,22 +580,23 @@ struct ffea_world {
         step_initial = rstep;
         truncate_to(files.trajectory, cut);
 
-        // the measurement files lose their last line, the step that is repeated
-        // (World.cpp:838-866, detailed: 867-912)
-        drop_last_line(files.measurement);
-        if (detailed) drop_last_line(files.detailed);
+        // the measurement files lose their last line, the step that is repeated, and one more
+        // per deleted frame (World.cpp:838-866, detailed: 867-912)
+        drop_last_lines(files.measurement, frames_to_delete + 1);
+        if (detailed) drop_last_lines(files.detailed, frames_to_delete + 1);
     }
 
-    // from the end, back to the newline before the last character, and cut there
-    static void drop_last_line(const std::string &path) {
+    // from the end back over `lines` newlines (starting before the last character), and
+    // cut after the last one found
+    static void drop_last_lines(const std::string &path, int lines) {
         const std::string m = read_whole(path);
         long long mp = (long long)m.size();
-        for (;;) {
+        for (int found = 0; found < lines;) {
             if (mp - 2 < 0) fail(FFEA_GPU_ERR_IO, "Error when trying to find last frame from file " + path);
             mp -= 2;
             const char c = m[(size_t)mp];
             mp += 1;
-            if (c == '\n') break;
+            if (c == '\n') ++found;
         }
         truncate_to(path, mp);
     }
@@ -600,11 +605,12 @@ struct ffea_world {
 extern "C" {
 
 int ffea_world_create(const char *path, int precision, int device, ffea_world **out) {
-    return ffea_world_create_ex(path, precision, device, 0, out);
+    return ffea_world_create_ex(path, precision, device, 0, 0, out);
 }
 
-int ffea_world_create_ex(const char *path, int precision, int device, int flags, ffea_world **out) {
-    if (!path || !out || (flags & ~FFEA_WORLD_NO_DETAIL) != 0) {
+int ffea_world_create_ex(const char *path, int precision, int device, int flags, int frames_to_delete,
+                         ffea_world **out) {
+    if (!path || !out || (flags & ~FFEA_WORLD_NO_DETAIL) != 0 || frames_to_delete < 0) {
         ffea_internal_set_error("ffea_world_create: invalid argument");
         return FFEA_GPU_ERR_INVALID;
     }
@@ -615,6 +621,7 @@ int ffea_world_create_ex(const char *path, int precision, int device, int flags,
     ffea_world *w = new ffea_world();
     w->precision = precision;
     w->detailed = (flags & FFEA_WORLD_NO_DETAIL) == 0;
+    w->frames_to_delete = frames_to_delete;
     try {
         ffea_script_get_params(s, &w->p);
         ffea_internal_script_files(s, &w->files);

@@ -33,12 +33,13 @@ class _BlobView(GpuBlob):
 
 
 class World:
-    def __init__(self, script_path: str, precision: str = "fp64", device: int = 0, detailed: bool = True):
+    def __init__(self, script_path: str, precision: str = "fp64", device: int = 0, detailed: bool = True,
+                 delete_frames: int = 0):
         L = _native.lib()
         prec = {"fp64": _native.FP64, "mixed": _native.MIXED}[precision]
         h = C.c_void_p()
         flags = 0 if detailed else _native.WORLD_NO_DETAIL
-        rc = L.ffea_world_create_ex(script_path.encode(), prec, int(device), flags, C.byref(h))
+        rc = L.ffea_world_create_ex(script_path.encode(), prec, int(device), flags, int(delete_frames), C.byref(h))
         if rc != _native.OK:
             raise FFEAException(rc, L.ffea_gpu_last_error().decode())
         self.h = h
@@ -90,9 +91,11 @@ def main(argv=None) -> int:
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("-d", "--no-detail", action="store_true",
                     help="write no detailed measurement file (.fdm), as the reference's ffea -d")
+    ap.add_argument("-l", "--delete-frames", type=int, default=0,
+                    help="when restarting, drop this many final frames first, as the reference's ffea -l")
     a = ap.parse_args(argv)
     try:
-        with World(a.script, a.precision, a.device, detailed=not a.no_detail) as w:
+        with World(a.script, a.precision, a.device, detailed=not a.no_detail, delete_frames=a.delete_frames) as w:
             w.run()
     except FFEAException as e:
         print(f"FFEA: {e}", file=sys.stderr)

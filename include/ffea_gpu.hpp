@@ -176,11 +176,12 @@ private:
 // (World.cpp:155-930, 1747-2060), restart included.
 class World {
 public:
-    // detailed = false: no detailed measurement file (.fdm), as the reference's ffea -d
+    // detailed = false: no detailed measurement file (.fdm), as the reference's ffea -d;
+    // delete_frames: on restart, complete frames dropped first, as ffea -l
     explicit World(const std::string &script_path, int precision = FFEA_GPU_FP64, int device = 0,
-                   bool detailed = true) {
+                   bool detailed = true, int delete_frames = 0) {
         if (int rc = ffea_world_create_ex(script_path.c_str(), precision, device, detailed ? 0 : FFEA_WORLD_NO_DETAIL,
-                                          &w_))
+                                          delete_frames, &w_))
             throw Exception(rc, ffea_gpu_last_error());
     }
     ~World() { ffea_world_destroy(w_); }

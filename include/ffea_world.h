@@ -49,10 +49,14 @@ typedef struct ffea_world ffea_world;
  * that names no measurement file gets none. */
 int ffea_world_create(const char *ffea_script_path, int precision, int device, ffea_world **out);
 
-/* ffea_world_create with flags: FFEA_WORLD_NO_DETAIL writes no detailed measurement file
- * (the reference's -d / --no-detail, ffea.cpp:86, 166) */
+/* ffea_world_create with the reference runner's options (ffea.cpp:86-88, 166):
+ * flags FFEA_WORLD_NO_DETAIL writes no detailed measurement file (-d / --no-detail);
+ * frames_to_delete >= 0: with restart = 1, the last complete frames dropped from the
+ * trajectory and measurement files before the run resumes (-l / --delete-frames,
+ * World.cpp:697-912); the noise stream still comes from the checkpoint */
 #define FFEA_WORLD_NO_DETAIL 1
-int ffea_world_create_ex(const char *ffea_script_path, int precision, int device, int flags, ffea_world **out);
+int ffea_world_create_ex(const char *ffea_script_path, int precision, int device, int flags, int frames_to_delete,
+                         ffea_world **out);
 
 /* runs the loop from the current step; max_steps < 0 runs to num_steps (inclusive, as
  * the reference's `step < num_steps + 1`), otherwise at most max_steps loop iterations.

@@ -15,7 +15,7 @@ import pytest
 
 from conftest import rel_err
 from ffea_ci_demo_amd import _native, units
-from ffea_ci_demo_amd.blob import GpuBlob
+from ffea_ci_demo_amd.blob import FFEAException, GpuBlob
 from ffea_ci_demo_amd.script import Script
 from ffea_ci_demo_amd.world import World
 
@@ -307,3 +307,36 @@ def test_detailed_file_switch_and_backups(tmp_path):
     with World(named) as w:
         w.run()
     assert (tmp_path / "named.details").exists() and not (tmp_path / "named_measurement.fdm").exists()
+
+
+def test_restart_deleting_frames(tmp_path):
+    """ffea -l 2 (delete_frames): a restart drops the last two complete frames and their
+    measurement lines and resumes from the frame before them; with noise off the resumed run
+    retraces the uninterrupted one (measurements within 5e-4, as the reference's restart
+    check)"""
+    full = sphere_script(tmp_path, "fullrun", num_steps=8, calc_noise=0, traj="full_traj.out", meas="full_meas.out")
+    part = sphere_script(tmp_path, "part", num_steps=8, calc_noise=0, traj="part_traj.out", meas="part_meas.out")
+    again = sphere_script(tmp_path, "again2", num_steps=8, calc_noise=0, restart=1, traj="part_traj.out",
+                          meas="part_meas.out", extra=[("checkpoint_in", "part.fcp")])
+    print("[case] world restart deleting 2 frames", flush=True)
+    for path in (full, part):
+        with World(path) as w:
+            w.run()
+    with World(again, delete_frames=2) as w:
+        assert w.step == 6
+        w.run()
+        assert w.step == 9
+    t1 = read_trajectory(str(tmp_path / "full_traj.out"))
+    t2 = read_trajectory(str(tmp_path / "part_traj.out"))
+    assert [f[0] for f in t2] == list(range(9)) == [f[0] for f in t1]
+    _, m1 = read_measurements(str(tmp_path / "full_meas.out"))
+    _, m2 = read_measurements(str(tmp_path / "part_meas.out"))
+    _, d2 = read_detailed(str(tmp_path / "part_meas.fdm"), 1)
+    assert m1.shape == m2.shape == d2.shape == (9, 6)
+    assert np.array_equal(m1[:6], m2[:6])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.where(m1 != 0, np.abs((m2 - m1) / m1), 0.0)
+    assert d.max() <= 5e-4, d.max()
+    assert close(d2, m2, PRINT_RTOL, 0.0)
+    with pytest.raises(FFEAException):
+        World(again, delete_frames=-1)
