@@ -168,6 +168,7 @@ struct Params {
     // output and checkpoint files (SimulationParams.cpp:682-749, 972-990), as written: the
     // script's directory is prepended when the runner resolves them
     std::string traj_out, meas_out, det_out, ckpt_in, ckpt_out;
+    bool hard_wall = false;  // some wall_* = HARD (SimulationParams.cpp:580-690)
 };
 
 // defaults of SimulationParams::SimulationParams (SimulationParams.cpp:26-117)
@@ -230,6 +231,9 @@ void assign(Params &P, const std::string &lv, const std::string &rv) {
     else if (lv == "trajectory_out_fname") P.traj_out = rv;
     else if (lv == "measurement_out_fname") P.meas_out = rv;
     else if (lv == "det_measurement_out_fname") P.det_out = rv;
+    else if (lv.compare(0, 5, "wall_") == 0 && lv.size() == 8 && (lv[5] == 'x' || lv[5] == 'y' || lv[5] == 'z') &&
+             lv[6] == '_' && (lv[7] == '1' || lv[7] == '2'))
+        P.hard_wall |= rv == "HARD";
     else if (lv == "checkpoint_in") P.ckpt_in = rv;
     else if (lv == "checkpoint_out") P.ckpt_out = rv;
     else {
@@ -237,8 +241,7 @@ void assign(Params &P, const std::string &lv, const std::string &rv) {
             "num_rods", "num_couplings", "num_conformations", "num_states", "es_update", "kinetics_update",
             "sticky_wall_xz", "es_h", "kappa", "dielec_ext", "epsilon_0", "calc_steric_rod", "calc_vdw_rod",
             "pbc_rod", "inc_self_ssint", "inc_self_vdw", "ssint_type", "vdw_type", "force_pbc", "steric_factor",
-            "steric_dr", "flow_profile", "shear_rate", "flow_velocity", "wall_x_1", "wall_x_2", "wall_y_1",
-            "wall_y_2", "wall_z_1", "wall_z_2", "trajectory_out_fname",
+            "steric_dr", "flow_profile", "shear_rate", "flow_velocity", "trajectory_out_fname",
             "measurement_out_fname", "kinetics_out_fname", "vdw_in_fname", "ssint_in_fname",
             "vdw_forcefield_params", "ssint_forcefield_params", "ljparams", "lj_params", "rod_lj_in_fname",
             "rod_vdw_forcefield_params", "rod_lj_params", "checkpoint_in", "checkpoint_out", "beads_out_fname",
@@ -785,6 +788,11 @@ void load_script(ffea_script &S) {
         B.solver = solver == "masslumped" ? FFEA_GPU_SOLVER_MASSLUMPED
                    : solver == "CG"       ? FFEA_GPU_SOLVER_ITERATIVE
                                           : FFEA_GPU_SOLVER_NOMASS_CG;
+        // a HARD wall zeroes outward velocities of surface nodes before each step
+        // (Blob::enforce_box_boundaries, Blob.cpp:1800-1845): the mass-less solve overwrites
+        // them, an inertial blob would keep them; that is not built
+        if (P.hard_wall && B.solver != FFEA_GPU_SOLVER_NOMASS_CG)
+            fail("Blob %d: HARD walls with an inertial solver are not part of this build.", b);
         load_nodes(B, f_nodes);
         load_topology(B, f_top);
         load_surface(B, f_surf);

@@ -125,3 +125,22 @@ def test_errors_carry_reference_messages(tmp_path):
         Script(bad)
     open(bad, "w").write(txt.replace("<!--", "").replace("<scale", "<!-- a comment --><scale"))
     Script(bad)
+
+
+def test_hard_walls(tmp_path):
+    """wall_* = HARD zeroes outward surface-node velocities before each step
+    (Blob::enforce_box_boundaries): the mass-less solve overwrites them, so such scripts load;
+    an inertial solver with a HARD wall is refused rather than run without it"""
+    mesh, script = write_cube(tmp_path)
+    txt = open(script).read()
+    walled = str(tmp_path / "walled.ffea")
+    open(walled, "w").write(txt.replace("<calc_noise = 1>", "<calc_noise = 1>\n\t<wall_x_1 = HARD>\n\t<wall_z_2 = PBC>"))
+    Script(walled)
+    inertial = str(tmp_path / "inertial.ffea")
+    body = txt.replace("<calc_noise = 1>", "<calc_noise = 1>\n\t<wall_y_2 = HARD>")
+    assert "<solver = CG_nomass>" in body
+    open(inertial, "w").write(body.replace("<solver = CG_nomass>", "<solver = masslumped>"))
+    with pytest.raises(ScriptError, match="HARD walls with an inertial solver"):
+        Script(inertial)
+    open(inertial, "w").write(txt.replace("<solver = CG_nomass>", "<solver = masslumped>"))
+    Script(inertial)
