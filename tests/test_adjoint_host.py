@@ -57,3 +57,11 @@ def test_backward_in_order_is_accepted():
         x = trajectory(b, G, E, A, B, np.zeros((2, 3)), 3)
         x.sum().backward()
     assert [c[0] for c in b.calls] == ["run", "adjoint", "run", "adjoint"]
+
+
+def test_mixed_adjoint_without_refinement_warns():
+    """GpuBlob.adjoint warns for an FP32-mixed run without FP64 refinement only"""
+    from ffea_ci_demo_amd.blob import mixed_adjoint_warning
+    assert "set_refinement" in mixed_adjoint_warning("mixed", 0.0)
+    assert mixed_adjoint_warning("mixed", 1e-10) is None
+    assert mixed_adjoint_warning("fp64", 0.0) is None
