@@ -198,6 +198,11 @@ struct MgCsrArgs {
     double omega;
     double *y, *y2;
     const float *x32 = nullptr;  // level-0 restriction of the FP32-mixed form: x's packed FP32 copy, gathered
+    // POST as a step of a polynomial smoother: y = alpha x + beta xold + omega D^-1 (b - A x)
+    // (xold read at the row itself only, so y may overwrite it); alpha 1 and no xold is the
+    // damped-Jacobi sweep
+    double alpha = 1.0, beta = 0.0;
+    const double *xold = nullptr;
 };
 
 template <int BR, int BC, int MODE>
@@ -256,7 +261,12 @@ __global__ __launch_bounds__(kMgBlock) void k_mg_csr(MgCsrArgs a) {
         if (a.y2) a.y2[o] = a.omega * a.dinv[o] * mine;
     } else {
         static_assert(MODE == MG_POST, "k_mg_csr: residual, restriction or post-smoothing");
-        a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - mine);
+        if (a.xold == nullptr && a.alpha == 1.0) {
+            a.y[o] = a.x[o] + a.omega * a.dinv[o] * (a.b[o] - mine);
+        } else {
+            const double xo = a.xold ? a.beta * a.xold[o] : 0.0;
+            a.y[o] = a.alpha * a.x[o] + xo + a.omega * a.dinv[o] * (a.b[o] - mine);
+        }
     }
 }
 

@@ -51,6 +51,11 @@ using namespace ffk;
 // kMgBigLevel1 aggregates, and kMgCoarserSweeps on levels 2 .. L-2
 constexpr int kMgBigLevel1 = 10000;
 constexpr int kMgCoarserSweeps = 4;
+// a level 1 with two sweeps smooths with a degree-2 Chebyshev polynomial of D^-1 A over
+// [l/kMgChebRatio, l] for l = 1.1 times its largest-eigenvalue estimate, at the cost of
+// two damped-Jacobi sweeps (false: the sweeps)
+constexpr bool kMgChebyshev1 = true;
+constexpr double kMgChebRatio = 10.0;
 // unknowns at which a single blob's coarsening stops: its coarsest level, up to this size,
 // is solved with a dense inverse (an ensemble's blobs keep amg::Options' default)
 constexpr int kMgMaxCoarseSingle = 4800;
@@ -1351,6 +1356,8 @@ struct ffea_gpu_blob {
     struct MgLevel {
         int n = 0;             // block rows
         double omega = 0.0;    // smoother weight
+        double lmax = 0.0, fold = 0.0;  // levels >= 1 of a single blob: the power estimate of lambda_max(D^-1 A)
+                                        // and the weight folded into dinv
         MgCsr A, R;            // A: levels >= 1 (level 0 is the CG's matrix); R: to level + 1
         MgEll P;               // prolongation from level + 1 (few blocks per row: a thread per row)
         DevBuf<double> dinv, b, xa, res, xb, e;  // levels >= 1, [n][6]
@@ -2283,6 +2290,8 @@ void ffea_gpu_blob::mg_setup() {
         auto lev = std::make_unique<MgLevel>();
         lev->n = nlev[l];
         lev->omega = l == 0 ? H[0].lv[0].omega : 1.0;  // levels above 0: folded into dinv
+        lev->lmax = H[0].lv[l].lmax;
+        lev->fold = H[0].lv[l].omega;
         long long blocks = 0;
         for (int b = 0; b < nblob; ++b) blocks += H[b].lv[l].A.nnzb();
         if (l >= 1) {
@@ -2432,24 +2441,46 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
     const int sweeps1 = nblob == 1 && L > 1 && mg[1]->n > kMgBigLevel1 ? 2 : 1;
     auto sweeps = [&](int l) { return l == 1 ? sweeps1 : kMgCoarserSweeps; };
     std::vector<double *> pre(L, nullptr), out(L, nullptr);
-    auto post_sweep = [&](MgLevel &lv, const double *x, double *y) {
-        MgCsrArgs po{S0, lv.n, lv.A.rp.p, lv.A.ci.p, lv.A.val.p, lv.A.mscale, x, lv.b.p, lv.dinv.p, lv.omega, y,
+    // y = alpha x + beta xold + w (omega D^-1, folded into dinv) (b - A x)
+    auto post_sweep = [&](MgLevel &lv, const double *x, double *y, double w = 1.0, double alpha = 1.0,
+                          double beta = 0.0, const double *xold = nullptr) {
+        MgCsrArgs po{S0, lv.n, lv.A.rp.p, lv.A.ci.p, lv.A.val.p, lv.A.mscale, x, lv.b.p, lv.dinv.p, lv.omega * w, y,
                      nullptr};
+        po.alpha = alpha;
+        po.beta = beta;
+        po.xold = xold;
         hipLaunchKernelGGL((k_mg_csr<6, 6, MG_POST>), wgrid(lv.n), dim3(kMgBlock), 0, stream, po);
     };
+    // level 1's degree-2 Chebyshev smoother (kMgChebyshev1): x1 = x0 + D^-1 (b - A x0) / theta,
+    // x2 = x1 + rho1 rho0 (x1 - x0) + (2 rho1 / delta) D^-1 (b - A x1), theta and delta the
+    // centre and half-width of the interval, rho0 = delta / theta, rho1 = 1 / (2 theta / delta
+    // - rho0); before the correction x0 = 0, after it x0 is the prolonged iterate. The same
+    // polynomial both times keeps M symmetric. Weights relative to the folded one:
+    const bool cheb1 = kMgChebyshev1 && sweeps1 == 2;
+    double cw1 = 1.0, ca2 = 1.0, cb2 = 0.0, cw2 = 1.0;
+    if (cheb1) {
+        const double hi = 1.1 * mg[1]->lmax, lo = hi / kMgChebRatio;
+        const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sg = th / de;
+        const double rho0 = 1.0 / sg, rho1 = 1.0 / (2.0 * sg - rho0);
+        cw1 = 1.0 / (th * mg[1]->fold);
+        ca2 = 1.0 + rho1 * rho0;
+        cb2 = -rho1 * rho0;
+        cw2 = 2.0 * rho1 / de / mg[1]->fold;
+    }
     for (int l = 0; l + 1 < L; ++l) {  // down: restrict and pre-smooth, residual
         MgLevel &cur = *mg[l], &nx = *mg[l + 1];
         const bool coarsest = l + 1 == L - 1;
         MgCsrArgs ra{S0, nx.n, cur.R.rp.p, cur.R.ci.p, cur.R.val.p, cur.R.mscale, l == 0 ? mg_res0.p : cur.res.p,
-                     nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega, nx.b.p, coarsest ? nullptr : nx.xa.p,
-                     l == 0 ? mg_res32.p : nullptr};
+                     nullptr, coarsest ? nullptr : nx.dinv.p, nx.omega * (l == 0 && cheb1 ? cw1 : 1.0), nx.b.p,
+                     coarsest ? nullptr : nx.xa.p, l == 0 ? mg_res32.p : nullptr};
         if (l == 0) hipLaunchKernelGGL((k_mg_csr<6, 3, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         else hipLaunchKernelGGL((k_mg_csr<6, 6, MG_RESTRICT>), wgrid(nx.n), dim3(kMgBlock), 0, stream, ra);
         if (!coarsest) {
             double *xp = nx.xa.p;
             for (int k = 1; k < sweeps(l + 1); ++k) {
                 double *to = xp == nx.xa.p ? nx.e.p : nx.xa.p;  // e is free until the way up
-                post_sweep(nx, xp, to);
+                if (l == 0 && cheb1) post_sweep(nx, xp, to, cw2, ca2);  // x0 = 0: no xold term
+                else post_sweep(nx, xp, to);
                 xp = to;
             }
             pre[l + 1] = xp;
@@ -2472,7 +2503,14 @@ void ffea_gpu_blob::launch_mg_vcycle(double *Pout) {
         double *xp = cur.xb.p;
         for (int k = 0; k < sweeps(l); ++k) {
             double *to = xp == cur.xb.p ? cur.e.p : cur.xb.p;
-            post_sweep(cur, xp, to);
+            if (l == 1 && cheb1) {
+                // the second step reads x0 (the prolonged iterate, in xb) at its own row and
+                // overwrites it
+                if (k == 0) post_sweep(cur, xp, to, cw1);
+                else post_sweep(cur, xp, to, cw2, ca2, cb2, to);
+            } else {
+                post_sweep(cur, xp, to);
+            }
             xp = to;
         }
         out[l] = xp;
