@@ -75,6 +75,34 @@ def test_free_blob_mixed_against_fp64_oracle(oracle_mod):
     g.close()
 
 
+def test_large_blob_chebyshev_level1():
+    """a single blob whose level 1 has more than 10^4 aggregates smooths it with a degree-2
+    Chebyshev polynomial; a 70^3 blob (2.06M tets) steps with noise to the positions of
+    the FP64 Jacobi-PCG form (held to the oracle by the tests above and in
+    test_gpu_parity.py at sizes the oracle runs): FP64+multigrid within 1e-10, the mixed
+    form within 1e-8, the same noise draws"""
+    spec = workloads.cube_blob(70, rng_seed=8, name="cube70")
+    print(f"[case] multigrid {spec.name} ne={spec.num_elements}: FP64 Jacobi, fp64+mg, mixed+mg", flush=True)
+    jac = GpuBlob.from_spec(spec)
+    mg = GpuBlob.from_spec(spec)
+    info = mg.set_preconditioner("multigrid")
+    assert info["rows"][1] > 10000, info["rows"]
+    mx = GpuBlob.from_spec(spec, precision="mixed")
+    mx.set_preconditioner("multigrid")
+    for b in (jac, mg, mx):
+        b.step(5)
+    ref = jac.positions()
+    e64, e32 = rel_err(mg.positions(), ref), rel_err(mx.positions(), ref)
+    k, kj = mg.cg_iterations()[0], jac.cg_iterations()[0]
+    print(f"levels {info['rows']}: fp64+mg {e64:.2e}, mixed+mg {e32:.2e}, iterations {k} vs Jacobi {kj}", flush=True)
+    assert e64 <= POS_TOL
+    assert e32 <= 1e-8
+    assert k * 10 <= kj
+    assert np.array_equal(mg.rng_state(), jac.rng_state())
+    for b in (jac, mg, mx):
+        b.close()
+
+
 def test_pinned_blob_with_load(oracle_mod):
     spec = workloads.config3(n=12)
     o, g = pair(oracle_mod, spec)
