@@ -7,8 +7,10 @@
 //
 // One V-cycle z = M r, with level 0 the corner-node viscosity matrix K (3x3 blocks, the
 // CG's own sliced-ELL storage) and levels 1.. the Galerkin operators over aggregates
-// (6x6 blocks). The smoother is one damped-Jacobi step before and after the coarse
-// correction, x += w D^-1 (b - A x), which keeps M symmetric. Level-0 vectors are the
+// (6x6 blocks). The same smoothing runs before and after each coarse correction, which
+// keeps M symmetric: damped steps x += w B^-1 (b - A x), B the 3x3 diagonal blocks on
+// level 0 (k_mg_fine) and the diagonal on coarser levels, where a step may also carry a
+// polynomial smoother's alpha x + beta xold terms (k_mg_csr). Level-0 vectors are the
 // CG's 64-row tiled FP64 vectors; coarse vectors are FP64 [rows][6]. The preconditioner's
 // matrices - a copy of K written by the assembly, the transfers and the coarse operators -
 // are FP16 scaled by a power of two per matrix (largest entry near 2^10; a sum is

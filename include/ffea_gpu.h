@@ -417,8 +417,12 @@ int ffea_gpu_blob_use_graph(ffea_gpu_blob *blob, int enable, int iterations);
 /* Preconditioner of the viscosity-matrix CG. JACOBI is the reference's
  * (NoMassCGSolver.cpp:268-281, parallel_apply_preconditioner) and the default.
  * MULTIGRID has no counterpart in the reference: a smoothed-aggregation multigrid V-cycle
- * whose coarse unknowns are the 3 translations and 3 rotations of node aggregates, with
- * one damped-Jacobi smoothing step on each side of every coarse correction. Its
+ * whose coarse unknowns are the 3 translations and 3 rotations of node aggregates. The
+ * same smoothing runs before and after every coarse correction: a damped block-Jacobi
+ * step (the 3x3 diagonal blocks) on the finest level; on level 1 a damped-Jacobi step, or
+ * a degree-2 Chebyshev polynomial for a single blob of more than 10^4 level-1 aggregates;
+ * four damped-Jacobi steps on any level between level 1 and the coarsest, which is solved
+ * with its dense inverse. Its
  * hierarchy is built when the preconditioner is set, from the viscosity matrix at the
  * current positions (an element pass and an assembly that leave positions, velocities
  * and noise streams untouched); every later step applies its own matrix on the finest
